@@ -1,0 +1,935 @@
+// cholla_oracle.cpp -- CPU restatement of Cholla's hydro hot path (FP64).
+//
+// TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP product
+// path. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// may load it, and only to check or to time a CPU baseline; the product path
+// (cholla_amd/) never links, loads or calls it.
+//
+// Every stage restates the reference kernel it names (file:line under
+// src/ of cholla-hydro/cholla) in the same operation order and association,
+// unfused, over full field-major arrays, exactly as the reference's stage
+// kernels see them.  It is compiled with -ffp-contract=off so every product
+// and sum rounds as written (IEEE double, no fused multiply-add).
+//
+// Layout: U[f*n_cells + i + j*nx + k*nx*ny], field order density, momentum_x,
+// momentum_y, momentum_z, Energy (src/grid/grid_enum.h:20-30).
+//
+// Parity pins (which golden vector or invariant checks which function) are
+// listed in tests/test_oracle.py; every function here is checked by at least
+// one of them.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <vector>
+
+#ifdef _OPENMP
+  #include <omp.h>
+#endif
+
+namespace
+{
+
+constexpr double kTiny = 1.0e-20;  // TINY_NUMBER, src/global/global.h:25
+constexpr double kCfl  = 0.3;      // C_cfl, src/grid/grid3D.cpp:157
+
+inline size_t Idx(int i, int j, int k, int nx, int ny)
+{
+  return size_t(i) + size_t(j) * size_t(nx) + size_t(k) * size_t(nx) * size_t(ny);
+}
+
+// math_utils::dotProduct / SquareMagnitude, src/utils/math_utilities.h:78-97
+inline double SquareMagnitude(double a, double b, double c) { return a * a + ((b * b) + (c * c)); }
+
+// SIGN macro, src/global/global.h:144
+inline double SignOf(double a) { return (a < 0.) ? -1. : 1.; }
+
+struct Prim {
+  double d, vx, vy, vz, p;
+};
+struct Char {
+  double a0, a1, a2, a3, a4;
+};
+
+// reconstruction::Thread_Guard, src/reconstruction/reconstruction.h:95-119
+bool ThreadGuard(int order, int nx, int ny, int nz, int xid, int yid, int zid)
+{
+  bool out = xid < order - 1 || xid >= nx - order;
+  if (ny > 1) out = yid < order - 1 || yid >= ny - order || out;
+  if (nz > 1) {
+    out = zid < order - 1 || zid >= nz - order || out;
+  } else {
+    out = zid >= nz || out;
+  }
+  return out;
+}
+
+// hydro_utilities::Calc_Pressure_Primitive, src/utils/hydro_utilities.h:34-45
+inline double PressurePrimitive(double E, double d, double vx, double vy, double vz, double gamma)
+{
+  double pressure = E - 0.5 * d * SquareMagnitude(vx, vy, vz);
+  return std::fmax((gamma - 1.) * pressure, kTiny);
+}
+
+// hydro_utilities::Calc_Energy_Primitive, src/utils/hydro_utilities.h:106-118
+inline double EnergyPrimitive(double P, double d, double vx, double vy, double vz, double gamma)
+{
+  return (std::fmax(P, kTiny) / (gamma - 1.)) + 0.5 * d * SquareMagnitude(vx, vy, vz);
+}
+
+// reconstruction::Load_Data, src/reconstruction/reconstruction.h:139-212 (hydro, no DE)
+Prim LoadData(const double *U, int xid, int yid, int zid, int nx, int ny, size_t n_cells, int o1, int o2, int o3,
+              double gamma)
+{
+  size_t id = Idx(xid, yid, zid, nx, ny);
+  Prim q;
+  q.d  = U[id];
+  q.vx = U[o1 * n_cells + id] / q.d;
+  q.vy = U[o2 * n_cells + id] / q.d;
+  q.vz = U[o3 * n_cells + id] / q.d;
+  q.p  = PressurePrimitive(U[4 * n_cells + id], q.d, q.vx, q.vy, q.vz, gamma);
+  return q;
+}
+
+// reconstruction::Compute_Slope, reconstruction.h:224-251
+Prim Slope(const Prim &l, const Prim &r, double coef = 1.0)
+{
+  return {coef * (r.d - l.d), coef * (r.vx - l.vx), coef * (r.vy - l.vy), coef * (r.vz - l.vz), coef * (r.p - l.p)};
+}
+
+// reconstruction::Van_Leer_Slope, reconstruction.h:262-296
+inline double VlSlope(double l, double r) { return (l * r > 0.0) ? 2.0 * l * r / (l + r) : 0.0; }
+Prim VanLeer(const Prim &l, const Prim &r)
+{
+  return {VlSlope(l.d, r.d), VlSlope(l.vx, r.vx), VlSlope(l.vy, r.vy), VlSlope(l.vz, r.vz), VlSlope(l.p, r.p)};
+}
+
+// reconstruction::Primitive_To_Characteristic (hydro branch), reconstruction.h:423-431
+Char PrimToChar(const Prim &q, const Prim &s, double c, double c2)
+{
+  Char a;
+  a.a0 = -q.d * s.vx / (2.0 * c) + s.p / (2.0 * c2);
+  a.a1 = s.d - s.p / (c2);
+  a.a2 = s.vy;
+  a.a3 = s.vz;
+  a.a4 = q.d * s.vx / (2.0 * c) + s.p / (2.0 * c2);
+  return a;
+}
+
+// reconstruction::Characteristic_To_Primitive (hydro branch), reconstruction.h:482-488
+Prim CharToPrim(const Prim &q, const Char &a, double c, double c2)
+{
+  Prim o;
+  o.d  = a.a0 + a.a1 + a.a4;
+  o.vx = c / q.d * (a.a4 - a.a0);
+  o.vy = a.a2;
+  o.vz = a.a3;
+  o.p  = c2 * (a.a0 + a.a4);
+  return o;
+}
+
+// Monotonize lambda of Monotonize_Characteristic_Return_Primitive, reconstruction.h:519-527
+inline double MonotonizeOne(double left, double right, double centered, double van_leer)
+{
+  if (left * right > 0.0) {
+    double lim_a = 2.0 * std::fmin(std::fabs(left), std::fabs(right));
+    double lim_b = std::fmin(std::fabs(centered), std::fabs(van_leer));
+    return std::copysign(std::fmin(lim_a, lim_b), centered);
+  }
+  return 0.0;
+}
+
+// reconstruction::Monotonize_Characteristic_Return_Primitive, reconstruction.h:512-559
+Prim MonotonizeReturnPrim(const Prim &q, const Char &L, const Char &R, const Char &C, const Char &G, double c,
+                          double c2)
+{
+  Char m;
+  m.a0 = MonotonizeOne(L.a0, R.a0, C.a0, G.a0);
+  m.a1 = MonotonizeOne(L.a1, R.a1, C.a1, G.a1);
+  m.a2 = MonotonizeOne(L.a2, R.a2, C.a2, G.a2);
+  m.a3 = MonotonizeOne(L.a3, R.a3, C.a3, G.a3);
+  m.a4 = MonotonizeOne(L.a4, R.a4, C.a4, G.a4);
+  return CharToPrim(q, m, c, c2);
+}
+
+// reconstruction::Calc_Interface_Linear, reconstruction.h:645-673
+Prim InterfaceLinear(const Prim &q, const Prim &s, double sign)
+{
+  auto f = [&](double state, double slope) { return state + sign * 0.5 * slope; };
+  return {f(q.d, s.d), f(q.vx, s.vx), f(q.vy, s.vy), f(q.vz, s.vz), f(q.p, s.p)};
+}
+
+// reconstruction::Plm_Limit_Interfaces, reconstruction.h:686-723
+void PlmLimit(Prim &L, Prim &R, const Prim &im, const Prim &i, const Prim &ip)
+{
+  auto lim = [](double &l_iph, double &r_imh, double v_imo, double v_i, double v_ipo) {
+    r_imh = std::fmax(std::fmin(v_i, v_imo), r_imh);
+    r_imh = std::fmin(std::fmax(v_i, v_imo), r_imh);
+    l_iph = std::fmax(std::fmin(v_i, v_ipo), l_iph);
+    l_iph = std::fmin(std::fmax(v_i, v_ipo), l_iph);
+  };
+  lim(L.d, R.d, im.d, i.d, ip.d);
+  lim(L.vx, R.vx, im.vx, i.vx, ip.vx);
+  lim(L.vy, R.vy, im.vy, i.vy, ip.vy);
+  lim(L.vz, R.vz, im.vz, i.vz, ip.vz);
+  lim(L.p, R.p, im.p, i.p, ip.p);
+}
+
+// reconstruction::PPM_Single_Variable, reconstruction.h:794-901
+void PpmSingle(double im2, double im1, double i0, double ip1, double ip2, double &L_iph, double &R_imh)
+{
+  const double C2   = 1.25;
+  auto limit_iface = [&](double c_i, double c_im1, double iface, double s2_i, double s2_im1) -> double {
+    double s2c     = 3.0 * (c_im1 + c_i - 2.0 * iface);
+    double limited = 0.0;
+    if (SignOf(s2c) == SignOf(s2_im1) && SignOf(s2c) == SignOf(s2_i)) {
+      limited = SignOf(s2c) * std::fmin(C2 * std::fabs(s2_im1), std::fmin(C2 * std::fabs(s2_i), std::fabs(s2c)));
+    }
+    double dl = iface - c_im1;
+    double dr = c_i - iface;
+    if (dl * dr < 0.0) return 0.5 * (c_im1 + c_i) - limited / 6.0;
+    return iface;
+  };
+
+  double slope_left    = (i0 - im1);
+  double slope_right   = (ip1 - i0);
+  double slope_avg_im1 = 0.5 * slope_left + 0.5 * (im1 - im2);
+  double slope_avg_i   = 0.5 * slope_right + 0.5 * slope_left;
+  double slope_avg_ip1 = 0.5 * (ip2 - ip1) + 0.5 * slope_right;
+
+  R_imh = 0.5 * (im1 + i0) + (slope_avg_im1 - slope_avg_i) / 6.0;
+  L_iph = 0.5 * (i0 + ip1) + (slope_avg_i - slope_avg_ip1) / 6.0;
+
+  double s2_im1 = im2 + i0 - 2.0 * im1;
+  double s2_i   = im1 + ip1 - 2.0 * i0;
+  double s2_ip1 = i0 + ip2 - 2.0 * ip1;
+
+  R_imh = limit_iface(i0, im1, R_imh, s2_i, s2_im1);
+  L_iph = limit_iface(ip1, i0, L_iph, s2_ip1, s2_i);
+
+  double s2_face    = 6.0 * (R_imh + L_iph - 2.0 * i0);
+  double s2_limited = 0.0;
+  if (SignOf(s2_im1) == SignOf(s2_i) && SignOf(s2_im1) == SignOf(s2_ip1) && SignOf(s2_im1) == SignOf(s2_face)) {
+    s2_limited = SignOf(s2_face) * std::fmin(std::fmin(C2 * std::fabs(s2_im1), C2 * std::fabs(s2_i)),
+                                             std::fmin(C2 * std::fabs(s2_ip1), std::fabs(s2_face)));
+  }
+
+  double cell_max = std::fmax(std::fabs(im2), std::fabs(im1));
+  cell_max        = std::fmax(cell_max, std::fabs(i0));
+  cell_max        = std::fmax(cell_max, std::fabs(ip1));
+  cell_max        = std::fmax(cell_max, std::fabs(ip2));
+
+  double rho = (std::fabs(s2_face) > (1.0e-12) * cell_max) ? s2_limited / s2_face : 0.0;
+
+  double sfl = i0 - R_imh;
+  double sfr = L_iph - i0;
+  if ((sfl * sfr) <= 0.0 || ((ip1 - i0) * (i0 - im1)) <= 0.0) {
+    if (rho <= (1.0 - (1.0e-12))) {
+      R_imh = i0 - rho * sfl;
+      L_iph = i0 + rho * sfr;
+    }
+  } else {
+    if (std::fabs(sfl) >= 2.0 * std::fabs(sfr)) R_imh = i0 - 2.0 * sfr;
+    if (std::fabs(sfr) >= 2.0 * std::fabs(sfl)) L_iph = i0 + 2.0 * sfl;
+  }
+}
+
+// reconstruction::Write_Data (hydro), reconstruction.h:918-968
+void WriteData(const Prim &s, double *Q, size_t id, size_t n_cells, int o1, int o2, int o3, double gamma)
+{
+  Q[id]               = s.d;
+  Q[o1 * n_cells + id] = s.d * s.vx;
+  Q[o2 * n_cells + id] = s.d * s.vy;
+  Q[o3 * n_cells + id] = s.d * s.vz;
+  Q[4 * n_cells + id]  = EnergyPrimitive(s.p, s.d, s.vx, s.vy, s.vz, gamma);
+}
+
+void DirOrder(int dir, int &o1, int &o2, int &o3)
+{
+  if (dir == 0) {
+    o1 = 1, o2 = 2, o3 = 3;
+  } else if (dir == 1) {
+    o1 = 2, o2 = 3, o3 = 1;
+  } else {
+    o1 = 3, o2 = 1, o3 = 2;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Scalar helpers exposed for the known-answer tests
+// ---------------------------------------------------------------------------
+void orc_ppm_single_variable(double im2, double im1, double i0, double ip1, double ip2, double *L_iph, double *R_imh)
+{
+  PpmSingle(im2, im1, i0, ip1, ip2, *L_iph, *R_imh);
+}
+
+int orc_thread_guard(int order, int nx, int ny, int nz, int xid, int yid, int zid)
+{
+  return ThreadGuard(order, nx, ny, nz, xid, yid, zid) ? 1 : 0;
+}
+
+// Load_Data on a field-major grid; out = {d, vx, vy, vz, p}
+void orc_load_data(const double *U, int xid, int yid, int zid, int nx, int ny, int nz, int dir, double gamma,
+                   double *out)
+{
+  int o1, o2, o3;
+  DirOrder(dir, o1, o2, o3);
+  Prim q = LoadData(U, xid, yid, zid, nx, ny, size_t(nx) * ny * nz, o1, o2, o3, gamma);
+  out[0] = q.d, out[1] = q.vx, out[2] = q.vy, out[3] = q.vz, out[4] = q.p;
+}
+
+// Compute_Slope / Van_Leer_Slope / Calc_Interface_Linear on 5-vectors {d, vx, vy, vz, p}
+void orc_compute_slope(const double *l, const double *r, double coef, double *out)
+{
+  Prim s = Slope({l[0], l[1], l[2], l[3], l[4]}, {r[0], r[1], r[2], r[3], r[4]}, coef);
+  out[0] = s.d, out[1] = s.vx, out[2] = s.vy, out[3] = s.vz, out[4] = s.p;
+}
+void orc_van_leer_slope(const double *l, const double *r, double *out)
+{
+  Prim s = VanLeer({l[0], l[1], l[2], l[3], l[4]}, {r[0], r[1], r[2], r[3], r[4]});
+  out[0] = s.d, out[1] = s.vx, out[2] = s.vy, out[3] = s.vz, out[4] = s.p;
+}
+void orc_interface_linear(const double *q, const double *s, double sign, double *out)
+{
+  Prim o = InterfaceLinear({q[0], q[1], q[2], q[3], q[4]}, {s[0], s[1], s[2], s[3], s[4]}, sign);
+  out[0] = o.d, out[1] = o.vx, out[2] = o.vy, out[3] = o.vz, out[4] = o.p;
+}
+// Monotonize_Characteristic_Return_Primitive (hydro); chars are 5-vectors a0..a4
+void orc_monotonize_char_return_prim(const double *q, const double *aL, const double *aR, const double *aC,
+                                     const double *aG, double c, double c2, double *out)
+{
+  auto ch = [](const double *a) { return Char{a[0], a[1], a[2], a[3], a[4]}; };
+  Prim o  = MonotonizeReturnPrim({q[0], q[1], q[2], q[3], q[4]}, ch(aL), ch(aR), ch(aC), ch(aG), c, c2);
+  out[0] = o.d, out[1] = o.vx, out[2] = o.vy, out[3] = o.vz, out[4] = o.p;
+}
+// Plm_Limit_Interfaces; L, R updated in place
+void orc_plm_limit_interfaces(double *L, double *R, const double *im, const double *i0, const double *ip)
+{
+  Prim l{L[0], L[1], L[2], L[3], L[4]}, r{R[0], R[1], R[2], R[3], R[4]};
+  PlmLimit(l, r, {im[0], im[1], im[2], im[3], im[4]}, {i0[0], i0[1], i0[2], i0[3], i0[4]},
+           {ip[0], ip[1], ip[2], ip[3], ip[4]});
+  L[0] = l.d, L[1] = l.vx, L[2] = l.vy, L[3] = l.vz, L[4] = l.p;
+  R[0] = r.d, R[1] = r.vx, R[2] = r.vy, R[3] = r.vz, R[4] = r.p;
+}
+// Write_Data into a field-major interface array
+void orc_write_data(const double *s, double *Q, long id, long n_cells, int dir, double gamma)
+{
+  int o1, o2, o3;
+  DirOrder(dir, o1, o2, o3);
+  WriteData({s[0], s[1], s[2], s[3], s[4]}, Q, size_t(id), size_t(n_cells), o1, o2, o3, gamma);
+}
+
+// hydroInverseCrossingTime, src/hydro/hydro_cuda.cu:404-418
+double orc_hydro_inverse_crossing_time(double E, double d, double d_inv, double vx, double vy, double vz, double dx,
+                                       double dy, double dz, double gamma)
+{
+  double P  = (E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+  double cs = std::sqrt(d_inv * gamma * P);
+  double m  = std::fmax((std::fabs(vx) + cs) / dx, (std::fabs(vy) + cs) / dy);
+  m         = std::fmax(m, (std::fabs(vz) + cs) / dz);
+  m         = std::fmax(m, 0.0);
+  return m;
+}
+
+// std::mt19937_64 + std::uniform_real_distribution<double>, the generator the
+// reference's unit tests draw their inputs from (e.g. plmc_cuda_tests.cu:34-48)
+void orc_fill_uniform(uint64_t seed, double a, double b, long n, double *out)
+{
+  std::mt19937_64 prng(seed);
+  std::uniform_real_distribution<double> dist(a, b);
+  for (long i = 0; i < n; i++) out[i] = dist(prng);
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernels (one call = one reference kernel launch over the whole grid)
+// ---------------------------------------------------------------------------
+
+// PCM_Reconstruction_3D, src/reconstruction/pcm_cuda.cu:256-... (hydro)
+void orc_pcm_3d(const double *U, double *QLx, double *QRx, double *QLy, double *QRy, double *QLz, double *QRz, int nx,
+                int ny, int nz, int n_fields)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        size_t id = Idx(i, j, k, nx, ny);
+        for (int f = 0; f < n_fields; f++) {
+          double v               = U[f * n_cells + id];
+          QLx[f * n_cells + id] = v;
+          QLy[f * n_cells + id] = v;
+          QLz[f * n_cells + id] = v;
+          if (i > 0) QRx[f * n_cells + Idx(i - 1, j, k, nx, ny)] = v;
+          if (j > 0) QRy[f * n_cells + Idx(i, j - 1, k, nx, ny)] = v;
+          if (k > 0) QRz[f * n_cells + Idx(i, j, k - 1, nx, ny)] = v;
+        }
+      }
+}
+
+// PCM_Reconstruction_1D, pcm_cuda.cu:13-97
+void orc_pcm_1d(const double *U, double *QL, double *QR, int n_cells, int n_fields)
+{
+  for (int x = 0; x < n_cells - 1; x++)
+    for (int f = 0; f < n_fields; f++) {
+      QL[size_t(f) * n_cells + x] = U[size_t(f) * n_cells + x];
+      QR[size_t(f) * n_cells + x] = U[size_t(f) * n_cells + x + 1];
+    }
+}
+
+// PLMC_cuda (VL build: no characteristic tracing), src/reconstruction/plmc_cuda.cu:24-300
+void orc_plmc(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  int o1, o2, o3;
+  DirOrder(dir, o1, o2, o3);
+  int ix = dir == 0, iy = dir == 1, iz = dir == 2;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        if (ThreadGuard(2, nx, ny, nz, i, j, k)) continue;
+        Prim ci  = LoadData(U, i, j, k, nx, ny, n_cells, o1, o2, o3, gamma);
+        Prim cim = LoadData(U, i - ix, j - iy, k - iz, nx, ny, n_cells, o1, o2, o3, gamma);
+        Prim cip = LoadData(U, i + ix, j + iy, k + iz, nx, ny, n_cells, o1, o2, o3, gamma);
+        double c  = std::sqrt(gamma * ci.p / ci.d);  // Calc_Sound_Speed, hydro_utilities.h:209-212
+        double c2 = c * c;
+        Prim dL   = Slope(cim, ci);
+        Prim dR   = Slope(ci, cip);
+        Prim dC   = Slope(cim, cip, 0.5);
+        Prim dG   = VanLeer(dL, dR);
+        Char aL = PrimToChar(ci, dL, c, c2), aR = PrimToChar(ci, dR, c, c2), aC = PrimToChar(ci, dC, c, c2),
+             aG = PrimToChar(ci, dG, c, c2);
+        Prim dm = MonotonizeReturnPrim(ci, aL, aR, aC, aG, c, c2);
+        Prim L  = InterfaceLinear(ci, dm, 1.0);
+        Prim R  = InterfaceLinear(ci, dm, -1.0);
+        PlmLimit(L, R, cim, ci, cip);
+        R.d = std::fmax(R.d, kTiny);
+        L.d = std::fmax(L.d, kTiny);
+        R.p = std::fmax(R.p, kTiny);
+        L.p = std::fmax(L.p, kTiny);
+        WriteData(L, QL, Idx(i, j, k, nx, ny), n_cells, o1, o2, o3, gamma);
+        WriteData(R, QR, Idx(i - ix, j - iy, k - iz, nx, ny), n_cells, o1, o2, o3, gamma);
+      }
+}
+
+// PPMC_VL, src/reconstruction/ppmc_cuda.cu:540-695 (hydro, no DE/scalars)
+void orc_ppmc_vl(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  int o1, o2, o3;
+  DirOrder(dir, o1, o2, o3);
+  int ix = dir == 0, iy = dir == 1, iz = dir == 2;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        if (ThreadGuard(3, nx, ny, nz, i, j, k)) continue;
+        Prim c0  = LoadData(U, i, j, k, nx, ny, n_cells, o1, o2, o3, gamma);
+        Prim cm1 = LoadData(U, i - ix, j - iy, k - iz, nx, ny, n_cells, o1, o2, o3, gamma);
+        Prim cp1 = LoadData(U, i + ix, j + iy, k + iz, nx, ny, n_cells, o1, o2, o3, gamma);
+        Prim cm2 = LoadData(U, i - 2 * ix, j - 2 * iy, k - 2 * iz, nx, ny, n_cells, o1, o2, o3, gamma);
+        Prim cp2 = LoadData(U, i + 2 * ix, j + 2 * iy, k + 2 * iz, nx, ny, n_cells, o1, o2, o3, gamma);
+        double c  = std::sqrt(gamma * c0.p / c0.d);
+        double c2 = c * c;
+        Char a0 = PrimToChar(c0, c0, c, c2), am1 = PrimToChar(c0, cm1, c, c2), am2 = PrimToChar(c0, cm2, c, c2),
+             ap1 = PrimToChar(c0, cp1, c, c2), ap2 = PrimToChar(c0, cp2, c, c2);
+        Char L, R;
+        PpmSingle(am2.a0, am1.a0, a0.a0, ap1.a0, ap2.a0, L.a0, R.a0);
+        PpmSingle(am2.a1, am1.a1, a0.a1, ap1.a1, ap2.a1, L.a1, R.a1);
+        PpmSingle(am2.a2, am1.a2, a0.a2, ap1.a2, ap2.a2, L.a2, R.a2);
+        PpmSingle(am2.a3, am1.a3, a0.a3, ap1.a3, ap2.a3, L.a3, R.a3);
+        PpmSingle(am2.a4, am1.a4, a0.a4, ap1.a4, ap2.a4, L.a4, R.a4);
+        Prim pL = CharToPrim(c0, L, c, c2);
+        Prim pR = CharToPrim(c0, R, c, c2);
+        pR.d    = std::fmax(pR.d, kTiny);
+        pL.d    = std::fmax(pL.d, kTiny);
+        pR.p    = std::fmax(pR.p, kTiny);
+        pL.p    = std::fmax(pL.p, kTiny);
+        WriteData(pL, QL, Idx(i, j, k, nx, ny), n_cells, o1, o2, o3, gamma);
+        WriteData(pR, QR, Idx(i - ix, j - iy, k - iz, nx, ny), n_cells, o1, o2, o3, gamma);
+      }
+}
+
+// Calculate_HLLC_Fluxes_CUDA, src/riemann_solvers/hllc_cuda.cu:19-307 (no DE/scalars)
+void orc_hllc(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma, int dir)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  int o1, o2, o3;
+  DirOrder(dir, o1, o2, o3);
+  const double g1 = gamma - 1.0;
+#pragma omp parallel for schedule(static)
+  for (long t = 0; t < long(n_cells); t++) {
+    double dl = QL[t], mxl = QL[o1 * n_cells + t], myl = QL[o2 * n_cells + t], mzl = QL[o3 * n_cells + t],
+           El = QL[4 * n_cells + t];
+    double dr = QR[t], mxr = QR[o1 * n_cells + t], myr = QR[o2 * n_cells + t], mzr = QR[o3 * n_cells + t],
+           Er = QR[4 * n_cells + t];
+
+    double vxl = mxl / dl, vyl = myl / dl, vzl = mzl / dl;
+    double pl  = (El - 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl)) * (gamma - 1.0);
+    pl         = std::fmax(pl, kTiny);
+    double vxr = mxr / dr, vyr = myr / dr, vzr = mzr / dr;
+    double pr  = (Er - 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr)) * (gamma - 1.0);
+    pr         = std::fmax(pr, kTiny);
+
+    double Hl = (El + pl) / dl;
+    double Hr = (Er + pr) / dr;
+
+    double sqrtdl = std::sqrt(dl), sqrtdr = std::sqrt(dr);
+    double vx  = (sqrtdl * vxl + sqrtdr * vxr) / (sqrtdl + sqrtdr);
+    double vy  = (sqrtdl * vyl + sqrtdr * vyr) / (sqrtdl + sqrtdr);
+    double vz  = (sqrtdl * vzl + sqrtdr * vzr) / (sqrtdl + sqrtdr);
+    double H   = (sqrtdl * Hl + sqrtdr * Hr) / (sqrtdl + sqrtdr);
+    double vsq = (vx * vx + vy * vy + vz * vz);
+    double asq = g1 * (H - 0.5 * vsq);
+    double a   = std::sqrt(asq);
+    double lambda_m = vx - a, lambda_p = vx + a;
+    double cfl = std::sqrt(gamma * pl / dl), cfr = std::sqrt(gamma * pr / dr);
+    double Sl = std::fmin(lambda_m, vxl - cfl);
+    double Sr = std::fmax(lambda_p, vxr + cfr);
+    // H-correction hook with etah = 0 (hllc_cuda.cu:52,174-175): sgn(S)*fmax(|S|, 0)
+    Sl = (Sl < 0 ? -1.0 : 1.0) * std::fmax(std::fabs(Sl), 0.0);
+    Sr = (Sr < 0 ? -1.0 : 1.0) * std::fmax(std::fabs(Sr), 0.0);
+
+    double f_d_l = mxl, f_mx_l = mxl * vxl + pl, f_my_l = myl * vxl, f_mz_l = mzl * vxl, f_E_l = (El + pl) * vxl;
+    double f_d_r = mxr, f_mx_r = mxr * vxr + pr, f_my_r = myr * vxr, f_mz_r = mzr * vxr, f_E_r = (Er + pr) * vxr;
+
+    double fd, fmx, fmy, fmz, fE;
+    if (Sl > 0.0) {
+      fd = f_d_l, fmx = f_mx_l, fmy = f_my_l, fmz = f_mz_l, fE = f_E_l;
+    } else if (Sr < 0.0) {
+      fd = f_d_r, fmx = f_mx_r, fmy = f_my_r, fmz = f_mz_r, fE = f_E_r;
+    } else {
+      double Sm = (dr * vxr * (Sr - vxr) - dl * vxl * (Sl - vxl) + pl - pr) / (dr * (Sr - vxr) - dl * (Sl - vxl));
+      double ps = dl * (vxl - Sl) * (vxl - Sm) + pl;
+      double dls  = dl * (Sl - vxl) / (Sl - Sm);
+      double mxls = (mxl * (Sl - vxl) + ps - pl) / (Sl - Sm);
+      double myls = dls * vyl, mzls = dls * vzl;
+      double Els  = (El * (Sl - vxl) - pl * vxl + ps * Sm) / (Sl - Sm);
+      double drs  = dr * (Sr - vxr) / (Sr - Sm);
+      double mxrs = (mxr * (Sr - vxr) + ps - pr) / (Sr - Sm);
+      double myrs = drs * vyr, mzrs = drs * vzr;
+      double Ers  = (Er * (Sr - vxr) - pr * vxr + ps * Sm) / (Sr - Sm);
+      double aSm  = std::fabs(Sm);
+      fd  = 0.5 * (f_d_l + f_d_r + (Sr - aSm) * drs + (Sl + aSm) * dls - Sl * dl - Sr * dr);
+      fmx = 0.5 * (f_mx_l + f_mx_r + (Sr - aSm) * mxrs + (Sl + aSm) * mxls - Sl * mxl - Sr * mxr);
+      fmy = 0.5 * (f_my_l + f_my_r + (Sr - aSm) * myrs + (Sl + aSm) * myls - Sl * myl - Sr * myr);
+      fmz = 0.5 * (f_mz_l + f_mz_r + (Sr - aSm) * mzrs + (Sl + aSm) * mzls - Sl * mzl - Sr * mzr);
+      fE  = 0.5 * (f_E_l + f_E_r + (Sr - aSm) * Ers + (Sl + aSm) * Els - Sl * El - Sr * Er);
+    }
+    F[t]               = fd;
+    F[o1 * n_cells + t] = fmx;
+    F[o2 * n_cells + t] = fmy;
+    F[o3 * n_cells + t] = fmz;
+    F[4 * n_cells + t]  = fE;
+  }
+}
+
+// Update_Conserved_Variables_3D_half, src/integrators/VL_3D_cuda.cu:379-495
+// (density floor branch of :479-493 when density_floor_on)
+void orc_update_half_3d(const double *U, double *Uh, const double *Fx, const double *Fy, const double *Fz, int nx,
+                        int ny, int nz, double dx, double dy, double dz, double dt, int n_fields, int density_floor_on,
+                        double density_floor)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  double dtodx = dt / dx, dtody = dt / dy, dtodz = dt / dz;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k < nz - 1; k++)
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) {
+        size_t id = Idx(i, j, k, nx, ny), imo = Idx(i - 1, j, k, nx, ny), jmo = Idx(i, j - 1, k, nx, ny),
+               kmo = Idx(i, j, k - 1, nx, ny);
+        for (int f = 0; f < n_fields; f++) {
+          size_t o = size_t(f) * n_cells;
+          Uh[o + id] = U[o + id] + dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]) +
+                       dtodz * (Fz[o + kmo] - Fz[o + id]);
+        }
+        if (density_floor_on && Uh[id] < density_floor) {
+          double d0 = Uh[id];
+          Uh[id]    = density_floor;
+          for (int f = 1; f < 5; f++) Uh[f * n_cells + id] *= (density_floor / d0);
+        }
+      }
+}
+
+// Average_Cell_Single_Field, src/hydro/hydro_cuda.cu:1155-1177 (density only)
+static void AverageCellSingleField(int f, int i, int j, int k, int nx, int ny, size_t n_cells, double *U)
+{
+  double vl = U[f * n_cells + Idx(i - 1, j, k, nx, ny)], vr = U[f * n_cells + Idx(i + 1, j, k, nx, ny)];
+  double vd = U[f * n_cells + Idx(i, j - 1, k, nx, ny)], vu = U[f * n_cells + Idx(i, j + 1, k, nx, ny)];
+  double vb = U[f * n_cells + Idx(i, j, k - 1, nx, ny)], vt = U[f * n_cells + Idx(i, j, k + 1, nx, ny)];
+  U[f * n_cells + Idx(i, j, k, nx, ny)] = (vl + vr + vd + vu + vb + vt) / 6;
+}
+
+// Update_Conserved_Variables_3D, src/hydro/hydro_cuda.cu:172-402 (no gravity, no scalars)
+void orc_update_3d(double *U, const double *Fx, const double *Fy, const double *Fz, int nx, int ny, int nz,
+                   int n_ghost, double dx, double dy, double dz, double dt, int n_fields, int density_floor_on,
+                   double density_floor)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  double dtodx = dt / dx, dtody = dt / dy, dtodz = dt / dz;
+  std::vector<uint8_t> repair(n_cells, 0);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = n_ghost; k < nz - n_ghost; k++)
+    for (int j = n_ghost; j < ny - n_ghost; j++)
+      for (int i = n_ghost; i < nx - n_ghost; i++) {
+        size_t id = Idx(i, j, k, nx, ny), imo = Idx(i - 1, j, k, nx, ny), jmo = Idx(i, j - 1, k, nx, ny),
+               kmo = Idx(i, j, k - 1, nx, ny);
+        for (int f = 0; f < n_fields; f++) {
+          size_t o = size_t(f) * n_cells;
+          U[o + id] += dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]) +
+                       dtodz * (Fz[o + kmo] - Fz[o + id]);
+        }
+        if (density_floor_on && U[id] < density_floor) {
+          if (U[id] > 0) {
+            double d0 = U[id];
+            U[id]     = density_floor;
+            for (int f = 1; f < 5; f++) U[f * n_cells + id] *= (density_floor / d0);
+          } else {
+            repair[id] = 1;
+          }
+        }
+      }
+  // Negative-density repair (hydro_cuda.cu:293-297). The reference averages
+  // the six neighbours while they are being updated concurrently (a race); here
+  // it runs after the sweep on the updated neighbours, serially in cell order.
+  for (int k = n_ghost; k < nz - n_ghost; k++)
+    for (int j = n_ghost; j < ny - n_ghost; j++)
+      for (int i = n_ghost; i < nx - n_ghost; i++)
+        if (repair[Idx(i, j, k, nx, ny)]) AverageCellSingleField(0, i, j, k, nx, ny, n_cells, U);
+}
+
+// Update_Conserved_Variables_1D_half, src/integrators/VL_1D_cuda.cu:158-...
+void orc_update_half_1d(const double *U, double *Uh, const double *F, int n_cells, double dx, double dt, int n_fields)
+{
+  double dtodx = dt / dx;
+  for (int id = 1; id < n_cells - 1; id++)
+    for (int f = 0; f < n_fields; f++) {
+      size_t o   = size_t(f) * n_cells;
+      Uh[o + id] = U[o + id] + dtodx * (F[o + id - 1] - F[o + id]);
+    }
+}
+
+// Update_Conserved_Variables_1D, src/hydro/hydro_cuda.cu:20-82
+void orc_update_1d(double *U, const double *F, int n_cells, int n_ghost, double dx, double dt, int n_fields)
+{
+  double dtodx = dt / dx;
+  for (int id = n_ghost; id < n_cells - n_ghost; id++)
+    for (int f = 0; f < n_fields; f++) {
+      size_t o = size_t(f) * n_cells;
+      U[o + id] += dtodx * (F[o + id - 1] - F[o + id]);
+    }
+}
+
+// Temperature_Floor_Kernel, src/hydro/hydro_cuda.cu:1117-1153 (no DE)
+void orc_temperature_floor(double *U, int nx, int ny, int nz, int n_ghost, double U_floor)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  // The reference kernel's real-cell guard uses n_ghost in every direction
+  // (hydro_cuda.cu:1130-1131), so on a 1D or 2D grid it touches no cell.
+  if (ny == 1 || nz == 1) return;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = n_ghost; k < nz - n_ghost; k++)
+    for (int j = n_ghost; j < ny - n_ghost; j++)
+      for (int i = n_ghost; i < nx - n_ghost; i++) {
+        size_t id   = Idx(i, j, k, nx, ny);
+        double d    = U[id];
+        double dinv = 1.0 / d;
+        double vx = U[n_cells + id] * dinv, vy = U[2 * n_cells + id] * dinv, vz = U[3 * n_cells + id] * dinv;
+        double E    = U[4 * n_cells + id];
+        double Ekin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
+        double u    = (E - Ekin) / d;
+        if (u < U_floor) U[4 * n_cells + id] = Ekin + d * U_floor;
+      }
+}
+
+// Calc_dt_3D + Calc_dt_1D, src/hydro/hydro_cuda.cu:437-560, returns max inverse dt
+double orc_calc_dti(const double *U, int nx, int ny, int nz, int n_ghost, double dx, double dy, double dz,
+                    double gamma)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  double max_dti = -std::numeric_limits<double>::max();
+  if (nx > 1 && ny == 1 && nz == 1) {
+    for (int id = n_ghost; id < nx - n_ghost; id++) {
+      double d = U[id], dinv = 1.0 / d;
+      double vx = U[n_cells + id] * dinv, vy = U[2 * n_cells + id] * dinv, vz = U[3 * n_cells + id] * dinv;
+      double P = (U[4 * n_cells + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+      P        = std::fmax(P, kTiny);
+      double cs = std::sqrt(dinv * gamma * P);
+      max_dti   = std::fmax(max_dti, (std::fabs(vx) + cs) / dx);
+    }
+    return max_dti;
+  }
+#pragma omp parallel for collapse(2) reduction(max : max_dti) schedule(static)
+  for (int k = n_ghost; k < nz - n_ghost; k++)
+    for (int j = n_ghost; j < ny - n_ghost; j++)
+      for (int i = n_ghost; i < nx - n_ghost; i++) {
+        size_t id = Idx(i, j, k, nx, ny);
+        double d = U[id], dinv = 1.0 / d;
+        double vx = U[n_cells + id] * dinv, vy = U[2 * n_cells + id] * dinv, vz = U[3 * n_cells + id] * dinv;
+        double E = U[4 * n_cells + id];
+        double v = orc_hydro_inverse_crossing_time(E, d, dinv, vx, vy, vz, dx, dy, dz, gamma);
+        // fmax semantics: a NaN crossing time never wins (hydro_cuda.cu:551)
+        max_dti = std::fmax(max_dti, v);
+      }
+  return max_dti;
+}
+
+// FindIndex, src/grid/cuda_boundaries.cu:240-302
+static int FindIndex(int ig, int n, int flag, int face, int n_ghost, double *a)
+{
+  if (face == 0) {
+    switch (flag) {
+      case 2:
+        *a = -1.0;
+        return 2 * n_ghost - ig - 1;
+      case 3:
+        return n_ghost;
+      case 4:
+        return -1;
+      case 5:
+        return ig;
+      default:
+        return ig + n - 2 * n_ghost;
+    }
+  }
+  switch (flag) {
+    case 2:
+      *a = -1.0;
+      return 2 * (n - n_ghost) - ig - 1;
+    case 3:
+      return n - n_ghost - 1;
+    case 4:
+      return -1;
+    case 5:
+      return ig;
+    default:
+      return ig - n + 2 * n_ghost;
+  }
+}
+
+// SetBoundaryMapping, src/grid/cuda_boundaries.cu:161-238
+static long BoundaryMapping(int ig, int jg, int kg, double *a, const int *flags, int nx, int ny, int nz, int n_ghost)
+{
+  long idx = 0;
+  int ir, jr, kr;
+  if (nx > 1) {
+    if (ig < n_ghost)
+      ir = FindIndex(ig, nx, flags[0], 0, n_ghost, &a[0]);
+    else if (ig >= nx - n_ghost)
+      ir = FindIndex(ig, nx, flags[1], 1, n_ghost, &a[0]);
+    else
+      ir = ig;
+    if (ir < 0) return -1;
+    idx += ir;
+  }
+  if (ny > 1) {
+    if (jg < n_ghost)
+      jr = FindIndex(jg, ny, flags[2], 0, n_ghost, &a[1]);
+    else if (jg >= ny - n_ghost)
+      jr = FindIndex(jg, ny, flags[3], 1, n_ghost, &a[1]);
+    else
+      jr = jg;
+    if (jr < 0) return -1;
+    idx += long(nx) * jr;
+  }
+  if (nz > 1) {
+    if (kg < n_ghost)
+      kr = FindIndex(kg, nz, flags[4], 0, n_ghost, &a[2]);
+    else if (kg >= nz - n_ghost)
+      kr = FindIndex(kg, nz, flags[5], 1, n_ghost, &a[2]);
+    else
+      kr = kg;
+    if (kr < 0) return -1;
+    idx += long(nx) * ny * kr;
+  }
+  return idx;
+}
+
+// SetGhostCellsKernel over one face slab, src/grid/cuda_boundaries.cu:71-159
+void orc_set_ghost_cells(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags, int isize,
+                         int jsize, int ksize, int imin, int jmin, int kmin, int dir)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  for (int kk = 0; kk < ksize; kk++)
+    for (int jj = 0; jj < jsize; jj++)
+      for (int ii = 0; ii < isize; ii++) {
+        int i = ii + imin, j = jj + jmin, k = kk + kmin;
+        size_t g = Idx(i, j, k, nx, ny);
+        double a[3] = {1., 1., 1.};
+        long idx     = BoundaryMapping(i, j, k, a, flags, nx, ny, nz, n_ghost);
+        if (idx < 0) continue;
+        for (int f = 0; f < n_fields; f++) c[g + f * n_cells] = c[idx + f * n_cells];
+        if (flags[0] == 2 || flags[1] == 2) c[g + n_cells] *= a[0];
+        if (flags[2] == 2 || flags[3] == 2) c[g + 2 * n_cells] *= a[1];
+        if (flags[4] == 2 || flags[5] == 2) c[g + 3 * n_cells] *= a[2];
+        if (flags[dir] == 3) {  // hydro outflow diode, cuda_boundaries.cu:121-147
+          size_t mom = g + size_t(dir / 2 + 1) * n_cells;
+          if (dir % 2 == 0) {
+            if (c[mom] > 0.0) {
+              c[g + 4 * n_cells] -= 0.5 * (c[mom] * c[mom]) / c[g];
+              c[mom] = 0.0;
+            }
+          } else {
+            if (c[mom] < 0.0) {
+              c[g + 4 * n_cells] -= 0.5 * (c[mom] * c[mom]) / c[g];
+              c[mom] = 0.0;
+            }
+          }
+        }
+      }
+}
+
+// Set_Boundary_Conditions (single process) -> Set_Boundaries(dir) for dir 0..5 with
+// the slab extents of Set_Boundary_Extents, src/grid/boundary_conditions.cpp:96-111, 413-505
+void orc_set_boundaries(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags_in)
+{
+  int flags[6] = {0, 0, 0, 0, 0, 0};
+  if (nx > 1) flags[0] = flags_in[0], flags[1] = flags_in[1];
+  if (ny > 1) flags[2] = flags_in[2], flags[3] = flags_in[3];
+  if (nz > 1) flags[4] = flags_in[4], flags[5] = flags_in[5];
+  int jl = 0, ju = ny, kl = 0, ku = nz;
+  if (ny > 1) jl = n_ghost, ju = ny - n_ghost;
+  if (nz > 1) kl = n_ghost, ku = nz - n_ghost;
+  int ndirs = nz > 1 ? 6 : (ny > 1 ? 4 : 2);
+  for (int dir = 0; dir < ndirs; dir++) {
+    if (flags[dir] == 4 || flags[dir] == 5) continue;  // custom / MPI faces are not set here
+    int imin[3], imax[3];
+    if (dir < 2) {
+      imin[0] = dir == 0 ? 0 : nx - n_ghost, imax[0] = dir == 0 ? n_ghost : nx;
+      imin[1] = jl, imax[1] = ju, imin[2] = kl, imax[2] = ku;
+    } else if (dir < 4) {
+      imin[0] = 0, imax[0] = nx;
+      imin[1] = dir == 2 ? 0 : ny - n_ghost, imax[1] = dir == 2 ? n_ghost : ny;
+      imin[2] = kl, imax[2] = ku;
+    } else {
+      imin[0] = 0, imax[0] = nx, imin[1] = 0, imax[1] = ny;
+      imin[2] = dir == 4 ? 0 : nz - n_ghost, imax[2] = dir == 4 ? n_ghost : nz;
+    }
+    orc_set_ghost_cells(c, nx, ny, nz, n_fields, n_ghost, flags, imax[0] - imin[0], imax[1] - imin[1],
+                        imax[2] - imin[2], imin[0], imin[1], imin[2], dir);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Integrators
+// ---------------------------------------------------------------------------
+
+enum { ORC_PCM = 0, ORC_PLMC = 1, ORC_PPMC = 2 };
+
+struct Scratch {
+  size_t n = 0;
+  std::vector<double> Uh, QLx, QRx, QLy, QRy, QLz, QRz, Fx, Fy, Fz;
+  void Resize(size_t n_total)
+  {
+    if (n == n_total) return;
+    n = n_total;
+    for (auto *v : {&Uh, &QLx, &QRx, &QLy, &QRy, &QLz, &QRz, &Fx, &Fy, &Fz}) v->assign(n_total, 0.0);
+  }
+};
+static Scratch g_scratch;
+
+// VL_Algorithm_3D_CUDA, src/integrators/VL_3D_cuda.cu:38-360 (hydro; HLLC)
+void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double dy, double dz, double dt,
+              double gamma, int n_fields, int recon, int density_floor_on, double density_floor)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  Scratch &s     = g_scratch;
+  s.Resize(n_cells * n_fields);
+  orc_pcm_3d(U, s.QLx.data(), s.QRx.data(), s.QLy.data(), s.QRy.data(), s.QLz.data(), s.QRz.data(), nx, ny, nz,
+             n_fields);
+  orc_hllc(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, nz, gamma, 0);
+  orc_hllc(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, nz, gamma, 1);
+  orc_hllc(s.QLz.data(), s.QRz.data(), s.Fz.data(), nx, ny, nz, gamma, 2);
+  orc_update_half_3d(U, s.Uh.data(), s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, dx, dy, dz, 0.5 * dt,
+                     n_fields, density_floor_on, density_floor);
+  if (recon == ORC_PLMC) {
+    orc_plmc(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, nz, gamma, 0);
+    orc_plmc(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, nz, gamma, 1);
+    orc_plmc(s.Uh.data(), s.QLz.data(), s.QRz.data(), nx, ny, nz, gamma, 2);
+  } else if (recon == ORC_PPMC) {
+    orc_ppmc_vl(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, nz, gamma, 0);
+    orc_ppmc_vl(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, nz, gamma, 1);
+    orc_ppmc_vl(s.Uh.data(), s.QLz.data(), s.QRz.data(), nx, ny, nz, gamma, 2);
+  } else {
+    orc_pcm_3d(s.Uh.data(), s.QLx.data(), s.QRx.data(), s.QLy.data(), s.QRy.data(), s.QLz.data(), s.QRz.data(), nx,
+               ny, nz, n_fields);
+  }
+  orc_hllc(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, nz, gamma, 0);
+  orc_hllc(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, nz, gamma, 1);
+  orc_hllc(s.QLz.data(), s.QRz.data(), s.Fz.data(), nx, ny, nz, gamma, 2);
+  orc_update_3d(U, s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, n_ghost, dx, dy, dz, dt, n_fields,
+                density_floor_on, density_floor);
+}
+
+// VL_Algorithm_1D_CUDA, src/integrators/VL_1D_cuda.cu:30-146 (HLLC)
+void orc_vl1d(double *U, int nx, int n_ghost, double dx, double dt, double gamma, int n_fields, int recon)
+{
+  size_t n_cells = size_t(nx);
+  Scratch &s     = g_scratch;
+  s.Resize(n_cells * n_fields);
+  orc_pcm_1d(U, s.QLx.data(), s.QRx.data(), nx, n_fields);
+  orc_hllc(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, 1, 1, gamma, 0);
+  orc_update_half_1d(U, s.Uh.data(), s.Fx.data(), nx, dx, 0.5 * dt, n_fields);
+  if (recon == ORC_PLMC) {
+    orc_plmc(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, 1, 1, gamma, 0);
+  } else if (recon == ORC_PPMC) {
+    orc_ppmc_vl(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, 1, 1, gamma, 0);
+  } else {
+    orc_pcm_1d(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, n_fields);
+  }
+  orc_hllc(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, 1, 1, gamma, 0);
+  orc_update_1d(U, s.Fx.data(), nx, n_ghost, dx, dt, n_fields);
+}
+
+// The driver loop of src/main.cpp:245-379 restricted to the hydro path:
+//   dti = Calc_Inverse_Timestep()
+//   while t < tout and step < max_steps:
+//     dt = C_cfl / dti; clamp to tout (main.cpp:258-261)
+//     Update_Hydro_Grid: VL step, temperature floor, dti of the new state (grid3D.cpp:474-585)
+//     t += dt; n_step++; Set_Boundary_Conditions_Grid
+// The state must already hold its ghost cells (main.cpp:190 sets them before the loop).
+// Returns the number of steps taken; *t_io is advanced in place.
+int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double dx, double dy, double dz,
+            double gamma, int recon, const int *flags, double *t_io, double tout, int max_steps, int density_floor_on,
+            double density_floor, int temperature_floor_on, double U_floor, double *dt_log)
+{
+  double t   = *t_io;
+  double dti = orc_calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma);
+  int steps  = 0;
+  bool one_d = ny == 1 && nz == 1;
+  while (t < tout && steps < max_steps) {
+    double dt = kCfl / dti;
+    if (t + dt > tout) dt = tout - t;
+    if (one_d) {
+      orc_vl1d(U, nx, n_ghost, dx, dt, gamma, n_fields, recon);
+    } else {
+      orc_vl3d(U, nx, ny, nz, n_ghost, dx, dy, dz, dt, gamma, n_fields, recon, density_floor_on, density_floor);
+      if (temperature_floor_on) orc_temperature_floor(U, nx, ny, nz, n_ghost, U_floor);
+    }
+    dti = orc_calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma);
+    if (dt_log) dt_log[steps] = dt;
+    t += dt;
+    steps++;
+    orc_set_boundaries(U, nx, ny, nz, n_fields, n_ghost, flags);
+  }
+  *t_io = t;
+  return steps;
+}
+
+int orc_num_threads()
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

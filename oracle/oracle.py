@@ -1,0 +1,109 @@
+"""ctypes binding of the CPU oracle (libcholla_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker, and by bench.py's cpu_baseline leg.  The product package (cholla_amd)
+never imports this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcholla_oracle.so")
+
+PCM, PLMC, PPMC = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (g++ -ffp-contract=off, OpenMP)."""
+    import subprocess
+
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        d, i, l = ctypes.c_double, ctypes.c_int, ctypes.c_long
+        sig = {
+            "orc_ppm_single_variable": (None, [d, d, d, d, d, _dp, _dp]),
+            "orc_thread_guard": (i, [i, i, i, i, i, i, i]),
+            "orc_load_data": (None, [_dp, i, i, i, i, i, i, i, d, _dp]),
+            "orc_compute_slope": (None, [_dp, _dp, d, _dp]),
+            "orc_van_leer_slope": (None, [_dp, _dp, _dp]),
+            "orc_interface_linear": (None, [_dp, _dp, d, _dp]),
+            "orc_monotonize_char_return_prim": (None, [_dp, _dp, _dp, _dp, _dp, d, d, _dp]),
+            "orc_plm_limit_interfaces": (None, [_dp, _dp, _dp, _dp, _dp]),
+            "orc_write_data": (None, [_dp, _dp, l, l, i, d]),
+            "orc_hydro_inverse_crossing_time": (d, [d, d, d, d, d, d, d, d, d, d]),
+            "orc_fill_uniform": (None, [ctypes.c_uint64, d, d, l, _dp]),
+            "orc_pcm_3d": (None, [_dp] * 7 + [i, i, i, i]),
+            "orc_pcm_1d": (None, [_dp, _dp, _dp, i, i]),
+            "orc_plmc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_ppmc_vl": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_hllc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_update_half_3d": (None, [_dp] * 5 + [i, i, i, d, d, d, d, i, i, d]),
+            "orc_update_3d": (None, [_dp] * 4 + [i, i, i, i, d, d, d, d, i, i, d]),
+            "orc_temperature_floor": (None, [_dp, i, i, i, i, d]),
+            "orc_calc_dti": (d, [_dp, i, i, i, i, d, d, d, d]),
+            "orc_set_ghost_cells": (None, [_dp, i, i, i, i, i, _ip, i, i, i, i, i, i, i]),
+            "orc_set_boundaries": (None, [_dp, i, i, i, i, i, _ip]),
+            "orc_vl3d": (None, [_dp, i, i, i, i, d, d, d, d, d, i, i, i, d]),
+            "orc_vl1d": (None, [_dp, i, i, d, d, d, i, i]),
+            "orc_run": (i, [_dp, i, i, i, i, i, d, d, d, d, i, _ip, _dp, d, i, i, d, i, d, _dp]),
+            "orc_num_threads": (i, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    """Pointer to a C-contiguous float64 array (no copy)."""
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def uniform(seed, a, b, n):
+    out = np.empty(n, dtype=np.float64)
+    lib().orc_fill_uniform(seed, a, b, n, ptr(out))
+    return out
+
+
+def calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma):
+    return lib().orc_calc_dti(ptr(U), nx, ny, nz, n_ghost, dx, dy, dz, gamma)
+
+
+def set_boundaries(U, nx, ny, nz, n_fields, n_ghost, flags):
+    f, fp = iptr(flags)
+    lib().orc_set_boundaries(ptr(U), nx, ny, nz, n_fields, n_ghost, fp)
+
+
+def run(U, nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, recon, flags, t, tout, max_steps,
+        density_floor_on=True, density_floor=0.0, temperature_floor_on=True, U_floor=0.0, dt_log=None):
+    """main.cpp's hydro loop; U (with ghost cells already set) is advanced in place.
+
+    Returns (steps, t)."""
+    f, fp = iptr(flags)
+    tt = ctypes.c_double(t)
+    log = dt_log if dt_log is not None else None
+    n = lib().orc_run(ptr(U), nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, recon, fp,
+                      ctypes.byref(tt), tout, max_steps, int(density_floor_on), density_floor,
+                      int(temperature_floor_on), U_floor, ptr(log) if log is not None else None)
+    return n, tt.value

@@ -1,0 +1,148 @@
+"""Shared test helpers: initial conditions, error norms, the exact Riemann solver."""
+import math
+
+import numpy as np
+
+
+def rel_l1(test, ref):
+    """Per-field relative L1, sum|a-b| / sum|b| (python_scripts/numdiff.py:57-59 convention)."""
+    test = np.asarray(test, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    den = np.abs(ref).sum()
+    num = np.abs(test - ref).sum()
+    return num / den if den > 0 else num
+
+
+def interior(U, n_fields, nz, ny, nx, ng):
+    """View of the real cells, shape (n_fields, nz-2ng, ny-2ng, nx-2ng) (1D: ny=nz=1 keep all)."""
+    V = U.reshape(n_fields, nz, ny, nx)
+    ks = slice(ng, nz - ng) if nz > 1 else slice(0, 1)
+    js = slice(ng, ny - ng) if ny > 1 else slice(0, 1)
+    return V[:, ks, js, ng:nx - ng]
+
+
+def linear_wave_ic(nx_real, ny_real, nz_real, ng, xlen, ylen, zlen, gamma, rho, vx, P, A, eig, wave_length=1.0):
+    """Grid3D::Linear_Wave (src/grid/initial_conditions.cpp:302-436) with yaw = pitch = 0, hydro only.
+
+    Returns the field-major state with total dims (nx, ny, nz) = real + 2*ng.  Ghost
+    cells other than the first lower layer are left zero (the boundary fill sets them)."""
+    nx, ny, nz = nx_real + 2 * ng, ny_real + 2 * ng, nz_real + 2 * ng
+    dx = xlen / nx_real
+    U = np.zeros((5, nz, ny, nx))
+    k = 2.0 * math.pi / wave_length
+    e_rho, e_mx, e_my, e_mz, e_E = eig
+    for i in range(ng - 1, nx - ng):
+        x_pos = 0.0 + dx * (i - ng) + 0.5 * dx
+        s = math.sin(x_pos * k)
+        sl = (slice(ng - 1, nz - ng), slice(ng - 1, ny - ng), i)
+        U[0][sl] = rho + A * e_rho * s
+        U[1][sl] = rho * vx + A * e_mx * s
+        U[2][sl] = A * e_my * s
+        U[3][sl] = A * e_mz * s
+        U[4][sl] = P / (gamma - 1.0) + 0.5 * rho * vx * vx + A * s * e_E
+    return U.reshape(-1).copy(), (nx, ny, nz), (dx, ylen / ny_real, zlen / nz_real)
+
+
+def sod_ic(n_real=100, ng=3, gamma=1.4, rho_l=1.0, p_l=1.0, rho_r=0.1, p_r=0.1, diaph=0.5):
+    """Grid3D::Riemann (initial_conditions.cpp:509-595) on a 1D grid, zero velocities."""
+    nx = n_real + 2 * ng
+    dx = 1.0 / n_real
+    U = np.zeros((5, nx))
+    for i in range(ng - 1, nx - ng):
+        x = 0.0 + dx * (i - ng) + 0.5 * dx
+        if x < diaph:
+            U[0, i], U[4, i] = rho_l, max(p_l, 1e-20) / (gamma - 1.0)
+        else:
+            U[0, i], U[4, i] = rho_r, max(p_r, 1e-20) / (gamma - 1.0)
+    return U.reshape(-1).copy(), nx, dx
+
+
+def exact_riemann(rl, ul, pl, rr, ur, pr, gamma, xi):
+    """Exact solution of the 1D Euler Riemann problem (Toro, Riemann Solvers, ch. 4), sampled at xi = x/t.
+
+    Independent of the reference; returns (rho, u, p) arrays."""
+    cl, cr = math.sqrt(gamma * pl / rl), math.sqrt(gamma * pr / rr)
+    g1, g2 = (gamma - 1) / (2 * gamma), (gamma + 1) / (2 * gamma)
+
+    def f(p, rk, pk, ck):
+        if p > pk:
+            A, B = 2 / ((gamma + 1) * rk), (gamma - 1) / (gamma + 1) * pk
+            q = math.sqrt(A / (p + B))
+            return (p - pk) * q, q * (1 - 0.5 * (p - pk) / (B + p))
+        pr_ = p / pk
+        return 2 * ck / (gamma - 1) * (pr_ ** g1 - 1), 1 / (rk * ck) * pr_ ** (-g2)
+
+    p = max(1e-8, 0.5 * (pl + pr))
+    for _ in range(100):
+        fl, dfl = f(p, rl, pl, cl)
+        fr, dfr = f(p, rr, pr, cr)
+        dp = (fl + fr + ur - ul) / (dfl + dfr)
+        pn = max(1e-12, p - dp)
+        if abs(pn - p) < 1e-15 * (pn + p):
+            p = pn
+            break
+        p = pn
+    fl, _ = f(p, rl, pl, cl)
+    fr, _ = f(p, rr, pr, cr)
+    um = 0.5 * (ul + ur) + 0.5 * (fr - fl)
+
+    rho = np.empty_like(xi)
+    u = np.empty_like(xi)
+    pp = np.empty_like(xi)
+    for n, s in enumerate(xi):
+        if s <= um:  # left of contact
+            if p > pl:  # left shock
+                sl = ul - cl * math.sqrt(g2 * p / pl + g1)
+                if s <= sl:
+                    rho[n], u[n], pp[n] = rl, ul, pl
+                else:
+                    r = (p / pl + (gamma - 1) / (gamma + 1)) / ((gamma - 1) / (gamma + 1) * p / pl + 1)
+                    rho[n], u[n], pp[n] = rl * r, um, p
+            else:  # left rarefaction
+                shl = ul - cl
+                cml = cl * (p / pl) ** g1
+                stl = um - cml
+                if s <= shl:
+                    rho[n], u[n], pp[n] = rl, ul, pl
+                elif s > stl:
+                    rho[n], u[n], pp[n] = rl * (p / pl) ** (1 / gamma), um, p
+                else:
+                    c = 2 / (gamma + 1) * (cl + (gamma - 1) / 2 * (ul - s))
+                    rho[n] = rl * (c / cl) ** (2 / (gamma - 1))
+                    u[n] = 2 / (gamma + 1) * (cl + (gamma - 1) / 2 * ul + s)
+                    pp[n] = pl * (c / cl) ** (2 * gamma / (gamma - 1))
+        else:  # right of contact
+            if p > pr:  # right shock
+                sr = ur + cr * math.sqrt(g2 * p / pr + g1)
+                if s >= sr:
+                    rho[n], u[n], pp[n] = rr, ur, pr
+                else:
+                    r = (p / pr + (gamma - 1) / (gamma + 1)) / ((gamma - 1) / (gamma + 1) * p / pr + 1)
+                    rho[n], u[n], pp[n] = rr * r, um, p
+            else:  # right rarefaction
+                shr = ur + cr
+                cmr = cr * (p / pr) ** g1
+                str_ = um + cmr
+                if s >= shr:
+                    rho[n], u[n], pp[n] = rr, ur, pr
+                elif s < str_:
+                    rho[n], u[n], pp[n] = rr * (p / pr) ** (1 / gamma), um, p
+                else:
+                    c = 2 / (gamma + 1) * (cr - (gamma - 1) / 2 * (ur - s))
+                    rho[n] = rr * (c / cr) ** (2 / (gamma - 1))
+                    u[n] = 2 / (gamma + 1) * (-cr + (gamma - 1) / 2 * ur + s)
+                    pp[n] = pr * (c / cr) ** (2 * gamma / (gamma - 1))
+    return rho, u, pp
+
+
+def exact_riemann_cell_average(rl, ul, pl, rr, ur, pr, gamma, x_edges, x0, t, sub=64):
+    """Cell averages of the exact solution's conserved variables over cells with the given edges."""
+    n = len(x_edges) - 1
+    out = np.zeros((3, n))
+    for c in range(n):
+        xs = x_edges[c] + (np.arange(sub) + 0.5) / sub * (x_edges[c + 1] - x_edges[c])
+        r, u, p = exact_riemann(rl, ul, pl, rr, ur, pr, gamma, (xs - x0) / t)
+        out[0, c] = r.mean()
+        out[1, c] = (r * u).mean()
+        out[2, c] = (p / (gamma - 1) + 0.5 * r * u * u).mean()
+    return out
