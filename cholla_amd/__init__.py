@@ -1,0 +1,263 @@
+"""cholla_amd -- Python host binding of libcholla_amd.so (the MI355X hydro hot path).
+
+The product is the C ABI in include/cholla_amd.h; this module binds it with
+ctypes so tests, the smoke check and the benchmark can drive it.  Device memory
+is allocated through the library itself (no torch types cross the boundary).
+Every call raises ChollaError on a non-zero status.  There is no CPU fallback:
+without the built library or a HIP device the calls fail loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcholla_amd.so")
+
+CHL_OK = 0
+RECON_PCM, RECON_PLMC, RECON_PPMC = 0, 1, 2
+RIEMANN_HLLC = 0
+PREC_FP64_EXACT, PREC_FP64, PREC_MIXED = 0, 1, 2
+PRECISIONS = {"fp64_exact": PREC_FP64_EXACT, "fp64": PREC_FP64, "mixed": PREC_MIXED}
+BC_PERIODIC, BC_REFLECTIVE, BC_TRANSMISSIVE = 1, 2, 3
+
+# every symbol include/cholla_amd.h declares
+EXPORTS = [
+    "chl_version", "chl_last_error", "chl_device_count", "chl_device_name", "chl_malloc", "chl_free",
+    "chl_memcpy_htod", "chl_memcpy_dtoh", "chl_memcpy_dtod", "chl_synchronize", "chl_hydro_create",
+    "chl_hydro_destroy", "chl_hydro_vl_step", "chl_hydro_update_grid", "chl_hydro_run", "chl_hydro_time_steps",
+    "chl_hydro_kernel_times", "chl_hydro_autotune", "chl_hydro_get_config", "chl_calc_dt",
+    "chl_apply_temperature_floor", "chl_set_ghost_cells", "chl_set_boundaries", "chl_reconstruct",
+    "chl_hllc_fluxes", "chl_ic_kelvin_helmholtz",
+]
+
+
+class ChollaError(RuntimeError):
+    pass
+
+
+class HydroConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("n_ghost", ctypes.c_int),
+        ("n_fields", ctypes.c_int), ("reconstruction", ctypes.c_int), ("riemann", ctypes.c_int),
+        ("precision", ctypes.c_int), ("density_floor", ctypes.c_int), ("temperature_floor", ctypes.c_int),
+        ("bc_flags", ctypes.c_int * 6), ("tile1", ctypes.c_int), ("tile2", ctypes.c_int),
+        ("kchunk1", ctypes.c_int), ("kchunk2", ctypes.c_int),
+    ]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_d = ctypes.c_double
+_i = ctypes.c_int
+
+
+def lib():
+    """Load the built library (raises if it was never built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ChollaError(f"{LIB_PATH} is missing: build it with `python -m cholla_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "chl_version": (ctypes.c_char_p, []),
+        "chl_last_error": (ctypes.c_char_p, []),
+        "chl_device_count": (_i, [_ip]),
+        "chl_device_name": (_i, [ctypes.c_char_p, _i]),
+        "chl_malloc": (_i, [ctypes.POINTER(_vp), ctypes.c_size_t]),
+        "chl_free": (_i, [_vp]),
+        "chl_memcpy_htod": (_i, [_vp, _vp, ctypes.c_size_t]),
+        "chl_memcpy_dtoh": (_i, [_vp, _vp, ctypes.c_size_t]),
+        "chl_memcpy_dtod": (_i, [_vp, _vp, ctypes.c_size_t]),
+        "chl_synchronize": (_i, []),
+        "chl_hydro_create": (_i, [ctypes.POINTER(HydroConfig), ctypes.POINTER(_vp)]),
+        "chl_hydro_destroy": (_i, [_vp]),
+        "chl_hydro_vl_step": (_i, [_vp, _vp, _d, _d, _d, _d, _d, _d]),
+        "chl_hydro_update_grid": (_i, [_vp, _vp, _d, _d, _d, _d, _d, _d, _d, _i, _dp]),
+        "chl_hydro_run": (_i, [_vp, _vp, _d, _d, _d, _d, _d, _d, _d, _d, _i, _i, _dp, _ip]),
+        "chl_hydro_time_steps": (_i, [_vp, _vp, _d, _d, _d, _d, _i, _i, ctypes.POINTER(ctypes.c_float), _dp]),
+        "chl_hydro_kernel_times": (_i, [_vp, _vp, _d, _d, _d, _d, _i] + [ctypes.POINTER(ctypes.c_float)] * 3),
+        "chl_hydro_autotune": (_i, [_vp, _vp, _d, _d, _d, _d, _i, _ip, _ip, _ip, _ip]),
+        "chl_hydro_get_config": (_i, [_vp, ctypes.POINTER(HydroConfig)]),
+        "chl_calc_dt": (_i, [_vp, _i, _i, _i, _i, _i, _d, _d, _d, _d, _dp]),
+        "chl_apply_temperature_floor": (_i, [_vp, _i, _i, _i, _i, _i, _d]),
+        "chl_set_ghost_cells": (_i, [_vp, _i, _i, _i, _i, _i, _i, _ip, _i, _i, _i, _i, _i, _i, _i]),
+        "chl_set_boundaries": (_i, [_vp, _i, _i, _i, _i, _i, _ip]),
+        "chl_reconstruct": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _d, _i]),
+        "chl_hllc_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
+        "chl_ic_kelvin_helmholtz": (_i, [_dp, _i, _i, _d, ctypes.c_uint64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != CHL_OK:
+        raise ChollaError(f"cholla_amd error {rc}: {lib().chl_last_error().decode()}")
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    lib().chl_device_count(ctypes.byref(n))
+    return n.value
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    check(lib().chl_device_name(buf, 256))
+    return buf.value.decode()
+
+
+class DeviceArray:
+    """A float64 device buffer owned through the library."""
+
+    def __init__(self, n, fill=None):
+        self.n = int(n)
+        self.nbytes = self.n * 8
+        p = _vp()
+        check(lib().chl_malloc(ctypes.byref(p), max(self.nbytes, 8)))
+        self.ptr = p
+        if fill is not None:
+            self.upload(fill)
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        d = cls(a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        assert a.size == self.n
+        check(lib().chl_memcpy_htod(self.ptr, a.ctypes.data_as(_vp), self.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, dtype=np.float64)
+        check(lib().chl_memcpy_dtoh(out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None:
+            lib().chl_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Hydro:
+    """Integrator handle for one box (chl_hydro_* entry points)."""
+
+    def __init__(self, nx, ny, nz, n_ghost, reconstruction=RECON_PPMC, precision=PREC_FP64_EXACT,
+                 bc=(1, 1, 1, 1, 1, 1), density_floor=True, temperature_floor=True, tile1=-1, tile2=-1,
+                 kchunk1=0, kchunk2=0):
+        cfg = HydroConfig()
+        cfg.nx, cfg.ny, cfg.nz, cfg.n_ghost, cfg.n_fields = nx, ny, nz, n_ghost, 5
+        cfg.reconstruction, cfg.riemann, cfg.precision = reconstruction, RIEMANN_HLLC, precision
+        cfg.density_floor, cfg.temperature_floor = int(density_floor), int(temperature_floor)
+        for q in range(6):
+            cfg.bc_flags[q] = bc[q]
+        cfg.tile1, cfg.tile2, cfg.kchunk1, cfg.kchunk2 = tile1, tile2, kchunk1, kchunk2
+        h = _vp()
+        check(lib().chl_hydro_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.dims = (nx, ny, nz)
+        self.n_ghost = n_ghost
+
+    def config(self):
+        c = HydroConfig()
+        check(lib().chl_hydro_get_config(self.h, ctypes.byref(c)))
+        return c
+
+    def vl_step(self, U, dx, dy, dz, dt, gamma, density_floor=0.0):
+        check(lib().chl_hydro_vl_step(self.h, U.ptr, dx, dy, dz, dt, gamma, density_floor))
+
+    def update_grid(self, U, dx, dy, dz, dt, gamma, density_floor=0.0, U_floor=0.0, fill_ghosts=True):
+        dti = ctypes.c_double()
+        check(lib().chl_hydro_update_grid(self.h, U.ptr, dx, dy, dz, dt, gamma, density_floor, U_floor,
+                                          int(fill_ghosts), ctypes.byref(dti)))
+        return dti.value
+
+    def run(self, U, dx, dy, dz, gamma, t0, tout, max_steps, use_graph=True, density_floor=0.0, U_floor=0.0):
+        t = ctypes.c_double()
+        n = ctypes.c_int()
+        check(lib().chl_hydro_run(self.h, U.ptr, dx, dy, dz, gamma, density_floor, U_floor, t0, tout, max_steps,
+                                  int(use_graph), ctypes.byref(t), ctypes.byref(n)))
+        return n.value, t.value
+
+    def time_steps(self, U, dx, dy, dz, gamma, n_steps, use_graph=True):
+        ms = ctypes.c_float()
+        dt = ctypes.c_double()
+        check(lib().chl_hydro_time_steps(self.h, U.ptr, dx, dy, dz, gamma, n_steps, int(use_graph), ctypes.byref(ms),
+                                         ctypes.byref(dt)))
+        return ms.value, dt.value
+
+    def kernel_times(self, U, dx, dy, dz, gamma, n_steps):
+        a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        check(lib().chl_hydro_kernel_times(self.h, U.ptr, dx, dy, dz, gamma, n_steps, ctypes.byref(a),
+                                           ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def autotune(self, U, dx, dy, dz, gamma, steps=2):
+        out = [ctypes.c_int() for _ in range(4)]
+        check(lib().chl_hydro_autotune(self.h, U.ptr, dx, dy, dz, gamma, steps, *[ctypes.byref(o) for o in out]))
+        return tuple(o.value for o in out)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().chl_hydro_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma):
+    out = ctypes.c_double()
+    check(lib().chl_calc_dt(U.ptr, nx, ny, nz, n_ghost, 5, dx, dy, dz, gamma, ctypes.byref(out)))
+    return out.value
+
+
+def set_boundaries(U, nx, ny, nz, n_ghost, flags, n_fields=5):
+    f = (ctypes.c_int * 6)(*flags)
+    check(lib().chl_set_boundaries(U.ptr, nx, ny, nz, n_fields, n_ghost, f))
+
+
+def set_ghost_cells(U, nx, ny, nz, n_ghost, flags, isize, jsize, ksize, imin, jmin, kmin, direction, n_fields=5):
+    f = (ctypes.c_int * 6)(*flags)
+    check(lib().chl_set_ghost_cells(U.ptr, nx, ny, nz, n_fields, nx * ny * nz, n_ghost, f, isize, jsize, ksize, imin,
+                                    jmin, kmin, direction))
+
+
+def apply_temperature_floor(U, nx, ny, nz, n_ghost, U_floor):
+    check(lib().chl_apply_temperature_floor(U.ptr, nx, ny, nz, n_ghost, 5, U_floor))
+
+
+def reconstruct(recon, U, QL, QR, nx, ny, nz, gamma, direction):
+    check(lib().chl_reconstruct(recon, U.ptr, QL.ptr, QR.ptr, nx, ny, nz, gamma, direction))
+
+
+def hllc_fluxes(QL, QR, F, nx, ny, nz, gamma, direction):
+    check(lib().chl_hllc_fluxes(QL.ptr, QR.ptr, F.ptr, nx, ny, nz, gamma, direction))
+
+
+def ic_kelvin_helmholtz(n, ng, gamma=5.0 / 3.0, seed=2024):
+    """Host-built KH state (5 x (n+2ng)^3, ghosts filled periodically)."""
+    N = n + 2 * ng
+    U = np.empty(5 * N ** 3, dtype=np.float64)
+    check(lib().chl_ic_kelvin_helmholtz(U.ctypes.data_as(_dp), n, ng, gamma, seed))
+    return U

@@ -1,0 +1,661 @@
+// capi.cpp -- the extern "C" boundary of libcholla_amd.so (declared in include/cholla_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/cholla_amd.h"
+#include "support_kernels.hpp"
+#include "vl3d_launch.hpp"
+
+namespace
+{
+thread_local std::string g_err;
+
+int Fail(int code, const std::string &msg)
+{
+  g_err = msg;
+  return code;
+}
+
+#define CHL_HIP(expr)                                                                                \
+  do {                                                                                               \
+    hipError_t e_ = (expr);                                                                          \
+    if (e_ != hipSuccess) return Fail(CHL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHL_TRY(expr)       \
+  do {                      \
+    int rc_ = (expr);       \
+    if (rc_ != CHL_OK) return rc_; \
+  } while (0)
+
+constexpr double kCfl = 0.3;  // C_cfl, src/grid/grid3D.cpp:157
+
+int ReconRadius(int recon) { return recon == CHL_RECON_PPMC ? 2 : (recon == CHL_RECON_PLMC ? 1 : 0); }
+
+}  // namespace
+
+struct chl_hydro {
+  chl_hydro_config cfg{};
+  size_t n_cells = 0;
+  bool one_d     = false;
+  int num_cu     = 256;
+  void *W        = nullptr;  // half-step primitives
+  size_t W_bytes = 0;
+  // device scalars: [0] dti bits, [1] dt, [2] t, [3] n_step (int in a double slot)
+  unsigned char *scal = nullptr;
+  hipStream_t stream  = nullptr;
+  hipGraphExec_t graph = nullptr;
+  // key of the captured graph
+  double *g_U = nullptr;
+  double g_key[7]{};
+
+  unsigned long long *dti() { return reinterpret_cast<unsigned long long *>(scal); }
+  double *dt() { return reinterpret_cast<double *>(scal + 8); }
+  double *t() { return reinterpret_cast<double *>(scal + 16); }
+  int *nstep() { return reinterpret_cast<int *>(scal + 24); }
+
+  bool PeriodicFusable() const
+  {
+    for (int q = 0; q < 6; q++)
+      if (cfg.bc_flags[q] != CHL_BC_PERIODIC) return false;
+    int ng = cfg.n_ghost;
+    return cfg.nx - 2 * ng >= ng && cfg.ny - 2 * ng >= ng && cfg.nz - 2 * ng >= ng;
+  }
+};
+
+namespace
+{
+
+int CheckConfig(const chl_hydro_config &c)
+{
+  if (c.n_fields != 5) return Fail(CHL_ERR_UNSUPPORTED, "n_fields must be 5 (hydro without scalars or dual energy)");
+  if (c.riemann != CHL_RIEMANN_HLLC) return Fail(CHL_ERR_UNSUPPORTED, "only the HLLC solver is built");
+  if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC)
+    return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PLMC or PPMC");
+  if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
+  const int ng = c.n_ghost, r = ReconRadius(c.reconstruction);
+  if (ng < r + 2) return Fail(CHL_ERR_INVALID, "n_ghost too small for the reconstruction (PLMC needs 3, PPMC 4)");
+  if (c.nx < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "nx must exceed 2*n_ghost");
+  bool one_d = c.ny == 1 && c.nz == 1;
+  if (!one_d) {
+    if (c.ny < 2 * ng + 1 || c.nz < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "3D boxes need ny, nz > 2*n_ghost");
+    if (c.precision != CHL_PREC_FP64_EXACT || true) {
+      // any precision is fine in 3D
+    }
+    if (c.tile1 >= chl::kNumTiles || c.tile2 >= chl::kNumTiles) return Fail(CHL_ERR_INVALID, "tile index out of range");
+  } else {
+    if (c.precision != CHL_PREC_FP64_EXACT)
+      return Fail(CHL_ERR_UNSUPPORTED, "1D boxes run the FP64 reference-order form only");
+  }
+  // 2^31 guard on per-axis products used in 32-bit grid arithmetic of launch shapes
+  if (double(c.nx) * c.ny * c.nz * 5 > 9.0e15) return Fail(CHL_ERR_INVALID, "box too large");
+  for (int q = 0; q < 6; q++)
+    if (c.bc_flags[q] < 1 || c.bc_flags[q] > 3)
+      return Fail(CHL_ERR_UNSUPPORTED, "boundary flags must be 1 (periodic), 2 (reflective) or 3 (transmissive)");
+  return CHL_OK;
+}
+
+int DefaultChunk(int n_real) { return std::max(8, std::min(n_real, 32)); }
+
+// Enqueue one VL step.  mode: 0 = VL only, 1 = VL + floors + dti, (fill) ghost refresh.
+int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double gamma, double dt_host,
+                const double *dt_dev, double density_floor, double U_floor, bool update_grid, bool fill)
+{
+  const chl_hydro_config &c = h->cfg;
+  const int ng              = c.n_ghost;
+  if (h->one_d) {
+    const int n = c.nx;
+    bool fuse_dt = update_grid && n <= 1024;
+    CHL_HIP(chl::LaunchVl1d(U, reinterpret_cast<double *>(h->W), n, ng, c.reconstruction, dx, dt_dev, dt_host, gamma,
+                            fuse_dt ? h->dti() : nullptr, h->stream));
+    if (update_grid && !fuse_dt) {
+      CHL_HIP(hipMemsetAsync(h->dti(), 0, 8, h->stream));
+      CHL_HIP(chl::LaunchCalcDt(U, n, 1, 1, ng, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
+    }
+    if (fill) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
+    return CHL_OK;
+  }
+  chl::StepArgs a{};
+  a.U = U, a.W = h->W, a.nx = c.nx, a.ny = c.ny, a.nz = c.nz, a.ng = ng;
+  a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dt = dt_host, a.dt_dev = dt_dev, a.dti = h->dti();
+  a.density_floor = density_floor, a.U_floor = U_floor;
+  int flags = 0;
+  if (c.density_floor) flags |= chl::kFlagDensityFloor;
+  if (update_grid && c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
+  if (update_grid) flags |= chl::kFlagReduceDt;
+  const bool fused_ghosts = fill && h->PeriodicFusable();
+  if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
+  a.flags       = flags;
+  const int R1  = ReconRadius(c.reconstruction) + 1;
+  const int pol = c.precision;
+  a.kchunk      = c.kchunk1;
+  hipError_t e  = pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, c.tile1, h->stream)
+                                             : chl::LaunchStage1Fast(pol, a, R1, c.tile1, h->stream);
+  CHL_HIP(e);
+  a.kchunk = c.kchunk2;
+  e        = pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage2Exact(a, c.reconstruction, c.tile2, h->stream)
+                                        : chl::LaunchStage2Fast(pol, a, c.reconstruction, c.tile2, h->stream);
+  CHL_HIP(e);
+  if (fill && !fused_ghosts) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
+  return CHL_OK;
+}
+
+int ReadDti(chl_hydro *h, double *dti_out)
+{
+  unsigned long long bits = 0;
+  CHL_HIP(hipMemcpyAsync(&bits, h->dti(), 8, hipMemcpyDeviceToHost, h->stream));
+  CHL_HIP(hipStreamSynchronize(h->stream));
+  double v;
+  std::memcpy(&v, &bits, 8);
+  *dti_out = v;
+  return CHL_OK;
+}
+
+int InitialDti(chl_hydro *h, const double *U, double dx, double dy, double dz, double gamma)
+{
+  const chl_hydro_config &c = h->cfg;
+  CHL_HIP(hipMemsetAsync(h->dti(), 0, 8, h->stream));
+  CHL_HIP(chl::LaunchCalcDt(U, c.nx, c.ny, c.nz, c.n_ghost, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
+  return CHL_OK;
+}
+
+int SetTimeState(chl_hydro *h, double t0)
+{
+  unsigned char buf[16];
+  double z = 0.0;
+  int n0   = 0;
+  std::memcpy(buf, &t0, 8);
+  std::memcpy(buf + 8, &n0, 4);
+  std::memset(buf + 12, 0, 4);
+  (void)z;
+  CHL_HIP(hipMemcpyAsync(h->t(), buf, 16, hipMemcpyHostToDevice, h->stream));
+  return CHL_OK;
+}
+
+// One device-timed step: dt from dti on the device, then the fused step with ghost refresh.
+int EnqueueDeviceStep(chl_hydro *h, double *U, double dx, double dy, double dz, double gamma, double density_floor,
+                      double U_floor, double tout)
+{
+  CHL_HIP(chl::LaunchDtAdvance(h->dti(), h->dt(), h->t(), tout, kCfl, h->nstep(), h->stream));
+  return EnqueueStep(h, U, dx, dy, dz, gamma, 0.0, h->dt(), density_floor, U_floor, true, true);
+}
+
+int EnsureGraph(chl_hydro *h, double *U, double dx, double dy, double dz, double gamma, double density_floor,
+                double U_floor, double tout)
+{
+  const double key[7] = {dx, dy, dz, gamma, density_floor, U_floor, tout};
+  if (h->graph && h->g_U == U && std::memcmp(key, h->g_key, sizeof(key)) == 0) return CHL_OK;
+  if (h->graph) {
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  hipGraph_t g;
+  CHL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc        = EnqueueDeviceStep(h, U, dx, dy, dz, gamma, density_floor, U_floor, tout);
+  hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+  if (rc != CHL_OK) return rc;
+  CHL_HIP(e2);
+  hipError_t e3 = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  CHL_HIP(e3);
+  h->g_U = U;
+  std::memcpy(h->g_key, key, sizeof(key));
+  return CHL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *chl_version(void) { return "cholla_amd 0.1 (gfx950)"; }
+const char *chl_last_error(void) { return g_err.c_str(); }
+
+int chl_device_count(int *n)
+{
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *n = c;
+  return CHL_OK;
+}
+
+int chl_device_name(char *buf, int len)
+{
+  int dev;
+  CHL_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  CHL_HIP(hipGetDeviceProperties(&p, dev));
+  std::snprintf(buf, size_t(len), "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+  return CHL_OK;
+}
+
+int chl_malloc(void **ptr, size_t bytes)
+{
+  CHL_HIP(hipMalloc(ptr, bytes));
+  return CHL_OK;
+}
+int chl_free(void *ptr)
+{
+  CHL_HIP(hipFree(ptr));
+  return CHL_OK;
+}
+int chl_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+  CHL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return CHL_OK;
+}
+int chl_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+  CHL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return CHL_OK;
+}
+int chl_memcpy_dtod(void *dst, const void *src, size_t bytes)
+{
+  CHL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return CHL_OK;
+}
+int chl_synchronize(void)
+{
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
+{
+  if (!cfg || !out) return Fail(CHL_ERR_INVALID, "null argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return Fail(CHL_ERR_NO_DEVICE, "no HIP device");
+  chl_hydro_config c = *cfg;
+  if (c.tile1 < 0) c.tile1 = 0;
+  if (c.tile2 < 0) c.tile2 = 3;
+  CHL_TRY(CheckConfig(c));
+  auto *h    = new chl_hydro();
+  h->one_d   = c.ny == 1 && c.nz == 1;
+  h->n_cells = size_t(c.nx) * c.ny * c.nz;
+  if (c.kchunk1 <= 0) c.kchunk1 = DefaultChunk(c.nz - 2 * c.n_ghost + 2 * (ReconRadius(c.reconstruction) + 1));
+  if (c.kchunk2 <= 0) c.kchunk2 = DefaultChunk(c.nz - 2 * c.n_ghost);
+  h->cfg = c;
+  int dev;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) {
+    delete h;
+    return Fail(CHL_ERR_HIP, "hipGetDeviceProperties failed");
+  }
+  h->num_cu        = p.multiProcessorCount;
+  size_t elem      = c.precision == CHL_PREC_MIXED ? sizeof(float) : sizeof(double);
+  h->W_bytes       = 5 * h->n_cells * elem;
+  hipError_t e     = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&h->W, h->W_bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(h->W, 0, h->W_bytes, h->stream);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&h->scal), 64);
+  if (e == hipSuccess) e = hipMemsetAsync(h->scal, 0, 64, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    chl_hydro_destroy(h);
+    return Fail(CHL_ERR_HIP, std::string("allocating integrator scratch: ") + hipGetErrorString(e));
+  }
+  *out = h;
+  return CHL_OK;
+}
+
+int chl_hydro_destroy(chl_hydro *h)
+{
+  if (!h) return CHL_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  if (h->W) (void)hipFree(h->W);
+  if (h->scal) (void)hipFree(h->scal);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return CHL_OK;
+}
+
+int chl_hydro_get_config(const chl_hydro *h, chl_hydro_config *out)
+{
+  if (!h || !out) return Fail(CHL_ERR_INVALID, "null argument");
+  *out = h->cfg;
+  return CHL_OK;
+}
+
+int chl_hydro_vl_step(chl_hydro *h, double *U, double dx, double dy, double dz, double dt, double gamma,
+                      double density_floor)
+{
+  if (!h || !U) return Fail(CHL_ERR_INVALID, "null argument");
+  CHL_TRY(EnqueueStep(h, U, dx, dy, dz, gamma, dt, nullptr, density_floor, 0.0, false, false));
+  CHL_HIP(hipStreamSynchronize(h->stream));
+  return CHL_OK;
+}
+
+int chl_hydro_update_grid(chl_hydro *h, double *U, double dx, double dy, double dz, double dt, double gamma,
+                          double density_floor, double U_floor, int fill_ghosts, double *dti_out)
+{
+  if (!h || !U) return Fail(CHL_ERR_INVALID, "null argument");
+  CHL_TRY(EnqueueStep(h, U, dx, dy, dz, gamma, dt, nullptr, density_floor, U_floor, true, fill_ghosts != 0));
+  if (dti_out) return ReadDti(h, dti_out);
+  CHL_HIP(hipStreamSynchronize(h->stream));
+  return CHL_OK;
+}
+
+int chl_hydro_run(chl_hydro *h, double *U, double dx, double dy, double dz, double gamma, double density_floor,
+                  double U_floor, double t0, double tout, int max_steps, int use_graph, double *t_out, int *steps_out)
+{
+  if (!h || !U) return Fail(CHL_ERR_INVALID, "null argument");
+  CHL_TRY(InitialDti(h, U, dx, dy, dz, gamma));
+  CHL_TRY(SetTimeState(h, t0));
+  if (use_graph) CHL_TRY(EnsureGraph(h, U, dx, dy, dz, gamma, density_floor, U_floor, tout));
+  const int check_every = 32;
+  double t              = t0;
+  int done              = 0;
+  while (done < max_steps && t < tout) {
+    int batch = std::min(check_every, max_steps - done);
+    for (int s = 0; s < batch; s++) {
+      if (use_graph) {
+        CHL_HIP(hipGraphLaunch(h->graph, h->stream));
+      } else {
+        CHL_TRY(EnqueueDeviceStep(h, U, dx, dy, dz, gamma, density_floor, U_floor, tout));
+      }
+    }
+    done += batch;
+    CHL_HIP(hipMemcpyAsync(&t, h->t(), 8, hipMemcpyDeviceToHost, h->stream));
+    CHL_HIP(hipStreamSynchronize(h->stream));
+  }
+  int ns = 0;
+  CHL_HIP(hipMemcpy(&ns, h->nstep(), 4, hipMemcpyDeviceToHost));
+  if (t_out) *t_out = t;
+  if (steps_out) *steps_out = ns;
+  return CHL_OK;
+}
+
+int chl_hydro_time_steps(chl_hydro *h, double *U, double dx, double dy, double dz, double gamma, int n_steps,
+                         int use_graph, float *ms_out, double *dt_last_out)
+{
+  if (!h || !U || n_steps < 0) return Fail(CHL_ERR_INVALID, "bad argument");
+  const double tout = std::numeric_limits<double>::infinity();
+  CHL_TRY(InitialDti(h, U, dx, dy, dz, gamma));
+  CHL_TRY(SetTimeState(h, 0.0));
+  if (use_graph) CHL_TRY(EnsureGraph(h, U, dx, dy, dz, gamma, 0.0, 0.0, tout));
+  hipEvent_t e0, e1;
+  CHL_HIP(hipEventCreate(&e0));
+  CHL_HIP(hipEventCreate(&e1));
+  CHL_HIP(hipStreamSynchronize(h->stream));
+  CHL_HIP(hipEventRecord(e0, h->stream));
+  for (int s = 0; s < n_steps; s++) {
+    if (use_graph) {
+      CHL_HIP(hipGraphLaunch(h->graph, h->stream));
+    } else {
+      CHL_TRY(EnqueueDeviceStep(h, U, dx, dy, dz, gamma, 0.0, 0.0, tout));
+    }
+  }
+  CHL_HIP(hipEventRecord(e1, h->stream));
+  CHL_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  CHL_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_out) *ms_out = ms;
+  if (dt_last_out) CHL_HIP(hipMemcpy(dt_last_out, h->dt(), 8, hipMemcpyDeviceToHost));
+  return CHL_OK;
+}
+
+int chl_hydro_kernel_times(chl_hydro *h, double *U, double dx, double dy, double dz, double gamma, int n_steps,
+                           float *ms1, float *ms2, float *ms3)
+{
+  if (!h || !U || h->one_d) return Fail(CHL_ERR_INVALID, "bad argument (3D boxes only)");
+  const chl_hydro_config &c = h->cfg;
+  const double tout         = std::numeric_limits<double>::infinity();
+  CHL_TRY(InitialDti(h, U, dx, dy, dz, gamma));
+  CHL_TRY(SetTimeState(h, 0.0));
+  hipEvent_t ev[4];
+  for (auto &e : ev) CHL_HIP(hipEventCreate(&e));
+  double acc[3] = {0, 0, 0};
+  chl::StepArgs a{};
+  a.U = U, a.W = h->W, a.nx = c.nx, a.ny = c.ny, a.nz = c.nz, a.ng = c.n_ghost;
+  a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dt = 0.0, a.dt_dev = h->dt(), a.dti = h->dti();
+  a.density_floor = 0.0, a.U_floor = 0.0;
+  int flags = chl::kFlagReduceDt;
+  if (c.density_floor) flags |= chl::kFlagDensityFloor;
+  if (c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
+  const bool fused_ghosts = h->PeriodicFusable();
+  if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
+  a.flags       = flags;
+  const int R1  = ReconRadius(c.reconstruction) + 1;
+  const int pol = c.precision;
+  for (int s = 0; s < n_steps; s++) {
+    CHL_HIP(hipEventRecord(ev[0], h->stream));
+    CHL_HIP(chl::LaunchDtAdvance(h->dti(), h->dt(), h->t(), tout, kCfl, h->nstep(), h->stream));
+    a.kchunk = c.kchunk1;
+    CHL_HIP(hipEventRecord(ev[1], h->stream));
+    CHL_HIP(pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, c.tile1, h->stream)
+                                       : chl::LaunchStage1Fast(pol, a, R1, c.tile1, h->stream));
+    CHL_HIP(hipEventRecord(ev[2], h->stream));
+    a.kchunk = c.kchunk2;
+    CHL_HIP(pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage2Exact(a, c.reconstruction, c.tile2, h->stream)
+                                       : chl::LaunchStage2Fast(pol, a, c.reconstruction, c.tile2, h->stream));
+    CHL_HIP(hipEventRecord(ev[3], h->stream));
+    if (!fused_ghosts)
+      CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, c.n_ghost, c.bc_flags, h->stream));
+    CHL_HIP(hipEventSynchronize(ev[3]));
+    float t01, t12, t23;
+    CHL_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    CHL_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    CHL_HIP(hipEventElapsedTime(&t23, ev[2], ev[3]));
+    acc[0] += t12, acc[1] += t23, acc[2] += t01;
+  }
+  CHL_HIP(hipStreamSynchronize(h->stream));
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (ms1) *ms1 = float(acc[0] / std::max(1, n_steps));
+  if (ms2) *ms2 = float(acc[1] / std::max(1, n_steps));
+  if (ms3) *ms3 = float(acc[2] / std::max(1, n_steps));
+  return CHL_OK;
+}
+
+int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, double dz, double gamma, int steps,
+                       int *tile1, int *tile2, int *kchunk1, int *kchunk2)
+{
+  if (!h || !U || h->one_d) return Fail(CHL_ERR_INVALID, "bad argument (3D boxes only)");
+  chl_hydro_config &c = h->cfg;
+  const size_t bytes  = size_t(c.n_fields) * h->n_cells * sizeof(double);
+  double *S           = nullptr;
+  CHL_HIP(hipMalloc(&S, bytes));
+  hipEvent_t e0, e1;
+  CHL_HIP(hipEventCreate(&e0));
+  CHL_HIP(hipEventCreate(&e1));
+  chl::StepArgs a{};
+  a.U = S, a.W = h->W, a.nx = c.nx, a.ny = c.ny, a.nz = c.nz, a.ng = c.n_ghost;
+  a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dti = h->dti();
+  a.flags = chl::kFlagReduceDt | (c.density_floor ? chl::kFlagDensityFloor : 0) |
+            (c.temperature_floor ? chl::kFlagTemperatureFloor : 0);
+  // a small fixed dt keeps every candidate on the same, physical state
+  {
+    double dti = 0;
+    CHL_TRY(chl_calc_dt(U, c.nx, c.ny, c.nz, c.n_ghost, c.n_fields, dx, dy, dz, gamma, &dti));
+    a.dt = 0.1 * kCfl / dti;
+  }
+  const int R1  = ReconRadius(c.reconstruction) + 1;
+  const int pol = c.precision;
+  const int nz1 = c.nz - 2 * c.n_ghost + 2 * R1, nz2 = c.nz - 2 * c.n_ghost;
+  std::vector<int> chunks = {8, 16, 32, 64, 128};
+  float best1 = 1e30f, best2 = 1e30f;
+  int b_t1 = c.tile1, b_t2 = c.tile2, b_k1 = c.kchunk1, b_k2 = c.kchunk2;
+  int rc = CHL_OK;
+  for (int t = 0; t < chl::kNumTiles && rc == CHL_OK; t++) {
+    for (int kc : chunks) {
+      if (kc > 2 * nz1 && kc != chunks.front()) continue;
+      a.kchunk = std::min(kc, nz1);
+      float ms = 0;
+      for (int rep = 0; rep < 2; rep++) {  // rep 0 warms up
+        if (hipMemcpyAsync(S, U, bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) rc = CHL_ERR_HIP;
+        (void)hipEventRecord(e0, h->stream);
+        for (int s = 0; s < steps; s++) {
+          hipError_t e = pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, t, h->stream)
+                                                    : chl::LaunchStage1Fast(pol, a, R1, t, h->stream);
+          if (e != hipSuccess) rc = Fail(CHL_ERR_HIP, hipGetErrorString(e));
+        }
+        (void)hipEventRecord(e1, h->stream);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
+      }
+      if (ms < best1) best1 = ms, b_t1 = t, b_k1 = a.kchunk;
+    }
+  }
+  for (int t = 0; t < chl::kNumTiles && rc == CHL_OK; t++) {
+    for (int kc : chunks) {
+      if (kc > 2 * nz2 && kc != chunks.front()) continue;
+      a.kchunk = std::min(kc, nz2);
+      float ms = 0;
+      for (int rep = 0; rep < 2; rep++) {
+        if (hipMemcpyAsync(S, U, bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) rc = CHL_ERR_HIP;
+        a.kchunk = c.kchunk1;
+        (void)(pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, b_t1, h->stream)
+                                          : chl::LaunchStage1Fast(pol, a, R1, b_t1, h->stream));
+        a.kchunk = std::min(kc, nz2);
+        (void)hipEventRecord(e0, h->stream);
+        for (int s = 0; s < steps; s++) {
+          hipError_t e = pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage2Exact(a, c.reconstruction, t, h->stream)
+                                                    : chl::LaunchStage2Fast(pol, a, c.reconstruction, t, h->stream);
+          if (e != hipSuccess) rc = Fail(CHL_ERR_HIP, hipGetErrorString(e));
+        }
+        (void)hipEventRecord(e1, h->stream);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
+      }
+      if (ms < best2) best2 = ms, b_t2 = t, b_k2 = a.kchunk;
+    }
+  }
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(S);
+  if (rc != CHL_OK) return rc;
+  c.tile1 = b_t1, c.tile2 = b_t2, c.kchunk1 = b_k1, c.kchunk2 = b_k2;
+  if (h->graph) {
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  if (tile1) *tile1 = b_t1;
+  if (tile2) *tile2 = b_t2;
+  if (kchunk1) *kchunk1 = b_k1;
+  if (kchunk2) *kchunk2 = b_k2;
+  return CHL_OK;
+}
+
+int chl_calc_dt(const double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double dx, double dy, double dz,
+                double gamma, double *dti_out)
+{
+  (void)n_fields;  // unused in 3D, as in the reference (grid3D.cpp:119 passes n_cells here)
+  if (!U || !dti_out) return Fail(CHL_ERR_INVALID, "null argument");
+  unsigned long long *d = nullptr;
+  CHL_HIP(hipMalloc(&d, 8));
+  int num_cu = 256, dev;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) num_cu = p.multiProcessorCount;
+  hipError_t e = hipMemset(d, 0, 8);
+  if (e == hipSuccess) e = chl::LaunchCalcDt(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma, d, num_cu, nullptr);
+  unsigned long long bits = 0;
+  if (e == hipSuccess) e = hipMemcpy(&bits, d, 8, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  CHL_HIP(e);
+  std::memcpy(dti_out, &bits, 8);
+  return CHL_OK;
+}
+
+int chl_apply_temperature_floor(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double U_floor)
+{
+  (void)n_fields;
+  if (!U) return Fail(CHL_ERR_INVALID, "null argument");
+  CHL_HIP(chl::LaunchTemperatureFloor(U, nx, ny, nz, n_ghost, U_floor, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_set_ghost_cells(double *c_head, int nx, int ny, int nz, int n_fields, int n_cells, int n_ghost,
+                        const int flags[6], int isize, int jsize, int ksize, int imin, int jmin, int kmin, int dir)
+{
+  if (!c_head || !flags) return Fail(CHL_ERR_INVALID, "null argument");
+  if (long(nx) * ny * nz != long(n_cells)) return Fail(CHL_ERR_INVALID, "n_cells != nx*ny*nz");
+  if (flags[dir] == 4 || flags[dir] == 5) return CHL_OK;
+  CHL_HIP(chl::LaunchGhostSlab(c_head, nx, ny, nz, n_fields, n_ghost, flags, isize, jsize, ksize, imin, jmin, kmin,
+                               dir, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_set_boundaries(double *U, int nx, int ny, int nz, int n_fields, int n_ghost, const int flags_in[6])
+{
+  if (!U || !flags_in) return Fail(CHL_ERR_INVALID, "null argument");
+  int flags[6] = {0, 0, 0, 0, 0, 0};
+  if (nx > 1) flags[0] = flags_in[0], flags[1] = flags_in[1];
+  if (ny > 1) flags[2] = flags_in[2], flags[3] = flags_in[3];
+  if (nz > 1) flags[4] = flags_in[4], flags[5] = flags_in[5];
+  CHL_HIP(chl::LaunchGhostFill(U, nx, ny, nz, n_fields, n_ghost, flags, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_reconstruct(int recon, const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
+{
+  if (!U || !QL || !QR || dir < 0 || dir > 2) return Fail(CHL_ERR_INVALID, "bad argument");
+  if (recon != CHL_RECON_PLMC && recon != CHL_RECON_PPMC) return Fail(CHL_ERR_UNSUPPORTED, "PLMC or PPMC only");
+  CHL_HIP(chl::LaunchReconGrid(recon, U, QL, QR, nx, ny, nz, gamma, dir, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_hllc_fluxes(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma, int dir)
+{
+  if (!QL || !QR || !F || dir < 0 || dir > 2) return Fail(CHL_ERR_INVALID, "bad argument");
+  CHL_HIP(chl::LaunchHllcGrid(QL, QR, F, nx, ny, nz, gamma, dir, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_ic_kelvin_helmholtz(double *U, int n, int ng, double gamma, uint64_t seed)
+{
+  if (!U || n < ng || ng < 1) return Fail(CHL_ERR_INVALID, "bad argument");
+  const int N     = n + 2 * ng;
+  const size_t nc = size_t(N) * N * N;
+  const double dx = 1.0 / n;
+  std::mt19937_64 prng(seed);
+  std::uniform_real_distribution<double> noise(-1.0, 1.0);
+  for (int k = ng; k < N - ng; k++)
+    for (int j = ng; j < N - ng; j++)
+      for (int i = ng; i < N - ng; i++) {
+        // cell centres as Grid3D::Get_Position (src/grid/grid3D.cpp:84-86)
+        double x = 0.0 + dx * (i - ng) + 0.5 * dx;
+        double y = 0.0 + dx * (j - ng) + 0.5 * dx;
+        double u = noise(prng);
+        bool in  = std::fabs(y - 0.5) < 0.25;
+        double d = in ? 2.0 : 1.0, vx = in ? 0.5 : -0.5;
+        double vy = 0.01 * std::sin(4.0 * M_PI * x) + 1.0e-4 * u, vz = 0.0, p = 2.5;
+        size_t id     = size_t(i) + size_t(j) * N + size_t(k) * N * N;
+        U[id]         = d;
+        U[nc + id]    = d * vx;
+        U[2 * nc + id] = d * vy;
+        U[3 * nc + id] = d * vz;
+        // Calc_Energy_Primitive, src/utils/hydro_utilities.h:106-118
+        U[4 * nc + id] = (std::fmax(p, 1.0e-20) / (gamma - 1.)) + 0.5 * d * (vx * vx + ((vy * vy) + (vz * vz)));
+      }
+  // periodic ghost cells (flag 1 on every face)
+  auto wrap = [&](int a) { return a < ng ? a + n : (a >= N - ng ? a - n : a); };
+  for (int k = 0; k < N; k++)
+    for (int j = 0; j < N; j++)
+      for (int i = 0; i < N; i++) {
+        int ir = wrap(i), jr = wrap(j), kr = wrap(k);
+        if (ir == i && jr == j && kr == k) continue;
+        size_t g = size_t(i) + size_t(j) * N + size_t(k) * N * N;
+        size_t s = size_t(ir) + size_t(jr) * N + size_t(kr) * N * N;
+        for (int f = 0; f < 5; f++) U[f * nc + g] = U[f * nc + s];
+      }
+  return CHL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,491 @@
+// hydro_physics.hpp -- device physics of the VL hydro step for gfx950.
+//
+// One source, three arithmetic policies:
+//   PolicyExact  FP64, every expression in the reference's operation order and
+//                association; the translation unit that instantiates it is
+//                compiled with -ffp-contract=off so results are bit-identical to
+//                a strict IEEE restatement of the reference kernels.
+//   PolicyFast   FP64, divisions by a common denominator replaced by one
+//                reciprocal and products, FMA contraction allowed.
+//   PolicyMixed  FP32 reconstruction / characteristic projection / Riemann
+//                solve with hardware rcp and sqrt; the conserved state, its
+//                flux-difference accumulation, pressure recovery and dt stay FP64
+//                (handled by the callers, which convert at the boundary).
+//
+// States are held "rotated": vx / mx is the component normal to the sweep
+// direction, (vy, vz) the two transverse ones in the reference's cyclic order
+// (o1, o2, o3) = (x,y,z), (y,z,x), (z,x,y)  (src/riemann_solvers/hllc_cuda.cu:54-69,
+// src/reconstruction/ppmc_cuda.cu:557-574).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace chl
+{
+
+constexpr double kTinyNumber = 1.0e-20;  // TINY_NUMBER, src/global/global.h:25
+
+struct PolicyExact {
+  using T                      = double;
+  static constexpr bool kExact = true;
+  static constexpr int kId     = 0;
+};
+struct PolicyFast {
+  using T                      = double;
+  static constexpr bool kExact = false;
+  static constexpr int kId     = 1;
+};
+struct PolicyMixed {
+  using T                      = float;
+  static constexpr bool kExact = false;
+  static constexpr int kId     = 2;
+};
+
+template <typename T>
+struct Prim {
+  T d, vx, vy, vz, p;
+};
+template <typename T>
+struct Cons {
+  T d, mx, my, mz, E;
+};
+template <typename T>
+struct Char5 {
+  T a0, a1, a2, a3, a4;
+};
+
+// ---------------------------------------------------------------------------- scalar helpers
+template <class P>
+__device__ __forceinline__ typename P::T Sqrt(typename P::T x)
+{
+  if constexpr (P::kId == PolicyMixed::kId) {
+    return __builtin_amdgcn_sqrtf(x);
+  } else {
+    return __builtin_sqrt(x);
+  }
+}
+
+// Reciprocal used only by the non-exact policies (dead code under PolicyExact).
+template <class P>
+__device__ __forceinline__ typename P::T Rcp(typename P::T x)
+{
+  if constexpr (P::kId == PolicyMixed::kId) {
+    return __builtin_amdgcn_rcpf(x);
+  } else {
+    return 1.0 / x;
+  }
+}
+
+// a / b, or a * inv_b where the caller precomputed inv_b = Rcp(b)
+template <class P>
+__device__ __forceinline__ typename P::T Div(typename P::T a, typename P::T b, typename P::T inv_b)
+{
+  if constexpr (P::kExact) {
+    return a / b;
+  } else {
+    return a * inv_b;
+  }
+}
+
+// a / b where no shared reciprocal exists
+template <class P>
+__device__ __forceinline__ typename P::T Div1(typename P::T a, typename P::T b)
+{
+  if constexpr (P::kExact) {
+    return a / b;
+  } else {
+    return a * Rcp<P>(b);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T Fmax(T a, T b)
+{
+  return __builtin_fmax(a, b);
+}
+template <typename T>
+__device__ __forceinline__ T Fmin(T a, T b)
+{
+  return __builtin_fmin(a, b);
+}
+template <typename T>
+__device__ __forceinline__ T Fabs(T a)
+{
+  return __builtin_fabs(a);
+}
+template <typename T>
+__device__ __forceinline__ T SignOf(T a)  // SIGN macro, src/global/global.h:144
+{
+  return (a < T(0)) ? T(-1) : T(1);
+}
+
+// math_utils::SquareMagnitude, src/utils/math_utilities.h:78-97
+template <typename T>
+__device__ __forceinline__ T SquareMag(T a, T b, T c)
+{
+  return a * a + ((b * b) + (c * c));
+}
+
+// ---------------------------------------------------------------------------- EOS
+// hydro_utilities::Calc_Pressure_Primitive, src/utils/hydro_utilities.h:34-45
+template <typename T>
+__device__ __forceinline__ T PressureFromCons(T E, T d, T vx, T vy, T vz, T gamma)
+{
+  T pressure = E - T(0.5) * d * SquareMag(vx, vy, vz);
+  return Fmax((gamma - T(1.)) * pressure, T(kTinyNumber));
+}
+
+// reconstruction::Load_Data, src/reconstruction/reconstruction.h:139-212: velocities
+// by division, pressure by Calc_Pressure_Primitive.  Always evaluated in double:
+// the mixed form recovers pressure in FP64 and narrows afterwards.
+__device__ __forceinline__ Prim<double> LoadPrim(double d, double mx, double my, double mz, double E, double gamma)
+{
+  Prim<double> q;
+  q.d  = d;
+  q.vx = mx / d;
+  q.vy = my / d;
+  q.vz = mz / d;
+  q.p  = PressureFromCons(E, d, q.vx, q.vy, q.vz, gamma);
+  return q;
+}
+
+// hydro_utilities::Calc_Energy_Primitive, src/utils/hydro_utilities.h:106-118
+template <class P>
+__device__ __forceinline__ typename P::T EnergyFromPrim(const Prim<typename P::T> &s, typename P::T gamma,
+                                                        typename P::T inv_gm1)
+{
+  using T = typename P::T;
+  return Div<P>(Fmax(s.p, T(kTinyNumber)), gamma - T(1.), inv_gm1) + T(0.5) * s.d * SquareMag(s.vx, s.vy, s.vz);
+}
+
+// ---------------------------------------------------------------------------- characteristic maps (hydro)
+template <class P>
+struct SoundSpeed {
+  using T = typename P::T;
+  T c, c2;
+  T inv_2c, inv_2c2, inv_c2, inv_d;
+  __device__ __forceinline__ SoundSpeed(const Prim<T> &q, T gamma)
+  {
+    c       = Sqrt<P>(gamma * q.p / q.d);  // Calc_Sound_Speed, hydro_utilities.h:209-212
+    c2      = c * c;
+    inv_2c  = Rcp<P>(T(2.0) * c);
+    inv_2c2 = Rcp<P>(T(2.0) * c2);
+    inv_c2  = Rcp<P>(c2);
+    inv_d   = Rcp<P>(q.d);
+  }
+};
+
+// Primitive_To_Characteristic (hydro branch), src/reconstruction/reconstruction.h:423-431
+template <class P>
+__device__ __forceinline__ Char5<typename P::T> PrimToChar(const Prim<typename P::T> &q, const Prim<typename P::T> &s,
+                                                           const SoundSpeed<P> &cs)
+{
+  using T = typename P::T;
+  Char5<T> a;
+  a.a0 = Div<P>(-q.d * s.vx, T(2.0) * cs.c, cs.inv_2c) + Div<P>(s.p, T(2.0) * cs.c2, cs.inv_2c2);
+  a.a1 = s.d - Div<P>(s.p, cs.c2, cs.inv_c2);
+  a.a2 = s.vy;
+  a.a3 = s.vz;
+  a.a4 = Div<P>(q.d * s.vx, T(2.0) * cs.c, cs.inv_2c) + Div<P>(s.p, T(2.0) * cs.c2, cs.inv_2c2);
+  return a;
+}
+
+// Characteristic_To_Primitive (hydro branch), reconstruction.h:482-488
+template <class P>
+__device__ __forceinline__ Prim<typename P::T> CharToPrim(const Prim<typename P::T> &q, const Char5<typename P::T> &a,
+                                                          const SoundSpeed<P> &cs)
+{
+  using T = typename P::T;
+  Prim<T> o;
+  o.d  = a.a0 + a.a1 + a.a4;
+  o.vx = Div<P>(cs.c, q.d, cs.inv_d) * (a.a4 - a.a0);
+  o.vy = a.a2;
+  o.vz = a.a3;
+  o.p  = cs.c2 * (a.a0 + a.a4);
+  return o;
+}
+
+// ---------------------------------------------------------------------------- PPM limiter
+// reconstruction::PPM_Single_Variable, src/reconstruction/reconstruction.h:794-901
+template <class P>
+__device__ __forceinline__ void PpmSingle(typename P::T im2, typename P::T im1, typename P::T i0, typename P::T ip1,
+                                          typename P::T ip2, typename P::T &L_iph, typename P::T &R_imh)
+{
+  using T            = typename P::T;
+  const T C2         = T(1.25);
+  const T inv6       = T(1.0 / 6.0);
+  auto limit_iface = [&](T c_i, T c_im1, T iface, T s2_i, T s2_im1) -> T {
+    T s2c     = T(3.0) * (c_im1 + c_i - T(2.0) * iface);
+    T limited = T(0.0);
+    if (SignOf(s2c) == SignOf(s2_im1) && SignOf(s2c) == SignOf(s2_i)) {
+      limited = SignOf(s2c) * Fmin(C2 * Fabs(s2_im1), Fmin(C2 * Fabs(s2_i), Fabs(s2c)));
+    }
+    T dl = iface - c_im1;
+    T dr = c_i - iface;
+    if (dl * dr < T(0.0)) return T(0.5) * (c_im1 + c_i) - Div<P>(limited, T(6.0), inv6);
+    return iface;
+  };
+
+  T slope_left    = (i0 - im1);
+  T slope_right   = (ip1 - i0);
+  T slope_avg_im1 = T(0.5) * slope_left + T(0.5) * (im1 - im2);
+  T slope_avg_i   = T(0.5) * slope_right + T(0.5) * slope_left;
+  T slope_avg_ip1 = T(0.5) * (ip2 - ip1) + T(0.5) * slope_right;
+
+  R_imh = T(0.5) * (im1 + i0) + Div<P>(slope_avg_im1 - slope_avg_i, T(6.0), inv6);
+  L_iph = T(0.5) * (i0 + ip1) + Div<P>(slope_avg_i - slope_avg_ip1, T(6.0), inv6);
+
+  T s2_im1 = im2 + i0 - T(2.0) * im1;
+  T s2_i   = im1 + ip1 - T(2.0) * i0;
+  T s2_ip1 = i0 + ip2 - T(2.0) * ip1;
+
+  R_imh = limit_iface(i0, im1, R_imh, s2_i, s2_im1);
+  L_iph = limit_iface(ip1, i0, L_iph, s2_ip1, s2_i);
+
+  T s2_face    = T(6.0) * (R_imh + L_iph - T(2.0) * i0);
+  T s2_limited = T(0.0);
+  if (SignOf(s2_im1) == SignOf(s2_i) && SignOf(s2_im1) == SignOf(s2_ip1) && SignOf(s2_im1) == SignOf(s2_face)) {
+    s2_limited =
+        SignOf(s2_face) * Fmin(Fmin(C2 * Fabs(s2_im1), C2 * Fabs(s2_i)), Fmin(C2 * Fabs(s2_ip1), Fabs(s2_face)));
+  }
+
+  T cell_max = Fmax(Fabs(im2), Fabs(im1));
+  cell_max   = Fmax(cell_max, Fabs(i0));
+  cell_max   = Fmax(cell_max, Fabs(ip1));
+  cell_max   = Fmax(cell_max, Fabs(ip2));
+
+  // The 1e-12 round-off guards are the reference's; in FP32 they are below the
+  // format's resolution and act as "exactly zero" tests, which keeps the same
+  // branch structure.
+  T rho = (Fabs(s2_face) > T(1.0e-12) * cell_max) ? Div1<P>(s2_limited, s2_face) : T(0.0);
+
+  T sfl = i0 - R_imh;
+  T sfr = L_iph - i0;
+  if ((sfl * sfr) <= T(0.0) || ((ip1 - i0) * (i0 - im1)) <= T(0.0)) {
+    if (rho <= (T(1.0) - T(1.0e-12))) {
+      R_imh = i0 - rho * sfl;
+      L_iph = i0 + rho * sfr;
+    }
+  } else {
+    if (Fabs(sfl) >= T(2.0) * Fabs(sfr)) R_imh = i0 - T(2.0) * sfr;
+    if (Fabs(sfr) >= T(2.0) * Fabs(sfl)) L_iph = i0 + T(2.0) * sfl;
+  }
+}
+
+// ---------------------------------------------------------------------------- PPMC (VL)
+// PPMC_VL body, src/reconstruction/ppmc_cuda.cu:603-684: project the 5-point stencil
+// on cell i's characteristics, limit each characteristic field with PPM, project
+// back and floor density/pressure.  s[0..4] = cells i-2..i+2 (rotated primitives).
+template <class P>
+__device__ __forceinline__ void ReconPpmc(const Prim<typename P::T> s[5], typename P::T gamma,
+                                          Prim<typename P::T> &L_iph, Prim<typename P::T> &R_imh)
+{
+  using T = typename P::T;
+  const Prim<T> &q = s[2];
+  SoundSpeed<P> cs(q, gamma);
+  Char5<T> a[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) a[m] = PrimToChar<P>(q, s[m], cs);
+  Char5<T> L, R;
+  PpmSingle<P>(a[0].a0, a[1].a0, a[2].a0, a[3].a0, a[4].a0, L.a0, R.a0);
+  PpmSingle<P>(a[0].a1, a[1].a1, a[2].a1, a[3].a1, a[4].a1, L.a1, R.a1);
+  PpmSingle<P>(a[0].a2, a[1].a2, a[2].a2, a[3].a2, a[4].a2, L.a2, R.a2);
+  PpmSingle<P>(a[0].a3, a[1].a3, a[2].a3, a[3].a3, a[4].a3, L.a3, R.a3);
+  PpmSingle<P>(a[0].a4, a[1].a4, a[2].a4, a[3].a4, a[4].a4, L.a4, R.a4);
+  L_iph   = CharToPrim<P>(q, L, cs);
+  R_imh   = CharToPrim<P>(q, R, cs);
+  R_imh.d = Fmax(R_imh.d, T(kTinyNumber));
+  L_iph.d = Fmax(L_iph.d, T(kTinyNumber));
+  R_imh.p = Fmax(R_imh.p, T(kTinyNumber));
+  L_iph.p = Fmax(L_iph.p, T(kTinyNumber));
+}
+
+// ---------------------------------------------------------------------------- PLMC (VL)
+template <typename T>
+__device__ __forceinline__ T VlSlope(T l, T r, T inv_sum)  // Van_Leer_Slope, reconstruction.h:266-272
+{
+  return (l * r > T(0.0)) ? T(2.0) * l * r / (l + r) : T(0.0);
+}
+
+// Monotonize lambda, reconstruction.h:519-527
+template <typename T>
+__device__ __forceinline__ T MonotonizeOne(T left, T right, T centered, T van_leer)
+{
+  if (left * right > T(0.0)) {
+    T lim_a = T(2.0) * Fmin(Fabs(left), Fabs(right));
+    T lim_b = Fmin(Fabs(centered), Fabs(van_leer));
+    return __builtin_copysign(Fmin(lim_a, lim_b), centered);
+  }
+  return T(0.0);
+}
+
+// PLMC_cuda body (VL build: no characteristic tracing), src/reconstruction/plmc_cuda.cu:73-291.
+// s[0..2] = cells i-1, i, i+1.
+template <class P>
+__device__ __forceinline__ void ReconPlmc(const Prim<typename P::T> s[3], typename P::T gamma,
+                                          Prim<typename P::T> &L_iph, Prim<typename P::T> &R_imh)
+{
+  using T = typename P::T;
+  const Prim<T> &im = s[0], &q = s[1], &ip = s[2];
+  SoundSpeed<P> cs(q, gamma);
+  // Compute_Slope (coef 1 and 0.5), reconstruction.h:224-251
+  Prim<T> dL{T(1.0) * (q.d - im.d), T(1.0) * (q.vx - im.vx), T(1.0) * (q.vy - im.vy), T(1.0) * (q.vz - im.vz),
+             T(1.0) * (q.p - im.p)};
+  Prim<T> dR{T(1.0) * (ip.d - q.d), T(1.0) * (ip.vx - q.vx), T(1.0) * (ip.vy - q.vy), T(1.0) * (ip.vz - q.vz),
+             T(1.0) * (ip.p - q.p)};
+  Prim<T> dC{T(0.5) * (ip.d - im.d), T(0.5) * (ip.vx - im.vx), T(0.5) * (ip.vy - im.vy), T(0.5) * (ip.vz - im.vz),
+             T(0.5) * (ip.p - im.p)};
+  Prim<T> dG{VlSlope(dL.d, dR.d, T(0)), VlSlope(dL.vx, dR.vx, T(0)), VlSlope(dL.vy, dR.vy, T(0)),
+             VlSlope(dL.vz, dR.vz, T(0)), VlSlope(dL.p, dR.p, T(0))};
+  Char5<T> aL = PrimToChar<P>(q, dL, cs), aR = PrimToChar<P>(q, dR, cs), aC = PrimToChar<P>(q, dC, cs),
+           aG = PrimToChar<P>(q, dG, cs);
+  Char5<T> m;
+  m.a0       = MonotonizeOne(aL.a0, aR.a0, aC.a0, aG.a0);
+  m.a1       = MonotonizeOne(aL.a1, aR.a1, aC.a1, aG.a1);
+  m.a2       = MonotonizeOne(aL.a2, aR.a2, aC.a2, aG.a2);
+  m.a3       = MonotonizeOne(aL.a3, aR.a3, aC.a3, aG.a3);
+  m.a4       = MonotonizeOne(aL.a4, aR.a4, aC.a4, aG.a4);
+  Prim<T> dm = CharToPrim<P>(q, m, cs);
+  // Calc_Interface_Linear, reconstruction.h:645-673
+  const T hp = T(1.0) * T(0.5), hm = T(-1.0) * T(0.5);
+  L_iph = {q.d + hp * dm.d, q.vx + hp * dm.vx, q.vy + hp * dm.vy, q.vz + hp * dm.vz, q.p + hp * dm.p};
+  R_imh = {q.d + hm * dm.d, q.vx + hm * dm.vx, q.vy + hm * dm.vy, q.vz + hm * dm.vz, q.p + hm * dm.p};
+  // Plm_Limit_Interfaces, reconstruction.h:686-723
+  auto lim = [](T &l_iph, T &r_imh, T v_imo, T v_i, T v_ipo) {
+    r_imh = Fmax(Fmin(v_i, v_imo), r_imh);
+    r_imh = Fmin(Fmax(v_i, v_imo), r_imh);
+    l_iph = Fmax(Fmin(v_i, v_ipo), l_iph);
+    l_iph = Fmin(Fmax(v_i, v_ipo), l_iph);
+  };
+  lim(L_iph.d, R_imh.d, im.d, q.d, ip.d);
+  lim(L_iph.vx, R_imh.vx, im.vx, q.vx, ip.vx);
+  lim(L_iph.vy, R_imh.vy, im.vy, q.vy, ip.vy);
+  lim(L_iph.vz, R_imh.vz, im.vz, q.vz, ip.vz);
+  lim(L_iph.p, R_imh.p, im.p, q.p, ip.p);
+  R_imh.d = Fmax(R_imh.d, T(kTinyNumber));
+  L_iph.d = Fmax(L_iph.d, T(kTinyNumber));
+  R_imh.p = Fmax(R_imh.p, T(kTinyNumber));
+  L_iph.p = Fmax(L_iph.p, T(kTinyNumber));
+}
+
+// ---------------------------------------------------------------------------- HLLC
+// Riemann input for one side: conserved and primitive forms of the same state.
+template <typename T>
+struct RState {
+  T d, mx, my, mz, E;  // conserved (rotated)
+  T vx, vy, vz, p;     // primitive
+  T inv_d;             // 1/d (non-exact policies only)
+};
+
+// Side state from a conserved interface value, as Calculate_HLLC_Fluxes_CUDA
+// derives it (hllc_cuda.cu:104-131): velocities by division, pressure from the
+// energy with the sum ((vx^2 + vy^2) + vz^2), floored at TINY_NUMBER.
+template <class P>
+__device__ __forceinline__ RState<typename P::T> RStateFromCons(typename P::T d, typename P::T mx, typename P::T my,
+                                                                typename P::T mz, typename P::T E, typename P::T gamma)
+{
+  using T = typename P::T;
+  RState<T> s;
+  s.d = d, s.mx = mx, s.my = my, s.mz = mz, s.E = E;
+  s.inv_d = Rcp<P>(d);
+  s.vx    = Div<P>(mx, d, s.inv_d);
+  s.vy    = Div<P>(my, d, s.inv_d);
+  s.vz    = Div<P>(mz, d, s.inv_d);
+  s.p     = (E - T(0.5) * d * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz)) * (gamma - T(1.0));
+  s.p     = Fmax(s.p, T(kTinyNumber));
+  return s;
+}
+
+// Side state from a reconstructed primitive interface value.  PolicyExact goes
+// through the reference's conserved round trip (Write_Data then the solver's own
+// primitive recovery, reconstruction.h:918-958 + hllc_cuda.cu:104-131); the other
+// policies use the primitive values directly.
+template <class P>
+__device__ __forceinline__ RState<typename P::T> RStateFromPrim(const Prim<typename P::T> &s, typename P::T gamma,
+                                                                typename P::T inv_gm1)
+{
+  using T = typename P::T;
+  T d = s.d, mx = s.d * s.vx, my = s.d * s.vy, mz = s.d * s.vz;
+  T E = EnergyFromPrim<P>(s, gamma, inv_gm1);
+  if constexpr (P::kExact) {
+    return RStateFromCons<P>(d, mx, my, mz, E, gamma);
+  } else {
+    RState<T> r;
+    r.d = d, r.mx = mx, r.my = my, r.mz = mz, r.E = E;
+    r.vx = s.vx, r.vy = s.vy, r.vz = s.vz, r.p = s.p;
+    r.inv_d = Rcp<P>(d);
+    return r;
+  }
+}
+
+// Calculate_HLLC_Fluxes_CUDA body, src/riemann_solvers/hllc_cuda.cu:141-305 (etah = 0).
+// Returns the flux in rotated order.
+template <class P>
+__device__ __forceinline__ Cons<typename P::T> Hllc(const RState<typename P::T> &l, const RState<typename P::T> &r,
+                                                    typename P::T gamma)
+{
+  using T    = typename P::T;
+  const T g1 = gamma - T(1.0);
+  T Hl       = Div<P>(l.E + l.p, l.d, l.inv_d);
+  T Hr       = Div<P>(r.E + r.p, r.d, r.inv_d);
+
+  T sqrtdl = Sqrt<P>(l.d), sqrtdr = Sqrt<P>(r.d);
+  T den     = sqrtdl + sqrtdr;
+  T inv_den = Rcp<P>(den);
+  T vx      = Div<P>(sqrtdl * l.vx + sqrtdr * r.vx, den, inv_den);
+  T vy      = Div<P>(sqrtdl * l.vy + sqrtdr * r.vy, den, inv_den);
+  T vz      = Div<P>(sqrtdl * l.vz + sqrtdr * r.vz, den, inv_den);
+  T H       = Div<P>(sqrtdl * Hl + sqrtdr * Hr, den, inv_den);
+  T vsq     = (vx * vx + vy * vy + vz * vz);
+  T asq     = g1 * (H - T(0.5) * vsq);
+  T a       = Sqrt<P>(asq);
+  T lambda_m = vx - a, lambda_p = vx + a;
+  T cfl = Sqrt<P>(Div<P>(gamma * l.p, l.d, l.inv_d));
+  T cfr = Sqrt<P>(Div<P>(gamma * r.p, r.d, r.inv_d));
+  T Sl  = Fmin(lambda_m, l.vx - cfl);
+  T Sr  = Fmax(lambda_p, r.vx + cfr);
+  // H-correction hook with etah = 0 (hllc_cuda.cu:52,174-175)
+  Sl = ((Sl < T(0)) ? T(-1) : T(1)) * Fmax(Fabs(Sl), T(0));
+  Sr = ((Sr < T(0)) ? T(-1) : T(1)) * Fmax(Fabs(Sr), T(0));
+
+  Cons<T> fl{l.mx, l.mx * l.vx + l.p, l.my * l.vx, l.mz * l.vx, (l.E + l.p) * l.vx};
+  Cons<T> fr{r.mx, r.mx * r.vx + r.p, r.my * r.vx, r.mz * r.vx, (r.E + r.p) * r.vx};
+  if (Sl > T(0.0)) return fl;
+  if (Sr < T(0.0)) return fr;
+
+  T Sm = Div1<P>(r.d * r.vx * (Sr - r.vx) - l.d * l.vx * (Sl - l.vx) + l.p - r.p,
+                 r.d * (Sr - r.vx) - l.d * (Sl - l.vx));
+  T ps      = l.d * (l.vx - Sl) * (l.vx - Sm) + l.p;
+  T inv_lsm = Rcp<P>(Sl - Sm), inv_rsm = Rcp<P>(Sr - Sm);
+  T dls     = Div<P>(l.d * (Sl - l.vx), Sl - Sm, inv_lsm);
+  T mxls    = Div<P>(l.mx * (Sl - l.vx) + ps - l.p, Sl - Sm, inv_lsm);
+  T myls = dls * l.vy, mzls = dls * l.vz;
+  T Els  = Div<P>(l.E * (Sl - l.vx) - l.p * l.vx + ps * Sm, Sl - Sm, inv_lsm);
+  T drs  = Div<P>(r.d * (Sr - r.vx), Sr - Sm, inv_rsm);
+  T mxrs = Div<P>(r.mx * (Sr - r.vx) + ps - r.p, Sr - Sm, inv_rsm);
+  T myrs = drs * r.vy, mzrs = drs * r.vz;
+  T Ers  = Div<P>(r.E * (Sr - r.vx) - r.p * r.vx + ps * Sm, Sr - Sm, inv_rsm);
+  T aSm  = Fabs(Sm);
+  Cons<T> f;
+  f.d  = T(0.5) * (fl.d + fr.d + (Sr - aSm) * drs + (Sl + aSm) * dls - Sl * l.d - Sr * r.d);
+  f.mx = T(0.5) * (fl.mx + fr.mx + (Sr - aSm) * mxrs + (Sl + aSm) * mxls - Sl * l.mx - Sr * r.mx);
+  f.my = T(0.5) * (fl.my + fr.my + (Sr - aSm) * myrs + (Sl + aSm) * myls - Sl * l.my - Sr * r.my);
+  f.mz = T(0.5) * (fl.mz + fr.mz + (Sr - aSm) * mzrs + (Sl + aSm) * mzls - Sl * l.mz - Sr * r.mz);
+  f.E  = T(0.5) * (fl.E + fr.E + (Sr - aSm) * Ers + (Sl + aSm) * Els - Sl * l.E - Sr * r.E);
+  return f;
+}
+
+// ---------------------------------------------------------------------------- dt
+// hydroInverseCrossingTime, src/hydro/hydro_cuda.cu:404-418 (FP64 in every form)
+__device__ __forceinline__ double InverseCrossingTime(double E, double d, double d_inv, double vx, double vy, double vz,
+                                                      double dx, double dy, double dz, double gamma)
+{
+  double P  = (E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+  double cs = __builtin_sqrt(d_inv * gamma * P);
+  double m  = Fmax((Fabs(vx) + cs) / dx, (Fabs(vy) + cs) / dy);
+  m         = Fmax(m, (Fabs(vz) + cs) / dz);
+  m         = Fmax(m, 0.0);
+  return m;
+}
+
+}  // namespace chl

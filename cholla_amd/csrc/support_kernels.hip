@@ -1,0 +1,549 @@
+// support_kernels.hip -- the rest of the hot path for gfx950, plus the
+// reference-shaped stage kernels used to check the device physics against the
+// reference's own unit-test vectors.
+//
+// Built with -ffp-contract=off (PolicyExact arithmetic).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hydro_physics.hpp"
+#include "support_kernels.hpp"
+#include "vl3d_kernels.hpp"
+
+namespace chl
+{
+
+// ============================================================================ ghost cells
+// FindIndex, src/grid/cuda_boundaries.cu:240-302
+__device__ __forceinline__ int FindIndexDev(int ig, int n, int flag, int face, int ng, double *a)
+{
+  if (face == 0) {
+    switch (flag) {
+      case 2:
+        *a = -1.0;
+        return 2 * ng - ig - 1;
+      case 3:
+        return ng;
+      case 4:
+        return -1;
+      case 5:
+        return ig;
+      default:
+        return ig + n - 2 * ng;
+    }
+  }
+  switch (flag) {
+    case 2:
+      *a = -1.0;
+      return 2 * (n - ng) - ig - 1;
+    case 3:
+      return n - ng - 1;
+    case 4:
+      return -1;
+    case 5:
+      return ig;
+    default:
+      return ig - n + 2 * ng;
+  }
+}
+
+// One thread per ghost cell of all six faces.  Each ghost cell belongs to exactly
+// one face slab of Set_Boundary_Extents (boundary_conditions.cpp:419-505): z slabs
+// span everything, y slabs the real k range, x slabs the real j and k ranges.  The
+// value written is what SetGhostCellsKernel (cuda_boundaries.cu:71-149) writes for
+// that slab: the mapped real cell (all three indices mapped by SetBoundaryMapping,
+// :161-238), momentum sign flips for reflective faces and the outflow diode of
+// transmissive faces.  Sources are always real cells, so one launch reproduces the
+// six sequential launches exactly.
+__global__ void GhostFillKernel(double *c, int nx, int ny, int nz, int n_fields, int ng, GhostFlags fl, long n_x,
+                                long n_y, long n_z)
+{
+  long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n_x + n_y + n_z) return;
+  const int jl = ny > 1 ? ng : 0, ju = ny > 1 ? ny - ng : ny;
+  const int kl = nz > 1 ? ng : 0, ku = nz > 1 ? nz - ng : nz;
+  int i, j, k, dir;
+  if (t < n_x) {  // x slabs: 2*ng x (ju-jl) x (ku-kl)
+    int w = 2 * ng, hy = ju - jl;
+    int ii = int(t % w);
+    long r = t / w;
+    j      = jl + int(r % hy);
+    k      = kl + int(r / hy);
+    i      = ii < ng ? ii : nx - 2 * ng + ii;
+    dir    = ii < ng ? 0 : 1;
+  } else if (t < n_x + n_y) {  // y slabs: nx x 2*ng x (ku-kl)
+    long s = t - n_x;
+    i      = int(s % nx);
+    long r = s / nx;
+    int jj = int(r % (2 * ng));
+    k      = kl + int(r / (2 * ng));
+    j      = jj < ng ? jj : ny - 2 * ng + jj;
+    dir    = jj < ng ? 2 : 3;
+  } else {  // z slabs: nx x ny x 2*ng
+    long s = t - n_x - n_y;
+    i      = int(s % nx);
+    long r = s / nx;
+    j      = int(r % ny);
+    int kk = int(r / ny);
+    k      = kk < ng ? kk : nz - 2 * ng + kk;
+    dir    = kk < ng ? 4 : 5;
+  }
+  const int *flags = fl.f;
+  if (flags[dir] == 4 || flags[dir] == 5) return;  // custom / MPI faces are set elsewhere
+  double a[3] = {1., 1., 1.};
+  long idx    = 0;
+  int ir, jr, kr;
+  if (nx > 1) {
+    ir = i < ng ? FindIndexDev(i, nx, flags[0], 0, ng, &a[0])
+                : (i >= nx - ng ? FindIndexDev(i, nx, flags[1], 1, ng, &a[0]) : i);
+    if (ir < 0) return;
+    idx += ir;
+  }
+  if (ny > 1) {
+    jr = j < ng ? FindIndexDev(j, ny, flags[2], 0, ng, &a[1])
+                : (j >= ny - ng ? FindIndexDev(j, ny, flags[3], 1, ng, &a[1]) : j);
+    if (jr < 0) return;
+    idx += long(nx) * jr;
+  }
+  if (nz > 1) {
+    kr = k < ng ? FindIndexDev(k, nz, flags[4], 0, ng, &a[2])
+                : (k >= nz - ng ? FindIndexDev(k, nz, flags[5], 1, ng, &a[2]) : k);
+    if (kr < 0) return;
+    idx += long(nx) * ny * kr;
+  }
+  const size_t nc = size_t(nx) * ny * nz;
+  const size_t g  = Lin(i, j, k, nx, ny);
+  for (int f = 0; f < n_fields; f++) c[g + f * nc] = c[idx + f * nc];
+  if (flags[0] == 2 || flags[1] == 2) c[g + nc] *= a[0];
+  if (flags[2] == 2 || flags[3] == 2) c[g + 2 * nc] *= a[1];
+  if (flags[4] == 2 || flags[5] == 2) c[g + 3 * nc] *= a[2];
+  if (flags[dir] == 3) {
+    size_t mom = g + size_t(dir / 2 + 1) * nc;
+    if (dir % 2 == 0) {
+      if (c[mom] > 0.0) {
+        c[g + 4 * nc] -= 0.5 * (c[mom] * c[mom]) / c[g];
+        c[mom] = 0.0;
+      }
+    } else {
+      if (c[mom] < 0.0) {
+        c[g + 4 * nc] -= 0.5 * (c[mom] * c[mom]) / c[g];
+        c[mom] = 0.0;
+      }
+    }
+  }
+}
+
+// One face slab exactly as SetGhostCells(...) launches it (cuda_boundaries.cu:151-159).
+__global__ void GhostSlabKernel(double *c, int nx, int ny, int nz, int n_fields, int ng, GhostFlags fl, int isize,
+                                int jsize, int ksize, int imin, int jmin, int kmin, int dir)
+{
+  long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= long(isize) * jsize * ksize) return;
+  int k = int(id / (long(isize) * jsize));
+  int j = int((id - long(k) * isize * jsize) / isize);
+  int i = int(id - long(k) * isize * jsize - long(j) * isize);
+  i += imin, j += jmin, k += kmin;
+  const int *flags = fl.f;
+  double a[3]      = {1., 1., 1.};
+  long idx         = 0;
+  if (nx > 1) {
+    int ir = i < ng ? FindIndexDev(i, nx, flags[0], 0, ng, &a[0])
+                    : (i >= nx - ng ? FindIndexDev(i, nx, flags[1], 1, ng, &a[0]) : i);
+    if (ir < 0) return;
+    idx += ir;
+  }
+  if (ny > 1) {
+    int jr = j < ng ? FindIndexDev(j, ny, flags[2], 0, ng, &a[1])
+                    : (j >= ny - ng ? FindIndexDev(j, ny, flags[3], 1, ng, &a[1]) : j);
+    if (jr < 0) return;
+    idx += long(nx) * jr;
+  }
+  if (nz > 1) {
+    int kr = k < ng ? FindIndexDev(k, nz, flags[4], 0, ng, &a[2])
+                    : (k >= nz - ng ? FindIndexDev(k, nz, flags[5], 1, ng, &a[2]) : k);
+    if (kr < 0) return;
+    idx += long(nx) * ny * kr;
+  }
+  const size_t nc = size_t(nx) * ny * nz;
+  const size_t g  = Lin(i, j, k, nx, ny);
+  for (int f = 0; f < n_fields; f++) c[g + f * nc] = c[idx + f * nc];
+  if (flags[0] == 2 || flags[1] == 2) c[g + nc] *= a[0];
+  if (flags[2] == 2 || flags[3] == 2) c[g + 2 * nc] *= a[1];
+  if (flags[4] == 2 || flags[5] == 2) c[g + 3 * nc] *= a[2];
+  if (flags[dir] == 3) {
+    size_t mom = g + size_t(dir / 2 + 1) * nc;
+    if (dir % 2 == 0) {
+      if (c[mom] > 0.0) {
+        c[g + 4 * nc] -= 0.5 * (c[mom] * c[mom]) / c[g];
+        c[mom] = 0.0;
+      }
+    } else {
+      if (c[mom] < 0.0) {
+        c[g + 4 * nc] -= 0.5 * (c[mom] * c[mom]) / c[g];
+        c[mom] = 0.0;
+      }
+    }
+  }
+}
+
+// ============================================================================ dt and floors
+__device__ __forceinline__ void BlockMaxToGlobal(double v, unsigned long long *out)
+{
+  __shared__ double red[16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = Fmax(v, __shfl_xor(v, off, 64));
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int q = 1; q < int(blockDim.x >> 6); q++) m = Fmax(m, red[q]);
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// Calc_dt_3D, src/hydro/hydro_cuda.cu:514-560 (grid-stride over all cells, real cells only)
+__global__ void CalcDt3DKernel(const double *U, int nx, int ny, int nz, int ng, double dx, double dy, double dz,
+                               double gamma, unsigned long long *dti)
+{
+  const size_t nc = size_t(nx) * ny * nz;
+  double m        = 0.0;
+  for (size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x; id < nc; id += size_t(blockDim.x) * gridDim.x) {
+    int zid = int(id / (size_t(nx) * ny));
+    int yid = int((id - size_t(zid) * nx * ny) / nx);
+    int xid = int(id - size_t(zid) * nx * ny - size_t(yid) * nx);
+    if (xid > ng - 1 && xid < nx - ng && yid > ng - 1 && yid < ny - ng && zid > ng - 1 && zid < nz - ng) {
+      double d = U[id], dinv = 1.0 / d;
+      double vx = U[nc + id] * dinv, vy = U[2 * nc + id] * dinv, vz = U[3 * nc + id] * dinv;
+      m = Fmax(m, InverseCrossingTime(U[4 * nc + id], d, dinv, vx, vy, vz, dx, dy, dz, gamma));
+    }
+  }
+  BlockMaxToGlobal(m, dti);
+}
+
+// Calc_dt_1D, src/hydro/hydro_cuda.cu:437-472
+__global__ void CalcDt1DKernel(const double *U, int nx, int ng, double dx, double gamma, unsigned long long *dti)
+{
+  const size_t nc = size_t(nx);
+  double m        = 0.0;
+  for (size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x; id < nc; id += size_t(blockDim.x) * gridDim.x) {
+    if (long(id) > long(ng) - 1 && long(id) < long(nx) - ng) {
+      double d = U[id], dinv = 1.0 / d;
+      double vx = U[nc + id] * dinv, vy = U[2 * nc + id] * dinv, vz = U[3 * nc + id] * dinv;
+      double P  = (U[4 * nc + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+      P         = Fmax(P, kTinyNumber);
+      double cs = __builtin_sqrt(dinv * gamma * P);
+      m         = Fmax(m, (Fabs(vx) + cs) / dx);
+    }
+  }
+  BlockMaxToGlobal(m, dti);
+}
+
+// Temperature_Floor_Kernel, src/hydro/hydro_cuda.cu:1117-1153 (no DE)
+__global__ void TemperatureFloorKernel(double *U, int nx, int ny, int nz, int ng, double U_floor)
+{
+  const size_t nc = size_t(nx) * ny * nz;
+  size_t id       = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= nc) return;
+  int zid = int(id / (size_t(nx) * ny));
+  int yid = int((id - size_t(zid) * nx * ny) / nx);
+  int xid = int(id - size_t(zid) * nx * ny - size_t(yid) * nx);
+  if (xid > ng - 1 && xid < nx - ng && yid > ng - 1 && yid < ny - ng && zid > ng - 1 && zid < nz - ng) {
+    double d = U[id], dinv = 1.0 / d;
+    double vx = U[nc + id] * dinv, vy = U[2 * nc + id] * dinv, vz = U[3 * nc + id] * dinv;
+    double E    = U[4 * nc + id];
+    double Ekin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
+    double u    = (E - Ekin) / d;
+    if (u < U_floor) U[4 * nc + id] = Ekin + d * U_floor;
+  }
+}
+
+// Device-side time control for graph-captured stepping: dt = C_cfl / dti clamped
+// to the output time as in main.cpp:254-261, and t advanced by dt.
+__global__ void DtAdvanceKernel(const unsigned long long *dti_bits, double *dt_dev, double *t_dev, double tout,
+                                double cfl, int *n_step)
+{
+  double dti = __longlong_as_double((long long)*dti_bits);
+  double t   = *t_dev;
+  double dt  = cfl / dti;
+  if (t + dt > tout) dt = tout - t;
+  if (!(t < tout)) dt = 0.0;
+  *dt_dev = dt;
+  *t_dev  = t + dt;
+  if (dt > 0.0) *n_step += 1;
+}
+
+// ============================================================================ VL 1D
+// Stage 1 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:64-91): PCM states, HLLC on
+// faces id-1/2 and id+1/2 of every cell 1..n-2, the dt/2 update, then the half
+// state's primitives.
+__global__ void Vl1dStage1Kernel(const double *U, double *W, int n, double dx, const double *dt_dev, double dt_host,
+                                 double gamma, unsigned long long *dti_reset)
+{
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dti_reset && id == 0) *dti_reset = 0ull;  // stage 2 (next launch) reduces into it
+  if (id < 1 || id >= n - 1) return;
+  const double dt    = 0.5 * (dt_dev ? *dt_dev : dt_host);
+  const double dtodx = dt / dx;
+  auto side = [&](int c) {
+    return RStateFromCons<PolicyExact>(U[c], U[n + c], U[2 * n + c], U[3 * n + c], U[4 * n + c], gamma);
+  };
+  RState<double> a = side(id - 1), b = side(id), cc = side(id + 1);
+  Cons<double> fm = Hllc<PolicyExact>(a, b, gamma);
+  Cons<double> fp = Hllc<PolicyExact>(b, cc, gamma);
+  double uh[5];
+  uh[0]          = U[id] + dtodx * (fm.d - fp.d);
+  uh[1]          = U[n + id] + dtodx * (fm.mx - fp.mx);
+  uh[2]          = U[2 * n + id] + dtodx * (fm.my - fp.my);
+  uh[3]          = U[3 * n + id] + dtodx * (fm.mz - fp.mz);
+  uh[4]          = U[4 * n + id] + dtodx * (fm.E - fp.E);
+  Prim<double> w = LoadPrim(uh[0], uh[1], uh[2], uh[3], uh[4], gamma);
+  W[id]          = w.d;
+  W[n + id]      = w.vx;
+  W[2 * n + id]  = w.vy;
+  W[3 * n + id]  = w.vz;
+  W[4 * n + id]  = w.p;
+}
+
+template <int RK>
+__device__ __forceinline__ void Recon1D(const double *W, int n, int c, double gamma, Prim<double> &L, Prim<double> &R)
+{
+  constexpr int r  = ReconTraits<RK>::kRadius;
+  constexpr int NS = 2 * r + 1;
+  Prim<double> s[NS];
+#pragma unroll
+  for (int m = 0; m < NS; m++) {
+    int q = c - r + m;
+    s[m]  = {W[q], W[n + q], W[2 * n + q], W[3 * n + q], W[4 * n + q]};
+  }
+  if constexpr (RK == kPPMC) {
+    ReconPpmc<PolicyExact>(s, gamma, L, R);
+  } else {
+    ReconPlmc<PolicyExact>(s, gamma, L, R);
+  }
+}
+
+// Stage 2 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:95-137) for the real cells, with the
+// next inverse dt (Calc_dt_1D) reduced in the epilogue.
+template <int RK>
+__global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, double dx, const double *dt_dev,
+                                 double dt_host, double gamma, unsigned long long *dti)
+{
+  int id       = blockIdx.x * blockDim.x + threadIdx.x;
+  double m     = 0.0;
+  const double dt = dt_dev ? *dt_dev : dt_host;
+  if (id >= ng && id < n - ng) {
+    const double inv_gm1 = 1.0 / (gamma - 1.0);
+    Prim<double> Lm, Rm, L0, R0, Lp, Rp;
+    Recon1D<RK>(W, n, id - 1, gamma, Lm, Rm);
+    Recon1D<RK>(W, n, id, gamma, L0, R0);
+    Recon1D<RK>(W, n, id + 1, gamma, Lp, Rp);
+    Cons<double> fm = Hllc<PolicyExact>(RStateFromPrim<PolicyExact>(Lm, gamma, inv_gm1),
+                                        RStateFromPrim<PolicyExact>(R0, gamma, inv_gm1), gamma);
+    Cons<double> fp = Hllc<PolicyExact>(RStateFromPrim<PolicyExact>(L0, gamma, inv_gm1),
+                                        RStateFromPrim<PolicyExact>(Rp, gamma, inv_gm1), gamma);
+    const double dtodx = dt / dx;
+    double un[5];
+    un[0] = U[id] + dtodx * (fm.d - fp.d);
+    un[1] = U[n + id] + dtodx * (fm.mx - fp.mx);
+    un[2] = U[2 * n + id] + dtodx * (fm.my - fp.my);
+    un[3] = U[3 * n + id] + dtodx * (fm.mz - fp.mz);
+    un[4] = U[4 * n + id] + dtodx * (fm.E - fp.E);
+    for (int q = 0; q < 5; q++) U[q * n + id] = un[q];
+    double d = un[0], dinv = 1.0 / d;
+    double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
+    double P  = (un[4] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+    P         = Fmax(P, kTinyNumber);
+    double cs = __builtin_sqrt(dinv * gamma * P);
+    m         = (Fabs(vx) + cs) / dx;
+  }
+  if (dti) BlockMaxToGlobal(m, dti);
+}
+
+// ============================================================================ reference-shaped stage kernels
+__device__ __forceinline__ bool ThreadGuard(int order, int nx, int ny, int nz, int xid, int yid, int zid)
+{
+  bool out = xid < order - 1 || xid >= nx - order;
+  if (ny > 1) out = yid < order - 1 || yid >= ny - order || out;
+  if (nz > 1) {
+    out = zid < order - 1 || zid >= nz - order || out;
+  } else {
+    out = zid >= nz || out;
+  }
+  return out;
+}
+
+__device__ __forceinline__ void DirSlots(int dir, int &o1, int &o2, int &o3)
+{
+  o1 = dir == 0 ? 1 : (dir == 1 ? 2 : 3);
+  o2 = dir == 0 ? 2 : (dir == 1 ? 3 : 1);
+  o3 = dir == 0 ? 3 : (dir == 1 ? 1 : 2);
+}
+
+// PLMC_cuda / PPMC_VL over a whole grid with the reference's thread guard and
+// interface layout (plmc_cuda.cu:24-300, ppmc_cuda.cu:540-695): Q_L at cell i,
+// Q_R at cell i-1 along dir, conserved variables in global momentum slots.
+template <int RK>
+__global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
+{
+  constexpr int r  = ReconTraits<RK>::kRadius;
+  constexpr int NS = 2 * r + 1;
+  const size_t nc  = size_t(nx) * ny * nz;
+  long t           = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  int zid = int(t / (long(nx) * ny)), yid = int((t - long(zid) * nx * ny) / nx);
+  int xid = int(t - long(zid) * nx * ny - long(yid) * nx);
+  if (ThreadGuard(r + 1, nx, ny, nz, xid, yid, zid)) return;
+  int o1, o2, o3;
+  DirSlots(dir, o1, o2, o3);
+  int ix = dir == 0, iy = dir == 1, iz = dir == 2;
+  Prim<double> s[NS];
+#pragma unroll
+  for (int m = 0; m < NS; m++) {
+    size_t id = Lin(xid + (m - r) * ix, yid + (m - r) * iy, zid + (m - r) * iz, nx, ny);
+    s[m]      = LoadPrim(U[id], U[o1 * nc + id], U[o2 * nc + id], U[o3 * nc + id], U[4 * nc + id], gamma);
+  }
+  Prim<double> L, R;
+  if constexpr (RK == kPPMC) {
+    ReconPpmc<PolicyExact>(s, gamma, L, R);
+  } else {
+    ReconPlmc<PolicyExact>(s, gamma, L, R);
+  }
+  const double inv_gm1 = 1.0 / (gamma - 1.0);
+  auto write = [&](const Prim<double> &q, double *Q, size_t id) {  // Write_Data, reconstruction.h:918-958
+    Q[id]           = q.d;
+    Q[o1 * nc + id] = q.d * q.vx;
+    Q[o2 * nc + id] = q.d * q.vy;
+    Q[o3 * nc + id] = q.d * q.vz;
+    Q[4 * nc + id]  = EnergyFromPrim<PolicyExact>(q, gamma, inv_gm1);
+  };
+  write(L, QL, Lin(xid, yid, zid, nx, ny));
+  write(R, QR, Lin(xid - ix, yid - iy, zid - iz, nx, ny));
+}
+
+// Calculate_HLLC_Fluxes_CUDA over a whole grid (hllc_cuda.cu:19-307)
+__global__ void HllcGridKernel(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
+                               int dir)
+{
+  const size_t nc = size_t(nx) * ny * nz;
+  size_t t        = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nc) return;
+  int o1, o2, o3;
+  DirSlots(dir, o1, o2, o3);
+  RState<double> l = RStateFromCons<PolicyExact>(QL[t], QL[o1 * nc + t], QL[o2 * nc + t], QL[o3 * nc + t],
+                                                 QL[4 * nc + t], gamma);
+  RState<double> r = RStateFromCons<PolicyExact>(QR[t], QR[o1 * nc + t], QR[o2 * nc + t], QR[o3 * nc + t],
+                                                 QR[4 * nc + t], gamma);
+  Cons<double> f   = Hllc<PolicyExact>(l, r, gamma);
+  F[t]             = f.d;
+  F[o1 * nc + t]   = f.mx;
+  F[o2 * nc + t]   = f.my;
+  F[o3 * nc + t]   = f.mz;
+  F[4 * nc + t]    = f.E;
+}
+
+// ============================================================================ launchers
+static inline unsigned Blocks(long n, int tpb) { return unsigned((n + tpb - 1) / tpb); }
+
+hipError_t LaunchGhostFill(double *U, int nx, int ny, int nz, int n_fields, int ng, const int flags[6],
+                           hipStream_t s)
+{
+  GhostFlags fl;
+  for (int q = 0; q < 6; q++) fl.f[q] = flags[q];
+  if (nx <= 1) return hipSuccess;
+  const int jl = ny > 1 ? ng : 0, ju = ny > 1 ? ny - ng : ny;
+  const int kl = nz > 1 ? ng : 0, ku = nz > 1 ? nz - ng : nz;
+  long n_x = long(2 * ng) * (ju - jl) * (ku - kl);
+  long n_y = ny > 1 ? long(nx) * (2 * ng) * (ku - kl) : 0;
+  long n_z = nz > 1 ? long(nx) * ny * (2 * ng) : 0;
+  long tot = n_x + n_y + n_z;
+  if (tot == 0) return hipSuccess;
+  hipLaunchKernelGGL(GhostFillKernel, dim3(Blocks(tot, 256)), dim3(256), 0, s, U, nx, ny, nz, n_fields, ng, fl, n_x,
+                     n_y, n_z);
+  return hipGetLastError();
+}
+
+hipError_t LaunchGhostSlab(double *U, int nx, int ny, int nz, int n_fields, int ng, const int flags[6], int isize,
+                           int jsize, int ksize, int imin, int jmin, int kmin, int dir, hipStream_t s)
+{
+  GhostFlags fl;
+  for (int q = 0; q < 6; q++) fl.f[q] = flags[q];
+  long tot = long(isize) * jsize * ksize;
+  if (tot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(GhostSlabKernel, dim3(Blocks(tot, 256)), dim3(256), 0, s, U, nx, ny, nz, n_fields, ng, fl, isize,
+                     jsize, ksize, imin, jmin, kmin, dir);
+  return hipGetLastError();
+}
+
+hipError_t LaunchCalcDt(const double *U, int nx, int ny, int nz, int ng, double dx, double dy, double dz,
+                        double gamma, unsigned long long *dti, int num_cu, hipStream_t s)
+{
+  // Calc_dt_GPU's dimension test (hydro_cuda.cu:572-592); every other shape,
+  // including a single cell, takes the 3D kernel
+  if (nx > 1 && ny == 1 && nz == 1) {
+    hipLaunchKernelGGL(CalcDt1DKernel, dim3(1), dim3(256), 0, s, U, nx, ng, dx, gamma, dti);
+  } else {
+    long nc     = long(nx) * ny * nz;
+    unsigned nb = unsigned(std::min<long>(Blocks(nc, 256), long(num_cu) * 8));
+    hipLaunchKernelGGL(CalcDt3DKernel, dim3(nb), dim3(256), 0, s, U, nx, ny, nz, ng, dx, dy, dz, gamma, dti);
+  }
+  return hipGetLastError();
+}
+
+hipError_t LaunchTemperatureFloor(double *U, int nx, int ny, int nz, int ng, double U_floor, hipStream_t s)
+{
+  long nc = long(nx) * ny * nz;
+  hipLaunchKernelGGL(TemperatureFloorKernel, dim3(Blocks(nc, 256)), dim3(256), 0, s, U, nx, ny, nz, ng, U_floor);
+  return hipGetLastError();
+}
+
+hipError_t LaunchDtAdvance(const unsigned long long *dti, double *dt_dev, double *t_dev, double tout, double cfl,
+                           int *n_step, hipStream_t s)
+{
+  hipLaunchKernelGGL(DtAdvanceKernel, dim3(1), dim3(1), 0, s, dti, dt_dev, t_dev, tout, cfl, n_step);
+  return hipGetLastError();
+}
+
+hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, double dx, const double *dt_dev, double dt,
+                      double gamma, unsigned long long *dti, hipStream_t s)
+{
+  hipLaunchKernelGGL(Vl1dStage1Kernel, dim3(Blocks(n, 256)), dim3(256), 0, s, U, W, n, dx, dt_dev, dt, gamma, dti);
+  // one block when a dt reduction is fused so the block max is the grid max
+  unsigned nb = dti ? 1u : Blocks(n, 256);
+  int tpb     = dti ? 1024 : 256;
+  if (dti && n > 1024) return hipErrorInvalidValue;
+  if (recon == kPPMC) {
+    hipLaunchKernelGGL(Vl1dStage2Kernel<kPPMC>, dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma, dti);
+  } else if (recon == kPLMC) {
+    hipLaunchKernelGGL(Vl1dStage2Kernel<kPLMC>, dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma, dti);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t LaunchReconGrid(int recon, const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma,
+                           int dir, hipStream_t s)
+{
+  long nc = long(nx) * ny * nz;
+  if (recon == kPPMC) {
+    hipLaunchKernelGGL(ReconGridKernel<kPPMC>, dim3(Blocks(nc, 64)), dim3(64), 0, s, U, QL, QR, nx, ny, nz, gamma,
+                       dir);
+  } else if (recon == kPLMC) {
+    hipLaunchKernelGGL(ReconGridKernel<kPLMC>, dim3(Blocks(nc, 64)), dim3(64), 0, s, U, QL, QR, nx, ny, nz, gamma,
+                       dir);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t LaunchHllcGrid(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
+                          int dir, hipStream_t s)
+{
+  long nc = long(nx) * ny * nz;
+  hipLaunchKernelGGL(HllcGridKernel, dim3(Blocks(nc, 64)), dim3(64), 0, s, QL, QR, F, nx, ny, nz, gamma, dir);
+  return hipGetLastError();
+}
+
+}  // namespace chl

@@ -1,0 +1,28 @@
+// support_kernels.hpp -- launchers for the non-fused parts of the hot path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace chl
+{
+
+struct GhostFlags {
+  int f[6];
+};
+
+hipError_t LaunchGhostFill(double *U, int nx, int ny, int nz, int n_fields, int ng, const int flags[6], hipStream_t s);
+hipError_t LaunchGhostSlab(double *U, int nx, int ny, int nz, int n_fields, int ng, const int flags[6], int isize,
+                           int jsize, int ksize, int imin, int jmin, int kmin, int dir, hipStream_t s);
+hipError_t LaunchCalcDt(const double *U, int nx, int ny, int nz, int ng, double dx, double dy, double dz,
+                        double gamma, unsigned long long *dti, int num_cu, hipStream_t s);
+hipError_t LaunchTemperatureFloor(double *U, int nx, int ny, int nz, int ng, double U_floor, hipStream_t s);
+hipError_t LaunchDtAdvance(const unsigned long long *dti, double *dt_dev, double *t_dev, double tout, double cfl,
+                           int *n_step, hipStream_t s);
+hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, double dx, const double *dt_dev, double dt,
+                      double gamma, unsigned long long *dti, hipStream_t s);
+hipError_t LaunchReconGrid(int recon, const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma,
+                           int dir, hipStream_t s);
+hipError_t LaunchHllcGrid(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
+                          int dir, hipStream_t s);
+
+}  // namespace chl

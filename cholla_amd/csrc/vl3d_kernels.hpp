@@ -1,0 +1,552 @@
+// vl3d_kernels.hpp -- fused kernels of one van Leer (VL) step on a 3D box, for gfx950.
+//
+// The reference runs one VL step as ~20 unfused launches that stream ~225
+// field-arrays per cell through HBM (src/integrators/VL_3D_cuda.cu:38-360).  Here
+// a step is two kernels, both "2.5D marches": a workgroup owns a TX x TY column
+// of cells and walks up the z axis; the z stencil lives in each lane's registers,
+// the x/y stencils of the current z-plane are staged in LDS, and interface states
+// and face fluxes are exchanged through LDS so each face is solved exactly once.
+//
+//   Stage1Kernel  PCM states + HLLC on every face of the half-step region, the
+//                 dt/2 conserved update (Update_Conserved_Variables_3D_half,
+//                 VL_3D_cuda.cu:379-495), then the primitive variables of the
+//                 half state (Load_Data, reconstruction.h:139-212) written once
+//                 per cell as W (FP64, or FP32 in the mixed form).
+//   Stage2Kernel  PLMC/PPMC on W + HLLC per face + the full dt update of the real
+//                 cells in place (hydro_cuda.cu:172-402), with the density floor,
+//                 temperature floor (hydro_cuda.cu:1117-1153), the next-step
+//                 inverse-dt reduction (Calc_dt_3D, hydro_cuda.cu:514-560) and,
+//                 for all-periodic boxes, the ghost-cell images
+//                 (cuda_boundaries.cu:71-159 with flag 1) fused into the epilogue.
+//
+// Index convention: U[f*n_cells + i + j*nx + k*nx*ny], 64-bit offsets.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hydro_physics.hpp"
+
+namespace chl
+{
+
+enum ReconKind : int { kPCM = 0, kPLMC = 1, kPPMC = 2 };
+
+template <int R>
+struct ReconTraits;
+template <>
+struct ReconTraits<kPLMC> {
+  static constexpr int kRadius = 1;
+};
+template <>
+struct ReconTraits<kPPMC> {
+  static constexpr int kRadius = 2;
+};
+
+// Parameters shared by both stages.
+struct StepArgs {
+  double *U;         // conserved state, n_fields x n_cells (in/out)
+  void *W;           // half-step primitives, 5 x n_cells of P::T
+  int nx, ny, nz;    // total cells per axis, ghosts included
+  int ng;            // ghost depth
+  double dx, dy, dz;
+  double gamma;
+  double dt;               // full step (used when dt_dev is null)
+  const double *dt_dev;    // device-resident dt (device time-stepping), or null
+  unsigned long long *dti; // inverse-dt max accumulator (bits of a non-negative double)
+  double density_floor;
+  double U_floor;
+  int flags;               // kFlag* bits
+  int kchunk;              // z cells per workgroup
+};
+
+enum : int {
+  kFlagDensityFloor     = 1,
+  kFlagTemperatureFloor = 2,
+  kFlagPeriodicGhosts   = 4,  // write periodic ghost images from the stage-2 epilogue
+  kFlagReduceDt         = 8,
+};
+
+__device__ __forceinline__ size_t Lin(int i, int j, int k, int nx, int ny)
+{
+  return size_t(i) + size_t(j) * size_t(nx) + size_t(k) * size_t(nx) * size_t(ny);
+}
+
+__device__ __forceinline__ double StepDt(const StepArgs &a) { return a.dt_dev ? *a.dt_dev : a.dt; }
+
+// Rotate a global-order (x,y,z) triple into the sweep frame of dir.
+template <int DIR, typename T>
+__device__ __forceinline__ void Rot(T gx, T gy, T gz, T &n, T &t1, T &t2)
+{
+  if constexpr (DIR == 0) {
+    n = gx, t1 = gy, t2 = gz;
+  } else if constexpr (DIR == 1) {
+    n = gy, t1 = gz, t2 = gx;
+  } else {
+    n = gz, t1 = gx, t2 = gy;
+  }
+}
+// Back to global order: rotated (n, t1, t2) -> (x, y, z)
+template <int DIR, typename T>
+__device__ __forceinline__ void Unrot(T n, T t1, T t2, T &gx, T &gy, T &gz)
+{
+  if constexpr (DIR == 0) {
+    gx = n, gy = t1, gz = t2;
+  } else if constexpr (DIR == 1) {
+    gy = n, gz = t1, gx = t2;
+  } else {
+    gz = n, gx = t1, gy = t2;
+  }
+}
+
+// ============================================================================ stage 1
+// Per-cell values staged for the PCM Riemann problems: the conserved state
+// plus the velocities (divisions done once per cell) and the direction-free
+// pressure used by the non-exact policies.
+template <typename T>
+struct PcmCell {
+  T d, mx, my, mz, E, vx, vy, vz, p;
+};
+
+template <class P, int DIR>
+__device__ __forceinline__ RState<typename P::T> PcmSide(const PcmCell<typename P::T> &c, typename P::T gamma)
+{
+  using T = typename P::T;
+  RState<T> s;
+  s.d = c.d, s.E = c.E;
+  Rot<DIR>(c.mx, c.my, c.mz, s.mx, s.my, s.mz);
+  Rot<DIR>(c.vx, c.vy, c.vz, s.vx, s.vy, s.vz);
+  if constexpr (P::kExact) {
+    // hllc_cuda.cu:111-113: the pressure sum follows the rotated component order
+    s.p = (s.E - T(0.5) * s.d * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz)) * (gamma - T(1.0));
+    s.p = Fmax(s.p, T(kTinyNumber));
+  } else {
+    s.p = c.p;
+  }
+  s.inv_d = Rcp<P>(s.d);
+  return s;
+}
+
+template <class P>
+__device__ __forceinline__ PcmCell<typename P::T> MakePcmCell(double d, double mx, double my, double mz, double E,
+                                                              double gamma)
+{
+  using T = typename P::T;
+  PcmCell<T> c;
+  if constexpr (P::kExact) {
+    c.d = d, c.mx = mx, c.my = my, c.mz = mz, c.E = E;
+    c.vx = mx / d, c.vy = my / d, c.vz = mz / d;
+    c.p = 0;
+  } else {
+    // FP64 primitive recovery, then narrowing (a no-op for PolicyFast)
+    double inv = 1.0 / d;
+    double vx = mx * inv, vy = my * inv, vz = mz * inv;
+    double p  = PressureFromCons(E, d, vx, vy, vz, gamma);
+    c.d = T(d), c.mx = T(mx), c.my = T(my), c.mz = T(mz), c.E = T(E);
+    c.vx = T(vx), c.vy = T(vy), c.vz = T(vz), c.p = T(p);
+  }
+  return c;
+}
+
+template <class P, int DIR>
+__device__ __forceinline__ void FluxGlobal(const Cons<typename P::T> &f, typename P::T out[5])
+{
+  out[0] = f.d;
+  Unrot<DIR>(f.mx, f.my, f.mz, out[1], out[2], out[3]);
+  out[4] = f.E;
+}
+
+// Stage 1: region of half-step cells [lo, hi) per axis with lo = ng - R1, hi = n - ng + R1.
+template <class P, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
+{
+  using T            = typename P::T;
+  constexpr int NT   = TX * TY;
+  constexpr int PX   = TX + 2, PY = TY + 2;  // staged plane with a 1-cell halo
+  constexpr int NFX  = (TX + 1) * TY;
+  constexpr int NFY  = TX * (TY + 1);
+  __shared__ PcmCell<T> plane[PY][PX];
+  __shared__ T fx[5][TY][TX + 1];
+  __shared__ T fy[5][TY + 1][TX];
+
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const size_t nc = size_t(nx) * ny * nz;
+  const int lo_x = a.ng - R1, hi_x = nx - a.ng + R1;
+  const int lo_y = a.ng - R1, hi_y = ny - a.ng + R1;
+  const int lo_z = a.ng - R1, hi_z = nz - a.ng + R1;
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int i0 = lo_x + blockIdx.x * TX, j0 = lo_y + blockIdx.y * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int k0 = lo_z + blockIdx.z * a.kchunk;
+  const int k1 = min(k0 + a.kchunk, hi_z);
+  const bool own = (i < hi_x) && (j < hi_y);
+  const T gamma  = T(a.gamma);
+  const double dt    = 0.5 * StepDt(a);
+  const double dtodx = dt / a.dx, dtody = dt / a.dy, dtodz = dt / a.dz;
+  const double *U    = a.U;
+  T *W               = reinterpret_cast<T *>(a.W);
+
+  // the inverse-dt accumulator is reset here; stage 2 (next launch) fills it
+  if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid == 0) *a.dti = 0ull;
+  if (dt == 0.0) return;  // a device-timed step past the output time (uniform per launch)
+
+  // Lanes past the region's end still stage their own cell: it is the right
+  // neighbour of the last owned cell.  Only the index is clamped, to stay in memory.
+  const int ic = min(i, nx - 1), jc = min(j, ny - 1);
+  auto load_cell = [&](int ii, int jj, int kk) {
+    size_t id = Lin(ii, jj, kk, nx, ny);
+    return MakePcmCell<P>(U[id], U[nc + id], U[2 * nc + id], U[3 * nc + id], U[4 * nc + id], a.gamma);
+  };
+
+  PcmCell<T> c_lo = load_cell(ic, jc, k0 - 1);  // cell k-1
+  PcmCell<T> c_mid = load_cell(ic, jc, k0);     // cell k
+  double u_mid[5];
+  T fz_lo[5];
+  {
+    Cons<T> f = Hllc<P>(PcmSide<P, 2>(c_lo, gamma), PcmSide<P, 2>(c_mid, gamma), gamma);
+    FluxGlobal<P, 2>(f, fz_lo);
+  }
+
+  for (int k = k0; k < k1; k++) {
+    {
+      size_t id = Lin(ic, jc, k, nx, ny);
+#pragma unroll
+      for (int f = 0; f < 5; f++) u_mid[f] = U[f * nc + id];
+    }
+    PcmCell<T> c_hi = load_cell(ic, jc, k + 1);
+    __syncthreads();  // previous plane fully consumed
+    plane[ty + 1][tx + 1] = c_mid;
+    // halo: x sides (PY-2 rows x 2) and y sides (PX-2 cols x 2), no corners
+    for (int h = tid; h < 2 * TY + 2 * TX; h += NT) {
+      int pi, pj;
+      if (h < 2 * TY) {
+        pj = 1 + (h >> 1);
+        pi = (h & 1) ? TX + 1 : 0;
+      } else {
+        int g = h - 2 * TY;
+        pi    = 1 + (g >> 1);
+        pj    = (g & 1) ? TY + 1 : 0;
+      }
+      int gi = min(i0 - 1 + pi, nx - 1), gj = min(j0 - 1 + pj, ny - 1);
+      plane[pj][pi] = load_cell(gi, gj, k);
+    }
+    __syncthreads();
+    // x faces: face fi sits between plane cells (fi, fj+1) and (fi+1, fj+1)
+    for (int it = tid; it < NFX; it += NT) {
+      int fj = it / (TX + 1), fi = it - fj * (TX + 1);
+      Cons<T> f = Hllc<P>(PcmSide<P, 0>(plane[fj + 1][fi], gamma), PcmSide<P, 0>(plane[fj + 1][fi + 1], gamma), gamma);
+      T g[5];
+      FluxGlobal<P, 0>(f, g);
+#pragma unroll
+      for (int q = 0; q < 5; q++) fx[q][fj][fi] = g[q];
+    }
+    for (int it = tid; it < NFY; it += NT) {
+      int fj = it / TX, fi = it - fj * TX;
+      Cons<T> f = Hllc<P>(PcmSide<P, 1>(plane[fj][fi + 1], gamma), PcmSide<P, 1>(plane[fj + 1][fi + 1], gamma), gamma);
+      T g[5];
+      FluxGlobal<P, 1>(f, g);
+#pragma unroll
+      for (int q = 0; q < 5; q++) fy[q][fj][fi] = g[q];
+    }
+    T fz_hi[5];
+    {
+      Cons<T> f = Hllc<P>(PcmSide<P, 2>(c_mid, gamma), PcmSide<P, 2>(c_hi, gamma), gamma);
+      FluxGlobal<P, 2>(f, fz_hi);
+    }
+    __syncthreads();
+    if (own) {
+      double uh[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        // (((U + dtodx dFx) + dtody dFy) + dtodz dFz), VL_3D_cuda.cu:444-461
+        uh[q] = u_mid[q] + dtodx * (double(fx[q][ty][tx]) - double(fx[q][ty][tx + 1])) +
+                dtody * (double(fy[q][ty][tx]) - double(fy[q][ty + 1][tx])) +
+                dtodz * (double(fz_lo[q]) - double(fz_hi[q]));
+      }
+      if ((a.flags & kFlagDensityFloor) && uh[0] < a.density_floor) {  // VL_3D_cuda.cu:479-493
+        double d0 = uh[0];
+        uh[0]     = a.density_floor;
+#pragma unroll
+        for (int q = 1; q < 5; q++) uh[q] *= (a.density_floor / d0);
+      }
+      Prim<double> w = LoadPrim(uh[0], uh[1], uh[2], uh[3], uh[4], a.gamma);
+      size_t id      = Lin(i, j, k, nx, ny);
+      W[id]          = T(w.d);
+      W[nc + id]     = T(w.vx);
+      W[2 * nc + id] = T(w.vy);
+      W[3 * nc + id] = T(w.vz);
+      W[4 * nc + id] = P::kExact ? T(uh[4]) : T(w.p);  // see WCell
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) fz_lo[q] = fz_hi[q];
+    c_mid = c_hi;
+  }
+}
+
+// ============================================================================ stage 2
+// Half-step cell as stored in W.  The fifth slot holds the pressure, except under
+// PolicyExact where it holds the energy: Load_Data recovers pressure from the
+// velocities in the sweep's rotated order (reconstruction.h:194-201 with
+// SquareMagnitude, math_utilities.h:78-97), so its last bit depends on the sweep
+// direction and is recomputed per sweep.
+template <typename T>
+struct WCell {
+  T d, vx, vy, vz, p_or_E;
+};
+
+template <class P, int RK, int DIR>
+__device__ __forceinline__ void ReconDir(const WCell<typename P::T> *s, typename P::T gamma,
+                                         Prim<typename P::T> &L, Prim<typename P::T> &R)
+{
+  using T          = typename P::T;
+  constexpr int NS = 2 * ReconTraits<RK>::kRadius + 1;
+  Prim<T> st[NS];
+#pragma unroll
+  for (int m = 0; m < NS; m++) {
+    st[m].d = s[m].d;
+    Rot<DIR>(s[m].vx, s[m].vy, s[m].vz, st[m].vx, st[m].vy, st[m].vz);
+    if constexpr (P::kExact) {
+      st[m].p = PressureFromCons(s[m].p_or_E, st[m].d, st[m].vx, st[m].vy, st[m].vz, gamma);
+    } else {
+      st[m].p = s[m].p_or_E;
+    }
+  }
+  if constexpr (RK == kPPMC) {
+    ReconPpmc<P>(st, gamma, L, R);
+  } else {
+    ReconPlmc<P>(st, gamma, L, R);
+  }
+}
+
+// LDS footprint of stage 2 (bytes) for a given tile; used by the host to pick tiles.
+template <class P, int RK, int TX, int TY>
+struct Stage2Shape {
+  using T                 = typename P::T;
+  static constexpr int H  = ReconTraits<RK>::kRadius + 1;  // plane halo
+  static constexpr int PX = TX + 2 * H, PY = TY + 2 * H;
+  static constexpr int NQX = (TX + 2) * TY, NQY = TX * (TY + 2);
+  static constexpr int NQ  = NQX > NQY ? NQX : NQY;
+  static constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1);
+  static constexpr int NF  = NFX > NFY ? NFX : NFY;
+};
+
+template <class P, int RK, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) Stage2Kernel(StepArgs a)
+{
+  using T          = typename P::T;
+  using S          = Stage2Shape<P, RK, TX, TY>;
+  constexpr int NT = TX * TY;
+  constexpr int r  = ReconTraits<RK>::kRadius;
+  constexpr int NS = 2 * r + 1;
+  constexpr int H  = S::H;
+  __shared__ WCell<T> plane[S::PY][S::PX];
+  __shared__ Prim<T> qL[S::NQ];
+  __shared__ Prim<T> qR[S::NQ];
+  __shared__ T fl[5][S::NF];
+  __shared__ double red[NT / 64];
+
+  const int nx = a.nx, ny = a.ny, nz = a.nz, ng = a.ng;
+  const size_t nc = size_t(nx) * ny * nz;
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int i0 = ng + blockIdx.x * TX, j0 = ng + blockIdx.y * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const int k0 = ng + blockIdx.z * a.kchunk;
+  const int k1 = min(k0 + a.kchunk, nz - ng);
+  const bool own = (i < nx - ng) && (j < ny - ng);
+  // Lanes past the last real cell still stage their own column of W: those
+  // cells are the stencil of the last owned cells.  Only the index is clamped.
+  const int ic = min(i, nx - 1), jc = min(j, ny - 1);
+  const T gamma      = T(a.gamma);
+  const T inv_gm1    = Rcp<P>(gamma - T(1.));
+  const double dt    = StepDt(a);
+  const double dtodx = dt / a.dx, dtody = dt / a.dy, dtodz = dt / a.dz;
+  const T *W         = reinterpret_cast<const T *>(a.W);
+  double *U          = a.U;
+  if (dt == 0.0) return;  // a device-timed step past the output time (uniform per launch)
+
+  auto loadW = [&](int ii, int jj, int kk) {
+    size_t id = Lin(ii, jj, kk, nx, ny);
+    return WCell<T>{W[id], W[nc + id], W[2 * nc + id], W[3 * nc + id], W[4 * nc + id]};
+  };
+
+  // z window: win[m] = W(k - r + m)
+  WCell<T> win[NS];
+#pragma unroll
+  for (int m = 0; m < NS - 1; m++) win[m + 1] = loadW(ic, jc, k0 - 1 - r + m);
+
+  Prim<T> qLz_prev;
+  T fz_prev[5];
+  double acc_prev[5], u_prev[5];
+  double dti_max = 0.0;
+
+  for (int k = k0 - 1; k <= k1; k++) {
+#pragma unroll
+    for (int m = 0; m < NS - 1; m++) win[m] = win[m + 1];
+    win[NS - 1] = loadW(ic, jc, k + r);
+
+    // z reconstruction of cell k, face k-1/2, and the epilogue of cell k-1
+    Prim<T> qLz, qRz;
+    ReconDir<P, RK, 2>(win, gamma, qLz, qRz);
+    T fz[5];
+    if (k >= k0) {
+      Cons<T> f = Hllc<P>(RStateFromPrim<P>(qLz_prev, gamma, inv_gm1), RStateFromPrim<P>(qRz, gamma, inv_gm1), gamma);
+      FluxGlobal<P, 2>(f, fz);
+      if (k - 1 >= k0 && own) {
+        // U += ((dtodx dFx + dtody dFy) + dtodz dFz), hydro_cuda.cu:241-257
+        double un[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) un[q] = u_prev[q] + (acc_prev[q] + dtodz * (double(fz_prev[q]) - double(fz[q])));
+        const int kk  = k - 1;
+        const size_t id = Lin(i, j, kk, nx, ny);
+        if ((a.flags & kFlagDensityFloor) && un[0] < a.density_floor) {  // hydro_cuda.cu:279-299
+          if (un[0] > 0) {
+            double d0 = un[0];
+            un[0]     = a.density_floor;
+#pragma unroll
+            for (int q = 1; q < 5; q++) un[q] *= (a.density_floor / d0);
+          } else {
+            // Average_Cell_Single_Field (hydro_cuda.cu:1155-1177) over the six
+            // neighbours as they are in memory now; the reference reads them
+            // while other threads update them as well.
+            un[0] = (U[id - 1] + U[id + 1] + U[id - nx] + U[id + nx] + U[id - size_t(nx) * ny] +
+                     U[id + size_t(nx) * ny]) /
+                    6;
+          }
+        }
+        double dinv = 1.0 / un[0];
+        double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
+        if (a.flags & kFlagTemperatureFloor) {  // Temperature_Floor_Kernel, hydro_cuda.cu:1132-1146
+          double Ekin = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz);
+          double uint = (un[4] - Ekin) / un[0];
+          if (uint < a.U_floor) un[4] = Ekin + un[0] * a.U_floor;
+        }
+#pragma unroll
+        for (int q = 0; q < 5; q++) U[q * nc + id] = un[q];
+        if (a.flags & kFlagReduceDt) {
+          dti_max = Fmax(dti_max, InverseCrossingTime(un[4], un[0], dinv, vx, vy, vz, a.dx, a.dy, a.dz, a.gamma));
+        }
+        if (a.flags & kFlagPeriodicGhosts) {
+          // every periodic image of this real cell inside the ghost layers
+          const int Lx = nx - 2 * ng, Ly = ny - 2 * ng, Lz = nz - 2 * ng;
+          // (the host enables this only when every axis holds >= ng real cells,
+          // so each image lands in a ghost layer and maps back to this cell)
+          int xs[3] = {i, i, i}, ys[3] = {j, j, j}, zs[3] = {kk, kk, kk};
+          int nxi = 1, nyi = 1, nzi = 1;
+          if (i < 2 * ng) xs[nxi++] = i + Lx;
+          if (i >= nx - 2 * ng) xs[nxi++] = i - Lx;
+          if (j < 2 * ng) ys[nyi++] = j + Ly;
+          if (j >= ny - 2 * ng) ys[nyi++] = j - Ly;
+          if (kk < 2 * ng) zs[nzi++] = kk + Lz;
+          if (kk >= nz - 2 * ng) zs[nzi++] = kk - Lz;
+          for (int c = 1; c < nxi * nyi * nzi; c++) {
+            int ax = c % nxi, ay = (c / nxi) % nyi, az = c / (nxi * nyi);
+            size_t g = Lin(xs[ax], ys[ay], zs[az], nx, ny);
+#pragma unroll
+            for (int q = 0; q < 5; q++) U[q * nc + g] = un[q];
+          }
+        }
+      }
+    }
+    if (k >= k0) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) fz_prev[q] = fz[q];
+    }
+    qLz_prev = qLz;
+
+    if (k >= k0 && k < k1) {
+      // ---- x/y sweeps of plane k through LDS
+      {
+        size_t id = Lin(ic, jc, k, nx, ny);
+#pragma unroll
+        for (int q = 0; q < 5; q++) u_prev[q] = U[q * nc + id];
+      }
+      __syncthreads();  // previous plane / buffers consumed
+      plane[ty + H][tx + H] = win[r];
+      for (int h = tid; h < 2 * H * TY + 2 * H * TX; h += NT) {
+        int pi, pj;
+        if (h < 2 * H * TY) {
+          int row = h / (2 * H), c = h - row * 2 * H;
+          pj = H + row;
+          pi = c < H ? c : TX + c;
+        } else {
+          int g = h - 2 * H * TY;
+          int col = g / (2 * H), c = g - col * 2 * H;
+          pi = H + col;
+          pj = c < H ? c : TY + c;
+        }
+        int gi = min(i0 - H + pi, nx - 1), gj = min(j0 - H + pj, ny - 1);
+        plane[pj][pi] = loadW(gi, gj, k);
+      }
+      __syncthreads();
+      // x reconstruction of cells -1..TX of every row
+      for (int it = tid; it < S::NQX; it += NT) {
+        int rj = it / (TX + 2), ri = it - rj * (TX + 2);  // plane column of cell = ri - 1 + H
+        WCell<T> s[NS];
+#pragma unroll
+        for (int m = 0; m < NS; m++) s[m] = plane[rj + H][ri - 1 + H - r + m];
+        ReconDir<P, RK, 0>(s, gamma, qL[it], qR[it]);
+      }
+      __syncthreads();
+      for (int it = tid; it < S::NFX; it += NT) {
+        int fj = it / (TX + 1), fi = it - fj * (TX + 1);  // between recon cells fi and fi+1 of row fj
+        int ql = fj * (TX + 2) + fi;
+        Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL[ql], gamma, inv_gm1), RStateFromPrim<P>(qR[ql + 1], gamma, inv_gm1),
+                            gamma);
+        T g[5];
+        FluxGlobal<P, 0>(f, g);
+#pragma unroll
+        for (int q = 0; q < 5; q++) fl[q][it] = g[q];
+      }
+      __syncthreads();
+      double accx[5];
+      {
+        int fa = ty * (TX + 1) + tx;
+#pragma unroll
+        for (int q = 0; q < 5; q++) accx[q] = dtodx * (double(fl[q][fa]) - double(fl[q][fa + 1]));
+      }
+      // y reconstruction of cells -1..TY of every column
+      for (int it = tid; it < S::NQY; it += NT) {
+        int rj = it / TX, ri = it - rj * TX;
+        WCell<T> s[NS];
+#pragma unroll
+        for (int m = 0; m < NS; m++) s[m] = plane[rj - 1 + H - r + m][ri + H];
+        ReconDir<P, RK, 1>(s, gamma, qL[it], qR[it]);
+      }
+      __syncthreads();
+      for (int it = tid; it < S::NFY; it += NT) {
+        int fj = it / TX, fi = it - fj * TX;  // between recon rows fj and fj+1 of column fi
+        Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL[fj * TX + fi], gamma, inv_gm1),
+                            RStateFromPrim<P>(qR[(fj + 1) * TX + fi], gamma, inv_gm1), gamma);
+        T g[5];
+        FluxGlobal<P, 1>(f, g);
+#pragma unroll
+        for (int q = 0; q < 5; q++) fl[q][it] = g[q];
+      }
+      __syncthreads();
+      {
+        int fa = ty * TX + tx;
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+          acc_prev[q] = accx[q] + dtody * (double(fl[q][fa]) - double(fl[q][fa + TX]));
+      }
+    }
+  }
+
+  if (a.flags & kFlagReduceDt) {
+    // wave max (wave64 cross-lane), then one LDS pass and one atomic per workgroup
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) dti_max = Fmax(dti_max, __shfl_xor(dti_max, off, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = dti_max;
+    __syncthreads();
+    if (tid == 0) {
+      double m = red[0];
+      for (int w = 1; w < NT / 64; w++) m = Fmax(m, red[w]);
+      atomicMax(a.dti, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
+}  // namespace chl

@@ -1,0 +1,87 @@
+// vl3d_launch.hpp -- host-side dispatch of the fused VL kernels over tile shapes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "vl3d_kernels.hpp"
+
+namespace chl
+{
+
+// Tile shapes (TX x TY lanes) compiled for both stages.  The autotuner picks a
+// pair (stage-1 tile, stage-2 tile) and the z-chunk lengths per box size.
+struct TileShape {
+  int tx, ty;
+};
+constexpr int kNumTiles                = 4;
+constexpr TileShape kTiles[kNumTiles] = {{32, 8}, {16, 16}, {64, 4}, {16, 8}};
+
+inline int CeilDiv(int a, int b) { return (a + b - 1) / b; }
+
+template <class P, int TX, int TY>
+hipError_t LaunchStage1T(const StepArgs &a, int R1, hipStream_t s)
+{
+  int ox = a.nx - 2 * a.ng + 2 * R1, oy = a.ny - 2 * a.ng + 2 * R1, oz = a.nz - 2 * a.ng + 2 * R1;
+  dim3 grid(CeilDiv(ox, TX), CeilDiv(oy, TY), CeilDiv(oz, a.kchunk));
+  hipLaunchKernelGGL((Stage1Kernel<P, TX, TY>), grid, dim3(TX * TY), 0, s, a, R1);
+  return hipGetLastError();
+}
+
+template <class P, int RK, int TX, int TY>
+hipError_t LaunchStage2T(const StepArgs &a, hipStream_t s)
+{
+  int ox = a.nx - 2 * a.ng, oy = a.ny - 2 * a.ng, oz = a.nz - 2 * a.ng;
+  dim3 grid(CeilDiv(ox, TX), CeilDiv(oy, TY), CeilDiv(oz, a.kchunk));
+  hipLaunchKernelGGL((Stage2Kernel<P, RK, TX, TY>), grid, dim3(TX * TY), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class P>
+hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
+{
+  switch (tile) {
+    case 0:
+      return LaunchStage1T<P, 32, 8>(a, R1, s);
+    case 1:
+      return LaunchStage1T<P, 16, 16>(a, R1, s);
+    case 2:
+      return LaunchStage1T<P, 64, 4>(a, R1, s);
+    case 3:
+      return LaunchStage1T<P, 16, 8>(a, R1, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class P, int RK>
+hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
+{
+  switch (tile) {
+    case 0:
+      return LaunchStage2T<P, RK, 32, 8>(a, s);
+    case 1:
+      return LaunchStage2T<P, RK, 16, 16>(a, s);
+    case 2:
+      return LaunchStage2T<P, RK, 64, 4>(a, s);
+    case 3:
+      return LaunchStage2T<P, RK, 16, 8>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class P>
+hipError_t LaunchStage2(const StepArgs &a, int recon, int tile, hipStream_t s)
+{
+  if (recon == kPPMC) return LaunchStage2R<P, kPPMC>(a, tile, s);
+  if (recon == kPLMC) return LaunchStage2R<P, kPLMC>(a, tile, s);
+  return hipErrorInvalidValue;
+}
+
+// Explicitly instantiated per policy in hydro_exact.hip / hydro_fast.hip.
+hipError_t LaunchStage1Policy(int policy, const StepArgs &a, int R1, int tile, hipStream_t s);
+hipError_t LaunchStage2Policy(int policy, const StepArgs &a, int recon, int tile, hipStream_t s);
+hipError_t LaunchStage1Exact(const StepArgs &a, int R1, int tile, hipStream_t s);
+hipError_t LaunchStage2Exact(const StepArgs &a, int recon, int tile, hipStream_t s);
+hipError_t LaunchStage1Fast(int policy, const StepArgs &a, int R1, int tile, hipStream_t s);
+hipError_t LaunchStage2Fast(int policy, const StepArgs &a, int recon, int tile, hipStream_t s);
+
+}  // namespace chl

@@ -1,0 +1,158 @@
+/* cholla_amd.h -- C ABI of the MI355X (gfx950) hydro hot path.
+ *
+ * The library replaces the device side of Cholla's van Leer hydro step: the VL
+ * integrator with PLMC/PPMC reconstruction and HLLC fluxes, the conserved update,
+ * the temperature floor, the CFL inverse-timestep reduction and the ghost-cell fill.
+ * Every entry point takes plain pointers and sizes; device pointers hold the
+ * conserved state in Cholla's own layout: field-major doubles,
+ *   U[f*n_cells + i + j*nx + k*nx*ny],  f = density, momentum_x, momentum_y,
+ *   momentum_z, Energy  (src/grid/grid_enum.h:20-30, src/grid/grid3D.cpp:321-342),
+ * with nx, ny, nz the total cell counts including n_ghost ghost cells per side.
+ *
+ * All calls are synchronous at return (the reference synchronises after every
+ * stage, src/utils/gpu.hpp:139-155).  They return CHL_OK or an error code;
+ * chl_last_error() gives the message.  The C++ adapter (cholla_dropin.hpp) turns
+ * a non-zero code into the reference's abort.
+ */
+#ifndef CHOLLA_AMD_H
+#define CHOLLA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes */
+#define CHL_OK 0
+#define CHL_ERR_INVALID 1     /* bad argument or unsupported configuration */
+#define CHL_ERR_HIP 2         /* HIP runtime error (message in chl_last_error) */
+#define CHL_ERR_NO_DEVICE 3   /* no HIP device visible */
+#define CHL_ERR_UNSUPPORTED 4 /* option compiled out / not implemented */
+
+/* ---- build options of the reference's make.type files, as run-time values */
+#define CHL_RECON_PCM 0  /* PCM  */
+#define CHL_RECON_PLMC 1 /* PLMC (builds/make.type.hydro:5) */
+#define CHL_RECON_PPMC 2 /* PPMC */
+#define CHL_RIEMANN_HLLC 0
+/* arithmetic forms of the same step */
+#define CHL_PREC_FP64_EXACT 0 /* FP64, reference operation order, no FMA contraction */
+#define CHL_PREC_FP64 1       /* FP64, shared reciprocals + FMA */
+#define CHL_PREC_MIXED 2      /* FP32 reconstruction/Riemann, FP64 state, accumulation, pressure, dt */
+
+/* boundary flags, Parameters::xl_bcnd..zu_bcnd (src/global/global.h:223-228) */
+#define CHL_BC_PERIODIC 1
+#define CHL_BC_REFLECTIVE 2
+#define CHL_BC_TRANSMISSIVE 3
+
+typedef struct chl_hydro_config {
+  int nx, ny, nz;        /* total cells per axis incl. ghosts (Header::nx, ny, nz) */
+  int n_ghost;           /* Header::n_ghost: PLMC 3, PPMC 4 (src/grid/grid3D.cpp:53-76) */
+  int n_fields;          /* 5 (hydro, no scalars / dual energy) */
+  int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL */
+  int riemann;           /* CHL_RIEMANN_HLLC */
+  int precision;         /* CHL_PREC_* */
+  int density_floor;     /* DENSITY_FLOOR build option, 0/1 */
+  int temperature_floor; /* TEMPERATURE_FLOOR build option, 0/1 */
+  int bc_flags[6];       /* xl, xu, yl, yu, zl, zu */
+  int tile1, tile2;      /* tile shapes of the two fused stages, -1 = default */
+  int kchunk1, kchunk2;  /* z cells per workgroup in the two stages, 0 = default */
+} chl_hydro_config;
+
+typedef struct chl_hydro chl_hydro; /* owns the integrator scratch of one box */
+
+const char *chl_version(void);
+const char *chl_last_error(void);
+int chl_device_count(int *n);
+int chl_device_name(char *buf, int len);
+
+/* ---- device memory helpers for hosts without their own allocator */
+int chl_malloc(void **ptr, size_t bytes);
+int chl_free(void *ptr);
+int chl_memcpy_htod(void *dst, const void *src, size_t bytes);
+int chl_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int chl_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int chl_synchronize(void);
+
+/* ---- integrator handle.  Replaces the static scratch that VL_Algorithm_3D_CUDA
+ * allocates on first call and Free_Memory_VL_3D releases
+ * (src/integrators/VL_3D_cuda.cu:58-133, :362-377; src/global/global_cuda.h:18-32). */
+int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out);
+int chl_hydro_destroy(chl_hydro *h);
+
+/* One VL step of the state d_conserved by dt (VL_Algorithm_3D_CUDA,
+ * src/integrators/VL_3D_cuda.h:9-11; VL_Algorithm_1D_CUDA, VL_1D_cuda.cu:30 when
+ * ny == nz == 1).  Real cells are updated in place; ghost cells are left to
+ * chl_set_boundaries, as in the reference. */
+int chl_hydro_vl_step(chl_hydro *h, double *d_conserved, double dx, double dy, double dz, double dt, double gamma,
+                      double density_floor);
+
+/* Grid3D::Update_Hydro_Grid (src/grid/grid3D.cpp:474-585) fused: VL step, temperature
+ * floor (U_floor as grid3D.cpp:497), and the next inverse timestep (Calc_dt_GPU)
+ * returned in *dti_out.  With fill_ghosts != 0 the ghost cells are also refreshed,
+ * as the following Set_Boundary_Conditions_Grid would (main.cpp:287-290). */
+int chl_hydro_update_grid(chl_hydro *h, double *d_conserved, double dx, double dy, double dz, double dt,
+                          double gamma, double density_floor, double U_floor, int fill_ghosts, double *dti_out);
+
+/* The reference driver loop (src/main.cpp:247-290) with dt kept on the device:
+ * for up to max_steps steps while t < tout: dt = 0.3/dti clamped to tout, VL step,
+ * floors, next dti, t += dt, ghost fill.  The state must hold valid ghost cells on
+ * entry.  With use_graph != 0 one step is captured once in a hipGraph and replayed.
+ * Returns the final time and the number of steps taken. */
+int chl_hydro_run(chl_hydro *h, double *d_conserved, double dx, double dy, double dz, double gamma,
+                  double density_floor, double U_floor, double t0, double tout, int max_steps, int use_graph,
+                  double *t_out, int *steps_out);
+
+/* Run n_steps device-timed steps (as chl_hydro_run with tout = +inf) and report the
+ * elapsed time measured with HIP events on the library's stream. */
+int chl_hydro_time_steps(chl_hydro *h, double *d_conserved, double dx, double dy, double dz, double gamma,
+                         int n_steps, int use_graph, float *ms_out, double *dt_last_out);
+
+/* Average duration of each fused kernel over n_steps steps, timed with HIP events
+ * around each launch on the library's stream (per-kernel roofline figures). */
+int chl_hydro_kernel_times(chl_hydro *h, double *d_conserved, double dx, double dy, double dz, double gamma,
+                           int n_steps, float *ms_stage1, float *ms_stage2, float *ms_other);
+
+/* Pick the stage tile shapes and z-chunks for this box by timing candidates on a
+ * scratch copy of d_conserved (the state itself is not modified). */
+int chl_hydro_autotune(chl_hydro *h, const double *d_conserved, double dx, double dy, double dz, double gamma,
+                       int steps_per_candidate, int *tile1, int *tile2, int *kchunk1, int *kchunk2);
+int chl_hydro_get_config(const chl_hydro *h, chl_hydro_config *out);
+
+/* ---- individual stages, same semantics and arguments as the reference functions */
+/* Calc_dt_GPU (src/hydro/hydro_cuda.h:69-70): max over real cells of the inverse
+ * crossing time; dt = C_cfl / *dti_out. */
+int chl_calc_dt(const double *dev_conserved, int nx, int ny, int nz, int n_ghost, int n_fields, double dx, double dy,
+                double dz, double gamma, double *dti_out);
+/* Apply_Temperature_Floor (src/hydro/hydro_cuda.h:87) */
+int chl_apply_temperature_floor(double *dev_conserved, int nx, int ny, int nz, int n_ghost, int n_fields,
+                                double U_floor);
+/* SetGhostCells (src/grid/cuda_boundaries.h:15-16): one face slab */
+int chl_set_ghost_cells(double *c_head, int nx, int ny, int nz, int n_fields, int n_cells, int n_ghost,
+                        const int flags[6], int isize, int jsize, int ksize, int imin, int jmin, int kmin, int dir);
+/* Set_Boundary_Conditions (single process, src/grid/boundary_conditions.cpp:96-111):
+ * all faces in one launch */
+int chl_set_boundaries(double *dev_conserved, int nx, int ny, int nz, int n_fields, int n_ghost, const int flags[6]);
+
+/* ---- reference-shaped stage kernels (PolicyExact), for checking the device
+ * physics against the reference's unit-test vectors */
+/* PLMC_cuda / PPMC_VL (plmc_cuda.cu:24, ppmc_cuda.cu:540) on device arrays */
+int chl_reconstruct(int reconstruction, const double *dev_conserved, double *dev_bounds_L, double *dev_bounds_R,
+                    int nx, int ny, int nz, double gamma, int dir);
+/* Calculate_HLLC_Fluxes_CUDA (hllc_cuda.cu:19) on device arrays */
+int chl_hllc_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, double *dev_flux, int nx, int ny, int nz,
+                    double gamma, int dir);
+
+/* ---- problem setup on the host (not timed): initial states of the benchmark boxes */
+/* Kelvin-Helmholtz box: n^3 real cells + ng ghosts, unit box, periodic, central slab
+ * |y-0.5|<0.25 rho=2 vx=+0.5, outside rho=1 vx=-0.5, p=2.5, vy = 0.01 sin(4 pi x)
+ * + 1e-4 u, u ~ U(-1,1) from std::mt19937_64(seed) drawn k, j, i with i fastest.
+ * host_U must hold 5*(n+2ng)^3 doubles; ghost cells are filled periodically. */
+int chl_ic_kelvin_helmholtz(double *host_U, int n, int ng, double gamma, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHOLLA_AMD_H */
