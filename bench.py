@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the hydro hot path on one MI355X: cell-updates/s of the VL
+PPMC+HLLC step on the 512^3 Kelvin-Helmholtz box, mixed FP32/FP64 form against
+the FP64 forms, with the per-field L1 of the mixed form against FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--forms mixed,fp64,fp64_exact]
+
+One step = one full VL step (stage 1 + stage 2 with fused floors, dt reduction
+and periodic ghost fill, plus the device-side dt update), graph-captured, state
+resident in HBM.  Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+BYTES_PER_CELL_UPDATE = 80  # FP64 state (5 fields) read once + written once
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--n", type=int, default=512, help="real cells per axis")
+    p.add_argument("--forms", default="mixed,fp64,fp64_exact")
+    p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--no-autotune", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--kernel-steps", type=int, default=10)
+    p.add_argument("--tiles", default="", help="fixed tile1,tile2,kchunk1,kchunk2 (skips the autotuner)")
+    p.add_argument("--calibrate-copy", type=float, default=0.0,
+                   help="only run a plain device copy of this many GiB (HBM counter calibration) and exit")
+    return p.parse_args()
+
+
+def calibrate_copy(gib):
+    """A plain copy kernel of known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {n * 8} bytes x3")
+
+
+def cpu_baseline(n_sample=192, steps=12):
+    """The oracle (C++ restatement, OpenMP) on the host cores, bounded sample of the same workload."""
+    from oracle import oracle as orc
+    import cholla_amd as chl
+
+    ng, g = 4, 5.0 / 3.0
+    N = n_sample + 2 * ng
+    U = chl.ic_kelvin_helmholtz(n_sample, ng, g, seed=7)
+    h = 1.0 / n_sample
+    threads = orc.lib().orc_num_threads()
+    orc.run(U, N, N, N, ng, 5, h, h, h, g, orc.PPMC, [1] * 6, 0.0, 1e30, 1)  # warm up (allocation, page-in)
+    t0 = time.perf_counter()
+    n_done, _ = orc.run(U, N, N, N, ng, 5, h, h, h, g, orc.PPMC, [1] * 6, 0.0, 1e30, steps)
+    sec = time.perf_counter() - t0
+    rate = n_sample ** 3 * n_done / sec
+    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"KH {n_sample}^3 PPMC+HLLC FP64, {n_done} steps, {sec:.2f} s (oracle, OpenMP)"}
+
+
+def main():
+    args = parse()
+    if args.calibrate_copy > 0:
+        calibrate_copy(args.calibrate_copy)
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU layout is a design note, "
+              "not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+    import cholla_amd as chl
+
+    if chl.device_count() < 1:
+        log("no HIP device visible")
+        sys.exit(1)
+    log("device:", chl.device_name())
+    n, ng, g = args.n, 4, 5.0 / 3.0
+    N = n + 2 * ng
+    h = 1.0 / n
+    cells = n ** 3
+    t0 = time.perf_counter()
+    U0 = chl.ic_kelvin_helmholtz(n, ng, g, seed=args.seed)
+    log(f"KH {n}^3 initial state built on host in {time.perf_counter() - t0:.1f} s")
+    D0 = chl.DeviceArray.from_host(U0)
+    del U0
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    res, finals = {}, {}
+    for form in forms:
+        prec = chl.PRECISIONS[form]
+        U = chl.DeviceArray(D0.n)
+        chl.check(chl.lib().chl_memcpy_dtod(U.ptr, D0.ptr, D0.nbytes))
+        tiles = [int(x) for x in args.tiles.split(",")] if args.tiles else [-1, -1, 0, 0]
+        hy = chl.Hydro(N, N, N, ng, chl.RECON_PPMC, prec, tile1=tiles[0], tile2=tiles[1], kchunk1=tiles[2],
+                       kchunk2=tiles[3])
+        tune = tuple(tiles) if args.tiles else None
+        if not args.no_autotune and not args.tiles:
+            t1 = time.perf_counter()
+            tune = hy.autotune(U, h, h, h, g, steps=2)
+            log(f"[{form}] autotune tile1,tile2,kchunk1,kchunk2 = {tune} ({time.perf_counter() - t1:.1f} s)")
+        # warm-up steps advance the state (like the reference's run), then K timed steps continue it
+        hy.time_steps(U, h, h, h, g, args.warmup, use_graph=True)
+        chl.check(chl.lib().chl_synchronize())
+        w0 = time.perf_counter()
+        ms, dt_last = hy.time_steps(U, h, h, h, g, args.steps, use_graph=True)
+        chl.check(chl.lib().chl_synchronize())
+        wall = time.perf_counter() - w0
+        s1, s2, so = hy.kernel_times(U, h, h, h, g, args.kernel_steps)
+        rate_ev = cells * args.steps / (ms * 1e-3)
+        rate_wall = cells * args.steps / wall
+        res[form] = {
+            "cell_updates_per_s": min(rate_ev, rate_wall), "ms_per_step": max(ms / args.steps, 1e3 * wall / args.steps),
+            "event_ms_per_step": ms / args.steps, "wall_ms_per_step": 1e3 * wall / args.steps,
+            "stage1_ms": s1, "stage2_ms": s2, "dt_kernel_ms": so, "tile": tune, "dt_last": dt_last,
+        }
+        log(f"[{form}] {res[form]['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s, "
+            f"{res[form]['ms_per_step']:.2f} ms/step (stage1 {s1:.2f} ms, stage2 {s2:.2f} ms)")
+        finals[form] = U
+        hy.close()
+    # mixed vs FP64: per-field relative L1 after the same number of steps from the same state
+    l1 = None
+    ref_form = "fp64" if "fp64" in finals else ("fp64_exact" if "fp64_exact" in finals else None)
+    if "mixed" in finals and ref_form:
+        a = finals["mixed"].download().reshape(5, N, N, N)[:, ng:-ng, ng:-ng, ng:-ng]
+        b = finals[ref_form].download().reshape(5, N, N, N)[:, ng:-ng, ng:-ng, ng:-ng]
+        mag = np.sqrt(b[1] ** 2 + b[2] ** 2 + b[3] ** 2).sum()
+        names = ["density", "momentum_x", "momentum_y", "momentum_z", "Energy"]
+        l1 = {"vs": ref_form, "strict": {}, "momentum_scaled": {}}
+        for f in range(5):
+            diff = np.abs(a[f] - b[f]).sum()
+            l1["strict"][names[f]] = float(diff / np.abs(b[f]).sum())
+            l1["momentum_scaled"][names[f]] = float(diff / (mag if f in (1, 2, 3) else np.abs(b[f]).sum()))
+        del a, b
+        log("mixed vs", ref_form, "L1:", json.dumps(l1))
+    head = "mixed" if "mixed" in res else forms[0]
+    r = res[head]
+    # roofline of the dominant kernel (stage 2 completes the cell update): algorithmic
+    # 80 B per cell-update x n^3 cells per launch / the launch's average duration
+    dom_ms = max(r["stage1_ms"], r["stage2_ms"])
+    dom = "stage2" if r["stage2_ms"] >= r["stage1_ms"] else "stage1"
+    achieved = BYTES_PER_CELL_UPDATE * cells / (r["stage2_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": "cell-updates/s at 512^3 PPMC+HLLC, mixed-FP32 vs FP64, at stated L1 error",
+        "value": r["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Kelvin-Helmholtz {n}^3 periodic, PPMC+HLLC, VL, CFL 0.3, form={head}",
+                   "cells": cells, "n_ghost": ng, "gamma": g, "seed": args.seed},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "Stage2Kernel",
+                     "dominant": dom, "dominant_ms": dom_ms,
+                     "algorithmic_bytes_per_launch": BYTES_PER_CELL_UPDATE * cells},
+        "step_roofline_frac": r["cell_updates_per_s"] * BYTES_PER_CELL_UPDATE / 1e9 / HBM_PEAK_GBS,
+        "forms": res,
+        "l1_mixed_vs_fp64": l1,
+    }
+    if "fp64" in res and "mixed" in res:
+        out["speedup_mixed_vs_fp64"] = res["mixed"]["cell_updates_per_s"] / res["fp64"]["cell_updates_per_s"]
+    if "fp64_exact" in res and "mixed" in res:
+        out["speedup_mixed_vs_fp64_exact"] = res["mixed"]["cell_updates_per_s"] / res["fp64_exact"]["cell_updates_per_s"]
+    if not args.no_cpu:
+        try:
+            cb = cpu_baseline()
+            log("cpu baseline:", json.dumps(cb))
+            out["cpu_baseline"] = cb
+        except Exception as e:  # the CPU leg must not hide the GPU measurement
+            log("cpu baseline failed:", e)
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

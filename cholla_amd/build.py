@@ -30,8 +30,14 @@ UNITS = [
     ("support_kernels.hip", ["-ffp-contract=off"]),
     ("hydro_fast.hip", ["-ffp-contract=fast"]),
     ("capi.cpp", ["-ffp-contract=off"]),
-    ("cholla_dropin.cpp", ["-ffp-contract=off"]),
 ]
+# Host programs built on the reference-signature adapter (cholla_dropin.cpp), one per
+# reconstruction build option, as Cholla's make.type files select them.  They need no HIP
+# headers: plain g++ against the C ABI.
+DRIVERS = {
+    "dropin_driver_plmc": ["-DPLMC", "-DDENSITY_FLOOR", "-DTEMPERATURE_FLOOR"],
+    "dropin_driver_ppmc": ["-DPPMC", "-DDENSITY_FLOOR", "-DTEMPERATURE_FLOOR"],
+}
 
 
 def hipcc():
@@ -63,8 +69,16 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr}")
     os.replace(tmp, LIB)
+    for name, defs in DRIVERS.items():
+        out = os.path.join(HERE, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", *defs, f"-I{os.path.join(ROOT, 'include')}",
+               os.path.join(CSRC, "cholla_dropin.cpp"), os.path.join(CSRC, "dropin_driver.cpp"),
+               f"-L{HERE}", "-lcholla_amd", "-Wl,-rpath,$ORIGIN", "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {', '.join(DRIVERS)}")
     return LIB
 
 
