@@ -166,12 +166,12 @@ struct SoundSpeed {
   T inv_2c, inv_2c2, inv_c2, inv_d;
   __device__ __forceinline__ SoundSpeed(const Prim<T> &q, T gamma)
   {
-    c       = Sqrt<P>(gamma * q.p / q.d);  // Calc_Sound_Speed, hydro_utilities.h:209-212
+    inv_d   = Rcp<P>(q.d);
+    c       = Sqrt<P>(Div<P>(gamma * q.p, q.d, inv_d));  // Calc_Sound_Speed, hydro_utilities.h:209-212
     c2      = c * c;
     inv_2c  = Rcp<P>(T(2.0) * c);
     inv_2c2 = Rcp<P>(T(2.0) * c2);
     inv_c2  = Rcp<P>(c2);
-    inv_d   = Rcp<P>(q.d);
   }
 };
 
@@ -206,24 +206,28 @@ __device__ __forceinline__ Prim<typename P::T> CharToPrim(const Prim<typename P:
 }
 
 // ---------------------------------------------------------------------------- PPM limiter
-// reconstruction::PPM_Single_Variable, src/reconstruction/reconstruction.h:794-901
+// reconstruction::PPM_Single_Variable, src/reconstruction/reconstruction.h:794-901.
+// Every branch of the reference is evaluated and the result selected, so a wave
+// never splits on the limiter's data-dependent tests; each candidate is computed
+// with the reference's own expression, so the selected value is the reference's.
+// SIGN(a) == SIGN(b) is (a < 0) == (b < 0), and SIGN(a) * m is (a < 0 ? -m : m).
 template <class P>
 __device__ __forceinline__ void PpmSingle(typename P::T im2, typename P::T im1, typename P::T i0, typename P::T ip1,
                                           typename P::T ip2, typename P::T &L_iph, typename P::T &R_imh)
 {
-  using T            = typename P::T;
-  const T C2         = T(1.25);
-  const T inv6       = T(1.0 / 6.0);
+  using T      = typename P::T;
+  const T C2   = T(1.25);
+  const T inv6 = T(1.0 / 6.0);
+  auto neg     = [](T x) { return x < T(0); };
   auto limit_iface = [&](T c_i, T c_im1, T iface, T s2_i, T s2_im1) -> T {
     T s2c     = T(3.0) * (c_im1 + c_i - T(2.0) * iface);
-    T limited = T(0.0);
-    if (SignOf(s2c) == SignOf(s2_im1) && SignOf(s2c) == SignOf(s2_i)) {
-      limited = SignOf(s2c) * Fmin(C2 * Fabs(s2_im1), Fmin(C2 * Fabs(s2_i), Fabs(s2c)));
-    }
-    T dl = iface - c_im1;
-    T dr = c_i - iface;
-    if (dl * dr < T(0.0)) return T(0.5) * (c_im1 + c_i) - Div<P>(limited, T(6.0), inv6);
-    return iface;
+    bool same = (neg(s2c) == neg(s2_im1)) && (neg(s2c) == neg(s2_i));
+    T mag     = Fmin(C2 * Fabs(s2_im1), Fmin(C2 * Fabs(s2_i), Fabs(s2c)));
+    T limited = same ? (neg(s2c) ? -mag : mag) : T(0.0);
+    T dl      = iface - c_im1;
+    T dr      = c_i - iface;
+    T extr    = T(0.5) * (c_im1 + c_i) - Div<P>(limited, T(6.0), inv6);
+    return (dl * dr < T(0.0)) ? extr : iface;
   };
 
   T slope_left    = (i0 - im1);
@@ -242,12 +246,10 @@ __device__ __forceinline__ void PpmSingle(typename P::T im2, typename P::T im1, 
   R_imh = limit_iface(i0, im1, R_imh, s2_i, s2_im1);
   L_iph = limit_iface(ip1, i0, L_iph, s2_ip1, s2_i);
 
-  T s2_face    = T(6.0) * (R_imh + L_iph - T(2.0) * i0);
-  T s2_limited = T(0.0);
-  if (SignOf(s2_im1) == SignOf(s2_i) && SignOf(s2_im1) == SignOf(s2_ip1) && SignOf(s2_im1) == SignOf(s2_face)) {
-    s2_limited =
-        SignOf(s2_face) * Fmin(Fmin(C2 * Fabs(s2_im1), C2 * Fabs(s2_i)), Fmin(C2 * Fabs(s2_ip1), Fabs(s2_face)));
-  }
+  T s2_face   = T(6.0) * (R_imh + L_iph - T(2.0) * i0);
+  bool smooth = neg(s2_im1) == neg(s2_i) && neg(s2_im1) == neg(s2_ip1) && neg(s2_im1) == neg(s2_face);
+  T mag2      = Fmin(Fmin(C2 * Fabs(s2_im1), C2 * Fabs(s2_i)), Fmin(C2 * Fabs(s2_ip1), Fabs(s2_face)));
+  T s2_limited = smooth ? (neg(s2_face) ? -mag2 : mag2) : T(0.0);
 
   T cell_max = Fmax(Fabs(im2), Fabs(im1));
   cell_max   = Fmax(cell_max, Fabs(i0));
@@ -257,19 +259,70 @@ __device__ __forceinline__ void PpmSingle(typename P::T im2, typename P::T im1, 
   // The 1e-12 round-off guards are the reference's; in FP32 they are below the
   // format's resolution and act as "exactly zero" tests, which keeps the same
   // branch structure.
-  T rho = (Fabs(s2_face) > T(1.0e-12) * cell_max) ? Div1<P>(s2_limited, s2_face) : T(0.0);
+  T ratio = Div1<P>(s2_limited, s2_face);
+  T rho   = (Fabs(s2_face) > T(1.0e-12) * cell_max) ? ratio : T(0.0);
 
-  T sfl = i0 - R_imh;
-  T sfr = L_iph - i0;
-  if ((sfl * sfr) <= T(0.0) || ((ip1 - i0) * (i0 - im1)) <= T(0.0)) {
-    if (rho <= (T(1.0) - T(1.0e-12))) {
-      R_imh = i0 - rho * sfl;
-      L_iph = i0 + rho * sfr;
+  T sfl         = i0 - R_imh;
+  T sfr         = L_iph - i0;
+  bool extremum = (sfl * sfr) <= T(0.0) || ((ip1 - i0) * (i0 - im1)) <= T(0.0);
+  bool smooth_x = rho <= (T(1.0) - T(1.0e-12));
+  T R_ext       = smooth_x ? i0 - rho * sfl : R_imh;
+  T L_ext       = smooth_x ? i0 + rho * sfr : L_iph;
+  T R_mono      = (Fabs(sfl) >= T(2.0) * Fabs(sfr)) ? i0 - T(2.0) * sfr : R_imh;
+  T L_mono      = (Fabs(sfr) >= T(2.0) * Fabs(sfl)) ? i0 + T(2.0) * sfl : L_iph;
+  R_imh         = extremum ? R_ext : R_mono;
+  L_iph         = extremum ? L_ext : L_mono;
+}
+
+// The same limiter for the mixed form, algebraically reduced (valid in exact
+// arithmetic, rounded differently): the averaged-slope interface values of
+// reconstruction.h:830-839 are the fourth-order interpolants
+//   R_{i-1/2} = 7/12 (a_{i-1} + a_i) - 1/12 (a_{i-2} + a_{i+1}),
+//   L_{i+1/2} = 7/12 (a_i + a_{i+1}) - 1/12 (a_{i-1} + a_{i+2}),
+// and "SIGN agrees for all of x, y, z" is min3 >= 0 or max3 < 0.  Data-dependent
+// choices stay ternaries so a wave that agrees runs one side only.
+template <typename T>
+__device__ __forceinline__ void PpmSingleReduced(T im2, T im1, T i0, T ip1, T ip2, T &L_iph, T &R_imh)
+{
+  const T C2   = T(1.25);
+  const T sixth = T(1.0 / 6.0);
+  T sL = i0 - im1, sR = ip1 - i0;
+  T a_m = im1 + i0, a_p = i0 + ip1;
+  T Rv = T(7.0 / 12.0) * a_m - T(1.0 / 12.0) * (im2 + ip1);
+  T Lv = T(7.0 / 12.0) * a_p - T(1.0 / 12.0) * (im1 + ip2);
+  T s2m = im2 + i0 - T(2.0) * im1;
+  T s2c = im1 + ip1 - T(2.0) * i0;
+  T s2p = i0 + ip2 - T(2.0) * ip1;
+  auto lim = [&](T asum, T c_left, T c_right, T iface, T s2a, T s2b) -> T {
+    T s2f   = T(3.0) * asum - T(6.0) * iface;
+    T mn    = __builtin_fmin(__builtin_fmin(s2f, s2a), s2b);
+    T mx    = __builtin_fmax(__builtin_fmax(s2f, s2a), s2b);
+    T mag   = Fmin(C2 * Fmin(Fabs(s2a), Fabs(s2b)), Fabs(s2f));
+    T limit = mn >= T(0) ? mag : (mx < T(0) ? -mag : T(0));
+    bool ext = (iface - c_left) * (c_right - iface) < T(0);
+    return ext ? T(0.5) * asum - limit * sixth : iface;
+  };
+  T R = lim(a_m, im1, i0, Rv, s2c, s2m);
+  T L = lim(a_p, i0, ip1, Lv, s2p, s2c);
+  T s2face = T(6.0) * (R + L) - T(12.0) * i0;
+  T mn4    = __builtin_fmin(__builtin_fmin(__builtin_fmin(s2m, s2c), s2p), s2face);
+  T mx4    = __builtin_fmax(__builtin_fmax(__builtin_fmax(s2m, s2c), s2p), s2face);
+  T mag2   = Fmin(C2 * Fmin(Fmin(Fabs(s2m), Fabs(s2c)), Fabs(s2p)), Fabs(s2face));
+  T s2lim  = mn4 >= T(0) ? mag2 : (mx4 < T(0) ? -mag2 : T(0));
+  T cmax   = Fmax(Fmax(Fmax(Fabs(im2), Fabs(im1)), Fmax(Fabs(i0), Fabs(ip1))), Fabs(ip2));
+  T rho    = Fabs(s2face) > T(1.0e-12) * cmax ? s2lim * __builtin_amdgcn_rcpf(s2face) : T(0);
+  T sfl = i0 - R, sfr = L - i0;
+  if ((sfl * sfr) <= T(0) || (sR * sL) <= T(0)) {
+    if (rho <= T(1.0) - T(1.0e-12)) {
+      R = i0 - rho * sfl;
+      L = i0 + rho * sfr;
     }
   } else {
-    if (Fabs(sfl) >= T(2.0) * Fabs(sfr)) R_imh = i0 - T(2.0) * sfr;
-    if (Fabs(sfr) >= T(2.0) * Fabs(sfl)) L_iph = i0 + T(2.0) * sfl;
+    if (Fabs(sfl) >= T(2.0) * Fabs(sfr)) R = i0 - T(2.0) * sfr;
+    if (Fabs(sfr) >= T(2.0) * Fabs(sfl)) L = i0 + T(2.0) * sfl;
   }
+  R_imh = R;
+  L_iph = L;
 }
 
 // ---------------------------------------------------------------------------- PPMC (VL)
@@ -287,11 +340,19 @@ __device__ __forceinline__ void ReconPpmc(const Prim<typename P::T> s[5], typena
 #pragma unroll
   for (int m = 0; m < 5; m++) a[m] = PrimToChar<P>(q, s[m], cs);
   Char5<T> L, R;
-  PpmSingle<P>(a[0].a0, a[1].a0, a[2].a0, a[3].a0, a[4].a0, L.a0, R.a0);
-  PpmSingle<P>(a[0].a1, a[1].a1, a[2].a1, a[3].a1, a[4].a1, L.a1, R.a1);
-  PpmSingle<P>(a[0].a2, a[1].a2, a[2].a2, a[3].a2, a[4].a2, L.a2, R.a2);
-  PpmSingle<P>(a[0].a3, a[1].a3, a[2].a3, a[3].a3, a[4].a3, L.a3, R.a3);
-  PpmSingle<P>(a[0].a4, a[1].a4, a[2].a4, a[3].a4, a[4].a4, L.a4, R.a4);
+  if constexpr (P::kId == PolicyMixed::kId) {
+    PpmSingleReduced(a[0].a0, a[1].a0, a[2].a0, a[3].a0, a[4].a0, L.a0, R.a0);
+    PpmSingleReduced(a[0].a1, a[1].a1, a[2].a1, a[3].a1, a[4].a1, L.a1, R.a1);
+    PpmSingleReduced(a[0].a2, a[1].a2, a[2].a2, a[3].a2, a[4].a2, L.a2, R.a2);
+    PpmSingleReduced(a[0].a3, a[1].a3, a[2].a3, a[3].a3, a[4].a3, L.a3, R.a3);
+    PpmSingleReduced(a[0].a4, a[1].a4, a[2].a4, a[3].a4, a[4].a4, L.a4, R.a4);
+  } else {
+    PpmSingle<P>(a[0].a0, a[1].a0, a[2].a0, a[3].a0, a[4].a0, L.a0, R.a0);
+    PpmSingle<P>(a[0].a1, a[1].a1, a[2].a1, a[3].a1, a[4].a1, L.a1, R.a1);
+    PpmSingle<P>(a[0].a2, a[1].a2, a[2].a2, a[3].a2, a[4].a2, L.a2, R.a2);
+    PpmSingle<P>(a[0].a3, a[1].a3, a[2].a3, a[3].a3, a[4].a3, L.a3, R.a3);
+    PpmSingle<P>(a[0].a4, a[1].a4, a[2].a4, a[3].a4, a[4].a4, L.a4, R.a4);
+  }
   L_iph   = CharToPrim<P>(q, L, cs);
   R_imh   = CharToPrim<P>(q, R, cs);
   R_imh.d = Fmax(R_imh.d, T(kTinyNumber));
@@ -476,7 +537,27 @@ __device__ __forceinline__ Cons<typename P::T> Hllc(const RState<typename P::T> 
 }
 
 // ---------------------------------------------------------------------------- dt
-// hydroInverseCrossingTime, src/hydro/hydro_cuda.cu:404-418 (FP64 in every form)
+// hydroInverseCrossingTime, src/hydro/hydro_cuda.cu:404-418, FP64 in every form; the
+// non-exact policies multiply by precomputed 1/dx, 1/dy, 1/dz instead of dividing.
+template <class P>
+__device__ __forceinline__ double InverseCrossingTimeP(double E, double d, double d_inv, double vx, double vy,
+                                                       double vz, double dx, double dy, double dz, double idx,
+                                                       double idy, double idz, double gamma)
+{
+  double P_ = (E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+  double cs = __builtin_sqrt(d_inv * gamma * P_);
+  double ax, ay, az;
+  if constexpr (P::kExact) {
+    ax = (Fabs(vx) + cs) / dx, ay = (Fabs(vy) + cs) / dy, az = (Fabs(vz) + cs) / dz;
+  } else {
+    ax = (Fabs(vx) + cs) * idx, ay = (Fabs(vy) + cs) * idy, az = (Fabs(vz) + cs) * idz;
+  }
+  double m = Fmax(ax, ay);
+  m        = Fmax(m, az);
+  m        = Fmax(m, 0.0);
+  return m;
+}
+
 __device__ __forceinline__ double InverseCrossingTime(double E, double d, double d_inv, double vx, double vy, double vz,
                                                       double dx, double dy, double dz, double gamma)
 {

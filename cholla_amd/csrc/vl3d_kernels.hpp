@@ -332,8 +332,17 @@ struct Stage2Shape {
   static constexpr int NF  = NFX > NFY ? NFX : NFY;
 };
 
+// Minimum waves per SIMD requested from the register allocator.  Left to itself it
+// gives the FP64 forms a 256-VGPR budget (one wave per SIMD), which leaves the long
+// FP64 divide/sqrt chains and LDS round trips without latency cover.
+template <class P>
+struct Stage2Occupancy {
+  static constexpr int kMinWaves = P::kId == PolicyMixed::kId ? 1 : 2;
+};
+
 template <class P, int RK, int TX, int TY>
-__global__ void __launch_bounds__(TX *TY) Stage2Kernel(StepArgs a)
+__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Stage2Occupancy<P>::kMinWaves)))
+Stage2Kernel(StepArgs a)
 {
   using T          = typename P::T;
   using S          = Stage2Shape<P, RK, TX, TY>;
@@ -365,6 +374,7 @@ __global__ void __launch_bounds__(TX *TY) Stage2Kernel(StepArgs a)
   const double dtodx = dt / a.dx, dtody = dt / a.dy, dtodz = dt / a.dz;
   const T *W         = reinterpret_cast<const T *>(a.W);
   double *U          = a.U;
+  const double inv_dx = 1.0 / a.dx, inv_dy = 1.0 / a.dy, inv_dz = 1.0 / a.dz;
   if (dt == 0.0) return;  // a device-timed step past the output time (uniform per launch)
 
   auto loadW = [&](int ii, int jj, int kk) {
@@ -372,167 +382,166 @@ __global__ void __launch_bounds__(TX *TY) Stage2Kernel(StepArgs a)
     return WCell<T>{W[id], W[nc + id], W[2 * nc + id], W[3 * nc + id], W[4 * nc + id]};
   };
 
-  // z window: win[m] = W(k - r + m)
+  // z window of this lane's column: win[m] = W(c - r + m) for the cell c whose z
+  // interfaces are reconstructed next.  Prologue: cells k0-1 and k0 give the face k0-1/2.
   WCell<T> win[NS];
 #pragma unroll
-  for (int m = 0; m < NS - 1; m++) win[m + 1] = loadW(ic, jc, k0 - 1 - r + m);
-
-  Prim<T> qLz_prev;
-  T fz_prev[5];
-  double acc_prev[5], u_prev[5];
+  for (int m = 0; m < NS; m++) win[m] = loadW(ic, jc, k0 - 1 - r + m);
+  Prim<T> qLz, qRz;
+  ReconDir<P, RK, 2>(win, gamma, qLz, qRz);  // cell k0-1
+  Prim<T> qL_lo = qLz;                       // left state of face k0-1/2
+#pragma unroll
+  for (int m = 0; m < NS - 1; m++) win[m] = win[m + 1];
+  win[NS - 1] = loadW(ic, jc, k0 + r);
+  ReconDir<P, RK, 2>(win, gamma, qLz, qRz);  // cell k0
+  T fz_lo[5];                                // flux through face k-1/2 of the current cell
+  {
+    Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL_lo, gamma, inv_gm1), RStateFromPrim<P>(qRz, gamma, inv_gm1), gamma);
+    FluxGlobal<P, 2>(f, fz_lo);
+  }
+  qL_lo = qLz;  // left state of face k0+1/2
   double dti_max = 0.0;
 
-  for (int k = k0 - 1; k <= k1; k++) {
+  for (int k = k0; k < k1; k++) {
+    // Stage plane k (the window centre) into LDS.  Every read of the previous plane
+    // and of the interface/flux buffers finished before the last barrier of the
+    // previous iteration.
+    plane[ty + H][tx + H] = win[r];
+    for (int h = tid; h < 2 * H * TY + 2 * H * TX; h += NT) {
+      int pi, pj;
+      if (h < 2 * H * TY) {
+        int row = h / (2 * H), c = h - row * 2 * H;
+        pj = H + row;
+        pi = c < H ? c : TX + c;
+      } else {
+        int g = h - 2 * H * TY;
+        int col = g / (2 * H), c = g - col * 2 * H;
+        pi = H + col;
+        pj = c < H ? c : TY + c;
+      }
+      int gi = min(i0 - H + pi, nx - 1), gj = min(j0 - H + pj, ny - 1);
+      plane[pj][pi] = loadW(gi, gj, k);
+    }
+    // z: reconstruct cell k+1 and solve face k+1/2 while the halo loads land
 #pragma unroll
     for (int m = 0; m < NS - 1; m++) win[m] = win[m + 1];
-    win[NS - 1] = loadW(ic, jc, k + r);
-
-    // z reconstruction of cell k, face k-1/2, and the epilogue of cell k-1
-    Prim<T> qLz, qRz;
+    win[NS - 1] = loadW(ic, jc, k + 1 + r);
     ReconDir<P, RK, 2>(win, gamma, qLz, qRz);
-    T fz[5];
-    if (k >= k0) {
-      Cons<T> f = Hllc<P>(RStateFromPrim<P>(qLz_prev, gamma, inv_gm1), RStateFromPrim<P>(qRz, gamma, inv_gm1), gamma);
-      FluxGlobal<P, 2>(f, fz);
-      if (k - 1 >= k0 && own) {
-        // U += ((dtodx dFx + dtody dFy) + dtodz dFz), hydro_cuda.cu:241-257
-        double un[5];
-#pragma unroll
-        for (int q = 0; q < 5; q++) un[q] = u_prev[q] + (acc_prev[q] + dtodz * (double(fz_prev[q]) - double(fz[q])));
-        const int kk  = k - 1;
-        const size_t id = Lin(i, j, kk, nx, ny);
-        if ((a.flags & kFlagDensityFloor) && un[0] < a.density_floor) {  // hydro_cuda.cu:279-299
-          if (un[0] > 0) {
-            double d0 = un[0];
-            un[0]     = a.density_floor;
-#pragma unroll
-            for (int q = 1; q < 5; q++) un[q] *= (a.density_floor / d0);
-          } else {
-            // Average_Cell_Single_Field (hydro_cuda.cu:1155-1177) over the six
-            // neighbours as they are in memory now; the reference reads them
-            // while other threads update them as well.
-            un[0] = (U[id - 1] + U[id + 1] + U[id - nx] + U[id + nx] + U[id - size_t(nx) * ny] +
-                     U[id + size_t(nx) * ny]) /
-                    6;
-          }
-        }
-        double dinv = 1.0 / un[0];
-        double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
-        if (a.flags & kFlagTemperatureFloor) {  // Temperature_Floor_Kernel, hydro_cuda.cu:1132-1146
-          double Ekin = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz);
-          double uint = (un[4] - Ekin) / un[0];
-          if (uint < a.U_floor) un[4] = Ekin + un[0] * a.U_floor;
-        }
-#pragma unroll
-        for (int q = 0; q < 5; q++) U[q * nc + id] = un[q];
-        if (a.flags & kFlagReduceDt) {
-          dti_max = Fmax(dti_max, InverseCrossingTime(un[4], un[0], dinv, vx, vy, vz, a.dx, a.dy, a.dz, a.gamma));
-        }
-        if (a.flags & kFlagPeriodicGhosts) {
-          // every periodic image of this real cell inside the ghost layers
-          const int Lx = nx - 2 * ng, Ly = ny - 2 * ng, Lz = nz - 2 * ng;
-          // (the host enables this only when every axis holds >= ng real cells,
-          // so each image lands in a ghost layer and maps back to this cell)
-          int xs[3] = {i, i, i}, ys[3] = {j, j, j}, zs[3] = {kk, kk, kk};
-          int nxi = 1, nyi = 1, nzi = 1;
-          if (i < 2 * ng) xs[nxi++] = i + Lx;
-          if (i >= nx - 2 * ng) xs[nxi++] = i - Lx;
-          if (j < 2 * ng) ys[nyi++] = j + Ly;
-          if (j >= ny - 2 * ng) ys[nyi++] = j - Ly;
-          if (kk < 2 * ng) zs[nzi++] = kk + Lz;
-          if (kk >= nz - 2 * ng) zs[nzi++] = kk - Lz;
-          for (int c = 1; c < nxi * nyi * nzi; c++) {
-            int ax = c % nxi, ay = (c / nxi) % nyi, az = c / (nxi * nyi);
-            size_t g = Lin(xs[ax], ys[ay], zs[az], nx, ny);
-#pragma unroll
-            for (int q = 0; q < 5; q++) U[q * nc + g] = un[q];
-          }
-        }
-      }
+    T fz_hi[5];
+    {
+      Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL_lo, gamma, inv_gm1), RStateFromPrim<P>(qRz, gamma, inv_gm1), gamma);
+      FluxGlobal<P, 2>(f, fz_hi);
     }
-    if (k >= k0) {
+    qL_lo = qLz;
+    __syncthreads();
+    // x reconstruction of cells -1..TX of every row
+    for (int it = tid; it < S::NQX; it += NT) {
+      int rj = it / (TX + 2), ri = it - rj * (TX + 2);  // plane column of cell = ri - 1 + H
+      WCell<T> s[NS];
 #pragma unroll
-      for (int q = 0; q < 5; q++) fz_prev[q] = fz[q];
+      for (int m = 0; m < NS; m++) s[m] = plane[rj + H][ri - 1 + H - r + m];
+      ReconDir<P, RK, 0>(s, gamma, qL[it], qR[it]);
     }
-    qLz_prev = qLz;
-
-    if (k >= k0 && k < k1) {
-      // ---- x/y sweeps of plane k through LDS
-      {
-        size_t id = Lin(ic, jc, k, nx, ny);
+    __syncthreads();
+    for (int it = tid; it < S::NFX; it += NT) {
+      int fj = it / (TX + 1), fi = it - fj * (TX + 1);  // between recon cells fi and fi+1 of row fj
+      int ql = fj * (TX + 2) + fi;
+      Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL[ql], gamma, inv_gm1), RStateFromPrim<P>(qR[ql + 1], gamma, inv_gm1),
+                          gamma);
+      T g[5];
+      FluxGlobal<P, 0>(f, g);
 #pragma unroll
-        for (int q = 0; q < 5; q++) u_prev[q] = U[q * nc + id];
-      }
-      __syncthreads();  // previous plane / buffers consumed
-      plane[ty + H][tx + H] = win[r];
-      for (int h = tid; h < 2 * H * TY + 2 * H * TX; h += NT) {
-        int pi, pj;
-        if (h < 2 * H * TY) {
-          int row = h / (2 * H), c = h - row * 2 * H;
-          pj = H + row;
-          pi = c < H ? c : TX + c;
-        } else {
-          int g = h - 2 * H * TY;
-          int col = g / (2 * H), c = g - col * 2 * H;
-          pi = H + col;
-          pj = c < H ? c : TY + c;
-        }
-        int gi = min(i0 - H + pi, nx - 1), gj = min(j0 - H + pj, ny - 1);
-        plane[pj][pi] = loadW(gi, gj, k);
-      }
-      __syncthreads();
-      // x reconstruction of cells -1..TX of every row
-      for (int it = tid; it < S::NQX; it += NT) {
-        int rj = it / (TX + 2), ri = it - rj * (TX + 2);  // plane column of cell = ri - 1 + H
-        WCell<T> s[NS];
+      for (int q = 0; q < 5; q++) fl[q][it] = g[q];
+    }
+    __syncthreads();
+    double acc[5];
+    {
+      int fa = ty * (TX + 1) + tx;
 #pragma unroll
-        for (int m = 0; m < NS; m++) s[m] = plane[rj + H][ri - 1 + H - r + m];
-        ReconDir<P, RK, 0>(s, gamma, qL[it], qR[it]);
-      }
-      __syncthreads();
-      for (int it = tid; it < S::NFX; it += NT) {
-        int fj = it / (TX + 1), fi = it - fj * (TX + 1);  // between recon cells fi and fi+1 of row fj
-        int ql = fj * (TX + 2) + fi;
-        Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL[ql], gamma, inv_gm1), RStateFromPrim<P>(qR[ql + 1], gamma, inv_gm1),
-                            gamma);
-        T g[5];
-        FluxGlobal<P, 0>(f, g);
+      for (int q = 0; q < 5; q++) acc[q] = dtodx * (double(fl[q][fa]) - double(fl[q][fa + 1]));
+    }
+    // y reconstruction of cells -1..TY of every column
+    for (int it = tid; it < S::NQY; it += NT) {
+      int rj = it / TX, ri = it - rj * TX;
+      WCell<T> s[NS];
 #pragma unroll
-        for (int q = 0; q < 5; q++) fl[q][it] = g[q];
-      }
-      __syncthreads();
-      double accx[5];
-      {
-        int fa = ty * (TX + 1) + tx;
+      for (int m = 0; m < NS; m++) s[m] = plane[rj - 1 + H - r + m][ri + H];
+      ReconDir<P, RK, 1>(s, gamma, qL[it], qR[it]);
+    }
+    __syncthreads();
+    for (int it = tid; it < S::NFY; it += NT) {
+      int fj = it / TX, fi = it - fj * TX;  // between recon rows fj and fj+1 of column fi
+      Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL[fj * TX + fi], gamma, inv_gm1),
+                          RStateFromPrim<P>(qR[(fj + 1) * TX + fi], gamma, inv_gm1), gamma);
+      T g[5];
+      FluxGlobal<P, 1>(f, g);
 #pragma unroll
-        for (int q = 0; q < 5; q++) accx[q] = dtodx * (double(fl[q][fa]) - double(fl[q][fa + 1]));
-      }
-      // y reconstruction of cells -1..TY of every column
-      for (int it = tid; it < S::NQY; it += NT) {
-        int rj = it / TX, ri = it - rj * TX;
-        WCell<T> s[NS];
-#pragma unroll
-        for (int m = 0; m < NS; m++) s[m] = plane[rj - 1 + H - r + m][ri + H];
-        ReconDir<P, RK, 1>(s, gamma, qL[it], qR[it]);
-      }
-      __syncthreads();
-      for (int it = tid; it < S::NFY; it += NT) {
-        int fj = it / TX, fi = it - fj * TX;  // between recon rows fj and fj+1 of column fi
-        Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL[fj * TX + fi], gamma, inv_gm1),
-                            RStateFromPrim<P>(qR[(fj + 1) * TX + fi], gamma, inv_gm1), gamma);
-        T g[5];
-        FluxGlobal<P, 1>(f, g);
-#pragma unroll
-        for (int q = 0; q < 5; q++) fl[q][it] = g[q];
-      }
-      __syncthreads();
+      for (int q = 0; q < 5; q++) fl[q][it] = g[q];
+    }
+    __syncthreads();
+    if (own) {
+      const size_t id = Lin(i, j, k, nx, ny);
+      double un[5];
       {
         int fa = ty * TX + tx;
+        // U += ((dtodx dFx + dtody dFy) + dtodz dFz), hydro_cuda.cu:241-257
 #pragma unroll
-        for (int q = 0; q < 5; q++)
-          acc_prev[q] = accx[q] + dtody * (double(fl[q][fa]) - double(fl[q][fa + TX]));
+        for (int q = 0; q < 5; q++) {
+          double axy = acc[q] + dtody * (double(fl[q][fa]) - double(fl[q][fa + TX]));
+          un[q]      = U[q * nc + id] + (axy + dtodz * (double(fz_lo[q]) - double(fz_hi[q])));
+        }
+      }
+      if ((a.flags & kFlagDensityFloor) && un[0] < a.density_floor) {  // hydro_cuda.cu:279-299
+        if (un[0] > 0) {
+          double d0 = un[0];
+          un[0]     = a.density_floor;
+#pragma unroll
+          for (int q = 1; q < 5; q++) un[q] *= (a.density_floor / d0);
+        } else {
+          // Average_Cell_Single_Field (hydro_cuda.cu:1155-1177) over the six
+          // neighbours as they are in memory now; the reference reads them
+          // while other threads update them as well.
+          un[0] = (U[id - 1] + U[id + 1] + U[id - nx] + U[id + nx] + U[id - size_t(nx) * ny] +
+                   U[id + size_t(nx) * ny]) /
+                  6;
+        }
+      }
+      double dinv = 1.0 / un[0];
+      double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
+      if (a.flags & kFlagTemperatureFloor) {  // Temperature_Floor_Kernel, hydro_cuda.cu:1132-1146
+        double Ekin = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz);
+        double uint = P::kExact ? (un[4] - Ekin) / un[0] : (un[4] - Ekin) * dinv;
+        if (uint < a.U_floor) un[4] = Ekin + un[0] * a.U_floor;
+      }
+#pragma unroll
+      for (int q = 0; q < 5; q++) U[q * nc + id] = un[q];
+      if (a.flags & kFlagReduceDt) {
+        dti_max = Fmax(dti_max, InverseCrossingTimeP<P>(un[4], un[0], dinv, vx, vy, vz, a.dx, a.dy, a.dz, inv_dx,
+                                                        inv_dy, inv_dz, a.gamma));
+      }
+      if (a.flags & kFlagPeriodicGhosts) {
+        // every periodic image of this real cell inside the ghost layers (the host
+        // enables this only when every axis holds >= ng real cells, so each image
+        // lands in a ghost layer and maps back to this cell)
+        const int Lx = nx - 2 * ng, Ly = ny - 2 * ng, Lz = nz - 2 * ng;
+        int xs[3] = {i, i, i}, ys[3] = {j, j, j}, zs[3] = {k, k, k};
+        int nxi = 1, nyi = 1, nzi = 1;
+        if (i < 2 * ng) xs[nxi++] = i + Lx;
+        if (i >= nx - 2 * ng) xs[nxi++] = i - Lx;
+        if (j < 2 * ng) ys[nyi++] = j + Ly;
+        if (j >= ny - 2 * ng) ys[nyi++] = j - Ly;
+        if (k < 2 * ng) zs[nzi++] = k + Lz;
+        if (k >= nz - 2 * ng) zs[nzi++] = k - Lz;
+        for (int c = 1; c < nxi * nyi * nzi; c++) {
+          int ax = c % nxi, ay = (c / nxi) % nyi, az = c / (nxi * nyi);
+          size_t g = Lin(xs[ax], ys[ay], zs[az], nx, ny);
+#pragma unroll
+          for (int q = 0; q < 5; q++) U[q * nc + g] = un[q];
+        }
       }
     }
+#pragma unroll
+    for (int q = 0; q < 5; q++) fz_lo[q] = fz_hi[q];
   }
 
   if (a.flags & kFlagReduceDt) {

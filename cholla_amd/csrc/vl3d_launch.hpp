@@ -13,8 +13,8 @@ namespace chl
 struct TileShape {
   int tx, ty;
 };
-constexpr int kNumTiles                = 4;
-constexpr TileShape kTiles[kNumTiles] = {{32, 8}, {16, 16}, {64, 4}, {16, 8}};
+constexpr int kNumTiles                = 5;
+constexpr TileShape kTiles[kNumTiles] = {{32, 8}, {16, 16}, {64, 4}, {16, 8}, {32, 16}};
 
 inline int CeilDiv(int a, int b) { return (a + b - 1) / b; }
 
@@ -48,6 +48,8 @@ hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
       return LaunchStage1T<P, 64, 4>(a, R1, s);
     case 3:
       return LaunchStage1T<P, 16, 8>(a, R1, s);
+    case 4:
+      return LaunchStage1T<P, 32, 16>(a, R1, s);
   }
   return hipErrorInvalidValue;
 }
@@ -64,6 +66,8 @@ hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
       return LaunchStage2T<P, RK, 64, 4>(a, s);
     case 3:
       return LaunchStage2T<P, RK, 16, 8>(a, s);
+    case 4:
+      return LaunchStage2T<P, RK, 32, 16>(a, s);
   }
   return hipErrorInvalidValue;
 }
