@@ -4,6 +4,7 @@ PPMC+HLLC step on the 512^3 Kelvin-Helmholtz box, mixed FP32/FP64 form against
 the FP64 forms, with the per-field L1 of the mixed form against FP64.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--forms mixed,fp64,fp64_exact]
+    python bench.py --workload ensemble [--boxes 384] [--steps 100]   # batched-ensemble line
 
 One step = one full VL step (stage 1 + stage 2 with fused floors, dt reduction
 and periodic ghost fill, plus the device-side dt update), graph-captured, state
@@ -40,6 +41,11 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--kernel-steps", type=int, default=10)
     p.add_argument("--tiles", default="", help="fixed tile1,tile2,kchunk1,kchunk2 (skips the autotuner)")
+    p.add_argument("--workload", default="kh", choices=["kh", "ensemble"])
+    p.add_argument("--boxes", type=int, default=384, help="ensemble: number of boxes")
+    p.add_argument("--looped-steps", type=int, default=10, help="ensemble: steps of the per-box loop baseline")
+    p.add_argument("--launch", default="auto", choices=["auto", "split", "persistent"],
+                   help="ensemble: launch form (auto = autotuned)")
     p.add_argument("--calibrate-copy", type=float, default=0.0,
                    help="only run a plain device copy of this many GiB (HBM counter calibration) and exit")
     return p.parse_args()
@@ -77,6 +83,105 @@ def cpu_baseline(n_sample=192, steps=12):
             "sample": f"KH {n_sample}^3 PPMC+HLLC FP64, {n_done} steps, {sec:.2f} s (oracle, OpenMP)"}
 
 
+def bench_ensemble(args):
+    """Batched ensemble: boxes of edges from {32..128} and rms Mach from U[0.5, 3] (seeded),
+    periodic turbulence, PPMC+HLLC, per-box CFL dt; the fused launch (autotuned form and
+    tiles, one graph replay per step) against a loop of per-box launches, and the mixed
+    form's per-field L1 against the FP64 form after the same steps."""
+    import cholla_amd as chl
+
+    ng, g = 4, 5.0 / 3.0
+    steps = args.steps if args.steps != 200 else 100
+    edges, mach = chl.ensemble_layout(args.seed, args.boxes)
+    dims = [(int(e) + 2 * ng,) * 3 for e in edges]
+    dxyz = np.array([[1.0 / e] * 3 for e in edges]).reshape(-1)
+    cells = int(sum(int(e) ** 3 for e in edges))
+    t0 = time.perf_counter()
+    init = []
+    for b, e in enumerate(edges):
+        U = chl.DeviceArray(5 * (int(e) + 2 * ng) ** 3)
+        chl.ic_turbulence(U, int(e), ng, g, 1.0, 1.0, float(mach[b]), args.seed * 1000 + b)
+        init.append(U)
+    log(f"ensemble: {args.boxes} boxes, {cells / 1e6:.1f} M real cells, ICs on device in "
+        f"{time.perf_counter() - t0:.1f} s; edge counts "
+        + str({int(v): int((edges == v).sum()) for v in np.unique(edges)}))
+
+    def copies():
+        out = []
+        for U in init:
+            c = chl.DeviceArray(U.n)
+            chl.check(chl.lib().chl_memcpy_dtod(c.ptr, U.ptr, U.nbytes))
+            out.append(c)
+        return out
+
+    launch = {"auto": chl.LAUNCH_SPLIT, "split": chl.LAUNCH_SPLIT, "persistent": chl.LAUNCH_PERSISTENT}[args.launch]
+    tiles = [int(x) for x in args.tiles.split(",")] if args.tiles else [-1, -1, 0, 0]
+    res, finals = {}, {}
+    for form in [f.strip() for f in args.forms.split(",") if f.strip() in ("mixed", "fp64")]:
+        prec = chl.PRECISIONS[form]
+        states = copies()
+        ens = chl.Ensemble(dims, ng, chl.RECON_PPMC, prec, launch, tile1=tiles[0], tile2=tiles[1],
+                           kchunk1=tiles[2], kchunk2=tiles[3])
+        ens.bind(states, dxyz)
+        ens.start(g)
+        if args.launch == "auto" and not args.tiles and not args.no_autotune:
+            t1 = time.perf_counter()
+            cfg = ens.autotune(steps=2)
+            log(f"[{form}] ensemble autotune: {cfg.as_dict()} ({time.perf_counter() - t1:.1f} s)")
+        cfg = ens.config().as_dict()
+        ens.advance(args.warmup)
+        chl.check(chl.lib().chl_synchronize())
+        w0 = time.perf_counter()
+        ms = ens.advance(steps)
+        wall = time.perf_counter() - w0
+        t_box, dt_box, ns = ens.clocks()
+        rate = cells * steps / max(ms * 1e-3, wall)
+        r = {"cell_updates_per_s": rate, "ms_per_step": max(ms, 1e3 * wall) / steps, "config": cfg,
+             "steps_per_box": [int(ns.min()), int(ns.max())]}
+        if form == "mixed" or "mixed" not in args.forms:
+            # the per-box loop baseline on the same kernels, continuing from the same state
+            lms = ens.advance_looped(args.looped_steps)
+            r["looped_cell_updates_per_s"] = cells * args.looped_steps / (lms * 1e-3)
+            r["looped_ms_per_step"] = lms / args.looped_steps
+            r["fused_vs_looped"] = rate / r["looped_cell_updates_per_s"]
+            # and the same number of fused steps, so both forms compare at equal step counts
+            ens.advance(args.looped_steps)
+        log(f"[{form}] ensemble {rate / 1e9:.3f} G cell-updates/s ({r['ms_per_step']:.2f} ms/step)"
+            + (f", looped {r['looped_cell_updates_per_s'] / 1e9:.3f} G ({r['looped_ms_per_step']:.2f} ms/step)"
+               if "looped_cell_updates_per_s" in r else ""))
+        res[form] = r
+        finals[form] = (states, ens)
+    l1 = None
+    if "mixed" in finals and "fp64" in finals:
+        names = ["density", "momentum_x", "momentum_y", "momentum_z", "Energy"]
+        num, den, mag = np.zeros(5), np.zeros(5), 0.0
+        for b, e in enumerate(edges):
+            n = int(e)
+            a = finals["mixed"][0][b].download().reshape(5, n + 2 * ng, n + 2 * ng, n + 2 * ng)[:, ng:-ng, ng:-ng, ng:-ng]
+            o = finals["fp64"][0][b].download().reshape(5, n + 2 * ng, n + 2 * ng, n + 2 * ng)[:, ng:-ng, ng:-ng, ng:-ng]
+            num += np.abs(a - o).reshape(5, -1).sum(1)
+            den += np.abs(o).reshape(5, -1).sum(1)
+            mag += np.sqrt(o[1] ** 2 + o[2] ** 2 + o[3] ** 2).sum()
+        l1 = {"vs": "fp64", "strict": {names[f]: float(num[f] / den[f]) for f in range(5)},
+              "momentum_scaled": {names[f]: float(num[f] / (mag if f in (1, 2, 3) else den[f])) for f in range(5)}}
+        log("ensemble mixed vs fp64 L1:", json.dumps(l1))
+    head = "mixed" if "mixed" in res else next(iter(res))
+    r = res[head]
+    out = {
+        "metric": f"aggregate cell-updates/s, batched ensemble of {args.boxes} periodic turbulence boxes, "
+                  "PPMC+HLLC, per-box CFL dt",
+        "value": r["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if head == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": f"ensemble {args.boxes} boxes, edges U{{32,48,64,96,128}}, Mach U[0.5,3], "
+                               f"k^-2 solenoidal modes 1<=|k|<=4, seed {args.seed}, form={head}",
+                   "cells": cells, "n_ghost": ng, "gamma": g},
+        "step_roofline_frac": r["cell_updates_per_s"] * BYTES_PER_CELL_UPDATE / 1e9 / HBM_PEAK_GBS,
+        "forms": res, "l1_mixed_vs_fp64": l1,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy > 0:
@@ -87,6 +192,14 @@ def main():
               "not a measured configuration.", file=sys.stderr)
         sys.exit(2)
     import cholla_amd as chl
+
+    if args.workload == "ensemble":
+        if chl.device_count() < 1:
+            log("no HIP device visible")
+            sys.exit(1)
+        log("device:", chl.device_name())
+        bench_ensemble(args)
+        return
 
     if chl.device_count() < 1:
         log("no HIP device visible")

@@ -28,8 +28,12 @@ EXPORTS = [
     "chl_hydro_destroy", "chl_hydro_vl_step", "chl_hydro_update_grid", "chl_hydro_run", "chl_hydro_time_steps",
     "chl_hydro_kernel_times", "chl_hydro_autotune", "chl_hydro_get_config", "chl_calc_dt",
     "chl_apply_temperature_floor", "chl_set_ghost_cells", "chl_set_boundaries", "chl_reconstruct",
-    "chl_hllc_fluxes", "chl_ic_kelvin_helmholtz",
+    "chl_hllc_fluxes", "chl_ic_kelvin_helmholtz", "chl_ensemble_create", "chl_ensemble_destroy",
+    "chl_ensemble_get_config", "chl_ensemble_persistent_pairs", "chl_ensemble_bind", "chl_ensemble_start",
+    "chl_ensemble_advance", "chl_ensemble_advance_looped", "chl_ensemble_clocks", "chl_ensemble_autotune",
+    "chl_ensemble_layout", "chl_turbulence_modes", "chl_ic_turbulence",
 ]
+LAUNCH_SPLIT, LAUNCH_PERSISTENT = 0, 1
 
 
 class ChollaError(RuntimeError):
@@ -44,6 +48,18 @@ class HydroConfig(ctypes.Structure):
         ("bc_flags", ctypes.c_int * 6), ("tile1", ctypes.c_int), ("tile2", ctypes.c_int),
         ("kchunk1", ctypes.c_int), ("kchunk2", ctypes.c_int),
     ]
+
+
+class EnsembleConfig(ctypes.Structure):
+    _fields_ = [
+        ("n_ghost", ctypes.c_int), ("n_fields", ctypes.c_int), ("reconstruction", ctypes.c_int),
+        ("riemann", ctypes.c_int), ("precision", ctypes.c_int), ("density_floor", ctypes.c_int),
+        ("temperature_floor", ctypes.c_int), ("launch", ctypes.c_int), ("tile1", ctypes.c_int),
+        ("tile2", ctypes.c_int), ("kchunk1", ctypes.c_int), ("kchunk2", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None
@@ -89,6 +105,19 @@ def lib():
         "chl_reconstruct": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_hllc_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_ic_kelvin_helmholtz": (_i, [_dp, _i, _i, _d, ctypes.c_uint64]),
+        "chl_ensemble_create": (_i, [ctypes.POINTER(EnsembleConfig), _i, _ip, ctypes.POINTER(_vp)]),
+        "chl_ensemble_destroy": (_i, [_vp]),
+        "chl_ensemble_get_config": (_i, [_vp, ctypes.POINTER(EnsembleConfig)]),
+        "chl_ensemble_persistent_pairs": (_i, [_ip, _i, _ip]),
+        "chl_ensemble_bind": (_i, [_vp, ctypes.POINTER(_vp), _dp]),
+        "chl_ensemble_start": (_i, [_vp, _d, _d, _d, _dp, _dp]),
+        "chl_ensemble_advance": (_i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float)]),
+        "chl_ensemble_advance_looped": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_float)]),
+        "chl_ensemble_clocks": (_i, [_vp, _dp, _dp, _ip]),
+        "chl_ensemble_autotune": (_i, [_vp, _i, ctypes.POINTER(EnsembleConfig)]),
+        "chl_ensemble_layout": (_i, [ctypes.c_uint64, _i, _ip, _dp]),
+        "chl_turbulence_modes": (_i, [ctypes.c_uint64, _dp, _i, _ip]),
+        "chl_ic_turbulence": (_i, [_vp, _i, _i, _d, _d, _d, _d, ctypes.c_uint64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -261,3 +290,109 @@ def ic_kelvin_helmholtz(n, ng, gamma=5.0 / 3.0, seed=2024):
     U = np.empty(5 * N ** 3, dtype=np.float64)
     check(lib().chl_ic_kelvin_helmholtz(U.ctypes.data_as(_dp), n, ng, gamma, seed))
     return U
+
+
+# ---------------------------------------------------------------- ensembles of boxes
+
+class Ensemble:
+    """Independent periodic boxes advanced together (chl_ensemble_* entry points).
+
+    dims: list of (nx, ny, nz) totals including ghosts.  States are DeviceArrays
+    owned by the caller and bound with bind()."""
+
+    def __init__(self, dims, n_ghost, reconstruction=RECON_PPMC, precision=PREC_MIXED, launch=LAUNCH_SPLIT,
+                 density_floor=True, temperature_floor=True, tile1=-1, tile2=-1, kchunk1=0, kchunk2=0):
+        cfg = EnsembleConfig()
+        cfg.n_ghost, cfg.n_fields, cfg.reconstruction, cfg.riemann = n_ghost, 5, reconstruction, RIEMANN_HLLC
+        cfg.precision, cfg.density_floor, cfg.temperature_floor = precision, int(density_floor), int(temperature_floor)
+        cfg.launch, cfg.tile1, cfg.tile2, cfg.kchunk1, cfg.kchunk2 = launch, tile1, tile2, kchunk1, kchunk2
+        d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1, 3))
+        self.dims = [tuple(int(v) for v in row) for row in d]
+        self.n = len(self.dims)
+        h = _vp()
+        check(lib().chl_ensemble_create(ctypes.byref(cfg), self.n, d.ctypes.data_as(_ip), ctypes.byref(h)))
+        self.h = h
+        self.n_ghost = n_ghost
+        self._states = None
+
+    def config(self):
+        c = EnsembleConfig()
+        check(lib().chl_ensemble_get_config(self.h, ctypes.byref(c)))
+        return c
+
+    def bind(self, states, dxyz):
+        self._states = list(states)  # keep the buffers alive while bound
+        ptrs = (_vp * self.n)(*[s.ptr for s in self._states])
+        dx = np.ascontiguousarray(np.asarray(dxyz, dtype=np.float64).reshape(-1))
+        assert dx.size == 3 * self.n
+        check(lib().chl_ensemble_bind(self.h, ptrs, dx.ctypes.data_as(_dp)))
+
+    def start(self, gamma, density_floor=0.0, U_floor=0.0, t0=None, tout=None):
+        arr = lambda v: None if v is None else np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+        t0a, touta = arr(t0), arr(tout)
+        check(lib().chl_ensemble_start(self.h, gamma, density_floor, U_floor,
+                                       None if t0a is None else t0a.ctypes.data_as(_dp),
+                                       None if touta is None else touta.ctypes.data_as(_dp)))
+
+    def advance(self, n_steps, use_graph=True):
+        ms = ctypes.c_float()
+        check(lib().chl_ensemble_advance(self.h, n_steps, int(use_graph), ctypes.byref(ms)))
+        return ms.value
+
+    def advance_looped(self, n_steps):
+        ms = ctypes.c_float()
+        check(lib().chl_ensemble_advance_looped(self.h, n_steps, ctypes.byref(ms)))
+        return ms.value
+
+    def clocks(self):
+        t = np.empty(self.n)
+        dt = np.empty(self.n)
+        ns = np.empty(self.n, dtype=np.int32)
+        check(lib().chl_ensemble_clocks(self.h, t.ctypes.data_as(_dp), dt.ctypes.data_as(_dp),
+                                        ns.ctypes.data_as(_ip)))
+        return t, dt, ns
+
+    def autotune(self, steps=1):
+        c = EnsembleConfig()
+        check(lib().chl_ensemble_autotune(self.h, steps, ctypes.byref(c)))
+        return c
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().chl_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def persistent_pairs():
+    n = ctypes.c_int()
+    buf = (ctypes.c_int * 64)()
+    check(lib().chl_ensemble_persistent_pairs(buf, 32, ctypes.byref(n)))
+    return [(buf[2 * i], buf[2 * i + 1]) for i in range(n.value)]
+
+
+def ensemble_layout(seed, n_boxes):
+    """Edges and rms Mach numbers of the ensemble boxes (chl_ensemble_layout)."""
+    edges = np.empty(n_boxes, dtype=np.int32)
+    mach = np.empty(n_boxes)
+    check(lib().chl_ensemble_layout(seed, n_boxes, edges.ctypes.data_as(_ip), mach.ctypes.data_as(_dp)))
+    return edges, mach
+
+
+def turbulence_modes(seed):
+    """(m, 9) array: k, a, b of every mode of the turbulent velocity field."""
+    n = ctypes.c_int()
+    check(lib().chl_turbulence_modes(seed, None, 0, ctypes.byref(n)))
+    out = np.empty((n.value, 9))
+    check(lib().chl_turbulence_modes(seed, out.ctypes.data_as(_dp), n.value, ctypes.byref(n)))
+    return out
+
+
+def ic_turbulence(U, n, ng, gamma=5.0 / 3.0, rho=1.0, p=1.0, mach=1.0, seed=1):
+    """Turbulence box on the device (U: DeviceArray of 5*(n+2ng)^3)."""
+    check(lib().chl_ic_turbulence(U.ptr, n, ng, gamma, rho, p, mach, seed))

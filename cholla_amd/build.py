@@ -4,8 +4,8 @@
 
 Every HIP translation unit is compiled for gfx950 with packed-FP32 instructions
 disabled.  The FP64 reference-order kernels (hydro_exact.hip, support_kernels.hip)
-are built with -ffp-contract=off; the fast FP64 and mixed-precision kernels
-(hydro_fast.hip) allow FMA contraction.
+(and ensemble_exact.hip) are built with -ffp-contract=off; the fast FP64 and
+mixed-precision kernels (hydro_fast.hip, ensemble_fast.hip) allow FMA contraction.
 """
 import os
 import shutil
@@ -29,7 +29,10 @@ UNITS = [
     ("hydro_exact.hip", ["-ffp-contract=off"]),
     ("support_kernels.hip", ["-ffp-contract=off"]),
     ("hydro_fast.hip", ["-ffp-contract=fast"]),
+    ("ensemble_exact.hip", ["-ffp-contract=off"]),
+    ("ensemble_fast.hip", ["-ffp-contract=fast"]),
     ("capi.cpp", ["-ffp-contract=off"]),
+    ("ensemble.cpp", ["-ffp-contract=off"]),
 ]
 # Host programs built on the reference-signature adapter (cholla_dropin.cpp), one per
 # reconstruction build option, as Cholla's make.type files select them.  They need no HIP

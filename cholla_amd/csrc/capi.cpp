@@ -11,35 +11,28 @@
 #include <vector>
 
 #include "../../include/cholla_amd.h"
+#include "capi_internal.hpp"
 #include "support_kernels.hpp"
 #include "vl3d_launch.hpp"
 
+namespace chl_capi
+{
 namespace
 {
 thread_local std::string g_err;
-
+}
 int Fail(int code, const std::string &msg)
 {
   g_err = msg;
   return code;
 }
+}  // namespace chl_capi
 
-#define CHL_HIP(expr)                                                                                \
-  do {                                                                                               \
-    hipError_t e_ = (expr);                                                                          \
-    if (e_ != hipSuccess) return Fail(CHL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-#define CHL_TRY(expr)       \
-  do {                      \
-    int rc_ = (expr);       \
-    if (rc_ != CHL_OK) return rc_; \
-  } while (0)
-
-constexpr double kCfl = 0.3;  // C_cfl, src/grid/grid3D.cpp:157
-
-int ReconRadius(int recon) { return recon == CHL_RECON_PPMC ? 2 : (recon == CHL_RECON_PLMC ? 1 : 0); }
-
+namespace
+{
+using chl_capi::Fail;
+using chl_capi::kCfl;
+using chl_capi::ReconRadius;
 }  // namespace
 
 struct chl_hydro {
@@ -216,7 +209,7 @@ int EnsureGraph(chl_hydro *h, double *U, double dx, double dy, double dz, double
 extern "C" {
 
 const char *chl_version(void) { return "cholla_amd 0.1 (gfx950)"; }
-const char *chl_last_error(void) { return g_err.c_str(); }
+const char *chl_last_error(void) { return chl_capi::g_err.c_str(); }
 
 int chl_device_count(int *n)
 {

@@ -1,0 +1,228 @@
+// ensemble_kernels.hpp -- one VL step over an ensemble of independent periodic boxes.
+//
+// The reference advances one box per process and launches ~20 kernels per step
+// for it (src/integrators/VL_3D_cuda.cu:38-360).  An ensemble of small boxes
+// launched that way leaves most of the GPU idle: a 40^3 box is a few workgroups.
+// Here every box's tiles go into one work list and a step is:
+//
+//   EnsembleClockKernel      per-box dt = C_cfl / dti (clamped to the box's output
+//                            time), t += dt, dti reset (the reference's set_dt /
+//                            Update_Time, src/grid/grid3D.cpp:373-411, 587-600)
+//   then either
+//   EnsembleStage1Kernel +   one launch per stage over the tiles of all boxes
+//   EnsembleStage2Kernel     ("split" launch)
+//   or
+//   EnsemblePersistentKernel one launch per step: resident workgroups pull items
+//                            from a queue holding every box's stage-1 tiles and,
+//                            one box behind, its stage-2 tiles; a stage-2 tile
+//                            waits until its box's stage-1 tiles are all done
+//                            ("persistent" launch).
+//
+// Each box keeps its own dt: the tile bodies are the single-box ones
+// (Stage1Tile / Stage2Tile) fed with that box's StepArgs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "vl3d_kernels.hpp"
+
+namespace chl
+{
+
+// One box of the ensemble, resident on the device.
+struct BoxDesc {
+  double *U;           // conserved state, 5 x cells (in/out, owned by the caller)
+  void *W;             // half-step primitives of this box
+  int nx, ny, nz;      // totals, ghosts included
+  int t1[3], t2[3];    // tile counts (x, y, z-chunks) of stage 1 and stage 2
+  int kchunk1, kchunk2;
+  int items1;          // t1[0] * t1[1] * t1[2]
+  int pad_;
+  double dx, dy, dz;
+  double tout;         // output time of the box (dt is clamped to reach it)
+};
+
+// Time state of one box.
+struct BoxClock {
+  double dt, t;
+  unsigned long long dti;  // bits of the non-negative inverse dt (atomicMax target)
+  int nstep;
+  int done1;  // stage-1 tiles finished in the current step (persistent launch)
+};
+
+// A work item: box and the tile's linear index l = bx + tx * (by + ty * bz).
+struct WorkItem {
+  int box;
+  int stage;  // 0 or 1 (persistent queue only)
+  int local;
+  int pad_;
+};
+
+struct EnsembleArgs {
+  const BoxDesc *box;
+  BoxClock *clk;
+  const WorkItem *items;  // stage-1 list, stage-2 list or the persistent queue
+  int n_items;
+  int n_boxes;
+  int ng;
+  int R1;  // stage-1 region growth: reconstruction radius + 1
+  double gamma, density_floor, U_floor;
+  int flags;
+  unsigned int *head;  // persistent queue head
+  int *status;         // set non-zero if a dependency wait gave up
+  long long spin_limit;
+};
+
+__device__ __forceinline__ StepArgs BoxStepArgs(const EnsembleArgs &e, int b, int kchunk)
+{
+  const BoxDesc &d = e.box[b];
+  StepArgs a;
+  a.U = d.U, a.W = d.W;
+  a.nx = d.nx, a.ny = d.ny, a.nz = d.nz, a.ng = e.ng;
+  a.dx = d.dx, a.dy = d.dy, a.dz = d.dz;
+  a.gamma         = e.gamma;
+  a.dt            = 0.0;
+  a.dt_dev        = &e.clk[b].dt;
+  a.dti           = &e.clk[b].dti;
+  a.density_floor = e.density_floor;
+  a.U_floor       = e.U_floor;
+  a.flags         = e.flags;
+  a.kchunk        = kchunk;
+  return a;
+}
+
+__device__ __forceinline__ void TileCoords(int l, const int t[3], int &bx, int &by, int &bz)
+{
+  bx = l % t[0];
+  int r = l / t[0];
+  by = r % t[1];
+  bz = r / t[1];
+}
+
+// Workgroups are handed to the 8 XCDs round-robin by id; this gives each XCD a
+// contiguous run of work items, so neighbouring tiles (which share halo cells)
+// are resident in the same L2.  The grid is padded to a multiple of 8.
+__device__ __forceinline__ int XcdItem(int g, int n_items)
+{
+  const int per = (n_items + 7) / 8;
+  return (g & 7) * per + (g >> 3);
+}
+
+// Per-box dt from the previous step's reduction, as DtAdvanceKernel does for one box.
+__global__ void EnsembleClockKernel(BoxClock *clk, const BoxDesc *box, int n_boxes, double cfl, unsigned int *head);
+
+template <class P, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) EnsembleStage1Kernel(EnsembleArgs e)
+{
+  __shared__ Stage1Smem<P, TX, TY> sm;
+  const int q = XcdItem(blockIdx.x, e.n_items);
+  if (q >= e.n_items) return;
+  const WorkItem it = e.items[q];
+  const BoxDesc &d  = e.box[it.box];
+  int bx, by, bz;
+  TileCoords(it.local, d.t1, bx, by, bz);
+  StepArgs a = BoxStepArgs(e, it.box, d.kchunk1);
+  Stage1Tile<P, TX, TY>(a, e.R1, bx, by, bz, sm);
+}
+
+template <class P, int RK, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Stage2Occupancy<P>::kMinWaves)))
+EnsembleStage2Kernel(EnsembleArgs e)
+{
+  __shared__ Stage2Smem<P, RK, TX, TY> sm;
+  const int q = XcdItem(blockIdx.x, e.n_items);
+  if (q >= e.n_items) return;
+  const WorkItem it = e.items[q];
+  const BoxDesc &d  = e.box[it.box];
+  int bx, by, bz;
+  TileCoords(it.local, d.t2, bx, by, bz);
+  StepArgs a = BoxStepArgs(e, it.box, d.kchunk2);
+  Stage2Tile<P, RK, TX, TY>(a, bx, by, bz, sm);
+}
+
+template <class A, class B>
+union SmemPair {
+  A s1;
+  B s2;
+};
+
+// One whole step in one launch.  Deadlock freedom: items are taken from the queue
+// in order, and every stage-2 item of box b sits after all stage-1 items of b, so
+// when a workgroup waits on box b, each stage-1 item of b is already held by a
+// running workgroup, and stage-1 items never wait.  The wait is still bounded
+// (spin_limit sleeps); if it gives up, *status is set and the host reports it.
+//
+// Cross-XCD visibility: the W writes of a stage-1 tile are released at device
+// (agent) scope by every lane before the tile is counted, and a stage-2 tile
+// acquires at the same scope after it has seen the count.
+// The persistent kernel holds both stage bodies; left to itself the register
+// allocator sizes it for two waves per SIMD, one fewer than the mixed form's
+// stage kernels run at.
+template <class P>
+struct PersistentOccupancy {
+  static constexpr int kMinWaves = P::kId == PolicyMixed::kId ? 3 : 2;
+};
+
+template <class P, int RK, int TX1, int TY1, int TX2, int TY2>
+__global__ void __launch_bounds__(TX2 *TY2) __attribute__((amdgpu_waves_per_eu(PersistentOccupancy<P>::kMinWaves)))
+EnsemblePersistentKernel(EnsembleArgs e)
+{
+  static_assert(TX1 * TY1 == TX2 * TY2, "both stages run on the same workgroup");
+  __shared__ SmemPair<Stage1Smem<P, TX1, TY1>, Stage2Smem<P, RK, TX2, TY2>> sm;
+  __shared__ int s_item;
+  const int tid = threadIdx.x;
+  for (;;) {
+    __syncthreads();  // the previous item is finished with LDS and s_item
+    if (tid == 0) s_item = int(atomicAdd(e.head, 1u));
+    __syncthreads();
+    const int q = __builtin_amdgcn_readfirstlane(s_item);
+    if (q >= e.n_items) break;
+    const WorkItem it = e.items[q];
+    const int b       = __builtin_amdgcn_readfirstlane(it.box);
+    const BoxDesc &d  = e.box[b];
+    int bx, by, bz;
+    if (__builtin_amdgcn_readfirstlane(it.stage) == 0) {
+      TileCoords(it.local, d.t1, bx, by, bz);
+      StepArgs a = BoxStepArgs(e, b, d.kchunk1);
+      Stage1Tile<P, TX1, TY1>(a, e.R1, bx, by, bz, sm.s1);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this lane's W stores, device-wide
+      __syncthreads();
+      if (tid == 0) atomicAdd(&e.clk[b].done1, 1);
+    } else {
+      if (tid == 0) {
+        const int need = d.items1;
+        long long n    = 0;
+        while (__hip_atomic_load(&e.clk[b].done1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++n > e.spin_limit) {
+            atomicOr(e.status, 1);
+            break;
+          }
+          // once one wait has given up, the others stop waiting too
+          if ((n & 1023) == 0 && __hip_atomic_load(e.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        }
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the box's W as its stage-1 tiles left it
+      TileCoords(it.local, d.t2, bx, by, bz);
+      StepArgs a = BoxStepArgs(e, b, d.kchunk2);
+      Stage2Tile<P, RK, TX2, TY2>(a, bx, by, bz, sm.s2);
+    }
+  }
+}
+
+// Tile pairs (stage-1 tile, stage-2 tile; indices into kTiles) compiled for the
+// persistent launch.  Both stages of a pair use 256 lanes.
+constexpr int kNumPairs           = 4;
+constexpr int kPairs[kNumPairs][2] = {{2, 1}, {2, 0}, {0, 0}, {1, 1}};
+
+// Host launchers, instantiated per policy in ensemble_exact.hip / ensemble_fast.hip.
+hipError_t LaunchEnsembleStage1(int policy, const EnsembleArgs &e, int tile, hipStream_t s);
+hipError_t LaunchEnsembleStage2(int policy, const EnsembleArgs &e, int recon, int tile, hipStream_t s);
+hipError_t LaunchEnsemblePersistent(int policy, const EnsembleArgs &e, int recon, int pair, int grid, hipStream_t s);
+// Resident workgroups per CU of the persistent kernel (the grid is that times the CU count).
+hipError_t EnsemblePersistentSlots(int policy, int recon, int pair, int *per_cu);
+hipError_t LaunchEnsembleClock(BoxClock *clk, const BoxDesc *box, int n_boxes, double cfl, unsigned int *head,
+                               hipStream_t s);
+
+}  // namespace chl

@@ -1,0 +1,119 @@
+// ensemble_launch.hpp -- host dispatch of the ensemble kernels over tile shapes.
+#pragma once
+
+#include "ensemble_kernels.hpp"
+#include "vl3d_launch.hpp"
+
+namespace chl
+{
+
+inline int PaddedItems(int n) { return (n + 7) / 8 * 8; }
+
+template <class P, int TX, int TY>
+hipError_t EnsStage1T(const EnsembleArgs &e, hipStream_t s)
+{
+  if (e.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL((EnsembleStage1Kernel<P, TX, TY>), dim3(PaddedItems(e.n_items)), dim3(TX * TY), 0, s, e);
+  return hipGetLastError();
+}
+
+template <class P, int RK, int TX, int TY>
+hipError_t EnsStage2T(const EnsembleArgs &e, hipStream_t s)
+{
+  if (e.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL((EnsembleStage2Kernel<P, RK, TX, TY>), dim3(PaddedItems(e.n_items)), dim3(TX * TY), 0, s, e);
+  return hipGetLastError();
+}
+
+template <class P>
+hipError_t EnsStage1(const EnsembleArgs &e, int tile, hipStream_t s)
+{
+  switch (tile) {
+    case 0:
+      return EnsStage1T<P, 32, 8>(e, s);
+    case 1:
+      return EnsStage1T<P, 16, 16>(e, s);
+    case 2:
+      return EnsStage1T<P, 64, 4>(e, s);
+    case 3:
+      return EnsStage1T<P, 16, 8>(e, s);
+    case 4:
+      return EnsStage1T<P, 32, 16>(e, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class P, int RK>
+hipError_t EnsStage2R(const EnsembleArgs &e, int tile, hipStream_t s)
+{
+  switch (tile) {
+    case 0:
+      return EnsStage2T<P, RK, 32, 8>(e, s);
+    case 1:
+      return EnsStage2T<P, RK, 16, 16>(e, s);
+    case 2:
+      return EnsStage2T<P, RK, 64, 4>(e, s);
+    case 3:
+      return EnsStage2T<P, RK, 16, 8>(e, s);
+    case 4:
+      return EnsStage2T<P, RK, 32, 16>(e, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class P>
+hipError_t EnsStage2(const EnsembleArgs &e, int recon, int tile, hipStream_t s)
+{
+  if (recon == kPPMC) return EnsStage2R<P, kPPMC>(e, tile, s);
+  if (recon == kPLMC) return EnsStage2R<P, kPLMC>(e, tile, s);
+  return hipErrorInvalidValue;
+}
+
+template <class P, int RK, int A, int B>
+struct PersistentPair {
+  static constexpr int TX1 = kTiles[A].tx, TY1 = kTiles[A].ty, TX2 = kTiles[B].tx, TY2 = kTiles[B].ty;
+  static constexpr int NT  = TX2 * TY2;
+  static hipError_t Launch(const EnsembleArgs &e, int grid, hipStream_t s)
+  {
+    if (e.n_items == 0) return hipSuccess;
+    hipLaunchKernelGGL((EnsemblePersistentKernel<P, RK, TX1, TY1, TX2, TY2>), dim3(grid), dim3(NT), 0, s, e);
+    return hipGetLastError();
+  }
+  static hipError_t Slots(int *per_cu)
+  {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, EnsemblePersistentKernel<P, RK, TX1, TY1, TX2, TY2>,
+                                                        NT, 0);
+  }
+};
+
+template <class P, int RK>
+hipError_t EnsPersistentR(const EnsembleArgs &e, int pair, int grid, int *per_cu, hipStream_t s)
+{
+  static_assert(kNumPairs == 4, "pair table and dispatch below go together");
+  switch (pair) {
+    case 0:
+      return per_cu ? PersistentPair<P, RK, kPairs[0][0], kPairs[0][1]>::Slots(per_cu)
+                    : PersistentPair<P, RK, kPairs[0][0], kPairs[0][1]>::Launch(e, grid, s);
+    case 1:
+      return per_cu ? PersistentPair<P, RK, kPairs[1][0], kPairs[1][1]>::Slots(per_cu)
+                    : PersistentPair<P, RK, kPairs[1][0], kPairs[1][1]>::Launch(e, grid, s);
+    case 2:
+      return per_cu ? PersistentPair<P, RK, kPairs[2][0], kPairs[2][1]>::Slots(per_cu)
+                    : PersistentPair<P, RK, kPairs[2][0], kPairs[2][1]>::Launch(e, grid, s);
+    case 3:
+      return per_cu ? PersistentPair<P, RK, kPairs[3][0], kPairs[3][1]>::Slots(per_cu)
+                    : PersistentPair<P, RK, kPairs[3][0], kPairs[3][1]>::Launch(e, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// per_cu != null: occupancy query instead of a launch.
+template <class P>
+hipError_t EnsPersistent(const EnsembleArgs &e, int recon, int pair, int grid, int *per_cu, hipStream_t s)
+{
+  if (recon == kPPMC) return EnsPersistentR<P, kPPMC>(e, pair, grid, per_cu, s);
+  if (recon == kPLMC) return EnsPersistentR<P, kPLMC>(e, pair, grid, per_cu, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace chl

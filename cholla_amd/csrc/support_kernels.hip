@@ -7,6 +7,7 @@
 
 #include <cstdint>
 
+#include "ensemble_kernels.hpp"
 #include "hydro_physics.hpp"
 #include "support_kernels.hpp"
 #include "vl3d_kernels.hpp"
@@ -274,6 +275,61 @@ __global__ void DtAdvanceKernel(const unsigned long long *dti_bits, double *dt_d
   if (dt > 0.0) *n_step += 1;
 }
 
+// One thread per box: the same dt and clock arithmetic as DtAdvanceKernel, then the
+// accumulators of the coming step are cleared.  Thread 0 also rewinds the queue of
+// the persistent launch.
+__global__ void EnsembleClockKernel(BoxClock *clk, const BoxDesc *box, int n_boxes, double cfl, unsigned int *head)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0 && head) *head = 0u;
+  if (b >= n_boxes) return;
+  BoxClock c       = clk[b];
+  const double tout = box[b].tout;
+  double dti       = __longlong_as_double((long long)c.dti);
+  double dt        = cfl / dti;
+  if (c.t + dt > tout) dt = tout - c.t;
+  if (!(c.t < tout)) dt = 0.0;
+  c.dt = dt;
+  c.t  = c.t + dt;
+  if (dt > 0.0) c.nstep += 1;
+  c.dti   = 0ull;
+  c.done1 = 0;
+  clk[b]  = c;
+}
+
+// ============================================================================ turbulence IC
+// Velocity field sum_m [a_m cos(2 pi k_m.x) + b_m sin(2 pi k_m.x)] * vscale at the cell
+// centres x = (i - ng + 1/2) / n of a periodic unit box (positions as Grid3D::Get_Position,
+// src/grid/grid3D.cpp:84-86); ghost cells take their periodic image's value.  Density and
+// pressure uniform; energy as Calc_Energy_Primitive (src/utils/hydro_utilities.h:106-118).
+// modes: 9 doubles per mode (kx, ky, kz, ax, ay, az, bx, by, bz).
+__global__ void TurbulenceIcKernel(double *U, int n, int ng, int n_modes, const double *modes, double vscale,
+                                   double rho, double p, double gamma)
+{
+  const int N     = n + 2 * ng;
+  const size_t nc = size_t(N) * N * N;
+  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= nc) return;
+  const int i = int(id % N), j = int((id / N) % N), k = int(id / (size_t(N) * N));
+  auto wrap   = [&](int a) { return a < ng ? a + n : (a >= N - ng ? a - n : a); };
+  const double x = (wrap(i) - ng + 0.5) / n, y = (wrap(j) - ng + 0.5) / n, z = (wrap(k) - ng + 0.5) / n;
+  double vx = 0, vy = 0, vz = 0;
+  for (int m = 0; m < n_modes; m++) {
+    const double *md = modes + 9 * m;
+    double sn, cs;
+    sincospi(2.0 * (md[0] * x + md[1] * y + md[2] * z), &sn, &cs);
+    vx += md[3] * cs + md[6] * sn;
+    vy += md[4] * cs + md[7] * sn;
+    vz += md[5] * cs + md[8] * sn;
+  }
+  vx *= vscale, vy *= vscale, vz *= vscale;
+  U[id]          = rho;
+  U[nc + id]     = rho * vx;
+  U[2 * nc + id] = rho * vy;
+  U[3 * nc + id] = rho * vz;
+  U[4 * nc + id] = (fmax(p, kTinyNumber) / (gamma - 1.)) + 0.5 * rho * (vx * vx + ((vy * vy) + (vz * vz)));
+}
+
 // ============================================================================ VL 1D
 // Stage 1 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:64-91): PCM states, HLLC on
 // faces id-1/2 and id+1/2 of every cell 1..n-2, the dt/2 update, then the half
@@ -501,6 +557,23 @@ hipError_t LaunchDtAdvance(const unsigned long long *dti, double *dt_dev, double
                            int *n_step, hipStream_t s)
 {
   hipLaunchKernelGGL(DtAdvanceKernel, dim3(1), dim3(1), 0, s, dti, dt_dev, t_dev, tout, cfl, n_step);
+  return hipGetLastError();
+}
+
+hipError_t LaunchEnsembleClock(BoxClock *clk, const BoxDesc *box, int n_boxes, double cfl, unsigned int *head,
+                               hipStream_t s)
+{
+  hipLaunchKernelGGL(EnsembleClockKernel, dim3(Blocks(std::max(n_boxes, 1), 256)), dim3(256), 0, s, clk, box, n_boxes,
+                     cfl, head);
+  return hipGetLastError();
+}
+
+hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const double *modes, double vscale, double rho,
+                              double p, double gamma, hipStream_t s)
+{
+  const long N = n + 2 * ng;
+  hipLaunchKernelGGL(TurbulenceIcKernel, dim3(Blocks(N * N * N, 256)), dim3(256), 0, s, U, n, ng, n_modes, modes,
+                     vscale, rho, p, gamma);
   return hipGetLastError();
 }
 

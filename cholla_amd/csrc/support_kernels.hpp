@@ -18,6 +18,8 @@ hipError_t LaunchCalcDt(const double *U, int nx, int ny, int nz, int ng, double 
 hipError_t LaunchTemperatureFloor(double *U, int nx, int ny, int nz, int ng, double U_floor, hipStream_t s);
 hipError_t LaunchDtAdvance(const unsigned long long *dti, double *dt_dev, double *t_dev, double tout, double cfl,
                            int *n_step, hipStream_t s);
+hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const double *modes, double vscale, double rho,
+                              double p, double gamma, hipStream_t s);
 hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, double dx, const double *dt_dev, double dt,
                       double gamma, unsigned long long *dti, hipStream_t s);
 hipError_t LaunchReconGrid(int recon, const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma,

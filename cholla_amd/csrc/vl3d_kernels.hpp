@@ -157,18 +157,28 @@ __device__ __forceinline__ void FluxGlobal(const Cons<typename P::T> &f, typenam
   out[4] = f.E;
 }
 
-// Stage 1: region of half-step cells [lo, hi) per axis with lo = ng - R1, hi = n - ng + R1.
+// LDS of one stage-1 tile.
 template <class P, int TX, int TY>
-__global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
+struct Stage1Smem {
+  using T = typename P::T;
+  PcmCell<T> plane[TY + 2][TX + 2];  // staged plane with a 1-cell halo
+  T fx[5][TY][TX + 1];
+  T fy[5][TY + 1][TX];
+};
+
+// Stage 1 of one tile (bx, by) x z-chunk bz of the half-step region [lo, hi) per
+// axis, lo = ng - R1, hi = n - ng + R1.  Called by every lane of a TX*TY workgroup.
+template <class P, int TX, int TY>
+__device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, int by, int bz,
+                                           Stage1Smem<P, TX, TY> &sm)
 {
   using T            = typename P::T;
   constexpr int NT   = TX * TY;
-  constexpr int PX   = TX + 2, PY = TY + 2;  // staged plane with a 1-cell halo
   constexpr int NFX  = (TX + 1) * TY;
   constexpr int NFY  = TX * (TY + 1);
-  __shared__ PcmCell<T> plane[PY][PX];
-  __shared__ T fx[5][TY][TX + 1];
-  __shared__ T fy[5][TY + 1][TX];
+  auto &plane        = sm.plane;
+  auto &fx           = sm.fx;
+  auto &fy           = sm.fy;
 
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t nc = size_t(nx) * ny * nz;
@@ -177,9 +187,9 @@ __global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
   const int lo_z = a.ng - R1, hi_z = nz - a.ng + R1;
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const int i0 = lo_x + blockIdx.x * TX, j0 = lo_y + blockIdx.y * TY;
+  const int i0 = lo_x + bx * TX, j0 = lo_y + by * TY;
   const int i = i0 + tx, j = j0 + ty;
-  const int k0 = lo_z + blockIdx.z * a.kchunk;
+  const int k0 = lo_z + bz * a.kchunk;
   const int k1 = min(k0 + a.kchunk, hi_z);
   const bool own = (i < hi_x) && (j < hi_y);
   const T gamma  = T(a.gamma);
@@ -188,9 +198,7 @@ __global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
   const double *U    = a.U;
   T *W               = reinterpret_cast<T *>(a.W);
 
-  // the inverse-dt accumulator is reset here; stage 2 (next launch) fills it
-  if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid == 0) *a.dti = 0ull;
-  if (dt == 0.0) return;  // a device-timed step past the output time (uniform per launch)
+  if (dt == 0.0) return;  // a device-timed step past the output time (uniform per box)
 
   // Lanes past the region's end still stage their own cell: it is the right
   // neighbour of the last owned cell.  Only the index is clamped, to stay in memory.
@@ -285,6 +293,16 @@ __global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
   }
 }
 
+template <class P, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
+{
+  __shared__ Stage1Smem<P, TX, TY> sm;
+  // the inverse-dt accumulator is reset here; stage 2 (next launch) fills it
+  if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0)
+    *a.dti = 0ull;
+  Stage1Tile<P, TX, TY>(a, R1, blockIdx.x, blockIdx.y, blockIdx.z, sm);
+}
+
 // ============================================================================ stage 2
 // Half-step cell as stored in W.  The fifth slot holds the pressure, except under
 // PolicyExact where it holds the energy: Load_Data recovers pressure from the
@@ -340,9 +358,22 @@ struct Stage2Occupancy {
   static constexpr int kMinWaves = P::kId == PolicyMixed::kId ? 1 : 2;
 };
 
+// LDS of one stage-2 tile.
 template <class P, int RK, int TX, int TY>
-__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Stage2Occupancy<P>::kMinWaves)))
-Stage2Kernel(StepArgs a)
+struct Stage2Smem {
+  using T = typename P::T;
+  using S = Stage2Shape<P, RK, TX, TY>;
+  WCell<T> plane[S::PY][S::PX];
+  Prim<T> qL[S::NQ];
+  Prim<T> qR[S::NQ];
+  T fl[5][S::NF];
+  double red[TX * TY / 64];
+};
+
+// Stage 2 of one tile (bx, by) x z-chunk bz of the real cells.  Called by every
+// lane of a TX*TY workgroup; the block's inverse-dt maximum goes to *a.dti.
+template <class P, int RK, int TX, int TY>
+__device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, int bz, Stage2Smem<P, RK, TX, TY> &sm)
 {
   using T          = typename P::T;
   using S          = Stage2Shape<P, RK, TX, TY>;
@@ -350,19 +381,19 @@ Stage2Kernel(StepArgs a)
   constexpr int r  = ReconTraits<RK>::kRadius;
   constexpr int NS = 2 * r + 1;
   constexpr int H  = S::H;
-  __shared__ WCell<T> plane[S::PY][S::PX];
-  __shared__ Prim<T> qL[S::NQ];
-  __shared__ Prim<T> qR[S::NQ];
-  __shared__ T fl[5][S::NF];
-  __shared__ double red[NT / 64];
+  auto &plane      = sm.plane;
+  auto &qL         = sm.qL;
+  auto &qR         = sm.qR;
+  auto &fl         = sm.fl;
+  auto &red        = sm.red;
 
   const int nx = a.nx, ny = a.ny, nz = a.nz, ng = a.ng;
   const size_t nc = size_t(nx) * ny * nz;
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const int i0 = ng + blockIdx.x * TX, j0 = ng + blockIdx.y * TY;
+  const int i0 = ng + bx * TX, j0 = ng + by * TY;
   const int i = i0 + tx, j = j0 + ty;
-  const int k0 = ng + blockIdx.z * a.kchunk;
+  const int k0 = ng + bz * a.kchunk;
   const int k1 = min(k0 + a.kchunk, nz - ng);
   const bool own = (i < nx - ng) && (j < ny - ng);
   // Lanes past the last real cell still stage their own column of W: those
@@ -375,7 +406,7 @@ Stage2Kernel(StepArgs a)
   const T *W         = reinterpret_cast<const T *>(a.W);
   double *U          = a.U;
   const double inv_dx = 1.0 / a.dx, inv_dy = 1.0 / a.dy, inv_dz = 1.0 / a.dz;
-  if (dt == 0.0) return;  // a device-timed step past the output time (uniform per launch)
+  if (dt == 0.0) return;  // a device-timed step past the output time (uniform per box)
 
   auto loadW = [&](int ii, int jj, int kk) {
     size_t id = Lin(ii, jj, kk, nx, ny);
@@ -523,18 +554,21 @@ Stage2Kernel(StepArgs a)
         // every periodic image of this real cell inside the ghost layers (the host
         // enables this only when every axis holds >= ng real cells, so each image
         // lands in a ghost layer and maps back to this cell)
+        // Per axis: the cell itself, then up to two images (+L, -L).  Selects instead
+        // of small arrays keep the coordinates in registers.
         const int Lx = nx - 2 * ng, Ly = ny - 2 * ng, Lz = nz - 2 * ng;
-        int xs[3] = {i, i, i}, ys[3] = {j, j, j}, zs[3] = {k, k, k};
-        int nxi = 1, nyi = 1, nzi = 1;
-        if (i < 2 * ng) xs[nxi++] = i + Lx;
-        if (i >= nx - 2 * ng) xs[nxi++] = i - Lx;
-        if (j < 2 * ng) ys[nyi++] = j + Ly;
-        if (j >= ny - 2 * ng) ys[nyi++] = j - Ly;
-        if (k < 2 * ng) zs[nzi++] = k + Lz;
-        if (k >= nz - 2 * ng) zs[nzi++] = k - Lz;
+        const bool xp = i < 2 * ng, xm = i >= nx - 2 * ng;
+        const bool yp = j < 2 * ng, ym = j >= ny - 2 * ng;
+        const bool zp = k < 2 * ng, zm = k >= nz - 2 * ng;
+        const int x1 = xp ? i + Lx : i - Lx, y1 = yp ? j + Ly : j - Ly, z1 = zp ? k + Lz : k - Lz;
+        const int x2 = i - Lx, y2 = j - Ly, z2 = k - Lz;  // second image only when both apply
+        const int nxi = 1 + xp + xm, nyi = 1 + yp + ym, nzi = 1 + zp + zm;
         for (int c = 1; c < nxi * nyi * nzi; c++) {
           int ax = c % nxi, ay = (c / nxi) % nyi, az = c / (nxi * nyi);
-          size_t g = Lin(xs[ax], ys[ay], zs[az], nx, ny);
+          const int gx = ax == 0 ? i : (ax == 1 ? x1 : x2);
+          const int gy = ay == 0 ? j : (ay == 1 ? y1 : y2);
+          const int gz = az == 0 ? k : (az == 1 ? z1 : z2);
+          size_t g = Lin(gx, gy, gz, nx, ny);
 #pragma unroll
           for (int q = 0; q < 5; q++) U[q * nc + g] = un[q];
         }
@@ -556,6 +590,14 @@ Stage2Kernel(StepArgs a)
       atomicMax(a.dti, (unsigned long long)__double_as_longlong(m));
     }
   }
+}
+
+template <class P, int RK, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Stage2Occupancy<P>::kMinWaves)))
+Stage2Kernel(StepArgs a)
+{
+  __shared__ Stage2Smem<P, RK, TX, TY> sm;
+  Stage2Tile<P, RK, TX, TY>(a, blockIdx.x, blockIdx.y, blockIdx.z, sm);
 }
 
 }  // namespace chl

@@ -144,12 +144,80 @@ int chl_reconstruct(int reconstruction, const double *dev_conserved, double *dev
 int chl_hllc_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, double *dev_flux, int nx, int ny, int nz,
                     double gamma, int dir);
 
+/* ---- ensembles of independent periodic boxes.  The reference runs one box per
+ * process (src/main.cpp:247-379); this advances many boxes of different sizes
+ * together, each with its own CFL dt and clock, with the step of every box fused
+ * into one launch per stage ("split") or one launch per step ("persistent"). */
+#define CHL_LAUNCH_SPLIT 0
+#define CHL_LAUNCH_PERSISTENT 1
+
+typedef struct chl_ensemble_config {
+  int n_ghost;           /* as chl_hydro_config; every box shares it */
+  int n_fields;          /* 5 */
+  int reconstruction;    /* CHL_RECON_PLMC or CHL_RECON_PPMC */
+  int riemann;           /* CHL_RIEMANN_HLLC */
+  int precision;         /* CHL_PREC_* */
+  int density_floor;     /* 0/1 */
+  int temperature_floor; /* 0/1 */
+  int launch;            /* CHL_LAUNCH_SPLIT or CHL_LAUNCH_PERSISTENT */
+  int tile1, tile2;      /* stage tile shapes, -1 = default (the persistent launch
+                            takes the pairs listed by chl_ensemble_persistent_pairs) */
+  int kchunk1, kchunk2;  /* z cells per work item, 0 = default */
+} chl_ensemble_config;
+
+typedef struct chl_ensemble chl_ensemble; /* owns the scratch and work lists of the boxes */
+
+/* dims: 3*n_boxes totals (nx, ny, nz incl. ghosts) per box; every box is periodic
+ * on all faces and needs at least n_ghost real cells per axis. */
+int chl_ensemble_create(const chl_ensemble_config *cfg, int n_boxes, const int *dims, chl_ensemble **out);
+int chl_ensemble_destroy(chl_ensemble *e);
+int chl_ensemble_get_config(const chl_ensemble *e, chl_ensemble_config *out);
+/* The (tile1, tile2) pairs compiled for the persistent launch: pairs[2*i], pairs[2*i+1]. */
+int chl_ensemble_persistent_pairs(int *pairs, int max_pairs, int *n_pairs);
+/* Borrow the boxes' device states (one pointer per box, Cholla layout) and their
+ * cell widths (dxyz: dx, dy, dz per box). */
+int chl_ensemble_bind(chl_ensemble *e, double *const *d_states, const double *dxyz);
+/* Start the clocks: t0 and tout per box (null: 0 and +inf), initial inverse dt
+ * from each box's state (Calc_dt_GPU).  States must hold valid ghost cells. */
+int chl_ensemble_start(chl_ensemble *e, double gamma, double density_floor, double U_floor, const double *t0,
+                       const double *tout);
+/* Advance every box n_steps steps (a box at its tout stays put), one hipGraph
+ * replay per step with use_graph != 0; *ms_out (may be null) is the HIP-event
+ * time of the n_steps steps on the library's stream. */
+int chl_ensemble_advance(chl_ensemble *e, int n_steps, int use_graph, float *ms_out);
+/* The same steps as a loop over boxes, each box launched on its own (dt kernel,
+ * stage 1, stage 2 per box per step): the baseline the fused launch replaces. */
+int chl_ensemble_advance_looped(chl_ensemble *e, int n_steps, float *ms_out);
+/* Per-box clocks (each array may be null). */
+int chl_ensemble_clocks(chl_ensemble *e, double *t, double *dt, int *n_steps);
+/* Pick launch form, tiles and z-chunks by timing candidates on a scratch copy of
+ * the bound states (states and clocks are left as they were).  Needs a started ensemble. */
+int chl_ensemble_autotune(chl_ensemble *e, int steps_per_candidate, chl_ensemble_config *chosen);
+
 /* ---- problem setup on the host (not timed): initial states of the benchmark boxes */
 /* Kelvin-Helmholtz box: n^3 real cells + ng ghosts, unit box, periodic, central slab
  * |y-0.5|<0.25 rho=2 vx=+0.5, outside rho=1 vx=-0.5, p=2.5, vy = 0.01 sin(4 pi x)
  * + 1e-4 u, u ~ U(-1,1) from std::mt19937_64(seed) drawn k, j, i with i fastest.
  * host_U must hold 5*(n+2ng)^3 doubles; ghost cells are filled periodically. */
 int chl_ic_kelvin_helmholtz(double *host_U, int n, int ng, double gamma, uint64_t seed);
+
+/* Ensemble layout: per box an edge drawn uniformly from {32, 48, 64, 96, 128}
+ * (std::mt19937_64(seed)() % 5) and an rms Mach number from U[0.5, 3]
+ * (0.5 + 2.5 * u, u = 53 high bits of the next draw / 2^53), box by box. */
+int chl_ensemble_layout(uint64_t seed, int n_boxes, int *edges, double *mach);
+/* Modes of the turbulent velocity field: every integer wavevector k with
+ * 1 <= |k| <= 4 in the half space (kx > 0, or kx == 0 and ky > 0, or kx == ky == 0
+ * and kz > 0), in the order kx, ky, kz ascending (kz fastest).  Per mode two
+ * 3-vectors a, b of independent normals (Box-Muller on std::mt19937_64(seed),
+ * drawn ax, ay, az, bx, by, bz) with standard deviation |k|^-1, so the
+ * per-mode power is proportional to k^-2, then projected solenoidal
+ * (a -= k (k.a)/|k|^2, likewise b).  modes: 9 doubles per mode (k, a, b). */
+int chl_turbulence_modes(uint64_t seed, double *modes, int max_modes, int *n_modes);
+/* Periodic turbulence box on the device: n^3 real cells + ng ghosts, unit box,
+ * rho and p uniform, v = s * sum_m [a_m cos(2 pi k_m.x) + b_m sin(2 pi k_m.x)] at
+ * the cell centres with s set so the rms |v| over the box is mach * sqrt(gamma p / rho);
+ * ghost cells hold their periodic images.  d_U: 5*(n+2ng)^3 doubles on the device. */
+int chl_ic_turbulence(double *d_U, int n, int ng, double gamma, double rho, double p, double mach, uint64_t seed);
 
 #ifdef __cplusplus
 }

@@ -65,3 +65,53 @@ def test_kh_initial_state_host():
     # same seed, same state; different seed, different noise
     assert np.array_equal(U.reshape(-1), chl.ic_kelvin_helmholtz(n, ng, 5 / 3, seed=1))
     assert not np.array_equal(U.reshape(-1), chl.ic_kelvin_helmholtz(n, ng, 5 / 3, seed=2))
+
+
+# ----------------------------------------------------------------------------- ensemble setup (host side)
+def test_ensemble_no_device_fails_loudly():
+    if chl.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    with pytest.raises(chl.ChollaError):
+        chl.Ensemble([(40, 40, 40)], 4)
+    with pytest.raises(chl.ChollaError):
+        chl.check(chl.lib().chl_ic_turbulence(None, 16, 4, 5 / 3, 1.0, 1.0, 1.0, 1))
+
+
+def test_ensemble_layout_draws():
+    edges, mach = chl.ensemble_layout(2024, 384)
+    assert set(np.unique(edges)) <= {32, 48, 64, 96, 128}
+    assert len(np.unique(edges)) == 5
+    assert mach.min() >= 0.5 and mach.max() <= 3.0
+    # uniform over the five edges and over [0.5, 3]: loose 5-sigma bounds for 384 draws
+    counts = np.array([(edges == e).sum() for e in (32, 48, 64, 96, 128)])
+    assert np.all(np.abs(counts - 384 / 5) < 5 * np.sqrt(384 * 0.2 * 0.8)), counts
+    assert abs(mach.mean() - 1.75) < 5 * 2.5 / np.sqrt(12 * 384)
+    e2, m2 = chl.ensemble_layout(2024, 384)
+    assert np.array_equal(edges, e2) and np.array_equal(mach, m2)
+    e3, _ = chl.ensemble_layout(2025, 384)
+    assert not np.array_equal(edges, e3)
+    # the first draw is the engine's first output mod 5 (std::mt19937_64, seed 5489 -> 14514284786278117030)
+    e0, _ = chl.ensemble_layout(5489, 1)
+    assert e0[0] == [32, 48, 64, 96, 128][14514284786278117030 % 5]
+
+
+def test_turbulence_modes():
+    m = chl.turbulence_modes(7)
+    k = m[:, :3]
+    # every half-space integer vector with 1 <= |k|^2 <= 16, each once, in kx, ky, kz order
+    want = [(kx, ky, kz) for kx in range(0, 5) for ky in range(-4, 5) for kz in range(-4, 5)
+            if 1 <= kx * kx + ky * ky + kz * kz <= 16 and (kx > 0 or (kx == 0 and (ky > 0 or (ky == 0 and kz > 0))))]
+    assert [tuple(int(v) for v in row) for row in k] == want
+    full = sum(1 for kx in range(-4, 5) for ky in range(-4, 5) for kz in range(-4, 5)
+               if 1 <= kx * kx + ky * ky + kz * kz <= 16)
+    assert 2 * len(want) == full
+    k2 = (k ** 2).sum(1)
+    a, b = m[:, 3:6], m[:, 6:9]
+    # solenoidal: k . a = k . b = 0
+    assert np.abs((k * a).sum(1)).max() < 1e-14 and np.abs((k * b).sum(1)).max() < 1e-14
+    # per-mode power ~ k^-2: after the projection two of three components remain, so
+    # E[|a|^2] = 2 / |k|^2; |a|^2 |k|^2 / 2 averages to 1 (chi-square with 2 d.o.f. / 2)
+    s = np.concatenate([(a ** 2).sum(1) * k2 / 2, (b ** 2).sum(1) * k2 / 2])
+    assert abs(s.mean() - 1.0) < 5 / np.sqrt(len(s)), s.mean()
+    assert not np.array_equal(m, chl.turbulence_modes(8))
+    assert np.array_equal(m, chl.turbulence_modes(7))
