@@ -60,6 +60,30 @@ def lib():
             "orc_vl1d": (None, [_dp, i, i, d, d, d, i, i]),
             "orc_run": (i, [_dp, i, i, i, i, i, d, d, d, d, i, _ip, _dp, d, i, i, d, i, d, _dp]),
             "orc_num_threads": (i, []),
+            # MHD (mhd_oracle.cpp)
+            "orc_mhd_fast_speed": (d, [d] * 6),
+            "orc_mhd_slow_speed": (d, [d] * 6),
+            "orc_mhd_alfven_speed": (d, [d, d]),
+            "orc_mhd_total_pressure": (d, [d] * 4),
+            "orc_mhd_pressure_prim": (d, [d] * 9),
+            "orc_mhd_energy_prim": (d, [d] * 9),
+            "orc_mhd_cell_centered_b": (None, [_dp, i, i, i, i, i, i, _dp]),
+            "orc_mhd_load_data": (None, [_dp, i, i, i, i, i, i, i, d, _dp]),
+            "orc_mhd_eigenvectors": (None, [_dp, d, d, d, _dp]),
+            "orc_mhd_prim_to_char": (None, [_dp, _dp, d, d, d, _dp]),
+            "orc_mhd_char_to_prim": (None, [_dp, _dp, d, d, d, _dp]),
+            "orc_mhd_inverse_crossing_time": (d, [d] * 13),
+            "orc_mhd_pcm_3d": (None, [_dp] * 7 + [i, i, i]),
+            "orc_mhd_plmc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_hlld": (None, [_dp, _dp, _dp, _dp, l, d, i]),
+            "orc_ct_slope": (d, [_dp, _dp, d, i, i, i, i, i, i, i, i, i, i, l, l, l]),
+            "orc_ct_electric_fields": (None, [_dp] * 5 + [i, i, i, i]),
+            "orc_update_magnetic": (None, [_dp, _dp, _dp, i, i, i, d, d, d, d]),
+            "orc_mhd_calc_dti": (d, [_dp, i, i, i, i, d, d, d, d]),
+            "orc_mhd_divergence": (d, [_dp, i, i, i, d, d, d]),
+            "orc_mhd_vl3d": (None, [_dp, i, i, i, i, d, d, d, d, d, i, d]),
+            "orc_mhd_run": (i, [_dp, i, i, i, i, d, d, d, d, _ip, _dp, d, i, i, d, i, d, _dp]),
+            "orc_fill_uniform32": (None, [ctypes.c_uint32, d, d, l, _dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -107,3 +131,31 @@ def run(U, nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, recon, flags, t, to
                       ctypes.byref(tt), tout, max_steps, int(density_floor_on), density_floor,
                       int(temperature_floor_on), U_floor, ptr(log) if log is not None else None)
     return n, tt.value
+
+
+def uniform32(seed, a, b, n):
+    """std::uniform_real_distribution<double>(a, b) draws of std::mt19937(seed)."""
+    out = np.empty(n, dtype=np.float64)
+    lib().orc_fill_uniform32(seed, a, b, n, ptr(out))
+    return out
+
+
+def mhd_run(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma, flags, t, tout, max_steps, density_floor_on=True,
+            density_floor=0.0, temperature_floor_on=True, U_floor=0.0, dt_log=None):
+    """main.cpp's loop for the MHD build (PLMC + HLLD + CT); U (8 fields, ghosts set) advances in place.
+
+    Returns (steps, t)."""
+    f, fp = iptr(flags)
+    tt = ctypes.c_double(t)
+    n = lib().orc_mhd_run(ptr(U), nx, ny, nz, n_ghost, dx, dy, dz, gamma, fp, ctypes.byref(tt), tout, max_steps,
+                          int(density_floor_on), density_floor, int(temperature_floor_on), U_floor,
+                          ptr(dt_log) if dt_log is not None else None)
+    return n, tt.value
+
+
+def mhd_calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma):
+    return lib().orc_mhd_calc_dti(ptr(U), nx, ny, nz, n_ghost, dx, dy, dz, gamma)
+
+
+def mhd_divergence(U, nx, ny, nz, dx, dy, dz):
+    return lib().orc_mhd_divergence(ptr(U), nx, ny, nz, dx, dy, dz)

@@ -146,3 +146,49 @@ def exact_riemann_cell_average(rl, ul, pl, rr, ur, pr, gamma, x_edges, x0, t, su
         out[1, c] = (r * u).mean()
         out[2, c] = (p / (gamma - 1) + 0.5 * r * u * u).mean()
     return out
+
+
+def mhd_linear_wave_ic(nx_real, ny_real, nz_real, ng, xlen, ylen, zlen, gamma, rho, vx, P, B, A, eig,
+                       wave_length=1.0):
+    """Grid3D::Linear_Wave with MHD (src/grid/initial_conditions.cpp:302-436), pitch = yaw = 0.
+
+    eig: right eigenvector (rho, mx, my, mz, E, Bx, By, Bz).  The face-centred fields of the
+    real cells take the wave evaluated at the cell centre, as the reference assigns them; the
+    hydro fields also fill the first lower ghost layer.  Returns the 8-field state with total
+    dims (nx, ny, nz) = real + 2*ng; the other ghost cells are left zero."""
+    nx, ny, nz = nx_real + 2 * ng, ny_real + 2 * ng, nz_real + 2 * ng
+    dx = xlen / nx_real
+    U = np.zeros((8, nz, ny, nx))
+    k = 2.0 * math.pi / wave_length
+    e_rho, e_mx, e_my, e_mz, e_E, e_bx, e_by, e_bz = eig
+    for i in range(ng, nx - ng):
+        s = math.sin((0.0 + dx * (i - ng) + 0.5 * dx) * k)
+        sl = (slice(ng, nz - ng), slice(ng, ny - ng), i)
+        U[5][sl] = B[0] + A * e_bx * s
+        U[6][sl] = B[1] + A * e_by * s
+        U[7][sl] = B[2] + A * e_bz * s
+    for i in range(ng - 1, nx - ng):
+        s = math.sin((0.0 + dx * (i - ng) + 0.5 * dx) * k)
+        sl = (slice(ng - 1, nz - ng), slice(ng - 1, ny - ng), i)
+        U[0][sl] = rho + A * e_rho * s
+        U[1][sl] = rho * vx + A * e_mx * s
+        U[2][sl] = A * e_my * s
+        U[3][sl] = A * e_mz * s
+        U[4][sl] = P / (gamma - 1.0) + 0.5 * rho * vx * vx + A * s * e_E + 0.5 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2])
+    return U.reshape(-1).copy(), (nx, ny, nz), (dx, ylen / ny_real, zlen / nz_real)
+
+
+def vector_potential_b(A, nx, ny, nz, dx, dy, dz):
+    """Face-centred B = curl A of an edge vector potential (A: 3 x nz x ny x nx), the
+    differences of mhd::utils::Init_Magnetic_Field_With_Vector_Potential
+    (src/utils/mhd_utilities.cu:22-44); faces with any index 0 stay 0."""
+    Ax, Ay, Az = A
+    B = np.zeros((3, nz, ny, nx))
+    s = (slice(1, None), slice(1, None), slice(1, None))
+    zm = (slice(0, -1), slice(1, None), slice(1, None))
+    ym = (slice(1, None), slice(0, -1), slice(1, None))
+    xm = (slice(1, None), slice(1, None), slice(0, -1))
+    B[0][s] = (Az[s] - Az[ym]) / dy - (Ay[s] - Ay[zm]) / dz
+    B[1][s] = (Ax[s] - Ax[zm]) / dz - (Az[s] - Az[xm]) / dx
+    B[2][s] = (Ay[s] - Ay[xm]) / dx - (Ax[s] - Ax[ym]) / dy
+    return B
