@@ -31,7 +31,10 @@ EXPORTS = [
     "chl_hllc_fluxes", "chl_ic_kelvin_helmholtz", "chl_ensemble_create", "chl_ensemble_destroy",
     "chl_ensemble_get_config", "chl_ensemble_persistent_pairs", "chl_ensemble_bind", "chl_ensemble_start",
     "chl_ensemble_advance", "chl_ensemble_advance_looped", "chl_ensemble_clocks", "chl_ensemble_autotune",
-    "chl_ensemble_layout", "chl_turbulence_modes", "chl_ic_turbulence",
+    "chl_ensemble_layout", "chl_turbulence_modes", "chl_ic_turbulence", "chl_mhd_create", "chl_mhd_destroy",
+    "chl_mhd_vl_step", "chl_mhd_run", "chl_mhd_time_steps", "chl_mhd_calc_dt", "chl_mhd_divergence",
+    "chl_hlld_fluxes", "chl_mhd_reconstruct_plmc", "chl_ct_electric_fields", "chl_update_magnetic_field",
+    "chl_magnetic_field_from_vector_potential", "chl_ic_orszag_tang",
 ]
 LAUNCH_SPLIT, LAUNCH_PERSISTENT = 0, 1
 
@@ -60,6 +63,14 @@ class EnsembleConfig(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MhdConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("n_ghost", ctypes.c_int),
+        ("precision", ctypes.c_int), ("density_floor", ctypes.c_int), ("temperature_floor", ctypes.c_int),
+        ("bc_flags", ctypes.c_int * 6),
+    ]
 
 
 _lib = None
@@ -118,6 +129,19 @@ def lib():
         "chl_ensemble_layout": (_i, [ctypes.c_uint64, _i, _ip, _dp]),
         "chl_turbulence_modes": (_i, [ctypes.c_uint64, _dp, _i, _ip]),
         "chl_ic_turbulence": (_i, [_vp, _i, _i, _d, _d, _d, _d, ctypes.c_uint64]),
+        "chl_mhd_create": (_i, [ctypes.POINTER(MhdConfig), ctypes.POINTER(_vp)]),
+        "chl_mhd_destroy": (_i, [_vp]),
+        "chl_mhd_vl_step": (_i, [_vp, _vp, _d, _d, _d, _d, _d, _d]),
+        "chl_mhd_run": (_i, [_vp, _vp, _d, _d, _d, _d, _d, _d, _d, _d, _i, _i, _dp, _ip]),
+        "chl_mhd_time_steps": (_i, [_vp, _vp, _d, _d, _d, _d, _i, _i, ctypes.POINTER(ctypes.c_float), _dp]),
+        "chl_mhd_calc_dt": (_i, [_vp, _i, _i, _i, _i, _d, _d, _d, _d, _dp]),
+        "chl_mhd_divergence": (_i, [_vp, _i, _i, _i, _d, _d, _d, _dp]),
+        "chl_hlld_fluxes": (_i, [_vp, _vp, _vp, _vp, ctypes.c_long, _d, _i]),
+        "chl_mhd_reconstruct_plmc": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
+        "chl_ct_electric_fields": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i]),
+        "chl_update_magnetic_field": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d]),
+        "chl_magnetic_field_from_vector_potential": (_i, [_dp, _i, _i, _i, _d, _d, _d, _dp]),
+        "chl_ic_orszag_tang": (_i, [_dp, _i, _i, _d, ctypes.c_uint64, _d]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -396,3 +420,94 @@ def turbulence_modes(seed):
 def ic_turbulence(U, n, ng, gamma=5.0 / 3.0, rho=1.0, p=1.0, mach=1.0, seed=1):
     """Turbulence box on the device (U: DeviceArray of 5*(n+2ng)^3)."""
     check(lib().chl_ic_turbulence(U.ptr, n, ng, gamma, rho, p, mach, seed))
+
+
+# ---------------------------------------------------------------- ideal MHD
+
+class Mhd:
+    """MHD integrator handle for one box (chl_mhd_* entry points); states hold 8 fields."""
+
+    def __init__(self, nx, ny, nz, n_ghost=4, precision=PREC_FP64_EXACT, bc=(1, 1, 1, 1, 1, 1), density_floor=True,
+                 temperature_floor=True):
+        cfg = MhdConfig()
+        cfg.nx, cfg.ny, cfg.nz, cfg.n_ghost, cfg.precision = nx, ny, nz, n_ghost, precision
+        cfg.density_floor, cfg.temperature_floor = int(density_floor), int(temperature_floor)
+        for q in range(6):
+            cfg.bc_flags[q] = bc[q]
+        h = _vp()
+        check(lib().chl_mhd_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.dims = (nx, ny, nz)
+        self.n_ghost = n_ghost
+
+    def vl_step(self, U, dx, dy, dz, dt, gamma, density_floor=0.0):
+        check(lib().chl_mhd_vl_step(self.h, U.ptr, dx, dy, dz, dt, gamma, density_floor))
+
+    def run(self, U, dx, dy, dz, gamma, t0, tout, max_steps, use_graph=True, density_floor=0.0, U_floor=0.0):
+        t = ctypes.c_double()
+        n = ctypes.c_int()
+        check(lib().chl_mhd_run(self.h, U.ptr, dx, dy, dz, gamma, density_floor, U_floor, t0, tout, max_steps,
+                                int(use_graph), ctypes.byref(t), ctypes.byref(n)))
+        return n.value, t.value
+
+    def time_steps(self, U, dx, dy, dz, gamma, n_steps, use_graph=True):
+        ms = ctypes.c_float()
+        dt = ctypes.c_double()
+        check(lib().chl_mhd_time_steps(self.h, U.ptr, dx, dy, dz, gamma, n_steps, int(use_graph), ctypes.byref(ms),
+                                       ctypes.byref(dt)))
+        return ms.value, dt.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().chl_mhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mhd_calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma):
+    out = ctypes.c_double()
+    check(lib().chl_mhd_calc_dt(U.ptr, nx, ny, nz, n_ghost, dx, dy, dz, gamma, ctypes.byref(out)))
+    return out.value
+
+
+def mhd_divergence(U, nx, ny, nz, dx, dy, dz):
+    out = ctypes.c_double()
+    check(lib().chl_mhd_divergence(U.ptr, nx, ny, nz, dx, dy, dz, ctypes.byref(out)))
+    return out.value
+
+
+def hlld_fluxes(QL, QR, magX, F, n_cells, gamma, direction):
+    check(lib().chl_hlld_fluxes(QL.ptr, QR.ptr, magX.ptr, F.ptr, n_cells, gamma, direction))
+
+
+def mhd_reconstruct_plmc(U, QL, QR, nx, ny, nz, gamma, direction):
+    check(lib().chl_mhd_reconstruct_plmc(U.ptr, QL.ptr, QR.ptr, nx, ny, nz, gamma, direction))
+
+
+def ct_electric_fields(Fx, Fy, Fz, U, E, nx, ny, nz):
+    check(lib().chl_ct_electric_fields(Fx.ptr, Fy.ptr, Fz.ptr, U.ptr, E.ptr, nx, ny, nz))
+
+
+def update_magnetic_field(src, dst, E, nx, ny, nz, dt, dx, dy, dz):
+    check(lib().chl_update_magnetic_field(src.ptr, dst.ptr, E.ptr, nx, ny, nz, dt, dx, dy, dz))
+
+
+def magnetic_field_from_vector_potential(U, nx, ny, nz, dx, dy, dz, A):
+    """Face fields of the host state U (8 fields, modified in place) from the edge potential A."""
+    A = np.ascontiguousarray(A, dtype=np.float64).reshape(-1)
+    assert U.dtype == np.float64 and U.flags["C_CONTIGUOUS"]
+    check(lib().chl_magnetic_field_from_vector_potential(U.ctypes.data_as(_dp), nx, ny, nz, dx, dy, dz,
+                                                         A.ctypes.data_as(_dp)))
+
+
+def ic_orszag_tang(n, ng=4, gamma=5.0 / 3.0, seed=2024, amplitude=1e-3):
+    """Host-built Orszag-Tang state (8 x (n+2ng)^3, ghosts filled periodically)."""
+    N = n + 2 * ng
+    U = np.empty(8 * N ** 3, dtype=np.float64)
+    check(lib().chl_ic_orszag_tang(U.ctypes.data_as(_dp), n, ng, gamma, seed, amplitude))
+    return U

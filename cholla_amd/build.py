@@ -31,8 +31,11 @@ UNITS = [
     ("hydro_fast.hip", ["-ffp-contract=fast"]),
     ("ensemble_exact.hip", ["-ffp-contract=off"]),
     ("ensemble_fast.hip", ["-ffp-contract=fast"]),
+    ("mhd_exact.hip", ["-ffp-contract=off"]),
+    ("mhd_fast.hip", ["-ffp-contract=fast"]),
     ("capi.cpp", ["-ffp-contract=off"]),
     ("ensemble.cpp", ["-ffp-contract=off"]),
+    ("mhd.cpp", ["-ffp-contract=off"]),
 ]
 # Host programs built on the reference-signature adapter (cholla_dropin.cpp), one per
 # reconstruction build option, as Cholla's make.type files select them.  They need no HIP

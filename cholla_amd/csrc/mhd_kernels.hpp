@@ -1,0 +1,845 @@
+// mhd_kernels.hpp -- one VL step of ideal MHD (PLMC + HLLD + constrained transport) for gfx950.
+//
+// The reference step (src/integrators/VL_3D_cuda.cu:38-360 with MHD) is PCM, HLLD x3,
+// CT electric fields, half update, half face-field update, PLMC x3, HLLD x3, CT,
+// full update, full face-field update: 14 launches that stream the 7-field interface
+// arrays Q_L/Q_R (6 of them) and the flux arrays through HBM.  Here:
+//
+//   MhdFlux1Kernel     PCM states + HLLD on the three faces i+1/2 of every cell
+//                      (no interface arrays: the PCM states are the cells themselves)
+//   MhdEmfKernel       CT edge fields (S&G 2009 eq. 22-24) over a region
+//   MhdHalfKernel      dt/2 update of the hydro fields and of the face fields
+//   MhdFlux2Kernel     PLMC of the two cells of each face (in registers) + HLLD
+//   MhdEmfKernel       CT again, on the half-step state
+//   MhdFullKernel      dt update of the real cells' hydro fields (density and
+//                      temperature floors fused) and of their face fields
+//   MhdCalcDtKernel    next inverse dt (mhdInverseCrossingTime)
+//
+// Only the cells and faces the real cells depend on are computed, every read in
+// bounds; the reference's edge cells that read past the grid do not feed real cells.
+//
+// Layout: U[f*n + i + j*nx + k*nx*ny], f = density, momentum x/y/z, Energy, Bx, By, Bz
+// (face-centred: B?[i] on the face i+1/2).  Flux arrays F_d (d = 0,1,2): 7 fields,
+// density, momentum x/y/z (global order), Energy, then the fluxes of the two
+// transverse fields in cyclic order (grid_enum fluxX_magnetic_z, fluxX_magnetic_y,
+// src/grid/grid_enum.h:80-100).  Electric fields E: 3 fields on cell edges.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hydro_physics.hpp"
+#include "vl3d_kernels.hpp"
+
+namespace chl
+{
+namespace mhd
+{
+
+constexpr double kHlldSmall = 1.0e-8;  // mhd::internal::_hlldSmallNumber, hlld_cuda.h:56
+constexpr int kBx = 5;                 // grid_enum::magnetic_x
+
+template <typename T>
+__device__ __forceinline__ T Dot(T a1, T a2, T a3, T b1, T b2, T b3)
+{
+  return a1 * b1 + ((a2 * b2) + (a3 * b3));  // math_utils::dotProduct
+}
+
+template <typename T>
+__device__ __forceinline__ T MagEnergy(T bx, T by, T bz)
+{
+  return T(0.5) * Dot(bx, by, bz, bx, by, bz);
+}
+
+// mhd::utils, src/utils/mhd_utilities.h:50-151
+template <class P>
+__device__ __forceinline__ typename P::T Magnetosonic(typename P::T d, typename P::T p, typename P::T bx,
+                                                      typename P::T by, typename P::T bz, typename P::T g,
+                                                      typename P::T choice)
+{
+  using T = typename P::T;
+  T bx2 = bx * bx;
+  T b2  = bx2 + ((by * by) + (bz * bz));
+  T t1  = g * p + b2;
+  T t2  = (t1 * t1) - T(4.) * g * p * bx2;
+  t2    = Sqrt<P>(t2);
+  return Sqrt<P>(Div1<P>(t1 + choice * t2, T(2.0) * Fmax(d, T(kTinyNumber))));
+}
+
+template <class P>
+__device__ __forceinline__ typename P::T Alfven(typename P::T bx, typename P::T d)
+{
+  using T = typename P::T;
+  return Div1<P>(Fabs(bx), Sqrt<P>(Fmax(d, T(kTinyNumber))));
+}
+
+// hydro_utilities::Calc_Pressure_Primitive / Calc_Energy_Primitive with MHD (hydro_utilities.h:34-45, 106-118)
+template <typename T>
+__device__ __forceinline__ T PressurePrimM(T E, T d, T vx, T vy, T vz, T g, T bx, T by, T bz)
+{
+  T p = E - T(0.5) * d * Dot(vx, vy, vz, vx, vy, vz);
+  p -= MagEnergy(bx, by, bz);
+  return Fmax((g - T(1.)) * p, T(kTinyNumber));
+}
+template <typename T>
+__device__ __forceinline__ T EnergyPrimM(T P, T d, T vx, T vy, T vz, T g, T bx, T by, T bz)
+{
+  T e = (Fmax(P, T(kTinyNumber)) / (g - T(1.))) + T(0.5) * d * Dot(vx, vy, vz, vx, vy, vz);
+  e += MagEnergy(bx, by, bz);
+  return e;
+}
+
+// mhd::utils::cellCenteredMagneticFields, mhd_utilities.h:177-207
+__device__ __forceinline__ void CellB(const double *U, size_t id, int x, int y, int z, size_t n, int nx, int ny,
+                                      double &bx, double &by, double &bz)
+{
+  const double *B = U + kBx * n;
+  bx = (x > 0) ? 0.5 * (B[id] + B[id - 1]) : B[id];
+  by = (y > 0) ? 0.5 * (B[n + id] + B[n + id - size_t(nx)]) : B[n + id];
+  bz = (z > 0) ? 0.5 * (B[2 * n + id] + B[2 * n + id - size_t(nx) * ny]) : B[2 * n + id];
+}
+
+template <typename T>
+struct PrimM {
+  T d, vx, vy, vz, p, bx, by, bz;
+};
+template <typename T>
+struct ConsM {  // rotated interface state: d, m (normal, t1, t2), E, B transverse
+  T d, mx, my, mz, E, by, bz;
+};
+
+// Rotate a global (x, y, z) triple into the frame of dir: (normal, o2, o3)
+template <typename T>
+__device__ __forceinline__ void RotM(int dir, T x, T y, T z, T &a, T &b, T &c)
+{
+  if (dir == 0) {
+    a = x, b = y, c = z;
+  } else if (dir == 1) {
+    a = y, b = z, c = x;
+  } else {
+    a = z, b = x, c = y;
+  }
+}
+
+// reconstruction::Load_Data with MHD (reconstruction.h:139-212); pressure recovered in FP64
+template <class P>
+__device__ __forceinline__ PrimM<typename P::T> LoadDataM(const double *U, int x, int y, int z, int nx, int ny,
+                                                          size_t n, int dir, double g)
+{
+  using T         = typename P::T;
+  const size_t id = size_t(x) + size_t(y) * nx + size_t(z) * size_t(nx) * ny;
+  double mx, my, mz, cx, cy, cz;
+  RotM(dir, U[n + id], U[2 * n + id], U[3 * n + id], mx, my, mz);
+  CellB(U, id, x, y, z, n, nx, ny, cx, cy, cz);
+  double bx, by, bz;
+  RotM(dir, cx, cy, cz, bx, by, bz);
+  const double d = U[id];
+  double vx, vy, vz;
+  if constexpr (P::kExact) {
+    vx = mx / d, vy = my / d, vz = mz / d;
+  } else {
+    const double inv = 1.0 / d;
+    vx = mx * inv, vy = my * inv, vz = mz * inv;
+  }
+  const double p = PressurePrimM(U[4 * n + id], d, vx, vy, vz, g, bx, by, bz);
+  return PrimM<T>{T(d), T(vx), T(vy), T(vz), T(p), T(bx), T(by), T(bz)};
+}
+
+// reconstruction::Compute_Eigenvectors with MHD (reconstruction.h:310-370)
+template <typename T>
+struct EigenM {
+  T cf, cs, af, as, by, bz, nfs, sign, qf, qs, Af, As, qpf, qps, Apf, Aps, sqrt_d;
+};
+
+template <class P>
+__device__ __forceinline__ EigenM<typename P::T> ComputeEigenM(const PrimM<typename P::T> &w, typename P::T cs,
+                                                               typename P::T cs2, typename P::T g)
+{
+  using T = typename P::T;
+  EigenM<T> e;
+  e.cf         = Magnetosonic<P>(w.d, w.p, w.bx, w.by, w.bz, g, T(1.0));
+  e.cs         = Magnetosonic<P>(w.d, w.p, w.bx, w.by, w.bz, g, T(-1.0));
+  const T cf2 = e.cf * e.cf, css = e.cs * e.cs;
+  const T denom = (cf2 - css), num2 = (cf2 - cs2);
+  if (denom <= T(0.0) or num2 <= T(0.0)) {
+    e.af = T(1.0), e.as = T(0.0);
+  } else {
+    const T num1 = (cs2 - css);
+    if (num1 <= T(0.0)) {
+      e.af = T(0.0), e.as = T(1.0);
+    } else {
+      e.af = Sqrt<P>(Div1<P>(num1, denom));
+      e.as = Sqrt<P>(Div1<P>(num2, denom));
+    }
+  }
+  const T bden = Div1<P>(T(1.0), Sqrt<P>(w.by * w.by + w.bz * w.bz));  // rhypot
+  const bool fin = __builtin_isfinite(bden);
+  e.by           = fin ? w.by * bden : T(1.0);
+  e.bz           = fin ? w.bz * bden : T(0.0);
+  e.sign         = __builtin_copysign(T(1.0), w.bx);
+  e.nfs          = Div1<P>(T(0.5), cs2);
+  e.qpf          = e.sign * e.nfs * e.af * e.cf;
+  e.qps          = e.sign * e.nfs * e.as * e.cs;
+  e.qf           = e.sign * e.af * e.cf;
+  e.qs           = e.sign * e.as * e.cs;
+  e.sqrt_d       = Sqrt<P>(w.d);
+  e.Af           = e.af * cs * e.sqrt_d;
+  e.As           = e.as * cs * e.sqrt_d;
+  e.Apf          = Div1<P>(T(0.5) * e.af, cs * e.sqrt_d);
+  e.Aps          = Div1<P>(T(0.5) * e.as, cs * e.sqrt_d);
+  return e;
+}
+
+template <typename T>
+struct Char7 {
+  T a0, a1, a2, a3, a4, a5, a6;
+};
+
+// reconstruction::Primitive_To_Characteristic with MHD (reconstruction.h:378-425)
+template <class P>
+__device__ __forceinline__ Char7<typename P::T> PrimToCharM(const PrimM<typename P::T> &w,
+                                                            const PrimM<typename P::T> &s,
+                                                            const EigenM<typename P::T> &e, typename P::T cs2,
+                                                            typename P::T inv_d, typename P::T inv_cs2)
+{
+  using T     = typename P::T;
+  const T isd = Div1<P>(T(1.0), e.sqrt_d);  // rsqrt
+  const T pd  = Div<P>(s.p, w.d, inv_d);
+  Char7<T> c;
+  c.a0 = e.nfs * e.af * (pd - e.cf * s.vx) + e.qps * (e.by * s.vy + e.bz * s.vz) + e.Aps * (e.by * s.by + e.bz * s.bz);
+  c.a1 = T(0.5) * (e.by * (s.bz * e.sign * isd + s.vz) - e.bz * (s.by * e.sign * isd + s.vy));
+  c.a2 = e.nfs * e.as * (pd - e.cs * s.vx) - e.qpf * (e.by * s.vy + e.bz * s.vz) - e.Apf * (e.by * s.by + e.bz * s.bz);
+  c.a3 = s.d - Div<P>(s.p, cs2, inv_cs2);
+  c.a4 = e.nfs * e.as * (pd + e.cs * s.vx) + e.qpf * (e.by * s.vy + e.bz * s.vz) - e.Apf * (e.by * s.by + e.bz * s.bz);
+  c.a5 = T(0.5) * (e.by * (s.bz * e.sign * isd - s.vz) - e.bz * (s.by * e.sign * isd - s.vy));
+  c.a6 = e.nfs * e.af * (pd + e.cf * s.vx) - e.qps * (e.by * s.vy + e.bz * s.vz) + e.Aps * (e.by * s.by + e.bz * s.bz);
+  return c;
+}
+
+// reconstruction::Characteristic_To_Primitive with MHD (reconstruction.h:450-490)
+template <class P>
+__device__ __forceinline__ PrimM<typename P::T> CharToPrimM(const PrimM<typename P::T> &w,
+                                                            const Char7<typename P::T> &a,
+                                                            const EigenM<typename P::T> &e, typename P::T cs2)
+{
+  using T = typename P::T;
+  PrimM<T> o;
+  o.d  = w.d * (e.af * (a.a0 + a.a6) + e.as * (a.a2 + a.a4)) + a.a3;
+  o.vx = e.cf * e.af * (a.a6 - a.a0) + e.cs * e.as * (a.a4 - a.a2);
+  o.vy = e.by * (e.qs * (a.a0 - a.a6) + e.qf * (a.a4 - a.a2)) + e.bz * (a.a5 - a.a1);
+  o.vz = e.bz * (e.qs * (a.a0 - a.a6) + e.qf * (a.a4 - a.a2)) + e.by * (a.a1 - a.a5);
+  o.p  = w.d * cs2 * (e.af * (a.a0 + a.a6) + e.as * (a.a2 + a.a4));
+  o.bx = T(0.0);
+  o.by = e.by * (e.As * (a.a0 + a.a6) - e.Af * (a.a2 + a.a4)) - e.bz * e.sign * e.sqrt_d * (a.a5 + a.a1);
+  o.bz = e.bz * (e.As * (a.a0 + a.a6) - e.Af * (a.a2 + a.a4)) + e.by * e.sign * e.sqrt_d * (a.a5 + a.a1);
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ T MonoM(T l, T r, T c, T v)  // Monotonize_Characteristic_Return_Primitive
+{
+  if (l * r > T(0.0)) {
+    T la = T(2.0) * Fmin(Fabs(l), Fabs(r));
+    T lb = Fmin(Fabs(c), Fabs(v));
+    return __builtin_copysign(Fmin(la, lb), c);
+  }
+  return T(0.0);
+}
+template <typename T>
+__device__ __forceinline__ T VlM(T l, T r)  // Van_Leer_Slope
+{
+  return (l * r > T(0.0)) ? T(2.0) * l * r / (l + r) : T(0.0);
+}
+template <typename T>
+__device__ __forceinline__ void LimitM(T &l, T &r, T vm, T v0, T vp)  // Plm_Limit_Interfaces
+{
+  r = Fmax(Fmin(v0, vm), r);
+  r = Fmin(Fmax(v0, vm), r);
+  l = Fmax(Fmin(v0, vp), l);
+  l = Fmin(Fmax(v0, vp), l);
+}
+
+// PLMC_cuda with MHD for one cell (plmc_cuda.cu:24-300, VL build): the interface
+// states at +1/2 (Lp) and -1/2 (Rm) of cell (x,y,z) along dir.
+template <class P>
+__device__ __forceinline__ void PlmcCellM(const double *U, int x, int y, int z, int nx, int ny, size_t n, int dir,
+                                          double g, PrimM<typename P::T> &Lp, PrimM<typename P::T> &Rm)
+{
+  using T        = typename P::T;
+  const int di   = dir == 0, dj = dir == 1, dk = dir == 2;
+  const PrimM<T> c0 = LoadDataM<P>(U, x, y, z, nx, ny, n, dir, g);
+  const PrimM<T> cm = LoadDataM<P>(U, x - di, y - dj, z - dk, nx, ny, n, dir, g);
+  const PrimM<T> cp = LoadDataM<P>(U, x + di, y + dj, z + dk, nx, ny, n, dir, g);
+  const T gg        = T(g);
+  const T inv_d     = Div1<P>(T(1.0), c0.d);
+  const T cs        = Sqrt<P>(Div<P>(gg * c0.p, c0.d, inv_d));  // Calc_Sound_Speed
+  const T cs2       = cs * cs;
+  const T inv_cs2   = Div1<P>(T(1.0), cs2);
+  const EigenM<T> e = ComputeEigenM<P>(c0, cs, cs2, gg);
+  auto slope = [](const PrimM<T> &l, const PrimM<T> &r, T c) {
+    return PrimM<T>{c * (r.d - l.d), c * (r.vx - l.vx), c * (r.vy - l.vy), c * (r.vz - l.vz),
+                    c * (r.p - l.p), T(0),             c * (r.by - l.by), c * (r.bz - l.bz)};
+  };
+  const PrimM<T> dL = slope(cm, c0, T(1.0)), dR = slope(c0, cp, T(1.0)), dC = slope(cm, cp, T(0.5));
+  const PrimM<T> dG{VlM(dL.d, dR.d), VlM(dL.vx, dR.vx), VlM(dL.vy, dR.vy), VlM(dL.vz, dR.vz),
+                    VlM(dL.p, dR.p), T(0),              VlM(dL.by, dR.by), VlM(dL.bz, dR.bz)};
+  const Char7<T> aL = PrimToCharM<P>(c0, dL, e, cs2, inv_d, inv_cs2);
+  const Char7<T> aR = PrimToCharM<P>(c0, dR, e, cs2, inv_d, inv_cs2);
+  const Char7<T> aC = PrimToCharM<P>(c0, dC, e, cs2, inv_d, inv_cs2);
+  const Char7<T> aG = PrimToCharM<P>(c0, dG, e, cs2, inv_d, inv_cs2);
+  Char7<T> am;
+  am.a0 = MonoM(aL.a0, aR.a0, aC.a0, aG.a0);
+  am.a1 = MonoM(aL.a1, aR.a1, aC.a1, aG.a1);
+  am.a2 = MonoM(aL.a2, aR.a2, aC.a2, aG.a2);
+  am.a3 = MonoM(aL.a3, aR.a3, aC.a3, aG.a3);
+  am.a4 = MonoM(aL.a4, aR.a4, aC.a4, aG.a4);
+  am.a5 = MonoM(aL.a5, aR.a5, aC.a5, aG.a5);
+  am.a6 = MonoM(aL.a6, aR.a6, aC.a6, aG.a6);
+  const PrimM<T> dm = CharToPrimM<P>(c0, am, e, cs2);
+  auto iface = [](T q, T s, T sign) { return q + sign * T(0.5) * s; };  // Calc_Interface_Linear
+  Lp = PrimM<T>{iface(c0.d, dm.d, T(1)),   iface(c0.vx, dm.vx, T(1)), iface(c0.vy, dm.vy, T(1)),
+                iface(c0.vz, dm.vz, T(1)), iface(c0.p, dm.p, T(1)),   T(0),
+                iface(c0.by, dm.by, T(1)), iface(c0.bz, dm.bz, T(1))};
+  Rm = PrimM<T>{iface(c0.d, dm.d, T(-1)),   iface(c0.vx, dm.vx, T(-1)), iface(c0.vy, dm.vy, T(-1)),
+                iface(c0.vz, dm.vz, T(-1)), iface(c0.p, dm.p, T(-1)),   T(0),
+                iface(c0.by, dm.by, T(-1)), iface(c0.bz, dm.bz, T(-1))};
+  LimitM(Lp.d, Rm.d, cm.d, c0.d, cp.d);
+  LimitM(Lp.vx, Rm.vx, cm.vx, c0.vx, cp.vx);
+  LimitM(Lp.vy, Rm.vy, cm.vy, c0.vy, cp.vy);
+  LimitM(Lp.vz, Rm.vz, cm.vz, c0.vz, cp.vz);
+  LimitM(Lp.p, Rm.p, cm.p, c0.p, cp.p);
+  LimitM(Lp.by, Rm.by, cm.by, c0.by, cp.by);
+  LimitM(Lp.bz, Rm.bz, cm.bz, c0.bz, cp.bz);
+  Rm.d = Fmax(Rm.d, T(kTinyNumber));
+  Lp.d = Fmax(Lp.d, T(kTinyNumber));
+  Rm.p = Fmax(Rm.p, T(kTinyNumber));
+  Lp.p = Fmax(Lp.p, T(kTinyNumber));
+}
+
+// reconstruction::Write_Data with MHD (reconstruction.h:918-970), kept in registers:
+// conserved interface state with the face field bx for the energy
+template <class P>
+__device__ __forceinline__ ConsM<typename P::T> WriteDataM(const PrimM<typename P::T> &s, typename P::T bx,
+                                                           typename P::T g)
+{
+  using T = typename P::T;
+  return ConsM<T>{s.d, s.d * s.vx, s.d * s.vy, s.d * s.vz, EnergyPrimM(s.p, s.d, s.vx, s.vy, s.vz, g, bx, s.by, s.bz),
+                  s.by, s.bz};
+}
+
+// ---------------------------------------------------------------------------- HLLD (hlld_cuda.cu)
+template <typename T>
+struct HStateM {
+  T d, vx, vy, vz, E, by, bz, pg, pt;
+};
+template <typename T>
+struct HStarM {
+  T d, vy, vz, E, by, bz;
+};
+
+// mhd::internal::loadState (hlld_cuda.cu:170-214)
+template <class P>
+__device__ __forceinline__ HStateM<typename P::T> LoadStateM(const ConsM<typename P::T> &q, typename P::T bx,
+                                                             typename P::T g)
+{
+  using T = typename P::T;
+  HStateM<T> s;
+  s.d       = Fmax(q.d, T(kTinyNumber));
+  const T inv = Div1<P>(T(1.0), s.d);
+  s.vx      = Div<P>(q.mx, s.d, inv);
+  s.vy      = Div<P>(q.my, s.d, inv);
+  s.vz      = Div<P>(q.mz, s.d, inv);
+  s.E       = Fmax(q.E, T(kTinyNumber));
+  s.by = q.by, s.bz = q.bz;
+  s.pg = PressurePrimM(s.E, s.d, s.vx, s.vy, s.vz, g, bx, s.by, s.bz);
+  s.pt = Fmax(s.pg + MagEnergy(bx, s.by, s.bz), T(kTinyNumber));  // computeTotalPressure
+  return s;
+}
+
+template <typename T>
+__device__ __forceinline__ ConsM<T> NonStarFluxM(const HStateM<T> &s, T bx)  // hlld_cuda.cu:262-283
+{
+  ConsM<T> f;
+  f.d  = s.d * s.vx;
+  f.mx = f.d * s.vx + s.pt - bx * bx;
+  f.my = f.d * s.vy - bx * s.by;
+  f.mz = f.d * s.vz - bx * s.bz;
+  f.by = s.by * s.vx - bx * s.vy;
+  f.bz = s.bz * s.vx - bx * s.vz;
+  f.E  = s.vx * (s.E + s.pt) - bx * (s.vx * bx + ((s.vy * s.by) + (s.vz * s.bz)));
+  return f;
+}
+
+template <class P>
+__device__ __forceinline__ HStarM<typename P::T> StarStateM(const HStateM<typename P::T> &s, typename P::T M,
+                                                            typename P::T side, typename P::T bx,
+                                                            typename P::T ptStar)  // hlld_cuda.cu:327-372
+{
+  using T = typename P::T;
+  HStarM<T> o;
+  o.d = Div1<P>(s.d * (side - s.vx), (side - M));
+  if (Fabs(s.d * (side - s.vx) * (side - M) - (bx * bx)) < ptStar * T(kHlldSmall)) {
+    o.vy = s.vy, o.vz = s.vz, o.by = s.by, o.bz = s.bz;
+  } else {
+    const T denom = s.d * (side - s.vx) * (side - M) - (bx * bx);
+    const T inv   = Div1<P>(T(1.0), denom);
+    T coef        = Div<P>(bx * (M - s.vx), denom, inv);
+    o.vy          = s.vy - s.by * coef;
+    o.vz          = s.vz - s.bz * coef;
+    T tp          = (side - s.vx);
+    tp            = tp * tp;
+    coef          = Div<P>((s.d * tp - (bx * bx)), denom, inv);
+    o.by          = s.by * coef;
+    o.bz          = s.bz * coef;
+  }
+  o.E = Div1<P>((s.E * (side - s.vx) - s.pt * s.vx + ptStar * M +
+                 bx * (Dot(s.vx, s.vy, s.vz, bx, s.by, s.bz) - Dot(M, o.vy, o.vz, bx, o.by, o.bz))),
+                (side - M));
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ ConsM<T> StarFluxM(const HStarM<T> &st, const HStateM<T> &s, const ConsM<T> &f, T M,
+                                              T side)  // hlld_cuda.cu:376-397
+{
+  ConsM<T> o;
+  o.d  = f.d + side * (st.d - s.d);
+  o.mx = f.mx + side * (st.d * M - s.d * s.vx);
+  o.my = f.my + side * (st.d * st.vy - s.d * s.vy);
+  o.mz = f.mz + side * (st.d * st.vz - s.d * s.vz);
+  o.E  = f.E + side * (st.E - s.E);
+  o.by = f.by + side * (st.by - s.by);
+  o.bz = f.bz + side * (st.bz - s.bz);
+  return o;
+}
+
+// Calculate_HLLD_Fluxes_CUDA for one interface (hlld_cuda.cu:31-164), rotated frame.
+// Returns false when no branch of the reference assigns a flux (non-finite speeds).
+template <class P>
+__device__ __forceinline__ bool HlldM(const ConsM<typename P::T> &qL, const ConsM<typename P::T> &qR, typename P::T bx,
+                                      typename P::T g, ConsM<typename P::T> &out)
+{
+  using T              = typename P::T;
+  const HStateM<T> sL = LoadStateM<P>(qL, bx, g), sR = LoadStateM<P>(qR, bx, g);
+  T SL, SR;
+  {
+    const T cL = Magnetosonic<P>(sL.d, sL.pg, bx, sL.by, sL.bz, g, T(1.0));
+    const T cR = Magnetosonic<P>(sR.d, sR.pg, bx, sR.by, sR.bz, g, T(1.0));
+    const T cM = Fmax(cL, cR);
+    SL         = Fmin(sL.vx, sR.vx) - cM;
+    SR         = Fmax(sL.vx, sR.vx) + cM;
+  }
+  const ConsM<T> fL = NonStarFluxM(sL, bx);
+  if (SL > T(0.0)) {
+    out = fL;
+    return true;
+  }
+  const ConsM<T> fR = NonStarFluxM(sR, bx);
+  if (SR < T(0.0)) {
+    out = fR;
+    return true;
+  }
+  const T rd = SR - sR.vx, ld = SL - sL.vx;
+  const T M  = Div1<P>((rd * sR.d * sR.vx - ld * sL.d * sL.vx - sR.pt + sL.pt), (rd * sR.d - ld * sL.d));
+  const T ptStar = Div1<P>((sR.d * sL.pt * (SR - sR.vx) - sL.d * sR.pt * (SL - sL.vx) +
+                            sL.d * sR.d * (SR - sR.vx) * (SL - sL.vx) * (sR.vx - sL.vx)),
+                           (sR.d * (SR - sR.vx) - sL.d * (SL - sL.vx)));
+  const HStarM<T> stL = StarStateM<P>(sL, M, SL, bx, ptStar);
+  const T SLs         = M + T(-1.0) * Alfven<P>(bx, stL.d);
+  if (SLs > T(0.0) and SL <= T(0.0)) {
+    out = StarFluxM(stL, sL, fL, M, SL);
+    return true;
+  }
+  const HStarM<T> stR = StarStateM<P>(sR, M, SR, bx, ptStar);
+  const T SRs         = M + T(1.0) * Alfven<P>(bx, stR.d);
+  if (SRs <= T(0.0) and SR >= T(0.0)) {
+    out = StarFluxM(stR, sR, fR, M, SR);
+    return true;
+  }
+  // computeDoubleStarState (hlld_cuda.cu:401-465)
+  T vy, vz, by, bz, EL, ER;
+  if (T(0.5) * (bx * bx) < T(kHlldSmall) * ptStar) {
+    if (M >= T(0.0)) {
+      vy = stL.vy, vz = stL.vz, by = stL.by, bz = stL.bz, EL = stL.E, ER = stR.E;
+    } else {
+      vy = stR.vy, vz = stR.vz, by = stR.by, bz = stR.bz, ER = stR.E, EL = stL.E;
+    }
+  } else {
+    const T sqL = Sqrt<P>(stL.d), sqR = Sqrt<P>(stR.d);
+    const T inv = Div1<P>(T(1.0), (sqL + sqR));
+    const T sgn = __builtin_copysign(T(1.0), bx);
+    vy          = inv * (sqL * stL.vy + sqR * stR.vy + sgn * (stR.by - stL.by));
+    vz          = inv * (sqL * stL.vz + sqR * stR.vz + sgn * (stR.bz - stL.bz));
+    by          = inv * (sqL * stR.by + sqR * stL.by + sgn * (sqL * sqR) * (stR.vy - stL.vy));
+    bz          = inv * (sqL * stR.bz + sqR * stL.bz + sgn * (sqL * sqR) * (stR.vz - stL.vz));
+    const T vb  = Dot(M, vy, vz, bx, by, bz);
+    EL          = stL.E - sqL * sgn * (Dot(M, stL.vy, stL.vz, bx, stL.by, stL.bz) - vb);
+    ER          = stR.E + sqR * sgn * (Dot(M, stR.vy, stR.vz, bx, stR.by, stR.bz) - vb);
+  }
+  // computeDoubleStarFluxes (hlld_cuda.cu:469-505)
+  auto dstar = [&](const HStarM<T> &st, const HStateM<T> &s, const ConsM<T> &f, T side, T sideStar, T dE) {
+    const T diff = sideStar - side;
+    ConsM<T> o;
+    o.d  = f.d - side * s.d - diff * st.d + sideStar * st.d;
+    o.mx = f.mx - side * (s.d * s.vx) - diff * (st.d * M) + sideStar * (st.d * M);
+    o.my = f.my - side * (s.d * s.vy) - diff * (st.d * st.vy) + sideStar * (st.d * vy);
+    o.mz = f.mz - side * (s.d * s.vz) - diff * (st.d * st.vz) + sideStar * (st.d * vz);
+    o.E  = f.E - side * s.E - diff * st.E + sideStar * dE;
+    o.by = f.by - side * s.by - diff * st.by + sideStar * by;
+    o.bz = f.bz - side * s.bz - diff * st.bz + sideStar * bz;
+    return o;
+  };
+  if (M > T(0.0) and SLs <= T(0.0)) {
+    out = dstar(stL, sL, fL, SL, SLs, EL);
+    return true;
+  }
+  if (SRs > T(0.0) and M <= T(0.0)) {
+    out = dstar(stR, sR, fR, SR, SRs, ER);
+    return true;
+  }
+  return false;
+}
+
+// mhd::internal::returnFluxes (hlld_cuda.cu:287-304): rotated flux to the global slots
+template <typename TF, typename T>
+__device__ __forceinline__ void StoreFluxM(TF *F, size_t id, size_t n, int dir, const ConsM<T> &f)
+{
+  T gx, gy, gz;
+  if (dir == 0) {
+    gx = f.mx, gy = f.my, gz = f.mz;
+  } else if (dir == 1) {
+    gy = f.mx, gz = f.my, gx = f.mz;
+  } else {
+    gz = f.mx, gx = f.my, gy = f.mz;
+  }
+  F[id]         = TF(f.d);
+  F[n + id]     = TF(gx);
+  F[2 * n + id] = TF(gy);
+  F[3 * n + id] = TF(gz);
+  F[4 * n + id] = TF(f.E);
+  F[5 * n + id] = TF(f.by);
+  F[6 * n + id] = TF(f.bz);
+}
+
+// ---------------------------------------------------------------------------- CT (ct_electric_fields.cu/.h)
+// Reads outside [0, total) return 0; within the regions the step kernels compute none occurs.
+template <typename TF>
+__device__ __forceinline__ double AtF(const TF *a, long i, long total)
+{
+  return (i >= 0 && i < total) ? double(a[i]) : 0.0;
+}
+
+// mhd::internal::_ctSlope (ct_electric_fields.h:66-119)
+template <typename TF>
+__device__ __forceinline__ double CtSlopeM(const TF *flux, const double *U, double fluxSign, int ctDir, int cq1,
+                                           int cq2, int fq1, int fq2, int x, int y, int z, int nx, int ny, long n,
+                                           long nF, long nU)
+{
+  const int m1 = (ctDir + 1) % 3, m2 = (ctDir + 2) % 3;
+  const int xc = x - int(cq1 == 0) - int(cq2 == 0);
+  const int yc = y - int(cq1 == 1) - int(cq2 == 1);
+  const int zc = z - int(cq1 == 2) - int(cq2 == 2);
+  auto lin     = [&](long i, long j, long k) { return i + j * long(nx) + k * long(nx) * long(ny); };
+  const long ic    = lin(xc, yc, zc);
+  const long iflux = lin(x - int(fq1 == 0) - int(fq2 == 0), y - int(fq1 == 1) - int(fq2 == 1),
+                         z - int(fq1 == 2) - int(fq2 == 2));
+  const long ib2 = lin(xc - int(m1 == 0), yc - int(m1 == 1), zc - int(m1 == 2));
+  const long ib3 = lin(xc - int(m2 == 0), yc - int(m2 == 1), zc - int(m2 == 2));
+  const double d   = AtF(U, ic, nU);
+  const double mm2 = AtF(U, ic + (m1 + 1) * n, nU);
+  const double mm3 = AtF(U, ic + (m2 + 1) * n, nU);
+  const double b2  = 0.5 * (AtF(U, ic + (m1 + kBx) * n, nU) + AtF(U, ib2 + (m1 + kBx) * n, nU));
+  const double b3  = 0.5 * (AtF(U, ic + (m2 + kBx) * n, nU) + AtF(U, ib3 + (m2 + kBx) * n, nU));
+  const double ec  = (mm3 * b2 - mm2 * b3) / d;
+  const double ef  = fluxSign * AtF(flux, iflux + (int(fluxSign == 1) + kBx) * n, nF);
+  return ef - ec;
+}
+
+// mhd::Calculate_CT_Electric_Fields for edges with every index in [lo, hi] (ct_electric_fields.cu:20-279)
+template <typename TF>
+__global__ void __launch_bounds__(256) MhdEmfKernel(const TF *Fx, const TF *Fy, const TF *Fz, const double *U,
+                                                    double *E, int nx, int ny, int nz, int lo, int hix, int hiy,
+                                                    int hiz, const double *dt_dev)
+{
+  const size_t n  = size_t(nx) * ny * nz;
+  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  if (dt_dev && *dt_dev == 0.0) return;
+  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
+  if (x < lo || y < lo || z < lo || x > hix || y > hiy || z > hiz) return;
+  const long nF = 7 * long(n), nU = 8 * long(n);
+  auto F = [&](const TF *a, int i, int j, int k, int f) {
+    return AtF(a, long(i) + long(j) * nx + long(k) * long(nx) * ny + long(f) * long(n), nF);
+  };
+  auto S = [&](const TF *fl, double sgn, int ctd, int a, int b, int c, int d, int i, int j, int k) {
+    return CtSlopeM(fl, U, sgn, ctd, a, b, c, d, i, j, k, nx, ny, long(n), nF, nU);
+  };
+  auto pick = [](double s, double pos, double neg) {
+    return s > 0.0 ? pos : (s < 0.0 ? neg : 0.5 * (pos + neg));
+  };
+  const int i = x, j = y, k = z;
+  {  // x electric field
+    const double syp = pick(F(Fz, i, j, k - 1, 0), S(Fy, -1, 0, 2, -1, 1, 2, i, j, k), S(Fy, -1, 0, -1, -1, 1, -1, i, j, k));
+    const double syn = pick(F(Fz, i, j - 1, k - 1, 0), S(Fy, -1, 0, 1, 2, 1, 2, i, j, k), S(Fy, -1, 0, 1, -1, 1, -1, i, j, k));
+    const double szp = pick(F(Fy, i, j - 1, k, 0), S(Fz, 1, 0, 1, -1, 1, 2, i, j, k), S(Fz, 1, 0, -1, -1, 2, -1, i, j, k));
+    const double szn = pick(F(Fy, i, j - 1, k - 1, 0), S(Fz, 1, 0, 1, 2, 1, 2, i, j, k), S(Fz, 1, 0, 2, -1, -1, 2, i, j, k));
+    const double fyp = +F(Fz, i, j, k - 1, 6), fyn = +F(Fz, i, j - 1, k - 1, 6);
+    const double fzp = -F(Fy, i, j - 1, k, 5), fzn = -F(Fy, i, j - 1, k - 1, 5);
+    E[id] = 0.25 * (+fyp + fyn + fzp + fzn + syp + syn + szp + szn);
+  }
+  {  // y electric field
+    const double sxp = pick(F(Fz, i, j, k - 1, 0), S(Fx, 1, 1, 2, -1, 0, 2, i, j, k), S(Fx, 1, 1, -1, -1, 0, -1, i, j, k));
+    const double sxn = pick(F(Fz, i - 1, j, k - 1, 0), S(Fx, 1, 1, 0, 2, 0, 2, i, j, k), S(Fx, 1, 1, 0, -1, 0, -1, i, j, k));
+    const double szp = pick(F(Fx, i - 1, j, k, 0), S(Fz, -1, 1, 0, -1, 0, 2, i, j, k), S(Fz, -1, 1, -1, -1, 2, -1, i, j, k));
+    const double szn = pick(F(Fx, i - 1, j, k - 1, 0), S(Fz, -1, 1, 0, 2, 0, 2, i, j, k), S(Fz, -1, 1, 2, -1, 2, -1, i, j, k));
+    const double fxp = -F(Fz, i, j, k - 1, 5), fxn = -F(Fz, i - 1, j, k - 1, 5);
+    const double fzp = +F(Fx, i - 1, j, k, 6), fzn = +F(Fx, i - 1, j, k - 1, 6);
+    E[n + id] = 0.25 * (+fxp + fxn + fzp + fzn + sxp + sxn + szp + szn);
+  }
+  {  // z electric field
+    const double syp = pick(F(Fx, i - 1, j, k, 0), S(Fy, 1, 2, 0, -1, 0, 1, i, j, k), S(Fy, 1, 2, -1, -1, 1, -1, i, j, k));
+    const double syn = pick(F(Fx, i - 1, j - 1, k, 0), S(Fy, 1, 2, 0, 1, 0, 1, i, j, k), S(Fy, 1, 2, 1, -1, 1, -1, i, j, k));
+    const double sxp = pick(F(Fy, i, j - 1, k, 0), S(Fx, -1, 2, 1, -1, 0, 1, i, j, k), S(Fx, -1, 2, -1, -1, 0, -1, i, j, k));
+    const double sxn = pick(F(Fy, i - 1, j - 1, k, 0), S(Fx, -1, 2, 0, 1, 0, 1, i, j, k), S(Fx, -1, 2, 0, -1, 0, -1, i, j, k));
+    const double fxp = +F(Fy, i, j - 1, k, 6), fxn = +F(Fy, i - 1, j - 1, k, 6);
+    const double fyp = -F(Fx, i - 1, j, k, 5), fyn = -F(Fx, i - 1, j - 1, k, 5);
+    E[2 * n + id] = 0.25 * (+fxp + fxn + fyp + fyn + sxp + sxn + syp + syn);
+  }
+}
+
+// ---------------------------------------------------------------------------- step arguments
+struct MhdArgs {
+  double *U;          // state (8 fields, in/out)
+  double *Uh;         // half-step state (8 fields)
+  void *F[3];         // flux arrays (7 fields each, P::T)
+  double *E;          // edge electric fields (3 fields)
+  int nx, ny, nz, ng;
+  double dx, dy, dz, gamma;
+  const double *dt_dev;  // device dt (device-timed step), or null
+  double dt;
+  unsigned long long *dti;
+  double density_floor, U_floor;
+  int flags;  // kFlag* bits of vl3d_kernels.hpp
+};
+
+__device__ __forceinline__ double MhdDt(const MhdArgs &a) { return a.dt_dev ? *a.dt_dev : a.dt; }
+
+// Stage-1 fluxes: PCM states (the cells, with their cell-centred fields) + HLLD on the
+// face i+1/2 of cell i along every axis whose face lies inside the grid.
+template <class P>
+__global__ void __launch_bounds__(256) MhdFlux1Kernel(MhdArgs a)
+{
+  using T         = typename P::T;
+  const int nx    = a.nx, ny = a.ny, nz = a.nz;
+  const size_t n  = size_t(nx) * ny * nz;
+  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if ((a.flags & kFlagReduceDt) && id == 0) *a.dti = 0ull;  // MhdCalcDtKernel (after this step) fills it
+  if (id >= n) return;
+  if (MhdDt(a) == 0.0) return;
+  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
+  const double *U = a.U;
+  const T g       = T(a.gamma);
+  const int c[3] = {x, y, z}, nn[3] = {nx, ny, nz};
+  for (int dir = 0; dir < 3; dir++) {
+    if (c[dir] > nn[dir] - 2) continue;
+    const size_t step = dir == 0 ? 1 : (dir == 1 ? size_t(nx) : size_t(nx) * ny);
+    auto pcm = [&](int xx, int yy, int zz, size_t cid) {
+      double mx, my, mz, cx, cy, cz, bx, by, bz;
+      RotM(dir, U[n + cid], U[2 * n + cid], U[3 * n + cid], mx, my, mz);
+      CellB(U, cid, xx, yy, zz, n, nx, ny, cx, cy, cz);
+      RotM(dir, cx, cy, cz, bx, by, bz);
+      return ConsM<T>{T(U[cid]), T(mx), T(my), T(mz), T(U[4 * n + cid]), T(by), T(bz)};
+    };
+    const ConsM<T> qL = pcm(x, y, z, id);
+    const ConsM<T> qR = pcm(x + (dir == 0), y + (dir == 1), z + (dir == 2), id + step);
+    ConsM<T> f;
+    if (HlldM<P>(qL, qR, T(U[(kBx + dir) * n + id]), g, f))
+      StoreFluxM(reinterpret_cast<T *>(a.F[dir]), id, n, dir, f);
+  }
+}
+
+// Half update: hydro fields of cells [1, n-2] (Update_Conserved_Variables_3D_half,
+// VL_3D_cuda.cu:379-495, with the density floor) and their face fields
+// (Update_Magnetic_Field_3D, magnetic_update.cu:21-81, from U into Uh).
+template <class P>
+__global__ void __launch_bounds__(256) MhdHalfKernel(MhdArgs a)
+{
+  using T         = typename P::T;
+  const int nx    = a.nx, ny = a.ny, nz = a.nz;
+  const size_t n  = size_t(nx) * ny * nz;
+  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const double dt = 0.5 * MhdDt(a);
+  if (dt == 0.0) return;
+  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
+  if (x < 1 || y < 1 || z < 1 || x > nx - 2 || y > ny - 2 || z > nz - 2) return;
+  const double dtodx = dt / a.dx, dtody = dt / a.dy, dtodz = dt / a.dz;
+  const size_t sx = 1, sy = size_t(nx), sz = size_t(nx) * ny;
+  const T *Fx = reinterpret_cast<const T *>(a.F[0]), *Fy = reinterpret_cast<const T *>(a.F[1]),
+          *Fz = reinterpret_cast<const T *>(a.F[2]);
+  const double *U = a.U;
+  double *Uh      = a.Uh;
+  double uh[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    const size_t o = size_t(f) * n;
+    uh[f] = U[o + id] + dtodx * (double(Fx[o + id - sx]) - double(Fx[o + id])) +
+            dtody * (double(Fy[o + id - sy]) - double(Fy[o + id])) + dtodz * (double(Fz[o + id - sz]) - double(Fz[o + id]));
+  }
+  if ((a.flags & kFlagDensityFloor) && uh[0] < a.density_floor) {
+    const double d0 = uh[0];
+    uh[0]           = a.density_floor;
+#pragma unroll
+    for (int f = 1; f < 5; f++) uh[f] *= (a.density_floor / d0);
+  }
+#pragma unroll
+  for (int f = 0; f < 5; f++) Uh[size_t(f) * n + id] = uh[f];
+  // face fields whose edge fields were computed (edges [2, n-1]): own index >= 1, others >= 2
+  const double *E = a.E;
+  if (y >= 2 && z >= 2) {
+    const double ey1 = E[n + id + sx], ey3 = E[n + id + sx + sz];
+    const double ez1 = E[2 * n + id + sx], ez3 = E[2 * n + id + sx + sy];
+    Uh[5 * n + id] = U[5 * n + id] + dtodz * (ey3 - ey1) + dtody * (ez1 - ez3);
+  }
+  if (x >= 2 && z >= 2) {
+    const double ez2 = E[2 * n + id + sy], ez3 = E[2 * n + id + sx + sy];
+    const double ex1 = E[id + sy], ex3 = E[id + sy + sz];
+    Uh[6 * n + id] = U[6 * n + id] + dtodx * (ez3 - ez2) + dtodz * (ex1 - ex3);
+  }
+  if (x >= 2 && y >= 2) {
+    const double ex2 = E[id + sz], ex3 = E[id + sy + sz];
+    const double ey2 = E[n + id + sz], ey3 = E[n + id + sx + sz];
+    Uh[7 * n + id] = U[7 * n + id] + dtody * (ex3 - ex2) + dtodx * (ey2 - ey3);
+  }
+}
+
+// Stage-2 fluxes on faces with every index in [lo, hi]: PLMC of the two cells of the
+// face (PLMC_cuda writes them to Q_L at i and Q_R at i, plmc_cuda.cu:292-298) + HLLD.
+template <class P>
+__global__ void __launch_bounds__(256) MhdFlux2Kernel(MhdArgs a, int lo, int hix, int hiy, int hiz)
+{
+  using T         = typename P::T;
+  const int nx    = a.nx, ny = a.ny, nz = a.nz;
+  const size_t n  = size_t(nx) * ny * nz;
+  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  if (MhdDt(a) == 0.0) return;
+  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
+  if (x < lo || y < lo || z < lo || x > hix || y > hiy || z > hiz) return;
+  const double *Uh = a.Uh;
+  const T g        = T(a.gamma);
+  for (int dir = 0; dir < 3; dir++) {
+    PrimM<T> Lp, Rm, tmp;
+    PlmcCellM<P>(Uh, x, y, z, nx, ny, n, dir, a.gamma, Lp, tmp);
+    PlmcCellM<P>(Uh, x + (dir == 0), y + (dir == 1), z + (dir == 2), nx, ny, n, dir, a.gamma, tmp, Rm);
+    const T bx        = T(Uh[(kBx + dir) * n + id]);
+    const ConsM<T> qL = WriteDataM<P>(Lp, bx, g), qR = WriteDataM<P>(Rm, bx, g);
+    ConsM<T> f;
+    if (HlldM<P>(qL, qR, bx, g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[dir]), id, n, dir, f);
+  }
+}
+
+// Full update (Update_Conserved_Variables_3D, hydro_cuda.cu:172-402, and
+// Update_Magnetic_Field_3D in place): hydro fields of the real cells with the density
+// floor and the temperature floor (Temperature_Floor_Kernel, hydro_cuda.cu:1117-1153);
+// face fields whose own index is in [ng-1, n-ng-1] and other indices in [ng, n-ng-1].
+template <class P>
+__global__ void __launch_bounds__(256) MhdFullKernel(MhdArgs a)
+{
+  using T         = typename P::T;
+  const int nx    = a.nx, ny = a.ny, nz = a.nz, ng = a.ng;
+  const size_t n  = size_t(nx) * ny * nz;
+  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const double dt = MhdDt(a);
+  if (dt == 0.0) return;
+  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
+  if (x < ng - 1 || y < ng - 1 || z < ng - 1 || x > nx - ng - 1 || y > ny - ng - 1 || z > nz - ng - 1) return;
+  const bool rx = x >= ng, ry = y >= ng, rz = z >= ng;
+  const double dtodx = dt / a.dx, dtody = dt / a.dy, dtodz = dt / a.dz;
+  const size_t sx = 1, sy = size_t(nx), sz = size_t(nx) * ny;
+  double *U       = a.U;
+  if (rx && ry && rz) {
+    const T *Fx = reinterpret_cast<const T *>(a.F[0]), *Fy = reinterpret_cast<const T *>(a.F[1]),
+            *Fz = reinterpret_cast<const T *>(a.F[2]);
+    double un[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      const size_t o = size_t(f) * n;
+      un[f] = U[o + id] + (dtodx * (double(Fx[o + id - sx]) - double(Fx[o + id])) +
+                           dtody * (double(Fy[o + id - sy]) - double(Fy[o + id])) +
+                           dtodz * (double(Fz[o + id - sz]) - double(Fz[o + id])));
+    }
+    if ((a.flags & kFlagDensityFloor) && un[0] < a.density_floor) {
+      if (un[0] > 0) {
+        const double d0 = un[0];
+        un[0]           = a.density_floor;
+#pragma unroll
+        for (int f = 1; f < 5; f++) un[f] *= (a.density_floor / d0);
+      } else {
+        un[0] = (U[id - 1] + U[id + 1] + U[id - sy] + U[id + sy] + U[id - sz] + U[id + sz]) / 6;
+      }
+    }
+    if (a.flags & kFlagTemperatureFloor) {
+      const double dinv = 1.0 / un[0];
+      const double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
+      const double Ekin = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz);
+      if ((un[4] - Ekin) / un[0] < a.U_floor) un[4] = Ekin + un[0] * a.U_floor;
+    }
+#pragma unroll
+    for (int f = 0; f < 5; f++) U[size_t(f) * n + id] = un[f];
+  }
+  const double *E = a.E;
+  if (ry && rz) {
+    const double ey1 = E[n + id + sx], ey3 = E[n + id + sx + sz];
+    const double ez1 = E[2 * n + id + sx], ez3 = E[2 * n + id + sx + sy];
+    U[5 * n + id] = U[5 * n + id] + dtodz * (ey3 - ey1) + dtody * (ez1 - ez3);
+  }
+  if (rx && rz) {
+    const double ez2 = E[2 * n + id + sy], ez3 = E[2 * n + id + sx + sy];
+    const double ex1 = E[id + sy], ex3 = E[id + sy + sz];
+    U[6 * n + id] = U[6 * n + id] + dtodx * (ez3 - ez2) + dtodz * (ex1 - ex3);
+  }
+  if (rx && ry) {
+    const double ex2 = E[id + sz], ex3 = E[id + sy + sz];
+    const double ey2 = E[n + id + sz], ey3 = E[n + id + sx + sz];
+    U[7 * n + id] = U[7 * n + id] + dtody * (ex3 - ex2) + dtodx * (ey2 - ey3);
+  }
+}
+
+// ---------------------------------------------------------------------------- dt, divergence
+// mhdInverseCrossingTime (hydro_cuda.cu:420-435) with the cell-centred field
+__device__ __forceinline__ double MhdInverseCrossingTime(const double *U, size_t id, int x, int y, int z, size_t n,
+                                                         int nx, int ny, double dx, double dy, double dz, double g)
+{
+  const double d = U[id], dinv = 1.0 / d;
+  const double vx = U[n + id] * dinv, vy = U[2 * n + id] * dinv, vz = U[3 * n + id] * dinv;
+  double bx, by, bz;
+  CellB(U, id, x, y, z, n, nx, ny, bx, by, bz);
+  const double gasP = PressurePrimM(U[4 * n + id], d, vx, vy, vz, g, bx, by, bz);
+  const double cf   = Magnetosonic<PolicyExact>(d, gasP, bx, by, bz, g, 1.0);
+  double m          = Fmax((Fabs(vx) + cf) / dx, (Fabs(vy) + cf) / dy);
+  m                 = Fmax(m, (Fabs(vz) + cf) / dz);
+  return Fmax(m, 0.0);
+}
+
+}  // namespace mhd
+
+// Host launchers (mhd_exact.hip / mhd_fast.hip)
+hipError_t LaunchMhdStep(int policy, const mhd::MhdArgs &a, hipStream_t s);
+hipError_t LaunchMhdCalcDt(const double *U, int nx, int ny, int nz, int ng, double dx, double dy, double dz,
+                           double gamma, unsigned long long *dti, int num_cu, hipStream_t s);
+hipError_t LaunchMhdDivergence(const double *U, int nx, int ny, int nz, double dx, double dy, double dz,
+                               unsigned long long *out, int num_cu, hipStream_t s);
+// reference-shaped stage kernels (FP64, reference operation order) for the unit vectors
+hipError_t LaunchHlldGrid(const double *QL, const double *QR, const double *magX, double *F, long n_cells,
+                          double gamma, int dir, hipStream_t s);
+hipError_t LaunchMhdPlmcGrid(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir,
+                             hipStream_t s);
+hipError_t LaunchCtGrid(const double *Fx, const double *Fy, const double *Fz, const double *U, double *E, int nx,
+                        int ny, int nz, hipStream_t s);
+hipError_t LaunchMagneticUpdateGrid(const double *src, double *dst, const double *E, int nx, int ny, int nz,
+                                    double dt, double dx, double dy, double dz, hipStream_t s);
+
+}  // namespace chl

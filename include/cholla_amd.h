@@ -194,6 +194,53 @@ int chl_ensemble_clocks(chl_ensemble *e, double *t, double *dt, int *n_steps);
  * the bound states (states and clocks are left as they were).  Needs a started ensemble. */
 int chl_ensemble_autotune(chl_ensemble *e, int steps_per_candidate, chl_ensemble_config *chosen);
 
+/* ---- ideal MHD (the reference's MHD build: -DMHD -DPLMC -DHLLD -DVL, builds/make.type.mhd).
+ * The state holds 8 fields: density, momentum x/y/z, Energy, then the face-centred
+ * magnetic_x/y/z (magnetic_?[i] on the face i+1/2 of cell i; src/grid/grid_enum.h:60-66). */
+typedef struct chl_mhd_config {
+  int nx, ny, nz;        /* totals incl. ghosts */
+  int n_ghost;           /* PLMC 3 + 1 for the face fields = 4 (src/grid/grid3D.cpp:53-76) */
+  int precision;         /* CHL_PREC_*; the mixed form runs PLMC and HLLD in FP32 */
+  int density_floor;     /* 0/1 */
+  int temperature_floor; /* 0/1 */
+  int bc_flags[6];
+} chl_mhd_config;
+
+typedef struct chl_mhd chl_mhd; /* owns the MHD integrator scratch of one box */
+
+int chl_mhd_create(const chl_mhd_config *cfg, chl_mhd **out);
+int chl_mhd_destroy(chl_mhd *h);
+/* VL_Algorithm_3D_CUDA with MHD (src/integrators/VL_3D_cuda.cu:38-360): one step by dt;
+ * real cells and their face fields are updated in place, ghosts are left to the caller. */
+int chl_mhd_vl_step(chl_mhd *h, double *d_conserved, double dx, double dy, double dz, double dt, double gamma,
+                    double density_floor);
+/* The driver loop (src/main.cpp:247-290) for the MHD build with dt on the device, as
+ * chl_hydro_run: dt = 0.3/dti clamped to tout, step, temperature floor, next dti, ghosts. */
+int chl_mhd_run(chl_mhd *h, double *d_conserved, double dx, double dy, double dz, double gamma,
+                double density_floor, double U_floor, double t0, double tout, int max_steps, int use_graph,
+                double *t_out, int *steps_out);
+int chl_mhd_time_steps(chl_mhd *h, double *d_conserved, double dx, double dy, double dz, double gamma, int n_steps,
+                       int use_graph, float *ms_out, double *dt_last_out);
+/* Calc_dt_GPU with MHD (src/hydro/hydro_cuda.cu:514-597, mhdInverseCrossingTime) */
+int chl_mhd_calc_dt(const double *dev_conserved, int nx, int ny, int nz, int n_ghost, double dx, double dy,
+                    double dz, double gamma, double *dti_out);
+/* mhd::checkMagneticDivergence (src/mhd/magnetic_divergence.cu:79): max |div B| */
+int chl_mhd_divergence(const double *dev_conserved, int nx, int ny, int nz, double dx, double dy, double dz,
+                       double *max_divergence);
+/* reference-shaped MHD stage kernels (FP64, reference operation order) on device arrays */
+/* mhd::Calculate_HLLD_Fluxes_CUDA (src/riemann_solvers/hlld_cuda.cu:31) */
+int chl_hlld_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, const double *dev_magnetic_face,
+                    double *dev_flux, long n_cells, double gamma, int dir);
+/* PLMC_cuda with MHD (src/reconstruction/plmc_cuda.cu:24) */
+int chl_mhd_reconstruct_plmc(const double *dev_conserved, double *dev_bounds_L, double *dev_bounds_R, int nx, int ny,
+                             int nz, double gamma, int dir);
+/* mhd::Calculate_CT_Electric_Fields (src/mhd/ct_electric_fields.cu:20) */
+int chl_ct_electric_fields(const double *fluxX, const double *fluxY, const double *fluxZ,
+                           const double *dev_conserved, double *ctElectricFields, int nx, int ny, int nz);
+/* mhd::Update_Magnetic_Field_3D (src/mhd/magnetic_update.cu:21) */
+int chl_update_magnetic_field(const double *sourceGrid, double *destinationGrid, const double *ctElectricFields,
+                              int nx, int ny, int nz, double dt, double dx, double dy, double dz);
+
 /* ---- problem setup on the host (not timed): initial states of the benchmark boxes */
 /* Kelvin-Helmholtz box: n^3 real cells + ng ghosts, unit box, periodic, central slab
  * |y-0.5|<0.25 rho=2 vx=+0.5, outside rho=1 vx=-0.5, p=2.5, vy = 0.01 sin(4 pi x)
@@ -218,6 +265,17 @@ int chl_turbulence_modes(uint64_t seed, double *modes, int max_modes, int *n_mod
  * the cell centres with s set so the rms |v| over the box is mach * sqrt(gamma p / rho);
  * ghost cells hold their periodic images.  d_U: 5*(n+2ng)^3 doubles on the device. */
 int chl_ic_turbulence(double *d_U, int n, int ng, double gamma, double rho, double p, double mach, uint64_t seed);
+/* mhd::utils::Init_Magnetic_Field_With_Vector_Potential (src/utils/mhd_utilities.cu:22-44):
+ * face fields = differences of the edge potential A (3 x n_cells, x/y/z components) for
+ * every cell with i, j, k >= 1.  host_U: 8 fields. */
+int chl_magnetic_field_from_vector_potential(double *host_U, int nx, int ny, int nz, double dx, double dy, double dz,
+                                             const double *A);
+/* Orszag-Tang vortex (Grid3D::Orszag_Tang_Vortex, src/grid/initial_conditions.cpp:1926-1984) on a
+ * periodic unit box of n^3 real cells + ng ghosts, with a z-dependent divergence-free
+ * perturbation through the same potential: A_x, A_y += amplitude / (2 pi) *
+ * sum_{m=1..4} a_m / m cos(2 pi m z + phi_m), a ~ U(-1,1), phi ~ U(0, 2 pi) from
+ * std::mt19937_64(seed) drawn a_x, phi_x, a_y, phi_y per m.  host_U: 8 x (n+2ng)^3. */
+int chl_ic_orszag_tang(double *host_U, int n, int ng, double gamma, uint64_t seed, double amplitude);
 
 #ifdef __cplusplus
 }

@@ -1,11 +1,14 @@
 """CPU-side checks of the C-ABI library: it loads, exports every symbol that
 include/cholla_amd.h declares, refuses to run without a device (no silent
 fallback), and its host-only problem setup is correct."""
+import math
 import os
 import re
 import subprocess
 
 import numpy as np
+import sys
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import pytest
 
 import cholla_amd as chl
@@ -115,3 +118,53 @@ def test_turbulence_modes():
     assert abs(s.mean() - 1.0) < 5 / np.sqrt(len(s)), s.mean()
     assert not np.array_equal(m, chl.turbulence_modes(8))
     assert np.array_equal(m, chl.turbulence_modes(7))
+
+
+# ----------------------------------------------------------------------------- MHD setup (host side)
+def test_vector_potential_matches_reference_vector():
+    """mhd_utilities_tests.cu tMHDInitMagneticFieldWithVectorPotential on the product's host routine."""
+    import json
+
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "mhd_unit_vectors.json")))["vector_potential"]
+    n = g["n"]
+    U = np.full(8 * n ** 3, -999.0)
+    A = np.arange(3 * n ** 3, dtype=float)
+    chl.magnetic_field_from_vector_potential(U, n, n, n, g["d"], g["d"], g["d"], A)
+    for i, v in enumerate(U):
+        assert v == g["expect"].get(str(i), -999.0), (i, v)
+
+
+def test_orszag_tang_initial_state():
+    from helpers import vector_potential_b
+
+    n, ng, g = 16, 4, 5 / 3
+    N = n + 2 * ng
+    U = chl.ic_orszag_tang(n, ng, g, seed=5, amplitude=1e-3).reshape(8, N, N, N)
+    h = 1.0 / n
+    # divergence of the face field vanishes to round-off on every cell with a full stencil
+    B = U[5:8]
+    div = (B[0, 1:, 1:, 1:] - B[0, 1:, 1:, :-1]) / h + (B[1, 1:, 1:, 1:] - B[1, 1:, :-1, 1:]) / h + \
+          (B[2, 1:, 1:, 1:] - B[2, :-1, 1:, 1:]) / h
+    assert np.abs(div).max() < 1e-11
+    # background field: B = (-sin 2 pi y, sin 4 pi x, 0) / sqrt(4 pi) up to the discrete curl and the perturbation
+    x = (np.arange(N) - ng + 0.5) / n
+    B0 = 1 / math.sqrt(4 * math.pi)
+    rho = 25 / (36 * math.pi)
+    assert np.all(U[0, ng:-ng, ng:-ng, ng:-ng] == rho)
+    vx = U[1] / U[0]
+    assert np.allclose(vx[ng:-ng, ng:-ng, ng:-ng], np.sin(2 * math.pi * x)[None, ng:-ng, None], atol=1e-14)
+    by = U[6, ng:-ng, ng:-ng, ng:-ng]
+    assert np.abs(by.mean(axis=(0, 1)) - B0 * np.sin(4 * math.pi * (x[ng:-ng] - 0.5 / n))).max() < 0.05 * B0
+    assert np.abs(U[5, ng:-ng, ng:-ng, ng:-ng].mean()) < 1e-3 and np.abs(U[7]).max() < 1e-12
+    # the perturbation is there (Bx varies along z) and is of the requested size
+    dbx = U[5, ng:-ng, ng, ng] - U[5, ng:-ng, ng, ng].mean()
+    assert 1e-5 < np.abs(dbx).max() < 1e-2
+    # periodic ghosts, all fields
+    assert np.array_equal(U[:, :ng], U[:, n:n + ng])
+    assert np.array_equal(U[:, :, :, -ng:], U[:, :, :, ng:2 * ng])
+    # pressure recovered from the energy with the cell-centred field is the uniform background
+    bc = np.stack([0.5 * (B[0] + np.roll(B[0], 1, 2)), 0.5 * (B[1] + np.roll(B[1], 1, 1)),
+                   0.5 * (B[2] + np.roll(B[2], 1, 0))])
+    p = (U[4] - 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / U[0] - 0.5 * (bc ** 2).sum(0)) * (g - 1)
+    assert np.allclose(p[ng:-ng, ng:-ng, ng:-ng], 5 / (12 * math.pi), rtol=1e-12)
+    assert np.array_equal(U.reshape(-1), chl.ic_orszag_tang(n, ng, g, seed=5, amplitude=1e-3))

@@ -1,0 +1,265 @@
+"""GPU parity of the MHD path (PLMC + HLLD + CT) against the MHD oracle and the reference's vectors.
+
+All calls go through the C ABI.  Bars:
+  * reference-shaped stage kernels: the reference's unit-test fiducials (HLLD: 2.7e-15 or
+    7 ulp as the reference checks them; the others within 4 ulp);
+  * CHL_PREC_FP64_EXACT step and driver loop: bit-identical to the oracle;
+  * CHL_PREC_FP64: per-field relative L1 <= 1e-12 against the oracle;
+  * CHL_PREC_MIXED: per-field relative L1 <= 1e-5 against the oracle;
+  * every form: max |div B| stays at round-off (CT), mass conserved to round-off.
+"""
+import json
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cholla_amd as chl
+from helpers import interior, mhd_linear_wave_ic, rel_l1, vector_potential_b
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(__file__)
+GOLD = json.load(open(os.path.join(HERE, "golden", "mhd_unit_vectors.json")))
+HLLD = json.load(open(os.path.join(HERE, "golden", "hlld_vectors.json")))
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _require_gpu():
+    if chl.device_count() < 1:
+        pytest.fail("no HIP device visible: the GPU tests need a real MI355X")
+
+
+def near(a, b, ulps=4, absolute=1e-14):
+    a, b = float(a), float(b)
+    return a == b or abs(a - b) <= absolute or abs(a - b) <= ulps * np.spacing(max(abs(a), abs(b)))
+
+
+# ----------------------------------------------------------------------------- reference vectors
+def test_hlld_kernel_reference_vectors():
+    cases = HLLD["cases"]
+    m = len(cases)
+    for d in range(3):
+        say("HLLD reference vectors dir", d)
+        QL, QR, bx = np.zeros((7, m)), np.zeros((7, m)), np.zeros(m)
+        for c, case in enumerate(cases):
+            for Q, s in ((QL, case["left"]), (QR, case["right"])):
+                mom = list(s[1:4])
+                if d:
+                    mom = mom[3 - d:] + mom[:3 - d]
+                Q[:, c] = [s[0]] + mom + [s[4], s[6], s[7]]
+            bx[c] = case["left"][5]
+        F = chl.DeviceArray(7 * m, np.zeros(7 * m))
+        dL, dR, dB = (chl.DeviceArray.from_host(a) for a in (QL, QR, bx))
+        bad = []
+        for gam in sorted(set(c["gamma"] for c in cases)):
+            idx = [i for i, c in enumerate(cases) if c["gamma"] == gam]
+            chl.hlld_fluxes(dL, dR, dB, F, m, gam, d)
+            out = F.download().reshape(7, m)
+            for i in idx:
+                f = list(out[:, i])
+                f.insert(5, 0.0)
+                mom = f[1:4]
+                f[1:4] = mom[d:] + mom[:d]
+                for q, (a, b) in enumerate(zip(cases[i]["flux"], f)):
+                    ok = abs(a - b) <= 2.7e-15 or (np.sign(a) == np.sign(b) and abs(
+                        int(np.float64(a).view(np.int64)) - int(np.float64(b).view(np.int64))) <= 7)
+                    if not ok:
+                        bad.append((cases[i]["test"], cases[i]["states"], d, q, a, b))
+        assert not bad, bad[:5]
+
+
+def test_plmc_mhd_kernel_reference_vectors():
+    g = GOLD["plmc_mhd"]
+    nx = g["nx"]
+    n = nx ** 3
+    grid = orc.uniform(g["seed"], *g["dist"], 2 + 8 * n)[2:].copy()
+    U = chl.DeviceArray.from_host(grid)
+    for d in range(3):
+        say("PLMC-MHD reference vectors dir", d)
+        QL, QR = chl.DeviceArray(7 * n, np.zeros(7 * n)), chl.DeviceArray(7 * n, np.zeros(7 * n))
+        chl.mhd_reconstruct_plmc(U, QL, QR, nx, nx, nx, g["gamma"], d)
+        for arr, fid in ((QL.download(), g["left"][d]), (QR.download(), g["right"][d])):
+            for i, v in enumerate(arr):
+                assert near(v, fid.get(str(i), 0.0)), (d, i, v)
+
+
+@pytest.mark.parametrize("case", ["positive", "negative", "zero"])
+def test_ct_kernel_reference_vectors(case):
+    g = GOLD["ct_electric_fields"]
+    n = g["nx"] ** 3
+    fx = np.arange(7 * n, dtype=float)
+    fy = np.arange(7 * n, dtype=float) + fx[-1] + 1
+    fz = np.arange(7 * n, dtype=float) + fy[-1] + 1
+    grid = np.arange(8 * n, dtype=float) + fz[-1] + 1
+    for f in (fx, fy, fz):
+        if case == "negative":
+            f[:n] *= -1
+        if case == "zero":
+            f[:n] = 0
+    say("CT reference vectors", case)
+    E = chl.DeviceArray(3 * n, np.full(3 * n, g["default"]))
+    chl.ct_electric_fields(*(chl.DeviceArray.from_host(a) for a in (fx, fy, fz, grid)), E, 2, 2, 2)
+    want = g["cases"][case]["expect"]
+    for i, v in enumerate(E.download()):
+        assert near(v, want.get(str(i), g["default"])), (i, v)
+
+
+def test_magnetic_update_kernel_reference_vectors():
+    g = GOLD["magnetic_update"]
+    n = g["nx"] ** 3
+    src = np.arange(8 * n, dtype=float)
+    E = np.arange(3 * n, dtype=float) + src[-1] + 1
+    dst = chl.DeviceArray(8 * n, np.full(8 * n, g["default"]))
+    say("magnetic update reference vectors")
+    chl.update_magnetic_field(chl.DeviceArray.from_host(src), dst, chl.DeviceArray.from_host(E), g["nx"], g["nx"],
+                              g["nx"], g["dt"], g["dx"], g["dx"], g["dx"])
+    for i, v in enumerate(dst.download()):
+        assert near(v, g["expect"].get(str(i), g["default"])), (i, v)
+
+
+# ----------------------------------------------------------------------------- whole steps vs the oracle
+def random_mhd_box(nxr, nyr, nzr, ng, seed, g=5 / 3):
+    nx, ny, nz = nxr + 2 * ng, nyr + 2 * ng, nzr + 2 * ng
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) - ng + 0.5) / nxr
+    y = (np.arange(ny) - ng + 0.5) / nyr
+    z = (np.arange(nz) - ng + 0.5) / nzr
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    d = 1.0 + 0.2 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) + 0.02 * rng.uniform(-1, 1, X.shape)
+    v = 0.5 * rng.uniform(-1, 1, (3,) + X.shape)
+    p = 1.0 + 0.1 * rng.uniform(-1, 1, X.shape)
+    A = np.stack([0.05 * np.cos(2 * np.pi * (Y + Z)), 0.05 * np.sin(2 * np.pi * (X + Z)),
+                  0.3 / (2 * np.pi) * (np.cos(2 * np.pi * X) + np.sin(2 * np.pi * Y))])
+    dx, dy, dz = 1.0 / nxr, 1.0 / nyr, 1.0 / nzr
+    B = vector_potential_b(A, nx, ny, nz, dx, dy, dz)
+    bc = np.stack([0.5 * (B[0] + np.roll(B[0], 1, 2)), 0.5 * (B[1] + np.roll(B[1], 1, 1)),
+                   0.5 * (B[2] + np.roll(B[2], 1, 0))])
+    U = np.concatenate([np.stack([d, d * v[0], d * v[1], d * v[2],
+                                  p / (g - 1) + 0.5 * d * (v ** 2).sum(0) + 0.5 * (bc ** 2).sum(0)]), B])
+    U = U.reshape(-1).copy()
+    orc.set_boundaries(U, nx, ny, nz, 8, ng, [1] * 6)
+    return U, (nx, ny, nz), (dx, dy, dz)
+
+
+TOL = {chl.PREC_FP64_EXACT: 0.0, chl.PREC_FP64: 1e-12, chl.PREC_MIXED: 1e-5}
+
+
+def mhd_l1(A, B, dims, ng):
+    nx, ny, nz = dims
+    a, b = interior(A, 8, nz, ny, nx, ng), interior(B, 8, nz, ny, nx, ng)
+    mag = np.sqrt(b[1] ** 2 + b[2] ** 2 + b[3] ** 2).sum()
+    bmag = np.sqrt(b[5] ** 2 + b[6] ** 2 + b[7] ** 2).sum()
+    out = []
+    for f in range(8):
+        den = mag if f in (1, 2, 3) else (bmag if f >= 5 else np.abs(b[f]).sum())
+        out.append(np.abs(a[f] - b[f]).sum() / den)
+    return out
+
+
+@pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
+def test_single_mhd_step(precision):
+    ng, g = 4, 5 / 3
+    U, dims, (dx, dy, dz) = random_mhd_box(20, 16, 18, ng, seed=4)
+    nx, ny, nz = dims
+    dt = 0.3 / orc.mhd_calc_dti(U, nx, ny, nz, ng, dx, dy, dz, g)
+    say("MHD vl_step precision", precision, "dims", dims)
+    Uo = U.copy()
+    orc.lib().orc_mhd_vl3d(orc.ptr(Uo), nx, ny, nz, ng, dx, dy, dz, dt, g, 1, 0.0)
+    Ud = chl.DeviceArray.from_host(U)
+    h = chl.Mhd(nx, ny, nz, ng, precision)
+    h.vl_step(Ud, dx, dy, dz, dt, g)
+    A = Ud.download()
+    # ghost faces of the result are the boundary fill's business on both sides
+    orc.set_boundaries(A, nx, ny, nz, 8, ng, [1] * 6)
+    orc.set_boundaries(Uo, nx, ny, nz, 8, ng, [1] * 6)
+    l1 = mhd_l1(A, Uo, dims, ng)
+    say("  L1", ["%.2e" % v for v in l1], "max|diff| %.3e" % np.abs(A - Uo).max())
+    if precision == chl.PREC_FP64_EXACT:
+        np.testing.assert_array_equal(A, Uo)
+    else:
+        assert max(l1) <= TOL[precision], l1
+    h.close()
+
+
+@pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
+def test_orszag_tang_run(precision):
+    """Orszag-Tang 24^3 with the z perturbation, 12 device-timed steps in a hipGraph."""
+    n, ng, g = 24, 4, 5 / 3
+    N = n + 2 * ng
+    h_ = 1.0 / n
+    U = chl.ic_orszag_tang(n, ng, g, seed=11, amplitude=1e-3)
+    Uo = U.copy()
+    steps_o, t_o = orc.mhd_run(Uo, N, N, N, ng, h_, h_, h_, g, [1] * 6, 0.0, 1e30, 12)
+    say("Orszag-Tang 24^3 x12 precision", precision)
+    Ud = chl.DeviceArray.from_host(U)
+    h = chl.Mhd(N, N, N, ng, precision)
+    steps, t = h.run(Ud, h_, h_, h_, g, 0.0, 1e30, 12, use_graph=True)
+    A = Ud.download()
+    assert steps == steps_o == 12
+    l1 = mhd_l1(A, Uo, (N, N, N), ng)
+    div = chl.mhd_divergence(Ud, N, N, N, h_, h_, h_)
+    bmax = np.abs(A.reshape(8, N, N, N)[5:]).max()
+    say("  L1", ["%.2e" % v for v in l1], "t %.6g vs %.6g" % (t, t_o), "max|div B| %.2e" % div)
+    if precision == chl.PREC_FP64_EXACT:
+        assert t == t_o
+        np.testing.assert_array_equal(A, Uo)
+    else:
+        assert abs(t - t_o) <= max(TOL[precision], 1e-14) * t_o
+        assert max(l1) <= TOL[precision], l1
+    assert div < 1e-11 * bmax / h_
+    assert div == orc.mhd_divergence(A, N, N, N, h_, h_, h_)
+    h.close()
+
+
+def test_graph_replay_matches_eager_mhd():
+    n, ng, g = 16, 4, 5 / 3
+    N = n + 2 * ng
+    h_ = 1.0 / n
+    U = chl.ic_orszag_tang(n, ng, g, seed=3)
+    res = []
+    for use_graph in (False, True, True):
+        say("MHD graph replay", use_graph)
+        Ud = chl.DeviceArray.from_host(U)
+        h = chl.Mhd(N, N, N, ng, chl.PREC_FP64)
+        h.run(Ud, h_, h_, h_, g, 0.0, 1e30, 6, use_graph=use_graph)
+        res.append(Ud.download())
+        h.close()
+    np.testing.assert_array_equal(res[0], res[1])
+    np.testing.assert_array_equal(res[0], res[2])
+
+
+def test_mhd_calc_dt_matches_oracle():
+    ng, g = 4, 5 / 3
+    U, dims, (dx, dy, dz) = random_mhd_box(12, 14, 10, ng, seed=8)
+    want = orc.mhd_calc_dti(U, *dims, ng, dx, dy, dz, g)
+    got = chl.mhd_calc_dti(chl.DeviceArray.from_host(U), *dims, ng, dx, dy, dz, g)
+    assert got == want
+
+
+def test_mhd_fast_linear_wave():
+    """mhd_system_tests.cpp fast magnetosonic wave along x (PLMC): the reference's exact step
+    count and L1 bound, on the GPU in the FP64 reference-order form."""
+    g, ng, N = 5 / 3, 4, 32
+    eig = [x / (2 * math.sqrt(5)) for x in (2, 4, -2, 0, 9, 0, 4, 0)]
+    U, (nx, ny, nz), (dx, dy, dz) = mhd_linear_wave_ic(2 * N, N, N, ng, 1.0, 0.5, 0.5, g, 1.0, 0.0, 1 / g,
+                                                       (1.0, 1.5, 0.0), 1e-6, eig)
+    orc.set_boundaries(U, nx, ny, nz, 8, ng, [1] * 6)
+    U0 = interior(U, 8, nz, ny, nx, ng).copy()
+    say("MHD fast wave 64x32x32")
+    Ud = chl.DeviceArray.from_host(U)
+    h = chl.Mhd(nx, ny, nz, ng, chl.PREC_FP64_EXACT)
+    steps, t = h.run(Ud, dx, dy, dz, g, 0.0, 0.5, 100000)
+    assert steps == 214
+    U1 = interior(Ud.download(), 8, nz, ny, nx, ng)
+    l1 = [np.abs(U1[f] - U0[f]).mean() for f in range(8)]
+    assert max(l1) < 6.5e-8 and math.sqrt(sum(x * x for x in l1)) < 6.5e-8, l1
+    h.close()
