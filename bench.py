@@ -5,6 +5,7 @@ the FP64 forms, with the per-field L1 of the mixed form against FP64.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--forms mixed,fp64,fp64_exact]
     python bench.py --workload ensemble [--boxes 384] [--steps 100]   # batched-ensemble line
+    python bench.py --workload mhd [--n 256] [--steps 150]            # ideal-MHD line
 
 One step = one full VL step (stage 1 + stage 2 with fused floors, dt reduction
 and periodic ghost fill, plus the device-side dt update), graph-captured, state
@@ -41,7 +42,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--kernel-steps", type=int, default=10)
     p.add_argument("--tiles", default="", help="fixed tile1,tile2,kchunk1,kchunk2 (skips the autotuner)")
-    p.add_argument("--workload", default="kh", choices=["kh", "ensemble"])
+    p.add_argument("--workload", default="kh", choices=["kh", "ensemble", "mhd"])
     p.add_argument("--boxes", type=int, default=384, help="ensemble: number of boxes")
     p.add_argument("--looped-steps", type=int, default=10, help="ensemble: steps of the per-box loop baseline")
     p.add_argument("--launch", default="auto", choices=["auto", "split", "persistent"],
@@ -182,6 +183,88 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_mhd(args):
+    """Ideal MHD, Orszag-Tang vortex with the seeded z perturbation (1e-3), n^3 periodic,
+    PLMC + HLLD + CT: cell-updates/s of each form, the mixed form's L1 against FP64 and
+    max |div B| after the timed steps."""
+    import cholla_amd as chl
+
+    n = args.n if args.n != 512 else 256
+    steps = args.steps if args.steps != 200 else 150
+    ng, g = 4, 5.0 / 3.0
+    N = n + 2 * ng
+    h = 1.0 / n
+    cells = n ** 3
+    t0 = time.perf_counter()
+    U0 = chl.ic_orszag_tang(n, ng, g, seed=args.seed, amplitude=1e-3)
+    log(f"Orszag-Tang {n}^3 initial state built on host in {time.perf_counter() - t0:.1f} s")
+    D0 = chl.DeviceArray.from_host(U0)
+    del U0
+    res, finals = {}, {}
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    for form in forms:
+        U = chl.DeviceArray(D0.n)
+        chl.check(chl.lib().chl_memcpy_dtod(U.ptr, D0.ptr, D0.nbytes))
+        hm = chl.Mhd(N, N, N, ng, chl.PRECISIONS[form])
+        hm.time_steps(U, h, h, h, g, args.warmup, use_graph=True)
+        chl.check(chl.lib().chl_synchronize())
+        w0 = time.perf_counter()
+        ms, dt_last = hm.time_steps(U, h, h, h, g, steps, use_graph=True)
+        chl.check(chl.lib().chl_synchronize())
+        wall = time.perf_counter() - w0
+        rate = cells * steps / max(ms * 1e-3, wall)
+        div = chl.mhd_divergence(U, N, N, N, h, h, h)
+        res[form] = {"cell_updates_per_s": rate, "ms_per_step": max(ms, 1e3 * wall) / steps, "max_div_B": div,
+                     "dt_last": dt_last}
+        log(f"[mhd {form}] {rate / 1e9:.3f} G cell-updates/s, {res[form]['ms_per_step']:.2f} ms/step, "
+            f"max|div B| {div:.2e}")
+        finals[form] = U
+        hm.close()
+    l1 = None
+    ref_form = "fp64" if "fp64" in finals else ("fp64_exact" if "fp64_exact" in finals else None)
+    if "mixed" in finals and ref_form:
+        a = finals["mixed"].download().reshape(8, N, N, N)[:, ng:-ng, ng:-ng, ng:-ng]
+        b = finals[ref_form].download().reshape(8, N, N, N)[:, ng:-ng, ng:-ng, ng:-ng]
+        names = ["density", "momentum_x", "momentum_y", "momentum_z", "Energy", "magnetic_x", "magnetic_y",
+                 "magnetic_z"]
+        l1 = {"vs": ref_form, "strict": {}}
+        for f in range(8):
+            l1["strict"][names[f]] = float(np.abs(a[f] - b[f]).sum() / np.abs(b[f]).sum())
+        del a, b
+        log("mhd mixed vs", ref_form, "L1:", json.dumps(l1))
+    head = "mixed" if "mixed" in res else forms[0]
+    r = res[head]
+    out = {
+        "metric": f"cell-updates/s, ideal MHD Orszag-Tang {n}^3 PLMC+HLLD+CT",
+        "value": r["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if head == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": f"Orszag-Tang vortex {n}^3 periodic + 1e-3 z perturbation, PLMC+HLLD+CT, VL, "
+                               f"CFL 0.3, form={head}", "cells": cells, "n_ghost": ng, "gamma": g, "seed": args.seed},
+        "step_roofline_frac": r["cell_updates_per_s"] * 128 / 1e9 / HBM_PEAK_GBS,
+        "forms": res, "l1_mixed_vs_fp64": l1,
+    }
+    if not args.no_cpu:
+        try:
+            from oracle import oracle as orc
+
+            ns = 64
+            Nn = ns + 2 * ng
+            Us = chl.ic_orszag_tang(ns, ng, g, seed=args.seed, amplitude=1e-3)
+            orc.mhd_run(Us, Nn, Nn, Nn, ng, 1 / ns, 1 / ns, 1 / ns, g, [1] * 6, 0.0, 1e30, 1)
+            c0 = time.perf_counter()
+            k, _ = orc.mhd_run(Us, Nn, Nn, Nn, ng, 1 / ns, 1 / ns, 1 / ns, g, [1] * 6, 0.0, 1e30, 4)
+            sec = time.perf_counter() - c0
+            out["cpu_baseline"] = {"value": ns ** 3 * k / sec, "unit": "cell-updates/s",
+                                   "cores": orc.lib().orc_num_threads(), "kind": "port",
+                                   "sample": f"Orszag-Tang {ns}^3 PLMC+HLLD+CT FP64, {k} steps, {sec:.2f} s "
+                                             "(oracle, OpenMP)"}
+        except Exception as e:
+            log("cpu baseline failed:", e)
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy > 0:
@@ -193,12 +276,12 @@ def main():
         sys.exit(2)
     import cholla_amd as chl
 
-    if args.workload == "ensemble":
+    if args.workload in ("ensemble", "mhd"):
         if chl.device_count() < 1:
             log("no HIP device visible")
             sys.exit(1)
         log("device:", chl.device_name())
-        bench_ensemble(args)
+        (bench_ensemble if args.workload == "ensemble" else bench_mhd)(args)
         return
 
     if chl.device_count() < 1:

@@ -63,7 +63,14 @@ __device__ __forceinline__ typename P::T Magnetosonic(typename P::T d, typename 
   T b2  = bx2 + ((by * by) + (bz * bz));
   T t1  = g * p + b2;
   T t2  = (t1 * t1) - T(4.) * g * p * bx2;
-  t2    = Sqrt<P>(t2);
+  if constexpr (!P::kExact) {
+    // (gp + b^2)^2 - 4 gp bx^2 >= (gp - bx^2)^2 >= 0 and t1 >= t2 in exact arithmetic; the
+    // rounding of the reordered forms (FP32 above all) must not turn either into a NaN
+    t2 = Fmax(t2, T(0));
+    t2 = Sqrt<P>(t2);
+    return Sqrt<P>(Fmax(Div1<P>(t1 + choice * t2, T(2.0) * Fmax(d, T(kTinyNumber))), T(0)));
+  }
+  t2 = Sqrt<P>(t2);
   return Sqrt<P>(Div1<P>(t1 + choice * t2, T(2.0) * Fmax(d, T(kTinyNumber))));
 }
 
