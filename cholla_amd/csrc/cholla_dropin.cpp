@@ -44,6 +44,29 @@ struct Slot {
 };
 Slot g_vl3d, g_vl1d;
 
+#ifdef MHD
+// The MHD build's VL integrator (-DMHD -DPLMC -DHLLD -DVL, builds/make.type.mhd).
+struct MhdSlot {
+  chl_mhd *h = nullptr;
+  int nx = 0, ny = 0, nz = 0, ng = 0;
+};
+MhdSlot g_mhd;
+
+chl_mhd *AcquireMhd(int nx, int ny, int nz, int ng)
+{
+  MhdSlot &s = g_mhd;
+  if (s.h && s.nx == nx && s.ny == ny && s.nz == nz && s.ng == ng) return s.h;
+  if (s.h) Check(chl_mhd_destroy(s.h), "chl_mhd_destroy");
+  chl_mhd_config c{};
+  c.nx = nx, c.ny = ny, c.nz = nz, c.n_ghost = ng, c.precision = CHL_PRECISION;
+  c.density_floor = kDensityFloor, c.temperature_floor = 0;
+  for (int q = 0; q < 6; q++) c.bc_flags[q] = CHL_BC_PERIODIC;
+  Check(chl_mhd_create(&c, &s.h), "chl_mhd_create");
+  s.nx = nx, s.ny = ny, s.nz = nz, s.ng = ng;
+  return s.h;
+}
+#endif
+
 chl_hydro *Acquire(Slot &s, int nx, int ny, int nz, int ng, int nf)
 {
   if (s.h && s.nx == nx && s.ny == ny && s.nz == nz && s.ng == ng && s.nf == nf) return s.h;
@@ -67,8 +90,14 @@ void VL_Algorithm_3D_CUDA(Real *d_conserved, Real *d_grav_potential, int nx, int
 {
   (void)d_grav_potential, (void)x_off, (void)y_off, (void)z_off, (void)xbound, (void)ybound, (void)zbound;
   (void)custom_grav, (void)host_grav_potential;  // gravity is outside this path
+#ifdef MHD
+  (void)n_fields;  // 8: the hydro fields and the three face-centred magnetic fields
+  chl_mhd *m = AcquireMhd(nx, ny, nz, n_ghost);
+  Check(chl_mhd_vl_step(m, d_conserved, dx, dy, dz, dt, gama, density_floor), "VL_Algorithm_3D_CUDA");
+#else
   chl_hydro *h = Acquire(g_vl3d, nx, ny, nz, n_ghost, n_fields);
   Check(chl_hydro_vl_step(h, d_conserved, dx, dy, dz, dt, gama, density_floor), "VL_Algorithm_3D_CUDA");
+#endif
 }
 
 // The reference also frees the caller's state here through an alias
@@ -77,6 +106,10 @@ void Free_Memory_VL_3D()
 {
   if (g_vl3d.h) Check(chl_hydro_destroy(g_vl3d.h), "Free_Memory_VL_3D");
   g_vl3d = Slot{};
+#ifdef MHD
+  if (g_mhd.h) Check(chl_mhd_destroy(g_mhd.h), "Free_Memory_VL_3D");
+  g_mhd = MhdSlot{};
+#endif
 }
 
 void VL_Algorithm_1D_CUDA(Real *d_conserved, int nx, int x_off, int n_ghost, Real dx, Real xbound, Real dt,
@@ -97,7 +130,12 @@ Real Calc_dt_GPU(Real *dev_conserved, int nx, int ny, int nz, int n_ghost, int n
                  Real gamma)
 {
   Real dti = 0;
+#ifdef MHD
+  (void)n_fields;  // mhdInverseCrossingTime with the cell-centred field (hydro_cuda.cu:420-435, 546-551)
+  Check(chl_mhd_calc_dt(dev_conserved, nx, ny, nz, n_ghost, dx, dy, dz, gamma, &dti), "Calc_dt_GPU");
+#else
   Check(chl_calc_dt(dev_conserved, nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, &dti), "Calc_dt_GPU");
+#endif
   return dti;
 }
 

@@ -9,8 +9,11 @@
 // (src/grid/boundary_conditions.cpp:413).
 //
 // Build options read at compile time, as the reference's make.type files set them:
-//   PLMC or PPMC, DENSITY_FLOOR, TEMPERATURE_FLOOR, and CHL_PRECISION (0 = FP64 in the
-//   reference's operation order [default], 1 = FP64 fast, 2 = mixed FP32/FP64).
+//   PLMC or PPMC, DENSITY_FLOOR, TEMPERATURE_FLOOR, MHD (with PLMC: the HLLD + CT integrator on
+//   8-field states, builds/make.type.mhd), and CHL_PRECISION (0 = FP64 in the reference's
+//   operation order [default], 1 = FP64 fast, 2 = mixed FP32/FP64).  Under MHD,
+//   mhd::checkMagneticDivergence(G) maps to chl_mhd_divergence(G.C.device, G.H.nx, G.H.ny,
+//   G.H.nz, G.H.dx, G.H.dy, G.H.dz, &div) (src/mhd/magnetic_divergence.cu:79).
 #pragma once
 
 #ifndef TYPEDEF_DEFINED_REAL

@@ -24,11 +24,11 @@ GOLD = json.load(open(os.path.join(HERE, "golden", "reference_unit_vectors.json"
 HDR = "<12i8d"
 
 
-def run_driver(name, U, dims, ng, dxyz, gamma, t, tout, max_steps, flags, tmp_path, U_floor=0.0, dfloor=0.0):
+def run_driver(name, U, dims, ng, dxyz, gamma, t, tout, max_steps, flags, tmp_path, U_floor=0.0, dfloor=0.0, nf=5):
     nx, ny, nz = dims
     fin, fout = os.path.join(tmp_path, "in.bin"), os.path.join(tmp_path, "out.bin")
     with open(fin, "wb") as f:
-        f.write(struct.pack(HDR, nx, ny, nz, ng, 5, max_steps, *flags, *dxyz, gamma, t, tout, U_floor, dfloor))
+        f.write(struct.pack(HDR, nx, ny, nz, ng, nf, max_steps, *flags, *dxyz, gamma, t, tout, U_floor, dfloor))
         f.write(np.ascontiguousarray(U, dtype="<f8").tobytes())
     exe = os.path.join(os.path.dirname(chl.LIB_PATH), name)
     print("[gpu-case] driver", name, dims, flush=True)
@@ -36,7 +36,7 @@ def run_driver(name, U, dims, ng, dxyz, gamma, t, tout, max_steps, flags, tmp_pa
     assert r.returncode == 0, r.stderr
     raw = open(fout, "rb").read()
     hsz = struct.calcsize(HDR)
-    n = 5 * nx * ny * nz
+    n = nf * nx * ny * nz
     A = np.frombuffer(raw[hsz:hsz + 8 * n], dtype="<f8").copy()
     ns, = struct.unpack("<i", raw[hsz + 8 * n:hsz + 8 * n + 4])
     tt, = struct.unpack("<d", raw[hsz + 8 * n + 4:hsz + 8 * n + 12])
@@ -102,4 +102,18 @@ def test_dropin_mixed_boundaries_plmc(tmp_path):
     orc.set_boundaries(Uo, N, N, N, 5, ng, flags)
     steps_o, to = orc.run(Uo, N, N, N, ng, 5, 1 / n, 1 / n, 1 / n, g, orc.PLMC, flags, 0.0, 1e30, 8)
     assert ns == steps_o == 8 and t == to
+    np.testing.assert_array_equal(A, Uo)
+
+
+def test_dropin_mhd_orszag_tang(tmp_path):
+    """The MHD build of the adapter (-DMHD -DPLMC) driven like Cholla's loop: bit-identical to
+    the MHD oracle's driver loop on an Orszag-Tang box."""
+    n, ng, g = 16, 4, 5 / 3
+    N = n + 2 * ng
+    h = 1.0 / n
+    U = chl.ic_orszag_tang(n, ng, g, seed=9)
+    A, ns, t = run_driver("dropin_driver_mhd", U, (N, N, N), ng, (h, h, h), g, 0.0, 1e30, 6, [1] * 6, tmp_path, nf=8)
+    Uo = U.copy()
+    n_o, t_o = orc.mhd_run(Uo, N, N, N, ng, h, h, h, g, [1] * 6, 0.0, 1e30, 6)
+    assert ns == n_o == 6 and t == t_o
     np.testing.assert_array_equal(A, Uo)
