@@ -632,38 +632,35 @@ struct MhdArgs {
 
 __device__ __forceinline__ double MhdDt(const MhdArgs &a) { return a.dt_dev ? *a.dt_dev : a.dt; }
 
-// Stage-1 fluxes: PCM states (the cells, with their cell-centred fields) + HLLD on the
-// face i+1/2 of cell i along every axis whose face lies inside the grid.
-template <class P>
+// Stage-1 fluxes along DIR: PCM states (the cells, with their cell-centred fields) + HLLD on
+// the face i+1/2 of every cell whose face lies inside the grid.  One launch per direction.
+template <class P, int DIR>
 __global__ void __launch_bounds__(256) MhdFlux1Kernel(MhdArgs a)
 {
   using T         = typename P::T;
   const int nx    = a.nx, ny = a.ny, nz = a.nz;
   const size_t n  = size_t(nx) * ny * nz;
   const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if ((a.flags & kFlagReduceDt) && id == 0) *a.dti = 0ull;  // MhdCalcDtKernel (after this step) fills it
+  if (DIR == 0 && (a.flags & kFlagReduceDt) && id == 0) *a.dti = 0ull;  // MhdCalcDtKernel fills it later
   if (id >= n) return;
   if (MhdDt(a) == 0.0) return;
   const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
-  const double *U = a.U;
-  const T g       = T(a.gamma);
-  const int c[3] = {x, y, z}, nn[3] = {nx, ny, nz};
-  for (int dir = 0; dir < 3; dir++) {
-    if (c[dir] > nn[dir] - 2) continue;
-    const size_t step = dir == 0 ? 1 : (dir == 1 ? size_t(nx) : size_t(nx) * ny);
-    auto pcm = [&](int xx, int yy, int zz, size_t cid) {
-      double mx, my, mz, cx, cy, cz, bx, by, bz;
-      RotM(dir, U[n + cid], U[2 * n + cid], U[3 * n + cid], mx, my, mz);
-      CellB(U, cid, xx, yy, zz, n, nx, ny, cx, cy, cz);
-      RotM(dir, cx, cy, cz, bx, by, bz);
-      return ConsM<T>{T(U[cid]), T(mx), T(my), T(mz), T(U[4 * n + cid]), T(by), T(bz)};
-    };
-    const ConsM<T> qL = pcm(x, y, z, id);
-    const ConsM<T> qR = pcm(x + (dir == 0), y + (dir == 1), z + (dir == 2), id + step);
-    ConsM<T> f;
-    if (HlldM<P>(qL, qR, T(U[(kBx + dir) * n + id]), g, f))
-      StoreFluxM(reinterpret_cast<T *>(a.F[dir]), id, n, dir, f);
-  }
+  const int c = DIR == 0 ? x : (DIR == 1 ? y : z), nn = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
+  if (c > nn - 2) return;
+  const double *U   = a.U;
+  const T g         = T(a.gamma);
+  const size_t step = DIR == 0 ? 1 : (DIR == 1 ? size_t(nx) : size_t(nx) * ny);
+  auto pcm = [&](int xx, int yy, int zz, size_t cid) {
+    double mx, my, mz, cx, cy, cz, bx, by, bz;
+    RotM(DIR, U[n + cid], U[2 * n + cid], U[3 * n + cid], mx, my, mz);
+    CellB(U, cid, xx, yy, zz, n, nx, ny, cx, cy, cz);
+    RotM(DIR, cx, cy, cz, bx, by, bz);
+    return ConsM<T>{T(U[cid]), T(mx), T(my), T(mz), T(U[4 * n + cid]), T(by), T(bz)};
+  };
+  const ConsM<T> qL = pcm(x, y, z, id);
+  const ConsM<T> qR = pcm(x + (DIR == 0), y + (DIR == 1), z + (DIR == 2), id + step);
+  ConsM<T> f;
+  if (HlldM<P>(qL, qR, T(U[(kBx + DIR) * n + id]), g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[DIR]), id, n, DIR, f);
 }
 
 // Half update: hydro fields of cells [1, n-2] (Update_Conserved_Variables_3D_half,
@@ -721,9 +718,9 @@ __global__ void __launch_bounds__(256) MhdHalfKernel(MhdArgs a)
   }
 }
 
-// Stage-2 fluxes on faces with every index in [lo, hi]: PLMC of the two cells of the
-// face (PLMC_cuda writes them to Q_L at i and Q_R at i, plmc_cuda.cu:292-298) + HLLD.
-template <class P>
+// Stage-2 fluxes along DIR on faces with every index in [lo, hi]: PLMC of the two cells of
+// the face (PLMC_cuda writes them to Q_L at i and Q_R at i, plmc_cuda.cu:292-298) + HLLD.
+template <class P, int DIR>
 __global__ void __launch_bounds__(256) MhdFlux2Kernel(MhdArgs a, int lo, int hix, int hiy, int hiz)
 {
   using T         = typename P::T;
@@ -736,15 +733,13 @@ __global__ void __launch_bounds__(256) MhdFlux2Kernel(MhdArgs a, int lo, int hix
   if (x < lo || y < lo || z < lo || x > hix || y > hiy || z > hiz) return;
   const double *Uh = a.Uh;
   const T g        = T(a.gamma);
-  for (int dir = 0; dir < 3; dir++) {
-    PrimM<T> Lp, Rm, tmp;
-    PlmcCellM<P>(Uh, x, y, z, nx, ny, n, dir, a.gamma, Lp, tmp);
-    PlmcCellM<P>(Uh, x + (dir == 0), y + (dir == 1), z + (dir == 2), nx, ny, n, dir, a.gamma, tmp, Rm);
-    const T bx        = T(Uh[(kBx + dir) * n + id]);
-    const ConsM<T> qL = WriteDataM<P>(Lp, bx, g), qR = WriteDataM<P>(Rm, bx, g);
-    ConsM<T> f;
-    if (HlldM<P>(qL, qR, bx, g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[dir]), id, n, dir, f);
-  }
+  PrimM<T> Lp, Rm, tmp;
+  PlmcCellM<P>(Uh, x, y, z, nx, ny, n, DIR, a.gamma, Lp, tmp);
+  PlmcCellM<P>(Uh, x + (DIR == 0), y + (DIR == 1), z + (DIR == 2), nx, ny, n, DIR, a.gamma, tmp, Rm);
+  const T bx        = T(Uh[(kBx + DIR) * n + id]);
+  const ConsM<T> qL = WriteDataM<P>(Lp, bx, g), qR = WriteDataM<P>(Rm, bx, g);
+  ConsM<T> f;
+  if (HlldM<P>(qL, qR, bx, g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[DIR]), id, n, DIR, f);
 }
 
 // Full update (Update_Conserved_Variables_3D, hydro_cuda.cu:172-402, and

@@ -21,11 +21,17 @@ hipError_t MhdStepT(const mhd::MhdArgs &a, hipStream_t s)
   const T *Fx = reinterpret_cast<const T *>(a.F[0]), *Fy = reinterpret_cast<const T *>(a.F[1]),
           *Fz = reinterpret_cast<const T *>(a.F[2]);
   const int ng = a.ng;
-  hipLaunchKernelGGL((mhd::MhdFlux1Kernel<P>), dim3(g), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((mhd::MhdFlux1Kernel<P, 0>), dim3(g), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((mhd::MhdFlux1Kernel<P, 1>), dim3(g), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((mhd::MhdFlux1Kernel<P, 2>), dim3(g), dim3(256), 0, s, a);
   hipLaunchKernelGGL((mhd::MhdEmfKernel<T>), dim3(g), dim3(256), 0, s, Fx, Fy, Fz, a.U, a.E, a.nx, a.ny, a.nz, 2,
                      a.nx - 1, a.ny - 1, a.nz - 1, a.dt_dev);
   hipLaunchKernelGGL((mhd::MhdHalfKernel<P>), dim3(g), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((mhd::MhdFlux2Kernel<P>), dim3(g), dim3(256), 0, s, a, ng - 1, a.nx - ng, a.ny - ng,
+  hipLaunchKernelGGL((mhd::MhdFlux2Kernel<P, 0>), dim3(g), dim3(256), 0, s, a, ng - 1, a.nx - ng, a.ny - ng,
+                     a.nz - ng);
+  hipLaunchKernelGGL((mhd::MhdFlux2Kernel<P, 1>), dim3(g), dim3(256), 0, s, a, ng - 1, a.nx - ng, a.ny - ng,
+                     a.nz - ng);
+  hipLaunchKernelGGL((mhd::MhdFlux2Kernel<P, 2>), dim3(g), dim3(256), 0, s, a, ng - 1, a.nx - ng, a.ny - ng,
                      a.nz - ng);
   hipLaunchKernelGGL((mhd::MhdEmfKernel<T>), dim3(g), dim3(256), 0, s, Fx, Fy, Fz, a.Uh, a.E, a.nx, a.ny, a.nz, ng,
                      a.nx - ng, a.ny - ng, a.nz - ng, a.dt_dev);
