@@ -49,7 +49,7 @@ class HydroConfig(ctypes.Structure):
         ("n_fields", ctypes.c_int), ("reconstruction", ctypes.c_int), ("riemann", ctypes.c_int),
         ("precision", ctypes.c_int), ("density_floor", ctypes.c_int), ("temperature_floor", ctypes.c_int),
         ("bc_flags", ctypes.c_int * 6), ("tile1", ctypes.c_int), ("tile2", ctypes.c_int),
-        ("kchunk1", ctypes.c_int), ("kchunk2", ctypes.c_int),
+        ("kchunk1", ctypes.c_int), ("kchunk2", ctypes.c_int), ("tile_order", ctypes.c_int),
     ]
 
 
@@ -214,7 +214,7 @@ class Hydro:
 
     def __init__(self, nx, ny, nz, n_ghost, reconstruction=RECON_PPMC, precision=PREC_FP64_EXACT,
                  bc=(1, 1, 1, 1, 1, 1), density_floor=True, temperature_floor=True, tile1=-1, tile2=-1,
-                 kchunk1=0, kchunk2=0):
+                 kchunk1=0, kchunk2=0, tile_order=0):
         cfg = HydroConfig()
         cfg.nx, cfg.ny, cfg.nz, cfg.n_ghost, cfg.n_fields = nx, ny, nz, n_ghost, 5
         cfg.reconstruction, cfg.riemann, cfg.precision = reconstruction, RIEMANN_HLLC, precision
@@ -222,6 +222,7 @@ class Hydro:
         for q in range(6):
             cfg.bc_flags[q] = bc[q]
         cfg.tile1, cfg.tile2, cfg.kchunk1, cfg.kchunk2 = tile1, tile2, kchunk1, kchunk2
+        cfg.tile_order = tile_order
         h = _vp()
         check(lib().chl_hydro_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

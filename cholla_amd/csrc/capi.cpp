@@ -84,6 +84,7 @@ int CheckConfig(const chl_hydro_config &c)
       // any precision is fine in 3D
     }
     if (c.tile1 >= chl::kNumTiles || c.tile2 >= chl::kNumTiles) return Fail(CHL_ERR_INVALID, "tile index out of range");
+    if (c.tile_order != 0 && c.tile_order != 1) return Fail(CHL_ERR_INVALID, "tile_order must be 0 or 1");
   } else {
     if (c.precision != CHL_PREC_FP64_EXACT)
       return Fail(CHL_ERR_UNSUPPORTED, "1D boxes run the FP64 reference-order form only");
@@ -127,6 +128,7 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
   const bool fused_ghosts = fill && h->PeriodicFusable();
   if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
   a.flags       = flags;
+  a.xcd_order   = c.tile_order == 0;
   const int R1  = ReconRadius(c.reconstruction) + 1;
   const int pol = c.precision;
   a.kchunk      = c.kchunk1;
@@ -420,6 +422,7 @@ int chl_hydro_kernel_times(chl_hydro *h, double *U, double dx, double dy, double
   const bool fused_ghosts = h->PeriodicFusable();
   if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
   a.flags       = flags;
+  a.xcd_order   = c.tile_order == 0;
   const int R1  = ReconRadius(c.reconstruction) + 1;
   const int pol = c.precision;
   for (int s = 0; s < n_steps; s++) {
@@ -467,6 +470,7 @@ int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, doub
   a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dti = h->dti();
   a.flags = chl::kFlagReduceDt | (c.density_floor ? chl::kFlagDensityFloor : 0) |
             (c.temperature_floor ? chl::kFlagTemperatureFloor : 0);
+  a.xcd_order = c.tile_order == 0;
   // a small fixed dt keeps every candidate on the same, physical state
   {
     double dti = 0;

@@ -99,15 +99,6 @@ __device__ __forceinline__ void TileCoords(int l, const int t[3], int &bx, int &
   bz = r / t[1];
 }
 
-// Workgroups are handed to the 8 XCDs round-robin by id; this gives each XCD a
-// contiguous run of work items, so neighbouring tiles (which share halo cells)
-// are resident in the same L2.  The grid is padded to a multiple of 8.
-__device__ __forceinline__ int XcdItem(int g, int n_items)
-{
-  const int per = (n_items + 7) / 8;
-  return (g & 7) * per + (g >> 3);
-}
-
 // Per-box dt from the previous step's reduction, as DtAdvanceKernel does for one box.
 __global__ void EnsembleClockKernel(BoxClock *clk, const BoxDesc *box, int n_boxes, double cfl, unsigned int *head);
 

@@ -59,6 +59,8 @@ struct StepArgs {
   double U_floor;
   int flags;               // kFlag* bits
   int kchunk;              // z cells per workgroup
+  int gx, gy, gz;          // tile grid of the launch (set by the launcher)
+  int xcd_order;           // 1: give each XCD a contiguous run of tiles (see TileOf)
 };
 
 enum : int {
@@ -74,6 +76,29 @@ __device__ __forceinline__ size_t Lin(int i, int j, int k, int nx, int ny)
 }
 
 __device__ __forceinline__ double StepDt(const StepArgs &a) { return a.dt_dev ? *a.dt_dev : a.dt; }
+
+// Workgroups are handed to the 8 XCDs round-robin by id; this gives each XCD a
+// contiguous run of work items, so neighbouring tiles (which share halo cells)
+// are resident in the same L2.  The grid is padded to a multiple of 8.
+__device__ __forceinline__ int XcdItem(int g, int n_items)
+{
+  const int per = (n_items + 7) / 8;
+  return (g & 7) * per + (g >> 3);
+}
+
+// Tile (bx, by, bz) of this workgroup in a 1D launch over the gx x gy x gz tile grid.
+__device__ __forceinline__ bool TileOf(const StepArgs &a, int &bx, int &by, int &bz)
+{
+  const int T = a.gx * a.gy * a.gz;
+  int q       = int(blockIdx.x);
+  if (a.xcd_order) q = XcdItem(q, T);
+  if (q >= T) return false;
+  bx    = q % a.gx;
+  int r = q / a.gx;
+  by    = r % a.gy;
+  bz    = r / a.gy;
+  return true;
+}
 
 // Rotate a global-order (x,y,z) triple into the sweep frame of dir.
 template <int DIR, typename T>
@@ -298,9 +323,10 @@ __global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
 {
   __shared__ Stage1Smem<P, TX, TY> sm;
   // the inverse-dt accumulator is reset here; stage 2 (next launch) fills it
-  if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0)
-    *a.dti = 0ull;
-  Stage1Tile<P, TX, TY>(a, R1, blockIdx.x, blockIdx.y, blockIdx.z, sm);
+  if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && threadIdx.x == 0) *a.dti = 0ull;
+  int bx, by, bz;
+  if (!TileOf(a, bx, by, bz)) return;
+  Stage1Tile<P, TX, TY>(a, R1, bx, by, bz, sm);
 }
 
 // ============================================================================ stage 2
@@ -597,7 +623,9 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Sta
 Stage2Kernel(StepArgs a)
 {
   __shared__ Stage2Smem<P, RK, TX, TY> sm;
-  Stage2Tile<P, RK, TX, TY>(a, blockIdx.x, blockIdx.y, blockIdx.z, sm);
+  int bx, by, bz;
+  if (!TileOf(a, bx, by, bz)) return;
+  Stage2Tile<P, RK, TX, TY>(a, bx, by, bz, sm);
 }
 
 }  // namespace chl

@@ -18,12 +18,22 @@ constexpr TileShape kTiles[kNumTiles] = {{32, 8}, {16, 16}, {64, 4}, {16, 8}, {3
 
 inline int CeilDiv(int a, int b) { return (a + b - 1) / b; }
 
+// 1D launch over the tile grid (gx, gy, gz); padded to a multiple of 8 for the
+// XCD-contiguous order.
+inline unsigned TileGrid(StepArgs &b, int gx, int gy, int gz)
+{
+  b.gx = gx, b.gy = gy, b.gz = gz;
+  const long T = long(gx) * gy * gz;
+  return unsigned(b.xcd_order ? (T + 7) / 8 * 8 : T);
+}
+
 template <class P, int TX, int TY>
 hipError_t LaunchStage1T(const StepArgs &a, int R1, hipStream_t s)
 {
   int ox = a.nx - 2 * a.ng + 2 * R1, oy = a.ny - 2 * a.ng + 2 * R1, oz = a.nz - 2 * a.ng + 2 * R1;
-  dim3 grid(CeilDiv(ox, TX), CeilDiv(oy, TY), CeilDiv(oz, a.kchunk));
-  hipLaunchKernelGGL((Stage1Kernel<P, TX, TY>), grid, dim3(TX * TY), 0, s, a, R1);
+  StepArgs b     = a;
+  unsigned grid  = TileGrid(b, CeilDiv(ox, TX), CeilDiv(oy, TY), CeilDiv(oz, a.kchunk));
+  hipLaunchKernelGGL((Stage1Kernel<P, TX, TY>), dim3(grid), dim3(TX * TY), 0, s, b, R1);
   return hipGetLastError();
 }
 
@@ -31,8 +41,9 @@ template <class P, int RK, int TX, int TY>
 hipError_t LaunchStage2T(const StepArgs &a, hipStream_t s)
 {
   int ox = a.nx - 2 * a.ng, oy = a.ny - 2 * a.ng, oz = a.nz - 2 * a.ng;
-  dim3 grid(CeilDiv(ox, TX), CeilDiv(oy, TY), CeilDiv(oz, a.kchunk));
-  hipLaunchKernelGGL((Stage2Kernel<P, RK, TX, TY>), grid, dim3(TX * TY), 0, s, a);
+  StepArgs b    = a;
+  unsigned grid = TileGrid(b, CeilDiv(ox, TX), CeilDiv(oy, TY), CeilDiv(oz, a.kchunk));
+  hipLaunchKernelGGL((Stage2Kernel<P, RK, TX, TY>), dim3(grid), dim3(TX * TY), 0, s, b);
   return hipGetLastError();
 }
 

@@ -58,6 +58,7 @@ typedef struct chl_hydro_config {
   int bc_flags[6];       /* xl, xu, yl, yu, zl, zu */
   int tile1, tile2;      /* tile shapes of the two fused stages, -1 = default */
   int kchunk1, kchunk2;  /* z cells per workgroup in the two stages, 0 = default */
+  int tile_order;        /* 0: each XCD takes a contiguous run of tiles (default); 1: dispatch order */
 } chl_hydro_config;
 
 typedef struct chl_hydro chl_hydro; /* owns the integrator scratch of one box */
