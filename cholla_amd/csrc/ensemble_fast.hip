@@ -1,5 +1,5 @@
 // ensemble_fast.hip -- PolicyFast and PolicyMixed instantiations of the ensemble
-// kernels (FMA contraction allowed, like hydro_fast.hip), and the policy dispatch.
+// kernels (built with -ffp-contract=on, like hydro_fast.hip), and the policy dispatch.
 #include "ensemble_launch.hpp"
 
 namespace chl

@@ -1,6 +1,6 @@
 // hydro_fast.hip -- PolicyFast (FP64) and PolicyMixed (FP32 reconstruction and
 // Riemann solve, FP64 state/accumulation/pressure recovery/dt) instantiations.
-// FMA contraction is allowed here.
+// Built with -ffp-contract=on (FMA only within one expression).
 #include "vl3d_launch.hpp"
 
 namespace chl

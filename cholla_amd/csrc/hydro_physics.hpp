@@ -6,7 +6,9 @@
 //                compiled with -ffp-contract=off so results are bit-identical to
 //                a strict IEEE restatement of the reference kernels.
 //   PolicyFast   FP64, divisions by a common denominator replaced by one
-//                reciprocal and products, FMA contraction allowed.
+//                reciprocal and products; a product and a sum in one expression
+//                fuse into an FMA (-ffp-contract=on, so every kernel that inlines
+//                a tile body rounds it the same way).
 //   PolicyMixed  FP32 reconstruction / characteristic projection / Riemann
 //                solve with hardware rcp and sqrt; the conserved state, its
 //                flux-difference accumulation, pressure recovery and dt stay FP64

@@ -1,6 +1,6 @@
 // mhd_fast.hip -- PolicyFast (FP64) and PolicyMixed (FP32 reconstruction and HLLD, FP64
 // state, accumulation, edge fields and dt) instantiations of the MHD step, and the
-// policy dispatch.  FMA contraction is allowed here.
+// policy dispatch.  Built with -ffp-contract=on (FMA only within one expression).
 #include "mhd_launch.hpp"
 
 namespace chl
