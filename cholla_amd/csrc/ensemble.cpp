@@ -592,7 +592,7 @@ int chl_turbulence_modes(uint64_t seed, double *modes, int max_modes, int *n_mod
         normal_pair(v[2], v[3]);
         normal_pair(v[4], v[5]);
         if (modes && m < max_modes) {
-          const double sigma = 1.0 / std::sqrt(double(k2));
+          const double sigma = 1.0 / double(k2);  // |k|^-2: shell spectrum E(k) ~ k^2 |k|^-4 = k^-2
           double *md         = modes + 9 * m;
           md[0] = kx, md[1] = ky, md[2] = kz;
           for (int c = 0; c < 2; c++) {

@@ -257,9 +257,10 @@ int chl_ensemble_layout(uint64_t seed, int n_boxes, int *edges, double *mach);
  * 1 <= |k| <= 4 in the half space (kx > 0, or kx == 0 and ky > 0, or kx == ky == 0
  * and kz > 0), in the order kx, ky, kz ascending (kz fastest).  Per mode two
  * 3-vectors a, b of independent normals (Box-Muller on std::mt19937_64(seed),
- * drawn ax, ay, az, bx, by, bz) with standard deviation |k|^-1, so the
- * per-mode power is proportional to k^-2, then projected solenoidal
- * (a -= k (k.a)/|k|^2, likewise b).  modes: 9 doubles per mode (k, a, b). */
+ * drawn ax, ay, az, bx, by, bz) with standard deviation |k|^-2, so the per-mode
+ * power is proportional to k^-4 and the shell-summed spectrum E(k) (about 4 pi k^2
+ * modes per unit shell) to k^-2, then projected solenoidal (a -= k (k.a)/|k|^2,
+ * likewise b).  modes: 9 doubles per mode (k, a, b). */
 int chl_turbulence_modes(uint64_t seed, double *modes, int max_modes, int *n_modes);
 /* Periodic turbulence box on the device: n^3 real cells + ng ghosts, unit box,
  * rho and p uniform, v = s * sum_m [a_m cos(2 pi k_m.x) + b_m sin(2 pi k_m.x)] at

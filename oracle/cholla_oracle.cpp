@@ -12,7 +12,9 @@
 // and sum rounds as written (IEEE double, no fused multiply-add).
 //
 // Layout: U[f*n_cells + i + j*nx + k*nx*ny], field order density, momentum_x,
-// momentum_y, momentum_z, Energy (src/grid/grid_enum.h:20-30).
+// momentum_y, momentum_z, Energy (src/grid/grid_enum.h:20-30), and GasEnergy as
+// field 5 when the dual-energy option (the reference's -DDE build) is switched on
+// with orc_set_dual_energy (grid_enum.h:60-72 with no scalars and no MHD).
 //
 // Parity pins (which golden vector or invariant checks which function) are
 // listed in tests/test_oracle.py; every function here is checked by at least
@@ -35,6 +37,13 @@ namespace
 
 constexpr double kTiny = 1.0e-20;  // TINY_NUMBER, src/global/global.h:25
 constexpr double kCfl  = 0.3;      // C_cfl, src/grid/grid3D.cpp:157
+constexpr double kEta1 = 0.001;    // DE_ETA_1 (non-cosmology), src/global/global.h:62-69
+constexpr double kEta2 = 0.035;    // DE_ETA_2, src/global/global.h:71-73
+constexpr int kGE      = 5;        // grid_enum::GasEnergy with DE, no scalars, no MHD
+
+// The DE build option as a run-time switch (every stage below follows its #ifdef DE
+// branches when set).
+int g_de = 0;
 
 inline size_t Idx(int i, int j, int k, int nx, int ny)
 {
@@ -49,6 +58,7 @@ inline double SignOf(double a) { return (a < 0.) ? -1. : 1.; }
 
 struct Prim {
   double d, vx, vy, vz, p;
+  double ge = 0.0;  // specific gas energy GasEnergy/d (DE only)
 };
 struct Char {
   double a0, a1, a2, a3, a4;
@@ -80,7 +90,18 @@ inline double EnergyPrimitive(double P, double d, double vx, double vy, double v
   return (std::fmax(P, kTiny) / (gamma - 1.)) + 0.5 * d * SquareMagnitude(vx, vy, vz);
 }
 
-// reconstruction::Load_Data, src/reconstruction/reconstruction.h:139-212 (hydro, no DE)
+// hydro_utilities::Get_Pressure_From_DE, src/utils/hydro_utilities.h:136-151
+inline double PressureFromDE(double E, double U_total, double U_advected, double gamma)
+{
+  double U = (U_total / E > kEta1) ? U_total : U_advected;
+  double P = U * (gamma - 1.0);
+  return std::fmax(P, kTiny);
+}
+
+// hydro_utilities::Calc_Kinetic_Energy_From_Velocity, src/utils/hydro_utilities.h:162-166
+inline double KineticEnergy(double d, double vx, double vy, double vz) { return 0.5 * d * SquareMagnitude(vx, vy, vz); }
+
+// reconstruction::Load_Data, src/reconstruction/reconstruction.h:139-212 (hydro; DE branch :179-191)
 Prim LoadData(const double *U, int xid, int yid, int zid, int nx, int ny, size_t n_cells, int o1, int o2, int o3,
               double gamma)
 {
@@ -90,21 +111,31 @@ Prim LoadData(const double *U, int xid, int yid, int zid, int nx, int ny, size_t
   q.vx = U[o1 * n_cells + id] / q.d;
   q.vy = U[o2 * n_cells + id] / q.d;
   q.vz = U[o3 * n_cells + id] / q.d;
-  q.p  = PressurePrimitive(U[4 * n_cells + id], q.d, q.vx, q.vy, q.vz, gamma);
+  if (g_de) {
+    double E  = U[4 * n_cells + id];
+    double ge = U[kGE * n_cells + id];
+    double Ek = KineticEnergy(q.d, q.vx, q.vy, q.vz);
+    q.p       = PressureFromDE(E, E - Ek, ge, gamma);
+    q.ge      = ge / q.d;
+  } else {
+    q.p = PressurePrimitive(U[4 * n_cells + id], q.d, q.vx, q.vy, q.vz, gamma);
+  }
   return q;
 }
 
 // reconstruction::Compute_Slope, reconstruction.h:224-251
 Prim Slope(const Prim &l, const Prim &r, double coef = 1.0)
 {
-  return {coef * (r.d - l.d), coef * (r.vx - l.vx), coef * (r.vy - l.vy), coef * (r.vz - l.vz), coef * (r.p - l.p)};
+  return {coef * (r.d - l.d),   coef * (r.vx - l.vx), coef * (r.vy - l.vy),
+          coef * (r.vz - l.vz), coef * (r.p - l.p),   coef * (r.ge - l.ge)};
 }
 
 // reconstruction::Van_Leer_Slope, reconstruction.h:262-296
 inline double VlSlope(double l, double r) { return (l * r > 0.0) ? 2.0 * l * r / (l + r) : 0.0; }
 Prim VanLeer(const Prim &l, const Prim &r)
 {
-  return {VlSlope(l.d, r.d), VlSlope(l.vx, r.vx), VlSlope(l.vy, r.vy), VlSlope(l.vz, r.vz), VlSlope(l.p, r.p)};
+  return {VlSlope(l.d, r.d),   VlSlope(l.vx, r.vx), VlSlope(l.vy, r.vy),
+          VlSlope(l.vz, r.vz), VlSlope(l.p, r.p),   VlSlope(l.ge, r.ge)};
 }
 
 // reconstruction::Primitive_To_Characteristic (hydro branch), reconstruction.h:423-431
@@ -159,7 +190,7 @@ Prim MonotonizeReturnPrim(const Prim &q, const Char &L, const Char &R, const Cha
 Prim InterfaceLinear(const Prim &q, const Prim &s, double sign)
 {
   auto f = [&](double state, double slope) { return state + sign * 0.5 * slope; };
-  return {f(q.d, s.d), f(q.vx, s.vx), f(q.vy, s.vy), f(q.vz, s.vz), f(q.p, s.p)};
+  return {f(q.d, s.d), f(q.vx, s.vx), f(q.vy, s.vy), f(q.vz, s.vz), f(q.p, s.p), f(q.ge, s.ge)};
 }
 
 // reconstruction::Plm_Limit_Interfaces, reconstruction.h:686-723
@@ -176,6 +207,7 @@ void PlmLimit(Prim &L, Prim &R, const Prim &im, const Prim &i, const Prim &ip)
   lim(L.vy, R.vy, im.vy, i.vy, ip.vy);
   lim(L.vz, R.vz, im.vz, i.vz, ip.vz);
   lim(L.p, R.p, im.p, i.p, ip.p);
+  if (g_de) lim(L.ge, R.ge, im.ge, i.ge, ip.ge);
 }
 
 // reconstruction::PPM_Single_Variable, reconstruction.h:794-901
@@ -245,6 +277,7 @@ void WriteData(const Prim &s, double *Q, size_t id, size_t n_cells, int o1, int 
   Q[o2 * n_cells + id] = s.d * s.vy;
   Q[o3 * n_cells + id] = s.d * s.vz;
   Q[4 * n_cells + id]  = EnergyPrimitive(s.p, s.d, s.vx, s.vy, s.vz, gamma);
+  if (g_de) Q[kGE * n_cells + id] = s.d * s.ge;  // reconstruction.h:960-962
 }
 
 void DirOrder(int dir, int &o1, int &o2, int &o3)
@@ -407,6 +440,7 @@ void orc_plmc(const double *U, double *QL, double *QR, int nx, int ny, int nz, d
         Char aL = PrimToChar(ci, dL, c, c2), aR = PrimToChar(ci, dR, c, c2), aC = PrimToChar(ci, dC, c, c2),
              aG = PrimToChar(ci, dG, c, c2);
         Prim dm = MonotonizeReturnPrim(ci, aL, aR, aC, aG, c, c2);
+        if (g_de) dm.ge = MonotonizeOne(dL.ge, dR.ge, dC.ge, dG.ge);  // reconstruction.h:549-551
         Prim L  = InterfaceLinear(ci, dm, 1.0);
         Prim R  = InterfaceLinear(ci, dm, -1.0);
         PlmLimit(L, R, cim, ci, cip);
@@ -419,7 +453,7 @@ void orc_plmc(const double *U, double *QL, double *QR, int nx, int ny, int nz, d
       }
 }
 
-// PPMC_VL, src/reconstruction/ppmc_cuda.cu:540-695 (hydro, no DE/scalars)
+// PPMC_VL, src/reconstruction/ppmc_cuda.cu:540-695 (hydro, no scalars; DE branch :668-672)
 void orc_ppmc_vl(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
 {
   size_t n_cells = size_t(nx) * ny * nz;
@@ -448,6 +482,7 @@ void orc_ppmc_vl(const double *U, double *QL, double *QR, int nx, int ny, int nz
         PpmSingle(am2.a4, am1.a4, a0.a4, ap1.a4, ap2.a4, L.a4, R.a4);
         Prim pL = CharToPrim(c0, L, c, c2);
         Prim pR = CharToPrim(c0, R, c, c2);
+        if (g_de) PpmSingle(cm2.ge, cm1.ge, c0.ge, cp1.ge, cp2.ge, pL.ge, pR.ge);
         pR.d    = std::fmax(pR.d, kTiny);
         pL.d    = std::fmax(pL.d, kTiny);
         pR.p    = std::fmax(pR.p, kTiny);
@@ -457,7 +492,8 @@ void orc_ppmc_vl(const double *U, double *QL, double *QR, int nx, int ny, int nz
       }
 }
 
-// Calculate_HLLC_Fluxes_CUDA, src/riemann_solvers/hllc_cuda.cu:19-307 (no DE/scalars)
+// Calculate_HLLC_Fluxes_CUDA, src/riemann_solvers/hllc_cuda.cu:19-307 (no scalars; DE
+// branches :85-138, :183-198, :218-235, :251-303)
 void orc_hllc(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma, int dir)
 {
   size_t n_cells = size_t(nx) * ny * nz;
@@ -471,12 +507,28 @@ void orc_hllc(const double *QL, const double *QR, double *F, int nx, int ny, int
     double dr = QR[t], mxr = QR[o1 * n_cells + t], myr = QR[o2 * n_cells + t], mzr = QR[o3 * n_cells + t],
            Er = QR[4 * n_cells + t];
 
+    double dgel = g_de ? QL[kGE * n_cells + t] : 0.0, dger = g_de ? QR[kGE * n_cells + t] : 0.0;
+
     double vxl = mxl / dl, vyl = myl / dl, vzl = mzl / dl;
-    double pl  = (El - 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl)) * (gamma - 1.0);
+    double pl;
+    if (g_de) {
+      double E_kin = 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl);
+      pl           = PressureFromDE(El, El - E_kin, dgel, gamma);
+    } else {
+      pl = (El - 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl)) * (gamma - 1.0);
+    }
     pl         = std::fmax(pl, kTiny);
+    double gel = g_de ? dgel / dl : 0.0;
     double vxr = mxr / dr, vyr = myr / dr, vzr = mzr / dr;
-    double pr  = (Er - 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr)) * (gamma - 1.0);
+    double pr;
+    if (g_de) {
+      double E_kin = 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr);
+      pr           = PressureFromDE(Er, Er - E_kin, dger, gamma);
+    } else {
+      pr = (Er - 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr)) * (gamma - 1.0);
+    }
     pr         = std::fmax(pr, kTiny);
+    double ger = g_de ? dger / dr : 0.0;
 
     double Hl = (El + pl) / dl;
     double Hr = (Er + pr) / dr;
@@ -500,11 +552,13 @@ void orc_hllc(const double *QL, const double *QR, double *F, int nx, int ny, int
     double f_d_l = mxl, f_mx_l = mxl * vxl + pl, f_my_l = myl * vxl, f_mz_l = mzl * vxl, f_E_l = (El + pl) * vxl;
     double f_d_r = mxr, f_mx_r = mxr * vxr + pr, f_my_r = myr * vxr, f_mz_r = mzr * vxr, f_E_r = (Er + pr) * vxr;
 
-    double fd, fmx, fmy, fmz, fE;
+    double f_ge_l = dgel * vxl, f_ge_r = dger * vxr;
+
+    double fd, fmx, fmy, fmz, fE, fge = 0.0;
     if (Sl > 0.0) {
-      fd = f_d_l, fmx = f_mx_l, fmy = f_my_l, fmz = f_mz_l, fE = f_E_l;
+      fd = f_d_l, fmx = f_mx_l, fmy = f_my_l, fmz = f_mz_l, fE = f_E_l, fge = f_ge_l;
     } else if (Sr < 0.0) {
-      fd = f_d_r, fmx = f_mx_r, fmy = f_my_r, fmz = f_mz_r, fE = f_E_r;
+      fd = f_d_r, fmx = f_mx_r, fmy = f_my_r, fmz = f_mz_r, fE = f_E_r, fge = f_ge_r;
     } else {
       double Sm = (dr * vxr * (Sr - vxr) - dl * vxl * (Sl - vxl) + pl - pr) / (dr * (Sr - vxr) - dl * (Sl - vxl));
       double ps = dl * (vxl - Sl) * (vxl - Sm) + pl;
@@ -522,39 +576,100 @@ void orc_hllc(const double *QL, const double *QR, double *F, int nx, int ny, int
       fmy = 0.5 * (f_my_l + f_my_r + (Sr - aSm) * myrs + (Sl + aSm) * myls - Sl * myl - Sr * myr);
       fmz = 0.5 * (f_mz_l + f_mz_r + (Sr - aSm) * mzrs + (Sl + aSm) * mzls - Sl * mzl - Sr * mzr);
       fE  = 0.5 * (f_E_l + f_E_r + (Sr - aSm) * Ers + (Sl + aSm) * Els - Sl * El - Sr * Er);
+      if (g_de) {
+        double gels = dls * gel, gers = drs * ger;
+        fge         = 0.5 * (f_ge_l + f_ge_r + (Sr - aSm) * gers + (Sl + aSm) * gels - Sl * dgel - Sr * dger);
+      }
     }
     F[t]               = fd;
     F[o1 * n_cells + t] = fmx;
     F[o2 * n_cells + t] = fmy;
     F[o3 * n_cells + t] = fmz;
     F[4 * n_cells + t]  = fE;
+    if (g_de) F[kGE * n_cells + t] = fge;
   }
 }
 
+// The DE pressure of a cell and the centred velocity difference terms of the pdV
+// source, as Update_Conserved_Variables_3D_half (VL_3D_cuda.cu:409-441) and
+// Partial_Update_Advected_Internal_Energy_3D (hydro_cuda.cu:779-810) compute them:
+// returns 0.5 * P * (dtodx (vx_imo - vx_ipo) + dtody (vy_jmo - vy_jpo) + dtodz (vz_kmo - vz_kpo)).
+static double PdvTerm(const double *U, int i, int j, int k, int nx, int ny, size_t n_cells, double dtodx,
+                      double dtody, double dtodz, double gamma)
+{
+  size_t id    = Idx(i, j, k, nx, ny);
+  double d     = U[id];
+  double d_inv = 1.0 / d;
+  double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+  double E     = U[4 * n_cells + id];
+  double GE    = U[kGE * n_cells + id];
+  double E_kin = KineticEnergy(d, vx, vy, vz);
+  double P     = PressureFromDE(E, E - E_kin, GE, gamma);
+  P            = std::fmax(P, kTiny);
+  size_t imo = Idx(i - 1, j, k, nx, ny), ipo = Idx(i + 1, j, k, nx, ny);
+  size_t jmo = Idx(i, j - 1, k, nx, ny), jpo = Idx(i, j + 1, k, nx, ny);
+  size_t kmo = Idx(i, j, k - 1, nx, ny), kpo = Idx(i, j, k + 1, nx, ny);
+  double vx_imo = U[n_cells + imo] / U[imo], vx_ipo = U[n_cells + ipo] / U[ipo];
+  double vy_jmo = U[2 * n_cells + jmo] / U[jmo], vy_jpo = U[2 * n_cells + jpo] / U[jpo];
+  double vz_kmo = U[3 * n_cells + kmo] / U[kmo], vz_kpo = U[3 * n_cells + kpo] / U[kpo];
+  return 0.5 * P * (dtodx * (vx_imo - vx_ipo) + dtody * (vy_jmo - vy_jpo) + dtodz * (vz_kmo - vz_kpo));
+}
+
 // Update_Conserved_Variables_3D_half, src/integrators/VL_3D_cuda.cu:379-495
-// (density floor branch of :479-493 when density_floor_on)
+// (density floor branch of :479-493 when density_floor_on; DE branches :400-441, :471-478)
 void orc_update_half_3d(const double *U, double *Uh, const double *Fx, const double *Fy, const double *Fz, int nx,
-                        int ny, int nz, double dx, double dy, double dz, double dt, int n_fields, int density_floor_on,
-                        double density_floor)
+                        int ny, int nz, double dx, double dy, double dz, double dt, double gamma, int n_fields,
+                        int density_floor_on, double density_floor)
 {
   size_t n_cells = size_t(nx) * ny * nz;
   double dtodx = dt / dx, dtody = dt / dy, dtodz = dt / dz;
+  const int n_cons = g_de ? n_fields - 1 : n_fields;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k < nz - 1; k++)
     for (int j = 1; j < ny - 1; j++)
       for (int i = 1; i < nx - 1; i++) {
         size_t id = Idx(i, j, k, nx, ny), imo = Idx(i - 1, j, k, nx, ny), jmo = Idx(i, j - 1, k, nx, ny),
                kmo = Idx(i, j, k - 1, nx, ny);
-        for (int f = 0; f < n_fields; f++) {
+        for (int f = 0; f < n_cons; f++) {
           size_t o = size_t(f) * n_cells;
           Uh[o + id] = U[o + id] + dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]) +
                        dtodz * (Fz[o + kmo] - Fz[o + id]);
+        }
+        if (g_de) {
+          size_t o   = size_t(kGE) * n_cells;
+          double pdv = PdvTerm(U, i, j, k, nx, ny, n_cells, dtodx, dtody, dtodz, gamma);
+          Uh[o + id] = U[o + id] + dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]) +
+                       dtodz * (Fz[o + kmo] - Fz[o + id]) + pdv;
         }
         if (density_floor_on && Uh[id] < density_floor) {
           double d0 = Uh[id];
           Uh[id]    = density_floor;
           for (int f = 1; f < 5; f++) Uh[f * n_cells + id] *= (density_floor / d0);
+          if (g_de) Uh[kGE * n_cells + id] *= (density_floor / d0);
         }
+      }
+}
+
+// Partial_Update_Advected_Internal_Energy_3D, src/hydro/hydro_cuda.cu:754-833: the
+// full-step pdV source added to GasEnergy of the real cells before the update
+void orc_partial_update_ge_3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double dy, double dz,
+                              double dt, double gamma)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  double dtodx = dt / dx, dtody = dt / dy, dtodz = dt / dz;
+  // the term reads only density, momenta, Energy and the cell's own GasEnergy, so
+  // computing every term first and adding afterwards is the kernel's result
+  std::vector<double> add(n_cells, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = n_ghost; k < nz - n_ghost; k++)
+    for (int j = n_ghost; j < ny - n_ghost; j++)
+      for (int i = n_ghost; i < nx - n_ghost; i++)
+        add[Idx(i, j, k, nx, ny)] = PdvTerm(U, i, j, k, nx, ny, n_cells, dtodx, dtody, dtodz, gamma);
+  for (int k = n_ghost; k < nz - n_ghost; k++)
+    for (int j = n_ghost; j < ny - n_ghost; j++)
+      for (int i = n_ghost; i < nx - n_ghost; i++) {
+        size_t id = Idx(i, j, k, nx, ny);
+        U[kGE * n_cells + id] += add[id];
       }
 }
 
@@ -591,6 +706,7 @@ void orc_update_3d(double *U, const double *Fx, const double *Fy, const double *
             double d0 = U[id];
             U[id]     = density_floor;
             for (int f = 1; f < 5; f++) U[f * n_cells + id] *= (density_floor / d0);
+            if (g_de) U[kGE * n_cells + id] *= (density_floor / d0);  // hydro_cuda.cu:290-292
           } else {
             repair[id] = 1;
           }
@@ -627,7 +743,7 @@ void orc_update_1d(double *U, const double *F, int n_cells, int n_ghost, double 
     }
 }
 
-// Temperature_Floor_Kernel, src/hydro/hydro_cuda.cu:1117-1153 (no DE)
+// Temperature_Floor_Kernel, src/hydro/hydro_cuda.cu:1117-1153 (DE branch :1146-1151)
 void orc_temperature_floor(double *U, int nx, int ny, int nz, int n_ghost, double U_floor)
 {
   size_t n_cells = size_t(nx) * ny * nz;
@@ -646,6 +762,69 @@ void orc_temperature_floor(double *U, int nx, int ny, int nz, int n_ghost, doubl
         double Ekin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
         double u    = (E - Ekin) / d;
         if (u < U_floor) U[4 * n_cells + id] = Ekin + d * U_floor;
+        if (g_de) {
+          u = U[kGE * n_cells + id] / d;
+          if (u < U_floor) U[kGE * n_cells + id] = d * U_floor;
+        }
+      }
+}
+
+// Select_Internal_Energy_3D, src/hydro/hydro_cuda.cu:949-1016: GasEnergy of every
+// real cell becomes the internal energy the dual-energy condition selects
+void orc_select_internal_energy_3d(double *U, int nx, int ny, int nz, int n_ghost)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  std::vector<double> sel(n_cells, 0.0);
+  // every cell reads Energy only, which this kernel does not write
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = n_ghost; k < nz - n_ghost; k++)
+    for (int j = n_ghost; j < ny - n_ghost; j++)
+      for (int i = n_ghost; i < nx - n_ghost; i++) {
+        size_t id    = Idx(i, j, k, nx, ny);
+        size_t imo   = Idx(std::max(i - 1, n_ghost), j, k, nx, ny);
+        size_t ipo   = Idx(std::min(i + 1, nx - n_ghost - 1), j, k, nx, ny);
+        size_t jmo   = Idx(i, std::max(j - 1, n_ghost), k, nx, ny);
+        size_t jpo   = Idx(i, std::min(j + 1, ny - n_ghost - 1), k, nx, ny);
+        size_t kmo   = Idx(i, j, std::max(k - 1, n_ghost), nx, ny);
+        size_t kpo   = Idx(i, j, std::min(k + 1, nz - n_ghost - 1), nx, ny);
+        double d     = U[id];
+        double d_inv = 1.0 / d;
+        double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+        double E          = U[4 * n_cells + id];
+        double U_advected = U[kGE * n_cells + id];
+        double U_total    = E - 0.5 * d * (vx * vx + vy * vy + vz * vz);
+        const double *En  = U + 4 * n_cells;
+        double Emax       = std::fmax(En[imo], E);
+        Emax              = std::fmax(Emax, En[ipo]);
+        Emax              = std::fmax(Emax, En[jmo]);
+        Emax              = std::fmax(Emax, En[jpo]);
+        Emax              = std::fmax(Emax, En[kmo]);
+        Emax              = std::fmax(Emax, En[kpo]);
+        double u          = ((U_total / E > kEta1) || (U_total / Emax > kEta2)) ? U_total : U_advected;
+        sel[id]           = std::fmax(u, kTiny);
+      }
+  for (int k = n_ghost; k < nz - n_ghost; k++)
+    for (int j = n_ghost; j < ny - n_ghost; j++)
+      for (int i = n_ghost; i < nx - n_ghost; i++) {
+        size_t id             = Idx(i, j, k, nx, ny);
+        U[kGE * n_cells + id] = sel[id];
+      }
+}
+
+// Sync_Energies_3D, src/hydro/hydro_cuda.cu:1070-1099: Energy = kinetic + selected internal
+void orc_sync_energies_3d(double *U, int nx, int ny, int nz, int n_ghost)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = n_ghost; k < nz - n_ghost; k++)
+    for (int j = n_ghost; j < ny - n_ghost; j++)
+      for (int i = n_ghost; i < nx - n_ghost; i++) {
+        size_t id    = Idx(i, j, k, nx, ny);
+        double d     = U[id];
+        double d_inv = 1.0 / d;
+        double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+        double u = U[kGE * n_cells + id];
+        U[4 * n_cells + id] = 0.5 * d * (vx * vx + vy * vy + vz * vz) + u;
       }
 }
 
@@ -848,7 +1027,7 @@ void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double 
   orc_hllc(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, nz, gamma, 1);
   orc_hllc(s.QLz.data(), s.QRz.data(), s.Fz.data(), nx, ny, nz, gamma, 2);
   orc_update_half_3d(U, s.Uh.data(), s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, dx, dy, dz, 0.5 * dt,
-                     n_fields, density_floor_on, density_floor);
+                     gamma, n_fields, density_floor_on, density_floor);
   if (recon == ORC_PLMC) {
     orc_plmc(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, nz, gamma, 0);
     orc_plmc(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, nz, gamma, 1);
@@ -864,8 +1043,13 @@ void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double 
   orc_hllc(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, nz, gamma, 0);
   orc_hllc(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, nz, gamma, 1);
   orc_hllc(s.QLz.data(), s.QRz.data(), s.Fz.data(), nx, ny, nz, gamma, 2);
+  if (g_de) orc_partial_update_ge_3d(U, nx, ny, nz, n_ghost, dx, dy, dz, dt, gamma);  // VL_3D_cuda.cu:314-324
   orc_update_3d(U, s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, n_ghost, dx, dy, dz, dt, n_fields,
                 density_floor_on, density_floor);
+  if (g_de) {  // VL_3D_cuda.cu:349-357
+    orc_select_internal_energy_3d(U, nx, ny, nz, n_ghost);
+    orc_sync_energies_3d(U, nx, ny, nz, n_ghost);
+  }
 }
 
 // VL_Algorithm_1D_CUDA, src/integrators/VL_1D_cuda.cu:30-146 (HLLC)
@@ -904,6 +1088,7 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
   double dti = orc_calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma);
   int steps  = 0;
   bool one_d = ny == 1 && nz == 1;
+  if (g_de && (one_d || n_fields != 6)) return -1;  // DE: 3D, GasEnergy as field 5
   while (t < tout && steps < max_steps) {
     double dt = kCfl / dti;
     if (t + dt > tout) dt = tout - t;
@@ -921,6 +1106,16 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
   }
   *t_io = t;
   return steps;
+}
+
+// The DE build option (dual energy) as a run-time switch for every stage above.
+void orc_set_dual_energy(int on) { g_de = on ? 1 : 0; }
+int orc_get_dual_energy() { return g_de; }
+
+// hydro_utilities::Get_Pressure_From_DE, for the known-answer tests
+double orc_pressure_from_de(double E, double U_total, double U_advected, double gamma)
+{
+  return PressureFromDE(E, U_total, U_advected, gamma);
 }
 
 int orc_num_threads()

@@ -41,8 +41,9 @@
 
 extern "C" {
 void orc_update_half_3d(const double *U, double *Uh, const double *Fx, const double *Fy, const double *Fz, int nx,
-                        int ny, int nz, double dx, double dy, double dz, double dt, int n_fields, int density_floor_on,
-                        double density_floor);
+                        int ny, int nz, double dx, double dy, double dz, double dt, double gamma, int n_fields,
+                        int density_floor_on, double density_floor);
+int orc_get_dual_energy();
 void orc_update_3d(double *U, const double *Fx, const double *Fy, const double *Fz, int nx, int ny, int nz,
                    int n_ghost, double dx, double dy, double dz, double dt, int n_fields, int density_floor_on,
                    double density_floor);
@@ -837,7 +838,7 @@ void orc_mhd_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, dou
   orc_hlld(s.QLy.data(), s.QRy.data(), U + kBy * n, s.Fy.data(), long(n), g, 1);
   orc_hlld(s.QLz.data(), s.QRz.data(), U + kBz * n, s.Fz.data(), long(n), g, 2);
   orc_ct_electric_fields(s.Fx.data(), s.Fy.data(), s.Fz.data(), U, s.E.data(), nx, ny, nz, 8);
-  orc_update_half_3d(U, Uh, s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, dx, dy, dz, 0.5 * dt, 5,
+  orc_update_half_3d(U, Uh, s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, dx, dy, dz, 0.5 * dt, g, 5,
                      density_floor_on, density_floor);
   orc_update_magnetic(U, Uh, s.E.data(), nx, ny, nz, 0.5 * dt, dx, dy, dz);
   orc_mhd_plmc(Uh, s.QLx.data(), s.QRx.data(), nx, ny, nz, g, 0);
@@ -858,6 +859,7 @@ int orc_mhd_run(double *U, int nx, int ny, int nz, int n_ghost, double dx, doubl
                 const int *flags, double *t_io, double tout, int max_steps, int density_floor_on, double density_floor,
                 int temperature_floor_on, double U_floor, double *dt_log)
 {
+  if (orc_get_dual_energy()) return -1;  // the MHD restatement has no dual-energy branches
   double t   = *t_io;
   double dti = orc_mhd_calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, g);
   int steps  = 0;

@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
 checker, and by bench.py's cpu_baseline leg.  The product package (cholla_amd)
 never imports this module.
 """
+import contextlib
 import ctypes
 import os
 
@@ -50,7 +51,13 @@ def lib():
             "orc_plmc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_ppmc_vl": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_hllc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
-            "orc_update_half_3d": (None, [_dp] * 5 + [i, i, i, d, d, d, d, i, i, d]),
+            "orc_update_half_3d": (None, [_dp] * 5 + [i, i, i, d, d, d, d, d, i, i, d]),
+            "orc_partial_update_ge_3d": (None, [_dp, i, i, i, i, d, d, d, d, d]),
+            "orc_select_internal_energy_3d": (None, [_dp, i, i, i, i]),
+            "orc_sync_energies_3d": (None, [_dp, i, i, i, i]),
+            "orc_set_dual_energy": (None, [i]),
+            "orc_get_dual_energy": (i, []),
+            "orc_pressure_from_de": (d, [d, d, d, d]),
             "orc_update_3d": (None, [_dp] * 4 + [i, i, i, i, d, d, d, d, i, i, d]),
             "orc_temperature_floor": (None, [_dp, i, i, i, i, d]),
             "orc_calc_dti": (d, [_dp, i, i, i, i, d, d, d, d]),
@@ -119,17 +126,37 @@ def set_boundaries(U, nx, ny, nz, n_fields, n_ghost, flags):
     lib().orc_set_boundaries(ptr(U), nx, ny, nz, n_fields, n_ghost, fp)
 
 
+@contextlib.contextmanager
+def dual_energy(on=True):
+    """The reference's DE build option for the calls inside the block (GasEnergy = field 5)."""
+    prev = lib().orc_get_dual_energy()
+    lib().orc_set_dual_energy(int(on))
+    try:
+        yield
+    finally:
+        lib().orc_set_dual_energy(prev)
+
+
+def pressure_from_de(E, U_total, U_advected, gamma):
+    return lib().orc_pressure_from_de(E, U_total, U_advected, gamma)
+
+
 def run(U, nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, recon, flags, t, tout, max_steps,
-        density_floor_on=True, density_floor=0.0, temperature_floor_on=True, U_floor=0.0, dt_log=None):
+        density_floor_on=True, density_floor=0.0, temperature_floor_on=True, U_floor=0.0, dt_log=None,
+        dual_energy_on=False):
     """main.cpp's hydro loop; U (with ghost cells already set) is advanced in place.
+    dual_energy_on: the DE build (3D only, n_fields 6 with GasEnergy last).
 
     Returns (steps, t)."""
     f, fp = iptr(flags)
     tt = ctypes.c_double(t)
     log = dt_log if dt_log is not None else None
-    n = lib().orc_run(ptr(U), nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, recon, fp,
-                      ctypes.byref(tt), tout, max_steps, int(density_floor_on), density_floor,
-                      int(temperature_floor_on), U_floor, ptr(log) if log is not None else None)
+    with dual_energy(dual_energy_on):
+        n = lib().orc_run(ptr(U), nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, recon, fp,
+                          ctypes.byref(tt), tout, max_steps, int(density_floor_on), density_floor,
+                          int(temperature_floor_on), U_floor, ptr(log) if log is not None else None)
+    if n < 0:
+        raise ValueError("dual energy needs a 3D grid with n_fields == 6")
     return n, tt.value
 
 

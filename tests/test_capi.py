@@ -112,10 +112,20 @@ def test_turbulence_modes():
     a, b = m[:, 3:6], m[:, 6:9]
     # solenoidal: k . a = k . b = 0
     assert np.abs((k * a).sum(1)).max() < 1e-14 and np.abs((k * b).sum(1)).max() < 1e-14
-    # per-mode power ~ k^-2: after the projection two of three components remain, so
-    # E[|a|^2] = 2 / |k|^2; |a|^2 |k|^2 / 2 averages to 1 (chi-square with 2 d.o.f. / 2)
-    s = np.concatenate([(a ** 2).sum(1) * k2 / 2, (b ** 2).sum(1) * k2 / 2])
+    # per-mode power ~ k^-4 (shell spectrum E(k) ~ k^-2): after the projection two of
+    # three components remain, so E[|a|^2] = 2 / |k|^4; |a|^2 |k|^4 / 2 averages to 1
+    # (chi-square with 2 d.o.f. / 2)
+    s = np.concatenate([(a ** 2).sum(1) * k2 ** 2 / 2, (b ** 2).sum(1) * k2 ** 2 / 2])
     assert abs(s.mean() - 1.0) < 5 / np.sqrt(len(s)), s.mean()
+    # and the power of a shell follows the count of its modes times |k|^-4: the shell
+    # 1.5 <= |k| < 2.5 against 3.5 <= |k| <= 4 (expected ratio ~7; a |k|^-1 amplitude
+    # would give ~2), within sampling noise (~0.2 relative for these mode counts)
+    kk = np.sqrt(k2)
+    pw = (a ** 2).sum(1) + (b ** 2).sum(1)
+    s1, s2 = (kk >= 1.5) & (kk < 2.5), (kk >= 3.5) & (kk <= 4.0)
+    expect = (kk[s1] ** -4.0).sum() / (kk[s2] ** -4.0).sum()
+    ratio = pw[s1].sum() / pw[s2].sum()
+    assert abs(ratio / expect - 1) < 0.5, (ratio, expect)
     assert not np.array_equal(m, chl.turbulence_modes(8))
     assert np.array_equal(m, chl.turbulence_modes(7))
 
