@@ -50,6 +50,7 @@ class HydroConfig(ctypes.Structure):
         ("precision", ctypes.c_int), ("density_floor", ctypes.c_int), ("temperature_floor", ctypes.c_int),
         ("bc_flags", ctypes.c_int * 6), ("tile1", ctypes.c_int), ("tile2", ctypes.c_int),
         ("kchunk1", ctypes.c_int), ("kchunk2", ctypes.c_int), ("tile_order", ctypes.c_int),
+        ("dual_energy", ctypes.c_int),
     ]
 
 
@@ -214,9 +215,11 @@ class Hydro:
 
     def __init__(self, nx, ny, nz, n_ghost, reconstruction=RECON_PPMC, precision=PREC_FP64_EXACT,
                  bc=(1, 1, 1, 1, 1, 1), density_floor=True, temperature_floor=True, tile1=-1, tile2=-1,
-                 kchunk1=0, kchunk2=0, tile_order=0):
+                 kchunk1=0, kchunk2=0, tile_order=0, dual_energy=False):
         cfg = HydroConfig()
-        cfg.nx, cfg.ny, cfg.nz, cfg.n_ghost, cfg.n_fields = nx, ny, nz, n_ghost, 5
+        cfg.nx, cfg.ny, cfg.nz, cfg.n_ghost = nx, ny, nz, n_ghost
+        cfg.n_fields = 6 if dual_energy else 5
+        cfg.dual_energy = int(dual_energy)
         cfg.reconstruction, cfg.riemann, cfg.precision = reconstruction, RIEMANN_HLLC, precision
         cfg.density_floor, cfg.temperature_floor = int(density_floor), int(temperature_floor)
         for q in range(6):
@@ -228,6 +231,7 @@ class Hydro:
         self.h = h
         self.dims = (nx, ny, nz)
         self.n_ghost = n_ghost
+        self.n_fields = cfg.n_fields
 
     def config(self):
         c = HydroConfig()

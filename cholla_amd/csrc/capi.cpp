@@ -40,8 +40,9 @@ struct chl_hydro {
   size_t n_cells = 0;
   bool one_d     = false;
   int num_cu     = 256;
-  void *W        = nullptr;  // half-step primitives
+  void *W        = nullptr;  // half-step primitives (+ the pdV sources with dual energy)
   size_t W_bytes = 0;
+  double *pdv    = nullptr;  // inside W: dual energy's full-step pdV source per cell
   // device scalars: [0] dti bits, [1] dt, [2] t, [3] n_step (int in a double slot)
   unsigned char *scal = nullptr;
   hipStream_t stream  = nullptr;
@@ -69,7 +70,9 @@ namespace
 
 int CheckConfig(const chl_hydro_config &c)
 {
-  if (c.n_fields != 5) return Fail(CHL_ERR_UNSUPPORTED, "n_fields must be 5 (hydro without scalars or dual energy)");
+  if (c.dual_energy != 0 && c.dual_energy != 1) return Fail(CHL_ERR_INVALID, "dual_energy must be 0 or 1");
+  if (c.n_fields != 5 + c.dual_energy)
+    return Fail(CHL_ERR_UNSUPPORTED, "n_fields must be 5, or 6 with dual_energy (no passive scalars)");
   if (c.riemann != CHL_RIEMANN_HLLC) return Fail(CHL_ERR_UNSUPPORTED, "only the HLLC solver is built");
   if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC)
     return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PLMC or PPMC");
@@ -80,14 +83,14 @@ int CheckConfig(const chl_hydro_config &c)
   bool one_d = c.ny == 1 && c.nz == 1;
   if (!one_d) {
     if (c.ny < 2 * ng + 1 || c.nz < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "3D boxes need ny, nz > 2*n_ghost");
-    if (c.precision != CHL_PREC_FP64_EXACT || true) {
-      // any precision is fine in 3D
-    }
     if (c.tile1 >= chl::kNumTiles || c.tile2 >= chl::kNumTiles) return Fail(CHL_ERR_INVALID, "tile index out of range");
+    if (c.dual_energy && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
+      return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for tile shapes 0 and 1 only");
     if (c.tile_order != 0 && c.tile_order != 1) return Fail(CHL_ERR_INVALID, "tile_order must be 0 or 1");
   } else {
     if (c.precision != CHL_PREC_FP64_EXACT)
       return Fail(CHL_ERR_UNSUPPORTED, "1D boxes run the FP64 reference-order form only");
+    if (c.dual_energy) return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for 3D boxes only");
   }
   // 2^31 guard on per-axis products used in 32-bit grid arithmetic of launch shapes
   if (double(c.nx) * c.ny * c.nz * 5 > 9.0e15) return Fail(CHL_ERR_INVALID, "box too large");
@@ -98,6 +101,54 @@ int CheckConfig(const chl_hydro_config &c)
 }
 
 int DefaultChunk(int n_real) { return std::max(8, std::min(n_real, 32)); }
+
+// Arguments of the fused stage kernels for this box (kchunk is set per stage).
+chl::StepArgs MakeStepArgs(chl_hydro *h, double *U, double dx, double dy, double dz, double gamma, double dt_host,
+                           const double *dt_dev, double density_floor, double U_floor, int flags)
+{
+  const chl_hydro_config &c = h->cfg;
+  chl::StepArgs a{};
+  a.U = U, a.W = h->W, a.pdv = h->pdv, a.nx = c.nx, a.ny = c.ny, a.nz = c.nz, a.ng = c.n_ghost;
+  a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dt = dt_host, a.dt_dev = dt_dev, a.dti = h->dti();
+  a.density_floor = density_floor, a.U_floor = U_floor;
+  a.flags     = flags;
+  a.xcd_order = c.tile_order == 0;
+  a.de        = c.dual_energy;
+  return a;
+}
+
+int LaunchStage1(chl_hydro *h, chl::StepArgs &a, int tile = -1, int kchunk = -1)
+{
+  const chl_hydro_config &c = h->cfg;
+  a.kchunk     = kchunk > 0 ? kchunk : c.kchunk1;
+  const int t  = tile >= 0 ? tile : c.tile1;
+  const int R1 = ReconRadius(c.reconstruction) + 1;
+  CHL_HIP(c.precision == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, t, h->stream)
+                                             : chl::LaunchStage1Fast(c.precision, a, R1, t, h->stream));
+  return CHL_OK;
+}
+
+int LaunchStage2(chl_hydro *h, chl::StepArgs &a, int tile = -1, int kchunk = -1)
+{
+  const chl_hydro_config &c = h->cfg;
+  a.kchunk    = kchunk > 0 ? kchunk : c.kchunk2;
+  const int t = tile >= 0 ? tile : c.tile2;
+  CHL_HIP(c.precision == CHL_PREC_FP64_EXACT ? chl::LaunchStage2Exact(a, c.reconstruction, t, h->stream)
+                                             : chl::LaunchStage2Fast(c.precision, a, c.reconstruction, t, h->stream));
+  return CHL_OK;
+}
+
+// Dual energy: the internal-energy selection and energy sync after stage 2
+// (VL_3D_cuda.cu:349-357), carrying the floor / dt / ghost work stage 2 leaves out.
+int LaunchDualEnergy(chl_hydro *h, const chl::StepArgs &a)
+{
+  if (!h->cfg.dual_energy) return CHL_OK;
+  const chl_hydro_config &c = h->cfg;
+  CHL_HIP(chl::LaunchDeSelect(a.U, c.nx, c.ny, c.nz, c.n_ghost, h->stream));
+  CHL_HIP(chl::LaunchDeSync(a.U, c.nx, c.ny, c.nz, c.n_ghost, a.flags, a.U_floor, a.dx, a.dy, a.dz, a.gamma, a.dti,
+                            h->stream));
+  return CHL_OK;
+}
 
 // Enqueue one VL step.  mode: 0 = VL only, 1 = VL + floors + dti, (fill) ghost refresh.
 int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double gamma, double dt_host,
@@ -117,28 +168,16 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
     if (fill) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
     return CHL_OK;
   }
-  chl::StepArgs a{};
-  a.U = U, a.W = h->W, a.nx = c.nx, a.ny = c.ny, a.nz = c.nz, a.ng = ng;
-  a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dt = dt_host, a.dt_dev = dt_dev, a.dti = h->dti();
-  a.density_floor = density_floor, a.U_floor = U_floor;
   int flags = 0;
   if (c.density_floor) flags |= chl::kFlagDensityFloor;
   if (update_grid && c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
   if (update_grid) flags |= chl::kFlagReduceDt;
   const bool fused_ghosts = fill && h->PeriodicFusable();
   if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
-  a.flags       = flags;
-  a.xcd_order   = c.tile_order == 0;
-  const int R1  = ReconRadius(c.reconstruction) + 1;
-  const int pol = c.precision;
-  a.kchunk      = c.kchunk1;
-  hipError_t e  = pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, c.tile1, h->stream)
-                                             : chl::LaunchStage1Fast(pol, a, R1, c.tile1, h->stream);
-  CHL_HIP(e);
-  a.kchunk = c.kchunk2;
-  e        = pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage2Exact(a, c.reconstruction, c.tile2, h->stream)
-                                        : chl::LaunchStage2Fast(pol, a, c.reconstruction, c.tile2, h->stream);
-  CHL_HIP(e);
+  chl::StepArgs a = MakeStepArgs(h, U, dx, dy, dz, gamma, dt_host, dt_dev, density_floor, U_floor, flags);
+  CHL_TRY(LaunchStage1(h, a));
+  CHL_TRY(LaunchStage2(h, a));
+  CHL_TRY(LaunchDualEnergy(h, a));
   if (fill && !fused_ghosts) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
   return CHL_OK;
 }
@@ -271,7 +310,7 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return Fail(CHL_ERR_NO_DEVICE, "no HIP device");
   chl_hydro_config c = *cfg;
   if (c.tile1 < 0) c.tile1 = 0;
-  if (c.tile2 < 0) c.tile2 = 3;
+  if (c.tile2 < 0) c.tile2 = c.dual_energy ? 0 : 3;
   CHL_TRY(CheckConfig(c));
   auto *h    = new chl_hydro();
   h->one_d   = c.ny == 1 && c.nz == 1;
@@ -287,9 +326,12 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   }
   h->num_cu        = p.multiProcessorCount;
   size_t elem      = c.precision == CHL_PREC_MIXED ? sizeof(float) : sizeof(double);
-  h->W_bytes       = 5 * h->n_cells * elem;
+  const size_t w_fields = 5 + size_t(c.dual_energy);
+  h->W_bytes            = w_fields * h->n_cells * elem + (c.dual_energy ? h->n_cells * sizeof(double) : 0);
   hipError_t e     = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&h->W, h->W_bytes);
+  if (e == hipSuccess && c.dual_energy)
+    h->pdv = reinterpret_cast<double *>(static_cast<unsigned char *>(h->W) + w_fields * h->n_cells * elem);
   if (e == hipSuccess) e = hipMemsetAsync(h->W, 0, h->W_bytes, h->stream);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&h->scal), 64);
   if (e == hipSuccess) e = hipMemsetAsync(h->scal, 0, 64, h->stream);
@@ -409,42 +451,34 @@ int chl_hydro_kernel_times(chl_hydro *h, double *U, double dx, double dy, double
   const double tout         = std::numeric_limits<double>::infinity();
   CHL_TRY(InitialDti(h, U, dx, dy, dz, gamma));
   CHL_TRY(SetTimeState(h, 0.0));
-  hipEvent_t ev[4];
+  hipEvent_t ev[5];
   for (auto &e : ev) CHL_HIP(hipEventCreate(&e));
   double acc[3] = {0, 0, 0};
-  chl::StepArgs a{};
-  a.U = U, a.W = h->W, a.nx = c.nx, a.ny = c.ny, a.nz = c.nz, a.ng = c.n_ghost;
-  a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dt = 0.0, a.dt_dev = h->dt(), a.dti = h->dti();
-  a.density_floor = 0.0, a.U_floor = 0.0;
   int flags = chl::kFlagReduceDt;
   if (c.density_floor) flags |= chl::kFlagDensityFloor;
   if (c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
   const bool fused_ghosts = h->PeriodicFusable();
   if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
-  a.flags       = flags;
-  a.xcd_order   = c.tile_order == 0;
-  const int R1  = ReconRadius(c.reconstruction) + 1;
-  const int pol = c.precision;
+  chl::StepArgs a = MakeStepArgs(h, U, dx, dy, dz, gamma, 0.0, h->dt(), 0.0, 0.0, flags);
   for (int s = 0; s < n_steps; s++) {
     CHL_HIP(hipEventRecord(ev[0], h->stream));
     CHL_HIP(chl::LaunchDtAdvance(h->dti(), h->dt(), h->t(), tout, kCfl, h->nstep(), h->stream));
-    a.kchunk = c.kchunk1;
     CHL_HIP(hipEventRecord(ev[1], h->stream));
-    CHL_HIP(pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, c.tile1, h->stream)
-                                       : chl::LaunchStage1Fast(pol, a, R1, c.tile1, h->stream));
+    CHL_TRY(LaunchStage1(h, a));
     CHL_HIP(hipEventRecord(ev[2], h->stream));
-    a.kchunk = c.kchunk2;
-    CHL_HIP(pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage2Exact(a, c.reconstruction, c.tile2, h->stream)
-                                       : chl::LaunchStage2Fast(pol, a, c.reconstruction, c.tile2, h->stream));
+    CHL_TRY(LaunchStage2(h, a));
     CHL_HIP(hipEventRecord(ev[3], h->stream));
+    CHL_TRY(LaunchDualEnergy(h, a));
+    CHL_HIP(hipEventRecord(ev[4], h->stream));
     if (!fused_ghosts)
       CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, c.n_ghost, c.bc_flags, h->stream));
-    CHL_HIP(hipEventSynchronize(ev[3]));
-    float t01, t12, t23;
+    CHL_HIP(hipEventSynchronize(ev[4]));
+    float t01, t12, t23, t34;
     CHL_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
     CHL_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
     CHL_HIP(hipEventElapsedTime(&t23, ev[2], ev[3]));
-    acc[0] += t12, acc[1] += t23, acc[2] += t01;
+    CHL_HIP(hipEventElapsedTime(&t34, ev[3], ev[4]));
+    acc[0] += t12, acc[1] += t23, acc[2] += t01 + t34;
   }
   CHL_HIP(hipStreamSynchronize(h->stream));
   for (auto &e : ev) (void)hipEventDestroy(e);
@@ -465,67 +499,54 @@ int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, doub
   hipEvent_t e0, e1;
   CHL_HIP(hipEventCreate(&e0));
   CHL_HIP(hipEventCreate(&e1));
-  chl::StepArgs a{};
-  a.U = S, a.W = h->W, a.nx = c.nx, a.ny = c.ny, a.nz = c.nz, a.ng = c.n_ghost;
-  a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dti = h->dti();
-  a.flags = chl::kFlagReduceDt | (c.density_floor ? chl::kFlagDensityFloor : 0) |
-            (c.temperature_floor ? chl::kFlagTemperatureFloor : 0);
-  a.xcd_order = c.tile_order == 0;
   // a small fixed dt keeps every candidate on the same, physical state
+  double dt_fixed = 0;
   {
     double dti = 0;
     CHL_TRY(chl_calc_dt(U, c.nx, c.ny, c.nz, c.n_ghost, c.n_fields, dx, dy, dz, gamma, &dti));
-    a.dt = 0.1 * kCfl / dti;
+    dt_fixed = 0.1 * kCfl / dti;
   }
-  const int R1  = ReconRadius(c.reconstruction) + 1;
-  const int pol = c.precision;
-  const int nz1 = c.nz - 2 * c.n_ghost + 2 * R1, nz2 = c.nz - 2 * c.n_ghost;
+  chl::StepArgs a = MakeStepArgs(h, S, dx, dy, dz, gamma, dt_fixed, nullptr, 0.0, 0.0,
+                                 chl::kFlagReduceDt | (c.density_floor ? chl::kFlagDensityFloor : 0) |
+                                     (c.temperature_floor ? chl::kFlagTemperatureFloor : 0));
+  const int R1     = ReconRadius(c.reconstruction) + 1;
+  const int nz1    = c.nz - 2 * c.n_ghost + 2 * R1, nz2 = c.nz - 2 * c.n_ghost;
+  const int ntiles = c.dual_energy ? chl::kNumDeTiles : chl::kNumTiles;
   std::vector<int> chunks = {8, 16, 32, 64, 128};
   float best1 = 1e30f, best2 = 1e30f;
   int b_t1 = c.tile1, b_t2 = c.tile2, b_k1 = c.kchunk1, b_k2 = c.kchunk2;
   int rc = CHL_OK;
-  for (int t = 0; t < chl::kNumTiles && rc == CHL_OK; t++) {
+  for (int t = 0; t < ntiles && rc == CHL_OK; t++) {
     for (int kc : chunks) {
       if (kc > 2 * nz1 && kc != chunks.front()) continue;
-      a.kchunk = std::min(kc, nz1);
-      float ms = 0;
-      for (int rep = 0; rep < 2; rep++) {  // rep 0 warms up
+      const int k1 = std::min(kc, nz1);
+      float ms     = 0;
+      for (int rep = 0; rep < 2 && rc == CHL_OK; rep++) {  // rep 0 warms up
         if (hipMemcpyAsync(S, U, bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) rc = CHL_ERR_HIP;
         (void)hipEventRecord(e0, h->stream);
-        for (int s = 0; s < steps; s++) {
-          hipError_t e = pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, t, h->stream)
-                                                    : chl::LaunchStage1Fast(pol, a, R1, t, h->stream);
-          if (e != hipSuccess) rc = Fail(CHL_ERR_HIP, hipGetErrorString(e));
-        }
+        for (int s = 0; s < steps && rc == CHL_OK; s++) rc = LaunchStage1(h, a, t, k1);
         (void)hipEventRecord(e1, h->stream);
         (void)hipEventSynchronize(e1);
         (void)hipEventElapsedTime(&ms, e0, e1);
       }
-      if (ms < best1) best1 = ms, b_t1 = t, b_k1 = a.kchunk;
+      if (ms < best1) best1 = ms, b_t1 = t, b_k1 = k1;
     }
   }
-  for (int t = 0; t < chl::kNumTiles && rc == CHL_OK; t++) {
+  for (int t = 0; t < ntiles && rc == CHL_OK; t++) {
     for (int kc : chunks) {
       if (kc > 2 * nz2 && kc != chunks.front()) continue;
-      a.kchunk = std::min(kc, nz2);
-      float ms = 0;
-      for (int rep = 0; rep < 2; rep++) {
+      const int k2 = std::min(kc, nz2);
+      float ms     = 0;
+      for (int rep = 0; rep < 2 && rc == CHL_OK; rep++) {
         if (hipMemcpyAsync(S, U, bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) rc = CHL_ERR_HIP;
-        a.kchunk = c.kchunk1;
-        (void)(pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, b_t1, h->stream)
-                                          : chl::LaunchStage1Fast(pol, a, R1, b_t1, h->stream));
-        a.kchunk = std::min(kc, nz2);
+        if (rc == CHL_OK) rc = LaunchStage1(h, a, b_t1, b_k1);
         (void)hipEventRecord(e0, h->stream);
-        for (int s = 0; s < steps; s++) {
-          hipError_t e = pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage2Exact(a, c.reconstruction, t, h->stream)
-                                                    : chl::LaunchStage2Fast(pol, a, c.reconstruction, t, h->stream);
-          if (e != hipSuccess) rc = Fail(CHL_ERR_HIP, hipGetErrorString(e));
-        }
+        for (int s = 0; s < steps && rc == CHL_OK; s++) rc = LaunchStage2(h, a, t, k2);
         (void)hipEventRecord(e1, h->stream);
         (void)hipEventSynchronize(e1);
         (void)hipEventElapsedTime(&ms, e0, e1);
       }
-      if (ms < best2) best2 = ms, b_t2 = t, b_k2 = a.kchunk;
+      if (ms < best2) best2 = ms, b_t2 = t, b_k2 = k2;
     }
   }
   (void)hipStreamSynchronize(h->stream);
@@ -567,9 +588,8 @@ int chl_calc_dt(const double *U, int nx, int ny, int nz, int n_ghost, int n_fiel
 
 int chl_apply_temperature_floor(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double U_floor)
 {
-  (void)n_fields;
   if (!U) return Fail(CHL_ERR_INVALID, "null argument");
-  CHL_HIP(chl::LaunchTemperatureFloor(U, nx, ny, nz, n_ghost, U_floor, nullptr));
+  CHL_HIP(chl::LaunchTemperatureFloor(U, nx, ny, nz, n_ghost, U_floor, n_fields, nullptr));
   CHL_HIP(hipDeviceSynchronize());
   return CHL_OK;
 }

@@ -77,6 +77,9 @@ chl_hydro *Acquire(Slot &s, int nx, int ny, int nz, int ng, int nf)
   c.density_floor = kDensityFloor, c.temperature_floor = 0;  // the floor is its own call here
   for (int q = 0; q < 6; q++) c.bc_flags[q] = CHL_BC_PERIODIC;  // unused: VL never fills ghosts
   c.tile1 = -1, c.tile2 = -1;
+#ifdef DE
+  c.dual_energy = 1;  // the DE build: GasEnergy is the last of nf == 6 fields
+#endif
   Check(chl_hydro_create(&c, &s.h), "chl_hydro_create");
   s.nx = nx, s.ny = ny, s.nz = nz, s.ng = ng, s.nf = nf;
   return s.h;

@@ -26,6 +26,11 @@ namespace chl
 {
 
 constexpr double kTinyNumber = 1.0e-20;  // TINY_NUMBER, src/global/global.h:25
+// Dual energy (the DE build option): GasEnergy is field 5 (no scalars, no MHD) and the
+// internal energy switch thresholds are DE_ETA_1 / DE_ETA_2 (src/global/global.h:62-73).
+constexpr int kGasEnergy = 5;
+constexpr double kDeEta1 = 0.001;
+constexpr double kDeEta2 = 0.035;
 
 struct PolicyExact {
   using T                      = double;
@@ -135,6 +140,14 @@ __device__ __forceinline__ T PressureFromCons(T E, T d, T vx, T vy, T vz, T gamm
 {
   T pressure = E - T(0.5) * d * SquareMag(vx, vy, vz);
   return Fmax((gamma - T(1.)) * pressure, T(kTinyNumber));
+}
+
+// hydro_utilities::Get_Pressure_From_DE, src/utils/hydro_utilities.h:136-151
+template <typename T>
+__device__ __forceinline__ T PressureFromDE(T E, T U_total, T U_advected, T gamma)
+{
+  T U = (U_total / E > T(kDeEta1)) ? U_total : U_advected;
+  return Fmax(U * (gamma - T(1.0)), T(kTinyNumber));
 }
 
 // reconstruction::Load_Data, src/reconstruction/reconstruction.h:139-212: velocities
@@ -431,6 +444,39 @@ __device__ __forceinline__ void ReconPlmc(const Prim<typename P::T> s[3], typena
   L_iph.p = Fmax(L_iph.p, T(kTinyNumber));
 }
 
+// A passive variable (the specific GasEnergy under dual energy) through the same
+// reconstructions: PLMC limits its primitive slopes (Monotonize_Characteristic_Return_Primitive,
+// reconstruction.h:549-551), interpolates (Calc_Interface_Linear :664) and limits the
+// interfaces (Plm_Limit_Interfaces :714-715); PPMC applies PPM_Single_Variable to it
+// (ppmc_cuda.cu:668-672).
+template <class P>
+__device__ __forceinline__ void ReconPlmcScalar(const typename P::T g[3], typename P::T &L_iph, typename P::T &R_imh)
+{
+  using T  = typename P::T;
+  T dL     = T(1.0) * (g[1] - g[0]);
+  T dR     = T(1.0) * (g[2] - g[1]);
+  T dC     = T(0.5) * (g[2] - g[0]);
+  T dG     = VlSlope(dL, dR, T(0));
+  T dm     = MonotonizeOne(dL, dR, dC, dG);
+  const T hp = T(1.0) * T(0.5), hm = T(-1.0) * T(0.5);
+  L_iph    = g[1] + hp * dm;
+  R_imh    = g[1] + hm * dm;
+  R_imh    = Fmax(Fmin(g[1], g[0]), R_imh);
+  R_imh    = Fmin(Fmax(g[1], g[0]), R_imh);
+  L_iph    = Fmax(Fmin(g[1], g[2]), L_iph);
+  L_iph    = Fmin(Fmax(g[1], g[2]), L_iph);
+}
+
+template <class P>
+__device__ __forceinline__ void ReconPpmcScalar(const typename P::T g[5], typename P::T &L_iph, typename P::T &R_imh)
+{
+  if constexpr (P::kId == PolicyMixed::kId) {
+    PpmSingleReduced(g[0], g[1], g[2], g[3], g[4], L_iph, R_imh);
+  } else {
+    PpmSingle<P>(g[0], g[1], g[2], g[3], g[4], L_iph, R_imh);
+  }
+}
+
 // ---------------------------------------------------------------------------- HLLC
 // Riemann input for one side: conserved and primitive forms of the same state.
 template <typename T>
@@ -438,14 +484,19 @@ struct RState {
   T d, mx, my, mz, E;  // conserved (rotated)
   T vx, vy, vz, p;     // primitive
   T inv_d;             // 1/d (non-exact policies only)
+  T dge;               // conserved GasEnergy (dual energy only)
 };
 
 // Side state from a conserved interface value, as Calculate_HLLC_Fluxes_CUDA
 // derives it (hllc_cuda.cu:104-131): velocities by division, pressure from the
 // energy with the sum ((vx^2 + vy^2) + vz^2), floored at TINY_NUMBER.
-template <class P>
+// With dual energy the pressure is Get_Pressure_From_DE of the energy, the
+// internal energy left after the kinetic part and the interface GasEnergy
+// (hllc_cuda.cu:107-112, 125-130).
+template <class P, bool DE = false>
 __device__ __forceinline__ RState<typename P::T> RStateFromCons(typename P::T d, typename P::T mx, typename P::T my,
-                                                                typename P::T mz, typename P::T E, typename P::T gamma)
+                                                                typename P::T mz, typename P::T E, typename P::T gamma,
+                                                                typename P::T dge = typename P::T(0))
 {
   using T = typename P::T;
   RState<T> s;
@@ -454,8 +505,14 @@ __device__ __forceinline__ RState<typename P::T> RStateFromCons(typename P::T d,
   s.vx    = Div<P>(mx, d, s.inv_d);
   s.vy    = Div<P>(my, d, s.inv_d);
   s.vz    = Div<P>(mz, d, s.inv_d);
-  s.p     = (E - T(0.5) * d * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz)) * (gamma - T(1.0));
-  s.p     = Fmax(s.p, T(kTinyNumber));
+  if constexpr (DE) {
+    T E_kin = T(0.5) * d * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+    s.p     = PressureFromDE(E, E - E_kin, dge, gamma);
+    s.dge   = dge;
+  } else {
+    s.p = (E - T(0.5) * d * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz)) * (gamma - T(1.0));
+  }
+  s.p = Fmax(s.p, T(kTinyNumber));
   return s;
 }
 
@@ -463,29 +520,41 @@ __device__ __forceinline__ RState<typename P::T> RStateFromCons(typename P::T d,
 // through the reference's conserved round trip (Write_Data then the solver's own
 // primitive recovery, reconstruction.h:918-958 + hllc_cuda.cu:104-131); the other
 // policies use the primitive values directly.
-template <class P>
+// With dual energy, ge is the reconstructed specific GasEnergy of the interface
+// (Write_Data stores density * gas_energy, reconstruction.h:960-962).  The non-exact
+// policies apply the DE_ETA_1 switch to the primitive state: the internal energy
+// p/(gamma-1) against the total energy, else the pressure of the GasEnergy.
+template <class P, bool DE = false>
 __device__ __forceinline__ RState<typename P::T> RStateFromPrim(const Prim<typename P::T> &s, typename P::T gamma,
-                                                                typename P::T inv_gm1)
+                                                                typename P::T inv_gm1,
+                                                                typename P::T ge = typename P::T(0))
 {
   using T = typename P::T;
   T d = s.d, mx = s.d * s.vx, my = s.d * s.vy, mz = s.d * s.vz;
   T E = EnergyFromPrim<P>(s, gamma, inv_gm1);
   if constexpr (P::kExact) {
-    return RStateFromCons<P>(d, mx, my, mz, E, gamma);
+    return RStateFromCons<P, DE>(d, mx, my, mz, E, gamma, DE ? s.d * ge : T(0));
   } else {
     RState<T> r;
     r.d = d, r.mx = mx, r.my = my, r.mz = mz, r.E = E;
     r.vx = s.vx, r.vy = s.vy, r.vz = s.vz, r.p = s.p;
     r.inv_d = Rcp<P>(d);
+    if constexpr (DE) {
+      r.dge  = s.d * ge;
+      T th   = Div<P>(Fmax(s.p, T(kTinyNumber)), gamma - T(1.), inv_gm1);
+      bool ok = th > T(kDeEta1) * E;  // (E - E_kin) / E > eta_1 with E > 0
+      r.p    = ok ? s.p : Fmax(r.dge * (gamma - T(1.0)), T(kTinyNumber));
+    }
     return r;
   }
 }
 
 // Calculate_HLLC_Fluxes_CUDA body, src/riemann_solvers/hllc_cuda.cu:141-305 (etah = 0).
-// Returns the flux in rotated order.
-template <class P>
+// Returns the flux in rotated order; with dual energy the GasEnergy flux goes to
+// *fge (hllc_cuda.cu:183-198, 218-235, 251-303).
+template <class P, bool DE = false>
 __device__ __forceinline__ Cons<typename P::T> Hllc(const RState<typename P::T> &l, const RState<typename P::T> &r,
-                                                    typename P::T gamma)
+                                                    typename P::T gamma, typename P::T *fge = nullptr)
 {
   using T    = typename P::T;
   const T g1 = gamma - T(1.0);
@@ -513,8 +582,16 @@ __device__ __forceinline__ Cons<typename P::T> Hllc(const RState<typename P::T> 
 
   Cons<T> fl{l.mx, l.mx * l.vx + l.p, l.my * l.vx, l.mz * l.vx, (l.E + l.p) * l.vx};
   Cons<T> fr{r.mx, r.mx * r.vx + r.p, r.my * r.vx, r.mz * r.vx, (r.E + r.p) * r.vx};
-  if (Sl > T(0.0)) return fl;
-  if (Sr < T(0.0)) return fr;
+  T f_ge_l = T(0), f_ge_r = T(0);
+  if constexpr (DE) f_ge_l = l.dge * l.vx, f_ge_r = r.dge * r.vx;
+  if (Sl > T(0.0)) {
+    if constexpr (DE) *fge = f_ge_l;
+    return fl;
+  }
+  if (Sr < T(0.0)) {
+    if constexpr (DE) *fge = f_ge_r;
+    return fr;
+  }
 
   T Sm = Div1<P>(r.d * r.vx * (Sr - r.vx) - l.d * l.vx * (Sl - l.vx) + l.p - r.p,
                  r.d * (Sr - r.vx) - l.d * (Sl - l.vx));
@@ -535,6 +612,11 @@ __device__ __forceinline__ Cons<typename P::T> Hllc(const RState<typename P::T> 
   f.my = T(0.5) * (fl.my + fr.my + (Sr - aSm) * myrs + (Sl + aSm) * myls - Sl * l.my - Sr * r.my);
   f.mz = T(0.5) * (fl.mz + fr.mz + (Sr - aSm) * mzrs + (Sl + aSm) * mzls - Sl * l.mz - Sr * r.mz);
   f.E  = T(0.5) * (fl.E + fr.E + (Sr - aSm) * Ers + (Sl + aSm) * Els - Sl * l.E - Sr * r.E);
+  if constexpr (DE) {
+    T gel = Div<P>(l.dge, l.d, l.inv_d), ger = Div<P>(r.dge, r.d, r.inv_d);
+    T gels = dls * gel, gers = drs * ger;
+    *fge   = T(0.5) * (f_ge_l + f_ge_r + (Sr - aSm) * gers + (Sl + aSm) * gels - Sl * l.dge - Sr * r.dge);
+  }
   return f;
 }
 

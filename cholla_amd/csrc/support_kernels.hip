@@ -241,8 +241,9 @@ __global__ void CalcDt1DKernel(const double *U, int nx, int ng, double dx, doubl
   BlockMaxToGlobal(m, dti);
 }
 
-// Temperature_Floor_Kernel, src/hydro/hydro_cuda.cu:1117-1153 (no DE)
-__global__ void TemperatureFloorKernel(double *U, int nx, int ny, int nz, int ng, double U_floor)
+// Temperature_Floor_Kernel, src/hydro/hydro_cuda.cu:1117-1153 (with the DE branch
+// :1146-1151 when the state carries GasEnergy, n_fields == 6)
+__global__ void TemperatureFloorKernel(double *U, int nx, int ny, int nz, int ng, double U_floor, int de)
 {
   const size_t nc = size_t(nx) * ny * nz;
   size_t id       = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -257,7 +258,86 @@ __global__ void TemperatureFloorKernel(double *U, int nx, int ny, int nz, int ng
     double Ekin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
     double u    = (E - Ekin) / d;
     if (u < U_floor) U[4 * nc + id] = Ekin + d * U_floor;
+    if (de) {
+      u = U[kGasEnergy * nc + id] / d;
+      if (u < U_floor) U[kGasEnergy * nc + id] = d * U_floor;
+    }
   }
+}
+
+// ============================================================================ dual energy
+// Real cell of a 1D launch over the real cells of an nx x ny x nz box.
+__device__ __forceinline__ bool RealCell(long t, int nx, int ny, int nz, int ng, int &i, int &j, int &k)
+{
+  const long rx = nx - 2 * ng, ry = ny - 2 * ng, rz = nz - 2 * ng;
+  if (t >= rx * ry * rz) return false;
+  i = int(t % rx) + ng;
+  long q = t / rx;
+  j = int(q % ry) + ng;
+  k = int(q / ry) + ng;
+  return true;
+}
+
+// Select_Internal_Energy_3D, src/hydro/hydro_cuda.cu:949-1016: GasEnergy of every
+// real cell becomes U_total = E - kinetic where the thermal fraction exceeds
+// DE_ETA_1 (or DE_ETA_2 of the largest neighbouring energy), else it keeps the
+// advected value; floored at TINY_NUMBER.  Energy is only read here.
+__global__ void DeSelectKernel(double *U, int nx, int ny, int nz, int ng)
+{
+  int i, j, k;
+  if (!RealCell(long(blockIdx.x) * blockDim.x + threadIdx.x, nx, ny, nz, ng, i, j, k)) return;
+  const size_t nc = size_t(nx) * ny * nz;
+  const size_t id = Lin(i, j, k, nx, ny);
+  const double *En = U + 4 * nc;
+  const size_t imo = Lin(max(i - 1, ng), j, k, nx, ny), ipo = Lin(min(i + 1, nx - ng - 1), j, k, nx, ny);
+  const size_t jmo = Lin(i, max(j - 1, ng), k, nx, ny), jpo = Lin(i, min(j + 1, ny - ng - 1), k, nx, ny);
+  const size_t kmo = Lin(i, j, max(k - 1, ng), nx, ny), kpo = Lin(i, j, min(k + 1, nz - ng - 1), nx, ny);
+  double d = U[id], d_inv = 1.0 / d;
+  double vx = U[nc + id] * d_inv, vy = U[2 * nc + id] * d_inv, vz = U[3 * nc + id] * d_inv;
+  double E          = En[id];
+  double U_advected = U[kGasEnergy * nc + id];
+  double U_total    = E - 0.5 * d * (vx * vx + vy * vy + vz * vz);
+  double Emax       = Fmax(En[imo], E);
+  Emax              = Fmax(Emax, En[ipo]);
+  Emax              = Fmax(Emax, En[jmo]);
+  Emax              = Fmax(Emax, En[jpo]);
+  Emax              = Fmax(Emax, En[kmo]);
+  Emax              = Fmax(Emax, En[kpo]);
+  double u          = ((U_total / E > kDeEta1) || (U_total / Emax > kDeEta2)) ? U_total : U_advected;
+  U[kGasEnergy * nc + id] = Fmax(u, kTinyNumber);
+}
+
+// Sync_Energies_3D (hydro_cuda.cu:1070-1099): Energy = kinetic + selected internal
+// energy; then, as Update_Hydro_Grid continues (grid3D.cpp:497-577), the temperature
+// floor of E and GasEnergy, the next inverse dt (Calc_dt_3D) and, for an
+// all-periodic box, the ghost images of the cell.
+__global__ void DeSyncKernel(double *U, int nx, int ny, int nz, int ng, int flags, double U_floor, double dx,
+                             double dy, double dz, double gamma, unsigned long long *dti)
+{
+  int i, j, k;
+  double m = 0.0;
+  if (RealCell(long(blockIdx.x) * blockDim.x + threadIdx.x, nx, ny, nz, ng, i, j, k)) {
+    const size_t nc = size_t(nx) * ny * nz;
+    const size_t id = Lin(i, j, k, nx, ny);
+    double un[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) un[q] = U[q * nc + id];
+    double d_inv = 1.0 / un[0];
+    double vx = un[1] * d_inv, vy = un[2] * d_inv, vz = un[3] * d_inv;
+    un[4] = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz) + un[kGasEnergy];
+    if (flags & kFlagTemperatureFloor) {  // Temperature_Floor_Kernel, hydro_cuda.cu:1132-1151
+      double Ekin = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz);
+      double u    = (un[4] - Ekin) / un[0];
+      if (u < U_floor) un[4] = Ekin + un[0] * U_floor;
+      u = un[kGasEnergy] / un[0];
+      if (u < U_floor) un[kGasEnergy] = un[0] * U_floor;
+    }
+    U[4 * nc + id]          = un[4];
+    U[kGasEnergy * nc + id] = un[kGasEnergy];
+    if (flags & kFlagReduceDt) m = InverseCrossingTime(un[4], un[0], d_inv, vx, vy, vz, dx, dy, dz, gamma);
+    if (flags & kFlagPeriodicGhosts) WritePeriodicImages<6>(U, nc, nx, ny, nz, ng, i, j, k, un);
+  }
+  if (flags & kFlagReduceDt) BlockMaxToGlobal(m, dti);
 }
 
 // Device-side time control for graph-captured stepping: dt = C_cfl / dti clamped
@@ -546,10 +626,30 @@ hipError_t LaunchCalcDt(const double *U, int nx, int ny, int nz, int ng, double 
   return hipGetLastError();
 }
 
-hipError_t LaunchTemperatureFloor(double *U, int nx, int ny, int nz, int ng, double U_floor, hipStream_t s)
+hipError_t LaunchTemperatureFloor(double *U, int nx, int ny, int nz, int ng, double U_floor, int n_fields,
+                                  hipStream_t s)
 {
   long nc = long(nx) * ny * nz;
-  hipLaunchKernelGGL(TemperatureFloorKernel, dim3(Blocks(nc, 256)), dim3(256), 0, s, U, nx, ny, nz, ng, U_floor);
+  hipLaunchKernelGGL(TemperatureFloorKernel, dim3(Blocks(nc, 256)), dim3(256), 0, s, U, nx, ny, nz, ng, U_floor,
+                     int(n_fields == 6));
+  return hipGetLastError();
+}
+
+hipError_t LaunchDeSelect(double *U, int nx, int ny, int nz, int ng, hipStream_t s)
+{
+  long nr = long(nx - 2 * ng) * (ny - 2 * ng) * (nz - 2 * ng);
+  if (nr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(DeSelectKernel, dim3(Blocks(nr, 256)), dim3(256), 0, s, U, nx, ny, nz, ng);
+  return hipGetLastError();
+}
+
+hipError_t LaunchDeSync(double *U, int nx, int ny, int nz, int ng, int flags, double U_floor, double dx, double dy,
+                        double dz, double gamma, unsigned long long *dti, hipStream_t s)
+{
+  long nr = long(nx - 2 * ng) * (ny - 2 * ng) * (nz - 2 * ng);
+  if (nr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(DeSyncKernel, dim3(Blocks(nr, 256)), dim3(256), 0, s, U, nx, ny, nz, ng, flags, U_floor, dx, dy,
+                     dz, gamma, dti);
   return hipGetLastError();
 }
 

@@ -15,7 +15,14 @@ hipError_t LaunchGhostSlab(double *U, int nx, int ny, int nz, int n_fields, int 
                            int jsize, int ksize, int imin, int jmin, int kmin, int dir, hipStream_t s);
 hipError_t LaunchCalcDt(const double *U, int nx, int ny, int nz, int ng, double dx, double dy, double dz,
                         double gamma, unsigned long long *dti, int num_cu, hipStream_t s);
-hipError_t LaunchTemperatureFloor(double *U, int nx, int ny, int nz, int ng, double U_floor, hipStream_t s);
+hipError_t LaunchTemperatureFloor(double *U, int nx, int ny, int nz, int ng, double U_floor, int n_fields,
+                                  hipStream_t s);
+// Dual energy: Select_Internal_Energy_3D, then Sync_Energies_3D fused with the
+// temperature floor, the inverse-dt reduction and periodic ghost images (flags: the
+// kFlag* bits of vl3d_kernels.hpp).
+hipError_t LaunchDeSelect(double *U, int nx, int ny, int nz, int ng, hipStream_t s);
+hipError_t LaunchDeSync(double *U, int nx, int ny, int nz, int ng, int flags, double U_floor, double dx, double dy,
+                        double dz, double gamma, unsigned long long *dti, hipStream_t s);
 hipError_t LaunchDtAdvance(const unsigned long long *dti, double *dt_dev, double *t_dev, double tout, double cfl,
                            int *n_step, hipStream_t s);
 hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const double *modes, double vscale, double rho,

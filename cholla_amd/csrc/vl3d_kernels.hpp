@@ -47,7 +47,8 @@ struct ReconTraits<kPPMC> {
 // Parameters shared by both stages.
 struct StepArgs {
   double *U;         // conserved state, n_fields x n_cells (in/out)
-  void *W;           // half-step primitives, 5 x n_cells of P::T
+  void *W;           // half-step primitives, 5 (6 with dual energy) x n_cells of P::T
+  double *pdv;       // dual energy: full-step pdV source of each real cell (n_cells), else null
   int nx, ny, nz;    // total cells per axis, ghosts included
   int ng;            // ghost depth
   double dx, dy, dz;
@@ -61,6 +62,7 @@ struct StepArgs {
   int kchunk;              // z cells per workgroup
   int gx, gy, gz;          // tile grid of the launch (set by the launcher)
   int xcd_order;           // 1: give each XCD a contiguous run of tiles (see TileOf)
+  int de;                  // dual energy (host-side dispatch only; the kernels are templated on it)
 };
 
 enum : int {
@@ -128,14 +130,23 @@ __device__ __forceinline__ void Unrot(T n, T t1, T t2, T &gx, T &gy, T &gz)
 // ============================================================================ stage 1
 // Per-cell values staged for the PCM Riemann problems: the conserved state
 // plus the velocities (divisions done once per cell) and the direction-free
-// pressure used by the non-exact policies.
-template <typename T>
+// pressure used by the non-exact policies; with dual energy also the conserved
+// GasEnergy.
+template <typename T, bool DE = false>
 struct PcmCell {
   T d, mx, my, mz, E, vx, vy, vz, p;
 };
+template <typename T>
+struct PcmCell<T, true> {
+  T d, mx, my, mz, E, vx, vy, vz, p, ge;
+};
 
-template <class P, int DIR>
-__device__ __forceinline__ RState<typename P::T> PcmSide(const PcmCell<typename P::T> &c, typename P::T gamma)
+// Number of conserved fields a stage carries: 5, or 6 with GasEnergy.
+template <bool DE>
+constexpr int kNumCons = DE ? 6 : 5;
+
+template <class P, int DIR, bool DE = false>
+__device__ __forceinline__ RState<typename P::T> PcmSide(const PcmCell<typename P::T, DE> &c, typename P::T gamma)
 {
   using T = typename P::T;
   RState<T> s;
@@ -143,22 +154,28 @@ __device__ __forceinline__ RState<typename P::T> PcmSide(const PcmCell<typename 
   Rot<DIR>(c.mx, c.my, c.mz, s.mx, s.my, s.mz);
   Rot<DIR>(c.vx, c.vy, c.vz, s.vx, s.vy, s.vz);
   if constexpr (P::kExact) {
-    // hllc_cuda.cu:111-113: the pressure sum follows the rotated component order
-    s.p = (s.E - T(0.5) * s.d * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz)) * (gamma - T(1.0));
+    // hllc_cuda.cu:107-113: the kinetic sum follows the rotated component order
+    if constexpr (DE) {
+      T E_kin = T(0.5) * s.d * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+      s.p     = PressureFromDE(s.E, s.E - E_kin, c.ge, gamma);
+    } else {
+      s.p = (s.E - T(0.5) * s.d * (s.vx * s.vx + s.vy * s.vy + s.vz * s.vz)) * (gamma - T(1.0));
+    }
     s.p = Fmax(s.p, T(kTinyNumber));
   } else {
     s.p = c.p;
   }
+  if constexpr (DE) s.dge = c.ge;
   s.inv_d = Rcp<P>(s.d);
   return s;
 }
 
-template <class P>
-__device__ __forceinline__ PcmCell<typename P::T> MakePcmCell(double d, double mx, double my, double mz, double E,
-                                                              double gamma)
+template <class P, bool DE = false>
+__device__ __forceinline__ PcmCell<typename P::T, DE> MakePcmCell(double d, double mx, double my, double mz, double E,
+                                                                  double ge, double gamma)
 {
   using T = typename P::T;
-  PcmCell<T> c;
+  PcmCell<T, DE> c;
   if constexpr (P::kExact) {
     c.d = d, c.mx = mx, c.my = my, c.mz = mz, c.E = E;
     c.vx = mx / d, c.vy = my / d, c.vz = mz / d;
@@ -167,10 +184,17 @@ __device__ __forceinline__ PcmCell<typename P::T> MakePcmCell(double d, double m
     // FP64 primitive recovery, then narrowing (a no-op for PolicyFast)
     double inv = 1.0 / d;
     double vx = mx * inv, vy = my * inv, vz = mz * inv;
-    double p  = PressureFromCons(E, d, vx, vy, vz, gamma);
+    double p;
+    if constexpr (DE) {
+      double E_kin = 0.5 * d * SquareMag(vx, vy, vz);
+      p            = PressureFromDE(E, E - E_kin, ge, gamma);
+    } else {
+      p = PressureFromCons(E, d, vx, vy, vz, gamma);
+    }
     c.d = T(d), c.mx = T(mx), c.my = T(my), c.mz = T(mz), c.E = T(E);
     c.vx = T(vx), c.vy = T(vy), c.vz = T(vz), c.p = T(p);
   }
+  if constexpr (DE) c.ge = T(ge);
   return c;
 }
 
@@ -182,25 +206,62 @@ __device__ __forceinline__ void FluxGlobal(const Cons<typename P::T> &f, typenam
   out[4] = f.E;
 }
 
+// HLLC on one face; the flux in global component order into out[0..4] and, with
+// dual energy, the GasEnergy flux into out[5].
+template <class P, int DIR, bool DE>
+__device__ __forceinline__ void FaceFlux(const RState<typename P::T> &l, const RState<typename P::T> &r,
+                                         typename P::T gamma, typename P::T *out)
+{
+  using T = typename P::T;
+  T fge   = T(0);
+  Cons<T> f = Hllc<P, DE>(l, r, gamma, &fge);
+  FluxGlobal<P, DIR>(f, out);
+  if constexpr (DE) out[kGasEnergy] = fge;
+}
+
+// Dual energy: the DE pressure of a cell and its centred velocity-difference source
+// 0.5 P (dtodx (vx_imo - vx_ipo) + dtody (vy_jmo - vy_jpo) + dtodz (vz_kmo - vz_kpo)),
+// as Update_Conserved_Variables_3D_half (VL_3D_cuda.cu:409-441, 471-478) and
+// Partial_Update_Advected_Internal_Energy_3D (hydro_cuda.cu:779-810) form it, in FP64.
+// u: the cell's conserved values; d*: the neighbours' velocity differences
+// (vx_imo - vx_ipo, vy_jmo - vy_jpo, vz_kmo - vz_kpo).
+__device__ __forceinline__ double DePressure(const double u[6], double gamma)
+{
+  double d_inv = 1.0 / u[0];
+  double vx = u[1] * d_inv, vy = u[2] * d_inv, vz = u[3] * d_inv;
+  double E_kin = 0.5 * u[0] * SquareMag(vx, vy, vz);
+  return Fmax(PressureFromDE(u[4], u[4] - E_kin, u[5], gamma), kTinyNumber);
+}
+__device__ __forceinline__ double PdvTerm(double P, double dtodx, double dtody, double dtodz, double dvx, double dvy,
+                                          double dvz)
+{
+  return 0.5 * P * (dtodx * dvx + dtody * dvy + dtodz * dvz);
+}
+
 // LDS of one stage-1 tile.
-template <class P, int TX, int TY>
+template <class P, int TX, int TY, bool DE = false>
 struct Stage1Smem {
   using T = typename P::T;
-  PcmCell<T> plane[TY + 2][TX + 2];  // staged plane with a 1-cell halo
-  T fx[5][TY][TX + 1];
-  T fy[5][TY + 1][TX];
+  PcmCell<T, DE> plane[TY + 2][TX + 2];  // staged plane with a 1-cell halo
+  T fx[kNumCons<DE>][TY][TX + 1];
+  T fy[kNumCons<DE>][TY + 1][TX];
 };
 
 // Stage 1 of one tile (bx, by) x z-chunk bz of the half-step region [lo, hi) per
 // axis, lo = ng - R1, hi = n - ng + R1.  Called by every lane of a TX*TY workgroup.
-template <class P, int TX, int TY>
+// With dual energy (DE) it also advances GasEnergy with its pdV source, stores the
+// half-step specific GasEnergy in W and the full-step pdV source of every real cell
+// in a.pdv (Partial_Update_Advected_Internal_Energy_3D reads the state at the start
+// of the step, which only this stage still sees).
+template <class P, int TX, int TY, bool DE = false>
 __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, int by, int bz,
-                                           Stage1Smem<P, TX, TY> &sm)
+                                           Stage1Smem<P, TX, TY, DE> &sm)
 {
   using T            = typename P::T;
   constexpr int NT   = TX * TY;
   constexpr int NFX  = (TX + 1) * TY;
   constexpr int NFY  = TX * (TY + 1);
+  constexpr int NC   = kNumCons<DE>;
   auto &plane        = sm.plane;
   auto &fx           = sm.fx;
   auto &fy           = sm.fy;
@@ -218,7 +279,8 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
   const int k1 = min(k0 + a.kchunk, hi_z);
   const bool own = (i < hi_x) && (j < hi_y);
   const T gamma  = T(a.gamma);
-  const double dt    = 0.5 * StepDt(a);
+  const double dt_full = StepDt(a);
+  const double dt    = 0.5 * dt_full;
   const double dtodx = dt / a.dx, dtody = dt / a.dy, dtodz = dt / a.dz;
   const double *U    = a.U;
   T *W               = reinterpret_cast<T *>(a.W);
@@ -230,25 +292,23 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
   const int ic = min(i, nx - 1), jc = min(j, ny - 1);
   auto load_cell = [&](int ii, int jj, int kk) {
     size_t id = Lin(ii, jj, kk, nx, ny);
-    return MakePcmCell<P>(U[id], U[nc + id], U[2 * nc + id], U[3 * nc + id], U[4 * nc + id], a.gamma);
+    return MakePcmCell<P, DE>(U[id], U[nc + id], U[2 * nc + id], U[3 * nc + id], U[4 * nc + id],
+                              DE ? U[kGasEnergy * nc + id] : 0.0, a.gamma);
   };
 
-  PcmCell<T> c_lo = load_cell(ic, jc, k0 - 1);  // cell k-1
-  PcmCell<T> c_mid = load_cell(ic, jc, k0);     // cell k
-  double u_mid[5];
-  T fz_lo[5];
-  {
-    Cons<T> f = Hllc<P>(PcmSide<P, 2>(c_lo, gamma), PcmSide<P, 2>(c_mid, gamma), gamma);
-    FluxGlobal<P, 2>(f, fz_lo);
-  }
+  PcmCell<T, DE> c_lo  = load_cell(ic, jc, k0 - 1);  // cell k-1
+  PcmCell<T, DE> c_mid = load_cell(ic, jc, k0);      // cell k
+  double u_mid[NC];
+  T fz_lo[NC];
+  FaceFlux<P, 2, DE>(PcmSide<P, 2, DE>(c_lo, gamma), PcmSide<P, 2, DE>(c_mid, gamma), gamma, fz_lo);
 
   for (int k = k0; k < k1; k++) {
     {
       size_t id = Lin(ic, jc, k, nx, ny);
 #pragma unroll
-      for (int f = 0; f < 5; f++) u_mid[f] = U[f * nc + id];
+      for (int f = 0; f < NC; f++) u_mid[f] = U[f * nc + id];
     }
-    PcmCell<T> c_hi = load_cell(ic, jc, k + 1);
+    PcmCell<T, DE> c_hi = load_cell(ic, jc, k + 1);
     __syncthreads();  // previous plane fully consumed
     plane[ty + 1][tx + 1] = c_mid;
     // halo: x sides (PY-2 rows x 2) and y sides (PX-2 cols x 2), no corners
@@ -269,64 +329,106 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
     // x faces: face fi sits between plane cells (fi, fj+1) and (fi+1, fj+1)
     for (int it = tid; it < NFX; it += NT) {
       int fj = it / (TX + 1), fi = it - fj * (TX + 1);
-      Cons<T> f = Hllc<P>(PcmSide<P, 0>(plane[fj + 1][fi], gamma), PcmSide<P, 0>(plane[fj + 1][fi + 1], gamma), gamma);
-      T g[5];
-      FluxGlobal<P, 0>(f, g);
+      T g[NC];
+      FaceFlux<P, 0, DE>(PcmSide<P, 0, DE>(plane[fj + 1][fi], gamma), PcmSide<P, 0, DE>(plane[fj + 1][fi + 1], gamma),
+                         gamma, g);
 #pragma unroll
-      for (int q = 0; q < 5; q++) fx[q][fj][fi] = g[q];
+      for (int q = 0; q < NC; q++) fx[q][fj][fi] = g[q];
     }
     for (int it = tid; it < NFY; it += NT) {
       int fj = it / TX, fi = it - fj * TX;
-      Cons<T> f = Hllc<P>(PcmSide<P, 1>(plane[fj][fi + 1], gamma), PcmSide<P, 1>(plane[fj + 1][fi + 1], gamma), gamma);
-      T g[5];
-      FluxGlobal<P, 1>(f, g);
+      T g[NC];
+      FaceFlux<P, 1, DE>(PcmSide<P, 1, DE>(plane[fj][fi + 1], gamma), PcmSide<P, 1, DE>(plane[fj + 1][fi + 1], gamma),
+                         gamma, g);
 #pragma unroll
-      for (int q = 0; q < 5; q++) fy[q][fj][fi] = g[q];
+      for (int q = 0; q < NC; q++) fy[q][fj][fi] = g[q];
     }
-    T fz_hi[5];
-    {
-      Cons<T> f = Hllc<P>(PcmSide<P, 2>(c_mid, gamma), PcmSide<P, 2>(c_hi, gamma), gamma);
-      FluxGlobal<P, 2>(f, fz_hi);
-    }
+    T fz_hi[NC];
+    FaceFlux<P, 2, DE>(PcmSide<P, 2, DE>(c_mid, gamma), PcmSide<P, 2, DE>(c_hi, gamma), gamma, fz_hi);
     __syncthreads();
     if (own) {
-      double uh[5];
+      double uh[NC];
 #pragma unroll
-      for (int q = 0; q < 5; q++) {
+      for (int q = 0; q < NC; q++) {
         // (((U + dtodx dFx) + dtody dFy) + dtodz dFz), VL_3D_cuda.cu:444-461
         uh[q] = u_mid[q] + dtodx * (double(fx[q][ty][tx]) - double(fx[q][ty][tx + 1])) +
                 dtody * (double(fy[q][ty][tx]) - double(fy[q][ty + 1][tx])) +
                 dtodz * (double(fz_lo[q]) - double(fz_hi[q]));
       }
+      if constexpr (DE) {
+        // velocity differences of the neighbours at the start of the step, each a
+        // division as in VL_3D_cuda.cu:434-439 (FP64; the mixed form's staged
+        // velocities are FP32 and are not used for this source)
+        double dvx, dvy, dvz;
+        if constexpr (P::kExact) {
+          dvx = plane[ty + 1][tx].vx - plane[ty + 1][tx + 2].vx;
+          dvy = plane[ty][tx + 1].vy - plane[ty + 2][tx + 1].vy;
+          dvz = c_lo.vz - c_hi.vz;
+        } else {
+          const size_t id = Lin(i, j, k, nx, ny), sy = size_t(nx), sz = size_t(nx) * ny;
+          dvx = U[nc + id - 1] / U[id - 1] - U[nc + id + 1] / U[id + 1];
+          dvy = U[2 * nc + id - sy] / U[id - sy] - U[2 * nc + id + sy] / U[id + sy];
+          dvz = U[3 * nc + id - sz] / U[id - sz] - U[3 * nc + id + sz] / U[id + sz];
+        }
+        const double Pde = DePressure(u_mid, a.gamma);
+        uh[kGasEnergy] += PdvTerm(Pde, dtodx, dtody, dtodz, dvx, dvy, dvz);
+        if (i >= a.ng && i < nx - a.ng && j >= a.ng && j < ny - a.ng && k >= a.ng && k < nz - a.ng) {
+          const double fx_ = dt_full / a.dx, fy_ = dt_full / a.dy, fz_ = dt_full / a.dz;
+          a.pdv[Lin(i, j, k, nx, ny)] = PdvTerm(Pde, fx_, fy_, fz_, dvx, dvy, dvz);
+        }
+      }
       if ((a.flags & kFlagDensityFloor) && uh[0] < a.density_floor) {  // VL_3D_cuda.cu:479-493
         double d0 = uh[0];
         uh[0]     = a.density_floor;
 #pragma unroll
-        for (int q = 1; q < 5; q++) uh[q] *= (a.density_floor / d0);
+        for (int q = 1; q < NC; q++) uh[q] *= (a.density_floor / d0);
       }
-      Prim<double> w = LoadPrim(uh[0], uh[1], uh[2], uh[3], uh[4], a.gamma);
-      size_t id      = Lin(i, j, k, nx, ny);
-      W[id]          = T(w.d);
-      W[nc + id]     = T(w.vx);
-      W[2 * nc + id] = T(w.vy);
-      W[3 * nc + id] = T(w.vz);
-      W[4 * nc + id] = P::kExact ? T(uh[4]) : T(w.p);  // see WCell
+      size_t id = Lin(i, j, k, nx, ny);
+      if constexpr (DE) {
+        // Load_Data under DE (reconstruction.h:179-191): the exact form keeps E and
+        // GasEnergy and recovers p and ge per sweep; the others store p and ge
+        if constexpr (P::kExact) {
+          W[id]          = T(uh[0]);
+          W[nc + id]     = T(uh[1] / uh[0]);
+          W[2 * nc + id] = T(uh[2] / uh[0]);
+          W[3 * nc + id] = T(uh[3] / uh[0]);
+          W[4 * nc + id] = T(uh[4]);
+          W[5 * nc + id] = T(uh[5]);
+        } else {
+          double vx = uh[1] / uh[0], vy = uh[2] / uh[0], vz = uh[3] / uh[0];
+          double E_kin   = 0.5 * uh[0] * SquareMag(vx, vy, vz);
+          W[id]          = T(uh[0]);
+          W[nc + id]     = T(vx);
+          W[2 * nc + id] = T(vy);
+          W[3 * nc + id] = T(vz);
+          W[4 * nc + id] = T(PressureFromDE(uh[4], uh[4] - E_kin, uh[5], a.gamma));
+          W[5 * nc + id] = T(uh[5] / uh[0]);
+        }
+      } else {
+        Prim<double> w = LoadPrim(uh[0], uh[1], uh[2], uh[3], uh[4], a.gamma);
+        W[id]          = T(w.d);
+        W[nc + id]     = T(w.vx);
+        W[2 * nc + id] = T(w.vy);
+        W[3 * nc + id] = T(w.vz);
+        W[4 * nc + id] = P::kExact ? T(uh[4]) : T(w.p);  // see WCell
+      }
     }
 #pragma unroll
-    for (int q = 0; q < 5; q++) fz_lo[q] = fz_hi[q];
+    for (int q = 0; q < NC; q++) fz_lo[q] = fz_hi[q];
+    c_lo  = c_mid;
     c_mid = c_hi;
   }
 }
 
-template <class P, int TX, int TY>
+template <class P, int TX, int TY, bool DE = false>
 __global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
 {
-  __shared__ Stage1Smem<P, TX, TY> sm;
+  __shared__ Stage1Smem<P, TX, TY, DE> sm;
   // the inverse-dt accumulator is reset here; stage 2 (next launch) fills it
   if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && threadIdx.x == 0) *a.dti = 0ull;
   int bx, by, bz;
   if (!TileOf(a, bx, by, bz)) return;
-  Stage1Tile<P, TX, TY>(a, R1, bx, by, bz, sm);
+  Stage1Tile<P, TX, TY, DE>(a, R1, bx, by, bz, sm);
 }
 
 // ============================================================================ stage 2
@@ -334,33 +436,50 @@ __global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
 // PolicyExact where it holds the energy: Load_Data recovers pressure from the
 // velocities in the sweep's rotated order (reconstruction.h:194-201 with
 // SquareMagnitude, math_utilities.h:78-97), so its last bit depends on the sweep
-// direction and is recomputed per sweep.
-template <typename T>
+// direction and is recomputed per sweep.  With dual energy the sixth slot holds the
+// specific GasEnergy, or under PolicyExact the conserved GasEnergy (Load_Data
+// divides it by the density, reconstruction.h:191).
+template <typename T, bool DE = false>
 struct WCell {
   T d, vx, vy, vz, p_or_E;
 };
+template <typename T>
+struct WCell<T, true> {
+  T d, vx, vy, vz, p_or_E, g;
+};
 
-template <class P, int RK, int DIR>
-__device__ __forceinline__ void ReconDir(const WCell<typename P::T> *s, typename P::T gamma,
-                                         Prim<typename P::T> &L, Prim<typename P::T> &R)
+template <class P, int RK, int DIR, bool DE = false>
+__device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, typename P::T gamma,
+                                         Prim<typename P::T> &L, Prim<typename P::T> &R, typename P::T *geL = nullptr,
+                                         typename P::T *geR = nullptr)
 {
   using T          = typename P::T;
   constexpr int NS = 2 * ReconTraits<RK>::kRadius + 1;
   Prim<T> st[NS];
+  T ge[NS];
 #pragma unroll
   for (int m = 0; m < NS; m++) {
     st[m].d = s[m].d;
     Rot<DIR>(s[m].vx, s[m].vy, s[m].vz, st[m].vx, st[m].vy, st[m].vz);
     if constexpr (P::kExact) {
-      st[m].p = PressureFromCons(s[m].p_or_E, st[m].d, st[m].vx, st[m].vy, st[m].vz, gamma);
+      if constexpr (DE) {
+        T E_kin  = T(0.5) * st[m].d * SquareMag(st[m].vx, st[m].vy, st[m].vz);
+        st[m].p  = PressureFromDE(s[m].p_or_E, s[m].p_or_E - E_kin, s[m].g, gamma);
+        ge[m]    = s[m].g / st[m].d;
+      } else {
+        st[m].p = PressureFromCons(s[m].p_or_E, st[m].d, st[m].vx, st[m].vy, st[m].vz, gamma);
+      }
     } else {
       st[m].p = s[m].p_or_E;
+      if constexpr (DE) ge[m] = s[m].g;
     }
   }
   if constexpr (RK == kPPMC) {
     ReconPpmc<P>(st, gamma, L, R);
+    if constexpr (DE) ReconPpmcScalar<P>(ge, *geL, *geR);
   } else {
     ReconPlmc<P>(st, gamma, L, R);
+    if constexpr (DE) ReconPlmcScalar<P>(ge, *geL, *geR);
   }
 }
 
@@ -384,22 +503,68 @@ struct Stage2Occupancy {
   static constexpr int kMinWaves = P::kId == PolicyMixed::kId ? 1 : 2;
 };
 
+// Interface values of the passive GasEnergy (dual energy only).
+template <typename T, int N, bool DE>
+struct GeIfaces {
+  __device__ T *L() { return nullptr; }
+  __device__ T *R() { return nullptr; }
+};
+template <typename T, int N>
+struct GeIfaces<T, N, true> {
+  T gL[N], gR[N];
+  __device__ T *L() { return gL; }
+  __device__ T *R() { return gR; }
+};
+
 // LDS of one stage-2 tile.
-template <class P, int RK, int TX, int TY>
+template <class P, int RK, int TX, int TY, bool DE = false>
 struct Stage2Smem {
   using T = typename P::T;
   using S = Stage2Shape<P, RK, TX, TY>;
-  WCell<T> plane[S::PY][S::PX];
+  WCell<T, DE> plane[S::PY][S::PX];
   Prim<T> qL[S::NQ];
   Prim<T> qR[S::NQ];
-  T fl[5][S::NF];
+  GeIfaces<T, S::NQ, DE> qg;
+  T fl[kNumCons<DE>][S::NF];
   double red[TX * TY / 64];
 };
 
+// Every periodic image of real cell (i, j, k) inside the ghost layers gets the
+// cell's values (the host enables this only when every axis holds >= ng real cells,
+// so each image lands in a ghost layer and maps back to this cell).  Per axis: the
+// cell itself, then up to two images (+L, -L); selects instead of small arrays keep
+// the coordinates in registers.
+template <int NF>
+__device__ __forceinline__ void WritePeriodicImages(double *U, size_t nc, int nx, int ny, int nz, int ng, int i, int j,
+                                                    int k, const double *un)
+{
+  const int Lx = nx - 2 * ng, Ly = ny - 2 * ng, Lz = nz - 2 * ng;
+  const bool xp = i < 2 * ng, xm = i >= nx - 2 * ng;
+  const bool yp = j < 2 * ng, ym = j >= ny - 2 * ng;
+  const bool zp = k < 2 * ng, zm = k >= nz - 2 * ng;
+  const int x1 = xp ? i + Lx : i - Lx, y1 = yp ? j + Ly : j - Ly, z1 = zp ? k + Lz : k - Lz;
+  const int x2 = i - Lx, y2 = j - Ly, z2 = k - Lz;  // second image only when both apply
+  const int nxi = 1 + xp + xm, nyi = 1 + yp + ym, nzi = 1 + zp + zm;
+  for (int c = 1; c < nxi * nyi * nzi; c++) {
+    int ax = c % nxi, ay = (c / nxi) % nyi, az = c / (nxi * nyi);
+    const int gx = ax == 0 ? i : (ax == 1 ? x1 : x2);
+    const int gy = ay == 0 ? j : (ay == 1 ? y1 : y2);
+    const int gz = az == 0 ? k : (az == 1 ? z1 : z2);
+    size_t g = Lin(gx, gy, gz, nx, ny);
+#pragma unroll
+    for (int q = 0; q < NF; q++) U[q * nc + g] = un[q];
+  }
+}
+
 // Stage 2 of one tile (bx, by) x z-chunk bz of the real cells.  Called by every
 // lane of a TX*TY workgroup; the block's inverse-dt maximum goes to *a.dti.
-template <class P, int RK, int TX, int TY>
-__device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, int bz, Stage2Smem<P, RK, TX, TY> &sm)
+// With dual energy (DE) it also updates GasEnergy (its flux differences on top of
+// the pdV source stage 1 stored) and leaves the temperature floor, the inverse-dt
+// reduction and the ghost images to the internal-energy selection that follows
+// (DeSelectKernel / DeSyncKernel).
+template <class P, int RK, int TX, int TY, bool DE = false>
+__device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, int bz,
+                                           Stage2Smem<P, RK, TX, TY, DE> &sm)
 {
   using T          = typename P::T;
   using S          = Stage2Shape<P, RK, TX, TY>;
@@ -407,9 +572,12 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
   constexpr int r  = ReconTraits<RK>::kRadius;
   constexpr int NS = 2 * r + 1;
   constexpr int H  = S::H;
+  constexpr int NC = kNumCons<DE>;
   auto &plane      = sm.plane;
   auto &qL         = sm.qL;
   auto &qR         = sm.qR;
+  T *qgL           = sm.qg.L();
+  T *qgR           = sm.qg.R();
   auto &fl         = sm.fl;
   auto &red        = sm.red;
 
@@ -436,27 +604,31 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
 
   auto loadW = [&](int ii, int jj, int kk) {
     size_t id = Lin(ii, jj, kk, nx, ny);
-    return WCell<T>{W[id], W[nc + id], W[2 * nc + id], W[3 * nc + id], W[4 * nc + id]};
+    WCell<T, DE> c;
+    c.d = W[id], c.vx = W[nc + id], c.vy = W[2 * nc + id], c.vz = W[3 * nc + id], c.p_or_E = W[4 * nc + id];
+    if constexpr (DE) c.g = W[5 * nc + id];
+    return c;
   };
 
   // z window of this lane's column: win[m] = W(c - r + m) for the cell c whose z
   // interfaces are reconstructed next.  Prologue: cells k0-1 and k0 give the face k0-1/2.
-  WCell<T> win[NS];
+  WCell<T, DE> win[NS];
 #pragma unroll
   for (int m = 0; m < NS; m++) win[m] = loadW(ic, jc, k0 - 1 - r + m);
   Prim<T> qLz, qRz;
-  ReconDir<P, RK, 2>(win, gamma, qLz, qRz);  // cell k0-1
-  Prim<T> qL_lo = qLz;                       // left state of face k0-1/2
+  T gLz = T(0), gRz = T(0);
+  ReconDir<P, RK, 2, DE>(win, gamma, qLz, qRz, &gLz, &gRz);  // cell k0-1
+  Prim<T> qL_lo = qLz;                                       // left state of face k0-1/2
+  T gL_lo       = gLz;
 #pragma unroll
   for (int m = 0; m < NS - 1; m++) win[m] = win[m + 1];
   win[NS - 1] = loadW(ic, jc, k0 + r);
-  ReconDir<P, RK, 2>(win, gamma, qLz, qRz);  // cell k0
-  T fz_lo[5];                                // flux through face k-1/2 of the current cell
-  {
-    Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL_lo, gamma, inv_gm1), RStateFromPrim<P>(qRz, gamma, inv_gm1), gamma);
-    FluxGlobal<P, 2>(f, fz_lo);
-  }
+  ReconDir<P, RK, 2, DE>(win, gamma, qLz, qRz, &gLz, &gRz);  // cell k0
+  T fz_lo[NC];                                               // flux through face k-1/2 of the current cell
+  FaceFlux<P, 2, DE>(RStateFromPrim<P, DE>(qL_lo, gamma, inv_gm1, gL_lo), RStateFromPrim<P, DE>(qRz, gamma, inv_gm1, gRz),
+                     gamma, fz_lo);
   qL_lo = qLz;  // left state of face k0+1/2
+  gL_lo = gLz;
   double dti_max = 0.0;
 
   for (int k = k0; k < k1; k++) {
@@ -483,69 +655,88 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
 #pragma unroll
     for (int m = 0; m < NS - 1; m++) win[m] = win[m + 1];
     win[NS - 1] = loadW(ic, jc, k + 1 + r);
-    ReconDir<P, RK, 2>(win, gamma, qLz, qRz);
-    T fz_hi[5];
-    {
-      Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL_lo, gamma, inv_gm1), RStateFromPrim<P>(qRz, gamma, inv_gm1), gamma);
-      FluxGlobal<P, 2>(f, fz_hi);
-    }
+    ReconDir<P, RK, 2, DE>(win, gamma, qLz, qRz, &gLz, &gRz);
+    T fz_hi[NC];
+    FaceFlux<P, 2, DE>(RStateFromPrim<P, DE>(qL_lo, gamma, inv_gm1, gL_lo),
+                       RStateFromPrim<P, DE>(qRz, gamma, inv_gm1, gRz), gamma, fz_hi);
     qL_lo = qLz;
+    gL_lo = gLz;
     __syncthreads();
     // x reconstruction of cells -1..TX of every row
     for (int it = tid; it < S::NQX; it += NT) {
       int rj = it / (TX + 2), ri = it - rj * (TX + 2);  // plane column of cell = ri - 1 + H
-      WCell<T> s[NS];
+      WCell<T, DE> s[NS];
 #pragma unroll
       for (int m = 0; m < NS; m++) s[m] = plane[rj + H][ri - 1 + H - r + m];
-      ReconDir<P, RK, 0>(s, gamma, qL[it], qR[it]);
+      if constexpr (DE) {
+        ReconDir<P, RK, 0, DE>(s, gamma, qL[it], qR[it], &qgL[it], &qgR[it]);
+      } else {
+        ReconDir<P, RK, 0>(s, gamma, qL[it], qR[it]);
+      }
     }
     __syncthreads();
     for (int it = tid; it < S::NFX; it += NT) {
       int fj = it / (TX + 1), fi = it - fj * (TX + 1);  // between recon cells fi and fi+1 of row fj
       int ql = fj * (TX + 2) + fi;
-      Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL[ql], gamma, inv_gm1), RStateFromPrim<P>(qR[ql + 1], gamma, inv_gm1),
-                          gamma);
-      T g[5];
-      FluxGlobal<P, 0>(f, g);
+      T g[NC];
+      if constexpr (DE) {
+        FaceFlux<P, 0, DE>(RStateFromPrim<P, DE>(qL[ql], gamma, inv_gm1, qgL[ql]),
+                           RStateFromPrim<P, DE>(qR[ql + 1], gamma, inv_gm1, qgR[ql + 1]), gamma, g);
+      } else {
+        FaceFlux<P, 0, DE>(RStateFromPrim<P>(qL[ql], gamma, inv_gm1), RStateFromPrim<P>(qR[ql + 1], gamma, inv_gm1),
+                           gamma, g);
+      }
 #pragma unroll
-      for (int q = 0; q < 5; q++) fl[q][it] = g[q];
+      for (int q = 0; q < NC; q++) fl[q][it] = g[q];
     }
     __syncthreads();
-    double acc[5];
+    double acc[NC];
     {
       int fa = ty * (TX + 1) + tx;
 #pragma unroll
-      for (int q = 0; q < 5; q++) acc[q] = dtodx * (double(fl[q][fa]) - double(fl[q][fa + 1]));
+      for (int q = 0; q < NC; q++) acc[q] = dtodx * (double(fl[q][fa]) - double(fl[q][fa + 1]));
     }
     // y reconstruction of cells -1..TY of every column
     for (int it = tid; it < S::NQY; it += NT) {
       int rj = it / TX, ri = it - rj * TX;
-      WCell<T> s[NS];
+      WCell<T, DE> s[NS];
 #pragma unroll
       for (int m = 0; m < NS; m++) s[m] = plane[rj - 1 + H - r + m][ri + H];
-      ReconDir<P, RK, 1>(s, gamma, qL[it], qR[it]);
+      if constexpr (DE) {
+        ReconDir<P, RK, 1, DE>(s, gamma, qL[it], qR[it], &qgL[it], &qgR[it]);
+      } else {
+        ReconDir<P, RK, 1>(s, gamma, qL[it], qR[it]);
+      }
     }
     __syncthreads();
     for (int it = tid; it < S::NFY; it += NT) {
       int fj = it / TX, fi = it - fj * TX;  // between recon rows fj and fj+1 of column fi
-      Cons<T> f = Hllc<P>(RStateFromPrim<P>(qL[fj * TX + fi], gamma, inv_gm1),
-                          RStateFromPrim<P>(qR[(fj + 1) * TX + fi], gamma, inv_gm1), gamma);
-      T g[5];
-      FluxGlobal<P, 1>(f, g);
+      const int lq = fj * TX + fi, rq = (fj + 1) * TX + fi;
+      T g[NC];
+      if constexpr (DE) {
+        FaceFlux<P, 1, DE>(RStateFromPrim<P, DE>(qL[lq], gamma, inv_gm1, qgL[lq]),
+                           RStateFromPrim<P, DE>(qR[rq], gamma, inv_gm1, qgR[rq]), gamma, g);
+      } else {
+        FaceFlux<P, 1, DE>(RStateFromPrim<P>(qL[lq], gamma, inv_gm1), RStateFromPrim<P>(qR[rq], gamma, inv_gm1), gamma,
+                           g);
+      }
 #pragma unroll
-      for (int q = 0; q < 5; q++) fl[q][it] = g[q];
+      for (int q = 0; q < NC; q++) fl[q][it] = g[q];
     }
     __syncthreads();
     if (own) {
       const size_t id = Lin(i, j, k, nx, ny);
-      double un[5];
+      double un[NC];
       {
         int fa = ty * TX + tx;
-        // U += ((dtodx dFx + dtody dFy) + dtodz dFz), hydro_cuda.cu:241-257
+        // U += ((dtodx dFx + dtody dFy) + dtodz dFz), hydro_cuda.cu:241-257; GasEnergy
+        // first takes the pdV source (the separate Partial_Update kernel, VL_3D_cuda.cu:314-324)
 #pragma unroll
-        for (int q = 0; q < 5; q++) {
+        for (int q = 0; q < NC; q++) {
           double axy = acc[q] + dtody * (double(fl[q][fa]) - double(fl[q][fa + TX]));
-          un[q]      = U[q * nc + id] + (axy + dtodz * (double(fz_lo[q]) - double(fz_hi[q])));
+          double u0  = U[q * nc + id];
+          if (DE && q == kGasEnergy) u0 = u0 + a.pdv[id];
+          un[q] = u0 + (axy + dtodz * (double(fz_lo[q]) - double(fz_hi[q])));
         }
       }
       if ((a.flags & kFlagDensityFloor) && un[0] < a.density_floor) {  // hydro_cuda.cu:279-299
@@ -553,7 +744,7 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
           double d0 = un[0];
           un[0]     = a.density_floor;
 #pragma unroll
-          for (int q = 1; q < 5; q++) un[q] *= (a.density_floor / d0);
+          for (int q = 1; q < NC; q++) un[q] *= (a.density_floor / d0);
         } else {
           // Average_Cell_Single_Field (hydro_cuda.cu:1155-1177) over the six
           // neighbours as they are in memory now; the reference reads them
@@ -563,48 +754,31 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
                   6;
         }
       }
-      double dinv = 1.0 / un[0];
-      double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
-      if (a.flags & kFlagTemperatureFloor) {  // Temperature_Floor_Kernel, hydro_cuda.cu:1132-1146
-        double Ekin = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz);
-        double uint = P::kExact ? (un[4] - Ekin) / un[0] : (un[4] - Ekin) * dinv;
-        if (uint < a.U_floor) un[4] = Ekin + un[0] * a.U_floor;
-      }
+      if constexpr (DE) {
 #pragma unroll
-      for (int q = 0; q < 5; q++) U[q * nc + id] = un[q];
-      if (a.flags & kFlagReduceDt) {
-        dti_max = Fmax(dti_max, InverseCrossingTimeP<P>(un[4], un[0], dinv, vx, vy, vz, a.dx, a.dy, a.dz, inv_dx,
-                                                        inv_dy, inv_dz, a.gamma));
-      }
-      if (a.flags & kFlagPeriodicGhosts) {
-        // every periodic image of this real cell inside the ghost layers (the host
-        // enables this only when every axis holds >= ng real cells, so each image
-        // lands in a ghost layer and maps back to this cell)
-        // Per axis: the cell itself, then up to two images (+L, -L).  Selects instead
-        // of small arrays keep the coordinates in registers.
-        const int Lx = nx - 2 * ng, Ly = ny - 2 * ng, Lz = nz - 2 * ng;
-        const bool xp = i < 2 * ng, xm = i >= nx - 2 * ng;
-        const bool yp = j < 2 * ng, ym = j >= ny - 2 * ng;
-        const bool zp = k < 2 * ng, zm = k >= nz - 2 * ng;
-        const int x1 = xp ? i + Lx : i - Lx, y1 = yp ? j + Ly : j - Ly, z1 = zp ? k + Lz : k - Lz;
-        const int x2 = i - Lx, y2 = j - Ly, z2 = k - Lz;  // second image only when both apply
-        const int nxi = 1 + xp + xm, nyi = 1 + yp + ym, nzi = 1 + zp + zm;
-        for (int c = 1; c < nxi * nyi * nzi; c++) {
-          int ax = c % nxi, ay = (c / nxi) % nyi, az = c / (nxi * nyi);
-          const int gx = ax == 0 ? i : (ax == 1 ? x1 : x2);
-          const int gy = ay == 0 ? j : (ay == 1 ? y1 : y2);
-          const int gz = az == 0 ? k : (az == 1 ? z1 : z2);
-          size_t g = Lin(gx, gy, gz, nx, ny);
-#pragma unroll
-          for (int q = 0; q < 5; q++) U[q * nc + g] = un[q];
+        for (int q = 0; q < NC; q++) U[q * nc + id] = un[q];
+      } else {
+        double dinv = 1.0 / un[0];
+        double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
+        if (a.flags & kFlagTemperatureFloor) {  // Temperature_Floor_Kernel, hydro_cuda.cu:1132-1146
+          double Ekin = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz);
+          double uint = P::kExact ? (un[4] - Ekin) / un[0] : (un[4] - Ekin) * dinv;
+          if (uint < a.U_floor) un[4] = Ekin + un[0] * a.U_floor;
         }
+#pragma unroll
+        for (int q = 0; q < 5; q++) U[q * nc + id] = un[q];
+        if (a.flags & kFlagReduceDt) {
+          dti_max = Fmax(dti_max, InverseCrossingTimeP<P>(un[4], un[0], dinv, vx, vy, vz, a.dx, a.dy, a.dz, inv_dx,
+                                                          inv_dy, inv_dz, a.gamma));
+        }
+        if (a.flags & kFlagPeriodicGhosts) WritePeriodicImages<5>(U, nc, nx, ny, nz, ng, i, j, k, un);
       }
     }
 #pragma unroll
-    for (int q = 0; q < 5; q++) fz_lo[q] = fz_hi[q];
+    for (int q = 0; q < NC; q++) fz_lo[q] = fz_hi[q];
   }
 
-  if (a.flags & kFlagReduceDt) {
+  if (!DE && (a.flags & kFlagReduceDt)) {
     // wave max (wave64 cross-lane), then one LDS pass and one atomic per workgroup
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) dti_max = Fmax(dti_max, __shfl_xor(dti_max, off, 64));
@@ -618,14 +792,14 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
   }
 }
 
-template <class P, int RK, int TX, int TY>
+template <class P, int RK, int TX, int TY, bool DE = false>
 __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Stage2Occupancy<P>::kMinWaves)))
 Stage2Kernel(StepArgs a)
 {
-  __shared__ Stage2Smem<P, RK, TX, TY> sm;
+  __shared__ Stage2Smem<P, RK, TX, TY, DE> sm;
   int bx, by, bz;
   if (!TileOf(a, bx, by, bz)) return;
-  Stage2Tile<P, RK, TX, TY>(a, bx, by, bz, sm);
+  Stage2Tile<P, RK, TX, TY, DE>(a, bx, by, bz, sm);
 }
 
 }  // namespace chl

@@ -27,29 +27,41 @@ inline unsigned TileGrid(StepArgs &b, int gx, int gy, int gz)
   return unsigned(b.xcd_order ? (T + 7) / 8 * 8 : T);
 }
 
-template <class P, int TX, int TY>
+template <class P, int TX, int TY, bool DE = false>
 hipError_t LaunchStage1T(const StepArgs &a, int R1, hipStream_t s)
 {
   int ox = a.nx - 2 * a.ng + 2 * R1, oy = a.ny - 2 * a.ng + 2 * R1, oz = a.nz - 2 * a.ng + 2 * R1;
   StepArgs b     = a;
   unsigned grid  = TileGrid(b, CeilDiv(ox, TX), CeilDiv(oy, TY), CeilDiv(oz, a.kchunk));
-  hipLaunchKernelGGL((Stage1Kernel<P, TX, TY>), dim3(grid), dim3(TX * TY), 0, s, b, R1);
+  hipLaunchKernelGGL((Stage1Kernel<P, TX, TY, DE>), dim3(grid), dim3(TX * TY), 0, s, b, R1);
   return hipGetLastError();
 }
 
-template <class P, int RK, int TX, int TY>
+template <class P, int RK, int TX, int TY, bool DE = false>
 hipError_t LaunchStage2T(const StepArgs &a, hipStream_t s)
 {
   int ox = a.nx - 2 * a.ng, oy = a.ny - 2 * a.ng, oz = a.nz - 2 * a.ng;
   StepArgs b    = a;
   unsigned grid = TileGrid(b, CeilDiv(ox, TX), CeilDiv(oy, TY), CeilDiv(oz, a.kchunk));
-  hipLaunchKernelGGL((Stage2Kernel<P, RK, TX, TY>), dim3(grid), dim3(TX * TY), 0, s, b);
+  hipLaunchKernelGGL((Stage2Kernel<P, RK, TX, TY, DE>), dim3(grid), dim3(TX * TY), 0, s, b);
   return hipGetLastError();
 }
+
+// Dual-energy kernels are compiled for the first kNumDeTiles tile shapes only.
+constexpr int kNumDeTiles = 2;
 
 template <class P>
 hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 {
+  if (a.de) {
+    switch (tile) {
+      case 0:
+        return LaunchStage1T<P, 32, 8, true>(a, R1, s);
+      case 1:
+        return LaunchStage1T<P, 16, 16, true>(a, R1, s);
+    }
+    return hipErrorInvalidValue;
+  }
   switch (tile) {
     case 0:
       return LaunchStage1T<P, 32, 8>(a, R1, s);
@@ -68,6 +80,15 @@ hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 template <class P, int RK>
 hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
 {
+  if (a.de) {
+    switch (tile) {
+      case 0:
+        return LaunchStage2T<P, RK, 32, 8, true>(a, s);
+      case 1:
+        return LaunchStage2T<P, RK, 16, 16, true>(a, s);
+    }
+    return hipErrorInvalidValue;
+  }
   switch (tile) {
     case 0:
       return LaunchStage2T<P, RK, 32, 8>(a, s);

@@ -49,7 +49,7 @@ extern "C" {
 typedef struct chl_hydro_config {
   int nx, ny, nz;        /* total cells per axis incl. ghosts (Header::nx, ny, nz) */
   int n_ghost;           /* Header::n_ghost: PLMC 3, PPMC 4 (src/grid/grid3D.cpp:53-76) */
-  int n_fields;          /* 5 (hydro, no scalars / dual energy) */
+  int n_fields;          /* 5 (hydro), or 6 with dual_energy (GasEnergy last); no scalars */
   int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL */
   int riemann;           /* CHL_RIEMANN_HLLC */
   int precision;         /* CHL_PREC_* */
@@ -59,6 +59,7 @@ typedef struct chl_hydro_config {
   int tile1, tile2;      /* tile shapes of the two fused stages, -1 = default */
   int kchunk1, kchunk2;  /* z cells per workgroup in the two stages, 0 = default */
   int tile_order;        /* 0: each XCD takes a contiguous run of tiles (default); 1: dispatch order */
+  int dual_energy;       /* DE build option, 0/1: GasEnergy as field 5 (n_fields 6), 3D boxes only */
 } chl_hydro_config;
 
 typedef struct chl_hydro chl_hydro; /* owns the integrator scratch of one box */

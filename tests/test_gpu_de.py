@@ -1,0 +1,137 @@
+"""GPU parity of the dual-energy build (chl_hydro_config.dual_energy, the reference's DE
+option) against the oracle's DE branches (tests/test_oracle_de.py pins those).
+
+Bars:
+  * PolicyExact: bit-identical to the oracle's driver loop (all six fields, ghosts
+    included) on a box that exercises both sides of the DE_ETA_1 switch (a cold
+    supersonic slab in a warm medium), with and without a temperature floor;
+  * PolicyFast: per-field relative L1 <= 1e-12, PolicyMixed <= 1e-5 against the oracle;
+  * tile shapes, z-chunks and graph replay give identical bits; a uniform supersonic
+    flow is a fixed point in every form; the drop-in adapter built with -DDE is
+    bit-identical to the oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cholla_amd as chl
+from helpers import interior, rel_l1
+from oracle import oracle as orc
+from test_oracle_de import periodic_state
+
+pytestmark = pytest.mark.gpu
+G = 5.0 / 3.0
+TOL = {chl.PREC_FP64_EXACT: 0.0, chl.PREC_FP64: 1e-12, chl.PREC_MIXED: 1e-5}
+NG = {chl.RECON_PLMC: 3, chl.RECON_PPMC: 4}
+ORC_RECON = {chl.RECON_PLMC: orc.PLMC, chl.RECON_PPMC: orc.PPMC}
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _require_gpu():
+    if chl.device_count() < 1:
+        pytest.fail("no HIP device visible: the GPU tests need a real MI355X")
+
+
+def slab_box(n, ng, seed=3):
+    """Warm medium (p = 1, |v| ~ 0.3) with a cold supersonic slab (p = 1e-5, vx = 1.5,
+    thermal fraction ~1e-5 < DE_ETA_1) at |y - 0.5| < 0.25, seeded noise."""
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*(((np.arange(n) + 0.5) / n),) * 3, indexing="ij")
+    slab = np.abs(y - 0.5) < 0.25
+    d = np.where(slab, 2.0, 1.0) * (1 + 0.05 * rng.uniform(-1, 1, (n, n, n)))
+    vx = np.where(slab, 1.5, -0.3) + 0.05 * np.sin(2 * np.pi * y) + 0.02 * rng.uniform(-1, 1, (n, n, n))
+    vy = 0.05 * np.sin(4 * np.pi * x) + 0.02 * rng.uniform(-1, 1, (n, n, n))
+    vz = 0.02 * rng.uniform(-1, 1, (n, n, n))
+    p = np.where(slab, 1e-5, 1.0) * (1 + 0.1 * rng.uniform(-1, 1, (n, n, n)))
+    return periodic_state(n, n, n, ng, d, vx, vy, vz, p)
+
+
+def oracle_run(U, dims, ng, recon, steps, U_floor=0.0):
+    Uo = U.copy()
+    nx, ny, nz = dims
+    h = 1.0 / (nx - 2 * ng)
+    dts = np.zeros(steps)
+    n, t = orc.run(Uo, nx, ny, nz, ng, 6, h, h, h, G, ORC_RECON[recon], [1] * 6, 0.0, 1e30, steps, U_floor=U_floor,
+                   dt_log=dts, dual_energy_on=True)
+    return Uo, n, t
+
+
+def gpu_run(U, dims, ng, recon, precision, steps, U_floor=0.0, use_graph=True, **kw):
+    nx, ny, nz = dims
+    h = 1.0 / (nx - 2 * ng)
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(nx, ny, nz, ng, recon, precision, dual_energy=True, **kw)
+    n, t = hy.run(Ud, h, h, h, G, 0.0, 1e30, steps, use_graph=use_graph, U_floor=U_floor)
+    A = Ud.download()
+    hy.close()
+    return A, n, t
+
+
+@pytest.mark.parametrize("U_floor", [0.0, 2e-5])
+@pytest.mark.parametrize("recon", [chl.RECON_PLMC, chl.RECON_PPMC])
+def test_de_exact_matches_oracle(recon, U_floor):
+    n, ng = 16, NG[recon]
+    U, dims = slab_box(n, ng)
+    N = dims[0]
+    say("DE exact recon", recon, "U_floor", U_floor)
+    Uo, no, to = oracle_run(U, dims, ng, recon, 10, U_floor)
+    # both sides of the DE_ETA_1 switch are present
+    a = interior(Uo, 6, N, N, N, ng)
+    frac = a[5] / a[4]
+    assert (frac < 1e-3).sum() > 100 and (frac > 1e-2).sum() > 100
+    A, ns, t = gpu_run(U, dims, ng, recon, chl.PREC_FP64_EXACT, 10, U_floor)
+    assert ns == no == 10 and t == to
+    np.testing.assert_array_equal(A, Uo)
+
+
+@pytest.mark.parametrize("precision", [chl.PREC_FP64, chl.PREC_MIXED])
+@pytest.mark.parametrize("recon", [chl.RECON_PLMC, chl.RECON_PPMC])
+def test_de_fast_and_mixed_l1(recon, precision):
+    n, ng = 16, NG[recon]
+    U, dims = slab_box(n, ng, seed=5)
+    N = dims[0]
+    Uo, no, to = oracle_run(U, dims, ng, recon, 10)
+    A, ns, t = gpu_run(U, dims, ng, recon, precision, 10)
+    assert ns == no == 10
+    a, b = interior(A, 6, N, N, N, ng), interior(Uo, 6, N, N, N, ng)
+    mag = np.sqrt(b[1] ** 2 + b[2] ** 2 + b[3] ** 2).sum()
+    l1 = [np.abs(a[f] - b[f]).sum() / (mag if f in (1, 2, 3) else np.abs(b[f]).sum()) for f in range(6)]
+    say("DE recon", recon, "precision", precision, "L1", ["%.2e" % v for v in l1])
+    assert max(l1) <= TOL[precision], l1
+    assert abs(t - to) <= 1e-6 * to
+
+
+def test_de_tiles_chunks_and_graph_identical():
+    recon, ng, n = chl.RECON_PPMC, 4, 20
+    U, dims = slab_box(n, ng, seed=7)
+    outs = []
+    for tiles, chunks, graph in (((0, 0), (0, 0), True), ((1, 1), (7, 5), True), ((0, 1), (5, 9), False)):
+        say("DE tiles", tiles, "chunks", chunks, "graph", graph)
+        A, ns, t = gpu_run(U, dims, ng, recon, chl.PREC_FP64_EXACT, 6, use_graph=graph, tile1=tiles[0],
+                           tile2=tiles[1], kchunk1=chunks[0], kchunk2=chunks[1])
+        outs.append(A)
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+
+
+@pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
+def test_de_uniform_supersonic_fixed_point(precision):
+    n, ng = 8, 4
+    one = np.ones((n, n, n))
+    U, dims = periodic_state(n, n, n, ng, one, 3.0 * one, -2.0 * one, 1.5 * one, 1e-4 * one)
+    A, ns, t = gpu_run(U, dims, ng, chl.RECON_PPMC, precision, 5)
+    assert ns == 5
+    np.testing.assert_array_equal(A, U)
+
+
+def test_de_config_checks():
+    with pytest.raises(chl.ChollaError):
+        chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, dual_energy=True, tile1=2)
+    with pytest.raises(chl.ChollaError):
+        chl.Hydro(100, 1, 1, 3, chl.RECON_PLMC, chl.PREC_FP64_EXACT, dual_energy=True)

@@ -117,3 +117,22 @@ def test_dropin_mhd_orszag_tang(tmp_path):
     n_o, t_o = orc.mhd_run(Uo, N, N, N, ng, h, h, h, g, [1] * 6, 0.0, 1e30, 6)
     assert ns == n_o == 6 and t == t_o
     np.testing.assert_array_equal(A, Uo)
+
+
+def test_dropin_dual_energy_ppmc(tmp_path):
+    """The DE build of the adapter (-DPPMC -DDE): six fields with GasEnergy last, driven like
+    Cholla's loop (integrator with internal-energy selection and sync, temperature floor of
+    E and GasEnergy, dt), bit-identical to the oracle's DE driver loop."""
+    from test_gpu_de import slab_box
+
+    n, ng, g = 16, 4, 5 / 3
+    U, dims = slab_box(n, ng, seed=21)
+    N = dims[0]
+    h = 1.0 / n
+    A, ns, t = run_driver("dropin_driver_ppmc_de", U, dims, ng, (h, h, h), g, 0.0, 1e30, 8, [1] * 6, tmp_path,
+                          U_floor=2e-5, nf=6)
+    Uo = U.copy()
+    steps_o, to = orc.run(Uo, N, N, N, ng, 6, h, h, h, g, orc.PPMC, [1] * 6, 0.0, 1e30, 8, U_floor=2e-5,
+                          dual_energy_on=True)
+    assert ns == steps_o == 8 and t == to
+    np.testing.assert_array_equal(A, Uo)
