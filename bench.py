@@ -24,6 +24,23 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 BYTES_PER_CELL_UPDATE = 80  # FP64 state (5 fields) read once + written once
+# HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE (scaled by a plain copy of known size); written next to its summary.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage2_kh512_hbm_traffic.json")
+
+
+def measured_traffic(form, tile, n):
+    """The profiled HBM bytes per launch of stage 2 for this form, tile choice and box, if
+    the committed profile covers exactly that configuration (else None)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for rec in prof.get("records", []):
+        if rec.get("form") == form and rec.get("n") == n and list(rec.get("tile", [])) == list(tile or []):
+            return rec
+    return None
 
 
 def log(*a):
@@ -363,6 +380,7 @@ def main():
     dom_ms = max(r["stage1_ms"], r["stage2_ms"])
     dom = "stage2" if r["stage2_ms"] >= r["stage1_ms"] else "stage1"
     achieved = BYTES_PER_CELL_UPDATE * cells / (r["stage2_ms"] * 1e-3) / 1e9
+    trec = measured_traffic(head, r["tile"], n)
     out = {
         "metric": "cell-updates/s at 512^3 PPMC+HLLC, mixed-FP32 vs FP64, at stated L1 error",
         "value": r["cell_updates_per_s"],
@@ -379,7 +397,8 @@ def main():
         "config": {"workload": f"Kelvin-Helmholtz {n}^3 periodic, PPMC+HLLC, VL, CFL 0.3, form={head}",
                    "cells": cells, "n_ghost": ng, "gamma": g, "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "Stage2Kernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": trec["bytes_per_launch"] if trec else None,
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if trec else None, "kernel": "Stage2Kernel",
                      "dominant": dom, "dominant_ms": dom_ms,
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL_UPDATE * cells},
         "step_roofline_frac": r["cell_updates_per_s"] * BYTES_PER_CELL_UPDATE / 1e9 / HBM_PEAK_GBS,

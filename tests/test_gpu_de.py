@@ -3,9 +3,12 @@ option) against the oracle's DE branches (tests/test_oracle_de.py pins those).
 
 Bars:
   * PolicyExact: bit-identical to the oracle's driver loop (all six fields, ghosts
-    included) on a box that exercises both sides of the DE_ETA_1 switch (a cold
-    supersonic slab in a warm medium), with and without a temperature floor;
-  * PolicyFast: per-field relative L1 <= 1e-12, PolicyMixed <= 1e-5 against the oracle;
+    included) on a box that exercises both sides of the DE_ETA_1 switch (a hypersonic
+    slab in a slow medium), with and without a temperature floor;
+  * PolicyFast: per-field relative L1 <= 1e-12 against the oracle; PolicyMixed <= 5e-5:
+    the slab's internal energy is 3e-4 of its total energy, so FP32 rounding of the
+    energy fluxes reaches the internal energy amplified by ~3e3 (measured: 6e-6 density,
+    1.2e-5 energy and momentum, 1.3e-5 GasEnergy after 10 PPMC steps);
   * tile shapes, z-chunks and graph replay give identical bits; a uniform supersonic
     flow is a fixed point in every form; the drop-in adapter built with -DDE is
     bit-identical to the oracle.
@@ -22,7 +25,7 @@ from test_oracle_de import periodic_state
 
 pytestmark = pytest.mark.gpu
 G = 5.0 / 3.0
-TOL = {chl.PREC_FP64_EXACT: 0.0, chl.PREC_FP64: 1e-12, chl.PREC_MIXED: 1e-5}
+TOL = {chl.PREC_FP64_EXACT: 0.0, chl.PREC_FP64: 1e-12, chl.PREC_MIXED: 5e-5}
 NG = {chl.RECON_PLMC: 3, chl.RECON_PPMC: 4}
 ORC_RECON = {chl.RECON_PLMC: orc.PLMC, chl.RECON_PPMC: orc.PPMC}
 
@@ -39,16 +42,18 @@ def _require_gpu():
 
 
 def slab_box(n, ng, seed=3):
-    """Warm medium (p = 1, |v| ~ 0.3) with a cold supersonic slab (p = 1e-5, vx = 1.5,
-    thermal fraction ~1e-5 < DE_ETA_1) at |y - 0.5| < 0.25, seeded noise."""
+    """Uniform pressure and density with a hypersonic slab: |y - 0.5| < 0.25 moves at
+    vx = 100 (Mach ~77, thermal fraction ~3e-4 < DE_ETA_1, so the advected internal
+    energy is used there) through a medium at vx ~ 0.3 (thermal fraction ~0.97), with
+    seeded noise; the shear layers heat the slab edges across the switch."""
     rng = np.random.default_rng(seed)
     z, y, x = np.meshgrid(*(((np.arange(n) + 0.5) / n),) * 3, indexing="ij")
     slab = np.abs(y - 0.5) < 0.25
-    d = np.where(slab, 2.0, 1.0) * (1 + 0.05 * rng.uniform(-1, 1, (n, n, n)))
-    vx = np.where(slab, 1.5, -0.3) + 0.05 * np.sin(2 * np.pi * y) + 0.02 * rng.uniform(-1, 1, (n, n, n))
+    d = 1 + 0.05 * rng.uniform(-1, 1, (n, n, n))
+    vx = np.where(slab, 100.0, -0.3) + 0.05 * np.sin(2 * np.pi * y) + 0.02 * rng.uniform(-1, 1, (n, n, n))
     vy = 0.05 * np.sin(4 * np.pi * x) + 0.02 * rng.uniform(-1, 1, (n, n, n))
     vz = 0.02 * rng.uniform(-1, 1, (n, n, n))
-    p = np.where(slab, 1e-5, 1.0) * (1 + 0.1 * rng.uniform(-1, 1, (n, n, n)))
+    p = 1 + 0.05 * rng.uniform(-1, 1, (n, n, n))
     return periodic_state(n, n, n, ng, d, vx, vy, vz, p)
 
 
@@ -73,7 +78,7 @@ def gpu_run(U, dims, ng, recon, precision, steps, U_floor=0.0, use_graph=True, *
     return A, n, t
 
 
-@pytest.mark.parametrize("U_floor", [0.0, 2e-5])
+@pytest.mark.parametrize("U_floor", [0.0, 1.5])
 @pytest.mark.parametrize("recon", [chl.RECON_PLMC, chl.RECON_PPMC])
 def test_de_exact_matches_oracle(recon, U_floor):
     n, ng = 16, NG[recon]

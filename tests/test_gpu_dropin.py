@@ -130,9 +130,9 @@ def test_dropin_dual_energy_ppmc(tmp_path):
     N = dims[0]
     h = 1.0 / n
     A, ns, t = run_driver("dropin_driver_ppmc_de", U, dims, ng, (h, h, h), g, 0.0, 1e30, 8, [1] * 6, tmp_path,
-                          U_floor=2e-5, nf=6)
+                          U_floor=1.5, nf=6)
     Uo = U.copy()
-    steps_o, to = orc.run(Uo, N, N, N, ng, 6, h, h, h, g, orc.PPMC, [1] * 6, 0.0, 1e30, 8, U_floor=2e-5,
+    steps_o, to = orc.run(Uo, N, N, N, ng, 6, h, h, h, g, orc.PPMC, [1] * 6, 0.0, 1e30, 8, U_floor=1.5,
                           dual_energy_on=True)
     assert ns == steps_o == 8 and t == to
     np.testing.assert_array_equal(A, Uo)
