@@ -31,7 +31,8 @@ EXPORTS = [
     "chl_hllc_fluxes", "chl_ic_kelvin_helmholtz", "chl_ensemble_create", "chl_ensemble_destroy",
     "chl_ensemble_get_config", "chl_ensemble_persistent_pairs", "chl_ensemble_bind", "chl_ensemble_start",
     "chl_ensemble_advance", "chl_ensemble_advance_looped", "chl_ensemble_clocks", "chl_ensemble_autotune",
-    "chl_ensemble_layout", "chl_turbulence_modes", "chl_ic_turbulence", "chl_mhd_create", "chl_mhd_destroy",
+    "chl_ensemble_layout", "chl_turbulence_modes", "chl_ic_turbulence", "chl_init_gas_energy", "chl_mhd_create",
+    "chl_mhd_destroy",
     "chl_mhd_vl_step", "chl_mhd_run", "chl_mhd_time_steps", "chl_mhd_calc_dt", "chl_mhd_divergence",
     "chl_hlld_fluxes", "chl_mhd_reconstruct_plmc", "chl_ct_electric_fields", "chl_update_magnetic_field",
     "chl_magnetic_field_from_vector_potential", "chl_ic_orszag_tang",
@@ -60,6 +61,7 @@ class EnsembleConfig(ctypes.Structure):
         ("riemann", ctypes.c_int), ("precision", ctypes.c_int), ("density_floor", ctypes.c_int),
         ("temperature_floor", ctypes.c_int), ("launch", ctypes.c_int), ("tile1", ctypes.c_int),
         ("tile2", ctypes.c_int), ("kchunk1", ctypes.c_int), ("kchunk2", ctypes.c_int),
+        ("dual_energy", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -130,6 +132,7 @@ def lib():
         "chl_ensemble_layout": (_i, [ctypes.c_uint64, _i, _ip, _dp]),
         "chl_turbulence_modes": (_i, [ctypes.c_uint64, _dp, _i, _ip]),
         "chl_ic_turbulence": (_i, [_vp, _i, _i, _d, _d, _d, _d, ctypes.c_uint64]),
+        "chl_init_gas_energy": (_i, [_vp, _i, _i, _i]),
         "chl_mhd_create": (_i, [ctypes.POINTER(MhdConfig), ctypes.POINTER(_vp)]),
         "chl_mhd_destroy": (_i, [_vp]),
         "chl_mhd_vl_step": (_i, [_vp, _vp, _d, _d, _d, _d, _d, _d]),
@@ -330,9 +333,11 @@ class Ensemble:
     owned by the caller and bound with bind()."""
 
     def __init__(self, dims, n_ghost, reconstruction=RECON_PPMC, precision=PREC_MIXED, launch=LAUNCH_SPLIT,
-                 density_floor=True, temperature_floor=True, tile1=-1, tile2=-1, kchunk1=0, kchunk2=0):
+                 density_floor=True, temperature_floor=True, tile1=-1, tile2=-1, kchunk1=0, kchunk2=0,
+                 dual_energy=False):
         cfg = EnsembleConfig()
-        cfg.n_ghost, cfg.n_fields, cfg.reconstruction, cfg.riemann = n_ghost, 5, reconstruction, RIEMANN_HLLC
+        cfg.n_ghost, cfg.reconstruction, cfg.riemann = n_ghost, reconstruction, RIEMANN_HLLC
+        cfg.n_fields, cfg.dual_energy = (6 if dual_energy else 5), int(dual_energy)
         cfg.precision, cfg.density_floor, cfg.temperature_floor = precision, int(density_floor), int(temperature_floor)
         cfg.launch, cfg.tile1, cfg.tile2, cfg.kchunk1, cfg.kchunk2 = launch, tile1, tile2, kchunk1, kchunk2
         d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1, 3))
@@ -423,8 +428,17 @@ def turbulence_modes(seed):
 
 
 def ic_turbulence(U, n, ng, gamma=5.0 / 3.0, rho=1.0, p=1.0, mach=1.0, seed=1):
-    """Turbulence box on the device (U: DeviceArray of 5*(n+2ng)^3)."""
+    """Turbulence box on the device (U: DeviceArray of 5*(n+2ng)^3, or 6*(n+2ng)^3 for a
+    dual-energy state, whose GasEnergy is then set too)."""
+    N = n + 2 * ng
     check(lib().chl_ic_turbulence(U.ptr, n, ng, gamma, rho, p, mach, seed))
+    if U.n == 6 * N ** 3:
+        init_gas_energy(U, N, N, N)
+
+
+def init_gas_energy(U, nx, ny, nz):
+    """GasEnergy (field 5) = E - kinetic energy of every cell (chl_init_gas_energy)."""
+    check(lib().chl_init_gas_energy(U.ptr, nx, ny, nz))
 
 
 # ---------------------------------------------------------------- ideal MHD

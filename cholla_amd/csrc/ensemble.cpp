@@ -24,11 +24,15 @@ using chl_capi::ReconRadius;
 
 constexpr int kDefaultTile1   = 2;  // 64x4
 constexpr int kDefaultTile2   = 1;  // 16x16
+constexpr int kDefaultDeTile  = 1;  // 16x16, both stages, with dual energy
 constexpr int kDefaultKchunk1 = 16;
 constexpr int kDefaultKchunk2 = 32;
 // Sleeps of a waiting stage-2 tile before it gives up (s_sleep 2 = 128 clocks each:
 // about half a second, three orders of magnitude above a stage-1 tile's run time).
 constexpr long long kSpinLimit = 1ll << 23;
+
+// Persistent pairs compiled with dual energy: both tiles among the dual-energy shapes.
+bool DePairIdx(int pair) { return chl::kPairs[pair][0] < chl::kNumDeTiles && chl::kPairs[pair][1] < chl::kNumDeTiles; }
 
 int PairIndex(int tile1, int tile2)
 {
@@ -77,7 +81,9 @@ namespace
 
 int CheckEnsembleConfig(const chl_ensemble_config &c)
 {
-  if (c.n_fields != 5) return Fail(CHL_ERR_UNSUPPORTED, "n_fields must be 5 (hydro without scalars or dual energy)");
+  if (c.dual_energy != 0 && c.dual_energy != 1) return Fail(CHL_ERR_INVALID, "dual_energy must be 0 or 1");
+  if (c.n_fields != 5 + c.dual_energy)
+    return Fail(CHL_ERR_UNSUPPORTED, "n_fields must be 5, or 6 with dual_energy (no passive scalars)");
   if (c.riemann != CHL_RIEMANN_HLLC) return Fail(CHL_ERR_UNSUPPORTED, "only the HLLC solver is built");
   if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC)
     return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PLMC or PPMC");
@@ -89,6 +95,8 @@ int CheckEnsembleConfig(const chl_ensemble_config &c)
     return Fail(CHL_ERR_INVALID, "tile index out of range");
   if (c.launch == CHL_LAUNCH_PERSISTENT && PairIndex(c.tile1, c.tile2) < 0)
     return Fail(CHL_ERR_UNSUPPORTED, "this (tile1, tile2) pair is not compiled for the persistent launch");
+  if (c.dual_energy && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
+    return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for tile shapes 0 and 1 only");
   if (c.kchunk1 < 1 || c.kchunk2 < 1) return Fail(CHL_ERR_INVALID, "kchunk must be positive");
   return CHL_OK;
 }
@@ -119,11 +127,14 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
     d.t2[1]   = TileCount(d.ny - 2 * ng, s2.ty);
     d.t2[2]   = TileCount(o2z, d.kchunk2);
     d.items1  = d.t1[0] * d.t1[1] * d.t1[2];
+    d.items2  = d.t2[0] * d.t2[1] * d.t2[2];
     d.U       = U_override ? (*U_override)[b] : (e->U.empty() ? nullptr : e->U[b]);
     d.dx = e->dxyz[3 * b], d.dy = e->dxyz[3 * b + 1], d.dz = e->dxyz[3 * b + 2];
     d.tout   = e->tout[b];
     w_off[b] = w_total;
-    w_total += (5 * e->Cells(b) * elem + 255) / 256 * 256;
+    // W: 5 (6 with dual energy) half-step fields, then dual energy's pdV sources in FP64
+    w_total += ((5 + c.dual_energy) * e->Cells(b) * elem + 255) / 256 * 256;
+    if (c.dual_energy) w_total += (e->Cells(b) * sizeof(double) + 255) / 256 * 256;
   }
   if (w_total > e->W_bytes) {
     if (e->W) (void)hipFree(e->W);
@@ -132,7 +143,12 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
     CHL_HIP(hipMalloc(reinterpret_cast<void **>(&e->W), w_total));
     e->W_bytes = w_total;
   }
-  for (int b = 0; b < e->n; b++) e->hbox[b].W = e->W + w_off[b];
+  for (int b = 0; b < e->n; b++) {
+    e->hbox[b].W   = e->W + w_off[b];
+    e->hbox[b].pdv = c.dual_energy ? reinterpret_cast<double *>(
+                                         e->W + w_off[b] + ((6 * e->Cells(b) * elem + 255) / 256 * 256))
+                                   : nullptr;
+  }
 
   std::vector<chl::WorkItem> l1, l2, q;
   auto push = [](std::vector<chl::WorkItem> &v, int b, int stage, int count) {
@@ -143,14 +159,16 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
     push(l1, b, 0, d.items1);
     push(l2, b, 1, d.t2[0] * d.t2[1] * d.t2[2]);
   }
-  // persistent queue: stage 1 of box order[i], then stage 2 of box order[i-1]
-  for (size_t i = 0; i <= e->order.size(); i++) {
-    if (i < e->order.size()) push(q, e->order[i], 0, e->hbox[e->order[i]].items1);
-    if (i > 0) {
-      const chl::BoxDesc &d = e->hbox[e->order[i - 1]];
-      push(q, e->order[i - 1], 1, d.t2[0] * d.t2[1] * d.t2[2]);
+  // persistent queue: stage 1 of box order[i], then stage 2 of box order[i-1] (and with
+  // dual energy the selection of box order[i-2] and the sync of box order[i-3]): every
+  // item of a box's phase sits after all items of its previous phase
+  const size_t nb = e->order.size(), lag = c.dual_energy ? 3 : 1;
+  for (size_t i = 0; i < nb + lag; i++)
+    for (size_t st = 0; st <= lag; st++) {
+      if (i < st || i - st >= nb) continue;
+      const int b = e->order[i - st];
+      push(q, b, int(st), st == 0 ? e->hbox[b].items1 : e->hbox[b].items2);
     }
-  }
   e->n1 = int(l1.size()), e->n2 = int(l2.size()), e->nq = int(q.size());
   const size_t need = l1.size() + l2.size() + q.size();
   if (need > e->items_cap) {
@@ -170,7 +188,8 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
   CHL_HIP(hipStreamSynchronize(e->stream));
   if (c.launch == CHL_LAUNCH_PERSISTENT) {
     int pc = 0;
-    CHL_HIP(chl::EnsemblePersistentSlots(c.precision, c.reconstruction, PairIndex(c.tile1, c.tile2), &pc));
+    CHL_HIP(chl::EnsemblePersistentSlots(c.precision, c.reconstruction, PairIndex(c.tile1, c.tile2), c.dual_energy,
+                                         &pc));
     e->per_cu = std::max(pc, 1);
   }
   e->DropGraph();
@@ -187,6 +206,7 @@ chl::EnsembleArgs Args(chl_ensemble *e, int list)
   if (c.density_floor) flags |= chl::kFlagDensityFloor;
   if (c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
   a.flags = flags, a.head = e->head, a.status = e->status(), a.spin_limit = kSpinLimit;
+  a.de = c.dual_energy;
   if (list == 0) a.items = e->items, a.n_items = e->n1;
   if (list == 1) a.items = e->items + e->n1, a.n_items = e->n2;
   if (list == 2) a.items = e->items + e->n1 + e->n2, a.n_items = e->nq;
@@ -208,6 +228,10 @@ int EnqueueStep(chl_ensemble *e)
   if (c.launch == CHL_LAUNCH_SPLIT) {
     CHL_TRY(EnqueueStage(e, 1));
     CHL_TRY(EnqueueStage(e, 2));
+    if (c.dual_energy) {
+      CHL_HIP(chl::LaunchEnsembleDe(0, Args(e, 1), c.tile2, e->stream));
+      CHL_HIP(chl::LaunchEnsembleDe(1, Args(e, 1), c.tile2, e->stream));
+    }
   } else {
     const int grid = std::min(e->per_cu * e->num_cu, std::max(e->nq, 1));
     CHL_HIP(chl::LaunchEnsemblePersistent(c.precision, Args(e, 2), c.reconstruction, PairIndex(c.tile1, c.tile2),
@@ -290,8 +314,8 @@ int chl_ensemble_create(const chl_ensemble_config *cfg, int n_boxes, const int *
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return Fail(CHL_ERR_NO_DEVICE, "no HIP device");
   chl_ensemble_config c = *cfg;
-  if (c.tile1 < 0) c.tile1 = kDefaultTile1;
-  if (c.tile2 < 0) c.tile2 = kDefaultTile2;
+  if (c.tile1 < 0) c.tile1 = c.dual_energy ? kDefaultDeTile : kDefaultTile1;
+  if (c.tile2 < 0) c.tile2 = c.dual_energy ? kDefaultDeTile : kDefaultTile2;
   if (c.kchunk1 <= 0) c.kchunk1 = kDefaultKchunk1;
   if (c.kchunk2 <= 0) c.kchunk2 = kDefaultKchunk2;
   CHL_TRY(CheckEnsembleConfig(c));
@@ -431,16 +455,22 @@ int chl_ensemble_advance_looped(chl_ensemble *e, int n_steps, float *ms_out)
         chl::BoxClock *k      = e->clk + b;
         CHL_HIP(chl::LaunchDtAdvance(&k->dti, &k->dt, &k->t, d.tout, kCfl, &k->nstep, e->stream));
         chl::StepArgs a{};
-        a.U = d.U, a.W = d.W, a.nx = d.nx, a.ny = d.ny, a.nz = d.nz, a.ng = c.n_ghost;
+        a.U = d.U, a.W = d.W, a.pdv = d.pdv, a.nx = d.nx, a.ny = d.ny, a.nz = d.nz, a.ng = c.n_ghost;
         a.dx = d.dx, a.dy = d.dy, a.dz = d.dz, a.gamma = e->gamma, a.dt = 0.0, a.dt_dev = &k->dt, a.dti = &k->dti;
         a.density_floor = e->density_floor, a.U_floor = e->U_floor;
         a.flags = Args(e, 0).flags;
+        a.de     = c.dual_energy;
         a.kchunk = d.kchunk1;
         CHL_HIP(pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, c.tile1, e->stream)
                                            : chl::LaunchStage1Fast(pol, a, R1, c.tile1, e->stream));
         a.kchunk = d.kchunk2;
         CHL_HIP(pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage2Exact(a, c.reconstruction, c.tile2, e->stream)
                                            : chl::LaunchStage2Fast(pol, a, c.reconstruction, c.tile2, e->stream));
+        if (c.dual_energy) {
+          CHL_HIP(chl::LaunchDeSelect(d.U, d.nx, d.ny, d.nz, c.n_ghost, e->stream));
+          CHL_HIP(chl::LaunchDeSync(d.U, d.nx, d.ny, d.nz, c.n_ghost, a.flags, a.U_floor, a.dx, a.dy, a.dz, a.gamma,
+                                    a.dti, e->stream));
+        }
       }
     }
     return CHL_OK;
@@ -472,14 +502,15 @@ int chl_ensemble_autotune(chl_ensemble *e, int steps, chl_ensemble_config *chose
   CHL_HIP(hipMemcpyAsync(saved.data(), e->clk, e->n * sizeof(chl::BoxClock), hipMemcpyDeviceToHost, e->stream));
   size_t total = 0;
   std::vector<size_t> off(e->n);
-  for (int b = 0; b < e->n; b++) off[b] = total, total += 5 * e->Cells(b);
+  const size_t nf = size_t(orig.n_fields);
+  for (int b = 0; b < e->n; b++) off[b] = total, total += nf * e->Cells(b);
   double *S = nullptr;
   CHL_HIP(hipMalloc(&S, total * sizeof(double)));
   std::vector<double *> scratch(e->n);
   int rc = CHL_OK;
   for (int b = 0; b < e->n && rc == CHL_OK; b++) {
     scratch[b] = S + off[b];
-    if (hipMemcpyAsync(scratch[b], e->U[b], 5 * e->Cells(b) * sizeof(double), hipMemcpyDeviceToDevice, e->stream) !=
+    if (hipMemcpyAsync(scratch[b], e->U[b], nf * e->Cells(b) * sizeof(double), hipMemcpyDeviceToDevice, e->stream) !=
         hipSuccess)
       rc = Fail(CHL_ERR_HIP, "scratch copy failed");
   }
@@ -510,7 +541,8 @@ int chl_ensemble_autotune(chl_ensemble *e, int steps, chl_ensemble_config *chose
   chl_ensemble_config sc = orig;
   sc.launch              = CHL_LAUNCH_SPLIT;
   float b1 = std::numeric_limits<float>::infinity(), b2 = b1;
-  for (int t = 0; t < chl::kNumTiles && rc == CHL_OK; t++)
+  const int ntiles = orig.dual_energy ? chl::kNumDeTiles : chl::kNumTiles;
+  for (int t = 0; t < ntiles && rc == CHL_OK; t++)
     for (int kc : chunks) {
       chl_ensemble_config c = sc;
       c.tile1 = t, c.kchunk1 = kc;
@@ -519,7 +551,7 @@ int chl_ensemble_autotune(chl_ensemble *e, int steps, chl_ensemble_config *chose
       if ((rc = timed([&] { return EnqueueStage(e, 1); }, &ms)) != CHL_OK) break;
       if (ms < b1) b1 = ms, sc.tile1 = t, sc.kchunk1 = kc;
     }
-  for (int t = 0; t < chl::kNumTiles && rc == CHL_OK; t++)
+  for (int t = 0; t < ntiles && rc == CHL_OK; t++)
     for (int kc : chunks) {
       chl_ensemble_config c = sc;
       c.tile2 = t, c.kchunk2 = kc;
@@ -536,6 +568,7 @@ int chl_ensemble_autotune(chl_ensemble *e, int steps, chl_ensemble_config *chose
   // persistent launch: whole steps
   for (int p = 0; p < chl::kNumPairs && rc == CHL_OK; p++)
     for (int kc : {8, 16, 32}) {
+      if (orig.dual_energy && !DePairIdx(p)) continue;
       chl_ensemble_config c = orig;
       c.launch = CHL_LAUNCH_PERSISTENT, c.tile1 = chl::kPairs[p][0], c.tile2 = chl::kPairs[p][1];
       c.kchunk1 = kc, c.kchunk2 = kc;
@@ -555,6 +588,14 @@ int chl_ensemble_autotune(chl_ensemble *e, int steps, chl_ensemble_config *chose
   if (rc != CHL_OK) return Fail(rc, err);
   if (rc2 != CHL_OK) return rc2;
   if (chosen) *chosen = e->cfg;
+  return CHL_OK;
+}
+
+int chl_init_gas_energy(double *d_U, int nx, int ny, int nz)
+{
+  if (!d_U || nx < 1 || ny < 1 || nz < 1) return Fail(CHL_ERR_INVALID, "bad argument");
+  CHL_HIP(chl::LaunchGasEnergyInit(d_U, long(nx) * ny * nz, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
   return CHL_OK;
 }
 

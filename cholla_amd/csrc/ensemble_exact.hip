@@ -17,4 +17,10 @@ hipError_t LaunchEnsemblePersistentExact(const EnsembleArgs &e, int recon, int p
 {
   return EnsPersistent<PolicyExact>(e, recon, pair, grid, per_cu, s);
 }
+hipError_t LaunchEnsembleDe(int phase, const EnsembleArgs &e, int tile, hipStream_t s)
+{
+  if (phase == 0) return EnsDe<0>(e, tile, s);
+  if (phase == 1) return EnsDe<1>(e, tile, s);
+  return hipErrorInvalidValue;
+}
 }  // namespace chl

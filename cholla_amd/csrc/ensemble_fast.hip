@@ -30,9 +30,10 @@ hipError_t LaunchEnsemblePersistent(int policy, const EnsembleArgs &e, int recon
   return EnsPersistent<PolicyFast>(e, recon, pair, grid, nullptr, s);
 }
 
-hipError_t EnsemblePersistentSlots(int policy, int recon, int pair, int *per_cu)
+hipError_t EnsemblePersistentSlots(int policy, int recon, int pair, int de, int *per_cu)
 {
   EnsembleArgs none{};
+  none.de = de;
   if (policy == PolicyExact::kId) return LaunchEnsemblePersistentExact(none, recon, pair, 0, per_cu, nullptr);
   if (policy == PolicyMixed::kId) return EnsPersistent<PolicyMixed>(none, recon, pair, 0, per_cu, nullptr);
   return EnsPersistent<PolicyFast>(none, recon, pair, 0, per_cu, nullptr);

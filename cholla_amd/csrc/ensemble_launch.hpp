@@ -9,25 +9,55 @@ namespace chl
 
 inline int PaddedItems(int n) { return (n + 7) / 8 * 8; }
 
-template <class P, int TX, int TY>
+template <class P, int TX, int TY, bool DE = false>
 hipError_t EnsStage1T(const EnsembleArgs &e, hipStream_t s)
 {
   if (e.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL((EnsembleStage1Kernel<P, TX, TY>), dim3(PaddedItems(e.n_items)), dim3(TX * TY), 0, s, e);
+  hipLaunchKernelGGL((EnsembleStage1Kernel<P, TX, TY, DE>), dim3(PaddedItems(e.n_items)), dim3(TX * TY), 0, s, e);
   return hipGetLastError();
 }
 
-template <class P, int RK, int TX, int TY>
+template <class P, int RK, int TX, int TY, bool DE = false>
 hipError_t EnsStage2T(const EnsembleArgs &e, hipStream_t s)
 {
   if (e.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL((EnsembleStage2Kernel<P, RK, TX, TY>), dim3(PaddedItems(e.n_items)), dim3(TX * TY), 0, s, e);
+  hipLaunchKernelGGL((EnsembleStage2Kernel<P, RK, TX, TY, DE>), dim3(PaddedItems(e.n_items)), dim3(TX * TY), 0, s,
+                     e);
   return hipGetLastError();
+}
+
+template <int TX, int TY, int PH>
+hipError_t EnsDeT(const EnsembleArgs &e, hipStream_t s)
+{
+  if (e.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL((EnsembleDeKernel<TX, TY, PH>), dim3(PaddedItems(e.n_items)), dim3(TX * TY), 0, s, e);
+  return hipGetLastError();
+}
+
+template <int PH>
+hipError_t EnsDe(const EnsembleArgs &e, int tile, hipStream_t s)
+{
+  switch (tile) {
+    case 0:
+      return EnsDeT<32, 8, PH>(e, s);
+    case 1:
+      return EnsDeT<16, 16, PH>(e, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <class P>
 hipError_t EnsStage1(const EnsembleArgs &e, int tile, hipStream_t s)
 {
+  if (e.de) {
+    switch (tile) {
+      case 0:
+        return EnsStage1T<P, 32, 8, true>(e, s);
+      case 1:
+        return EnsStage1T<P, 16, 16, true>(e, s);
+    }
+    return hipErrorInvalidValue;
+  }
   switch (tile) {
     case 0:
       return EnsStage1T<P, 32, 8>(e, s);
@@ -46,6 +76,15 @@ hipError_t EnsStage1(const EnsembleArgs &e, int tile, hipStream_t s)
 template <class P, int RK>
 hipError_t EnsStage2R(const EnsembleArgs &e, int tile, hipStream_t s)
 {
+  if (e.de) {
+    switch (tile) {
+      case 0:
+        return EnsStage2T<P, RK, 32, 8, true>(e, s);
+      case 1:
+        return EnsStage2T<P, RK, 16, 16, true>(e, s);
+    }
+    return hipErrorInvalidValue;
+  }
   switch (tile) {
     case 0:
       return EnsStage2T<P, RK, 32, 8>(e, s);
@@ -69,27 +108,42 @@ hipError_t EnsStage2(const EnsembleArgs &e, int recon, int tile, hipStream_t s)
   return hipErrorInvalidValue;
 }
 
-template <class P, int RK, int A, int B>
+template <class P, int RK, int A, int B, bool DE = false>
 struct PersistentPair {
   static constexpr int TX1 = kTiles[A].tx, TY1 = kTiles[A].ty, TX2 = kTiles[B].tx, TY2 = kTiles[B].ty;
   static constexpr int NT  = TX2 * TY2;
   static hipError_t Launch(const EnsembleArgs &e, int grid, hipStream_t s)
   {
     if (e.n_items == 0) return hipSuccess;
-    hipLaunchKernelGGL((EnsemblePersistentKernel<P, RK, TX1, TY1, TX2, TY2>), dim3(grid), dim3(NT), 0, s, e);
+    hipLaunchKernelGGL((EnsemblePersistentKernel<P, RK, TX1, TY1, TX2, TY2, DE>), dim3(grid), dim3(NT), 0, s, e);
     return hipGetLastError();
   }
   static hipError_t Slots(int *per_cu)
   {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, EnsemblePersistentKernel<P, RK, TX1, TY1, TX2, TY2>,
-                                                        NT, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu,
+                                                        EnsemblePersistentKernel<P, RK, TX1, TY1, TX2, TY2, DE>, NT, 0);
   }
 };
+
+// Pairs compiled with dual energy: both tiles among the dual-energy tile shapes.
+constexpr bool DePair(int pair) { return kPairs[pair][0] < kNumDeTiles && kPairs[pair][1] < kNumDeTiles; }
 
 template <class P, int RK>
 hipError_t EnsPersistentR(const EnsembleArgs &e, int pair, int grid, int *per_cu, hipStream_t s)
 {
   static_assert(kNumPairs == 4, "pair table and dispatch below go together");
+  static_assert(!DePair(0) && !DePair(1) && DePair(2) && DePair(3), "dual-energy pairs below");
+  if (e.de) {
+    switch (pair) {
+      case 2:
+        return per_cu ? PersistentPair<P, RK, kPairs[2][0], kPairs[2][1], true>::Slots(per_cu)
+                      : PersistentPair<P, RK, kPairs[2][0], kPairs[2][1], true>::Launch(e, grid, s);
+      case 3:
+        return per_cu ? PersistentPair<P, RK, kPairs[3][0], kPairs[3][1], true>::Slots(per_cu)
+                      : PersistentPair<P, RK, kPairs[3][0], kPairs[3][1], true>::Launch(e, grid, s);
+    }
+    return hipErrorInvalidValue;
+  }
   switch (pair) {
     case 0:
       return per_cu ? PersistentPair<P, RK, kPairs[0][0], kPairs[0][1]>::Slots(per_cu)

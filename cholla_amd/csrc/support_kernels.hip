@@ -7,6 +7,7 @@
 
 #include <cstdint>
 
+#include "de_cells.hpp"
 #include "ensemble_kernels.hpp"
 #include "hydro_physics.hpp"
 #include "support_kernels.hpp"
@@ -266,78 +267,35 @@ __global__ void TemperatureFloorKernel(double *U, int nx, int ny, int nz, int ng
 }
 
 // ============================================================================ dual energy
-// Real cell of a 1D launch over the real cells of an nx x ny x nz box.
-__device__ __forceinline__ bool RealCell(long t, int nx, int ny, int nz, int ng, int &i, int &j, int &k)
-{
-  const long rx = nx - 2 * ng, ry = ny - 2 * ng, rz = nz - 2 * ng;
-  if (t >= rx * ry * rz) return false;
-  i = int(t % rx) + ng;
-  long q = t / rx;
-  j = int(q % ry) + ng;
-  k = int(q / ry) + ng;
-  return true;
-}
-
-// Select_Internal_Energy_3D, src/hydro/hydro_cuda.cu:949-1016: GasEnergy of every
-// real cell becomes U_total = E - kinetic where the thermal fraction exceeds
-// DE_ETA_1 (or DE_ETA_2 of the largest neighbouring energy), else it keeps the
-// advected value; floored at TINY_NUMBER.  Energy is only read here.
+// Select_Internal_Energy_3D over the real cells (DeSelectCell), then Sync_Energies_3D fused
+// with the temperature floor, the inverse-dt reduction and the periodic ghost images
+// (DeSyncCell); two launches because the selection reads the neighbours' energy, which
+// the sync rewrites.
 __global__ void DeSelectKernel(double *U, int nx, int ny, int nz, int ng)
 {
   int i, j, k;
   if (!RealCell(long(blockIdx.x) * blockDim.x + threadIdx.x, nx, ny, nz, ng, i, j, k)) return;
-  const size_t nc = size_t(nx) * ny * nz;
-  const size_t id = Lin(i, j, k, nx, ny);
-  const double *En = U + 4 * nc;
-  const size_t imo = Lin(max(i - 1, ng), j, k, nx, ny), ipo = Lin(min(i + 1, nx - ng - 1), j, k, nx, ny);
-  const size_t jmo = Lin(i, max(j - 1, ng), k, nx, ny), jpo = Lin(i, min(j + 1, ny - ng - 1), k, nx, ny);
-  const size_t kmo = Lin(i, j, max(k - 1, ng), nx, ny), kpo = Lin(i, j, min(k + 1, nz - ng - 1), nx, ny);
-  double d = U[id], d_inv = 1.0 / d;
-  double vx = U[nc + id] * d_inv, vy = U[2 * nc + id] * d_inv, vz = U[3 * nc + id] * d_inv;
-  double E          = En[id];
-  double U_advected = U[kGasEnergy * nc + id];
-  double U_total    = E - 0.5 * d * (vx * vx + vy * vy + vz * vz);
-  double Emax       = Fmax(En[imo], E);
-  Emax              = Fmax(Emax, En[ipo]);
-  Emax              = Fmax(Emax, En[jmo]);
-  Emax              = Fmax(Emax, En[jpo]);
-  Emax              = Fmax(Emax, En[kmo]);
-  Emax              = Fmax(Emax, En[kpo]);
-  double u          = ((U_total / E > kDeEta1) || (U_total / Emax > kDeEta2)) ? U_total : U_advected;
-  U[kGasEnergy * nc + id] = Fmax(u, kTinyNumber);
+  DeSelectCell(U, size_t(nx) * ny * nz, nx, ny, nz, ng, i, j, k);
 }
 
-// Sync_Energies_3D (hydro_cuda.cu:1070-1099): Energy = kinetic + selected internal
-// energy; then, as Update_Hydro_Grid continues (grid3D.cpp:497-577), the temperature
-// floor of E and GasEnergy, the next inverse dt (Calc_dt_3D) and, for an
-// all-periodic box, the ghost images of the cell.
 __global__ void DeSyncKernel(double *U, int nx, int ny, int nz, int ng, int flags, double U_floor, double dx,
                              double dy, double dz, double gamma, unsigned long long *dti)
 {
   int i, j, k;
   double m = 0.0;
-  if (RealCell(long(blockIdx.x) * blockDim.x + threadIdx.x, nx, ny, nz, ng, i, j, k)) {
-    const size_t nc = size_t(nx) * ny * nz;
-    const size_t id = Lin(i, j, k, nx, ny);
-    double un[6];
-#pragma unroll
-    for (int q = 0; q < 6; q++) un[q] = U[q * nc + id];
-    double d_inv = 1.0 / un[0];
-    double vx = un[1] * d_inv, vy = un[2] * d_inv, vz = un[3] * d_inv;
-    un[4] = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz) + un[kGasEnergy];
-    if (flags & kFlagTemperatureFloor) {  // Temperature_Floor_Kernel, hydro_cuda.cu:1132-1151
-      double Ekin = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz);
-      double u    = (un[4] - Ekin) / un[0];
-      if (u < U_floor) un[4] = Ekin + un[0] * U_floor;
-      u = un[kGasEnergy] / un[0];
-      if (u < U_floor) un[kGasEnergy] = un[0] * U_floor;
-    }
-    U[4 * nc + id]          = un[4];
-    U[kGasEnergy * nc + id] = un[kGasEnergy];
-    if (flags & kFlagReduceDt) m = InverseCrossingTime(un[4], un[0], d_inv, vx, vy, vz, dx, dy, dz, gamma);
-    if (flags & kFlagPeriodicGhosts) WritePeriodicImages<6>(U, nc, nx, ny, nz, ng, i, j, k, un);
-  }
+  if (RealCell(long(blockIdx.x) * blockDim.x + threadIdx.x, nx, ny, nz, ng, i, j, k))
+    m = DeSyncCell(U, size_t(nx) * ny * nz, nx, ny, nz, ng, i, j, k, flags, U_floor, dx, dy, dz, gamma);
   if (flags & kFlagReduceDt) BlockMaxToGlobal(m, dti);
+}
+
+// GasEnergy = E - kinetic energy of every cell (the DE state of a 5-field state).
+__global__ void GasEnergyInitKernel(double *U, long nc)
+{
+  long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= nc) return;
+  double d = U[id];
+  double vx = U[nc + id] / d, vy = U[2 * nc + id] / d, vz = U[3 * nc + id] / d;
+  U[kGasEnergy * nc + id] = U[4 * nc + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz);
 }
 
 // Device-side time control for graph-captured stepping: dt = C_cfl / dti clamped
@@ -374,6 +332,8 @@ __global__ void EnsembleClockKernel(BoxClock *clk, const BoxDesc *box, int n_box
   if (dt > 0.0) c.nstep += 1;
   c.dti   = 0ull;
   c.done1 = 0;
+  c.done2 = 0;
+  c.done3 = 0;
   clk[b]  = c;
 }
 
@@ -632,6 +592,13 @@ hipError_t LaunchTemperatureFloor(double *U, int nx, int ny, int nz, int ng, dou
   long nc = long(nx) * ny * nz;
   hipLaunchKernelGGL(TemperatureFloorKernel, dim3(Blocks(nc, 256)), dim3(256), 0, s, U, nx, ny, nz, ng, U_floor,
                      int(n_fields == 6));
+  return hipGetLastError();
+}
+
+hipError_t LaunchGasEnergyInit(double *U, long n_cells, hipStream_t s)
+{
+  if (n_cells <= 0) return hipSuccess;
+  hipLaunchKernelGGL(GasEnergyInitKernel, dim3(Blocks(n_cells, 256)), dim3(256), 0, s, U, n_cells);
   return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@ hipError_t LaunchTemperatureFloor(double *U, int nx, int ny, int nz, int ng, dou
 // temperature floor, the inverse-dt reduction and periodic ghost images (flags: the
 // kFlag* bits of vl3d_kernels.hpp).
 hipError_t LaunchDeSelect(double *U, int nx, int ny, int nz, int ng, hipStream_t s);
+hipError_t LaunchGasEnergyInit(double *U, long n_cells, hipStream_t s);
 hipError_t LaunchDeSync(double *U, int nx, int ny, int nz, int ng, int flags, double U_floor, double dx, double dy,
                         double dz, double gamma, unsigned long long *dti, hipStream_t s);
 hipError_t LaunchDtAdvance(const unsigned long long *dti, double *dt_dev, double *t_dev, double tout, double cfl,

@@ -155,7 +155,7 @@ int chl_hllc_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, doub
 
 typedef struct chl_ensemble_config {
   int n_ghost;           /* as chl_hydro_config; every box shares it */
-  int n_fields;          /* 5 */
+  int n_fields;          /* 5, or 6 with dual_energy */
   int reconstruction;    /* CHL_RECON_PLMC or CHL_RECON_PPMC */
   int riemann;           /* CHL_RIEMANN_HLLC */
   int precision;         /* CHL_PREC_* */
@@ -165,6 +165,7 @@ typedef struct chl_ensemble_config {
   int tile1, tile2;      /* stage tile shapes, -1 = default (the persistent launch
                             takes the pairs listed by chl_ensemble_persistent_pairs) */
   int kchunk1, kchunk2;  /* z cells per work item, 0 = default */
+  int dual_energy;       /* DE build option, 0/1: GasEnergy as field 5 of every box (n_fields 6) */
 } chl_ensemble_config;
 
 typedef struct chl_ensemble chl_ensemble; /* owns the scratch and work lists of the boxes */
@@ -268,6 +269,11 @@ int chl_turbulence_modes(uint64_t seed, double *modes, int max_modes, int *n_mod
  * the cell centres with s set so the rms |v| over the box is mach * sqrt(gamma p / rho);
  * ghost cells hold their periodic images.  d_U: 5*(n+2ng)^3 doubles on the device. */
 int chl_ic_turbulence(double *d_U, int n, int ng, double gamma, double rho, double p, double mach, uint64_t seed);
+/* GasEnergy of every cell (ghosts included) set to the internal energy E - kinetic of the
+ * state's first five fields: the field the DE build adds, which the reference's initial
+ * conditions fill with P/(gamma-1) (e.g. src/grid/initial_conditions.cpp:228-229).
+ * d_U: 6 fields. */
+int chl_init_gas_energy(double *d_U, int nx, int ny, int nz);
 /* mhd::utils::Init_Magnetic_Field_With_Vector_Potential (src/utils/mhd_utilities.cu:22-44):
  * face fields = differences of the edge potential A (3 x n_cells, x/y/z components) for
  * every cell with i, j, k >= 1.  host_U: 8 fields. */

@@ -244,3 +244,96 @@ def test_turbulence_ensemble_step():
         m = interior(A, 5, *dims[b], ng)[0].sum()
         assert abs(m - mass0[b]) <= 1e-12 * mass0[b], (b, m, mass0[b])
     ens.close()
+
+
+# ----------------------------------------------------------------------------- dual energy
+def de_boxes(ng, seed0=500):
+    """Uneven 6-field boxes: uniform density and pressure with a hypersonic slab
+    (|y - 0.5| < 0.25 at vx = 100, thermal fraction ~3e-4 < DE_ETA_1) in a slow medium."""
+    from test_oracle_de import periodic_state
+
+    states, dims, dxyz = [], [], []
+    for b, (nxr, nyr, nzr) in enumerate(REAL):
+        rng = np.random.default_rng(seed0 + b)
+        z, y, x = np.meshgrid((np.arange(nzr) + 0.5) / nzr, (np.arange(nyr) + 0.5) / nyr, (np.arange(nxr) + 0.5) / nxr,
+                              indexing="ij")
+        sh = (nzr, nyr, nxr)
+        slab = np.abs(y - 0.5) < 0.25
+        d = 1 + 0.05 * rng.uniform(-1, 1, sh)
+        vx = np.where(slab, 100.0, -0.3) + 0.02 * rng.uniform(-1, 1, sh)
+        vy = 0.05 * np.sin(4 * np.pi * x) + 0.02 * rng.uniform(-1, 1, sh)
+        vz = 0.02 * rng.uniform(-1, 1, sh)
+        p = 1 + 0.05 * rng.uniform(-1, 1, sh)
+        U, dm = periodic_state(nxr, nyr, nzr, ng, d, vx, vy, vz, p)
+        states.append(U)
+        dims.append(dm)
+        dxyz.append((1.0 / nxr, 1.0 / nyr, 1.0 / nzr))
+    return states, dims, dxyz
+
+
+@pytest.mark.parametrize("launch", [chl.LAUNCH_SPLIT, chl.LAUNCH_PERSISTENT])
+def test_de_exact_ensemble_matches_oracle_per_box(launch):
+    ng, steps = 4, 6
+    states, dims, dxyz = de_boxes(ng)
+    say("DE exact ensemble vs oracle, launch", launch)
+    out, t, ns = run_ensemble(states, dims, dxyz, ng, steps, chl.PREC_FP64_EXACT, launch, dual_energy=True)
+    for b, U in enumerate(states):
+        Uo = U.copy()
+        nx, ny, nz = dims[b]
+        n_o, t_o = orc.run(Uo, nx, ny, nz, ng, 6, *dxyz[b], 5 / 3, orc.PPMC, [1] * 6, 0.0, 1e30, steps,
+                           dual_energy_on=True)
+        assert ns[b] == n_o == steps
+        assert t[b] == t_o, (b, t[b], t_o)
+        np.testing.assert_array_equal(out[b], Uo)
+
+
+@pytest.mark.parametrize("precision", [chl.PREC_FP64, chl.PREC_MIXED])
+def test_de_launch_forms_identical(precision):
+    """With dual energy: split, persistent (the dual-energy tile pairs) and the per-box loop
+    agree bit for bit."""
+    ng, steps = 4, 5
+    states, dims, dxyz = de_boxes(ng, seed0=700)
+    say("DE launch forms, precision", precision)
+    ref, t_ref, n_ref = run_ensemble(states, dims, dxyz, ng, steps, precision, chl.LAUNCH_SPLIT, dual_energy=True)
+    runs = [("looped", dict(launch=chl.LAUNCH_SPLIT, looped=True)),
+            ("split-eager", dict(launch=chl.LAUNCH_SPLIT, use_graph=False)),
+            ("split 0,0", dict(launch=chl.LAUNCH_SPLIT, tile1=0, tile2=0))]
+    for t1, t2 in ((0, 0), (1, 1)):
+        for kc in (8, 32):
+            runs.append((f"persistent {t1},{t2} kc{kc}",
+                         dict(launch=chl.LAUNCH_PERSISTENT, tile1=t1, tile2=t2, kchunk1=kc, kchunk2=kc)))
+    for name, kw in runs:
+        say("  form", name)
+        launch = kw.pop("launch")
+        out, t, ns = run_ensemble(states, dims, dxyz, ng, steps, precision, launch, dual_energy=True, **kw)
+        np.testing.assert_array_equal(ns, n_ref)
+        np.testing.assert_array_equal(t, t_ref)
+        for b in range(len(states)):
+            np.testing.assert_array_equal(out[b], ref[b])
+
+
+def test_de_turbulence_ensemble_stays_finite():
+    """The benchmark's hardest boxes (32^3 at rms Mach 2.5-2.8) with dual energy, mixed
+    form, persistent launch: 60 steps finite, mass conserved to round-off."""
+    edges, mach = chl.ensemble_layout(2024, 384)
+    pick = [372, 76, 36, 200]
+    ng = 4
+    dims = [(int(edges[b]) + 2 * ng,) * 3 for b in pick]
+    dev = []
+    for q, b in enumerate(pick):
+        U = chl.DeviceArray(6 * dims[q][0] ** 3)
+        chl.ic_turbulence(U, int(edges[b]), ng, 5 / 3, 1.0, 1.0, float(mach[b]), 2024 * 1000 + b)
+        dev.append(U)
+    mass0 = [interior(d.download(), 6, *dims[q], ng)[0].sum() for q, d in enumerate(dev)]
+    ens = chl.Ensemble(dims, ng, chl.RECON_PPMC, chl.PREC_MIXED, chl.LAUNCH_PERSISTENT, dual_energy=True)
+    ens.bind(dev, np.array([[1.0 / edges[b]] * 3 for b in pick]).reshape(-1))
+    ens.start(5 / 3)
+    ens.advance(60)
+    t, dt, ns = ens.clocks()
+    assert (ns == 60).all()
+    for q, d in enumerate(dev):
+        A = d.download()
+        assert np.isfinite(A).all(), pick[q]
+        m = interior(A, 6, *dims[q], ng)[0].sum()
+        assert abs(m - mass0[q]) <= 1e-12 * mass0[q], (pick[q], m, mass0[q])
+    ens.close()
