@@ -570,10 +570,21 @@ __device__ __forceinline__ Cons<typename P::T> Hllc(const RState<typename P::T> 
   T H       = Div<P>(sqrtdl * Hl + sqrtdr * Hr, den, inv_den);
   T vsq     = (vx * vx + vy * vy + vz * vz);
   T asq     = g1 * (H - T(0.5) * vsq);
+  // H - vsq/2 >= 0 holds for Roe averages in exact arithmetic; where both sides carry a
+  // (floored) near-zero pressure it is a difference of two nearly equal kinetic terms,
+  // and its rounding can go negative.  The non-exact forms take the root of max(asq, 0).
+  if constexpr (!P::kExact) asq = Fmax(asq, T(0));
   T a       = Sqrt<P>(asq);
   T lambda_m = vx - a, lambda_p = vx + a;
   T cfl = Sqrt<P>(Div<P>(gamma * l.p, l.d, l.inv_d));
   T cfr = Sqrt<P>(Div<P>(gamma * r.p, r.d, r.inv_d));
+  if constexpr (P::kId == PolicyMixed::kId) {
+    // FP32: a sound speed below the resolution of the velocity (a floored interface
+    // pressure) would put Sl on vxl or Sr on vxr, leave the star-state denominators at
+    // zero and turn the flux into 0/0; keep the signal speeds at 2^-20 |v| at least.
+    cfl = Fmax(cfl, T(0x1p-20) * Fabs(l.vx));
+    cfr = Fmax(cfr, T(0x1p-20) * Fabs(r.vx));
+  }
   T Sl  = Fmin(lambda_m, l.vx - cfl);
   T Sr  = Fmax(lambda_p, r.vx + cfr);
   // H-correction hook with etah = 0 (hllc_cuda.cu:52,174-175)
