@@ -51,7 +51,7 @@ class HydroConfig(ctypes.Structure):
         ("precision", ctypes.c_int), ("density_floor", ctypes.c_int), ("temperature_floor", ctypes.c_int),
         ("bc_flags", ctypes.c_int * 6), ("tile1", ctypes.c_int), ("tile2", ctypes.c_int),
         ("kchunk1", ctypes.c_int), ("kchunk2", ctypes.c_int), ("tile_order", ctypes.c_int),
-        ("dual_energy", ctypes.c_int),
+        ("dual_energy", ctypes.c_int), ("positivity_fallback", ctypes.c_int),
     ]
 
 
@@ -61,7 +61,7 @@ class EnsembleConfig(ctypes.Structure):
         ("riemann", ctypes.c_int), ("precision", ctypes.c_int), ("density_floor", ctypes.c_int),
         ("temperature_floor", ctypes.c_int), ("launch", ctypes.c_int), ("tile1", ctypes.c_int),
         ("tile2", ctypes.c_int), ("kchunk1", ctypes.c_int), ("kchunk2", ctypes.c_int),
-        ("dual_energy", ctypes.c_int),
+        ("dual_energy", ctypes.c_int), ("positivity_fallback", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -218,11 +218,12 @@ class Hydro:
 
     def __init__(self, nx, ny, nz, n_ghost, reconstruction=RECON_PPMC, precision=PREC_FP64_EXACT,
                  bc=(1, 1, 1, 1, 1, 1), density_floor=True, temperature_floor=True, tile1=-1, tile2=-1,
-                 kchunk1=0, kchunk2=0, tile_order=0, dual_energy=False):
+                 kchunk1=0, kchunk2=0, tile_order=0, dual_energy=False, positivity_fallback=False):
         cfg = HydroConfig()
         cfg.nx, cfg.ny, cfg.nz, cfg.n_ghost = nx, ny, nz, n_ghost
         cfg.n_fields = 6 if dual_energy else 5
         cfg.dual_energy = int(dual_energy)
+        cfg.positivity_fallback = int(positivity_fallback)
         cfg.reconstruction, cfg.riemann, cfg.precision = reconstruction, RIEMANN_HLLC, precision
         cfg.density_floor, cfg.temperature_floor = int(density_floor), int(temperature_floor)
         for q in range(6):
@@ -334,10 +335,11 @@ class Ensemble:
 
     def __init__(self, dims, n_ghost, reconstruction=RECON_PPMC, precision=PREC_MIXED, launch=LAUNCH_SPLIT,
                  density_floor=True, temperature_floor=True, tile1=-1, tile2=-1, kchunk1=0, kchunk2=0,
-                 dual_energy=False):
+                 dual_energy=False, positivity_fallback=False):
         cfg = EnsembleConfig()
         cfg.n_ghost, cfg.reconstruction, cfg.riemann = n_ghost, reconstruction, RIEMANN_HLLC
         cfg.n_fields, cfg.dual_energy = (6 if dual_energy else 5), int(dual_energy)
+        cfg.positivity_fallback = int(positivity_fallback)
         cfg.precision, cfg.density_floor, cfg.temperature_floor = precision, int(density_floor), int(temperature_floor)
         cfg.launch, cfg.tile1, cfg.tile2, cfg.kchunk1, cfg.kchunk2 = launch, tile1, tile2, kchunk1, kchunk2
         d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1, 3))

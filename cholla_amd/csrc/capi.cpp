@@ -77,6 +77,10 @@ int CheckConfig(const chl_hydro_config &c)
   if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC)
     return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PLMC or PPMC");
   if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
+  if (c.positivity_fallback != 0 && c.positivity_fallback != 1)
+    return Fail(CHL_ERR_INVALID, "positivity_fallback must be 0 or 1");
+  if (c.positivity_fallback && (c.reconstruction != CHL_RECON_PPMC || (c.ny == 1 && c.nz == 1)))
+    return Fail(CHL_ERR_UNSUPPORTED, "the positivity fallback is built for PPMC on 3D boxes only");
   const int ng = c.n_ghost, r = ReconRadius(c.reconstruction);
   if (ng < r + 2) return Fail(CHL_ERR_INVALID, "n_ghost too small for the reconstruction (PLMC needs 3, PPMC 4)");
   if (c.nx < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "nx must exceed 2*n_ghost");
@@ -172,6 +176,7 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
   if (c.density_floor) flags |= chl::kFlagDensityFloor;
   if (update_grid && c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
   if (update_grid) flags |= chl::kFlagReduceDt;
+  if (c.positivity_fallback) flags |= chl::kFlagPositivityFallback;
   const bool fused_ghosts = fill && h->PeriodicFusable();
   if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
   chl::StepArgs a = MakeStepArgs(h, U, dx, dy, dz, gamma, dt_host, dt_dev, density_floor, U_floor, flags);
@@ -457,6 +462,7 @@ int chl_hydro_kernel_times(chl_hydro *h, double *U, double dx, double dy, double
   int flags = chl::kFlagReduceDt;
   if (c.density_floor) flags |= chl::kFlagDensityFloor;
   if (c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
+  if (c.positivity_fallback) flags |= chl::kFlagPositivityFallback;
   const bool fused_ghosts = h->PeriodicFusable();
   if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
   chl::StepArgs a = MakeStepArgs(h, U, dx, dy, dz, gamma, 0.0, h->dt(), 0.0, 0.0, flags);

@@ -88,6 +88,10 @@ int CheckEnsembleConfig(const chl_ensemble_config &c)
   if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC)
     return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PLMC or PPMC");
   if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
+  if (c.positivity_fallback != 0 && c.positivity_fallback != 1)
+    return Fail(CHL_ERR_INVALID, "positivity_fallback must be 0 or 1");
+  if (c.positivity_fallback && c.reconstruction != CHL_RECON_PPMC)
+    return Fail(CHL_ERR_UNSUPPORTED, "the positivity fallback is built for PPMC only");
   if (c.n_ghost < ReconRadius(c.reconstruction) + 2)
     return Fail(CHL_ERR_INVALID, "n_ghost too small for the reconstruction (PLMC needs 3, PPMC 4)");
   if (c.launch != CHL_LAUNCH_SPLIT && c.launch != CHL_LAUNCH_PERSISTENT) return Fail(CHL_ERR_INVALID, "bad launch");
@@ -205,6 +209,7 @@ chl::EnsembleArgs Args(chl_ensemble *e, int list)
   int flags = chl::kFlagReduceDt | chl::kFlagPeriodicGhosts;
   if (c.density_floor) flags |= chl::kFlagDensityFloor;
   if (c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
+  if (c.positivity_fallback) flags |= chl::kFlagPositivityFallback;
   a.flags = flags, a.head = e->head, a.status = e->status(), a.spin_limit = kSpinLimit;
   a.de = c.dual_energy;
   if (list == 0) a.items = e->items, a.n_items = e->n1;

@@ -344,9 +344,17 @@ __device__ __forceinline__ void PpmSingleReduced(T im2, T im1, T i0, T ip1, T ip
 // PPMC_VL body, src/reconstruction/ppmc_cuda.cu:603-684: project the 5-point stencil
 // on cell i's characteristics, limit each characteristic field with PPM, project
 // back and floor density/pressure.  s[0..4] = cells i-2..i+2 (rotated primitives).
+//
+// fallback: the positivity fallback, an option with no counterpart in the reference.
+// When an interface density or pressure of the cell comes out non-positive, both of
+// the cell's interfaces take its cell-centred values before the floors; the
+// reference floors each quantity at TINY_NUMBER on its own, which can pair a 1e-20
+// density with a finite pressure (an unbounded sound speed, and an HLLC flux that
+// injects energy on the scale of p/1e-20).  Returns whether the fallback fired.
 template <class P>
-__device__ __forceinline__ void ReconPpmc(const Prim<typename P::T> s[5], typename P::T gamma,
-                                          Prim<typename P::T> &L_iph, Prim<typename P::T> &R_imh)
+__device__ __forceinline__ bool ReconPpmc(const Prim<typename P::T> s[5], typename P::T gamma,
+                                          Prim<typename P::T> &L_iph, Prim<typename P::T> &R_imh,
+                                          bool fallback = false)
 {
   using T = typename P::T;
   const Prim<T> &q = s[2];
@@ -370,10 +378,16 @@ __device__ __forceinline__ void ReconPpmc(const Prim<typename P::T> s[5], typena
   }
   L_iph   = CharToPrim<P>(q, L, cs);
   R_imh   = CharToPrim<P>(q, R, cs);
+  bool fb = false;
+  if (fallback) {
+    fb = !(L_iph.d > T(0) && R_imh.d > T(0) && L_iph.p > T(0) && R_imh.p > T(0));
+    if (fb) L_iph = R_imh = q;
+  }
   R_imh.d = Fmax(R_imh.d, T(kTinyNumber));
   L_iph.d = Fmax(L_iph.d, T(kTinyNumber));
   R_imh.p = Fmax(R_imh.p, T(kTinyNumber));
   L_iph.p = Fmax(L_iph.p, T(kTinyNumber));
+  return fb;
 }
 
 // ---------------------------------------------------------------------------- PLMC (VL)

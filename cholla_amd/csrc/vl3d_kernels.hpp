@@ -66,10 +66,11 @@ struct StepArgs {
 };
 
 enum : int {
-  kFlagDensityFloor     = 1,
-  kFlagTemperatureFloor = 2,
-  kFlagPeriodicGhosts   = 4,  // write periodic ghost images from the stage-2 epilogue
-  kFlagReduceDt         = 8,
+  kFlagDensityFloor       = 1,
+  kFlagTemperatureFloor   = 2,
+  kFlagPeriodicGhosts     = 4,  // write periodic ghost images from the stage-2 epilogue
+  kFlagReduceDt           = 8,
+  kFlagPositivityFallback = 16,  // PPMC positivity fallback (see ReconPpmc)
 };
 
 __device__ __forceinline__ size_t Lin(int i, int j, int k, int nx, int ny)
@@ -449,7 +450,7 @@ struct WCell<T, true> {
 };
 
 template <class P, int RK, int DIR, bool DE = false>
-__device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, typename P::T gamma,
+__device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, typename P::T gamma, bool pos_fallback,
                                          Prim<typename P::T> &L, Prim<typename P::T> &R, typename P::T *geL = nullptr,
                                          typename P::T *geR = nullptr)
 {
@@ -475,8 +476,11 @@ __device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, type
     }
   }
   if constexpr (RK == kPPMC) {
-    ReconPpmc<P>(st, gamma, L, R);
-    if constexpr (DE) ReconPpmcScalar<P>(ge, *geL, *geR);
+    bool fb = ReconPpmc<P>(st, gamma, L, R, pos_fallback);
+    if constexpr (DE) {
+      ReconPpmcScalar<P>(ge, *geL, *geR);
+      if (fb) *geL = *geR = ge[2];
+    }
   } else {
     ReconPlmc<P>(st, gamma, L, R);
     if constexpr (DE) ReconPlmcScalar<P>(ge, *geL, *geR);
@@ -601,6 +605,7 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
   double *U          = a.U;
   const double inv_dx = 1.0 / a.dx, inv_dy = 1.0 / a.dy, inv_dz = 1.0 / a.dz;
   if (dt == 0.0) return;  // a device-timed step past the output time (uniform per box)
+  const bool pf = (a.flags & kFlagPositivityFallback) != 0;
 
   auto loadW = [&](int ii, int jj, int kk) {
     size_t id = Lin(ii, jj, kk, nx, ny);
@@ -617,13 +622,13 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
   for (int m = 0; m < NS; m++) win[m] = loadW(ic, jc, k0 - 1 - r + m);
   Prim<T> qLz, qRz;
   T gLz = T(0), gRz = T(0);
-  ReconDir<P, RK, 2, DE>(win, gamma, qLz, qRz, &gLz, &gRz);  // cell k0-1
+  ReconDir<P, RK, 2, DE>(win, gamma, pf, qLz, qRz, &gLz, &gRz);  // cell k0-1
   Prim<T> qL_lo = qLz;                                       // left state of face k0-1/2
   T gL_lo       = gLz;
 #pragma unroll
   for (int m = 0; m < NS - 1; m++) win[m] = win[m + 1];
   win[NS - 1] = loadW(ic, jc, k0 + r);
-  ReconDir<P, RK, 2, DE>(win, gamma, qLz, qRz, &gLz, &gRz);  // cell k0
+  ReconDir<P, RK, 2, DE>(win, gamma, pf, qLz, qRz, &gLz, &gRz);  // cell k0
   T fz_lo[NC];                                               // flux through face k-1/2 of the current cell
   FaceFlux<P, 2, DE>(RStateFromPrim<P, DE>(qL_lo, gamma, inv_gm1, gL_lo), RStateFromPrim<P, DE>(qRz, gamma, inv_gm1, gRz),
                      gamma, fz_lo);
@@ -655,7 +660,7 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
 #pragma unroll
     for (int m = 0; m < NS - 1; m++) win[m] = win[m + 1];
     win[NS - 1] = loadW(ic, jc, k + 1 + r);
-    ReconDir<P, RK, 2, DE>(win, gamma, qLz, qRz, &gLz, &gRz);
+    ReconDir<P, RK, 2, DE>(win, gamma, pf, qLz, qRz, &gLz, &gRz);
     T fz_hi[NC];
     FaceFlux<P, 2, DE>(RStateFromPrim<P, DE>(qL_lo, gamma, inv_gm1, gL_lo),
                        RStateFromPrim<P, DE>(qRz, gamma, inv_gm1, gRz), gamma, fz_hi);
@@ -669,9 +674,9 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
 #pragma unroll
       for (int m = 0; m < NS; m++) s[m] = plane[rj + H][ri - 1 + H - r + m];
       if constexpr (DE) {
-        ReconDir<P, RK, 0, DE>(s, gamma, qL[it], qR[it], &qgL[it], &qgR[it]);
+        ReconDir<P, RK, 0, DE>(s, gamma, pf, qL[it], qR[it], &qgL[it], &qgR[it]);
       } else {
-        ReconDir<P, RK, 0>(s, gamma, qL[it], qR[it]);
+        ReconDir<P, RK, 0>(s, gamma, pf, qL[it], qR[it]);
       }
     }
     __syncthreads();
@@ -703,9 +708,9 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
 #pragma unroll
       for (int m = 0; m < NS; m++) s[m] = plane[rj - 1 + H - r + m][ri + H];
       if constexpr (DE) {
-        ReconDir<P, RK, 1, DE>(s, gamma, qL[it], qR[it], &qgL[it], &qgR[it]);
+        ReconDir<P, RK, 1, DE>(s, gamma, pf, qL[it], qR[it], &qgL[it], &qgR[it]);
       } else {
-        ReconDir<P, RK, 1>(s, gamma, qL[it], qR[it]);
+        ReconDir<P, RK, 1>(s, gamma, pf, qL[it], qR[it]);
       }
     }
     __syncthreads();

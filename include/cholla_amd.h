@@ -60,6 +60,10 @@ typedef struct chl_hydro_config {
   int kchunk1, kchunk2;  /* z cells per workgroup in the two stages, 0 = default */
   int tile_order;        /* 0: each XCD takes a contiguous run of tiles (default); 1: dispatch order */
   int dual_energy;       /* DE build option, 0/1: GasEnergy as field 5 (n_fields 6), 3D boxes only */
+  int positivity_fallback; /* 0/1, PPMC in 3D only; no counterpart in the reference: where a
+                              cell's PPMC interface density or pressure comes out non-positive,
+                              both of its interfaces in that sweep take the cell-centred values
+                              (the reference floors each at TINY_NUMBER, ppmc_cuda.cu:681-684) */
 } chl_hydro_config;
 
 typedef struct chl_hydro chl_hydro; /* owns the integrator scratch of one box */
@@ -166,6 +170,7 @@ typedef struct chl_ensemble_config {
                             takes the pairs listed by chl_ensemble_persistent_pairs) */
   int kchunk1, kchunk2;  /* z cells per work item, 0 = default */
   int dual_energy;       /* DE build option, 0/1: GasEnergy as field 5 of every box (n_fields 6) */
+  int positivity_fallback; /* 0/1, PPMC only: as chl_hydro_config.positivity_fallback */
 } chl_ensemble_config;
 
 typedef struct chl_ensemble chl_ensemble; /* owns the scratch and work lists of the boxes */

@@ -45,6 +45,13 @@ constexpr int kGE      = 5;        // grid_enum::GasEnergy with DE, no scalars, 
 // branches when set).
 int g_de = 0;
 
+// PPMC positivity fallback, an option of this implementation with no counterpart in
+// the reference: where PPMC_VL's interface density or pressure of a cell comes out
+// non-positive (the reference floors each at TINY_NUMBER independently, ppmc_cuda.cu:681-684,
+// which can pair a 1e-20 density with a finite pressure and so an unbounded sound
+// speed), both interfaces of that cell in that sweep take the cell-centred values.
+int g_pos_fallback = 0;
+
 inline size_t Idx(int i, int j, int k, int nx, int ny)
 {
   return size_t(i) + size_t(j) * size_t(nx) + size_t(k) * size_t(nx) * size_t(ny);
@@ -483,6 +490,7 @@ void orc_ppmc_vl(const double *U, double *QL, double *QR, int nx, int ny, int nz
         Prim pL = CharToPrim(c0, L, c, c2);
         Prim pR = CharToPrim(c0, R, c, c2);
         if (g_de) PpmSingle(cm2.ge, cm1.ge, c0.ge, cp1.ge, cp2.ge, pL.ge, pR.ge);
+        if (g_pos_fallback && !(pL.d > 0. && pR.d > 0. && pL.p > 0. && pR.p > 0.)) pL = pR = c0;
         pR.d    = std::fmax(pR.d, kTiny);
         pL.d    = std::fmax(pL.d, kTiny);
         pR.p    = std::fmax(pR.p, kTiny);
@@ -1110,6 +1118,7 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
 
 // The DE build option (dual energy) as a run-time switch for every stage above.
 void orc_set_dual_energy(int on) { g_de = on ? 1 : 0; }
+void orc_set_positivity_fallback(int on) { g_pos_fallback = on ? 1 : 0; }
 int orc_get_dual_energy() { return g_de; }
 
 // hydro_utilities::Get_Pressure_From_DE, for the known-answer tests
