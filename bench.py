@@ -30,15 +30,18 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage2_kh512_hbm_traffic.json")
 
 
 def measured_traffic(form, tile, n):
-    """The profiled HBM bytes per launch of stage 2 for this form, tile choice and box, if
-    the committed profile covers exactly that configuration (else None)."""
+    """The profiled HBM bytes per launch of stage 2 for this form, stage-2 tile and z-chunk
+    (tile = [tile1, tile2, kchunk1, kchunk2]; stage 2's traffic depends on its own launch
+    shape only) and box, if the committed profile covers that configuration (else None)."""
     try:
         with open(TRAFFIC_FILE) as f:
             prof = json.load(f)
     except (OSError, ValueError):
         return None
     for rec in prof.get("records", []):
-        if rec.get("form") == form and rec.get("n") == n and list(rec.get("tile", [])) == list(tile or []):
+        rt = list(rec.get("tile", []))
+        if rec.get("form") == form and rec.get("n") == n and tile and len(rt) == 4 and \
+                (rt[1], rt[3]) == (tile[1], tile[3]):
             return rec
     return None
 

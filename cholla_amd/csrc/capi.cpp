@@ -514,7 +514,8 @@ int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, doub
   }
   chl::StepArgs a = MakeStepArgs(h, S, dx, dy, dz, gamma, dt_fixed, nullptr, 0.0, 0.0,
                                  chl::kFlagReduceDt | (c.density_floor ? chl::kFlagDensityFloor : 0) |
-                                     (c.temperature_floor ? chl::kFlagTemperatureFloor : 0));
+                                     (c.temperature_floor ? chl::kFlagTemperatureFloor : 0) |
+                                     (c.positivity_fallback ? chl::kFlagPositivityFallback : 0));
   const int R1     = ReconRadius(c.reconstruction) + 1;
   const int nz1    = c.nz - 2 * c.n_ghost + 2 * R1, nz2 = c.nz - 2 * c.n_ghost;
   const int ntiles = c.dual_energy ? chl::kNumDeTiles : chl::kNumTiles;
