@@ -35,7 +35,8 @@ EXPORTS = [
     "chl_mhd_destroy",
     "chl_mhd_vl_step", "chl_mhd_run", "chl_mhd_time_steps", "chl_mhd_calc_dt", "chl_mhd_divergence",
     "chl_hlld_fluxes", "chl_mhd_reconstruct_plmc", "chl_ct_electric_fields", "chl_update_magnetic_field",
-    "chl_magnetic_field_from_vector_potential", "chl_ic_orszag_tang",
+    "chl_magnetic_field_from_vector_potential", "chl_ic_orszag_tang", "chl_pack_real_cells",
+    "chl_pack_real_cells_f32", "chl_unpack_real_cells",
 ]
 LAUNCH_SPLIT, LAUNCH_PERSISTENT = 0, 1
 
@@ -146,6 +147,9 @@ def lib():
         "chl_update_magnetic_field": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d]),
         "chl_magnetic_field_from_vector_potential": (_i, [_dp, _i, _i, _i, _d, _d, _d, _dp]),
         "chl_ic_orszag_tang": (_i, [_dp, _i, _i, _d, ctypes.c_uint64, _d]),
+        "chl_pack_real_cells": (_i, [_vp, _vp] + [_i] * 8),
+        "chl_pack_real_cells_f32": (_i, [_vp, _vp] + [_i] * 8),
+        "chl_unpack_real_cells": (_i, [_vp, _vp] + [_i] * 8),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

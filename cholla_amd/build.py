@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcholla_amd.so")
+IO_LIB = os.path.join(HERE, "libcholla_amd_io.so")
 BUILD_DIR = os.path.join(ROOT, "_build")
 
 ARCH = os.environ.get("CHL_OFFLOAD_ARCH", "gfx950")
@@ -33,6 +34,7 @@ COMMON = [
 UNITS = [
     ("hydro_exact.hip", ["-ffp-contract=off"]),
     ("support_kernels.hip", ["-ffp-contract=off"]),
+    ("snapshot_kernels.hip", ["-ffp-contract=off"]),
     ("hydro_fast.hip", ["-ffp-contract=on"]),
     ("ensemble_exact.hip", ["-ffp-contract=off"]),
     ("ensemble_fast.hip", ["-ffp-contract=on"]),
@@ -90,9 +92,39 @@ def build(verbose=True):
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stderr}")
+    io = build_io()
     if verbose:
-        print(f"built {LIB} and {', '.join(DRIVERS)}")
+        print(f"built {LIB}, {', '.join(DRIVERS)}" + (f" and {IO_LIB}" if io else " (no HDF5: snapshot library skipped)"))
     return LIB
+
+
+def hdf5_prefix():
+    """Install prefix of the HDF5 C library (headers and libhdf5.so), or None."""
+    for pre in [os.environ.get("CHL_HDF5_PREFIX", ""), "/opt/conda", "/usr"]:
+        if pre and os.path.exists(os.path.join(pre, "include", "hdf5.h")) and \
+                os.path.exists(os.path.join(pre, "lib", "libhdf5.so")):
+            return pre
+    return None
+
+
+def build_io():
+    """libcholla_amd_io.so: Cholla-format HDF5 snapshots over the core library's C ABI
+    (host code only: g++ against the HDF5 C library).  The HDF5 prefix goes in as a
+    RUNPATH (direct dependencies only) and the C++ runtime is linked statically, so the
+    prefix's own, older libstdc++ is never what the HIP runtime resolves to."""
+    pre = hdf5_prefix()
+    if pre is None:
+        return None
+    tmp = IO_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", f"-I{os.path.join(ROOT, 'include')}",
+           f"-I{os.path.join(pre, 'include')}", os.path.join(CSRC, "io_hdf5.cpp"), "-o", tmp,
+           f"-L{HERE}", "-lcholla_amd", f"-L{os.path.join(pre, 'lib')}", "-lhdf5",
+           "-static-libstdc++", "-static-libgcc", f"-Wl,--enable-new-dtags,-rpath,$ORIGIN:{os.path.join(pre, 'lib')}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"snapshot library build failed: {' '.join(cmd)}\n{r.stderr}")
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
 
 
 if __name__ == "__main__":

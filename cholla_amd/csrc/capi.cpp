@@ -12,6 +12,7 @@
 
 #include "../../include/cholla_amd.h"
 #include "capi_internal.hpp"
+#include "snapshot_kernels.hpp"
 #include "support_kernels.hpp"
 #include "vl3d_launch.hpp"
 
@@ -621,6 +622,63 @@ int chl_set_boundaries(double *U, int nx, int ny, int nz, int n_fields, int n_gh
   if (ny > 1) flags[2] = flags_in[2], flags[3] = flags_in[3];
   if (nz > 1) flags[4] = flags_in[4], flags[5] = flags_in[5];
   CHL_HIP(chl::LaunchGhostFill(U, nx, ny, nz, n_fields, n_ghost, flags, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+namespace
+{
+// The dataset region of one field (see the snapshot entry points in cholla_amd.h).
+int SnapshotRegion(int nx, int ny, int nz, int ng, int nxr, int nyr, int nzr, int mhd_direction, chl::SnapshotShape &s)
+{
+  const bool one_d = ny == 1 && nz == 1;
+  if (!one_d && (ny == 1 || nz == 1)) return Fail(CHL_ERR_UNSUPPORTED, "2D grids are not built");
+  if (mhd_direction < -1 || mhd_direction > 2 || (one_d && mhd_direction >= 0))
+    return Fail(CHL_ERR_INVALID, "bad mhd_direction");
+  if (one_d) nyr = nzr = 1;
+  s = {nx,
+       ny,
+       nz,
+       nxr,
+       nyr,
+       nzr,
+       ng - int(mhd_direction == 0),
+       one_d ? 0 : ng - int(mhd_direction == 1),
+       one_d ? 0 : ng - int(mhd_direction == 2)};
+  if (!s.Valid()) return Fail(CHL_ERR_INVALID, "dataset region outside the grid");
+  return CHL_OK;
+}
+}  // namespace
+
+int chl_pack_real_cells(const double *d_field, double *d_buf, int nx, int ny, int nz, int n_ghost, int nx_real,
+                        int ny_real, int nz_real, int mhd_direction)
+{
+  if (!d_field || !d_buf) return Fail(CHL_ERR_INVALID, "null argument");
+  chl::SnapshotShape s;
+  CHL_TRY(SnapshotRegion(nx, ny, nz, n_ghost, nx_real, ny_real, nz_real, mhd_direction, s));
+  CHL_HIP(chl::LaunchSnapshotPack(d_field, d_buf, s, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_pack_real_cells_f32(const double *d_field, float *d_buf, int nx, int ny, int nz, int n_ghost, int nx_real,
+                            int ny_real, int nz_real, int mhd_direction)
+{
+  if (!d_field || !d_buf) return Fail(CHL_ERR_INVALID, "null argument");
+  chl::SnapshotShape s;
+  CHL_TRY(SnapshotRegion(nx, ny, nz, n_ghost, nx_real, ny_real, nz_real, mhd_direction, s));
+  CHL_HIP(chl::LaunchSnapshotPack32(d_field, d_buf, s, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_unpack_real_cells(const double *d_buf, double *d_field, int nx, int ny, int nz, int n_ghost, int nx_real,
+                          int ny_real, int nz_real, int mhd_direction)
+{
+  if (!d_field || !d_buf) return Fail(CHL_ERR_INVALID, "null argument");
+  chl::SnapshotShape s;
+  CHL_TRY(SnapshotRegion(nx, ny, nz, n_ghost, nx_real, ny_real, nz_real, mhd_direction, s));
+  CHL_HIP(chl::LaunchSnapshotUnpack(d_buf, d_field, s, nullptr));
   CHL_HIP(hipDeviceSynchronize());
   return CHL_OK;
 }

@@ -150,6 +150,25 @@ int chl_reconstruct(int reconstruction, const double *dev_conserved, double *dev
 int chl_hllc_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, double *dev_flux, int nx, int ny, int nz,
                     double gamma, int dir);
 
+/* ---- snapshot layout of one field (the HDF5 datasets of src/io/io.cpp).  The grid is
+ * x-fastest with ghost cells; a dataset holds the real cells z-fastest,
+ * buf[k + j*nz_real + i*nz_real*ny_real] (1D: buf[i]).  nx_real, ny_real, nz_real are the
+ * dataset extents; mhd_direction 0/1/2 marks a face-centred magnetic field whose dataset has
+ * one more face along that axis and starts at grid index n_ghost - 1 there, -1 a
+ * cell-centred field.  3D and 1D grids (2D: CHL_ERR_UNSUPPORTED). */
+/* CopyReal3D_GPU_Kernel (double destination), src/io/io_gpu.cu:44-60 (and the 1D branch of
+ * Fill_HDF5_Buffer_From_Grid_CPU, io.cpp:1258-1264) */
+int chl_pack_real_cells(const double *d_field, double *d_buf, int nx, int ny, int nz, int n_ghost, int nx_real,
+                        int ny_real, int nz_real, int mhd_direction);
+/* CopyReal3D_GPU_Kernel (float destination), io_gpu.cu:64-83, for the float32 snapshots of
+ * Output_Float32 (io.cpp:267-378) */
+int chl_pack_real_cells_f32(const double *d_field, float *d_buf, int nx, int ny, int nz, int n_ghost, int nx_real,
+                            int ny_real, int nz_real, int mhd_direction);
+/* Fill_Grid_From_HDF5_Buffer (io.cpp:2450-2490) and the magnetic-field loops of
+ * Read_Grid_HDF5 (io.cpp:2566-2700), on the device */
+int chl_unpack_real_cells(const double *d_buf, double *d_field, int nx, int ny, int nz, int n_ghost, int nx_real,
+                          int ny_real, int nz_real, int mhd_direction);
+
 /* ---- ensembles of independent periodic boxes.  The reference runs one box per
  * process (src/main.cpp:247-379); this advances many boxes of different sizes
  * together, each with its own CFL dt and clock, with the step of every box fused
