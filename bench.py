@@ -62,7 +62,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--kernel-steps", type=int, default=10)
     p.add_argument("--tiles", default="", help="fixed tile1,tile2,kchunk1,kchunk2 (skips the autotuner)")
-    p.add_argument("--workload", default="kh", choices=["kh", "ensemble", "mhd"])
+    p.add_argument("--workload", default="kh", choices=["kh", "ensemble", "mhd", "snapshot"])
     p.add_argument("--boxes", type=int, default=384, help="ensemble: number of boxes")
     p.add_argument("--looped-steps", type=int, default=5, help="ensemble: steps of the per-box loop baseline")
     p.add_argument("--u-floor", type=float, default=1e-3,
@@ -319,6 +319,68 @@ def bench_mhd(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_snapshot(args):
+    """Snapshot I/O of a KH state: the device transposes between the grid layout and the
+    HDF5 dataset order (8 B read + 8 B written per real cell, HBM-bound) at n^3, and a whole
+    HDF5 snapshot written and read back (device -> file -> device, the file under $TMPDIR)
+    at min(n, 256)^3, checked to restore every real cell."""
+    import tempfile
+
+    import cholla_amd as chl
+    from cholla_amd import snapshot as snap
+
+    n, ng, g = args.n, 4, 5.0 / 3.0
+    N = n + 2 * ng
+    U = chl.ic_kelvin_helmholtz(n, ng, g, seed=args.seed)
+    D = chl.DeviceArray.from_host(U)
+    del U
+    buf = chl.DeviceArray(n ** 3)
+    L = chl.lib()
+    res = {}
+    for name, fn, a, b in (("pack", L.chl_pack_real_cells, D.ptr, buf.ptr),
+                           ("unpack", L.chl_unpack_real_cells, buf.ptr, D.ptr)):
+        chl.check(fn(a, b, N, N, N, ng, n, n, n, -1))
+        reps = max(args.kernel_steps, 5)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            chl.check(fn(a, b, N, N, N, ng, n, n, n, -1))
+        sec = (time.perf_counter() - t0) / reps
+        res[name] = {"ms": sec * 1e3, "GB_per_s": 16 * n ** 3 / sec / 1e9}
+        log(f"{name} {n}^3 field: {sec * 1e3:.3f} ms, {16 * n ** 3 / sec / 1e9:.0f} GB/s (16 B per real cell)")
+    buf.free()
+    m = min(n, 256)
+    M = m + 2 * ng
+    V = chl.ic_kelvin_helmholtz(m, ng, g, seed=args.seed)
+    S = chl.DeviceArray.from_host(V)
+    R = chl.DeviceArray(V.size, fill=np.zeros(V.size))
+    h = snap.header(M, M, M, ng, gamma=g, dx=(1.0 / m,) * 3)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "0.h5")
+        t0 = time.perf_counter()
+        snap.write_hdf5(path, S, h)
+        t_w = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        snap.read_hdf5(path, R, snap.header(M, M, M, ng))
+        t_r = time.perf_counter() - t0
+    ok = bool(np.array_equal(R.download().reshape(5, M, M, M)[:, ng:-ng, ng:-ng, ng:-ng],
+                             V.reshape(5, M, M, M)[:, ng:-ng, ng:-ng, ng:-ng]))
+    log(f"snapshot {m}^3: {size / 1e9:.2f} GB file, write {t_w:.2f} s ({size / t_w / 1e9:.2f} GB/s), "
+        f"read {t_r:.2f} s ({size / t_r / 1e9:.2f} GB/s), round trip exact: {ok}")
+    out = {
+        "metric": f"snapshot layout transpose GB/s ({n}^3 field, 16 B per real cell)",
+        "value": res["pack"]["GB_per_s"], "unit": "GB/s", "n_gpus": 1, "steps": max(args.kernel_steps, 5),
+        "warmup": 1, "ms_per_step": res["pack"]["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"KH {n}^3 state: pack/unpack of one field; HDF5 snapshot of the {m}^3 state"},
+        "roofline": {"bound": "hbm", "achieved": res["pack"]["GB_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": res["pack"]["GB_per_s"] / HBM_PEAK_GBS, "traffic": None},
+        "kernels": res,
+        "file": {"n": m, "bytes": size, "write_s": t_w, "read_s": t_r, "round_trip_exact": ok},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy > 0:
@@ -330,12 +392,12 @@ def main():
         sys.exit(2)
     import cholla_amd as chl
 
-    if args.workload in ("ensemble", "mhd"):
+    if args.workload in ("ensemble", "mhd", "snapshot"):
         if chl.device_count() < 1:
             log("no HIP device visible")
             sys.exit(1)
         log("device:", chl.device_name())
-        (bench_ensemble if args.workload == "ensemble" else bench_mhd)(args)
+        {"ensemble": bench_ensemble, "mhd": bench_mhd, "snapshot": bench_snapshot}[args.workload](args)
         return
 
     if chl.device_count() < 1:
