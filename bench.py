@@ -197,6 +197,30 @@ def bench_ensemble(args):
                 f"{r['non_finite_boxes'][:12]}")
         res[form] = r
         finals[form] = (states, ens)
+    if args.launch == "auto" and "mixed" in res and res["mixed"]["config"]["launch"] != chl.LAUNCH_PERSISTENT:
+        # the persistent launch (one kernel per step over a work queue), timed beside the
+        # autotuner's pick on a fresh copy of the initial boxes
+        states = copies()
+        sc = res["mixed"]["config"]  # the split pick's tiles and z-chunks, where that pair is compiled
+        pair = [sc["tile1"], sc["tile2"]] in [list(p) for p in chl.persistent_pairs()]
+        kw = dict(tile1=sc["tile1"], tile2=sc["tile2"], kchunk1=sc["kchunk1"], kchunk2=sc["kchunk2"]) if pair else {}
+        ens_p = chl.Ensemble(dims, ng, chl.RECON_PPMC, chl.PREC_MIXED, chl.LAUNCH_PERSISTENT, dual_energy=de,
+                             positivity_fallback=pf, **kw)
+        ens_p.bind(states, dxyz)
+        ens_p.start(g, U_floor=args.u_floor)
+        ens_p.advance(args.warmup)
+        chl.check(chl.lib().chl_synchronize())
+        pst = max(steps // 5, 10)
+        w0 = time.perf_counter()
+        ms = ens_p.advance(pst)
+        wall = time.perf_counter() - w0
+        prate = cells * pst / max(ms * 1e-3, wall)
+        res["mixed"]["persistent"] = {"cell_updates_per_s": prate, "ms_per_step": max(ms, 1e3 * wall) / pst,
+                                      "steps": pst, "config": ens_p.config().as_dict()}
+        log(f"[mixed] persistent launch {prate / 1e9:.3f} G cell-updates/s ({max(ms, 1e3 * wall) / pst:.2f} ms/step)")
+        ens_p.close()
+        for U in states:
+            U.free()
     l1 = None
     if "mixed" in finals and "fp64" in finals:
         names = ["density", "momentum_x", "momentum_y", "momentum_z", "Energy", "GasEnergy"][:nf]

@@ -64,10 +64,13 @@ struct chl_ensemble {
   hipGraphExec_t graph = nullptr;
   int num_cu          = 256;
   int per_cu          = 1;
+  int qoff[chl::kQueueXcds + 1] = {};  // persistent queue of XCD x: [qoff[x], qoff[x + 1])
   bool bound = false, started = false;
   double gamma = 5.0 / 3.0, density_floor = 0.0, U_floor = 0.0;
 
-  int *status() { return reinterpret_cast<int *>(head + 1); }
+  // head buffer: the persistent queues' heads (XCD x at x * kHeadStride), then the status word
+  static constexpr size_t kHeadBytes = 4096;
+  int *status() { return reinterpret_cast<int *>(head + chl::kQueueXcds * chl::kHeadStride); }
   size_t Cells(int b) const { return size_t(dims[3 * b]) * dims[3 * b + 1] * dims[3 * b + 2]; }
   void DropGraph()
   {
@@ -163,16 +166,54 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
     push(l1, b, 0, d.items1);
     push(l2, b, 1, d.t2[0] * d.t2[1] * d.t2[2]);
   }
-  // persistent queue: stage 1 of box order[i], then stage 2 of box order[i-1] (and with
-  // dual energy the selection of box order[i-2] and the sync of box order[i-3]): every
-  // item of a box's phase sits after all items of its previous phase
-  const size_t nb = e->order.size(), lag = c.dual_energy ? 3 : 1;
-  for (size_t i = 0; i < nb + lag; i++)
-    for (size_t st = 0; st <= lag; st++) {
-      if (i < st || i - st >= nb) continue;
-      const int b = e->order[i - st];
-      push(q, b, int(st), st == 0 ? e->hbox[b].items1 : e->hbox[b].items2);
+  // Persistent queues, one per XCD.  The boxes are dealt to the XCDs largest first, each
+  // to the XCD with the least work so far (cells), so every phase of a box runs on one XCD.
+  // Within a queue: the stage-1 items of its boxes in order; each later phase of a box
+  // (stage 2, and with dual energy the selection and the sync) goes in once at least
+  // `gap` further items have been queued after the box's previous phase, gap being the
+  // XCD's resident workgroups.  A phase item is then taken only after a whole generation
+  // of workgroups has started since the last item it depends on, so it rarely waits, also
+  // for small boxes whose phases hold a few items each.  Every item of a box's phase sits
+  // after all items of its previous phase in the same queue, and stage-1 items never
+  // wait: each queue drains without deadlock.
+  if (c.launch == CHL_LAUNCH_PERSISTENT) {
+    int pc = 0;
+    CHL_HIP(chl::EnsemblePersistentSlots(c.precision, c.reconstruction, PairIndex(c.tile1, c.tile2), c.dual_energy,
+                                         &pc));
+    e->per_cu = std::max(pc, 1);
+  }
+  {
+    std::vector<std::vector<int>> owned(chl::kQueueXcds);
+    std::vector<double> load(chl::kQueueXcds, 0.0);
+    for (int b : e->order) {  // order: largest box first
+      const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+      owned[x].push_back(b);
+      load[x] += double(e->Cells(b));
     }
+    const int last_phase = c.dual_energy ? 3 : 1;
+    const size_t gap     = size_t(e->per_cu) * size_t(std::max(e->num_cu / chl::kQueueXcds, 1));
+    struct Pending {
+      size_t ready_at;
+      int b, phase;
+    };
+    for (int x = 0; x < chl::kQueueXcds; x++) {
+      e->qoff[x] = int(q.size());
+      const size_t base = q.size();
+      std::vector<Pending> pend;  // FIFO; ready_at grows along it
+      size_t head = 0;
+      auto queue_phase = [&](Pending p) {
+        push(q, p.b, p.phase, e->hbox[p.b].items2);
+        if (p.phase < last_phase) pend.push_back({q.size() - base + gap, p.b, p.phase + 1});
+      };
+      for (int b : owned[x]) {
+        push(q, b, 0, e->hbox[b].items1);
+        pend.push_back({q.size() - base + gap, b, 1});
+        while (head < pend.size() && pend[head].ready_at <= q.size() - base) queue_phase(pend[head++]);
+      }
+      while (head < pend.size()) queue_phase(pend[head++]);
+    }
+    e->qoff[chl::kQueueXcds] = int(q.size());
+  }
   e->n1 = int(l1.size()), e->n2 = int(l2.size()), e->nq = int(q.size());
   const size_t need = l1.size() + l2.size() + q.size();
   if (need > e->items_cap) {
@@ -190,12 +231,6 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
   CHL_HIP(hipMemcpyAsync(e->items, all.data(), need * sizeof(chl::WorkItem), hipMemcpyHostToDevice, e->stream));
   CHL_HIP(hipMemcpyAsync(e->box, e->hbox.data(), e->n * sizeof(chl::BoxDesc), hipMemcpyHostToDevice, e->stream));
   CHL_HIP(hipStreamSynchronize(e->stream));
-  if (c.launch == CHL_LAUNCH_PERSISTENT) {
-    int pc = 0;
-    CHL_HIP(chl::EnsemblePersistentSlots(c.precision, c.reconstruction, PairIndex(c.tile1, c.tile2), c.dual_energy,
-                                         &pc));
-    e->per_cu = std::max(pc, 1);
-  }
   e->DropGraph();
   return CHL_OK;
 }
@@ -211,6 +246,7 @@ chl::EnsembleArgs Args(chl_ensemble *e, int list)
   if (c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
   if (c.positivity_fallback) flags |= chl::kFlagPositivityFallback;
   a.flags = flags, a.head = e->head, a.status = e->status(), a.spin_limit = kSpinLimit;
+  for (int x = 0; x <= chl::kQueueXcds; x++) a.qoff[x] = e->qoff[x];
   a.de = c.dual_energy;
   if (list == 0) a.items = e->items, a.n_items = e->n1;
   if (list == 1) a.items = e->items + e->n1, a.n_items = e->n2;
@@ -238,7 +274,9 @@ int EnqueueStep(chl_ensemble *e)
       CHL_HIP(chl::LaunchEnsembleDe(1, Args(e, 1), c.tile2, e->stream));
     }
   } else {
-    const int grid = std::min(e->per_cu * e->num_cu, std::max(e->nq, 1));
+    // every XCD gets its resident workgroups (dispatch deals workgroups round-robin), so
+    // every queue is served
+    const int grid = e->per_cu * e->num_cu;
     CHL_HIP(chl::LaunchEnsemblePersistent(c.precision, Args(e, 2), c.reconstruction, PairIndex(c.tile1, c.tile2),
                                           grid, e->stream));
   }
@@ -260,12 +298,23 @@ int EnsureGraph(chl_ensemble *e)
   return CHL_OK;
 }
 
-// Reports (and clears) a dependency wait of the persistent launch that gave up.
-int CheckStatus(chl_ensemble *e)
+// Reports (and clears) a dependency wait of the persistent launch that gave up, and a
+// persistent queue that the last step left undrained.
+int CheckStatus(chl_ensemble *e, bool stepped)
 {
   int st = 0;
   CHL_HIP(hipMemcpyAsync(&st, e->status(), sizeof(int), hipMemcpyDeviceToHost, e->stream));
   CHL_HIP(hipStreamSynchronize(e->stream));
+  if (e->cfg.launch == CHL_LAUNCH_PERSISTENT && stepped) {
+    std::vector<unsigned int> heads(chl::kQueueXcds * chl::kHeadStride);
+    CHL_HIP(hipMemcpyAsync(heads.data(), e->head, heads.size() * sizeof(unsigned int), hipMemcpyDeviceToHost,
+                           e->stream));
+    CHL_HIP(hipStreamSynchronize(e->stream));
+    for (int x = 0; x < chl::kQueueXcds; x++)
+      if (heads[x * chl::kHeadStride] < unsigned(e->qoff[x + 1] - e->qoff[x]))
+        return Fail(CHL_ERR_HIP, "persistent ensemble launch: the queue of XCD " + std::to_string(x) +
+                                     " was not drained; the step's result is not valid");
+  }
   if (st != 0) {
     CHL_HIP(hipMemsetAsync(e->status(), 0, sizeof(int), e->stream));
     CHL_HIP(hipStreamSynchronize(e->stream));
@@ -349,9 +398,9 @@ int chl_ensemble_create(const chl_ensemble_config *cfg, int n_boxes, const int *
   if (er == hipSuccess) er = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (er == hipSuccess) er = hipMalloc(reinterpret_cast<void **>(&e->box), n_boxes * sizeof(chl::BoxDesc));
   if (er == hipSuccess) er = hipMalloc(reinterpret_cast<void **>(&e->clk), n_boxes * sizeof(chl::BoxClock));
-  if (er == hipSuccess) er = hipMalloc(reinterpret_cast<void **>(&e->head), 256);
+  if (er == hipSuccess) er = hipMalloc(reinterpret_cast<void **>(&e->head), chl_ensemble::kHeadBytes);
   if (er == hipSuccess) er = hipMemsetAsync(e->clk, 0, n_boxes * sizeof(chl::BoxClock), e->stream);
-  if (er == hipSuccess) er = hipMemsetAsync(e->head, 0, 256, e->stream);
+  if (er == hipSuccess) er = hipMemsetAsync(e->head, 0, chl_ensemble::kHeadBytes, e->stream);
   if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
   if (er != hipSuccess) {
     chl_ensemble_destroy(e);
@@ -444,7 +493,7 @@ int chl_ensemble_advance(chl_ensemble *e, int n_steps, int use_graph, float *ms_
     return CHL_OK;
   };
   CHL_TRY(TimeOnStream(e, enqueue, ms_out));
-  return CheckStatus(e);
+  return CheckStatus(e, n_steps > 0);
 }
 
 int chl_ensemble_advance_looped(chl_ensemble *e, int n_steps, float *ms_out)
@@ -571,16 +620,17 @@ int chl_ensemble_autotune(chl_ensemble *e, int steps, chl_ensemble_config *chose
     if (rc == CHL_OK && ms < best_ms) best_ms = ms, best = sc;
   }
   // persistent launch: whole steps
+  const int kc_pairs[5][2] = {{8, 16}, {8, 32}, {16, 16}, {16, 32}, {32, 32}};
   for (int p = 0; p < chl::kNumPairs && rc == CHL_OK; p++)
-    for (int kc : {8, 16, 32}) {
+    for (const auto &kc : kc_pairs) {
       if (orig.dual_energy && !DePairIdx(p)) continue;
       chl_ensemble_config c = orig;
       c.launch = CHL_LAUNCH_PERSISTENT, c.tile1 = chl::kPairs[p][0], c.tile2 = chl::kPairs[p][1];
-      c.kchunk1 = kc, c.kchunk2 = kc;
+      c.kchunk1 = kc[0], c.kchunk2 = kc[1];
       float ms = 0;
       if ((rc = plan(c)) != CHL_OK) break;
       if ((rc = timed([&] { return EnqueueStep(e); }, &ms)) != CHL_OK) break;
-      if ((rc = CheckStatus(e)) != CHL_OK) break;
+      if ((rc = CheckStatus(e, true)) != CHL_OK) break;
       if (ms < best_ms) best_ms = ms, best = c;
     }
   const std::string err = rc == CHL_OK ? std::string() : std::string(chl_last_error());

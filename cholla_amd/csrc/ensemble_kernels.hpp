@@ -13,10 +13,11 @@
 //   EnsembleStage2Kernel     ("split" launch)
 //   or
 //   EnsemblePersistentKernel one launch per step: resident workgroups pull items
-//                            from a queue holding every box's stage-1 tiles and,
-//                            one box behind, its stage-2 tiles; a stage-2 tile
-//                            waits until its box's stage-1 tiles are all done
-//                            ("persistent" launch).
+//                            from their XCD's queue, which holds the stage-1
+//                            tiles of that XCD's boxes and, a generation of
+//                            workgroups later, each box's stage-2 tiles; a
+//                            stage-2 tile waits until its box's stage-1 tiles
+//                            are all done ("persistent" launch).
 //
 // Each box keeps its own dt: the tile bodies are the single-box ones
 // (Stage1Tile / Stage2Tile) fed with that box's StepArgs.
@@ -69,6 +70,10 @@ struct WorkItem {
   int pad_;
 };
 
+// The persistent launch keeps one queue per XCD (see EnsemblePersistentKernel).
+constexpr int kQueueXcds  = 8;
+constexpr int kHeadStride = 32;  // queue heads 128 bytes apart
+
 struct EnsembleArgs {
   const BoxDesc *box;
   BoxClock *clk;
@@ -79,8 +84,9 @@ struct EnsembleArgs {
   int R1;  // stage-1 region growth: reconstruction radius + 1
   double gamma, density_floor, U_floor;
   int flags;
-  unsigned int *head;  // persistent queue head
+  unsigned int *head;  // persistent queue heads, XCD x at head[x * kHeadStride]
   int *status;         // set non-zero if a dependency wait gave up
+  int qoff[kQueueXcds + 1];  // persistent queue of XCD x: items[qoff[x] .. qoff[x + 1])
   long long spin_limit;
   int de;              // dual energy (host-side dispatch only; the kernels are templated on it)
 };
@@ -202,14 +208,18 @@ union SmemPair {
   B s2;
 };
 
-// Persistent-launch dependencies.  The finishing side: every lane releases its stores
-// at device (agent) scope, the workgroup meets, one lane counts the tile.  The waiting
-// side: one lane spins on the count (bounded: after spin_limit sleeps it sets *status and
-// gives up, as does every later wait once one has), the workgroup meets, every lane
-// acquires at device scope before reading what the counted tiles wrote.
+// Persistent-launch dependencies.  Every phase of a box runs on one XCD (its queue's),
+// so what one tile of a box writes and another reads only has to cross that XCD's L2,
+// which all its CUs share, and each CU's vector L1, which is write-through.  The finishing
+// side: every lane waits until its stores have reached the L2, the workgroup meets, one
+// lane counts the tile.  The waiting side: one lane spins on the count (bounded: after
+// spin_limit sleeps it sets *status and gives up, as does every later wait once one has),
+// the workgroup meets and drops its CU's L1 lines, so the box's data is read from the L2.
+// (Device-scope release/acquire would write back and invalidate the whole L2 of the XCD
+// for every tile.)
 __device__ __forceinline__ void SignalTileDone(int *count)
 {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) atomicAdd(count, 1);
 }
@@ -228,18 +238,19 @@ __device__ __forceinline__ void WaitTiles(const EnsembleArgs &e, int *count, int
     }
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  asm volatile("buffer_inv sc0" ::: "memory");
 }
 
-// One whole step in one launch.  Deadlock freedom: items are taken from the queue
-// in order, and every stage-2 item of box b sits after all stage-1 items of b, so
-// when a workgroup waits on box b, each stage-1 item of b is already held by a
-// running workgroup, and stage-1 items never wait.  The wait is still bounded
-// (spin_limit sleeps); if it gives up, *status is set and the host reports it.
-//
-// Cross-XCD visibility: the W writes of a stage-1 tile are released at device
-// (agent) scope by every lane before the tile is counted, and a stage-2 tile
-// acquires at the same scope after it has seen the count.
+// XCD of the calling workgroup (the XCC_ID hardware register).
+__device__ __forceinline__ int XcdId() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) % kQueueXcds; }
+
+// One whole step in one launch.  Each workgroup serves the queue of the XCD it runs
+// on; the host splits the boxes over the XCD queues by cell count.  Deadlock freedom:
+// items are taken from a queue in order, and every stage-2 item of box b sits after all
+// stage-1 items of b in the same queue, so when a workgroup waits on box b, each stage-1
+// item of b is already held by a running workgroup of that XCD, and stage-1 items never
+// wait.  The wait is still bounded (spin_limit sleeps); if it gives up, *status is set
+// and the host reports it, as it does a queue that no workgroup drained.
 // The persistent kernel holds both stage bodies; left to itself the register
 // allocator sizes it for two waves per SIMD, one fewer than the mixed form's
 // stage kernels run at.
@@ -257,12 +268,15 @@ EnsemblePersistentKernel(EnsembleArgs e)
   __shared__ double s_red[TX2 * TY2 / 64];
   __shared__ int s_item;
   const int tid = threadIdx.x;
+  const int xcd = XcdId();
+  unsigned int *qhead = e.head + xcd * kHeadStride;
+  const int q_lo = e.qoff[xcd], q_hi = e.qoff[xcd + 1];
   for (;;) {
     __syncthreads();  // the previous item is finished with LDS and s_item
-    if (tid == 0) s_item = int(atomicAdd(e.head, 1u));
+    if (tid == 0) s_item = q_lo + int(atomicAdd(qhead, 1u));
     __syncthreads();
     const int q = __builtin_amdgcn_readfirstlane(s_item);
-    if (q >= e.n_items) break;
+    if (q >= q_hi) break;
     const WorkItem it = e.items[q];
     const int b       = __builtin_amdgcn_readfirstlane(it.box);
     const BoxDesc &d  = e.box[b];

@@ -314,12 +314,13 @@ __global__ void DtAdvanceKernel(const unsigned long long *dti_bits, double *dt_d
 }
 
 // One thread per box: the same dt and clock arithmetic as DtAdvanceKernel, then the
-// accumulators of the coming step are cleared.  Thread 0 also rewinds the queue of
-// the persistent launch.
+// accumulators of the coming step are cleared.  Thread 0 also rewinds the per-XCD
+// queues of the persistent launch.
 __global__ void EnsembleClockKernel(BoxClock *clk, const BoxDesc *box, int n_boxes, double cfl, unsigned int *head)
 {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b == 0 && head) *head = 0u;
+  if (b == 0 && head)
+    for (int x = 0; x < kQueueXcds; x++) head[x * kHeadStride] = 0u;
   if (b >= n_boxes) return;
   BoxClock c       = clk[b];
   const double tout = box[b].tout;
