@@ -171,7 +171,7 @@ hipError_t LaunchCtGrid(const double *Fx, const double *Fy, const double *Fz, co
                         int ny, int nz, hipStream_t s)
 {
   hipLaunchKernelGGL((mhd::MhdEmfKernel<double>), dim3(MhdBlocks(size_t(nx) * ny * nz)), dim3(256), 0, s, Fx, Fy,
-                     Fz, U, E, nx, ny, nz, 1, nx - 1, ny - 1, nz - 1, nullptr);
+                     Fz, U, E, nx, ny, nz, 1, nx - 1, ny - 1, nz - 1, nullptr, nullptr);
   return hipGetLastError();
 }
 

@@ -563,11 +563,47 @@ __device__ __forceinline__ double CtSlopeM(const TF *flux, const double *U, doub
   return ef - ec;
 }
 
-// mhd::Calculate_CT_Electric_Fields for edges with every index in [lo, hi] (ct_electric_fields.cu:20-279)
+// The cell-centred electric field term of _ctSlope (ct_electric_fields.h:102-114) of cell
+// ic for the edge direction ctDir: (m3 * B2 - m2 * B3) / d with the face fields averaged to
+// the cell centre; the same expression, operands and bounds guards as CtSlopeM.
+__device__ __forceinline__ double CellEmf(const double *U, int ctDir, long ic, int nx, int ny, long n, long nU)
+{
+  const int m1 = (ctDir + 1) % 3, m2 = (ctDir + 2) % 3;
+  const long ib2   = ic - (m1 == 0 ? 1 : (m1 == 1 ? long(nx) : long(nx) * long(ny)));
+  const long ib3   = ic - (m2 == 0 ? 1 : (m2 == 1 ? long(nx) : long(nx) * long(ny)));
+  const double d   = AtF(U, ic, nU);
+  const double mm2 = AtF(U, ic + (m1 + 1) * n, nU);
+  const double mm3 = AtF(U, ic + (m2 + 1) * n, nU);
+  const double b2  = 0.5 * (AtF(U, ic + (m1 + kBx) * n, nU) + AtF(U, ib2 + (m1 + kBx) * n, nU));
+  const double b3  = 0.5 * (AtF(U, ic + (m2 + kBx) * n, nU) + AtF(U, ib3 + (m2 + kBx) * n, nU));
+  return (mm3 * b2 - mm2 * b3) / d;
+}
+
+// The three cell-centred terms of every cell with each index in [lo, hi]: each is shared by
+// the 8 edge slopes around the cell, so the edge kernel reads them instead of recomputing
+// them (a division and 7 state loads) per slope.
+template <int kUnused = 0>  // a template, so every translation unit may instantiate it
+__global__ void __launch_bounds__(256) MhdCellEmfKernel(const double *U, double *Ec, int nx, int ny, int nz, int lo,
+                                                        int hix, int hiy, int hiz, const double *dt_dev)
+{
+  const size_t n  = size_t(nx) * ny * nz;
+  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  if (dt_dev && *dt_dev == 0.0) return;
+  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
+  if (x < lo || y < lo || z < lo || x > hix || y > hiy || z > hiz) return;
+  const long nU = 8 * long(n);
+#pragma unroll
+  for (int c = 0; c < 3; c++) Ec[c * n + id] = CellEmf(U, c, long(id), nx, ny, long(n), nU);
+}
+
+// mhd::Calculate_CT_Electric_Fields for edges with every index in [lo, hi] (ct_electric_fields.cu:20-279).
+// Ec: the cell-centred terms from MhdCellEmfKernel over the cells [lo - 1, hi] (every cell an
+// edge slope of the region reads), or null to compute them per slope.
 template <typename TF>
 __global__ void __launch_bounds__(256) MhdEmfKernel(const TF *Fx, const TF *Fy, const TF *Fz, const double *U,
                                                     double *E, int nx, int ny, int nz, int lo, int hix, int hiy,
-                                                    int hiz, const double *dt_dev)
+                                                    int hiz, const double *dt_dev, const double *Ec)
 {
   const size_t n  = size_t(nx) * ny * nz;
   const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -580,7 +616,14 @@ __global__ void __launch_bounds__(256) MhdEmfKernel(const TF *Fx, const TF *Fy, 
     return AtF(a, long(i) + long(j) * nx + long(k) * long(nx) * ny + long(f) * long(n), nF);
   };
   auto S = [&](const TF *fl, double sgn, int ctd, int a, int b, int c, int d, int i, int j, int k) {
-    return CtSlopeM(fl, U, sgn, ctd, a, b, c, d, i, j, k, nx, ny, long(n), nF, nU);
+    if (!Ec) return CtSlopeM(fl, U, sgn, ctd, a, b, c, d, i, j, k, nx, ny, long(n), nF, nU);
+    // CtSlopeM with its cell-centred term read back: ic and iflux as there
+    const long ic = long(i - int(a == 0) - int(b == 0)) + long(j - int(a == 1) - int(b == 1)) * nx +
+                    long(k - int(a == 2) - int(b == 2)) * long(nx) * ny;
+    const long iflux = long(i - int(c == 0) - int(d == 0)) + long(j - int(c == 1) - int(d == 1)) * nx +
+                       long(k - int(c == 2) - int(d == 2)) * long(nx) * ny;
+    const double ef = sgn * AtF(fl, iflux + (int(sgn == 1) + kBx) * long(n), nF);
+    return ef - Ec[ctd * long(n) + ic];
   };
   auto pick = [](double s, double pos, double neg) {
     return s > 0.0 ? pos : (s < 0.0 ? neg : 0.5 * (pos + neg));
@@ -621,6 +664,7 @@ struct MhdArgs {
   double *Uh;         // half-step state (8 fields)
   void *F[3];         // flux arrays (7 fields each, P::T)
   double *E;          // edge electric fields (3 fields)
+  double *Ec;         // cell-centred electric-field terms of the edge slopes (3 fields)
   int nx, ny, nz, ng;
   double dx, dy, dz, gamma;
   const double *dt_dev;  // device dt (device-timed step), or null

@@ -24,8 +24,10 @@ hipError_t MhdStepT(const mhd::MhdArgs &a, hipStream_t s)
   hipLaunchKernelGGL((mhd::MhdFlux1Kernel<P, 0>), dim3(g), dim3(256), 0, s, a);
   hipLaunchKernelGGL((mhd::MhdFlux1Kernel<P, 1>), dim3(g), dim3(256), 0, s, a);
   hipLaunchKernelGGL((mhd::MhdFlux1Kernel<P, 2>), dim3(g), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(mhd::MhdCellEmfKernel<0>, dim3(g), dim3(256), 0, s, a.U, a.Ec, a.nx, a.ny, a.nz, 1, a.nx - 1,
+                     a.ny - 1, a.nz - 1, a.dt_dev);
   hipLaunchKernelGGL((mhd::MhdEmfKernel<T>), dim3(g), dim3(256), 0, s, Fx, Fy, Fz, a.U, a.E, a.nx, a.ny, a.nz, 2,
-                     a.nx - 1, a.ny - 1, a.nz - 1, a.dt_dev);
+                     a.nx - 1, a.ny - 1, a.nz - 1, a.dt_dev, a.Ec);
   hipLaunchKernelGGL((mhd::MhdHalfKernel<P>), dim3(g), dim3(256), 0, s, a);
   hipLaunchKernelGGL((mhd::MhdFlux2Kernel<P, 0>), dim3(g), dim3(256), 0, s, a, ng - 1, a.nx - ng, a.ny - ng,
                      a.nz - ng);
@@ -33,8 +35,10 @@ hipError_t MhdStepT(const mhd::MhdArgs &a, hipStream_t s)
                      a.nz - ng);
   hipLaunchKernelGGL((mhd::MhdFlux2Kernel<P, 2>), dim3(g), dim3(256), 0, s, a, ng - 1, a.nx - ng, a.ny - ng,
                      a.nz - ng);
-  hipLaunchKernelGGL((mhd::MhdEmfKernel<T>), dim3(g), dim3(256), 0, s, Fx, Fy, Fz, a.Uh, a.E, a.nx, a.ny, a.nz, ng,
+  hipLaunchKernelGGL(mhd::MhdCellEmfKernel<0>, dim3(g), dim3(256), 0, s, a.Uh, a.Ec, a.nx, a.ny, a.nz, ng - 1,
                      a.nx - ng, a.ny - ng, a.nz - ng, a.dt_dev);
+  hipLaunchKernelGGL((mhd::MhdEmfKernel<T>), dim3(g), dim3(256), 0, s, Fx, Fy, Fz, a.Uh, a.E, a.nx, a.ny, a.nz, ng,
+                     a.nx - ng, a.ny - ng, a.nz - ng, a.dt_dev, a.Ec);
   hipLaunchKernelGGL((mhd::MhdFullKernel<P>), dim3(g), dim3(256), 0, s, a);
   return hipGetLastError();
 }
