@@ -597,6 +597,59 @@ __global__ void __launch_bounds__(256) MhdCellEmfKernel(const double *U, double 
   for (int c = 0; c < 3; c++) Ec[c * n + id] = CellEmf(U, c, long(id), nx, ny, long(n), nU);
 }
 
+// The three edge fields of edge id from the cell-centred terms Ec (MhdCellEmfKernel): the
+// arithmetic of the per-slope form below, term for term.  Every index is an offset of at
+// most one cell per axis below id, inside the grid for the kernel's regions, so the loads
+// need no bounds guards.
+template <typename TF>
+__device__ __forceinline__ void EmfEdgesFromCellTerms(const TF *Fx, const TF *Fy, const TF *Fz, const double *Ec,
+                                                      double *E, long id, long n, int nx, int ny)
+{
+  const long sx = 1, sy = nx, sz = long(nx) * ny;
+  auto sh  = [&](int ax) -> long { return ax == 0 ? sx : (ax == 1 ? sy : (ax == 2 ? sz : 0)); };
+  auto flx = [&](const TF *a, long o, int f) { return double(a[id - o + f * n]); };
+  // _ctSlope with its cell-centred term read back (CtSlopeM's operands)
+  auto slope = [&](const TF *fl, double sgn, int ctd, int a, int b, int c, int d) {
+    const double ef = sgn * flx(fl, sh(c) + sh(d), kBx + int(sgn == 1));
+    return ef - Ec[ctd * n + id - sh(a) - sh(b)];
+  };
+  auto up = [](double s, auto pos_f, auto neg_f) -> double {
+    // both slopes are loaded up front: their loads overlap, and a branch per lane would
+    // serialise them behind the sign test
+    const double pos = pos_f(), neg = neg_f();
+    return s > 0.0 ? pos : (s < 0.0 ? neg : 0.5 * (pos + neg));
+  };
+#define CHL_SLOPE(fl, sg, cd, a, b, c, d) [&] { return slope(fl, sg, cd, a, b, c, d); }
+  {  // x electric field
+    const double syp = up(flx(Fz, sz, 0), CHL_SLOPE(Fy, -1, 0, 2, -1, 1, 2), CHL_SLOPE(Fy, -1, 0, -1, -1, 1, -1));
+    const double syn = up(flx(Fz, sy + sz, 0), CHL_SLOPE(Fy, -1, 0, 1, 2, 1, 2), CHL_SLOPE(Fy, -1, 0, 1, -1, 1, -1));
+    const double szp = up(flx(Fy, sy, 0), CHL_SLOPE(Fz, 1, 0, 1, -1, 1, 2), CHL_SLOPE(Fz, 1, 0, -1, -1, 2, -1));
+    const double szn = up(flx(Fy, sy + sz, 0), CHL_SLOPE(Fz, 1, 0, 1, 2, 1, 2), CHL_SLOPE(Fz, 1, 0, 2, -1, -1, 2));
+    const double fyp = +flx(Fz, sz, 6), fyn = +flx(Fz, sy + sz, 6);
+    const double fzp = -flx(Fy, sy, 5), fzn = -flx(Fy, sy + sz, 5);
+    E[id] = 0.25 * (+fyp + fyn + fzp + fzn + syp + syn + szp + szn);
+  }
+  {  // y electric field
+    const double sxp = up(flx(Fz, sz, 0), CHL_SLOPE(Fx, 1, 1, 2, -1, 0, 2), CHL_SLOPE(Fx, 1, 1, -1, -1, 0, -1));
+    const double sxn = up(flx(Fz, sx + sz, 0), CHL_SLOPE(Fx, 1, 1, 0, 2, 0, 2), CHL_SLOPE(Fx, 1, 1, 0, -1, 0, -1));
+    const double szp = up(flx(Fx, sx, 0), CHL_SLOPE(Fz, -1, 1, 0, -1, 0, 2), CHL_SLOPE(Fz, -1, 1, -1, -1, 2, -1));
+    const double szn = up(flx(Fx, sx + sz, 0), CHL_SLOPE(Fz, -1, 1, 0, 2, 0, 2), CHL_SLOPE(Fz, -1, 1, 2, -1, 2, -1));
+    const double fxp = -flx(Fz, sz, 5), fxn = -flx(Fz, sx + sz, 5);
+    const double fzp = +flx(Fx, sx, 6), fzn = +flx(Fx, sx + sz, 6);
+    E[n + id] = 0.25 * (+fxp + fxn + fzp + fzn + sxp + sxn + szp + szn);
+  }
+  {  // z electric field
+    const double syp = up(flx(Fx, sx, 0), CHL_SLOPE(Fy, 1, 2, 0, -1, 0, 1), CHL_SLOPE(Fy, 1, 2, -1, -1, 1, -1));
+    const double syn = up(flx(Fx, sx + sy, 0), CHL_SLOPE(Fy, 1, 2, 0, 1, 0, 1), CHL_SLOPE(Fy, 1, 2, 1, -1, 1, -1));
+    const double sxp = up(flx(Fy, sy, 0), CHL_SLOPE(Fx, -1, 2, 1, -1, 0, 1), CHL_SLOPE(Fx, -1, 2, -1, -1, 0, -1));
+    const double sxn = up(flx(Fy, sx + sy, 0), CHL_SLOPE(Fx, -1, 2, 0, 1, 0, 1), CHL_SLOPE(Fx, -1, 2, 0, -1, 0, -1));
+    const double fxp = +flx(Fy, sy, 6), fxn = +flx(Fy, sx + sy, 6);
+    const double fyp = -flx(Fx, sx, 5), fyn = -flx(Fx, sx + sy, 5);
+    E[2 * n + id] = 0.25 * (+fxp + fxn + fyp + fyn + sxp + sxn + syp + syn);
+  }
+#undef CHL_SLOPE
+}
+
 // mhd::Calculate_CT_Electric_Fields for edges with every index in [lo, hi] (ct_electric_fields.cu:20-279).
 // Ec: the cell-centred terms from MhdCellEmfKernel over the cells [lo - 1, hi] (every cell an
 // edge slope of the region reads), or null to compute them per slope.
@@ -611,19 +664,16 @@ __global__ void __launch_bounds__(256) MhdEmfKernel(const TF *Fx, const TF *Fy, 
   if (dt_dev && *dt_dev == 0.0) return;
   const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
   if (x < lo || y < lo || z < lo || x > hix || y > hiy || z > hiz) return;
+  if (Ec) {
+    EmfEdgesFromCellTerms(Fx, Fy, Fz, Ec, E, long(id), long(n), nx, ny);
+    return;
+  }
   const long nF = 7 * long(n), nU = 8 * long(n);
   auto F = [&](const TF *a, int i, int j, int k, int f) {
     return AtF(a, long(i) + long(j) * nx + long(k) * long(nx) * ny + long(f) * long(n), nF);
   };
   auto S = [&](const TF *fl, double sgn, int ctd, int a, int b, int c, int d, int i, int j, int k) {
-    if (!Ec) return CtSlopeM(fl, U, sgn, ctd, a, b, c, d, i, j, k, nx, ny, long(n), nF, nU);
-    // CtSlopeM with its cell-centred term read back: ic and iflux as there
-    const long ic = long(i - int(a == 0) - int(b == 0)) + long(j - int(a == 1) - int(b == 1)) * nx +
-                    long(k - int(a == 2) - int(b == 2)) * long(nx) * ny;
-    const long iflux = long(i - int(c == 0) - int(d == 0)) + long(j - int(c == 1) - int(d == 1)) * nx +
-                       long(k - int(c == 2) - int(d == 2)) * long(nx) * ny;
-    const double ef = sgn * AtF(fl, iflux + (int(sgn == 1) + kBx) * long(n), nF);
-    return ef - Ec[ctd * long(n) + ic];
+    return CtSlopeM(fl, U, sgn, ctd, a, b, c, d, i, j, k, nx, ny, long(n), nF, nU);
   };
   auto pick = [](double s, double pos, double neg) {
     return s > 0.0 ? pos : (s < 0.0 ? neg : 0.5 * (pos + neg));
