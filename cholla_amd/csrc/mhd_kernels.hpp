@@ -836,6 +836,67 @@ __global__ void __launch_bounds__(256) MhdFlux2Kernel(MhdArgs a, int lo, int hix
   if (HlldM<P>(qL, qR, bx, g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[DIR]), id, n, DIR, f);
 }
 
+// Stage-2 fluxes along DIR as above, with each cell's PLMC computed once: a workgroup
+// takes 64 (x) by 4 (the slow axis: y, or z for DIR 2) faces of one plane of the third
+// axis, reconstructs the cells of those faces plus the next layer along DIR, keeps each
+// cell's face-(i+1/2) state in registers and passes its face-(i-1/2) state to the
+// neighbour through LDS.  The per-face kernel reconstructs every cell twice (once for each
+// of its faces); the arithmetic of each reconstruction and flux is the same.
+template <class P, int DIR>
+__global__ void __launch_bounds__(256) MhdFlux2TileKernel(MhdArgs a, int lo, int hix, int hiy, int hiz)
+{
+  using T          = typename P::T;
+  constexpr int TW = 64, TH = 4;                       // tile: x by the slow axis
+  constexpr int CW = TW + (DIR == 0), CH = TH + (DIR != 0);  // cells: one more layer along DIR
+  __shared__ PrimM<T> rm[CH][CW];                      // each cell's face-(i-1/2) state
+  if (MhdDt(a) == 0.0) return;
+  const int nx = a.nx, ny = a.ny;
+  const size_t n = size_t(nx) * ny * a.nz;
+  const int his = DIR == 2 ? hiz : hiy, hit = DIR == 2 ? hiy : hiz;  // slow and third axes' bounds
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+  const int x0 = lo + int(blockIdx.x) * TW, s0 = lo + int(blockIdx.y) * TH, t = lo + int(blockIdx.z);
+  if (t > hit) return;  // uniform per workgroup
+  const double *Uh = a.Uh;
+  auto cell_ok = [&](int cx, int cs) {  // a cell some face of the region reads
+    return cx <= hix + (DIR == 0) && cs <= his + (DIR != 0);
+  };
+  auto reconstruct = [&](int cx, int cs, PrimM<T> &Lp, PrimM<T> &Rm) {
+    const int y = DIR == 2 ? t : cs, z = DIR == 2 ? cs : t;
+    PlmcCellM<P>(Uh, cx, y, z, nx, ny, n, DIR, a.gamma, Lp, Rm);
+  };
+  PrimM<T> Lp, Rm;
+  const int fx = x0 + tx, fs = s0 + ty;
+  const bool mine = cell_ok(fx, fs);
+  if (mine) {
+    reconstruct(fx, fs, Lp, Rm);
+    rm[ty][tx] = Rm;
+  }
+  // the extra layer along DIR
+  if constexpr (DIR == 0) {
+    if (threadIdx.x < TH && cell_ok(x0 + TW, s0 + int(threadIdx.x))) {
+      PrimM<T> l2, r2;
+      reconstruct(x0 + TW, s0 + int(threadIdx.x), l2, r2);
+      rm[threadIdx.x][TW] = r2;
+    }
+  } else {
+    if (threadIdx.x < TW && cell_ok(x0 + int(threadIdx.x), s0 + TH)) {
+      PrimM<T> l2, r2;
+      reconstruct(x0 + int(threadIdx.x), s0 + TH, l2, r2);
+      rm[TH][threadIdx.x] = r2;
+    }
+  }
+  __syncthreads();
+  if (!mine || fx > hix || fs > his) return;
+  const int y = DIR == 2 ? t : fs, z = DIR == 2 ? fs : t;
+  const size_t id   = size_t(fx) + size_t(y) * nx + size_t(z) * size_t(nx) * ny;
+  const PrimM<T> Rn = DIR == 0 ? rm[ty][tx + 1] : rm[ty + 1][tx];
+  const T g         = T(a.gamma);
+  const T bx        = T(Uh[(kBx + DIR) * n + id]);
+  const ConsM<T> qL = WriteDataM<P>(Lp, bx, g), qR = WriteDataM<P>(Rn, bx, g);
+  ConsM<T> f;
+  if (HlldM<P>(qL, qR, bx, g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[DIR]), id, n, DIR, f);
+}
+
 // Full update (Update_Conserved_Variables_3D, hydro_cuda.cu:172-402, and
 // Update_Magnetic_Field_3D in place): hydro fields of the real cells with the density
 // floor and the temperature floor (Temperature_Floor_Kernel, hydro_cuda.cu:1117-1153);

@@ -29,12 +29,16 @@ hipError_t MhdStepT(const mhd::MhdArgs &a, hipStream_t s)
   hipLaunchKernelGGL((mhd::MhdEmfKernel<T>), dim3(g), dim3(256), 0, s, Fx, Fy, Fz, a.U, a.E, a.nx, a.ny, a.nz, 2,
                      a.nx - 1, a.ny - 1, a.nz - 1, a.dt_dev, a.Ec);
   hipLaunchKernelGGL((mhd::MhdHalfKernel<P>), dim3(g), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((mhd::MhdFlux2Kernel<P, 0>), dim3(g), dim3(256), 0, s, a, ng - 1, a.nx - ng, a.ny - ng,
-                     a.nz - ng);
-  hipLaunchKernelGGL((mhd::MhdFlux2Kernel<P, 1>), dim3(g), dim3(256), 0, s, a, ng - 1, a.nx - ng, a.ny - ng,
-                     a.nz - ng);
-  hipLaunchKernelGGL((mhd::MhdFlux2Kernel<P, 2>), dim3(g), dim3(256), 0, s, a, ng - 1, a.nx - ng, a.ny - ng,
-                     a.nz - ng);
+  {  // stage-2 faces with every index in [ng - 1, n - ng]
+    const int lo = ng - 1, hx = a.nx - ng, hy = a.ny - ng, hz = a.nz - ng;
+    const unsigned ex = unsigned(hx - lo + 1), ey = unsigned(hy - lo + 1), ez = unsigned(hz - lo + 1);
+    hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 0>), dim3((ex + 63) / 64, (ey + 3) / 4, ez), dim3(256), 0, s, a,
+                       lo, hx, hy, hz);
+    hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 1>), dim3((ex + 63) / 64, (ey + 3) / 4, ez), dim3(256), 0, s, a,
+                       lo, hx, hy, hz);
+    hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 2>), dim3((ex + 63) / 64, (ez + 3) / 4, ey), dim3(256), 0, s, a,
+                       lo, hx, hy, hz);
+  }
   hipLaunchKernelGGL(mhd::MhdCellEmfKernel<0>, dim3(g), dim3(256), 0, s, a.Uh, a.Ec, a.nx, a.ny, a.nz, ng - 1,
                      a.nx - ng, a.ny - ng, a.nz - ng, a.dt_dev);
   hipLaunchKernelGGL((mhd::MhdEmfKernel<T>), dim3(g), dim3(256), 0, s, Fx, Fy, Fz, a.Uh, a.E, a.nx, a.ny, a.nz, ng,
