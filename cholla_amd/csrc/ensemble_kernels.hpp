@@ -210,13 +210,15 @@ union SmemPair {
 
 // Persistent-launch dependencies.  Every phase of a box runs on one XCD (its queue's),
 // so what one tile of a box writes and another reads only has to cross that XCD's L2,
-// which all its CUs share, and each CU's vector L1, which is write-through.  The finishing
-// side: every lane waits until its stores have reached the L2, the workgroup meets, one
-// lane counts the tile.  The waiting side: one lane spins on the count (bounded: after
+// which all its CUs share.  The finishing side: every lane waits until its stores have
+// been acknowledged by the L2 (the vector L1 is write-through), the workgroup meets, one
+// lane counts the tile; a device-scope release would also write the XCD's whole L2 back to
+// HBM for every tile.  The waiting side: one lane spins on the count (bounded: after
 // spin_limit sleeps it sets *status and gives up, as does every later wait once one has),
-// the workgroup meets and drops its CU's L1 lines, so the box's data is read from the L2.
-// (Device-scope release/acquire would write back and invalidate the whole L2 of the XCD
-// for every tile.)
+// the workgroup meets and acquires at device scope, which drops the CU's L1 lines: the CU
+// may hold lines of the box read earlier in the same launch (a stage-1 tile reads the state
+// that the box's stage-2 tiles then overwrite).  A workgroup-scope L1 invalidate is not
+// enough: outside thread-group-split mode the hardware treats it as a no-op.
 __device__ __forceinline__ void SignalTileDone(int *count)
 {
   __builtin_amdgcn_s_waitcnt(0);
@@ -238,7 +240,7 @@ __device__ __forceinline__ void WaitTiles(const EnsembleArgs &e, int *count, int
     }
   }
   __syncthreads();
-  asm volatile("buffer_inv sc0" ::: "memory");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // XCD of the calling workgroup (the XCC_ID hardware register).
