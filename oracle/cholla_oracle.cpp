@@ -52,6 +52,9 @@ int g_de = 0;
 // speed), both interfaces of that cell in that sweep take the cell-centred values.
 int g_pos_fallback = 0;
 
+// The Riemann solver build option (-DHLLC or -DHLL) as a run-time switch: 0 HLLC, 1 HLL.
+int g_riemann = 0;
+
 inline size_t Idx(int i, int j, int k, int nx, int ny)
 {
   return size_t(i) + size_t(j) * size_t(nx) + size_t(k) * size_t(nx) * size_t(ny);
@@ -598,6 +601,82 @@ void orc_hllc(const double *QL, const double *QR, double *F, int nx, int ny, int
   }
 }
 
+// Calculate_HLL_Fluxes_CUDA, src/riemann_solvers/hll_cuda.cu:19-268 (no scalars; DE
+// branches :87-89, :100-112, :118-130, :176-178, :191-193, :206-208, :221-223, :235-237,
+// :253-255): Davis wave-speed estimates from the two sides' sound speeds, no Roe average.
+void orc_hll(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma, int dir)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  int o1, o2, o3;
+  DirOrder(dir, o1, o2, o3);
+#pragma omp parallel for schedule(static)
+  for (long t = 0; t < long(n_cells); t++) {
+    double dl = QL[t], mxl = QL[o1 * n_cells + t], myl = QL[o2 * n_cells + t], mzl = QL[o3 * n_cells + t],
+           El = QL[4 * n_cells + t];
+    double dr = QR[t], mxr = QR[o1 * n_cells + t], myr = QR[o2 * n_cells + t], mzr = QR[o3 * n_cells + t],
+           Er = QR[4 * n_cells + t];
+    double dgel = g_de ? QL[kGE * n_cells + t] : 0.0, dger = g_de ? QR[kGE * n_cells + t] : 0.0;
+
+    double vxl = mxl / dl, vyl = myl / dl, vzl = mzl / dl;
+    double pl;
+    if (g_de) {
+      double E_kin = 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl);
+      pl           = PressureFromDE(El, El - E_kin, dgel, gamma);
+    } else {
+      pl = (El - 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl)) * (gamma - 1.0);
+    }
+    pl         = std::fmax(pl, kTiny);
+    double vxr = mxr / dr, vyr = myr / dr, vzr = mzr / dr;
+    double pr;
+    if (g_de) {
+      double E_kin = 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr);
+      pr           = PressureFromDE(Er, Er - E_kin, dger, gamma);
+    } else {
+      pr = (Er - 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr)) * (gamma - 1.0);
+    }
+    pr = std::fmax(pr, kTiny);
+
+    double cfl = std::sqrt(gamma * pl / dl), cfr = std::sqrt(gamma * pr / dr);
+    double Sl = std::fmin(vxr - cfr, vxl - cfl);
+    double Sr = std::fmax(vxl + cfl, vxr + cfr);
+
+    double f_d_l = mxl, f_mx_l = mxl * vxl + pl, f_my_l = myl * vxl, f_mz_l = mzl * vxl, f_E_l = (El + pl) * vxl;
+    double f_d_r = mxr, f_mx_r = mxr * vxr + pr, f_my_r = myr * vxr, f_mz_r = mzr * vxr, f_E_r = (Er + pr) * vxr;
+    double f_ge_l = dgel * vxl, f_ge_r = dger * vxr;
+
+    double fd, fmx, fmy, fmz, fE, fge;
+    if (Sl > 0.0) {
+      fd = f_d_l, fmx = f_mx_l, fmy = f_my_l, fmz = f_mz_l, fE = f_E_l, fge = f_ge_l;
+    } else if (Sr < 0.0) {
+      fd = f_d_r, fmx = f_mx_r, fmy = f_my_r, fmz = f_mz_r, fE = f_E_r, fge = f_ge_r;
+    } else {
+      fd  = ((Sr * f_d_l) - (Sl * f_d_r) + Sl * Sr * (dr - dl)) / (Sr - Sl);
+      fmx = ((Sr * f_mx_l) - (Sl * f_mx_r) + Sl * Sr * (mxr - mxl)) / (Sr - Sl);
+      fmy = ((Sr * f_my_l) - (Sl * f_my_r) + Sl * Sr * (myr - myl)) / (Sr - Sl);
+      fmz = ((Sr * f_mz_l) - (Sl * f_mz_r) + Sl * Sr * (mzr - mzl)) / (Sr - Sl);
+      fE  = ((Sr * f_E_l) - (Sl * f_E_r) + Sl * Sr * (Er - El)) / (Sr - Sl);
+      fge = ((Sr * f_ge_l) - (Sl * f_ge_r) + Sl * Sr * (dger - dgel)) / (Sr - Sl);
+    }
+    F[t]               = fd;
+    F[o1 * n_cells + t] = fmx;
+    F[o2 * n_cells + t] = fmy;
+    F[o3 * n_cells + t] = fmz;
+    F[4 * n_cells + t]  = fE;
+    if (g_de) F[kGE * n_cells + t] = fge;
+  }
+}
+
+// The solver the build selects (g_riemann).
+static void RiemannFluxes(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
+                          int dir)
+{
+  if (g_riemann == 1) {
+    orc_hll(QL, QR, F, nx, ny, nz, gamma, dir);
+  } else {
+    orc_hllc(QL, QR, F, nx, ny, nz, gamma, dir);
+  }
+}
+
 // The DE pressure of a cell and the centred velocity difference terms of the pdV
 // source, as Update_Conserved_Variables_3D_half (VL_3D_cuda.cu:409-441) and
 // Partial_Update_Advected_Internal_Energy_3D (hydro_cuda.cu:779-810) compute them:
@@ -1031,9 +1110,9 @@ void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double 
   s.Resize(n_cells * n_fields);
   orc_pcm_3d(U, s.QLx.data(), s.QRx.data(), s.QLy.data(), s.QRy.data(), s.QLz.data(), s.QRz.data(), nx, ny, nz,
              n_fields);
-  orc_hllc(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, nz, gamma, 0);
-  orc_hllc(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, nz, gamma, 1);
-  orc_hllc(s.QLz.data(), s.QRz.data(), s.Fz.data(), nx, ny, nz, gamma, 2);
+  RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, nz, gamma, 0);
+  RiemannFluxes(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, nz, gamma, 1);
+  RiemannFluxes(s.QLz.data(), s.QRz.data(), s.Fz.data(), nx, ny, nz, gamma, 2);
   orc_update_half_3d(U, s.Uh.data(), s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, dx, dy, dz, 0.5 * dt,
                      gamma, n_fields, density_floor_on, density_floor);
   if (recon == ORC_PLMC) {
@@ -1048,9 +1127,9 @@ void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double 
     orc_pcm_3d(s.Uh.data(), s.QLx.data(), s.QRx.data(), s.QLy.data(), s.QRy.data(), s.QLz.data(), s.QRz.data(), nx,
                ny, nz, n_fields);
   }
-  orc_hllc(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, nz, gamma, 0);
-  orc_hllc(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, nz, gamma, 1);
-  orc_hllc(s.QLz.data(), s.QRz.data(), s.Fz.data(), nx, ny, nz, gamma, 2);
+  RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, nz, gamma, 0);
+  RiemannFluxes(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, nz, gamma, 1);
+  RiemannFluxes(s.QLz.data(), s.QRz.data(), s.Fz.data(), nx, ny, nz, gamma, 2);
   if (g_de) orc_partial_update_ge_3d(U, nx, ny, nz, n_ghost, dx, dy, dz, dt, gamma);  // VL_3D_cuda.cu:314-324
   orc_update_3d(U, s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, n_ghost, dx, dy, dz, dt, n_fields,
                 density_floor_on, density_floor);
@@ -1067,7 +1146,7 @@ void orc_vl1d(double *U, int nx, int n_ghost, double dx, double dt, double gamma
   Scratch &s     = g_scratch;
   s.Resize(n_cells * n_fields);
   orc_pcm_1d(U, s.QLx.data(), s.QRx.data(), nx, n_fields);
-  orc_hllc(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, 1, 1, gamma, 0);
+  RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, 1, 1, gamma, 0);
   orc_update_half_1d(U, s.Uh.data(), s.Fx.data(), nx, dx, 0.5 * dt, n_fields);
   if (recon == ORC_PLMC) {
     orc_plmc(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, 1, 1, gamma, 0);
@@ -1076,7 +1155,7 @@ void orc_vl1d(double *U, int nx, int n_ghost, double dx, double dt, double gamma
   } else {
     orc_pcm_1d(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, n_fields);
   }
-  orc_hllc(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, 1, 1, gamma, 0);
+  RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, 1, 1, gamma, 0);
   orc_update_1d(U, s.Fx.data(), nx, n_ghost, dx, dt, n_fields);
 }
 
@@ -1119,6 +1198,7 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
 // The DE build option (dual energy) as a run-time switch for every stage above.
 void orc_set_dual_energy(int on) { g_de = on ? 1 : 0; }
 void orc_set_positivity_fallback(int on) { g_pos_fallback = on ? 1 : 0; }
+void orc_set_riemann(int solver) { g_riemann = solver == 1 ? 1 : 0; }
 int orc_get_dual_energy() { return g_de; }
 
 // hydro_utilities::Get_Pressure_From_DE, for the known-answer tests

@@ -58,6 +58,8 @@ def lib():
             "orc_set_dual_energy": (None, [i]),
             "orc_get_dual_energy": (i, []),
             "orc_set_positivity_fallback": (None, [i]),
+            "orc_set_riemann": (None, [i]),
+            "orc_hll": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_pressure_from_de": (d, [d, d, d, d]),
             "orc_update_3d": (None, [_dp] * 4 + [i, i, i, i, d, d, d, d, i, i, d]),
             "orc_temperature_floor": (None, [_dp, i, i, i, i, d]),
@@ -138,6 +140,19 @@ def positivity_fallback(on=True):
         lib().orc_set_positivity_fallback(0)
 
 
+HLLC, HLL = 0, 1
+
+
+@contextlib.contextmanager
+def riemann(solver=HLLC):
+    """The Riemann solver build option (HLLC or HLL) for the calls inside the block."""
+    lib().orc_set_riemann(int(solver))
+    try:
+        yield
+    finally:
+        lib().orc_set_riemann(HLLC)
+
+
 @contextlib.contextmanager
 def dual_energy(on=True):
     """The reference's DE build option for the calls inside the block (GasEnergy = field 5)."""
@@ -155,7 +170,7 @@ def pressure_from_de(E, U_total, U_advected, gamma):
 
 def run(U, nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, recon, flags, t, tout, max_steps,
         density_floor_on=True, density_floor=0.0, temperature_floor_on=True, U_floor=0.0, dt_log=None,
-        dual_energy_on=False, positivity_fallback_on=False):
+        dual_energy_on=False, positivity_fallback_on=False, solver=HLLC):
     """main.cpp's hydro loop; U (with ghost cells already set) is advanced in place.
     dual_energy_on: the DE build (3D only, n_fields 6 with GasEnergy last).
 
@@ -163,7 +178,7 @@ def run(U, nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, recon, flags, t, to
     f, fp = iptr(flags)
     tt = ctypes.c_double(t)
     log = dt_log if dt_log is not None else None
-    with dual_energy(dual_energy_on), positivity_fallback(positivity_fallback_on):
+    with dual_energy(dual_energy_on), positivity_fallback(positivity_fallback_on), riemann(solver):
         n = lib().orc_run(ptr(U), nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, recon, fp,
                           ctypes.byref(tt), tout, max_steps, int(density_floor_on), density_floor,
                           int(temperature_floor_on), U_floor, ptr(log) if log is not None else None)
