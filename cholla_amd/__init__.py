@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libcholla_amd.so")
 CHL_OK = 0
 RECON_PCM, RECON_PLMC, RECON_PPMC = 0, 1, 2
 RIEMANN_HLLC = 0
+RIEMANN_HLL = 1
 PREC_FP64_EXACT, PREC_FP64, PREC_MIXED = 0, 1, 2
 PRECISIONS = {"fp64_exact": PREC_FP64_EXACT, "fp64": PREC_FP64, "mixed": PREC_MIXED}
 BC_PERIODIC, BC_REFLECTIVE, BC_TRANSMISSIVE = 1, 2, 3
@@ -28,7 +29,7 @@ EXPORTS = [
     "chl_hydro_destroy", "chl_hydro_vl_step", "chl_hydro_update_grid", "chl_hydro_run", "chl_hydro_time_steps",
     "chl_hydro_kernel_times", "chl_hydro_autotune", "chl_hydro_get_config", "chl_calc_dt",
     "chl_apply_temperature_floor", "chl_set_ghost_cells", "chl_set_boundaries", "chl_reconstruct",
-    "chl_hllc_fluxes", "chl_ic_kelvin_helmholtz", "chl_ensemble_create", "chl_ensemble_destroy",
+    "chl_hllc_fluxes", "chl_hll_fluxes", "chl_ic_kelvin_helmholtz", "chl_ensemble_create", "chl_ensemble_destroy",
     "chl_ensemble_get_config", "chl_ensemble_persistent_pairs", "chl_ensemble_bind", "chl_ensemble_start",
     "chl_ensemble_advance", "chl_ensemble_advance_looped", "chl_ensemble_clocks", "chl_ensemble_autotune",
     "chl_ensemble_layout", "chl_turbulence_modes", "chl_ic_turbulence", "chl_init_gas_energy", "chl_mhd_create",
@@ -119,6 +120,7 @@ def lib():
         "chl_set_boundaries": (_i, [_vp, _i, _i, _i, _i, _i, _ip]),
         "chl_reconstruct": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_hllc_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
+        "chl_hll_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_ic_kelvin_helmholtz": (_i, [_dp, _i, _i, _d, ctypes.c_uint64]),
         "chl_ensemble_create": (_i, [ctypes.POINTER(EnsembleConfig), _i, _ip, ctypes.POINTER(_vp)]),
         "chl_ensemble_destroy": (_i, [_vp]),
@@ -222,13 +224,14 @@ class Hydro:
 
     def __init__(self, nx, ny, nz, n_ghost, reconstruction=RECON_PPMC, precision=PREC_FP64_EXACT,
                  bc=(1, 1, 1, 1, 1, 1), density_floor=True, temperature_floor=True, tile1=-1, tile2=-1,
-                 kchunk1=0, kchunk2=0, tile_order=0, dual_energy=False, positivity_fallback=False):
+                 kchunk1=0, kchunk2=0, tile_order=0, dual_energy=False, positivity_fallback=False,
+                 riemann=RIEMANN_HLLC):
         cfg = HydroConfig()
         cfg.nx, cfg.ny, cfg.nz, cfg.n_ghost = nx, ny, nz, n_ghost
         cfg.n_fields = 6 if dual_energy else 5
         cfg.dual_energy = int(dual_energy)
         cfg.positivity_fallback = int(positivity_fallback)
-        cfg.reconstruction, cfg.riemann, cfg.precision = reconstruction, RIEMANN_HLLC, precision
+        cfg.reconstruction, cfg.riemann, cfg.precision = reconstruction, riemann, precision
         cfg.density_floor, cfg.temperature_floor = int(density_floor), int(temperature_floor)
         for q in range(6):
             cfg.bc_flags[q] = bc[q]
@@ -319,6 +322,10 @@ def reconstruct(recon, U, QL, QR, nx, ny, nz, gamma, direction):
 
 def hllc_fluxes(QL, QR, F, nx, ny, nz, gamma, direction):
     check(lib().chl_hllc_fluxes(QL.ptr, QR.ptr, F.ptr, nx, ny, nz, gamma, direction))
+
+
+def hll_fluxes(QL, QR, F, nx, ny, nz, gamma, direction):
+    check(lib().chl_hll_fluxes(QL.ptr, QR.ptr, F.ptr, nx, ny, nz, gamma, direction))
 
 
 def ic_kelvin_helmholtz(n, ng, gamma=5.0 / 3.0, seed=2024):

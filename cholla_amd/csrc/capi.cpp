@@ -74,7 +74,10 @@ int CheckConfig(const chl_hydro_config &c)
   if (c.dual_energy != 0 && c.dual_energy != 1) return Fail(CHL_ERR_INVALID, "dual_energy must be 0 or 1");
   if (c.n_fields != 5 + c.dual_energy)
     return Fail(CHL_ERR_UNSUPPORTED, "n_fields must be 5, or 6 with dual_energy (no passive scalars)");
-  if (c.riemann != CHL_RIEMANN_HLLC) return Fail(CHL_ERR_UNSUPPORTED, "only the HLLC solver is built");
+  if (c.riemann != CHL_RIEMANN_HLLC && c.riemann != CHL_RIEMANN_HLL)
+    return Fail(CHL_ERR_UNSUPPORTED, "riemann must be HLLC or HLL");
+  if (c.riemann == CHL_RIEMANN_HLL && c.dual_energy)
+    return Fail(CHL_ERR_UNSUPPORTED, "the HLL solver is built without dual energy");
   if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC)
     return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PLMC or PPMC");
   if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
@@ -91,6 +94,8 @@ int CheckConfig(const chl_hydro_config &c)
     if (c.tile1 >= chl::kNumTiles || c.tile2 >= chl::kNumTiles) return Fail(CHL_ERR_INVALID, "tile index out of range");
     if (c.dual_energy && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
       return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for tile shapes 0 and 1 only");
+    if (c.riemann == CHL_RIEMANN_HLL && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
+      return Fail(CHL_ERR_UNSUPPORTED, "the HLL solver is built for tile shapes 0 and 1 only");
     if (c.tile_order != 0 && c.tile_order != 1) return Fail(CHL_ERR_INVALID, "tile_order must be 0 or 1");
   } else {
     if (c.precision != CHL_PREC_FP64_EXACT)
@@ -119,6 +124,7 @@ chl::StepArgs MakeStepArgs(chl_hydro *h, double *U, double dx, double dy, double
   a.flags     = flags;
   a.xcd_order = c.tile_order == 0;
   a.de        = c.dual_energy;
+  a.riemann   = c.riemann == CHL_RIEMANN_HLL ? chl::kRiemannHll : chl::kRiemannHllc;
   return a;
 }
 
@@ -164,7 +170,8 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
   if (h->one_d) {
     const int n = c.nx;
     bool fuse_dt = update_grid && n <= 1024;
-    CHL_HIP(chl::LaunchVl1d(U, reinterpret_cast<double *>(h->W), n, ng, c.reconstruction, dx, dt_dev, dt_host, gamma,
+    CHL_HIP(chl::LaunchVl1d(U, reinterpret_cast<double *>(h->W), n, ng, c.reconstruction,
+                            c.riemann == CHL_RIEMANN_HLL ? chl::kRiemannHll : chl::kRiemannHllc, dx, dt_dev, dt_host, gamma,
                             fuse_dt ? h->dti() : nullptr, h->stream));
     if (update_grid && !fuse_dt) {
       CHL_HIP(hipMemsetAsync(h->dti(), 0, 8, h->stream));
@@ -316,7 +323,7 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return Fail(CHL_ERR_NO_DEVICE, "no HIP device");
   chl_hydro_config c = *cfg;
   if (c.tile1 < 0) c.tile1 = 0;
-  if (c.tile2 < 0) c.tile2 = c.dual_energy ? 0 : 3;
+  if (c.tile2 < 0) c.tile2 = (c.dual_energy || c.riemann == CHL_RIEMANN_HLL) ? 0 : 3;
   CHL_TRY(CheckConfig(c));
   auto *h    = new chl_hydro();
   h->one_d   = c.ny == 1 && c.nz == 1;
@@ -519,7 +526,7 @@ int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, doub
                                      (c.positivity_fallback ? chl::kFlagPositivityFallback : 0));
   const int R1     = ReconRadius(c.reconstruction) + 1;
   const int nz1    = c.nz - 2 * c.n_ghost + 2 * R1, nz2 = c.nz - 2 * c.n_ghost;
-  const int ntiles = c.dual_energy ? chl::kNumDeTiles : chl::kNumTiles;
+  const int ntiles = (c.dual_energy || c.riemann == CHL_RIEMANN_HLL) ? chl::kNumDeTiles : chl::kNumTiles;
   std::vector<int> chunks = {8, 16, 32, 64, 128};
   float best1 = 1e30f, best2 = 1e30f;
   int b_t1 = c.tile1, b_t2 = c.tile2, b_k1 = c.kchunk1, b_k2 = c.kchunk2;
@@ -695,7 +702,15 @@ int chl_reconstruct(int recon, const double *U, double *QL, double *QR, int nx, 
 int chl_hllc_fluxes(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma, int dir)
 {
   if (!QL || !QR || !F || dir < 0 || dir > 2) return Fail(CHL_ERR_INVALID, "bad argument");
-  CHL_HIP(chl::LaunchHllcGrid(QL, QR, F, nx, ny, nz, gamma, dir, nullptr));
+  CHL_HIP(chl::LaunchRiemannGrid(chl::kRiemannHllc, QL, QR, F, nx, ny, nz, gamma, dir, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_hll_fluxes(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma, int dir)
+{
+  if (!QL || !QR || !F || dir < 0 || dir > 2) return Fail(CHL_ERR_INVALID, "bad argument");
+  CHL_HIP(chl::LaunchRiemannGrid(chl::kRiemannHll, QL, QR, F, nx, ny, nz, gamma, dir, nullptr));
   CHL_HIP(hipDeviceSynchronize());
   return CHL_OK;
 }

@@ -21,6 +21,11 @@ constexpr int kRecon = CHL_RECON_PPMC;
 #else
 constexpr int kRecon = CHL_RECON_PLMC;  // builds/make.type.hydro:5 default
 #endif
+#if defined(HLL)
+constexpr int kRiemann = CHL_RIEMANN_HLL;  // -DHLL in place of -DHLLC
+#else
+constexpr int kRiemann = CHL_RIEMANN_HLLC;
+#endif
 #ifdef DENSITY_FLOOR
 constexpr int kDensityFloor = 1;
 #else
@@ -73,7 +78,7 @@ chl_hydro *Acquire(Slot &s, int nx, int ny, int nz, int ng, int nf)
   if (s.h) Check(chl_hydro_destroy(s.h), "chl_hydro_destroy");
   chl_hydro_config c{};
   c.nx = nx, c.ny = ny, c.nz = nz, c.n_ghost = ng, c.n_fields = nf;
-  c.reconstruction = kRecon, c.riemann = CHL_RIEMANN_HLLC, c.precision = CHL_PRECISION;
+  c.reconstruction = kRecon, c.riemann = kRiemann, c.precision = CHL_PRECISION;
   c.density_floor = kDensityFloor, c.temperature_floor = 0;  // the floor is its own call here
   for (int q = 0; q < 6; q++) c.bc_flags[q] = CHL_BC_PERIODIC;  // unused: VL never fills ghosts
   c.tile1 = -1, c.tile2 = -1;

@@ -32,20 +32,32 @@ constexpr int kGasEnergy = 5;
 constexpr double kDeEta1 = 0.001;
 constexpr double kDeEta2 = 0.035;
 
+// Riemann solvers (the reference's -DHLLC / -DHLL build options).
+constexpr int kRiemannHllc = 0;
+constexpr int kRiemannHll  = 1;
+
 struct PolicyExact {
   using T                      = double;
   static constexpr bool kExact = true;
   static constexpr int kId     = 0;
+  static constexpr int kSolver = kRiemannHllc;
 };
 struct PolicyFast {
   using T                      = double;
   static constexpr bool kExact = false;
   static constexpr int kId     = 1;
+  static constexpr int kSolver = kRiemannHllc;
 };
 struct PolicyMixed {
   using T                      = float;
   static constexpr bool kExact = false;
   static constexpr int kId     = 2;
+  static constexpr int kSolver = kRiemannHllc;
+};
+// An arithmetic form with the HLL solver in place of HLLC; everything else as Base.
+template <class Base>
+struct HllPolicy : Base {
+  static constexpr int kSolver = kRiemannHll;
 };
 
 template <typename T>
@@ -643,6 +655,60 @@ __device__ __forceinline__ Cons<typename P::T> Hllc(const RState<typename P::T> 
     *fge   = T(0.5) * (f_ge_l + f_ge_r + (Sr - aSm) * gers + (Sl + aSm) * gels - Sl * l.dge - Sr * r.dge);
   }
   return f;
+}
+
+// Calculate_HLL_Fluxes_CUDA body, src/riemann_solvers/hll_cuda.cu:141-266: Davis wave
+// speeds from the two sides' sound speeds (no Roe average), the upwind physical flux for a
+// supersonic face, else the HLL average.  With dual energy the GasEnergy flux goes to *fge.
+template <class P, bool DE = false>
+__device__ __forceinline__ Cons<typename P::T> Hll(const RState<typename P::T> &l, const RState<typename P::T> &r,
+                                                   typename P::T gamma, typename P::T *fge = nullptr)
+{
+  using T = typename P::T;
+  T cfl   = Sqrt<P>(Div<P>(gamma * l.p, l.d, l.inv_d));
+  T cfr   = Sqrt<P>(Div<P>(gamma * r.p, r.d, r.inv_d));
+  if constexpr (P::kId == PolicyMixed::kId) {
+    // as in Hllc: keep the speeds apart where FP32 cannot resolve a floored sound speed
+    cfl = Fmax(cfl, T(0x1p-20) * Fabs(l.vx));
+    cfr = Fmax(cfr, T(0x1p-20) * Fabs(r.vx));
+  }
+  const T Sl = Fmin(r.vx - cfr, l.vx - cfl);
+  const T Sr = Fmax(l.vx + cfl, r.vx + cfr);
+  Cons<T> fl{l.mx, l.mx * l.vx + l.p, l.my * l.vx, l.mz * l.vx, (l.E + l.p) * l.vx};
+  Cons<T> fr{r.mx, r.mx * r.vx + r.p, r.my * r.vx, r.mz * r.vx, (r.E + r.p) * r.vx};
+  T f_ge_l = T(0), f_ge_r = T(0);
+  if constexpr (DE) f_ge_l = l.dge * l.vx, f_ge_r = r.dge * r.vx;
+  if (Sl > T(0.0)) {
+    if constexpr (DE) *fge = f_ge_l;
+    return fl;
+  }
+  if (Sr < T(0.0)) {
+    if constexpr (DE) *fge = f_ge_r;
+    return fr;
+  }
+  const T den = Sr - Sl, inv = Rcp<P>(den);
+  auto avg    = [&](T a_l, T a_r, T u_l, T u_r) { return Div<P>((Sr * a_l) - (Sl * a_r) + Sl * Sr * (u_r - u_l), den, inv); };
+  Cons<T> f;
+  f.d  = avg(fl.d, fr.d, l.d, r.d);
+  f.mx = avg(fl.mx, fr.mx, l.mx, r.mx);
+  f.my = avg(fl.my, fr.my, l.my, r.my);
+  f.mz = avg(fl.mz, fr.mz, l.mz, r.mz);
+  f.E  = avg(fl.E, fr.E, l.E, r.E);
+  if constexpr (DE) *fge = avg(f_ge_l, f_ge_r, l.dge, r.dge);
+  return f;
+}
+
+// The solver of the form P (HLLC unless P is an HllPolicy).
+template <class P, bool DE = false>
+__device__ __forceinline__ Cons<typename P::T> RiemannFlux(const RState<typename P::T> &l,
+                                                           const RState<typename P::T> &r, typename P::T gamma,
+                                                           typename P::T *fge = nullptr)
+{
+  if constexpr (P::kSolver == kRiemannHll) {
+    return Hll<P, DE>(l, r, gamma, fge);
+  } else {
+    return Hllc<P, DE>(l, r, gamma, fge);
+  }
 }
 
 // ---------------------------------------------------------------------------- dt

@@ -372,9 +372,10 @@ __global__ void TurbulenceIcKernel(double *U, int n, int ng, int n_modes, const 
 }
 
 // ============================================================================ VL 1D
-// Stage 1 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:64-91): PCM states, HLLC on
-// faces id-1/2 and id+1/2 of every cell 1..n-2, the dt/2 update, then the half
-// state's primitives.
+// Stage 1 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:64-91): PCM states, the Riemann
+// solve of P (HLLC, or HLL under HllPolicy) on faces id-1/2 and id+1/2 of every
+// cell 1..n-2, the dt/2 update, then the half state's primitives.
+template <class P>
 __global__ void Vl1dStage1Kernel(const double *U, double *W, int n, double dx, const double *dt_dev, double dt_host,
                                  double gamma, unsigned long long *dti_reset)
 {
@@ -387,8 +388,8 @@ __global__ void Vl1dStage1Kernel(const double *U, double *W, int n, double dx, c
     return RStateFromCons<PolicyExact>(U[c], U[n + c], U[2 * n + c], U[3 * n + c], U[4 * n + c], gamma);
   };
   RState<double> a = side(id - 1), b = side(id), cc = side(id + 1);
-  Cons<double> fm = Hllc<PolicyExact>(a, b, gamma);
-  Cons<double> fp = Hllc<PolicyExact>(b, cc, gamma);
+  Cons<double> fm = RiemannFlux<P>(a, b, gamma);
+  Cons<double> fp = RiemannFlux<P>(b, cc, gamma);
   double uh[5];
   uh[0]          = U[id] + dtodx * (fm.d - fp.d);
   uh[1]          = U[n + id] + dtodx * (fm.mx - fp.mx);
@@ -423,7 +424,7 @@ __device__ __forceinline__ void Recon1D(const double *W, int n, int c, double ga
 
 // Stage 2 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:95-137) for the real cells, with the
 // next inverse dt (Calc_dt_1D) reduced in the epilogue.
-template <int RK>
+template <class P, int RK>
 __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, double dx, const double *dt_dev,
                                  double dt_host, double gamma, unsigned long long *dti)
 {
@@ -436,10 +437,10 @@ __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, doub
     Recon1D<RK>(W, n, id - 1, gamma, Lm, Rm);
     Recon1D<RK>(W, n, id, gamma, L0, R0);
     Recon1D<RK>(W, n, id + 1, gamma, Lp, Rp);
-    Cons<double> fm = Hllc<PolicyExact>(RStateFromPrim<PolicyExact>(Lm, gamma, inv_gm1),
-                                        RStateFromPrim<PolicyExact>(R0, gamma, inv_gm1), gamma);
-    Cons<double> fp = Hllc<PolicyExact>(RStateFromPrim<PolicyExact>(L0, gamma, inv_gm1),
-                                        RStateFromPrim<PolicyExact>(Rp, gamma, inv_gm1), gamma);
+    Cons<double> fm = RiemannFlux<P>(RStateFromPrim<PolicyExact>(Lm, gamma, inv_gm1),
+                                     RStateFromPrim<PolicyExact>(R0, gamma, inv_gm1), gamma);
+    Cons<double> fp = RiemannFlux<P>(RStateFromPrim<PolicyExact>(L0, gamma, inv_gm1),
+                                     RStateFromPrim<PolicyExact>(Rp, gamma, inv_gm1), gamma);
     const double dtodx = dt / dx;
     double un[5];
     un[0] = U[id] + dtodx * (fm.d - fp.d);
@@ -450,9 +451,9 @@ __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, doub
     for (int q = 0; q < 5; q++) U[q * n + id] = un[q];
     double d = un[0], dinv = 1.0 / d;
     double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
-    double P  = (un[4] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
-    P         = Fmax(P, kTinyNumber);
-    double cs = __builtin_sqrt(dinv * gamma * P);
+    double pr = (un[4] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+    pr        = Fmax(pr, kTinyNumber);
+    double cs = __builtin_sqrt(dinv * gamma * pr);
     m         = (Fabs(vx) + cs) / dx;
   }
   if (dti) BlockMaxToGlobal(m, dti);
@@ -518,8 +519,10 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
   write(R, QR, Lin(xid - ix, yid - iy, zid - iz, nx, ny));
 }
 
-// Calculate_HLLC_Fluxes_CUDA over a whole grid (hllc_cuda.cu:19-307)
-__global__ void HllcGridKernel(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
+// Calculate_HLLC_Fluxes_CUDA over a whole grid (hllc_cuda.cu:19-307), or, under
+// HllPolicy, Calculate_HLL_Fluxes_CUDA (hll_cuda.cu:19-268)
+template <class P>
+__global__ void RiemannGridKernel(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
                                int dir)
 {
   const size_t nc = size_t(nx) * ny * nz;
@@ -531,7 +534,7 @@ __global__ void HllcGridKernel(const double *QL, const double *QR, double *F, in
                                                  QL[4 * nc + t], gamma);
   RState<double> r = RStateFromCons<PolicyExact>(QR[t], QR[o1 * nc + t], QR[o2 * nc + t], QR[o3 * nc + t],
                                                  QR[4 * nc + t], gamma);
-  Cons<double> f   = Hllc<PolicyExact>(l, r, gamma);
+  Cons<double> f   = RiemannFlux<P>(l, r, gamma);
   F[t]             = f.d;
   F[o1 * nc + t]   = f.mx;
   F[o2 * nc + t]   = f.my;
@@ -645,22 +648,32 @@ hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const doubl
   return hipGetLastError();
 }
 
-hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, double dx, const double *dt_dev, double dt,
-                      double gamma, unsigned long long *dti, hipStream_t s)
+template <class P>
+static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, double dx, const double *dt_dev,
+                              double dt, double gamma, unsigned long long *dti, hipStream_t s)
 {
-  hipLaunchKernelGGL(Vl1dStage1Kernel, dim3(Blocks(n, 256)), dim3(256), 0, s, U, W, n, dx, dt_dev, dt, gamma, dti);
+  hipLaunchKernelGGL(Vl1dStage1Kernel<P>, dim3(Blocks(n, 256)), dim3(256), 0, s, U, W, n, dx, dt_dev, dt, gamma, dti);
   // one block when a dt reduction is fused so the block max is the grid max
   unsigned nb = dti ? 1u : Blocks(n, 256);
   int tpb     = dti ? 1024 : 256;
-  if (dti && n > 1024) return hipErrorInvalidValue;
   if (recon == kPPMC) {
-    hipLaunchKernelGGL(Vl1dStage2Kernel<kPPMC>, dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma, dti);
-  } else if (recon == kPLMC) {
-    hipLaunchKernelGGL(Vl1dStage2Kernel<kPLMC>, dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma, dti);
+    hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMC>), dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma,
+                       dti);
   } else {
-    return hipErrorInvalidValue;
+    hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMC>), dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma,
+                       dti);
   }
   return hipGetLastError();
+}
+
+hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, double dx, const double *dt_dev,
+                      double dt, double gamma, unsigned long long *dti, hipStream_t s)
+{
+  if (dti && n > 1024) return hipErrorInvalidValue;
+  if (recon != kPPMC && recon != kPLMC) return hipErrorInvalidValue;
+  if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+  if (riemann == kRiemannHllc) return LaunchVl1dP<PolicyExact>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t LaunchReconGrid(int recon, const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma,
@@ -679,11 +692,19 @@ hipError_t LaunchReconGrid(int recon, const double *U, double *QL, double *QR, i
   return hipGetLastError();
 }
 
-hipError_t LaunchHllcGrid(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
-                          int dir, hipStream_t s)
+hipError_t LaunchRiemannGrid(int riemann, const double *QL, const double *QR, double *F, int nx, int ny, int nz,
+                             double gamma, int dir, hipStream_t s)
 {
   long nc = long(nx) * ny * nz;
-  hipLaunchKernelGGL(HllcGridKernel, dim3(Blocks(nc, 64)), dim3(64), 0, s, QL, QR, F, nx, ny, nz, gamma, dir);
+  if (riemann == kRiemannHll) {
+    hipLaunchKernelGGL(RiemannGridKernel<HllPolicy<PolicyExact>>, dim3(Blocks(nc, 64)), dim3(64), 0, s, QL, QR, F, nx,
+                       ny, nz, gamma, dir);
+  } else if (riemann == kRiemannHllc) {
+    hipLaunchKernelGGL(RiemannGridKernel<PolicyExact>, dim3(Blocks(nc, 64)), dim3(64), 0, s, QL, QR, F, nx, ny, nz,
+                       gamma, dir);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -63,6 +63,7 @@ struct StepArgs {
   int gx, gy, gz;          // tile grid of the launch (set by the launcher)
   int xcd_order;           // 1: give each XCD a contiguous run of tiles (see TileOf)
   int de;                  // dual energy (host-side dispatch only; the kernels are templated on it)
+  int riemann;             // kRiemannHllc / kRiemannHll (host-side dispatch only, as de)
 };
 
 enum : int {
@@ -207,15 +208,15 @@ __device__ __forceinline__ void FluxGlobal(const Cons<typename P::T> &f, typenam
   out[4] = f.E;
 }
 
-// HLLC on one face; the flux in global component order into out[0..4] and, with
-// dual energy, the GasEnergy flux into out[5].
+// The Riemann solve (HLLC, or HLL under an HllPolicy) on one face; the flux in global
+// component order into out[0..4] and, with dual energy, the GasEnergy flux into out[5].
 template <class P, int DIR, bool DE>
 __device__ __forceinline__ void FaceFlux(const RState<typename P::T> &l, const RState<typename P::T> &r,
                                          typename P::T gamma, typename P::T *out)
 {
   using T = typename P::T;
   T fge   = T(0);
-  Cons<T> f = Hllc<P, DE>(l, r, gamma, &fge);
+  Cons<T> f = RiemannFlux<P, DE>(l, r, gamma, &fge);
   FluxGlobal<P, DIR>(f, out);
   if constexpr (DE) out[kGasEnergy] = fge;
 }

@@ -47,12 +47,18 @@ hipError_t LaunchStage2T(const StepArgs &a, hipStream_t s)
   return hipGetLastError();
 }
 
-// Dual-energy kernels are compiled for the first kNumDeTiles tile shapes only.
+// Dual-energy and HLL kernels are compiled for the first kNumDeTiles tile shapes only.
 constexpr int kNumDeTiles = 2;
 
 template <class P>
 hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 {
+  if (a.riemann == kRiemannHll) {  // HLL is compiled for the first two tile shapes, without DE
+    if (a.de) return hipErrorInvalidValue;
+    if (tile == 0) return LaunchStage1T<HllPolicy<P>, 32, 8>(a, R1, s);
+    if (tile == 1) return LaunchStage1T<HllPolicy<P>, 16, 16>(a, R1, s);
+    return hipErrorInvalidValue;
+  }
   if (a.de) {
     switch (tile) {
       case 0:
@@ -80,6 +86,12 @@ hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 template <class P, int RK>
 hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
 {
+  if (a.riemann == kRiemannHll) {
+    if (a.de) return hipErrorInvalidValue;
+    if (tile == 0) return LaunchStage2T<HllPolicy<P>, RK, 32, 8>(a, s);
+    if (tile == 1) return LaunchStage2T<HllPolicy<P>, RK, 16, 16>(a, s);
+    return hipErrorInvalidValue;
+  }
   if (a.de) {
     switch (tile) {
       case 0:

@@ -35,7 +35,8 @@ extern "C" {
 #define CHL_RECON_PCM 0  /* PCM  */
 #define CHL_RECON_PLMC 1 /* PLMC (builds/make.type.hydro:5) */
 #define CHL_RECON_PPMC 2 /* PPMC */
-#define CHL_RIEMANN_HLLC 0
+#define CHL_RIEMANN_HLLC 0 /* HLLC (builds/make.type.hydro) */
+#define CHL_RIEMANN_HLL 1  /* HLL, Davis wave speeds (-DHLL, src/riemann_solvers/hll_cuda.cu) */
 /* arithmetic forms of the same step */
 #define CHL_PREC_FP64_EXACT 0 /* FP64, reference operation order, no FMA contraction */
 #define CHL_PREC_FP64 1       /* FP64, shared reciprocals + FMA */
@@ -51,7 +52,7 @@ typedef struct chl_hydro_config {
   int n_ghost;           /* Header::n_ghost: PLMC 3, PPMC 4 (src/grid/grid3D.cpp:53-76) */
   int n_fields;          /* 5 (hydro), or 6 with dual_energy (GasEnergy last); no scalars */
   int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL */
-  int riemann;           /* CHL_RIEMANN_HLLC */
+  int riemann;           /* CHL_RIEMANN_HLLC, or CHL_RIEMANN_HLL (no dual energy; tile shapes 0, 1) */
   int precision;         /* CHL_PREC_* */
   int density_floor;     /* DENSITY_FLOOR build option, 0/1 */
   int temperature_floor; /* TEMPERATURE_FLOOR build option, 0/1 */
@@ -149,6 +150,9 @@ int chl_reconstruct(int reconstruction, const double *dev_conserved, double *dev
 /* Calculate_HLLC_Fluxes_CUDA (hllc_cuda.cu:19) on device arrays */
 int chl_hllc_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, double *dev_flux, int nx, int ny, int nz,
                     double gamma, int dir);
+/* Calculate_HLL_Fluxes_CUDA (hll_cuda.cu:19) on device arrays */
+int chl_hll_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, double *dev_flux, int nx, int ny, int nz,
+                   double gamma, int dir);
 
 /* ---- snapshot layout of one field (the HDF5 datasets of src/io/io.cpp).  The grid is
  * x-fastest with ghost cells; a dataset holds the real cells z-fastest,

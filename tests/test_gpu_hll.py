@@ -1,0 +1,132 @@
+"""GPU parity of the HLL Riemann solver option (the reference's -DHLL build,
+Calculate_HLL_Fluxes_CUDA, src/riemann_solvers/hll_cuda.cu:19-268; chl_hydro_config.riemann
+= CHL_RIEMANN_HLL) against the oracle's HLL (tests/test_oracle_hll.py pins it).
+
+Bars, as for HLLC:
+  * the component entry chl_hll_fluxes is bit-identical to the oracle on random interface
+    pairs in all three directions;
+  * PolicyExact: a VL step and a 3-step KH run with PLMC and PPMC are bit-identical to the
+    oracle with HLL (and differ from the HLLC run, so the switch is live); PolicyFast and
+    the mixed form within per-field relative L1 1e-12 and scaled L1 1e-5;
+  * 1D Sod with PLMC + HLL bit-identical to the oracle's 1D driver.  The reference's
+    VL_1D integrator has no HLL branch (VL_1D_cuda.cu:69-124 names Exact, Roe and HLLC
+    only); this 1D form follows the oracle's VL_1D with the solver swapped.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cholla_amd as chl
+from helpers import sod_ic
+from oracle import oracle as orc
+from test_gpu_parity import TOL, _random_periodic_box, field_l1, scaled_l1
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _require_gpu():
+    if chl.device_count() < 1:
+        pytest.fail("no HIP device visible: the GPU tests need a real MI355X")
+
+
+def test_hll_component_bitwise_vs_oracle():
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 9, 8, 7
+    nc = nx * ny * nz
+    g = 5 / 3
+    d = rng.uniform(0.2, 3, nc)
+    v = rng.uniform(-3, 3, (3, nc))  # subsonic and supersonic faces both occur
+    p = rng.uniform(0.1, 4, nc)
+    U = np.concatenate([d, d * v[0], d * v[1], d * v[2], p / (g - 1) + 0.5 * d * (v ** 2).sum(0)])
+    L2 = U.copy()
+    R2 = np.roll(U.reshape(5, nc), 1, axis=1).reshape(-1).copy()
+    for dr in range(3):
+        say("hll component dir", dr)
+        Fo = np.zeros(5 * nc)
+        orc.lib().orc_hll(orc.ptr(L2), orc.ptr(R2), orc.ptr(Fo), nx, ny, nz, g, dr)
+        F = chl.DeviceArray(5 * nc, np.zeros(5 * nc))
+        chl.hll_fluxes(chl.DeviceArray.from_host(L2), chl.DeviceArray.from_host(R2), F, nx, ny, nz, g, dr)
+        np.testing.assert_array_equal(F.download(), Fo)
+
+
+@pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
+@pytest.mark.parametrize("recon,ng", [(chl.RECON_PLMC, 3), (chl.RECON_PPMC, 4)])
+def test_single_vl_step_hll(recon, ng, precision):
+    n = 20
+    U, N, g = _random_periodic_box(n, ng, seed=9)
+    h = 1.0 / n
+    dt = 0.3 / orc.calc_dti(U, N, N, N, ng, h, h, h, g)
+    say("hll vl_step recon", recon, "precision", precision, "N", N)
+    Uo, Uc = U.copy(), U.copy()
+    with orc.riemann(orc.HLL):
+        orc.lib().orc_vl3d(orc.ptr(Uo), N, N, N, ng, h, h, h, dt, g, 5, recon, 1, 0.0)
+    orc.lib().orc_vl3d(orc.ptr(Uc), N, N, N, ng, h, h, h, dt, g, 5, recon, 1, 0.0)
+    assert not np.array_equal(Uo, Uc)
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(N, N, N, ng, recon, precision, riemann=chl.RIEMANN_HLL)
+    hy.vl_step(Ud, h, h, h, dt, g)
+    A = Ud.download()
+    hy.close()
+    l1, mx = field_l1(A, Uo, 5, N, N, N, ng)
+    say("  rel L1", ["%.3e" % x for x in l1], "max|diff| %.3e" % mx)
+    assert max(l1) <= TOL[precision], l1
+    if precision == chl.PREC_FP64_EXACT:
+        assert mx == 0.0, f"exact form differs from the oracle by {mx}"
+
+
+@pytest.mark.parametrize("tile", [0, 1])
+@pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
+def test_kelvin_helmholtz_hll(precision, tile):
+    n, ng, g = 24, 4, 5 / 3
+    N = n + 2 * ng
+    U = chl.ic_kelvin_helmholtz(n, ng, g, seed=5)
+    h = 1.0 / n
+    Uo = U.copy()
+    steps_o, to = orc.run(Uo, N, N, N, ng, 5, h, h, h, g, orc.PPMC, [1] * 6, 0.0, 1e30, 6, solver=orc.HLL)
+    say("hll KH 24^3 x6 precision", precision, "tile", tile)
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(N, N, N, ng, chl.RECON_PPMC, precision, riemann=chl.RIEMANN_HLL, tile1=tile, tile2=tile)
+    steps, t = hy.run(Ud, h, h, h, g, 0.0, 1e30, 6, use_graph=True)
+    hy.close()
+    assert steps == steps_o == 6
+    assert abs(t - to) <= TOL[precision] * to
+    A = Ud.download()
+    l1, mx = field_l1(A, Uo, 5, N, N, N, ng)
+    sl1 = scaled_l1(A, Uo, N, N, N, ng)
+    say("  rel L1", ["%.3e" % x for x in l1], "scaled", ["%.3e" % x for x in sl1], "max|diff| %.3e" % mx)
+    if precision == chl.PREC_FP64_EXACT:
+        assert mx == 0.0
+    else:
+        assert max(sl1) <= TOL[precision], sl1
+
+
+def test_sod_1d_hll():
+    U, nx, dx = sod_ic()
+    ng = 3
+    orc.set_boundaries(U, nx, 1, 1, 5, ng, [3, 3, 0, 0, 0, 0])
+    Uo = U.copy()
+    steps_o, to = orc.run(Uo, nx, 1, 1, ng, 5, dx, dx, dx, 1.4, orc.PLMC, [3, 3, 0, 0, 0, 0], 0.0, 0.2, 100000,
+                          solver=orc.HLL)
+    say("Sod 1D HLL")
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(nx, 1, 1, ng, chl.RECON_PLMC, chl.PREC_FP64_EXACT, bc=[3, 3, 1, 1, 1, 1],
+                   riemann=chl.RIEMANN_HLL)
+    steps, t = hy.run(Ud, dx, dx, dx, 1.4, 0.0, 0.2, 100000, use_graph=False)
+    hy.close()
+    assert steps == steps_o and t == to
+    A = Ud.download().reshape(5, nx)
+    np.testing.assert_array_equal(A[:, ng:nx - ng], Uo.reshape(5, nx)[:, ng:nx - ng])
+
+
+def test_hll_option_limits():
+    with pytest.raises(chl.ChollaError, match="dual energy"):
+        chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, riemann=chl.RIEMANN_HLL, dual_energy=True)
+    with pytest.raises(chl.ChollaError, match="tile shapes 0 and 1"):
+        chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, riemann=chl.RIEMANN_HLL, tile2=3)
