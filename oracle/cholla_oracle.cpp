@@ -463,6 +463,60 @@ void orc_plmc(const double *U, double *QL, double *QR, int nx, int ny, int nz, d
       }
 }
 
+// Interface_Values_PLM, src/reconstruction/plmp_cuda.cu:325-365: Van Leer limited slope of
+// one primitive variable; *left and *right are the values at the cell's left and right faces.
+static void PlmpInterface(double qm, double q0, double qp, double *left, double *right)
+{
+  double dL = q0 - qm, dR = qp - q0, dC = 0.5 * (qp - qm);
+  double dG = dL * dR > 0.0 ? 2.0 * dL * dR / (dL + dR) : 0.0;
+  double a = std::fmin(std::fabs(dL), std::fabs(dR));
+  double b = std::fmin(std::fabs(dC), std::fabs(dG));
+  double dm = (dC < 0 ? -1.0 : 1.0) * std::fmin(2.0 * a, b);  // sgn_CUDA, global_cuda.h:41-48
+  *left     = q0 - 0.5 * dm;
+  *right    = q0 + 0.5 * dm;
+}
+
+// PLMP_cuda in the VL build (no half-step evolution, plmp_cuda.cu:231-284 compiled out),
+// src/reconstruction/plmp_cuda.cu:20-320, hydro without DE or scalars: primitives by
+// division, pressure from the left-to-right sum of squared velocities, limited in the
+// primitive variables, then back to conserved variables.
+void orc_plmp(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  int o1, o2, o3;
+  DirOrder(dir, o1, o2, o3);
+  int ix = dir == 0, iy = dir == 1, iz = dir == 2;
+  // plmp_cuda.cu:80-103: one cell off each end along dir, the full range across it
+  int xs = ix, xe = ix ? nx - 2 : nx, ys = iy, ye = iy ? ny - 2 : ny, zs = iz, ze = iz ? nz - 2 : nz;
+  auto load = [&](int i, int j, int k, double q[5]) {
+    size_t id = Idx(i, j, k, nx, ny);
+    double d = U[id], vx = U[o1 * n_cells + id] / d, vy = U[o2 * n_cells + id] / d, vz = U[o3 * n_cells + id] / d;
+    double p = (U[4 * n_cells + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+    q[0] = d, q[1] = vx, q[2] = vy, q[3] = vz, q[4] = std::fmax(p, kTiny);
+  };
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = zs; k < ze; k++)
+    for (int j = ys; j < ye; j++)
+      for (int i = xs; i < xe; i++) {
+        double qm[5], q0[5], qp[5], lf[5], rf[5];
+        load(i - ix, j - iy, k - iz, qm);
+        load(i, j, k, q0);
+        load(i + ix, j + iy, k + iz, qp);
+        for (int v = 0; v < 5; v++) PlmpInterface(qm[v], q0[v], qp[v], &lf[v], &rf[v]);
+        lf[0] = std::fmax(lf[0], kTiny), rf[0] = std::fmax(rf[0], kTiny);
+        lf[4] = std::fmax(lf[4], kTiny), rf[4] = std::fmax(rf[4], kTiny);
+        auto write = [&](const double q[5], double *Q, size_t id) {
+          Q[id]                = q[0];
+          Q[o1 * n_cells + id] = q[0] * q[1];
+          Q[o2 * n_cells + id] = q[0] * q[2];
+          Q[o3 * n_cells + id] = q[0] * q[3];
+          Q[4 * n_cells + id]  = q[4] / (gamma - 1.0) + 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        };
+        write(lf, QR, Idx(i - ix, j - iy, k - iz, nx, ny));  // right state of face i-1/2
+        write(rf, QL, Idx(i, j, k, nx, ny));                 // left state of face i+1/2
+      }
+}
+
 // PPMC_VL, src/reconstruction/ppmc_cuda.cu:540-695 (hydro, no scalars; DE branch :668-672)
 void orc_ppmc_vl(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
 {
@@ -1087,7 +1141,7 @@ void orc_set_boundaries(double *c, int nx, int ny, int nz, int n_fields, int n_g
 // Integrators
 // ---------------------------------------------------------------------------
 
-enum { ORC_PCM = 0, ORC_PLMC = 1, ORC_PPMC = 2 };
+enum { ORC_PCM = 0, ORC_PLMC = 1, ORC_PPMC = 2, ORC_PLMP = 3 };
 
 struct Scratch {
   size_t n = 0;
@@ -1123,6 +1177,10 @@ void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double 
     orc_ppmc_vl(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, nz, gamma, 0);
     orc_ppmc_vl(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, nz, gamma, 1);
     orc_ppmc_vl(s.Uh.data(), s.QLz.data(), s.QRz.data(), nx, ny, nz, gamma, 2);
+  } else if (recon == ORC_PLMP) {  // VL_3D_cuda.cu:230-236
+    orc_plmp(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, nz, gamma, 0);
+    orc_plmp(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, nz, gamma, 1);
+    orc_plmp(s.Uh.data(), s.QLz.data(), s.QRz.data(), nx, ny, nz, gamma, 2);
   } else {
     orc_pcm_3d(s.Uh.data(), s.QLx.data(), s.QRx.data(), s.QLy.data(), s.QRy.data(), s.QLz.data(), s.QRz.data(), nx,
                ny, nz, n_fields);
@@ -1152,6 +1210,8 @@ void orc_vl1d(double *U, int nx, int n_ghost, double dx, double dt, double gamma
     orc_plmc(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, 1, 1, gamma, 0);
   } else if (recon == ORC_PPMC) {
     orc_ppmc_vl(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, 1, 1, gamma, 0);
+  } else if (recon == ORC_PLMP) {  // VL_1D_cuda.cu:98-101
+    orc_plmp(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, 1, 1, gamma, 0);
   } else {
     orc_pcm_1d(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, n_fields);
   }

@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcholla_oracle.so")
 
-PCM, PLMC, PPMC = 0, 1, 2
+PCM, PLMC, PPMC, PLMP = 0, 1, 2, 3
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -49,6 +49,7 @@ def lib():
             "orc_pcm_3d": (None, [_dp] * 7 + [i, i, i, i]),
             "orc_pcm_1d": (None, [_dp, _dp, _dp, i, i]),
             "orc_plmc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_plmp": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_ppmc_vl": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_hllc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_update_half_3d": (None, [_dp] * 5 + [i, i, i, d, d, d, d, d, i, i, d]),
