@@ -78,15 +78,19 @@ int CheckConfig(const chl_hydro_config &c)
     return Fail(CHL_ERR_UNSUPPORTED, "riemann must be HLLC or HLL");
   if (c.riemann == CHL_RIEMANN_HLL && c.dual_energy)
     return Fail(CHL_ERR_UNSUPPORTED, "the HLL solver is built without dual energy");
-  if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC)
-    return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PLMC or PPMC");
+  if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC && c.reconstruction != CHL_RECON_PLMP &&
+      c.reconstruction != CHL_RECON_PCM)
+    return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PCM, PLMC, PPMC or PLMP");
+  const bool simple_recon = c.reconstruction == CHL_RECON_PCM || c.reconstruction == CHL_RECON_PLMP;
+  if (simple_recon && c.dual_energy) return Fail(CHL_ERR_UNSUPPORTED, "PCM and PLMP are built without dual energy");
   if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
   if (c.positivity_fallback != 0 && c.positivity_fallback != 1)
     return Fail(CHL_ERR_INVALID, "positivity_fallback must be 0 or 1");
   if (c.positivity_fallback && (c.reconstruction != CHL_RECON_PPMC || (c.ny == 1 && c.nz == 1)))
     return Fail(CHL_ERR_UNSUPPORTED, "the positivity fallback is built for PPMC on 3D boxes only");
   const int ng = c.n_ghost, r = ReconRadius(c.reconstruction);
-  if (ng < r + 2) return Fail(CHL_ERR_INVALID, "n_ghost too small for the reconstruction (PLMC needs 3, PPMC 4)");
+  if (ng < r + 2)
+    return Fail(CHL_ERR_INVALID, "n_ghost too small for the reconstruction (PCM needs 2, PLMC and PLMP 3, PPMC 4)");
   if (c.nx < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "nx must exceed 2*n_ghost");
   bool one_d = c.ny == 1 && c.nz == 1;
   if (!one_d) {
@@ -96,6 +100,8 @@ int CheckConfig(const chl_hydro_config &c)
       return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for tile shapes 0 and 1 only");
     if (c.riemann == CHL_RIEMANN_HLL && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
       return Fail(CHL_ERR_UNSUPPORTED, "the HLL solver is built for tile shapes 0 and 1 only");
+    if (simple_recon && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
+      return Fail(CHL_ERR_UNSUPPORTED, "PCM and PLMP are built for tile shapes 0 and 1 only");
     if (c.tile_order != 0 && c.tile_order != 1) return Fail(CHL_ERR_INVALID, "tile_order must be 0 or 1");
   } else {
     if (c.precision != CHL_PREC_FP64_EXACT)
@@ -185,6 +191,7 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
   if (update_grid && c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
   if (update_grid) flags |= chl::kFlagReduceDt;
   if (c.positivity_fallback) flags |= chl::kFlagPositivityFallback;
+  if (c.reconstruction == CHL_RECON_PCM) flags |= chl::kFlagConservedW;
   const bool fused_ghosts = fill && h->PeriodicFusable();
   if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
   chl::StepArgs a = MakeStepArgs(h, U, dx, dy, dz, gamma, dt_host, dt_dev, density_floor, U_floor, flags);
@@ -323,7 +330,9 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return Fail(CHL_ERR_NO_DEVICE, "no HIP device");
   chl_hydro_config c = *cfg;
   if (c.tile1 < 0) c.tile1 = 0;
-  if (c.tile2 < 0) c.tile2 = (c.dual_energy || c.riemann == CHL_RIEMANN_HLL) ? 0 : 3;
+  const bool narrow = c.dual_energy || c.riemann == CHL_RIEMANN_HLL || c.reconstruction == CHL_RECON_PCM ||
+                     c.reconstruction == CHL_RECON_PLMP;  // kernels built for tile shapes 0 and 1 only
+  if (c.tile2 < 0) c.tile2 = narrow ? 0 : 3;
   CHL_TRY(CheckConfig(c));
   auto *h    = new chl_hydro();
   h->one_d   = c.ny == 1 && c.nz == 1;
@@ -471,6 +480,7 @@ int chl_hydro_kernel_times(chl_hydro *h, double *U, double dx, double dy, double
   if (c.density_floor) flags |= chl::kFlagDensityFloor;
   if (c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
   if (c.positivity_fallback) flags |= chl::kFlagPositivityFallback;
+  if (c.reconstruction == CHL_RECON_PCM) flags |= chl::kFlagConservedW;
   const bool fused_ghosts = h->PeriodicFusable();
   if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
   chl::StepArgs a = MakeStepArgs(h, U, dx, dy, dz, gamma, 0.0, h->dt(), 0.0, 0.0, flags);
@@ -523,10 +533,13 @@ int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, doub
   chl::StepArgs a = MakeStepArgs(h, S, dx, dy, dz, gamma, dt_fixed, nullptr, 0.0, 0.0,
                                  chl::kFlagReduceDt | (c.density_floor ? chl::kFlagDensityFloor : 0) |
                                      (c.temperature_floor ? chl::kFlagTemperatureFloor : 0) |
-                                     (c.positivity_fallback ? chl::kFlagPositivityFallback : 0));
+                                     (c.positivity_fallback ? chl::kFlagPositivityFallback : 0) |
+                                     (c.reconstruction == CHL_RECON_PCM ? chl::kFlagConservedW : 0));
   const int R1     = ReconRadius(c.reconstruction) + 1;
   const int nz1    = c.nz - 2 * c.n_ghost + 2 * R1, nz2 = c.nz - 2 * c.n_ghost;
-  const int ntiles = (c.dual_energy || c.riemann == CHL_RIEMANN_HLL) ? chl::kNumDeTiles : chl::kNumTiles;
+  const bool narrow = c.dual_energy || c.riemann == CHL_RIEMANN_HLL || c.reconstruction == CHL_RECON_PCM ||
+                     c.reconstruction == CHL_RECON_PLMP;
+  const int ntiles  = narrow ? chl::kNumDeTiles : chl::kNumTiles;
   std::vector<int> chunks = {8, 16, 32, 64, 128};
   float best1 = 1e30f, best2 = 1e30f;
   int b_t1 = c.tile1, b_t2 = c.tile2, b_k1 = c.kchunk1, b_k2 = c.kchunk2;

@@ -14,7 +14,10 @@ int Fail(int code, const std::string &msg);
 
 constexpr double kCfl = 0.3;  // C_cfl, src/grid/grid3D.cpp:157
 
-inline int ReconRadius(int recon) { return recon == CHL_RECON_PPMC ? 2 : (recon == CHL_RECON_PLMC ? 1 : 0); }
+inline int ReconRadius(int recon)
+{
+  return recon == CHL_RECON_PPMC ? 2 : ((recon == CHL_RECON_PLMC || recon == CHL_RECON_PLMP) ? 1 : 0);
+}
 }  // namespace chl_capi
 
 #define CHL_HIP(expr)                                                                                          \

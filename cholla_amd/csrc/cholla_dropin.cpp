@@ -18,6 +18,10 @@ namespace
 
 #if defined(PPMC)
 constexpr int kRecon = CHL_RECON_PPMC;
+#elif defined(PLMP)
+constexpr int kRecon = CHL_RECON_PLMP;
+#elif defined(PCM)
+constexpr int kRecon = CHL_RECON_PCM;
 #else
 constexpr int kRecon = CHL_RECON_PLMC;  // builds/make.type.hydro:5 default
 #endif

@@ -421,6 +421,41 @@ __device__ __forceinline__ T MonotonizeOne(T left, T right, T centered, T van_le
   return T(0.0);
 }
 
+// Interface_Values_PLM, src/reconstruction/plmp_cuda.cu:325-365: the Van Leer limited slope
+// of one primitive variable; lo / hi are the values at the cell's left / right face.
+template <class P>
+__device__ __forceinline__ void PlmpInterface(typename P::T qm, typename P::T q0, typename P::T qp, typename P::T &lo,
+                                              typename P::T &hi)
+{
+  using T = typename P::T;
+  const T dL = q0 - qm, dR = qp - q0, dC = T(0.5) * (qp - qm);
+  const T dG = dL * dR > T(0.0) ? Div1<P>(T(2.0) * dL * dR, dL + dR) : T(0.0);
+  const T a = Fmin(Fabs(dL), Fabs(dR));
+  const T b = Fmin(Fabs(dC), Fabs(dG));
+  const T dm = SignOf(dC) * Fmin(T(2.0) * a, b);  // sgn_CUDA, global_cuda.h:41-48
+  lo = q0 - T(0.5) * dm;
+  hi = q0 + T(0.5) * dm;
+}
+
+// PLMP_cuda body in the VL build (plmp_cuda.cu:199-224; the half-step evolution at
+// :231-284 is compiled out): each primitive limited on its own, then the density and
+// pressure floors.  s[0..2] = cells i-1, i, i+1.
+template <class P>
+__device__ __forceinline__ void ReconPlmp(const Prim<typename P::T> s[3], Prim<typename P::T> &L_iph,
+                                          Prim<typename P::T> &R_imh)
+{
+  using T = typename P::T;
+  PlmpInterface<P>(s[0].d, s[1].d, s[2].d, R_imh.d, L_iph.d);
+  PlmpInterface<P>(s[0].vx, s[1].vx, s[2].vx, R_imh.vx, L_iph.vx);
+  PlmpInterface<P>(s[0].vy, s[1].vy, s[2].vy, R_imh.vy, L_iph.vy);
+  PlmpInterface<P>(s[0].vz, s[1].vz, s[2].vz, R_imh.vz, L_iph.vz);
+  PlmpInterface<P>(s[0].p, s[1].p, s[2].p, R_imh.p, L_iph.p);
+  R_imh.d = Fmax(R_imh.d, T(kTinyNumber));
+  L_iph.d = Fmax(L_iph.d, T(kTinyNumber));
+  R_imh.p = Fmax(R_imh.p, T(kTinyNumber));
+  L_iph.p = Fmax(L_iph.p, T(kTinyNumber));
+}
+
 // PLMC_cuda body (VL build: no characteristic tracing), src/reconstruction/plmc_cuda.cu:73-291.
 // s[0..2] = cells i-1, i, i+1.
 template <class P>

@@ -374,7 +374,7 @@ __global__ void TurbulenceIcKernel(double *U, int n, int ng, int n_modes, const 
 // ============================================================================ VL 1D
 // Stage 1 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:64-91): PCM states, the Riemann
 // solve of P (HLLC, or HLL under HllPolicy) on faces id-1/2 and id+1/2 of every
-// cell 1..n-2, the dt/2 update, then the half state's primitives.
+// cell 1..n-2 and the dt/2 update; W holds the conserved half state.
 template <class P>
 __global__ void Vl1dStage1Kernel(const double *U, double *W, int n, double dx, const double *dt_dev, double dt_host,
                                  double gamma, unsigned long long *dti_reset)
@@ -396,14 +396,12 @@ __global__ void Vl1dStage1Kernel(const double *U, double *W, int n, double dx, c
   uh[2]          = U[2 * n + id] + dtodx * (fm.my - fp.my);
   uh[3]          = U[3 * n + id] + dtodx * (fm.mz - fp.mz);
   uh[4]          = U[4 * n + id] + dtodx * (fm.E - fp.E);
-  Prim<double> w = LoadPrim(uh[0], uh[1], uh[2], uh[3], uh[4], gamma);
-  W[id]          = w.d;
-  W[n + id]      = w.vx;
-  W[2 * n + id]  = w.vy;
-  W[3 * n + id]  = w.vz;
-  W[4 * n + id]  = w.p;
+  for (int q = 0; q < 5; q++) W[q * n + id] = uh[q];
 }
 
+// The interface values of cell c from the conserved half state W: primitives as
+// Load_Data forms them (PLMC, PPMC) or as PLMP_cuda does (plmp_cuda.cu:104-120), then
+// the reconstruction; PCM passes the conserved values through (pcm_cuda.cu:13-97).
 template <int RK>
 __device__ __forceinline__ void Recon1D(const double *W, int n, int c, double gamma, Prim<double> &L, Prim<double> &R)
 {
@@ -413,9 +411,21 @@ __device__ __forceinline__ void Recon1D(const double *W, int n, int c, double ga
 #pragma unroll
   for (int m = 0; m < NS; m++) {
     int q = c - r + m;
-    s[m]  = {W[q], W[n + q], W[2 * n + q], W[3 * n + q], W[4 * n + q]};
+    const double d = W[q], mx = W[n + q], my = W[2 * n + q], mz = W[3 * n + q], E = W[4 * n + q];
+    if constexpr (RK == kPCM) {
+      s[m] = {d, mx, my, mz, E};
+    } else if constexpr (RK == kPLMP) {
+      const double vx = mx / d, vy = my / d, vz = mz / d;
+      s[m] = {d, vx, vy, vz, Fmax((E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0), kTinyNumber)};
+    } else {
+      s[m] = LoadPrim(d, mx, my, mz, E, gamma);
+    }
   }
-  if constexpr (RK == kPPMC) {
+  if constexpr (RK == kPCM) {
+    L = R = s[0];
+  } else if constexpr (RK == kPLMP) {
+    ReconPlmp<PolicyExact>(s, L, R);
+  } else if constexpr (RK == kPPMC) {
     ReconPpmc<PolicyExact>(s, gamma, L, R);
   } else {
     ReconPlmc<PolicyExact>(s, gamma, L, R);
@@ -437,10 +447,10 @@ __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, doub
     Recon1D<RK>(W, n, id - 1, gamma, Lm, Rm);
     Recon1D<RK>(W, n, id, gamma, L0, R0);
     Recon1D<RK>(W, n, id + 1, gamma, Lp, Rp);
-    Cons<double> fm = RiemannFlux<P>(RStateFromPrim<PolicyExact>(Lm, gamma, inv_gm1),
-                                     RStateFromPrim<PolicyExact>(R0, gamma, inv_gm1), gamma);
-    Cons<double> fp = RiemannFlux<P>(RStateFromPrim<PolicyExact>(L0, gamma, inv_gm1),
-                                     RStateFromPrim<PolicyExact>(Rp, gamma, inv_gm1), gamma);
+    Cons<double> fm = RiemannFlux<P>(IfaceState<PolicyExact, RK>(Lm, gamma, inv_gm1),
+                                     IfaceState<PolicyExact, RK>(R0, gamma, inv_gm1), gamma);
+    Cons<double> fp = RiemannFlux<P>(IfaceState<PolicyExact, RK>(L0, gamma, inv_gm1),
+                                     IfaceState<PolicyExact, RK>(Rp, gamma, inv_gm1), gamma);
     const double dtodx = dt / dx;
     double un[5];
     un[0] = U[id] + dtodx * (fm.d - fp.d);
@@ -659,8 +669,14 @@ static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, do
   if (recon == kPPMC) {
     hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMC>), dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma,
                        dti);
-  } else {
+  } else if (recon == kPLMC) {
     hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMC>), dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma,
+                       dti);
+  } else if (recon == kPLMP) {
+    hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMP>), dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma,
+                       dti);
+  } else {
+    hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPCM>), dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma,
                        dti);
   }
   return hipGetLastError();
@@ -670,7 +686,7 @@ hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int rieman
                       double dt, double gamma, unsigned long long *dti, hipStream_t s)
 {
   if (dti && n > 1024) return hipErrorInvalidValue;
-  if (recon != kPPMC && recon != kPLMC) return hipErrorInvalidValue;
+  if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPCM) return hipErrorInvalidValue;
   if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
   if (riemann == kRiemannHllc) return LaunchVl1dP<PolicyExact>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
   return hipErrorInvalidValue;

@@ -31,7 +31,7 @@
 namespace chl
 {
 
-enum ReconKind : int { kPCM = 0, kPLMC = 1, kPPMC = 2 };
+enum ReconKind : int { kPCM = 0, kPLMC = 1, kPPMC = 2, kPLMP = 3 };
 
 template <int R>
 struct ReconTraits;
@@ -42,6 +42,14 @@ struct ReconTraits<kPLMC> {
 template <>
 struct ReconTraits<kPPMC> {
   static constexpr int kRadius = 2;
+};
+template <>
+struct ReconTraits<kPLMP> {
+  static constexpr int kRadius = 1;
+};
+template <>
+struct ReconTraits<kPCM> {
+  static constexpr int kRadius = 0;
 };
 
 // Parameters shared by both stages.
@@ -72,6 +80,7 @@ enum : int {
   kFlagPeriodicGhosts     = 4,  // write periodic ghost images from the stage-2 epilogue
   kFlagReduceDt           = 8,
   kFlagPositivityFallback = 16,  // PPMC positivity fallback (see ReconPpmc)
+  kFlagConservedW         = 32,  // exact form, PCM: stage 1 stores momenta, not velocities
 };
 
 __device__ __forceinline__ size_t Lin(int i, int j, int k, int nx, int ny)
@@ -408,10 +417,12 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
         }
       } else {
         Prim<double> w = LoadPrim(uh[0], uh[1], uh[2], uh[3], uh[4], a.gamma);
+        // PCM hands the conserved half state itself to the Riemann solver (pcm_cuda.cu)
+        const bool cons_w = P::kExact && (a.flags & kFlagConservedW);
         W[id]          = T(w.d);
-        W[nc + id]     = T(w.vx);
-        W[2 * nc + id] = T(w.vy);
-        W[3 * nc + id] = T(w.vz);
+        W[nc + id]     = cons_w ? T(uh[1]) : T(w.vx);
+        W[2 * nc + id] = cons_w ? T(uh[2]) : T(w.vy);
+        W[3 * nc + id] = cons_w ? T(uh[3]) : T(w.vz);
         W[4 * nc + id] = P::kExact ? T(uh[4]) : T(w.p);  // see WCell
       }
     }
@@ -463,7 +474,13 @@ __device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, type
   for (int m = 0; m < NS; m++) {
     st[m].d = s[m].d;
     Rot<DIR>(s[m].vx, s[m].vy, s[m].vz, st[m].vx, st[m].vy, st[m].vz);
-    if constexpr (P::kExact) {
+    if constexpr (RK == kPCM) {
+      st[m].p = s[m].p_or_E;  // exact form: (d, momenta, E) carried through as they are
+    } else if constexpr (P::kExact && RK == kPLMP) {
+      // plmp_cuda.cu:113-120: the squared velocities summed left to right
+      const T v2 = (st[m].vx * st[m].vx + st[m].vy * st[m].vy) + st[m].vz * st[m].vz;
+      st[m].p    = Fmax((s[m].p_or_E - T(0.5) * st[m].d * v2) * (gamma - T(1.0)), T(kTinyNumber));
+    } else if constexpr (P::kExact) {
       if constexpr (DE) {
         T E_kin  = T(0.5) * st[m].d * SquareMag(st[m].vx, st[m].vy, st[m].vz);
         st[m].p  = PressureFromDE(s[m].p_or_E, s[m].p_or_E - E_kin, s[m].g, gamma);
@@ -476,7 +493,11 @@ __device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, type
       if constexpr (DE) ge[m] = s[m].g;
     }
   }
-  if constexpr (RK == kPPMC) {
+  if constexpr (RK == kPCM) {
+    L = R = st[0];
+  } else if constexpr (RK == kPLMP) {
+    ReconPlmp<P>(st, L, R);
+  } else if constexpr (RK == kPPMC) {
     bool fb = ReconPpmc<P>(st, gamma, L, R, pos_fallback);
     if constexpr (DE) {
       ReconPpmcScalar<P>(ge, *geL, *geR);
@@ -485,6 +506,26 @@ __device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, type
   } else {
     ReconPlmc<P>(st, gamma, L, R);
     if constexpr (DE) ReconPlmcScalar<P>(ge, *geL, *geR);
+  }
+}
+
+// The Riemann input of one interface value: the conserved variables the reference's
+// reconstruction writes to dev_bounds, as the Riemann solver reads them back.  PCM and
+// PLMP form them in their own order under the exact form (pcm_cuda.cu copies the state;
+// plmp_cuda.cu:217-225 sums the squared velocities left to right).
+template <class P, int RK, bool DE = false>
+__device__ __forceinline__ RState<typename P::T> IfaceState(const Prim<typename P::T> &q, typename P::T gamma,
+                                                            typename P::T inv_gm1,
+                                                            typename P::T ge = typename P::T(0))
+{
+  using T = typename P::T;
+  if constexpr (P::kExact && RK == kPCM) {
+    return RStateFromCons<P>(q.d, q.vx, q.vy, q.vz, q.p, gamma);  // q holds (d, momenta, E)
+  } else if constexpr (P::kExact && RK == kPLMP) {
+    const T E = q.p / (gamma - T(1.0)) + T(0.5) * q.d * ((q.vx * q.vx + q.vy * q.vy) + q.vz * q.vz);
+    return RStateFromCons<P>(q.d, q.d * q.vx, q.d * q.vy, q.d * q.vz, E, gamma);
+  } else {
+    return RStateFromPrim<P, DE>(q, gamma, inv_gm1, ge);
   }
 }
 
@@ -631,7 +672,7 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
   win[NS - 1] = loadW(ic, jc, k0 + r);
   ReconDir<P, RK, 2, DE>(win, gamma, pf, qLz, qRz, &gLz, &gRz);  // cell k0
   T fz_lo[NC];                                               // flux through face k-1/2 of the current cell
-  FaceFlux<P, 2, DE>(RStateFromPrim<P, DE>(qL_lo, gamma, inv_gm1, gL_lo), RStateFromPrim<P, DE>(qRz, gamma, inv_gm1, gRz),
+  FaceFlux<P, 2, DE>(IfaceState<P, RK, DE>(qL_lo, gamma, inv_gm1, gL_lo), IfaceState<P, RK, DE>(qRz, gamma, inv_gm1, gRz),
                      gamma, fz_lo);
   qL_lo = qLz;  // left state of face k0+1/2
   gL_lo = gLz;
@@ -663,8 +704,8 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
     win[NS - 1] = loadW(ic, jc, k + 1 + r);
     ReconDir<P, RK, 2, DE>(win, gamma, pf, qLz, qRz, &gLz, &gRz);
     T fz_hi[NC];
-    FaceFlux<P, 2, DE>(RStateFromPrim<P, DE>(qL_lo, gamma, inv_gm1, gL_lo),
-                       RStateFromPrim<P, DE>(qRz, gamma, inv_gm1, gRz), gamma, fz_hi);
+    FaceFlux<P, 2, DE>(IfaceState<P, RK, DE>(qL_lo, gamma, inv_gm1, gL_lo),
+                       IfaceState<P, RK, DE>(qRz, gamma, inv_gm1, gRz), gamma, fz_hi);
     qL_lo = qLz;
     gL_lo = gLz;
     __syncthreads();
@@ -686,10 +727,10 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
       int ql = fj * (TX + 2) + fi;
       T g[NC];
       if constexpr (DE) {
-        FaceFlux<P, 0, DE>(RStateFromPrim<P, DE>(qL[ql], gamma, inv_gm1, qgL[ql]),
-                           RStateFromPrim<P, DE>(qR[ql + 1], gamma, inv_gm1, qgR[ql + 1]), gamma, g);
+        FaceFlux<P, 0, DE>(IfaceState<P, RK, DE>(qL[ql], gamma, inv_gm1, qgL[ql]),
+                           IfaceState<P, RK, DE>(qR[ql + 1], gamma, inv_gm1, qgR[ql + 1]), gamma, g);
       } else {
-        FaceFlux<P, 0, DE>(RStateFromPrim<P>(qL[ql], gamma, inv_gm1), RStateFromPrim<P>(qR[ql + 1], gamma, inv_gm1),
+        FaceFlux<P, 0, DE>(IfaceState<P, RK>(qL[ql], gamma, inv_gm1), IfaceState<P, RK>(qR[ql + 1], gamma, inv_gm1),
                            gamma, g);
       }
 #pragma unroll
@@ -720,10 +761,10 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
       const int lq = fj * TX + fi, rq = (fj + 1) * TX + fi;
       T g[NC];
       if constexpr (DE) {
-        FaceFlux<P, 1, DE>(RStateFromPrim<P, DE>(qL[lq], gamma, inv_gm1, qgL[lq]),
-                           RStateFromPrim<P, DE>(qR[rq], gamma, inv_gm1, qgR[rq]), gamma, g);
+        FaceFlux<P, 1, DE>(IfaceState<P, RK, DE>(qL[lq], gamma, inv_gm1, qgL[lq]),
+                           IfaceState<P, RK, DE>(qR[rq], gamma, inv_gm1, qgR[rq]), gamma, g);
       } else {
-        FaceFlux<P, 1, DE>(RStateFromPrim<P>(qL[lq], gamma, inv_gm1), RStateFromPrim<P>(qR[rq], gamma, inv_gm1), gamma,
+        FaceFlux<P, 1, DE>(IfaceState<P, RK>(qL[lq], gamma, inv_gm1), IfaceState<P, RK>(qR[rq], gamma, inv_gm1), gamma,
                            g);
       }
 #pragma unroll

@@ -47,7 +47,7 @@ hipError_t LaunchStage2T(const StepArgs &a, hipStream_t s)
   return hipGetLastError();
 }
 
-// Dual-energy and HLL kernels are compiled for the first kNumDeTiles tile shapes only.
+// Dual-energy, HLL, PCM and PLMP kernels are compiled for the first kNumDeTiles tile shapes only.
 constexpr int kNumDeTiles = 2;
 
 template <class P>
@@ -86,6 +86,13 @@ hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 template <class P, int RK>
 hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
 {
+  if constexpr (RK == kPCM || RK == kPLMP) {  // built for the first two tile shapes, without DE
+    if (a.de) return hipErrorInvalidValue;
+    const bool hll = a.riemann == kRiemannHll;
+    if (tile == 0) return hll ? LaunchStage2T<HllPolicy<P>, RK, 32, 8>(a, s) : LaunchStage2T<P, RK, 32, 8>(a, s);
+    if (tile == 1) return hll ? LaunchStage2T<HllPolicy<P>, RK, 16, 16>(a, s) : LaunchStage2T<P, RK, 16, 16>(a, s);
+    return hipErrorInvalidValue;
+  }
   if (a.riemann == kRiemannHll) {
     if (a.de) return hipErrorInvalidValue;
     if (tile == 0) return LaunchStage2T<HllPolicy<P>, RK, 32, 8>(a, s);
@@ -121,6 +128,8 @@ hipError_t LaunchStage2(const StepArgs &a, int recon, int tile, hipStream_t s)
 {
   if (recon == kPPMC) return LaunchStage2R<P, kPPMC>(a, tile, s);
   if (recon == kPLMC) return LaunchStage2R<P, kPLMC>(a, tile, s);
+  if (recon == kPLMP) return LaunchStage2R<P, kPLMP>(a, tile, s);
+  if (recon == kPCM) return LaunchStage2R<P, kPCM>(a, tile, s);
   return hipErrorInvalidValue;
 }
 

@@ -35,6 +35,7 @@ extern "C" {
 #define CHL_RECON_PCM 0  /* PCM  */
 #define CHL_RECON_PLMC 1 /* PLMC (builds/make.type.hydro:5) */
 #define CHL_RECON_PPMC 2 /* PPMC */
+#define CHL_RECON_PLMP 3 /* PLMP: limited in the primitive variables (src/reconstruction/plmp_cuda.cu) */
 #define CHL_RIEMANN_HLLC 0 /* HLLC (builds/make.type.hydro) */
 #define CHL_RIEMANN_HLL 1  /* HLL, Davis wave speeds (-DHLL, src/riemann_solvers/hll_cuda.cu) */
 /* arithmetic forms of the same step */
@@ -51,7 +52,8 @@ typedef struct chl_hydro_config {
   int nx, ny, nz;        /* total cells per axis incl. ghosts (Header::nx, ny, nz) */
   int n_ghost;           /* Header::n_ghost: PLMC 3, PPMC 4 (src/grid/grid3D.cpp:53-76) */
   int n_fields;          /* 5 (hydro), or 6 with dual_energy (GasEnergy last); no scalars */
-  int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL */
+  int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL.  PCM and PLMP: no dual
+                            energy, tile shapes 0 and 1 (n_ghost: PCM 2, PLMP 3) */
   int riemann;           /* CHL_RIEMANN_HLLC, or CHL_RIEMANN_HLL (no dual energy; tile shapes 0, 1) */
   int precision;         /* CHL_PREC_* */
   int density_floor;     /* DENSITY_FLOOR build option, 0/1 */
