@@ -720,12 +720,121 @@ void orc_hll(const double *QL, const double *QR, double *F, int nx, int ny, int 
   }
 }
 
+// Calculate_Roe_Fluxes_CUDA, src/riemann_solvers/roe_cuda.cu:19-400 (no scalars; etah is 0
+// there, so the eigenvalue floor is |lambda|): the upwind flux where the Roe-averaged
+// eigenvalues are one-signed, else the Roe flux, or HLLE where an intermediate state of the
+// linearised solution has non-positive density or negative pressure.
+void orc_roe(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma, int dir)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  int o1, o2, o3;
+  DirOrder(dir, o1, o2, o3);
+  const double g1 = gamma - 1.0;
+#pragma omp parallel for schedule(static)
+  for (long t = 0; t < long(n_cells); t++) {
+    const double dl = QL[t], mxl = QL[o1 * n_cells + t], myl = QL[o2 * n_cells + t], mzl = QL[o3 * n_cells + t],
+                 El = QL[4 * n_cells + t];
+    const double dr = QR[t], mxr = QR[o1 * n_cells + t], myr = QR[o2 * n_cells + t], mzr = QR[o3 * n_cells + t],
+                 Er = QR[4 * n_cells + t];
+    const double dgel = g_de ? QL[kGE * n_cells + t] : 0.0, dger = g_de ? QR[kGE * n_cells + t] : 0.0;
+    // primitives (roe_cuda.cu:118-160)
+    const double vxl = mxl / dl, vyl = myl / dl, vzl = mzl / dl;
+    const double vxr = mxr / dr, vyr = myr / dr, vzr = mzr / dr;
+    double pl, pr;
+    if (g_de) {
+      pl = PressureFromDE(El, El - 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl), dgel, gamma);
+      pr = PressureFromDE(Er, Er - 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr), dger, gamma);
+    } else {
+      pl = (El - 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl)) * (gamma - 1.0);
+      pr = (Er - 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr)) * (gamma - 1.0);
+    }
+    pl = std::fmax(pl, kTiny), pr = std::fmax(pr, kTiny);
+    const double gel = dgel / dl, ger = dger / dr;
+    // Roe averages (roe_cuda.cu:163-184; Toro 11.118)
+    const double Hl = (El + pl) / dl, Hr = (Er + pr) / dr;
+    const double sl = std::sqrt(dl), sr = std::sqrt(dr);
+    const double vx = (sl * vxl + sr * vxr) / (sl + sr), vy = (sl * vyl + sr * vyr) / (sl + sr);
+    const double vz = (sl * vzl + sr * vzr) / (sl + sr), H = (sl * Hl + sr * Hr) / (sl + sr);
+    const double vsq = (vx * vx + vy * vy + vz * vz);
+    const double asq = g1 * std::fmax((H - 0.5 * vsq), kTiny);
+    const double a   = std::sqrt(asq);
+    const double lm = vx - a, l0 = vx, lp = vx + a;
+    double fl[6] = {mxl, mxl * vxl + pl, mxl * vyl, mxl * vzl, (El + pl) * vxl, mxl * gel};
+    double fr[6] = {mxr, mxr * vxr + pr, mxr * vyr, mxr * vzr, (Er + pr) * vxr, mxr * ger};
+    double out[6];
+    if (lm >= 0.0) {  // roe_cuda.cu:216-247
+      for (int q = 0; q < 6; q++) out[q] = fl[q];
+    } else if (lp <= 0.0) {
+      for (int q = 0; q < 6; q++) out[q] = fr[q];
+    } else {
+      const double dd = dr - dl, dmx = mxr - mxl, dmy = myr - myl, dmz = mzr - mzl, dE = Er - El;
+      const double Na = 0.5 / asq;
+      double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      // projections on the left eigenvectors, each weighted by |lambda| (roe_cuda.cu:261-309)
+      const double a0 = dd * Na * (0.5 * g1 * vsq + vx * a) - dmx * Na * (g1 * vx + a) - dmy * Na * g1 * vy -
+                        dmz * Na * g1 * vz + dE * Na * g1;
+      double c = a0 * std::fabs(lm);
+      sum[0] += c, sum[1] += c * (vx - a), sum[2] += c * vy, sum[3] += c * vz, sum[4] += c * (H - vx * a);
+      const double a1 = -dd * vy + dmy;
+      c               = a1 * std::fabs(l0);
+      sum[2] += c, sum[4] += c * vy;
+      const double a2 = -dd * vz + dmz;
+      c               = a2 * std::fabs(l0);
+      sum[3] += c, sum[4] += c * vz;
+      const double a3 = dd * (1.0 - Na * g1 * vsq) + dmx * g1 * vx / asq + dmy * g1 * vy / asq + dmz * g1 * vz / asq -
+                        dE * g1 / asq;
+      c = a3 * std::fabs(l0);
+      sum[0] += c, sum[1] += c * vx, sum[2] += c * vy, sum[3] += c * vz, sum[4] += c * 0.5 * vsq;
+      const double a4 = dd * Na * (0.5 * g1 * vsq - vx * a) - dmx * Na * (g1 * vx - a) - dmy * Na * g1 * vy -
+                        dmz * Na * g1 * vz + dE * Na * g1;
+      c = a4 * std::fabs(lp);
+      sum[0] += c, sum[1] += c * (vx + a), sum[2] += c * vy, sum[3] += c * vz, sum[4] += c * (H + vx * a);
+      // positivity of the two intermediate states (roe_cuda.cu:311-343).  The second test
+      // adds the fourth wave's coefficient to the density only (roe_cuda.cu:330-334); kept
+      // as the reference has it, so the switch is not mirror-symmetric.
+      int hlle = 0;
+      double t0 = dl + a0, t1 = mxl + a0 * (vx - a), t2 = myl + a0 * vy, t3 = mzl + a0 * vz,
+             t4 = El + a0 * (H - vx * a);
+      if (l0 > lm) {
+        if (t0 <= 0.0) hlle = 1;
+        if (t4 - 0.5 * (t1 * t1 + t2 * t2 + t3 * t3) / t0 < 0.0) hlle = 2;
+      }
+      t0 += a3 + a4, t1 += a3 * vx, t2 += a1 + a3 * vy, t3 += a2 + a3 * vz, t4 += a1 * vy + a2 * vz + a3 * 0.5 * vsq;
+      if (lp > l0) {
+        if (t0 <= 0.0) hlle = 1;
+        if (t4 - 0.5 * (t1 * t1 + t2 * t2 + t3 * t3) / t0 < 0.0) hlle = 2;
+      }
+      if (hlle != 0) {  // HLLE (roe_cuda.cu:346-379)
+        const double cfl = std::sqrt(gamma * pl / dl), cfr = std::sqrt(gamma * pr / dr);
+        const double bm = std::fmin(std::fmin(lm, vxl - cfl), 0.0), bp = std::fmax(std::fmax(lp, vxr + cfr), 0.0);
+        const double hl[6] = {mxl - bm * dl,           mxl * (vxl - bm) + pl, myl * (vxl - bm),
+                              mzl * (vxl - bm),        El * (vxl - bm) + pl * vxl, dgel * (vxl - bm)};
+        const double hr[6] = {mxr - bp * dr,           mxr * (vxr - bp) + pr, myr * (vxr - bp),
+                              mzr * (vxr - bp),        Er * (vxr - bp) + pr * vxr, dger * (vxr - bp)};
+        const double tmp = 0.5 * (bp + bm) / (bp - bm);
+        for (int q = 0; q < 6; q++) out[q] = 0.5 * (hl[q] + hr[q]) + (hl[q] - hr[q]) * tmp;
+      } else {  // roe_cuda.cu:381-397
+        for (int q = 0; q < 5; q++) out[q] = 0.5 * (fl[q] + fr[q] - sum[q]);
+        out[5] = out[0] >= 0.0 ? out[0] * gel : out[0] * ger;
+      }
+    }
+    F[t]                = out[0];
+    F[o1 * n_cells + t] = out[1];
+    F[o2 * n_cells + t] = out[2];
+    F[o3 * n_cells + t] = out[3];
+    F[4 * n_cells + t]  = out[4];
+    if (g_de) F[kGE * n_cells + t] = out[5];
+  }
+}
+
 // The solver the build selects (g_riemann).
 static void RiemannFluxes(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
                           int dir)
 {
   if (g_riemann == 1) {
     orc_hll(QL, QR, F, nx, ny, nz, gamma, dir);
+  } else if (g_riemann == 2) {
+    orc_roe(QL, QR, F, nx, ny, nz, gamma, dir);
   } else {
     orc_hllc(QL, QR, F, nx, ny, nz, gamma, dir);
   }
@@ -1258,7 +1367,7 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
 // The DE build option (dual energy) as a run-time switch for every stage above.
 void orc_set_dual_energy(int on) { g_de = on ? 1 : 0; }
 void orc_set_positivity_fallback(int on) { g_pos_fallback = on ? 1 : 0; }
-void orc_set_riemann(int solver) { g_riemann = solver == 1 ? 1 : 0; }
+void orc_set_riemann(int solver) { g_riemann = (solver == 1 || solver == 2) ? solver : 0; }
 int orc_get_dual_energy() { return g_de; }
 
 // hydro_utilities::Get_Pressure_From_DE, for the known-answer tests

@@ -50,6 +50,7 @@ def lib():
             "orc_pcm_1d": (None, [_dp, _dp, _dp, i, i]),
             "orc_plmc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_plmp": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_roe": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_ppmc_vl": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_hllc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_update_half_3d": (None, [_dp] * 5 + [i, i, i, d, d, d, d, d, i, i, d]),
@@ -141,12 +142,12 @@ def positivity_fallback(on=True):
         lib().orc_set_positivity_fallback(0)
 
 
-HLLC, HLL = 0, 1
+HLLC, HLL, ROE = 0, 1, 2
 
 
 @contextlib.contextmanager
 def riemann(solver=HLLC):
-    """The Riemann solver build option (HLLC or HLL) for the calls inside the block."""
+    """The Riemann solver build option (HLLC, HLL or ROE) for the calls inside the block."""
     lib().orc_set_riemann(int(solver))
     try:
         yield
