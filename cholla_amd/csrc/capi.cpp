@@ -74,10 +74,10 @@ int CheckConfig(const chl_hydro_config &c)
   if (c.dual_energy != 0 && c.dual_energy != 1) return Fail(CHL_ERR_INVALID, "dual_energy must be 0 or 1");
   if (c.n_fields != 5 + c.dual_energy)
     return Fail(CHL_ERR_UNSUPPORTED, "n_fields must be 5, or 6 with dual_energy (no passive scalars)");
-  if (c.riemann != CHL_RIEMANN_HLLC && c.riemann != CHL_RIEMANN_HLL)
-    return Fail(CHL_ERR_UNSUPPORTED, "riemann must be HLLC or HLL");
-  if (c.riemann == CHL_RIEMANN_HLL && c.dual_energy)
-    return Fail(CHL_ERR_UNSUPPORTED, "the HLL solver is built without dual energy");
+  if (c.riemann != CHL_RIEMANN_HLLC && c.riemann != CHL_RIEMANN_HLL && c.riemann != CHL_RIEMANN_ROE)
+    return Fail(CHL_ERR_UNSUPPORTED, "riemann must be HLLC, HLL or ROE");
+  if (c.riemann != CHL_RIEMANN_HLLC && c.dual_energy)
+    return Fail(CHL_ERR_UNSUPPORTED, "the HLL and Roe solvers are built without dual energy");
   if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC && c.reconstruction != CHL_RECON_PLMP &&
       c.reconstruction != CHL_RECON_PCM)
     return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PCM, PLMC, PPMC or PLMP");
@@ -98,8 +98,8 @@ int CheckConfig(const chl_hydro_config &c)
     if (c.tile1 >= chl::kNumTiles || c.tile2 >= chl::kNumTiles) return Fail(CHL_ERR_INVALID, "tile index out of range");
     if (c.dual_energy && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
       return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for tile shapes 0 and 1 only");
-    if (c.riemann == CHL_RIEMANN_HLL && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
-      return Fail(CHL_ERR_UNSUPPORTED, "the HLL solver is built for tile shapes 0 and 1 only");
+    if (c.riemann != CHL_RIEMANN_HLLC && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
+      return Fail(CHL_ERR_UNSUPPORTED, "the HLL and Roe solvers are built for tile shapes 0 and 1 only");
     if (simple_recon && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
       return Fail(CHL_ERR_UNSUPPORTED, "PCM and PLMP are built for tile shapes 0 and 1 only");
     if (c.tile_order != 0 && c.tile_order != 1) return Fail(CHL_ERR_INVALID, "tile_order must be 0 or 1");
@@ -116,6 +116,11 @@ int CheckConfig(const chl_hydro_config &c)
   return CHL_OK;
 }
 
+int SolverKind(int riemann)
+{
+  return riemann == CHL_RIEMANN_HLL ? chl::kRiemannHll : (riemann == CHL_RIEMANN_ROE ? chl::kRiemannRoe : chl::kRiemannHllc);
+}
+
 int DefaultChunk(int n_real) { return std::max(8, std::min(n_real, 32)); }
 
 // Arguments of the fused stage kernels for this box (kchunk is set per stage).
@@ -130,7 +135,7 @@ chl::StepArgs MakeStepArgs(chl_hydro *h, double *U, double dx, double dy, double
   a.flags     = flags;
   a.xcd_order = c.tile_order == 0;
   a.de        = c.dual_energy;
-  a.riemann   = c.riemann == CHL_RIEMANN_HLL ? chl::kRiemannHll : chl::kRiemannHllc;
+  a.riemann   = SolverKind(c.riemann);
   return a;
 }
 
@@ -177,7 +182,7 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
     const int n = c.nx;
     bool fuse_dt = update_grid && n <= 1024;
     CHL_HIP(chl::LaunchVl1d(U, reinterpret_cast<double *>(h->W), n, ng, c.reconstruction,
-                            c.riemann == CHL_RIEMANN_HLL ? chl::kRiemannHll : chl::kRiemannHllc, dx, dt_dev, dt_host, gamma,
+                            SolverKind(c.riemann), dx, dt_dev, dt_host, gamma,
                             fuse_dt ? h->dti() : nullptr, h->stream));
     if (update_grid && !fuse_dt) {
       CHL_HIP(hipMemsetAsync(h->dti(), 0, 8, h->stream));
@@ -330,7 +335,7 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return Fail(CHL_ERR_NO_DEVICE, "no HIP device");
   chl_hydro_config c = *cfg;
   if (c.tile1 < 0) c.tile1 = 0;
-  const bool narrow = c.dual_energy || c.riemann == CHL_RIEMANN_HLL || c.reconstruction == CHL_RECON_PCM ||
+  const bool narrow = c.dual_energy || c.riemann != CHL_RIEMANN_HLLC || c.reconstruction == CHL_RECON_PCM ||
                      c.reconstruction == CHL_RECON_PLMP;  // kernels built for tile shapes 0 and 1 only
   if (c.tile2 < 0) c.tile2 = narrow ? 0 : 3;
   CHL_TRY(CheckConfig(c));
@@ -537,7 +542,7 @@ int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, doub
                                      (c.reconstruction == CHL_RECON_PCM ? chl::kFlagConservedW : 0));
   const int R1     = ReconRadius(c.reconstruction) + 1;
   const int nz1    = c.nz - 2 * c.n_ghost + 2 * R1, nz2 = c.nz - 2 * c.n_ghost;
-  const bool narrow = c.dual_energy || c.riemann == CHL_RIEMANN_HLL || c.reconstruction == CHL_RECON_PCM ||
+  const bool narrow = c.dual_energy || c.riemann != CHL_RIEMANN_HLLC || c.reconstruction == CHL_RECON_PCM ||
                      c.reconstruction == CHL_RECON_PLMP;
   const int ntiles  = narrow ? chl::kNumDeTiles : chl::kNumTiles;
   std::vector<int> chunks = {8, 16, 32, 64, 128};
@@ -724,6 +729,14 @@ int chl_hll_fluxes(const double *QL, const double *QR, double *F, int nx, int ny
 {
   if (!QL || !QR || !F || dir < 0 || dir > 2) return Fail(CHL_ERR_INVALID, "bad argument");
   CHL_HIP(chl::LaunchRiemannGrid(chl::kRiemannHll, QL, QR, F, nx, ny, nz, gamma, dir, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_roe_fluxes(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma, int dir)
+{
+  if (!QL || !QR || !F || dir < 0 || dir > 2) return Fail(CHL_ERR_INVALID, "bad argument");
+  CHL_HIP(chl::LaunchRiemannGrid(chl::kRiemannRoe, QL, QR, F, nx, ny, nz, gamma, dir, nullptr));
   CHL_HIP(hipDeviceSynchronize());
   return CHL_OK;
 }

@@ -27,6 +27,8 @@ constexpr int kRecon = CHL_RECON_PLMC;  // builds/make.type.hydro:5 default
 #endif
 #if defined(HLL)
 constexpr int kRiemann = CHL_RIEMANN_HLL;  // -DHLL in place of -DHLLC
+#elif defined(ROE)
+constexpr int kRiemann = CHL_RIEMANN_ROE;
 #else
 constexpr int kRiemann = CHL_RIEMANN_HLLC;
 #endif

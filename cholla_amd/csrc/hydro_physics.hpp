@@ -35,6 +35,7 @@ constexpr double kDeEta2 = 0.035;
 // Riemann solvers (the reference's -DHLLC / -DHLL build options).
 constexpr int kRiemannHllc = 0;
 constexpr int kRiemannHll  = 1;
+constexpr int kRiemannRoe  = 2;
 
 struct PolicyExact {
   using T                      = double;
@@ -54,11 +55,15 @@ struct PolicyMixed {
   static constexpr int kId     = 2;
   static constexpr int kSolver = kRiemannHllc;
 };
-// An arithmetic form with the HLL solver in place of HLLC; everything else as Base.
-template <class Base>
-struct HllPolicy : Base {
-  static constexpr int kSolver = kRiemannHll;
+// An arithmetic form with another Riemann solver in place of HLLC; everything else as Base.
+template <class Base, int S>
+struct SolverPolicy : Base {
+  static constexpr int kSolver = S;
 };
+template <class Base>
+using HllPolicy = SolverPolicy<Base, kRiemannHll>;
+template <class Base>
+using RoePolicy = SolverPolicy<Base, kRiemannRoe>;
 
 template <typename T>
 struct Prim {
@@ -733,7 +738,83 @@ __device__ __forceinline__ Cons<typename P::T> Hll(const RState<typename P::T> &
   return f;
 }
 
-// The solver of the form P (HLLC unless P is an HllPolicy).
+// Calculate_Roe_Fluxes_CUDA body, src/riemann_solvers/roe_cuda.cu:160-379 (hydro; its etah
+// is 0, so each eigenvalue enters as |lambda|): upwind where the Roe-averaged eigenvalues are
+// one-signed, else the Roe flux, or HLLE with Einfeldt speeds where an intermediate state of
+// the linearised solution has non-positive density or negative pressure.  The second of
+// those tests adds the fourth wave's coefficient to the density only (roe_cuda.cu:330-334);
+// it is kept so, as the switch decides which flux comes out.  Expressions keep the
+// reference's association so PolicyExact reproduces its bits.
+template <class P>
+__device__ __forceinline__ Cons<typename P::T> Roe(const RState<typename P::T> &l, const RState<typename P::T> &r,
+                                                   typename P::T gamma)
+{
+  using T    = typename P::T;
+  const T g1 = gamma - T(1.0);
+  const T Hl = Div1<P>(l.E + l.p, l.d), Hr = Div1<P>(r.E + r.p, r.d);
+  const T sl = Sqrt<P>(l.d), sr = Sqrt<P>(r.d);
+  const T inv_s = Rcp<P>(sl + sr);
+  const T vx  = Div<P>(sl * l.vx + sr * r.vx, sl + sr, inv_s);
+  const T vy  = Div<P>(sl * l.vy + sr * r.vy, sl + sr, inv_s);
+  const T vz  = Div<P>(sl * l.vz + sr * r.vz, sl + sr, inv_s);
+  const T H   = Div<P>(sl * Hl + sr * Hr, sl + sr, inv_s);
+  const T vsq = (vx * vx + vy * vy + vz * vz);
+  const T asq = g1 * Fmax((H - T(0.5) * vsq), T(kTinyNumber));
+  const T a   = Sqrt<P>(asq);
+  const T lm = vx - a, l0 = vx, lp = vx + a;
+  Cons<T> fl{l.mx, l.mx * l.vx + l.p, l.mx * l.vy, l.mx * l.vz, (l.E + l.p) * l.vx};
+  Cons<T> fr{r.mx, r.mx * r.vx + r.p, r.mx * r.vy, r.mx * r.vz, (r.E + r.p) * r.vx};
+  if (lm >= T(0.0)) return fl;
+  if (lp <= T(0.0)) return fr;
+  const T dd = r.d - l.d, dmx = r.mx - l.mx, dmy = r.my - l.my, dmz = r.mz - l.mz, dE = r.E - l.E;
+  const T Na = Div1<P>(T(0.5), asq);
+  const T inv_asq = Rcp<P>(asq);
+  T s0 = T(0.0), s1 = T(0.0), s2 = T(0.0), s3 = T(0.0), s4 = T(0.0);
+  const T a0 = dd * Na * (T(0.5) * g1 * vsq + vx * a) - dmx * Na * (g1 * vx + a) - dmy * Na * g1 * vy -
+               dmz * Na * g1 * vz + dE * Na * g1;
+  T c = a0 * Fabs(lm);
+  s0 += c, s1 += c * (vx - a), s2 += c * vy, s3 += c * vz, s4 += c * (H - vx * a);
+  const T a1 = -dd * vy + dmy;
+  c          = a1 * Fabs(l0);
+  s2 += c, s4 += c * vy;
+  const T a2 = -dd * vz + dmz;
+  c          = a2 * Fabs(l0);
+  s3 += c, s4 += c * vz;
+  const T a3 = dd * (T(1.0) - Na * g1 * vsq) + Div<P>(dmx * g1 * vx, asq, inv_asq) + Div<P>(dmy * g1 * vy, asq, inv_asq) +
+               Div<P>(dmz * g1 * vz, asq, inv_asq) - Div<P>(dE * g1, asq, inv_asq);
+  c = a3 * Fabs(l0);
+  s0 += c, s1 += c * vx, s2 += c * vy, s3 += c * vz, s4 += c * T(0.5) * vsq;
+  const T a4 = dd * Na * (T(0.5) * g1 * vsq - vx * a) - dmx * Na * (g1 * vx - a) - dmy * Na * g1 * vy -
+               dmz * Na * g1 * vz + dE * Na * g1;
+  c = a4 * Fabs(lp);
+  s0 += c, s1 += c * (vx + a), s2 += c * vy, s3 += c * vz, s4 += c * (H + vx * a);
+  bool hlle = false;
+  T t0 = l.d + a0, t1 = l.mx + a0 * (vx - a), t2 = l.my + a0 * vy, t3 = l.mz + a0 * vz, t4 = l.E + a0 * (H - vx * a);
+  if (l0 > lm) hlle = hlle || t0 <= T(0.0) || t4 - Div1<P>(T(0.5) * (t1 * t1 + t2 * t2 + t3 * t3), t0) < T(0.0);
+  t0 += a3 + a4, t1 += a3 * vx, t2 += a1 + a3 * vy, t3 += a2 + a3 * vz, t4 += a1 * vy + a2 * vz + a3 * T(0.5) * vsq;
+  if (lp > l0) hlle = hlle || t0 <= T(0.0) || t4 - Div1<P>(T(0.5) * (t1 * t1 + t2 * t2 + t3 * t3), t0) < T(0.0);
+  Cons<T> f;
+  if (hlle) {
+    const T cfl = Sqrt<P>(Div1<P>(gamma * l.p, l.d)), cfr = Sqrt<P>(Div1<P>(gamma * r.p, r.d));
+    const T bm = Fmin(Fmin(lm, l.vx - cfl), T(0.0)), bp = Fmax(Fmax(lp, r.vx + cfr), T(0.0));
+    const T tmp = Div1<P>(T(0.5) * (bp + bm), bp - bm);
+    auto hlle_avg = [&](T hl, T hr) { return T(0.5) * (hl + hr) + (hl - hr) * tmp; };
+    f.d  = hlle_avg(l.mx - bm * l.d, r.mx - bp * r.d);
+    f.mx = hlle_avg(l.mx * (l.vx - bm) + l.p, r.mx * (r.vx - bp) + r.p);
+    f.my = hlle_avg(l.my * (l.vx - bm), r.my * (r.vx - bp));
+    f.mz = hlle_avg(l.mz * (l.vx - bm), r.mz * (r.vx - bp));
+    f.E  = hlle_avg(l.E * (l.vx - bm) + l.p * l.vx, r.E * (r.vx - bp) + r.p * r.vx);
+  } else {
+    f.d  = T(0.5) * (fl.d + fr.d - s0);
+    f.mx = T(0.5) * (fl.mx + fr.mx - s1);
+    f.my = T(0.5) * (fl.my + fr.my - s2);
+    f.mz = T(0.5) * (fl.mz + fr.mz - s3);
+    f.E  = T(0.5) * (fl.E + fr.E - s4);
+  }
+  return f;
+}
+
+// The solver of the form P (HLLC unless P is a SolverPolicy).
 template <class P, bool DE = false>
 __device__ __forceinline__ Cons<typename P::T> RiemannFlux(const RState<typename P::T> &l,
                                                            const RState<typename P::T> &r, typename P::T gamma,
@@ -741,6 +822,9 @@ __device__ __forceinline__ Cons<typename P::T> RiemannFlux(const RState<typename
 {
   if constexpr (P::kSolver == kRiemannHll) {
     return Hll<P, DE>(l, r, gamma, fge);
+  } else if constexpr (P::kSolver == kRiemannRoe) {
+    static_assert(!DE, "the Roe solver is built without dual energy");
+    return Roe<P>(l, r, gamma);
   } else {
     return Hllc<P, DE>(l, r, gamma, fge);
   }

@@ -50,14 +50,24 @@ hipError_t LaunchStage2T(const StepArgs &a, hipStream_t s)
 // Dual-energy, HLL, PCM and PLMP kernels are compiled for the first kNumDeTiles tile shapes only.
 constexpr int kNumDeTiles = 2;
 
+// fn(Q{}) with Q = P carrying the solver `riemann` selects (HLLC: P itself).
+template <class P, class Fn>
+hipError_t WithSolver(int riemann, Fn &&fn)
+{
+  if (riemann == kRiemannHll) return fn(HllPolicy<P>{});
+  if (riemann == kRiemannRoe) return fn(RoePolicy<P>{});
+  return fn(P{});
+}
+
 template <class P>
 hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 {
-  if (a.riemann == kRiemannHll) {  // HLL is compiled for the first two tile shapes, without DE
-    if (a.de) return hipErrorInvalidValue;
-    if (tile == 0) return LaunchStage1T<HllPolicy<P>, 32, 8>(a, R1, s);
-    if (tile == 1) return LaunchStage1T<HllPolicy<P>, 16, 16>(a, R1, s);
-    return hipErrorInvalidValue;
+  if (a.riemann != kRiemannHllc) {  // HLL and Roe: the first two tile shapes, without DE
+    if (a.de || tile > 1) return hipErrorInvalidValue;
+    return WithSolver<P>(a.riemann, [&](auto q) {
+      using Q = decltype(q);
+      return tile == 0 ? LaunchStage1T<Q, 32, 8>(a, R1, s) : LaunchStage1T<Q, 16, 16>(a, R1, s);
+    });
   }
   if (a.de) {
     switch (tile) {
@@ -86,41 +96,41 @@ hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 template <class P, int RK>
 hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
 {
-  if constexpr (RK == kPCM || RK == kPLMP) {  // built for the first two tile shapes, without DE
-    if (a.de) return hipErrorInvalidValue;
-    const bool hll = a.riemann == kRiemannHll;
-    if (tile == 0) return hll ? LaunchStage2T<HllPolicy<P>, RK, 32, 8>(a, s) : LaunchStage2T<P, RK, 32, 8>(a, s);
-    if (tile == 1) return hll ? LaunchStage2T<HllPolicy<P>, RK, 16, 16>(a, s) : LaunchStage2T<P, RK, 16, 16>(a, s);
-    return hipErrorInvalidValue;
-  }
-  if (a.riemann == kRiemannHll) {
-    if (a.de) return hipErrorInvalidValue;
-    if (tile == 0) return LaunchStage2T<HllPolicy<P>, RK, 32, 8>(a, s);
-    if (tile == 1) return LaunchStage2T<HllPolicy<P>, RK, 16, 16>(a, s);
-    return hipErrorInvalidValue;
-  }
-  if (a.de) {
+  // PCM, PLMP, HLL and Roe: the first two tile shapes, without DE
+  auto narrow = [&]() {
+    if (a.de || tile > 1) return hipErrorInvalidValue;
+    return WithSolver<P>(a.riemann, [&](auto q) {
+      using Q = decltype(q);
+      return tile == 0 ? LaunchStage2T<Q, RK, 32, 8>(a, s) : LaunchStage2T<Q, RK, 16, 16>(a, s);
+    });
+  };
+  if constexpr (RK == kPCM || RK == kPLMP) {
+    return narrow();
+  } else {
+    if (a.riemann != kRiemannHllc) return narrow();
+    if (a.de) {
+      switch (tile) {
+        case 0:
+          return LaunchStage2T<P, RK, 32, 8, true>(a, s);
+        case 1:
+          return LaunchStage2T<P, RK, 16, 16, true>(a, s);
+      }
+      return hipErrorInvalidValue;
+    }
     switch (tile) {
       case 0:
-        return LaunchStage2T<P, RK, 32, 8, true>(a, s);
+        return LaunchStage2T<P, RK, 32, 8>(a, s);
       case 1:
-        return LaunchStage2T<P, RK, 16, 16, true>(a, s);
+        return LaunchStage2T<P, RK, 16, 16>(a, s);
+      case 2:
+        return LaunchStage2T<P, RK, 64, 4>(a, s);
+      case 3:
+        return LaunchStage2T<P, RK, 16, 8>(a, s);
+      case 4:
+        return LaunchStage2T<P, RK, 32, 16>(a, s);
     }
     return hipErrorInvalidValue;
   }
-  switch (tile) {
-    case 0:
-      return LaunchStage2T<P, RK, 32, 8>(a, s);
-    case 1:
-      return LaunchStage2T<P, RK, 16, 16>(a, s);
-    case 2:
-      return LaunchStage2T<P, RK, 64, 4>(a, s);
-    case 3:
-      return LaunchStage2T<P, RK, 16, 8>(a, s);
-    case 4:
-      return LaunchStage2T<P, RK, 32, 16>(a, s);
-  }
-  return hipErrorInvalidValue;
 }
 
 template <class P>

@@ -1,16 +1,20 @@
-"""GPU parity of the HLL Riemann solver option (the reference's -DHLL build,
-Calculate_HLL_Fluxes_CUDA, src/riemann_solvers/hll_cuda.cu:19-268; chl_hydro_config.riemann
-= CHL_RIEMANN_HLL) against the oracle's HLL (tests/test_oracle_hll.py pins it).
+"""GPU parity of the HLL and Roe Riemann solver options (the reference's -DHLL and -DROE
+builds: Calculate_HLL_Fluxes_CUDA, src/riemann_solvers/hll_cuda.cu:19-268, and
+Calculate_Roe_Fluxes_CUDA, src/riemann_solvers/roe_cuda.cu:19-400; chl_hydro_config.riemann
+= CHL_RIEMANN_HLL / CHL_RIEMANN_ROE) against the oracle's solvers (tests/test_oracle_hll.py
+and tests/test_oracle_roe.py pin them).
 
 Bars, as for HLLC:
-  * the component entry chl_hll_fluxes is bit-identical to the oracle on random interface
-    pairs in all three directions;
-  * PolicyExact: a VL step and a 3-step KH run with PLMC and PPMC are bit-identical to the
-    oracle with HLL (and differ from the HLLC run, so the switch is live); PolicyFast and
-    the mixed form within per-field relative L1 1e-12 and scaled L1 1e-5;
-  * 1D Sod with PLMC + HLL bit-identical to the oracle's 1D driver.  The reference's
-    VL_1D integrator has no HLL branch (VL_1D_cuda.cu:69-124 names Exact, Roe and HLLC
-    only); this 1D form follows the oracle's VL_1D with the solver swapped.
+  * the component entries chl_hll_fluxes / chl_roe_fluxes are bit-identical to the oracle
+    on random interface pairs in all three directions (for Roe, pairs far enough apart
+    that the HLLE fallback fires on some faces);
+  * PolicyExact: a VL step and a graph-captured KH run with PLMC and PPMC are
+    bit-identical to the oracle with the solver (and differ from the HLLC run, so the
+    switch is live); PolicyFast and the mixed form within per-field relative L1 1e-12
+    and scaled L1 1e-5;
+  * 1D Sod with PLMC bit-identical to the oracle's 1D driver.  The reference's VL_1D
+    integrator has no HLL branch (VL_1D_cuda.cu:69-124 names Exact, Roe and HLLC only);
+    the 1D HLL form follows the oracle's VL_1D with the solver swapped.
 """
 import sys
 
@@ -23,6 +27,9 @@ from oracle import oracle as orc
 from test_gpu_parity import TOL, _random_periodic_box, field_l1, scaled_l1
 
 pytestmark = pytest.mark.gpu
+# (C-ABI solver, oracle solver, component entry, oracle component)
+SOLVERS = {"hll": (chl.RIEMANN_HLL, orc.HLL, chl.hll_fluxes, "orc_hll"),
+           "roe": (chl.RIEMANN_ROE, orc.ROE, chl.roe_fluxes, "orc_roe")}
 
 
 def say(*a):
@@ -36,41 +43,45 @@ def _require_gpu():
         pytest.fail("no HIP device visible: the GPU tests need a real MI355X")
 
 
-def test_hll_component_bitwise_vs_oracle():
+@pytest.mark.parametrize("name", list(SOLVERS))
+def test_component_bitwise_vs_oracle(name):
+    solver, _, entry, orc_entry = SOLVERS[name]
     rng = np.random.default_rng(11)
     nx, ny, nz = 9, 8, 7
     nc = nx * ny * nz
     g = 5 / 3
     d = rng.uniform(0.2, 3, nc)
-    v = rng.uniform(-3, 3, (3, nc))  # subsonic and supersonic faces both occur
+    v = rng.uniform(-3, 3, (3, nc))  # subsonic and supersonic faces; receding pairs for Roe's HLLE
     p = rng.uniform(0.1, 4, nc)
     U = np.concatenate([d, d * v[0], d * v[1], d * v[2], p / (g - 1) + 0.5 * d * (v ** 2).sum(0)])
     L2 = U.copy()
     R2 = np.roll(U.reshape(5, nc), 1, axis=1).reshape(-1).copy()
     for dr in range(3):
-        say("hll component dir", dr)
+        say(name, "component dir", dr)
         Fo = np.zeros(5 * nc)
-        orc.lib().orc_hll(orc.ptr(L2), orc.ptr(R2), orc.ptr(Fo), nx, ny, nz, g, dr)
+        getattr(orc.lib(), orc_entry)(orc.ptr(L2), orc.ptr(R2), orc.ptr(Fo), nx, ny, nz, g, dr)
         F = chl.DeviceArray(5 * nc, np.zeros(5 * nc))
-        chl.hll_fluxes(chl.DeviceArray.from_host(L2), chl.DeviceArray.from_host(R2), F, nx, ny, nz, g, dr)
+        entry(chl.DeviceArray.from_host(L2), chl.DeviceArray.from_host(R2), F, nx, ny, nz, g, dr)
         np.testing.assert_array_equal(F.download(), Fo)
 
 
+@pytest.mark.parametrize("name", list(SOLVERS))
 @pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
 @pytest.mark.parametrize("recon,ng", [(chl.RECON_PLMC, 3), (chl.RECON_PPMC, 4)])
-def test_single_vl_step_hll(recon, ng, precision):
+def test_single_vl_step(recon, ng, precision, name):
+    solver, orc_solver = SOLVERS[name][:2]
     n = 20
     U, N, g = _random_periodic_box(n, ng, seed=9)
     h = 1.0 / n
     dt = 0.3 / orc.calc_dti(U, N, N, N, ng, h, h, h, g)
-    say("hll vl_step recon", recon, "precision", precision, "N", N)
+    say(name, "vl_step recon", recon, "precision", precision, "N", N)
     Uo, Uc = U.copy(), U.copy()
-    with orc.riemann(orc.HLL):
+    with orc.riemann(orc_solver):
         orc.lib().orc_vl3d(orc.ptr(Uo), N, N, N, ng, h, h, h, dt, g, 5, recon, 1, 0.0)
     orc.lib().orc_vl3d(orc.ptr(Uc), N, N, N, ng, h, h, h, dt, g, 5, recon, 1, 0.0)
     assert not np.array_equal(Uo, Uc)
     Ud = chl.DeviceArray.from_host(U)
-    hy = chl.Hydro(N, N, N, ng, recon, precision, riemann=chl.RIEMANN_HLL)
+    hy = chl.Hydro(N, N, N, ng, recon, precision, riemann=solver)
     hy.vl_step(Ud, h, h, h, dt, g)
     A = Ud.download()
     hy.close()
@@ -81,18 +92,20 @@ def test_single_vl_step_hll(recon, ng, precision):
         assert mx == 0.0, f"exact form differs from the oracle by {mx}"
 
 
+@pytest.mark.parametrize("name", list(SOLVERS))
 @pytest.mark.parametrize("tile", [0, 1])
 @pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
-def test_kelvin_helmholtz_hll(precision, tile):
+def test_kelvin_helmholtz(precision, tile, name):
+    solver, orc_solver = SOLVERS[name][:2]
     n, ng, g = 24, 4, 5 / 3
     N = n + 2 * ng
     U = chl.ic_kelvin_helmholtz(n, ng, g, seed=5)
     h = 1.0 / n
     Uo = U.copy()
-    steps_o, to = orc.run(Uo, N, N, N, ng, 5, h, h, h, g, orc.PPMC, [1] * 6, 0.0, 1e30, 6, solver=orc.HLL)
-    say("hll KH 24^3 x6 precision", precision, "tile", tile)
+    steps_o, to = orc.run(Uo, N, N, N, ng, 5, h, h, h, g, orc.PPMC, [1] * 6, 0.0, 1e30, 6, solver=orc_solver)
+    say(name, "KH 24^3 x6 precision", precision, "tile", tile)
     Ud = chl.DeviceArray.from_host(U)
-    hy = chl.Hydro(N, N, N, ng, chl.RECON_PPMC, precision, riemann=chl.RIEMANN_HLL, tile1=tile, tile2=tile)
+    hy = chl.Hydro(N, N, N, ng, chl.RECON_PPMC, precision, riemann=solver, tile1=tile, tile2=tile)
     steps, t = hy.run(Ud, h, h, h, g, 0.0, 1e30, 6, use_graph=True)
     hy.close()
     assert steps == steps_o == 6
@@ -107,17 +120,18 @@ def test_kelvin_helmholtz_hll(precision, tile):
         assert max(sl1) <= TOL[precision], sl1
 
 
-def test_sod_1d_hll():
+@pytest.mark.parametrize("name", list(SOLVERS))
+def test_sod_1d(name):
+    solver, orc_solver = SOLVERS[name][:2]
     U, nx, dx = sod_ic()
     ng = 3
     orc.set_boundaries(U, nx, 1, 1, 5, ng, [3, 3, 0, 0, 0, 0])
     Uo = U.copy()
     steps_o, to = orc.run(Uo, nx, 1, 1, ng, 5, dx, dx, dx, 1.4, orc.PLMC, [3, 3, 0, 0, 0, 0], 0.0, 0.2, 100000,
-                          solver=orc.HLL)
-    say("Sod 1D HLL")
+                          solver=orc_solver)
+    say("Sod 1D", name)
     Ud = chl.DeviceArray.from_host(U)
-    hy = chl.Hydro(nx, 1, 1, ng, chl.RECON_PLMC, chl.PREC_FP64_EXACT, bc=[3, 3, 1, 1, 1, 1],
-                   riemann=chl.RIEMANN_HLL)
+    hy = chl.Hydro(nx, 1, 1, ng, chl.RECON_PLMC, chl.PREC_FP64_EXACT, bc=[3, 3, 1, 1, 1, 1], riemann=solver)
     steps, t = hy.run(Ud, dx, dx, dx, 1.4, 0.0, 0.2, 100000, use_graph=False)
     hy.close()
     assert steps == steps_o and t == to
@@ -125,8 +139,10 @@ def test_sod_1d_hll():
     np.testing.assert_array_equal(A[:, ng:nx - ng], Uo.reshape(5, nx)[:, ng:nx - ng])
 
 
-def test_hll_option_limits():
+@pytest.mark.parametrize("name", list(SOLVERS))
+def test_option_limits(name):
+    solver = SOLVERS[name][0]
     with pytest.raises(chl.ChollaError, match="dual energy"):
-        chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, riemann=chl.RIEMANN_HLL, dual_energy=True)
+        chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, riemann=solver, dual_energy=True)
     with pytest.raises(chl.ChollaError, match="tile shapes 0 and 1"):
-        chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, riemann=chl.RIEMANN_HLL, tile2=3)
+        chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, riemann=solver, tile2=3)
