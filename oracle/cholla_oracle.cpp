@@ -827,6 +827,147 @@ void orc_roe(const double *QL, const double *QR, double *F, int nx, int ny, int 
   }
 }
 
+// powf as exact_cuda.cu calls it on doubles: both arguments narrowed to float and an FP32
+// result.  Evaluated as the correctly rounded float of the double pow of the narrowed
+// arguments, so any conforming powf differs from it by float rounding only.
+static double PowF(double x, double y) { return double(float(std::pow(double(float(x)), double(float(y))))); }
+
+// guessp_CUDA, exact_cuda.cu:137-158: the PVRS guess, then the two-shock guess
+static double ExactGuess(double dl, double vxl, double pl, double cl, double dr, double vxr, double pr, double cr,
+                         double gamma)
+{
+  const double TOL = 1.0e-6;
+  double ppv       = 0.5 * (pl + pr) + 0.125 * (vxl - vxr) * (dl + dr) * (cl + cr);
+  if (ppv < 0.0) ppv = 0.0;
+  double gl = std::sqrt((2.0 / ((gamma + 1.0) * dl)) / (((gamma - 1.0) / (gamma + 1.0)) * pl + ppv));
+  double gr = std::sqrt((2.0 / ((gamma + 1.0) * dr)) / (((gamma - 1.0) / (gamma + 1.0)) * pr + ppv));
+  double pm = (gl * pl + gr * pr - (vxr - vxl)) / (gl + gr);
+  return pm < 0.0 ? TOL : pm;
+}
+
+// prefun_CUDA, exact_cuda.cu:160-178: the pressure function of one side and its derivative
+static void ExactSide(double *f, double *fd, double p, double dk, double pk, double ck, double gamma)
+{
+  if (p <= pk) {
+    *f  = (2.0 / (gamma - 1.0)) * ck * (PowF(p / pk, (gamma - 1.0) / (2.0 * gamma)) - 1.0);
+    *fd = (1.0 / (dk * ck)) * PowF((p / pk), -((gamma + 1.0) / (2.0 * gamma)));
+  } else {
+    double qrt = std::sqrt(((2.0 / (gamma + 1.0)) / dk) / ((((gamma - 1.0) / (gamma + 1.0)) * pk) + p));
+    *f         = (p - pk) * qrt;
+    *fd        = (1.0 - 0.5 * (p - pk) / ((((gamma - 1.0) / (gamma + 1.0)) * pk) + p)) * qrt;
+  }
+}
+
+// starpv_CUDA, exact_cuda.cu:180-215: Newton iteration for the star pressure (at most 21
+// updates, relative change <= 1e-6), then the star velocity
+static void ExactStar(double *p, double *v, double dl, double vxl, double pl, double cl, double dr, double vxr,
+                      double pr, double cr, double gamma)
+{
+  const int nriter = 20;
+  const double TOL = 1.0e-6;
+  double fl = 0, fld = 0, fr = 0, frd = 0;
+  double pold = ExactGuess(dl, vxl, pl, cl, dr, vxr, pr, cr, gamma);
+  for (int i = 0; i <= nriter; i++) {
+    ExactSide(&fl, &fld, pold, dl, pl, cl, gamma);
+    ExactSide(&fr, &frd, pold, dr, pr, cr, gamma);
+    *p            = pold - (fl + fr + vxr - vxl) / (fld + frd);
+    double change = 2.0 * std::fabs((*p - pold) / (*p + pold));
+    if (change <= TOL) break;
+    if (*p < 0.0) *p = TOL;
+    pold = *p;
+  }
+  *v = 0.5 * (vxl + vxr + fr - fl);
+}
+
+// sample_CUDA, exact_cuda.cu:217-334: the solution on the interface (x/t = 0)
+static void ExactSample(double pm, double vm, double *d, double *v, double *p, double dl, double vxl, double pl,
+                        double cl, double dr, double vxr, double pr, double cr, double gamma)
+{
+  const double gm1 = gamma - 1.0, gp1 = gamma + 1.0;
+  if (vm >= 0) {
+    if (pm <= pl) {  // left rarefaction
+      if (vxl - cl >= 0) {
+        *d = dl, *v = vxl, *p = pl;
+      } else if (vm - cl * PowF(pm / pl, gm1 / (2.0 * gamma)) < 0) {
+        *d = dl * PowF(pm / pl, 1.0 / gamma), *v = vm, *p = pm;
+      } else {
+        double c = (2.0 / gp1) * (cl + (gm1 / 2.0) * vxl);
+        *v       = c;
+        *d       = dl * PowF(c / cl, 2.0 / gm1);
+        *p       = pl * PowF(c / cl, 2.0 * gamma / gm1);
+      }
+    } else {  // left shock
+      double sl = vxl - cl * std::sqrt((gp1 / (2.0 * gamma)) * (pm / pl) + (gm1 / (2.0 * gamma)));
+      if (sl >= 0) {
+        *d = dl, *v = vxl, *p = pl;
+      } else {
+        *d = dl * (pm / pl + (gm1 / gp1)) / ((pm / pl) * (gm1 / gp1) + 1.0), *v = vm, *p = pm;
+      }
+    }
+  } else {
+    if (pm > pr) {  // right shock
+      double sr = vxr + cr * std::sqrt((gp1 / (2.0 * gamma)) * (pm / pr) + (gm1 / (2.0 * gamma)));
+      if (sr <= 0) {
+        *d = dr, *v = vxr, *p = pr;
+      } else {
+        *d = dr * (pm / pr + (gm1 / gp1)) / ((pm / pr) * (gm1 / gp1) + 1.0), *v = vm, *p = pm;
+      }
+    } else {  // right rarefaction
+      if (vxr + cr <= 0) {
+        *d = dr, *v = vxr, *p = pr;
+      } else if (vm + cr * PowF(pm / pr, gm1 / (2.0 * gamma)) >= 0) {
+        *d = dr * PowF(pm / pr, (1.0 / gamma)), *v = vm, *p = pm;
+      } else {
+        double c = (2.0 / gp1) * (cr - (gm1 / 2.0) * vxr);
+        *v       = -c;
+        *d       = dr * PowF(c / cr, 2.0 / gm1);
+        *p       = pr * PowF(c / cr, 2.0 * gamma / gm1);
+      }
+    }
+  }
+}
+
+// Calculate_Exact_Fluxes_CUDA, src/riemann_solvers/exact_cuda.cu:20-135 (no scalars): the
+// Godunov flux of the exact solution sampled on the interface; transverse velocities (and
+// the specific GasEnergy) upwinded with the sampled normal velocity.
+void orc_exact(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma, int dir)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  int o1, o2, o3;
+  DirOrder(dir, o1, o2, o3);
+#pragma omp parallel for schedule(static)
+  for (long t = 0; t < long(n_cells); t++) {
+    const double dl = QL[t], vxl = QL[o1 * n_cells + t] / dl, vyl = QL[o2 * n_cells + t] / dl,
+                 vzl = QL[o3 * n_cells + t] / dl;
+    const double dr = QR[t], vxr = QR[o1 * n_cells + t] / dr, vyr = QR[o2 * n_cells + t] / dr,
+                 vzr = QR[o3 * n_cells + t] / dr;
+    const double El = QL[4 * n_cells + t], Er = QR[4 * n_cells + t];
+    const double dgel = g_de ? QL[kGE * n_cells + t] : 0.0, dger = g_de ? QR[kGE * n_cells + t] : 0.0;
+    double pl, pr;
+    if (g_de) {
+      pl = PressureFromDE(El, El - 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl), dgel, gamma);
+      pr = PressureFromDE(Er, Er - 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr), dger, gamma);
+    } else {
+      pl = (El - 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl)) * (gamma - 1.0);
+      pr = (Er - 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr)) * (gamma - 1.0);
+    }
+    pl = std::fmax(pl, kTiny), pr = std::fmax(pr, kTiny);
+    const double cl = std::sqrt(gamma * pl / dl), cr = std::sqrt(gamma * pr / dr);
+    double pm, vm, ds, vs, ps;
+    ExactStar(&pm, &vm, dl, vxl, pl, cl, dr, vxr, pr, cr, gamma);
+    ExactSample(pm, vm, &ds, &vs, &ps, dl, vxl, pl, cl, dr, vxr, pr, cr, gamma);
+    const bool left = vs >= 0;
+    const double vy = left ? vyl : vyr, vz = left ? vzl : vzr, ge = left ? dgel / dl : dger / dr;
+    const double Es = (ps / (gamma - 1.0)) + 0.5 * ds * (vs * vs + vy * vy + vz * vz);
+    F[t]                = ds * vs;
+    F[o1 * n_cells + t] = ds * vs * vs + ps;
+    F[o2 * n_cells + t] = ds * vs * vy;
+    F[o3 * n_cells + t] = ds * vs * vz;
+    F[4 * n_cells + t]  = (Es + ps) * vs;
+    if (g_de) F[kGE * n_cells + t] = ds * vs * ge;
+  }
+}
+
 // The solver the build selects (g_riemann).
 static void RiemannFluxes(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
                           int dir)
@@ -835,6 +976,8 @@ static void RiemannFluxes(const double *QL, const double *QR, double *F, int nx,
     orc_hll(QL, QR, F, nx, ny, nz, gamma, dir);
   } else if (g_riemann == 2) {
     orc_roe(QL, QR, F, nx, ny, nz, gamma, dir);
+  } else if (g_riemann == 3) {
+    orc_exact(QL, QR, F, nx, ny, nz, gamma, dir);
   } else {
     orc_hllc(QL, QR, F, nx, ny, nz, gamma, dir);
   }
@@ -1367,7 +1510,7 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
 // The DE build option (dual energy) as a run-time switch for every stage above.
 void orc_set_dual_energy(int on) { g_de = on ? 1 : 0; }
 void orc_set_positivity_fallback(int on) { g_pos_fallback = on ? 1 : 0; }
-void orc_set_riemann(int solver) { g_riemann = (solver == 1 || solver == 2) ? solver : 0; }
+void orc_set_riemann(int solver) { g_riemann = (solver >= 1 && solver <= 3) ? solver : 0; }
 int orc_get_dual_energy() { return g_de; }
 
 // hydro_utilities::Get_Pressure_From_DE, for the known-answer tests

@@ -51,6 +51,7 @@ def lib():
             "orc_plmc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_plmp": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_roe": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_exact": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_ppmc_vl": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_hllc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_update_half_3d": (None, [_dp] * 5 + [i, i, i, d, d, d, d, d, i, i, d]),
@@ -142,12 +143,12 @@ def positivity_fallback(on=True):
         lib().orc_set_positivity_fallback(0)
 
 
-HLLC, HLL, ROE = 0, 1, 2
+HLLC, HLL, ROE, EXACT = 0, 1, 2, 3
 
 
 @contextlib.contextmanager
 def riemann(solver=HLLC):
-    """The Riemann solver build option (HLLC, HLL or ROE) for the calls inside the block."""
+    """The Riemann solver build option (HLLC, HLL, ROE or EXACT) for the calls inside the block."""
     lib().orc_set_riemann(int(solver))
     try:
         yield
