@@ -19,6 +19,7 @@ RECON_PCM, RECON_PLMC, RECON_PPMC, RECON_PLMP = 0, 1, 2, 3
 RIEMANN_HLLC = 0
 RIEMANN_HLL = 1
 RIEMANN_ROE = 2
+RIEMANN_EXACT = 3
 PREC_FP64_EXACT, PREC_FP64, PREC_MIXED = 0, 1, 2
 PRECISIONS = {"fp64_exact": PREC_FP64_EXACT, "fp64": PREC_FP64, "mixed": PREC_MIXED}
 BC_PERIODIC, BC_REFLECTIVE, BC_TRANSMISSIVE = 1, 2, 3
@@ -30,7 +31,7 @@ EXPORTS = [
     "chl_hydro_destroy", "chl_hydro_vl_step", "chl_hydro_update_grid", "chl_hydro_run", "chl_hydro_time_steps",
     "chl_hydro_kernel_times", "chl_hydro_autotune", "chl_hydro_get_config", "chl_calc_dt",
     "chl_apply_temperature_floor", "chl_set_ghost_cells", "chl_set_boundaries", "chl_reconstruct",
-    "chl_hllc_fluxes", "chl_hll_fluxes", "chl_roe_fluxes", "chl_ic_kelvin_helmholtz", "chl_ensemble_create", "chl_ensemble_destroy",
+    "chl_hllc_fluxes", "chl_hll_fluxes", "chl_roe_fluxes", "chl_exact_fluxes", "chl_ic_kelvin_helmholtz", "chl_ensemble_create", "chl_ensemble_destroy",
     "chl_ensemble_get_config", "chl_ensemble_persistent_pairs", "chl_ensemble_bind", "chl_ensemble_start",
     "chl_ensemble_advance", "chl_ensemble_advance_looped", "chl_ensemble_clocks", "chl_ensemble_autotune",
     "chl_ensemble_layout", "chl_turbulence_modes", "chl_ic_turbulence", "chl_init_gas_energy", "chl_mhd_create",
@@ -123,6 +124,7 @@ def lib():
         "chl_hllc_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_hll_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_roe_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
+        "chl_exact_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_ic_kelvin_helmholtz": (_i, [_dp, _i, _i, _d, ctypes.c_uint64]),
         "chl_ensemble_create": (_i, [ctypes.POINTER(EnsembleConfig), _i, _ip, ctypes.POINTER(_vp)]),
         "chl_ensemble_destroy": (_i, [_vp]),
@@ -332,6 +334,10 @@ def hll_fluxes(QL, QR, F, nx, ny, nz, gamma, direction):
 
 def roe_fluxes(QL, QR, F, nx, ny, nz, gamma, direction):
     check(lib().chl_roe_fluxes(QL.ptr, QR.ptr, F.ptr, nx, ny, nz, gamma, direction))
+
+
+def exact_fluxes(QL, QR, F, nx, ny, nz, gamma, direction):
+    check(lib().chl_exact_fluxes(QL.ptr, QR.ptr, F.ptr, nx, ny, nz, gamma, direction))
 
 
 def ic_kelvin_helmholtz(n, ng, gamma=5.0 / 3.0, seed=2024):

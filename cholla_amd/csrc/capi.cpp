@@ -74,10 +74,10 @@ int CheckConfig(const chl_hydro_config &c)
   if (c.dual_energy != 0 && c.dual_energy != 1) return Fail(CHL_ERR_INVALID, "dual_energy must be 0 or 1");
   if (c.n_fields != 5 + c.dual_energy)
     return Fail(CHL_ERR_UNSUPPORTED, "n_fields must be 5, or 6 with dual_energy (no passive scalars)");
-  if (c.riemann != CHL_RIEMANN_HLLC && c.riemann != CHL_RIEMANN_HLL && c.riemann != CHL_RIEMANN_ROE)
-    return Fail(CHL_ERR_UNSUPPORTED, "riemann must be HLLC, HLL or ROE");
+  if (c.riemann < CHL_RIEMANN_HLLC || c.riemann > CHL_RIEMANN_EXACT)
+    return Fail(CHL_ERR_UNSUPPORTED, "riemann must be HLLC, HLL, ROE or EXACT");
   if (c.riemann != CHL_RIEMANN_HLLC && c.dual_energy)
-    return Fail(CHL_ERR_UNSUPPORTED, "the HLL and Roe solvers are built without dual energy");
+    return Fail(CHL_ERR_UNSUPPORTED, "the HLL, Roe and exact solvers are built without dual energy");
   if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC && c.reconstruction != CHL_RECON_PLMP &&
       c.reconstruction != CHL_RECON_PCM)
     return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PCM, PLMC, PPMC or PLMP");
@@ -99,7 +99,7 @@ int CheckConfig(const chl_hydro_config &c)
     if (c.dual_energy && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
       return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for tile shapes 0 and 1 only");
     if (c.riemann != CHL_RIEMANN_HLLC && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
-      return Fail(CHL_ERR_UNSUPPORTED, "the HLL and Roe solvers are built for tile shapes 0 and 1 only");
+      return Fail(CHL_ERR_UNSUPPORTED, "the HLL, Roe and exact solvers are built for tile shapes 0 and 1 only");
     if (simple_recon && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
       return Fail(CHL_ERR_UNSUPPORTED, "PCM and PLMP are built for tile shapes 0 and 1 only");
     if (c.tile_order != 0 && c.tile_order != 1) return Fail(CHL_ERR_INVALID, "tile_order must be 0 or 1");
@@ -118,7 +118,10 @@ int CheckConfig(const chl_hydro_config &c)
 
 int SolverKind(int riemann)
 {
-  return riemann == CHL_RIEMANN_HLL ? chl::kRiemannHll : (riemann == CHL_RIEMANN_ROE ? chl::kRiemannRoe : chl::kRiemannHllc);
+  if (riemann == CHL_RIEMANN_HLL) return chl::kRiemannHll;
+  if (riemann == CHL_RIEMANN_ROE) return chl::kRiemannRoe;
+  if (riemann == CHL_RIEMANN_EXACT) return chl::kRiemannExact;
+  return chl::kRiemannHllc;
 }
 
 int DefaultChunk(int n_real) { return std::max(8, std::min(n_real, 32)); }
@@ -737,6 +740,14 @@ int chl_roe_fluxes(const double *QL, const double *QR, double *F, int nx, int ny
 {
   if (!QL || !QR || !F || dir < 0 || dir > 2) return Fail(CHL_ERR_INVALID, "bad argument");
   CHL_HIP(chl::LaunchRiemannGrid(chl::kRiemannRoe, QL, QR, F, nx, ny, nz, gamma, dir, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_exact_fluxes(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma, int dir)
+{
+  if (!QL || !QR || !F || dir < 0 || dir > 2) return Fail(CHL_ERR_INVALID, "bad argument");
+  CHL_HIP(chl::LaunchRiemannGrid(chl::kRiemannExact, QL, QR, F, nx, ny, nz, gamma, dir, nullptr));
   CHL_HIP(hipDeviceSynchronize());
   return CHL_OK;
 }

@@ -29,6 +29,8 @@ constexpr int kRecon = CHL_RECON_PLMC;  // builds/make.type.hydro:5 default
 constexpr int kRiemann = CHL_RIEMANN_HLL;  // -DHLL in place of -DHLLC
 #elif defined(ROE)
 constexpr int kRiemann = CHL_RIEMANN_ROE;
+#elif defined(EXACT)
+constexpr int kRiemann = CHL_RIEMANN_EXACT;
 #else
 constexpr int kRiemann = CHL_RIEMANN_HLLC;
 #endif

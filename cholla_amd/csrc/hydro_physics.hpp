@@ -36,6 +36,7 @@ constexpr double kDeEta2 = 0.035;
 constexpr int kRiemannHllc = 0;
 constexpr int kRiemannHll  = 1;
 constexpr int kRiemannRoe  = 2;
+constexpr int kRiemannExact = 3;
 
 struct PolicyExact {
   using T                      = double;
@@ -64,6 +65,8 @@ template <class Base>
 using HllPolicy = SolverPolicy<Base, kRiemannHll>;
 template <class Base>
 using RoePolicy = SolverPolicy<Base, kRiemannRoe>;
+template <class Base>
+using ExactPolicy = SolverPolicy<Base, kRiemannExact>;
 
 template <typename T>
 struct Prim {
@@ -814,6 +817,121 @@ __device__ __forceinline__ Cons<typename P::T> Roe(const RState<typename P::T> &
   return f;
 }
 
+// ---------------------------------------------------------------------------- exact solver
+// powf as exact_cuda.cu calls it on doubles: both arguments narrowed to float and an FP32
+// result, evaluated as the correctly rounded float of the double pow of the narrowed
+// arguments (any conforming powf differs from it by float rounding only).
+__device__ __forceinline__ double PowF(double x, double y)
+{
+  return double(float(::pow(double(float(x)), double(float(y)))));
+}
+
+// guessp_CUDA, prefun_CUDA, starpv_CUDA and sample_CUDA, exact_cuda.cu:137-334: the star
+// state by Newton iteration from the PVRS / two-shock guess (at most 21 updates, relative
+// change <= 1e-6), then the solution sampled on the interface.  Always FP64.
+struct ExactSideState {
+  double d, v, p, c;
+};
+
+__device__ __forceinline__ void ExactPressureFn(double &f, double &fd, double p, const ExactSideState &k, double gamma)
+{
+  if (p <= k.p) {
+    f  = (2.0 / (gamma - 1.0)) * k.c * (PowF(p / k.p, (gamma - 1.0) / (2.0 * gamma)) - 1.0);
+    fd = (1.0 / (k.d * k.c)) * PowF((p / k.p), -((gamma + 1.0) / (2.0 * gamma)));
+  } else {
+    const double qrt = sqrt(((2.0 / (gamma + 1.0)) / k.d) / ((((gamma - 1.0) / (gamma + 1.0)) * k.p) + p));
+    f                = (p - k.p) * qrt;
+    fd               = (1.0 - 0.5 * (p - k.p) / ((((gamma - 1.0) / (gamma + 1.0)) * k.p) + p)) * qrt;
+  }
+}
+
+__device__ __forceinline__ void ExactStar(double &pm, double &vm, const ExactSideState &L, const ExactSideState &R,
+                                          double gamma)
+{
+  const double TOL = 1.0e-6;
+  double ppv       = 0.5 * (L.p + R.p) + 0.125 * (L.v - R.v) * (L.d + R.d) * (L.c + R.c);
+  if (ppv < 0.0) ppv = 0.0;
+  const double gl = sqrt((2.0 / ((gamma + 1.0) * L.d)) / (((gamma - 1.0) / (gamma + 1.0)) * L.p + ppv));
+  const double gr = sqrt((2.0 / ((gamma + 1.0) * R.d)) / (((gamma - 1.0) / (gamma + 1.0)) * R.p + ppv));
+  double pold     = (gl * L.p + gr * R.p - (R.v - L.v)) / (gl + gr);
+  if (pold < 0.0) pold = TOL;
+  double fl = 0.0, fld = 0.0, fr = 0.0, frd = 0.0;
+  for (int i = 0; i <= 20; i++) {
+    ExactPressureFn(fl, fld, pold, L, gamma);
+    ExactPressureFn(fr, frd, pold, R, gamma);
+    pm                  = pold - (fl + fr + R.v - L.v) / (fld + frd);
+    const double change = 2.0 * Fabs((pm - pold) / (pm + pold));
+    if (change <= TOL) break;
+    if (pm < 0.0) pm = TOL;
+    pold = pm;
+  }
+  vm = 0.5 * (L.v + R.v + fr - fl);
+}
+
+__device__ __forceinline__ void ExactSample(double pm, double vm, const ExactSideState &L, const ExactSideState &R,
+                                            double gamma, double &d, double &v, double &p)
+{
+  const double gm1 = gamma - 1.0, gp1 = gamma + 1.0;
+  if (vm >= 0) {
+    if (pm <= L.p) {  // left rarefaction
+      if (L.v - L.c >= 0) {
+        d = L.d, v = L.v, p = L.p;
+      } else if (vm - L.c * PowF(pm / L.p, gm1 / (2.0 * gamma)) < 0) {
+        d = L.d * PowF(pm / L.p, 1.0 / gamma), v = vm, p = pm;
+      } else {
+        const double c = (2.0 / gp1) * (L.c + (gm1 / 2.0) * L.v);
+        v = c, d = L.d * PowF(c / L.c, 2.0 / gm1), p = L.p * PowF(c / L.c, 2.0 * gamma / gm1);
+      }
+    } else {  // left shock
+      const double sl = L.v - L.c * sqrt((gp1 / (2.0 * gamma)) * (pm / L.p) + (gm1 / (2.0 * gamma)));
+      if (sl >= 0) {
+        d = L.d, v = L.v, p = L.p;
+      } else {
+        d = L.d * (pm / L.p + (gm1 / gp1)) / ((pm / L.p) * (gm1 / gp1) + 1.0), v = vm, p = pm;
+      }
+    }
+  } else {
+    if (pm > R.p) {  // right shock
+      const double sr = R.v + R.c * sqrt((gp1 / (2.0 * gamma)) * (pm / R.p) + (gm1 / (2.0 * gamma)));
+      if (sr <= 0) {
+        d = R.d, v = R.v, p = R.p;
+      } else {
+        d = R.d * (pm / R.p + (gm1 / gp1)) / ((pm / R.p) * (gm1 / gp1) + 1.0), v = vm, p = pm;
+      }
+    } else {  // right rarefaction
+      if (R.v + R.c <= 0) {
+        d = R.d, v = R.v, p = R.p;
+      } else if (vm + R.c * PowF(pm / R.p, gm1 / (2.0 * gamma)) >= 0) {
+        d = R.d * PowF(pm / R.p, (1.0 / gamma)), v = vm, p = pm;
+      } else {
+        const double c = (2.0 / gp1) * (R.c - (gm1 / 2.0) * R.v);
+        v = -c, d = R.d * PowF(c / R.c, 2.0 / gm1), p = R.p * PowF(c / R.c, 2.0 * gamma / gm1);
+      }
+    }
+  }
+}
+
+// Calculate_Exact_Fluxes_CUDA body, exact_cuda.cu:82-135: the Godunov flux of the sampled
+// state, transverse velocities upwinded by the sampled velocity.  The reference's vacuum
+// warning (a printf, :95-98) is left out.
+template <class P>
+__device__ __forceinline__ Cons<typename P::T> ExactFlux(const RState<typename P::T> &l,
+                                                         const RState<typename P::T> &r, typename P::T gamma_t)
+{
+  using T            = typename P::T;
+  const double gamma = double(gamma_t);
+  ExactSideState L{double(l.d), double(l.vx), double(l.p), 0.0}, R{double(r.d), double(r.vx), double(r.p), 0.0};
+  L.c = sqrt(gamma * L.p / L.d);
+  R.c = sqrt(gamma * R.p / R.d);
+  double pm = 0.0, vm = 0.0, ds, vs, ps;
+  ExactStar(pm, vm, L, R, gamma);
+  ExactSample(pm, vm, L, R, gamma, ds, vs, ps);
+  const bool left = vs >= 0;
+  const double vy = left ? double(l.vy) : double(r.vy), vz = left ? double(l.vz) : double(r.vz);
+  const double Es = (ps / (gamma - 1.0)) + 0.5 * ds * (vs * vs + vy * vy + vz * vz);
+  return Cons<T>{T(ds * vs), T(ds * vs * vs + ps), T(ds * vs * vy), T(ds * vs * vz), T((Es + ps) * vs)};
+}
+
 // The solver of the form P (HLLC unless P is a SolverPolicy).
 template <class P, bool DE = false>
 __device__ __forceinline__ Cons<typename P::T> RiemannFlux(const RState<typename P::T> &l,
@@ -825,6 +943,9 @@ __device__ __forceinline__ Cons<typename P::T> RiemannFlux(const RState<typename
   } else if constexpr (P::kSolver == kRiemannRoe) {
     static_assert(!DE, "the Roe solver is built without dual energy");
     return Roe<P>(l, r, gamma);
+  } else if constexpr (P::kSolver == kRiemannExact) {
+    static_assert(!DE, "the exact solver is built without dual energy");
+    return ExactFlux<P>(l, r, gamma);
   } else {
     return Hllc<P, DE>(l, r, gamma, fge);
   }

@@ -530,8 +530,8 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
 }
 
 // Calculate_HLLC_Fluxes_CUDA over a whole grid (hllc_cuda.cu:19-307), or, under
-// HllPolicy / RoePolicy, Calculate_HLL_Fluxes_CUDA (hll_cuda.cu:19-268) /
-// Calculate_Roe_Fluxes_CUDA (roe_cuda.cu:19-400)
+// HllPolicy / RoePolicy / ExactPolicy, Calculate_HLL_Fluxes_CUDA (hll_cuda.cu:19-268) /
+// Calculate_Roe_Fluxes_CUDA (roe_cuda.cu:19-400) / Calculate_Exact_Fluxes_CUDA (exact_cuda.cu:20-334)
 template <class P>
 __global__ void RiemannGridKernel(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
                                int dir)
@@ -690,6 +690,8 @@ hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int rieman
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPCM) return hipErrorInvalidValue;
   if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
   if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<PolicyExact>>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+  if (riemann == kRiemannExact)
+    return LaunchVl1dP<ExactPolicy<PolicyExact>>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
   if (riemann == kRiemannHllc) return LaunchVl1dP<PolicyExact>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
   return hipErrorInvalidValue;
 }
@@ -720,6 +722,9 @@ hipError_t LaunchRiemannGrid(int riemann, const double *QL, const double *QR, do
   } else if (riemann == kRiemannRoe) {
     hipLaunchKernelGGL(RiemannGridKernel<RoePolicy<PolicyExact>>, dim3(Blocks(nc, 64)), dim3(64), 0, s, QL, QR, F, nx,
                        ny, nz, gamma, dir);
+  } else if (riemann == kRiemannExact) {
+    hipLaunchKernelGGL(RiemannGridKernel<ExactPolicy<PolicyExact>>, dim3(Blocks(nc, 64)), dim3(64), 0, s, QL, QR, F,
+                       nx, ny, nz, gamma, dir);
   } else if (riemann == kRiemannHllc) {
     hipLaunchKernelGGL(RiemannGridKernel<PolicyExact>, dim3(Blocks(nc, 64)), dim3(64), 0, s, QL, QR, F, nx, ny, nz,
                        gamma, dir);

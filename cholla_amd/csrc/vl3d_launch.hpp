@@ -56,13 +56,14 @@ hipError_t WithSolver(int riemann, Fn &&fn)
 {
   if (riemann == kRiemannHll) return fn(HllPolicy<P>{});
   if (riemann == kRiemannRoe) return fn(RoePolicy<P>{});
+  if (riemann == kRiemannExact) return fn(ExactPolicy<P>{});
   return fn(P{});
 }
 
 template <class P>
 hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 {
-  if (a.riemann != kRiemannHllc) {  // HLL and Roe: the first two tile shapes, without DE
+  if (a.riemann != kRiemannHllc) {  // HLL, Roe, exact: the first two tile shapes, without DE
     if (a.de || tile > 1) return hipErrorInvalidValue;
     return WithSolver<P>(a.riemann, [&](auto q) {
       using Q = decltype(q);
@@ -96,7 +97,7 @@ hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 template <class P, int RK>
 hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
 {
-  // PCM, PLMP, HLL and Roe: the first two tile shapes, without DE
+  // PCM, PLMP, HLL, Roe and the exact solver: the first two tile shapes, without DE
   auto narrow = [&]() {
     if (a.de || tile > 1) return hipErrorInvalidValue;
     return WithSolver<P>(a.riemann, [&](auto q) {

@@ -39,6 +39,7 @@ extern "C" {
 #define CHL_RIEMANN_HLLC 0 /* HLLC (builds/make.type.hydro) */
 #define CHL_RIEMANN_HLL 1  /* HLL, Davis wave speeds (-DHLL, src/riemann_solvers/hll_cuda.cu) */
 #define CHL_RIEMANN_ROE 2  /* Roe with the HLLE fallback (-DROE, src/riemann_solvers/roe_cuda.cu) */
+#define CHL_RIEMANN_EXACT 3 /* exact solver, FP64 (-DEXACT, src/riemann_solvers/exact_cuda.cu) */
 /* arithmetic forms of the same step */
 #define CHL_PREC_FP64_EXACT 0 /* FP64, reference operation order, no FMA contraction */
 #define CHL_PREC_FP64 1       /* FP64, shared reciprocals + FMA */
@@ -55,7 +56,7 @@ typedef struct chl_hydro_config {
   int n_fields;          /* 5 (hydro), or 6 with dual_energy (GasEnergy last); no scalars */
   int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL.  PCM and PLMP: no dual
                             energy, tile shapes 0 and 1 (n_ghost: PCM 2, PLMP 3) */
-  int riemann;           /* CHL_RIEMANN_HLLC, or CHL_RIEMANN_HLL / _ROE (no dual energy; tile shapes 0, 1) */
+  int riemann;           /* CHL_RIEMANN_HLLC, or _HLL / _ROE / _EXACT (no dual energy; tile shapes 0, 1) */
   int precision;         /* CHL_PREC_* */
   int density_floor;     /* DENSITY_FLOOR build option, 0/1 */
   int temperature_floor; /* TEMPERATURE_FLOOR build option, 0/1 */
@@ -159,6 +160,9 @@ int chl_hll_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, doubl
 /* Calculate_Roe_Fluxes_CUDA (roe_cuda.cu:19) on device arrays */
 int chl_roe_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, double *dev_flux, int nx, int ny, int nz,
                    double gamma, int dir);
+/* Calculate_Exact_Fluxes_CUDA (exact_cuda.cu:20) on device arrays */
+int chl_exact_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, double *dev_flux, int nx, int ny, int nz,
+                     double gamma, int dir);
 
 /* ---- snapshot layout of one field (the HDF5 datasets of src/io/io.cpp).  The grid is
  * x-fastest with ghost cells; a dataset holds the real cells z-fastest,

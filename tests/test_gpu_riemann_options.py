@@ -1,8 +1,11 @@
-"""GPU parity of the HLL and Roe Riemann solver options (the reference's -DHLL and -DROE
-builds: Calculate_HLL_Fluxes_CUDA, src/riemann_solvers/hll_cuda.cu:19-268, and
-Calculate_Roe_Fluxes_CUDA, src/riemann_solvers/roe_cuda.cu:19-400; chl_hydro_config.riemann
-= CHL_RIEMANN_HLL / CHL_RIEMANN_ROE) against the oracle's solvers (tests/test_oracle_hll.py
-and tests/test_oracle_roe.py pin them).
+"""GPU parity of the HLL, Roe and exact Riemann solver options (the reference's -DHLL, -DROE
+and -DEXACT builds: Calculate_HLL_Fluxes_CUDA, src/riemann_solvers/hll_cuda.cu:19-268,
+Calculate_Roe_Fluxes_CUDA, src/riemann_solvers/roe_cuda.cu:19-400, and
+Calculate_Exact_Fluxes_CUDA, src/riemann_solvers/exact_cuda.cu:20-334;
+chl_hydro_config.riemann = CHL_RIEMANN_HLL / _ROE / _EXACT) against the oracle's solvers
+(tests/test_oracle_hll.py, test_oracle_roe.py and test_oracle_exact.py pin them).  The
+exact solver runs in FP64 in every form; its FP32 powers are the correctly rounded float
+of the double pow on both sides.
 
 Bars, as for HLLC:
   * the component entries chl_hll_fluxes / chl_roe_fluxes are bit-identical to the oracle
@@ -29,7 +32,8 @@ from test_gpu_parity import TOL, _random_periodic_box, field_l1, scaled_l1
 pytestmark = pytest.mark.gpu
 # (C-ABI solver, oracle solver, component entry, oracle component)
 SOLVERS = {"hll": (chl.RIEMANN_HLL, orc.HLL, chl.hll_fluxes, "orc_hll"),
-           "roe": (chl.RIEMANN_ROE, orc.ROE, chl.roe_fluxes, "orc_roe")}
+           "roe": (chl.RIEMANN_ROE, orc.ROE, chl.roe_fluxes, "orc_roe"),
+           "exact": (chl.RIEMANN_EXACT, orc.EXACT, chl.exact_fluxes, "orc_exact")}
 
 
 def say(*a):
