@@ -54,6 +54,9 @@ int g_pos_fallback = 0;
 
 // The Riemann solver build option (-DHLLC or -DHLL) as a run-time switch: 0 HLLC, 1 HLL.
 int g_riemann = 0;
+// gamma of the 1D step in progress (the 1D half update's DE term needs it)
+static double g_gamma_1d = 5.0 / 3.0;
+static double gamma_1d() { return g_gamma_1d; }
 
 inline size_t Idx(int i, int j, int k, int nx, int ny)
 {
@@ -1122,7 +1125,63 @@ void orc_update_half_1d(const double *U, double *Uh, const double *F, int n_cell
     for (int f = 0; f < n_fields; f++) {
       size_t o   = size_t(f) * n_cells;
       Uh[o + id] = U[o + id] + dtodx * (F[o + id - 1] - F[o + id]);
+      if (g_de && f == kGE) {
+        // the pdV source at the half step (hydro_cuda.cu:222-235, 259-263): pressure from
+        // the total energy, velocities by the inverse density, neighbours by division
+        const double d = U[id], d_inv = 1.0 / d;
+        const double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+        const double P      = (U[4 * n_cells + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma_1d() - 1.0);
+        const double vx_imo = U[n_cells + id - 1] / U[id - 1], vx_ipo = U[n_cells + id + 1] / U[id + 1];
+        Uh[o + id]          = U[o + id] + dtodx * (F[o + id - 1] - F[o + id]) + 0.5 * P * (dtodx * (vx_imo - vx_ipo));
+      }
     }
+}
+
+// Partial_Update_Advected_Internal_Energy_1D, src/hydro/hydro_cuda.cu:663-700: the pdV source
+// of the full step from the state at its start (DE pressure, neighbours' centred velocities)
+void orc_partial_update_ge_1d(double *U, int nx, int n_ghost, double dx, double dt, double gamma)
+{
+  const size_t n_cells = size_t(nx);
+  const double dtodx   = dt / dx;
+  for (int id = n_ghost; id < nx - n_ghost; id++) {
+    const double d = U[id], d_inv = 1.0 / d;
+    const double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+    const double E = U[4 * n_cells + id], GE = U[kGE * n_cells + id];
+    const double E_kin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
+    const double P     = std::fmax(PressureFromDE(E, E - E_kin, GE, gamma), kTiny);
+    const double vx_imo = U[n_cells + id - 1] / U[id - 1], vx_ipo = U[n_cells + id + 1] / U[id + 1];
+    U[kGE * n_cells + id] += 0.5 * P * (dtodx * (vx_imo - vx_ipo));
+  }
+}
+
+// Select_Internal_Energy_1D, src/hydro/hydro_cuda.cu:835-885 (neighbours clamped to real cells)
+void orc_select_internal_energy_1d(double *U, int nx, int n_ghost)
+{
+  const size_t n_cells = size_t(nx);
+  std::vector<double> sel(n_cells, 0.0);
+  for (int id = n_ghost; id < nx - n_ghost; id++) {
+    const int imo = std::max(id - 1, n_ghost), ipo = std::min(id + 1, nx - n_ghost - 1);
+    const double d = U[id], d_inv = 1.0 / d;
+    const double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+    const double E = U[4 * n_cells + id], U_advected = U[kGE * n_cells + id];
+    const double U_total = E - 0.5 * d * (vx * vx + vy * vy + vz * vz);
+    double Emax          = std::fmax(U[4 * n_cells + imo], E);
+    Emax                 = std::fmax(Emax, U[4 * n_cells + ipo]);
+    const double u       = ((U_total / E > kEta1) || (U_total / Emax > kEta2)) ? U_total : U_advected;
+    sel[id]              = std::fmax(u, kTiny);
+  }
+  for (int id = n_ghost; id < nx - n_ghost; id++) U[kGE * n_cells + id] = sel[id];
+}
+
+// Sync_Energies_1D, src/hydro/hydro_cuda.cu:1018-1040
+void orc_sync_energies_1d(double *U, int nx, int n_ghost)
+{
+  const size_t n_cells = size_t(nx);
+  for (int id = n_ghost; id < nx - n_ghost; id++) {
+    const double d = U[id], d_inv = 1.0 / d;
+    const double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+    U[4 * n_cells + id] = 0.5 * d * (vx * vx + vy * vy + vz * vz) + U[kGE * n_cells + id];
+  }
 }
 
 // Update_Conserved_Variables_1D, src/hydro/hydro_cuda.cu:20-82
@@ -1455,6 +1514,7 @@ void orc_vl1d(double *U, int nx, int n_ghost, double dx, double dt, double gamma
   size_t n_cells = size_t(nx);
   Scratch &s     = g_scratch;
   s.Resize(n_cells * n_fields);
+  g_gamma_1d = gamma;
   orc_pcm_1d(U, s.QLx.data(), s.QRx.data(), nx, n_fields);
   RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, 1, 1, gamma, 0);
   orc_update_half_1d(U, s.Uh.data(), s.Fx.data(), nx, dx, 0.5 * dt, n_fields);
@@ -1468,7 +1528,12 @@ void orc_vl1d(double *U, int nx, int n_ghost, double dx, double dt, double gamma
     orc_pcm_1d(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, n_fields);
   }
   RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, 1, 1, gamma, 0);
+  if (g_de) orc_partial_update_ge_1d(U, nx, n_ghost, dx, dt, gamma);  // VL_1D_cuda.cu:126-132
   orc_update_1d(U, s.Fx.data(), nx, n_ghost, dx, dt, n_fields);
+  if (g_de) {  // VL_1D_cuda.cu:137-142
+    orc_select_internal_energy_1d(U, nx, n_ghost);
+    orc_sync_energies_1d(U, nx, n_ghost);
+  }
 }
 
 // The driver loop of src/main.cpp:245-379 restricted to the hydro path:
@@ -1487,7 +1552,7 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
   double dti = orc_calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma);
   int steps  = 0;
   bool one_d = ny == 1 && nz == 1;
-  if (g_de && (one_d || n_fields != 6)) return -1;  // DE: 3D, GasEnergy as field 5
+  if (g_de && n_fields != 6) return -1;  // DE: GasEnergy as field 5
   while (t < tout && steps < max_steps) {
     double dt = kCfl / dti;
     if (t + dt > tout) dt = tout - t;

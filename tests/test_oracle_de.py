@@ -208,9 +208,3 @@ def test_temperature_floor_de_known_answer():
     want[0, 0, 1] = 1.0
     np.testing.assert_array_equal(out[5][r], want)
     np.testing.assert_array_equal(out[4][r], U[4][r])  # E - Ekin = 10 / 2 = 5 per mass > floor
-
-
-def test_dual_energy_refuses_1d():
-    U = np.ones(6 * 20)
-    with pytest.raises(ValueError):
-        orc.run(U, 20, 1, 1, 3, 6, 0.1, 0.1, 0.1, 1.4, orc.PLMC, [3] * 6, 0.0, 1.0, 1, dual_energy_on=True)
