@@ -106,7 +106,6 @@ int CheckConfig(const chl_hydro_config &c)
   } else {
     if (c.precision != CHL_PREC_FP64_EXACT)
       return Fail(CHL_ERR_UNSUPPORTED, "1D boxes run the FP64 reference-order form only");
-    if (c.dual_energy) return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for 3D boxes only");
   }
   // 2^31 guard on per-axis products used in 32-bit grid arithmetic of launch shapes
   if (double(c.nx) * c.ny * c.nz * 5 > 9.0e15) return Fail(CHL_ERR_INVALID, "box too large");
@@ -183,10 +182,11 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
   const int ng              = c.n_ghost;
   if (h->one_d) {
     const int n = c.nx;
-    bool fuse_dt = update_grid && n <= 1024;
-    CHL_HIP(chl::LaunchVl1d(U, reinterpret_cast<double *>(h->W), n, ng, c.reconstruction,
-                            SolverKind(c.riemann), dx, dt_dev, dt_host, gamma,
-                            fuse_dt ? h->dti() : nullptr, h->stream));
+    // the dt reduction rides on the last kernel of the step: the one-block stage 2, or with
+    // dual energy the energy sync (any number of blocks)
+    bool fuse_dt = update_grid && (n <= 1024 || c.dual_energy);
+    CHL_HIP(chl::LaunchVl1d(U, reinterpret_cast<double *>(h->W), n, ng, c.reconstruction, SolverKind(c.riemann),
+                            c.dual_energy != 0, dx, dt_dev, dt_host, gamma, fuse_dt ? h->dti() : nullptr, h->stream));
     if (update_grid && !fuse_dt) {
       CHL_HIP(hipMemsetAsync(h->dti(), 0, 8, h->stream));
       CHL_HIP(chl::LaunchCalcDt(U, n, 1, 1, ng, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));

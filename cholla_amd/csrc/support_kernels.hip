@@ -373,41 +373,57 @@ __global__ void TurbulenceIcKernel(double *U, int n, int ng, int n_modes, const 
 
 // ============================================================================ VL 1D
 // Stage 1 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:64-91): PCM states, the Riemann
-// solve of P (HLLC, or HLL under HllPolicy) on faces id-1/2 and id+1/2 of every
-// cell 1..n-2 and the dt/2 update; W holds the conserved half state.
-template <class P>
+// solve of P (HLLC, or another solver under a SolverPolicy) on faces id-1/2 and id+1/2
+// of every cell 1..n-2 and the dt/2 update; W holds the conserved half state.  With
+// dual energy (DE) the GasEnergy half update carries the half-step pdV source of
+// Update_Conserved_Variables_1D_half (hydro_cuda.cu:222-263): the pressure from the
+// total energy, the neighbours' velocities by division.
+template <class P, bool DE>
 __global__ void Vl1dStage1Kernel(const double *U, double *W, int n, double dx, const double *dt_dev, double dt_host,
                                  double gamma, unsigned long long *dti_reset)
 {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (dti_reset && id == 0) *dti_reset = 0ull;  // stage 2 (next launch) reduces into it
+  if (dti_reset && id == 0) *dti_reset = 0ull;  // a later launch of this step reduces into it
   if (id < 1 || id >= n - 1) return;
   const double dt    = 0.5 * (dt_dev ? *dt_dev : dt_host);
   const double dtodx = dt / dx;
   auto side = [&](int c) {
-    return RStateFromCons<PolicyExact>(U[c], U[n + c], U[2 * n + c], U[3 * n + c], U[4 * n + c], gamma);
+    return RStateFromCons<PolicyExact, DE>(U[c], U[n + c], U[2 * n + c], U[3 * n + c], U[4 * n + c], gamma,
+                                           DE ? U[5 * n + c] : 0.0);
   };
   RState<double> a = side(id - 1), b = side(id), cc = side(id + 1);
-  Cons<double> fm = RiemannFlux<P>(a, b, gamma);
-  Cons<double> fp = RiemannFlux<P>(b, cc, gamma);
-  double uh[5];
-  uh[0]          = U[id] + dtodx * (fm.d - fp.d);
-  uh[1]          = U[n + id] + dtodx * (fm.mx - fp.mx);
-  uh[2]          = U[2 * n + id] + dtodx * (fm.my - fp.my);
-  uh[3]          = U[3 * n + id] + dtodx * (fm.mz - fp.mz);
-  uh[4]          = U[4 * n + id] + dtodx * (fm.E - fp.E);
-  for (int q = 0; q < 5; q++) W[q * n + id] = uh[q];
+  double gm = 0.0, gp = 0.0;
+  Cons<double> fm = RiemannFlux<P, DE>(a, b, gamma, &gm);
+  Cons<double> fp = RiemannFlux<P, DE>(b, cc, gamma, &gp);
+  W[id]         = U[id] + dtodx * (fm.d - fp.d);
+  W[n + id]     = U[n + id] + dtodx * (fm.mx - fp.mx);
+  W[2 * n + id] = U[2 * n + id] + dtodx * (fm.my - fp.my);
+  W[3 * n + id] = U[3 * n + id] + dtodx * (fm.mz - fp.mz);
+  W[4 * n + id] = U[4 * n + id] + dtodx * (fm.E - fp.E);
+  if constexpr (DE) {
+    const double d = U[id], d_inv = 1.0 / d;
+    const double vx = U[n + id] * d_inv, vy = U[2 * n + id] * d_inv, vz = U[3 * n + id] * d_inv;
+    const double pr     = (U[4 * n + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+    const double vx_imo = U[n + id - 1] / U[id - 1], vx_ipo = U[n + id + 1] / U[id + 1];
+    W[5 * n + id]       = U[5 * n + id] + dtodx * (gm - gp) + 0.5 * pr * (dtodx * (vx_imo - vx_ipo));
+  }
 }
 
 // The interface values of cell c from the conserved half state W: primitives as
 // Load_Data forms them (PLMC, PPMC) or as PLMP_cuda does (plmp_cuda.cu:104-120), then
 // the reconstruction; PCM passes the conserved values through (pcm_cuda.cu:13-97).
-template <int RK>
-__device__ __forceinline__ void Recon1D(const double *W, int n, int c, double gamma, Prim<double> &L, Prim<double> &R)
+// With dual energy (PLMC and PPMC only) the pressure is Load_Data's DE pressure and the
+// specific GasEnergy is reconstructed beside (reconstruction.h:179-191, plmc_cuda.cu,
+// ppmc_cuda.cu:668-672).
+template <int RK, bool DE = false>
+__device__ __forceinline__ void Recon1D(const double *W, int n, int c, double gamma, Prim<double> &L, Prim<double> &R,
+                                        double &geL, double &geR)
 {
   constexpr int r  = ReconTraits<RK>::kRadius;
   constexpr int NS = 2 * r + 1;
+  static_assert(!DE || RK == kPLMC || RK == kPPMC, "dual energy is built for PLMC and PPMC");
   Prim<double> s[NS];
+  double ge[NS];
 #pragma unroll
   for (int m = 0; m < NS; m++) {
     int q = c - r + m;
@@ -417,6 +433,10 @@ __device__ __forceinline__ void Recon1D(const double *W, int n, int c, double ga
     } else if constexpr (RK == kPLMP) {
       const double vx = mx / d, vy = my / d, vz = mz / d;
       s[m] = {d, vx, vy, vz, Fmax((E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0), kTinyNumber)};
+    } else if constexpr (DE) {
+      const double vx = mx / d, vy = my / d, vz = mz / d, g = W[5 * n + q];
+      s[m]  = {d, vx, vy, vz, PressureFromDE(E, E - 0.5 * d * SquareMag(vx, vy, vz), g, gamma)};
+      ge[m] = g / d;
     } else {
       s[m] = LoadPrim(d, mx, my, mz, E, gamma);
     }
@@ -427,14 +447,18 @@ __device__ __forceinline__ void Recon1D(const double *W, int n, int c, double ga
     ReconPlmp<PolicyExact>(s, L, R);
   } else if constexpr (RK == kPPMC) {
     ReconPpmc<PolicyExact>(s, gamma, L, R);
+    if constexpr (DE) ReconPpmcScalar<PolicyExact>(ge, geL, geR);
   } else {
     ReconPlmc<PolicyExact>(s, gamma, L, R);
+    if constexpr (DE) ReconPlmcScalar<PolicyExact>(ge, geL, geR);
   }
 }
 
 // Stage 2 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:95-137) for the real cells, with the
-// next inverse dt (Calc_dt_1D) reduced in the epilogue.
-template <class P, int RK>
+// next inverse dt (Calc_dt_1D) reduced in the epilogue (without dual energy; with it the
+// energy sync that follows reduces it).  With DE the GasEnergy takes its flux difference
+// on top of the full-step pdV source Vl1dPdvKernel added.
+template <class P, int RK, bool DE>
 __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, double dx, const double *dt_dev,
                                  double dt_host, double gamma, unsigned long long *dti)
 {
@@ -444,13 +468,15 @@ __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, doub
   if (id >= ng && id < n - ng) {
     const double inv_gm1 = 1.0 / (gamma - 1.0);
     Prim<double> Lm, Rm, L0, R0, Lp, Rp;
-    Recon1D<RK>(W, n, id - 1, gamma, Lm, Rm);
-    Recon1D<RK>(W, n, id, gamma, L0, R0);
-    Recon1D<RK>(W, n, id + 1, gamma, Lp, Rp);
-    Cons<double> fm = RiemannFlux<P>(IfaceState<PolicyExact, RK>(Lm, gamma, inv_gm1),
-                                     IfaceState<PolicyExact, RK>(R0, gamma, inv_gm1), gamma);
-    Cons<double> fp = RiemannFlux<P>(IfaceState<PolicyExact, RK>(L0, gamma, inv_gm1),
-                                     IfaceState<PolicyExact, RK>(Rp, gamma, inv_gm1), gamma);
+    double gLm = 0, gRm = 0, gL0 = 0, gR0 = 0, gLp = 0, gRp = 0;
+    Recon1D<RK, DE>(W, n, id - 1, gamma, Lm, Rm, gLm, gRm);
+    Recon1D<RK, DE>(W, n, id, gamma, L0, R0, gL0, gR0);
+    Recon1D<RK, DE>(W, n, id + 1, gamma, Lp, Rp, gLp, gRp);
+    double gm = 0.0, gp = 0.0;
+    Cons<double> fm = RiemannFlux<P, DE>(IfaceState<PolicyExact, RK, DE>(Lm, gamma, inv_gm1, gLm),
+                                         IfaceState<PolicyExact, RK, DE>(R0, gamma, inv_gm1, gR0), gamma, &gm);
+    Cons<double> fp = RiemannFlux<P, DE>(IfaceState<PolicyExact, RK, DE>(L0, gamma, inv_gm1, gL0),
+                                         IfaceState<PolicyExact, RK, DE>(Rp, gamma, inv_gm1, gRp), gamma, &gp);
     const double dtodx = dt / dx;
     double un[5];
     un[0] = U[id] + dtodx * (fm.d - fp.d);
@@ -459,12 +485,68 @@ __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, doub
     un[3] = U[3 * n + id] + dtodx * (fm.mz - fp.mz);
     un[4] = U[4 * n + id] + dtodx * (fm.E - fp.E);
     for (int q = 0; q < 5; q++) U[q * n + id] = un[q];
-    double d = un[0], dinv = 1.0 / d;
-    double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
-    double pr = (un[4] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
-    pr        = Fmax(pr, kTinyNumber);
-    double cs = __builtin_sqrt(dinv * gamma * pr);
-    m         = (Fabs(vx) + cs) / dx;
+    if constexpr (DE) {
+      U[5 * n + id] += dtodx * (gm - gp);
+    } else {
+      double d = un[0], dinv = 1.0 / d;
+      double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
+      double pr = (un[4] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+      pr        = Fmax(pr, kTinyNumber);
+      double cs = __builtin_sqrt(dinv * gamma * pr);
+      m         = (Fabs(vx) + cs) / dx;
+    }
+  }
+  if (!DE && dti) BlockMaxToGlobal(m, dti);
+}
+
+// Partial_Update_Advected_Internal_Energy_1D, hydro_cuda.cu:663-700: the full-step pdV
+// source from the state at the start of the step (each cell writes only its GasEnergy and
+// reads its neighbours' density and momentum).
+__global__ void Vl1dPdvKernel(double *U, int n, int ng, double dx, const double *dt_dev, double dt_host, double gamma)
+{
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < ng || id >= n - ng) return;
+  const double dtodx = (dt_dev ? *dt_dev : dt_host) / dx;
+  const double d = U[id], d_inv = 1.0 / d;
+  const double vx = U[n + id] * d_inv, vy = U[2 * n + id] * d_inv, vz = U[3 * n + id] * d_inv;
+  const double E = U[4 * n + id], GE = U[5 * n + id];
+  const double E_kin  = 0.5 * d * (vx * vx + vy * vy + vz * vz);
+  const double pr     = Fmax(PressureFromDE(E, E - E_kin, GE, gamma), kTinyNumber);
+  const double vx_imo = U[n + id - 1] / U[id - 1], vx_ipo = U[n + id + 1] / U[id + 1];
+  U[5 * n + id] += 0.5 * pr * (dtodx * (vx_imo - vx_ipo));
+}
+
+// Select_Internal_Energy_1D, hydro_cuda.cu:835-885: writes GasEnergy only; the neighbours'
+// total energies (clamped to real cells) are read, not written.
+__global__ void Vl1dSelectKernel(double *U, int n, int ng)
+{
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < ng || id >= n - ng) return;
+  const int imo = max(id - 1, ng), ipo = min(id + 1, n - ng - 1);
+  const double d = U[id], d_inv = 1.0 / d;
+  const double vx = U[n + id] * d_inv, vy = U[2 * n + id] * d_inv, vz = U[3 * n + id] * d_inv;
+  const double E = U[4 * n + id], U_adv = U[5 * n + id];
+  const double U_tot = E - 0.5 * d * (vx * vx + vy * vy + vz * vz);
+  double Emax        = Fmax(U[4 * n + imo], E);
+  Emax               = Fmax(Emax, U[4 * n + ipo]);
+  const double u     = ((U_tot / E > kDeEta1) || (U_tot / Emax > kDeEta2)) ? U_tot : U_adv;
+  U[5 * n + id]      = Fmax(u, kTinyNumber);
+}
+
+// Sync_Energies_1D, hydro_cuda.cu:1018-1040, then the next inverse dt (Calc_dt_1D) of the
+// synchronised cell.
+__global__ void Vl1dSyncKernel(double *U, int n, int ng, double dx, double gamma, unsigned long long *dti)
+{
+  int id   = blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 0.0;
+  if (id >= ng && id < n - ng) {
+    const double d = U[id], d_inv = 1.0 / d;
+    const double vx = U[n + id] * d_inv, vy = U[2 * n + id] * d_inv, vz = U[3 * n + id] * d_inv;
+    const double E  = 0.5 * d * (vx * vx + vy * vy + vz * vz) + U[5 * n + id];
+    U[4 * n + id]   = E;
+    double pr       = (E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+    pr              = Fmax(pr, kTinyNumber);
+    m               = (Fabs(vx) + __builtin_sqrt(d_inv * gamma * pr)) / dx;
   }
   if (dti) BlockMaxToGlobal(m, dti);
 }
@@ -659,40 +741,59 @@ hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const doubl
   return hipGetLastError();
 }
 
-template <class P>
+template <class P, bool DE>
 static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, double dx, const double *dt_dev,
                               double dt, double gamma, unsigned long long *dti, hipStream_t s)
 {
-  hipLaunchKernelGGL(Vl1dStage1Kernel<P>, dim3(Blocks(n, 256)), dim3(256), 0, s, U, W, n, dx, dt_dev, dt, gamma, dti);
-  // one block when a dt reduction is fused so the block max is the grid max
-  unsigned nb = dti ? 1u : Blocks(n, 256);
-  int tpb     = dti ? 1024 : 256;
-  if (recon == kPPMC) {
-    hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMC>), dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma,
-                       dti);
-  } else if (recon == kPLMC) {
-    hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMC>), dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma,
-                       dti);
-  } else if (recon == kPLMP) {
-    hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMP>), dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma,
-                       dti);
+  const unsigned nb = Blocks(n, 256);
+  hipLaunchKernelGGL((Vl1dStage1Kernel<P, DE>), dim3(nb), dim3(256), 0, s, U, W, n, dx, dt_dev, dt, gamma, dti);
+  if constexpr (DE) {
+    hipLaunchKernelGGL(Vl1dPdvKernel, dim3(nb), dim3(256), 0, s, U, n, ng, dx, dt_dev, dt, gamma);
+    if (recon == kPPMC) {
+      hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMC, true>), dim3(nb), dim3(256), 0, s, U, W, n, ng, dx, dt_dev, dt,
+                         gamma, nullptr);
+    } else {
+      hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMC, true>), dim3(nb), dim3(256), 0, s, U, W, n, ng, dx, dt_dev, dt,
+                         gamma, nullptr);
+    }
+    hipLaunchKernelGGL(Vl1dSelectKernel, dim3(nb), dim3(256), 0, s, U, n, ng);
+    hipLaunchKernelGGL(Vl1dSyncKernel, dim3(nb), dim3(256), 0, s, U, n, ng, dx, gamma, dti);
+    return hipGetLastError();
   } else {
-    hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPCM>), dim3(nb), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt, gamma,
-                       dti);
+    // one block when a dt reduction is fused so the block max is the grid max
+    const unsigned nb2 = dti ? 1u : nb;
+    const int tpb      = dti ? 1024 : 256;
+    if (recon == kPPMC) {
+      hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMC, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
+                         gamma, dti);
+    } else if (recon == kPLMC) {
+      hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMC, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
+                         gamma, dti);
+    } else if (recon == kPLMP) {
+      hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMP, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
+                         gamma, dti);
+    } else {
+      hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPCM, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
+                         gamma, dti);
+    }
+    return hipGetLastError();
   }
-  return hipGetLastError();
 }
 
-hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, double dx, const double *dt_dev,
-                      double dt, double gamma, unsigned long long *dti, hipStream_t s)
+hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, bool de, double dx,
+                      const double *dt_dev, double dt, double gamma, unsigned long long *dti, hipStream_t s)
 {
-  if (dti && n > 1024) return hipErrorInvalidValue;
+  if (dti && n > 1024 && !de) return hipErrorInvalidValue;
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPCM) return hipErrorInvalidValue;
-  if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
-  if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<PolicyExact>>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+  if (de) {  // HLLC with PLMC or PPMC
+    if (riemann != kRiemannHllc || (recon != kPPMC && recon != kPLMC)) return hipErrorInvalidValue;
+    return LaunchVl1dP<PolicyExact, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+  }
+  if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+  if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
   if (riemann == kRiemannExact)
-    return LaunchVl1dP<ExactPolicy<PolicyExact>>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
-  if (riemann == kRiemannHllc) return LaunchVl1dP<PolicyExact>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+    return LaunchVl1dP<ExactPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+  if (riemann == kRiemannHllc) return LaunchVl1dP<PolicyExact, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
   return hipErrorInvalidValue;
 }
 

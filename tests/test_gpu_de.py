@@ -138,5 +138,41 @@ def test_de_uniform_supersonic_fixed_point(precision):
 def test_de_config_checks():
     with pytest.raises(chl.ChollaError):
         chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, dual_energy=True, tile1=2)
-    with pytest.raises(chl.ChollaError):
-        chl.Hydro(100, 1, 1, 3, chl.RECON_PLMC, chl.PREC_FP64_EXACT, dual_energy=True)
+    with pytest.raises(chl.ChollaError, match="1D boxes"):  # 1D runs the exact form only, DE or not
+        chl.Hydro(100, 1, 1, 3, chl.RECON_PLMC, chl.PREC_FP64, dual_energy=True)
+
+
+# ----------------------------------------------------------------------------- 1D
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("recon,ng", [(chl.RECON_PLMC, 3), (chl.RECON_PPMC, 4)])
+@pytest.mark.parametrize("case", ["warm", "cold", "wide"])
+def test_dual_energy_1d_exact_matches_oracle(case, recon, ng, use_graph):
+    """The 1D DE step (VL_1D_cuda.cu:126-142) bit for bit: a warm flow (total-energy pressure
+    selected), a cold supersonic compression (advected internal energy selected) and a
+    1500-cell box (the dt reduction over several blocks of the energy sync)."""
+    from test_oracle_de_1d import state_1d
+    n = 1500 if case == "wide" else 64
+    x = (np.arange(n) + 0.5) / n
+    if case == "cold":
+        d, vx, p = np.ones(n), 3.0 + 0.5 * np.sin(2 * np.pi * x), 1e-4 * np.ones(n)
+    else:
+        d, vx, p = 1.0 + 0.2 * np.sin(2 * np.pi * x), 0.5 * np.sin(4 * np.pi * x), 1.0 + 0.1 * np.cos(6 * np.pi * x)
+    U, nx = state_1d(d, vx, 0.3 * np.cos(2 * np.pi * x), 0.1 * np.ones(n), p, ng)
+    Uo = U.copy()
+    steps = 12
+    s_o, t_o = orc.run(Uo, nx, 1, 1, ng, 6, 1.0 / n, 1.0 / n, 1.0 / n, G, recon, [1, 1, 0, 0, 0, 0], 0.0, 1e30,
+                       steps, dual_energy_on=True)
+    say("DE 1D", case, "recon", recon, "graph", use_graph, "n", n)
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(nx, 1, 1, ng, recon, chl.PREC_FP64_EXACT, bc=[1] * 6, dual_energy=True)
+    s, t = hy.run(Ud, 1.0 / n, 1.0 / n, 1.0 / n, G, 0.0, 1e30, steps, use_graph=use_graph)
+    hy.close()
+    assert s == s_o == steps and t == t_o
+    np.testing.assert_array_equal(Ud.download(), Uo)
+
+
+def test_dual_energy_1d_limits():
+    with pytest.raises(chl.ChollaError, match="dual energy"):
+        chl.Hydro(100, 1, 1, 3, chl.RECON_PLMP, chl.PREC_FP64_EXACT, dual_energy=True)
+    with pytest.raises(chl.ChollaError, match="dual energy"):
+        chl.Hydro(100, 1, 1, 4, chl.RECON_PPMC, chl.PREC_FP64_EXACT, dual_energy=True, riemann=chl.RIEMANN_ROE)
