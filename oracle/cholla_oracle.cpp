@@ -1297,6 +1297,19 @@ double orc_calc_dti(const double *U, int nx, int ny, int nz, int n_ghost, double
     }
     return max_dti;
   }
+  if (ny > 1 && nz == 1) {  // Calc_dt_2D, hydro_cuda.cu:474-510
+    for (int j = n_ghost; j < ny - n_ghost; j++)
+      for (int i = n_ghost; i < nx - n_ghost; i++) {
+        size_t id = Idx(i, j, 0, nx, ny);
+        double d = U[id], dinv = 1.0 / d;
+        double vx = U[n_cells + id] * dinv, vy = U[2 * n_cells + id] * dinv, vz = U[3 * n_cells + id] * dinv;
+        double P  = (U[4 * n_cells + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+        P         = std::fmax(P, 1.0e-20);
+        double cs = std::sqrt(dinv * gamma * P);
+        max_dti   = std::fmax(max_dti, std::fmax((std::fabs(vx) + cs) / dx, (std::fabs(vy) + cs) / dy));
+      }
+    return max_dti;
+  }
 #pragma omp parallel for collapse(2) reduction(max : max_dti) schedule(static)
   for (int k = n_ghost; k < nz - n_ghost; k++)
     for (int j = n_ghost; j < ny - n_ghost; j++)
@@ -1508,6 +1521,70 @@ void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double 
   }
 }
 
+// Update_Conserved_Variables_2D_half, VL_2D_cuda.cu:183-256 (no DE): cells 1..n-2 in x and y,
+// the y term added after the x term
+void orc_update_half_2d(const double *U, double *Uh, const double *Fx, const double *Fy, int nx, int ny, double dx,
+                        double dy, double dt, int n_fields)
+{
+  const size_t n_cells = size_t(nx) * ny;
+  const double dtodx = dt / dx, dtody = dt / dy;
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const size_t id = Idx(i, j, 0, nx, ny), imo = id - 1, jmo = id - nx;
+      for (int f = 0; f < n_fields; f++) {
+        const size_t o = size_t(f) * n_cells;
+        Uh[o + id]     = U[o + id] + dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]);
+      }
+    }
+}
+
+// Update_Conserved_Variables_2D, hydro_cuda.cu:84-160 (no gravity, no DE): real cells, the
+// x and y terms summed before they are added
+void orc_update_2d(double *U, const double *Fx, const double *Fy, int nx, int ny, int n_ghost, double dx, double dy,
+                   double dt, int n_fields)
+{
+  const size_t n_cells = size_t(nx) * ny;
+  const double dtodx = dt / dx, dtody = dt / dy;
+  for (int j = n_ghost; j < ny - n_ghost; j++)
+    for (int i = n_ghost; i < nx - n_ghost; i++) {
+      const size_t id = Idx(i, j, 0, nx, ny), imo = id - 1, jmo = id - nx;
+      for (int f = 0; f < n_fields; f++) {
+        const size_t o = size_t(f) * n_cells;
+        U[o + id] += dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]);
+      }
+    }
+}
+
+// VL_Algorithm_2D_CUDA, src/integrators/VL_2D_cuda.cu:24-168 (no DE, no scalars)
+void orc_vl2d(double *U, int nx, int ny, int n_ghost, double dx, double dy, double dt, double gamma, int n_fields,
+              int recon)
+{
+  const size_t n_cells = size_t(nx) * ny;
+  Scratch &s           = g_scratch;
+  s.Resize(n_cells * n_fields);
+  orc_pcm_3d(U, s.QLx.data(), s.QRx.data(), s.QLy.data(), s.QRy.data(), s.QLz.data(), s.QRz.data(), nx, ny, 1,
+             n_fields);
+  RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, 1, gamma, 0);
+  RiemannFluxes(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, 1, gamma, 1);
+  orc_update_half_2d(U, s.Uh.data(), s.Fx.data(), s.Fy.data(), nx, ny, dx, dy, 0.5 * dt, n_fields);
+  if (recon == ORC_PLMC) {
+    orc_plmc(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, 1, gamma, 0);
+    orc_plmc(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, 1, gamma, 1);
+  } else if (recon == ORC_PPMC) {
+    orc_ppmc_vl(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, 1, gamma, 0);
+    orc_ppmc_vl(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, 1, gamma, 1);
+  } else if (recon == ORC_PLMP) {
+    orc_plmp(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, 1, gamma, 0);
+    orc_plmp(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, 1, gamma, 1);
+  } else {
+    orc_pcm_3d(s.Uh.data(), s.QLx.data(), s.QRx.data(), s.QLy.data(), s.QRy.data(), s.QLz.data(), s.QRz.data(), nx,
+               ny, 1, n_fields);
+  }
+  RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, 1, gamma, 0);
+  RiemannFluxes(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, 1, gamma, 1);
+  orc_update_2d(U, s.Fx.data(), s.Fy.data(), nx, ny, n_ghost, dx, dy, dt, n_fields);
+}
+
 // VL_Algorithm_1D_CUDA, src/integrators/VL_1D_cuda.cu:30-146 (HLLC)
 void orc_vl1d(double *U, int nx, int n_ghost, double dx, double dt, double gamma, int n_fields, int recon)
 {
@@ -1553,11 +1630,14 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
   int steps  = 0;
   bool one_d = ny == 1 && nz == 1;
   if (g_de && n_fields != 6) return -1;  // DE: GasEnergy as field 5
+  if (g_de && !one_d && nz == 1) return -1;  // DE is not restated for 2D
   while (t < tout && steps < max_steps) {
     double dt = kCfl / dti;
     if (t + dt > tout) dt = tout - t;
     if (one_d) {
       orc_vl1d(U, nx, n_ghost, dx, dt, gamma, n_fields, recon);
+    } else if (nz == 1) {  // 2D: no temperature floor (its guard, hydro_cuda.cu:1130-1131, admits no cell)
+      orc_vl2d(U, nx, ny, n_ghost, dx, dy, dt, gamma, n_fields, recon);
     } else {
       orc_vl3d(U, nx, ny, nz, n_ghost, dx, dy, dz, dt, gamma, n_fields, recon, density_floor_on, density_floor);
       if (temperature_floor_on) orc_temperature_floor(U, nx, ny, nz, n_ghost, U_floor);
