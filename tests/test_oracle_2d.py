@@ -36,23 +36,26 @@ def kh_2d(n, ng, seed=4):
     return U, N
 
 
+@pytest.mark.parametrize("aspect", [1.0, 0.5])
 @pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC, orc.PLMP, orc.PCM])
-def test_2d_matches_z_uniform_3d(recon):
+def test_2d_matches_z_uniform_3d(recon, aspect):
+    """aspect = dy / dx: non-square cells catch a dx used in place of dy."""
     ng, n = NG[recon], 16
     U2, N = kh_2d(n, ng)
     Nz = 3 * ng
     U3 = np.repeat(U2.reshape(5, 1, N, N), Nz, axis=1).reshape(-1).copy()
     h = 1.0 / n
-    s2, t2 = orc.run(U2, N, N, 1, ng, 5, h, h, h, G, recon, [1, 1, 1, 1, 0, 0], 0.0, 1e30, 8)
-    s3, t3 = orc.run(U3, N, N, Nz, ng, 5, h, h, h, G, recon, [1] * 6, 0.0, 1e30, 8)
+    hy = aspect * h
+    s2, t2 = orc.run(U2, N, N, 1, ng, 5, h, hy, h, G, recon, [1, 1, 1, 1, 0, 0], 0.0, 1e30, 8)
+    s3, t3 = orc.run(U3, N, N, Nz, ng, 5, h, hy, h, G, recon, [1] * 6, 0.0, 1e30, 8)
     assert s2 == s3 == 8 and abs(t2 - t3) <= 1e-14 * t3
     a = U2.reshape(5, N, N)[:, ng:-ng, ng:-ng]
     b = U3.reshape(5, Nz, N, N)[:, ng:-ng, ng:-ng, ng:-ng]
     for k in range(b.shape[1]):
         np.testing.assert_allclose(b[:, k], a, rtol=1e-12, atol=1e-13)
-    mass = a[0].sum() * h * h
+    mass = a[0].sum() * h * hy
     U0, _ = kh_2d(n, ng)
-    assert abs(mass - U0.reshape(5, N, N)[0, ng:-ng, ng:-ng].sum() * h * h) < 1e-13
+    assert abs(mass - U0.reshape(5, N, N)[0, ng:-ng, ng:-ng].sum() * h * hy) < 1e-13
 
 
 @pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC, orc.PLMP])
