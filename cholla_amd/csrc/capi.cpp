@@ -40,6 +40,7 @@ struct chl_hydro {
   chl_hydro_config cfg{};
   size_t n_cells = 0;
   bool one_d     = false;
+  bool two_d     = false;  // nz == 1 < ny: the unfused 2D step, W holds its scratch
   int num_cu     = 256;
   void *W        = nullptr;  // half-step primitives (+ the pdV sources with dual energy)
   size_t W_bytes = 0;
@@ -86,14 +87,20 @@ int CheckConfig(const chl_hydro_config &c)
   if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
   if (c.positivity_fallback != 0 && c.positivity_fallback != 1)
     return Fail(CHL_ERR_INVALID, "positivity_fallback must be 0 or 1");
-  if (c.positivity_fallback && (c.reconstruction != CHL_RECON_PPMC || (c.ny == 1 && c.nz == 1)))
+  if (c.positivity_fallback && (c.reconstruction != CHL_RECON_PPMC || c.nz == 1))
     return Fail(CHL_ERR_UNSUPPORTED, "the positivity fallback is built for PPMC on 3D boxes only");
   const int ng = c.n_ghost, r = ReconRadius(c.reconstruction);
   if (ng < r + 2)
     return Fail(CHL_ERR_INVALID, "n_ghost too small for the reconstruction (PCM needs 2, PLMC and PLMP 3, PPMC 4)");
   if (c.nx < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "nx must exceed 2*n_ghost");
   bool one_d = c.ny == 1 && c.nz == 1;
-  if (!one_d) {
+  bool two_d = c.nz == 1 && c.ny > 1;
+  if (two_d) {
+    if (c.ny < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "2D boxes need ny > 2*n_ghost");
+    if (c.precision != CHL_PREC_FP64_EXACT)
+      return Fail(CHL_ERR_UNSUPPORTED, "2D boxes run the FP64 reference-order form only");
+    if (c.dual_energy) return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for 3D and 1D boxes");
+  } else if (!one_d) {
     if (c.ny < 2 * ng + 1 || c.nz < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "3D boxes need ny, nz > 2*n_ghost");
     if (c.tile1 >= chl::kNumTiles || c.tile2 >= chl::kNumTiles) return Fail(CHL_ERR_INVALID, "tile index out of range");
     if (c.dual_energy && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
@@ -191,6 +198,13 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
       CHL_HIP(hipMemsetAsync(h->dti(), 0, 8, h->stream));
       CHL_HIP(chl::LaunchCalcDt(U, n, 1, 1, ng, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
     }
+    if (fill) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
+    return CHL_OK;
+  }
+  if (h->two_d) {  // no temperature floor: the reference's guard admits no 2D cell (hydro_cuda.cu:1130-1131)
+    CHL_HIP(chl::LaunchVl2d(U, reinterpret_cast<double *>(h->W), c.nx, c.ny, ng, c.reconstruction,
+                            SolverKind(c.riemann), dx, dy, dt_dev, dt_host, gamma, update_grid ? h->dti() : nullptr,
+                            h->stream));
     if (fill) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
     return CHL_OK;
   }
@@ -344,6 +358,7 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   CHL_TRY(CheckConfig(c));
   auto *h    = new chl_hydro();
   h->one_d   = c.ny == 1 && c.nz == 1;
+  h->two_d   = c.nz == 1 && c.ny > 1;
   h->n_cells = size_t(c.nx) * c.ny * c.nz;
   if (c.kchunk1 <= 0) c.kchunk1 = DefaultChunk(c.nz - 2 * c.n_ghost + 2 * (ReconRadius(c.reconstruction) + 1));
   if (c.kchunk2 <= 0) c.kchunk2 = DefaultChunk(c.nz - 2 * c.n_ghost);
@@ -358,6 +373,7 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   size_t elem      = c.precision == CHL_PREC_MIXED ? sizeof(float) : sizeof(double);
   const size_t w_fields = 5 + size_t(c.dual_energy);
   h->W_bytes            = w_fields * h->n_cells * elem + (c.dual_energy ? h->n_cells * sizeof(double) : 0);
+  if (h->two_d) h->W_bytes = 35 * h->n_cells * sizeof(double);  // half state, 4 interface and 2 flux arrays
   hipError_t e     = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&h->W, h->W_bytes);
   if (e == hipSuccess && c.dual_energy)
@@ -476,7 +492,7 @@ int chl_hydro_time_steps(chl_hydro *h, double *U, double dx, double dy, double d
 int chl_hydro_kernel_times(chl_hydro *h, double *U, double dx, double dy, double dz, double gamma, int n_steps,
                            float *ms1, float *ms2, float *ms3)
 {
-  if (!h || !U || h->one_d) return Fail(CHL_ERR_INVALID, "bad argument (3D boxes only)");
+  if (!h || !U || h->one_d || h->two_d) return Fail(CHL_ERR_INVALID, "bad argument (3D boxes only)");
   const chl_hydro_config &c = h->cfg;
   const double tout         = std::numeric_limits<double>::infinity();
   CHL_TRY(InitialDti(h, U, dx, dy, dz, gamma));
@@ -523,7 +539,7 @@ int chl_hydro_kernel_times(chl_hydro *h, double *U, double dx, double dy, double
 int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, double dz, double gamma, int steps,
                        int *tile1, int *tile2, int *kchunk1, int *kchunk2)
 {
-  if (!h || !U || h->one_d) return Fail(CHL_ERR_INVALID, "bad argument (3D boxes only)");
+  if (!h || !U || h->one_d || h->two_d) return Fail(CHL_ERR_INVALID, "bad argument (3D boxes only)");
   chl_hydro_config &c = h->cfg;
   const size_t bytes  = size_t(c.n_fields) * h->n_cells * sizeof(double);
   double *S           = nullptr;

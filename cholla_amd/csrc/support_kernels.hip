@@ -551,6 +551,100 @@ __global__ void Vl1dSyncKernel(double *U, int n, int ng, double dx, double gamma
   if (dti) BlockMaxToGlobal(m, dti);
 }
 
+// ============================================================================ VL 2D
+// PCM_Reconstruction_2D with the Riemann solve of P (VL_2D_cuda.cu:49-71, and stage 2 of
+// the PCM build): the x face i+1/2 and y face j+1/2 of every cell with a neighbour there,
+// from the conserved states as they are.  Thread 0 clears the step's dt reduction.
+template <class P>
+__global__ void Vl2dPcmFluxKernel(const double *U, double *Fx, double *Fy, int nx, int ny, double gamma,
+                                  unsigned long long *dti_reset)
+{
+  const long nc = long(nx) * ny;
+  const long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (dti_reset && id == 0) *dti_reset = 0ull;
+  if (id >= nc) return;
+  const int j = int(id / nx), i = int(id - long(j) * nx);
+  auto state = [&](long c, int o1, int o2, int o3) {
+    return RStateFromCons<PolicyExact>(U[c], U[o1 * nc + c], U[o2 * nc + c], U[o3 * nc + c], U[4 * nc + c], gamma);
+  };
+  auto store = [&](double *F, const Cons<double> &f, int o1, int o2, int o3) {
+    F[id] = f.d, F[o1 * nc + id] = f.mx, F[o2 * nc + id] = f.my, F[o3 * nc + id] = f.mz, F[4 * nc + id] = f.E;
+  };
+  if (i < nx - 1) store(Fx, RiemannFlux<P>(state(id, 1, 2, 3), state(id + 1, 1, 2, 3), gamma), 1, 2, 3);
+  if (j < ny - 1) store(Fy, RiemannFlux<P>(state(id, 2, 3, 1), state(id + nx, 2, 3, 1), gamma), 2, 3, 1);
+}
+
+// Update_Conserved_Variables_2D_half, VL_2D_cuda.cu:183-256 (no DE): cells 1..n-2 along
+// both axes, the y term added after the x term.
+__global__ void Vl2dHalfKernel(const double *U, double *Uh, const double *Fx, const double *Fy, int nx, int ny,
+                               double dx, double dy, const double *dt_dev, double dt_host)
+{
+  const long nc = long(nx) * ny;
+  const long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= nc) return;
+  const int j = int(id / nx), i = int(id - long(j) * nx);
+  if (i < 1 || i >= nx - 1 || j < 1 || j >= ny - 1) return;
+  const double dt = 0.5 * (dt_dev ? *dt_dev : dt_host), dtodx = dt / dx, dtody = dt / dy;
+  const long imo = id - 1, jmo = id - nx;
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    const long o = f * nc;
+    Uh[o + id]   = U[o + id] + dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]);
+  }
+}
+
+// Update_Conserved_Variables_2D (hydro_cuda.cu:84-160, no gravity, no DE) on the real
+// cells, the x and y terms summed before they are added, then Calc_dt_2D
+// (hydro_cuda.cu:474-510) of the updated cell into *dti.
+__global__ void Vl2dUpdateKernel(double *U, const double *Fx, const double *Fy, int nx, int ny, int ng, double dx,
+                                 double dy, const double *dt_dev, double dt_host, double gamma,
+                                 unsigned long long *dti)
+{
+  const long nc = long(nx) * ny;
+  const long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  double m      = 0.0;
+  if (id < nc) {
+    const int j = int(id / nx), i = int(id - long(j) * nx);
+    if (i >= ng && i < nx - ng && j >= ng && j < ny - ng) {
+      const double dt = dt_dev ? *dt_dev : dt_host, dtodx = dt / dx, dtody = dt / dy;
+      const long imo = id - 1, jmo = id - nx;
+      double un[5];
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        const long o = f * nc;
+        un[f]        = U[o + id] + (dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]));
+        U[o + id]    = un[f];
+      }
+      const double d = un[0], dinv = 1.0 / d;
+      const double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
+      double pr       = (un[4] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+      pr              = Fmax(pr, 1.0e-20);
+      const double cs = __builtin_sqrt(dinv * gamma * pr);
+      m               = Fmax((Fabs(vx) + cs) / dx, (Fabs(vy) + cs) / dy);
+    }
+  }
+  if (dti) BlockMaxToGlobal(m, dti);
+}
+
+// Calc_dt_2D, hydro_cuda.cu:474-510
+__global__ void CalcDt2DKernel(const double *U, int nx, int ny, int ng, double dx, double dy, double gamma,
+                               unsigned long long *dti)
+{
+  const long nc = long(nx) * ny;
+  double m      = 0.0;
+  for (long id = long(blockIdx.x) * blockDim.x + threadIdx.x; id < nc; id += long(blockDim.x) * gridDim.x) {
+    const int j = int(id / nx), i = int(id - long(j) * nx);
+    if (i < ng || i >= nx - ng || j < ng || j >= ny - ng) continue;
+    const double d = U[id], dinv = 1.0 / d;
+    const double vx = U[nc + id] * dinv, vy = U[2 * nc + id] * dinv, vz = U[3 * nc + id] * dinv;
+    double pr       = (U[4 * nc + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+    pr              = Fmax(pr, 1.0e-20);
+    const double cs = __builtin_sqrt(dinv * gamma * pr);
+    m               = Fmax(m, Fmax((Fabs(vx) + cs) / dx, (Fabs(vy) + cs) / dy));
+  }
+  BlockMaxToGlobal(m, dti);
+}
+
 // ============================================================================ reference-shaped stage kernels
 __device__ __forceinline__ bool ThreadGuard(int order, int nx, int ny, int nz, int xid, int yid, int zid)
 {
@@ -571,7 +665,7 @@ __device__ __forceinline__ void DirSlots(int dir, int &o1, int &o2, int &o3)
   o3 = dir == 0 ? 3 : (dir == 1 ? 1 : 2);
 }
 
-// PLMC_cuda / PPMC_VL over a whole grid with the reference's thread guard and
+// PLMC_cuda / PPMC_VL / PLMP_cuda over a whole grid with the reference's thread guard and
 // interface layout (plmc_cuda.cu:24-300, ppmc_cuda.cu:540-695): Q_L at cell i,
 // Q_R at cell i-1 along dir, conserved variables in global momentum slots.
 template <int RK>
@@ -583,19 +677,32 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
   long t           = long(blockIdx.x) * blockDim.x + threadIdx.x;
   int zid = int(t / (long(nx) * ny)), yid = int((t - long(zid) * nx * ny) / nx);
   int xid = int(t - long(zid) * nx * ny - long(yid) * nx);
-  if (ThreadGuard(r + 1, nx, ny, nz, xid, yid, zid)) return;
   int o1, o2, o3;
   DirSlots(dir, o1, o2, o3);
   int ix = dir == 0, iy = dir == 1, iz = dir == 2;
+  if constexpr (RK == kPLMP) {  // plmp_cuda.cu:80-103: one cell off each end along dir only
+    const int a = ix ? xid : (iy ? yid : zid), na = ix ? nx : (iy ? ny : nz);
+    if (xid >= nx || yid >= ny || zid >= nz || a < 1 || a >= na - 2) return;
+  } else {
+    if (ThreadGuard(r + 1, nx, ny, nz, xid, yid, zid)) return;
+  }
   Prim<double> s[NS];
 #pragma unroll
   for (int m = 0; m < NS; m++) {
     size_t id = Lin(xid + (m - r) * ix, yid + (m - r) * iy, zid + (m - r) * iz, nx, ny);
-    s[m]      = LoadPrim(U[id], U[o1 * nc + id], U[o2 * nc + id], U[o3 * nc + id], U[4 * nc + id], gamma);
+    if constexpr (RK == kPLMP) {  // plmp_cuda.cu:104-120
+      const double d = U[id], vx = U[o1 * nc + id] / d, vy = U[o2 * nc + id] / d, vz = U[o3 * nc + id] / d;
+      s[m] = {d, vx, vy, vz, Fmax((U[4 * nc + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0),
+                                  kTinyNumber)};
+    } else {
+      s[m] = LoadPrim(U[id], U[o1 * nc + id], U[o2 * nc + id], U[o3 * nc + id], U[4 * nc + id], gamma);
+    }
   }
   Prim<double> L, R;
   if constexpr (RK == kPPMC) {
     ReconPpmc<PolicyExact>(s, gamma, L, R);
+  } else if constexpr (RK == kPLMP) {
+    ReconPlmp<PolicyExact>(s, L, R);
   } else {
     ReconPlmc<PolicyExact>(s, gamma, L, R);
   }
@@ -605,7 +712,11 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
     Q[o1 * nc + id] = q.d * q.vx;
     Q[o2 * nc + id] = q.d * q.vy;
     Q[o3 * nc + id] = q.d * q.vz;
-    Q[4 * nc + id]  = EnergyFromPrim<PolicyExact>(q, gamma, inv_gm1);
+    if constexpr (RK == kPLMP) {  // plmp_cuda.cu:224-225
+      Q[4 * nc + id] = q.p / (gamma - 1.0) + 0.5 * q.d * (q.vx * q.vx + q.vy * q.vy + q.vz * q.vz);
+    } else {
+      Q[4 * nc + id] = EnergyFromPrim<PolicyExact>(q, gamma, inv_gm1);
+    }
   };
   write(L, QL, Lin(xid, yid, zid, nx, ny));
   write(R, QR, Lin(xid - ix, yid - iy, zid - iz, nx, ny));
@@ -671,10 +782,14 @@ hipError_t LaunchGhostSlab(double *U, int nx, int ny, int nz, int n_fields, int 
 hipError_t LaunchCalcDt(const double *U, int nx, int ny, int nz, int ng, double dx, double dy, double dz,
                         double gamma, unsigned long long *dti, int num_cu, hipStream_t s)
 {
-  // Calc_dt_GPU's dimension test (hydro_cuda.cu:572-592); every other shape,
-  // including a single cell, takes the 3D kernel
+  // Calc_dt_GPU's dimension test (hydro_cuda.cu:572-592); every other shape than 1D and
+  // 2D, including a single cell, takes the 3D kernel
   if (nx > 1 && ny == 1 && nz == 1) {
     hipLaunchKernelGGL(CalcDt1DKernel, dim3(1), dim3(256), 0, s, U, nx, ng, dx, gamma, dti);
+  } else if (nx > 1 && ny > 1 && nz == 1) {
+    long nc     = long(nx) * ny;
+    unsigned nb = unsigned(std::min<long>(Blocks(nc, 256), long(num_cu) * 8));
+    hipLaunchKernelGGL(CalcDt2DKernel, dim3(nb), dim3(256), 0, s, U, nx, ny, ng, dx, dy, gamma, dti);
   } else {
     long nc     = long(nx) * ny * nz;
     unsigned nb = unsigned(std::min<long>(Blocks(nc, 256), long(num_cu) * 8));
@@ -797,6 +912,47 @@ hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int rieman
   return hipErrorInvalidValue;
 }
 
+template <class P>
+static hipError_t LaunchVl2dP(double *U, double *scratch, int nx, int ny, int ng, int recon, double dx, double dy,
+                              const double *dt_dev, double dt, double gamma, unsigned long long *dti, hipStream_t s)
+{
+  const long nc = long(nx) * ny, fs = 5 * nc;
+  double *Uh = scratch, *QLx = scratch + fs, *QRx = scratch + 2 * fs, *QLy = scratch + 3 * fs;
+  double *QRy = scratch + 4 * fs, *Fx = scratch + 5 * fs, *Fy = scratch + 6 * fs;
+  const unsigned nb = Blocks(nc, 256);
+  hipLaunchKernelGGL(Vl2dPcmFluxKernel<P>, dim3(nb), dim3(256), 0, s, U, Fx, Fy, nx, ny, gamma, dti);
+  hipLaunchKernelGGL(Vl2dHalfKernel, dim3(nb), dim3(256), 0, s, U, Uh, Fx, Fy, nx, ny, dx, dy, dt_dev, dt);
+  if (recon == kPCM) {
+    hipLaunchKernelGGL(Vl2dPcmFluxKernel<P>, dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, gamma, nullptr);
+  } else {
+    hipError_t e = LaunchReconGrid(recon, Uh, QLx, QRx, nx, ny, 1, gamma, 0, s);
+    if (e == hipSuccess) e = LaunchReconGrid(recon, Uh, QLy, QRy, nx, ny, 1, gamma, 1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(RiemannGridKernel<P>, dim3(Blocks(nc, 64)), dim3(64), 0, s, QLx, QRx, Fx, nx, ny, 1, gamma, 0);
+    hipLaunchKernelGGL(RiemannGridKernel<P>, dim3(Blocks(nc, 64)), dim3(64), 0, s, QLy, QRy, Fy, nx, ny, 1, gamma, 1);
+  }
+  hipLaunchKernelGGL(Vl2dUpdateKernel, dim3(nb), dim3(256), 0, s, U, Fx, Fy, nx, ny, ng, dx, dy, dt_dev, dt, gamma,
+                     dti);
+  return hipGetLastError();
+}
+
+// VL_Algorithm_2D_CUDA (VL_2D_cuda.cu:24-168) in the exact form; scratch holds 7 x 5 x nx*ny
+// doubles (half state, four interface arrays, two flux arrays).
+hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, double dx, double dy,
+                      const double *dt_dev, double dt, double gamma, unsigned long long *dti, hipStream_t s)
+{
+  if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPCM) return hipErrorInvalidValue;
+  if (riemann == kRiemannHll)
+    return LaunchVl2dP<HllPolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, s);
+  if (riemann == kRiemannRoe)
+    return LaunchVl2dP<RoePolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, s);
+  if (riemann == kRiemannExact)
+    return LaunchVl2dP<ExactPolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, s);
+  if (riemann == kRiemannHllc)
+    return LaunchVl2dP<PolicyExact>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, s);
+  return hipErrorInvalidValue;
+}
+
 hipError_t LaunchReconGrid(int recon, const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma,
                            int dir, hipStream_t s)
 {
@@ -806,6 +962,9 @@ hipError_t LaunchReconGrid(int recon, const double *U, double *QL, double *QR, i
                        dir);
   } else if (recon == kPLMC) {
     hipLaunchKernelGGL(ReconGridKernel<kPLMC>, dim3(Blocks(nc, 64)), dim3(64), 0, s, U, QL, QR, nx, ny, nz, gamma,
+                       dir);
+  } else if (recon == kPLMP) {
+    hipLaunchKernelGGL(ReconGridKernel<kPLMP>, dim3(Blocks(nc, 64)), dim3(64), 0, s, U, QL, QR, nx, ny, nz, gamma,
                        dir);
   } else {
     return hipErrorInvalidValue;
