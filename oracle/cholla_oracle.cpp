@@ -1079,6 +1079,117 @@ static void AverageCellSingleField(int f, int i, int j, int k, int nx, int ny, s
 }
 
 // Update_Conserved_Variables_3D, src/hydro/hydro_cuda.cu:172-402 (no gravity, no scalars)
+// STATIC_GRAV build: the field id (Parameters::custom_grav, 0 = none) and the grid's place in
+// the domain (x_off.., xbound..), set by orc_set_static_gravity
+static int g_grav           = 0;
+static int g_grav_off[3]    = {0, 0, 0};
+static double g_grav_bnd[3] = {0.0, 0.0, 0.0};
+constexpr double kGN        = 4.49451e-18;  // GN, kpc^3 / M_sun / kyr^2 (src/global/global.h:29)
+constexpr double kMsunCgs   = 1.98847e33;   // MSUN_CGS (src/global/global.h:35)
+
+static double GravPos(int axis, int idx, int n_ghost, double h)
+{
+  return (g_grav_off[axis] + idx - n_ghost + 0.5) * h + g_grav_bnd[axis];
+}
+
+// calc_g_3D, src/gravity/static_grav.h:126-210: case 1, NFW halo + Miyamoto-Nagai disk
+static void StaticG3D(double x, double y, double z, double *gx, double *gy, double *gz)
+{
+  *gx = *gy = *gz = 0.0;
+  if (g_grav != 1) return;
+  const double r_disk = std::sqrt(x * x + y * y), r_halo = std::sqrt(x * x + y * y + z * z);
+  const double M_vir = 1.0e12, M_d = 6.5e10, R_d = 3.5, z_d = 3.5 / 5.0, R_vir = 261., c_vir = 20.0;
+  const double M_h = M_vir - M_d, R_h = R_vir / c_vir;
+  const double phi_0_h = kGN * M_h / (std::log(1.0 + c_vir) - c_vir / (1.0 + c_vir));
+  const double xx      = r_halo / R_h;
+  const double a_halo  = -phi_0_h * (std::log(1 + xx) - xx / (1 + xx)) / (r_halo * r_halo);
+  const double a_halo_r = a_halo * (r_disk / r_halo), a_halo_z = a_halo * (z / r_halo);
+  const double zs       = std::sqrt(z * z + z_d * z_d), b = R_d + zs;
+  const double a_disk_r = -kGN * M_d * r_disk * std::pow(r_disk * r_disk + b * b, -1.5);
+  const double a_disk_z = -kGN * M_d * z * b / (std::pow(r_disk * r_disk + b * b, 1.5) * zs);
+  *gx = (x / r_disk) * (a_disk_r + a_halo_r);
+  *gy = (y / r_disk) * (a_disk_r + a_halo_r);
+  *gz = a_disk_z + a_halo_z;
+}
+
+// calc_g_2D, static_grav.h:60-124: 1 Gresho vortex, 2 Rayleigh-Taylor, 3 Keplerian disk,
+// 4 Kuzmin disk + NFW halo
+static void StaticG2D(double x, double y, double *gx, double *gy)
+{
+  *gx = *gy = 0.0;
+  const double r = std::sqrt(x * x + y * y), phi = std::atan2(y, x);
+  if (g_grav == 1) {
+    if (r < 0.2) {
+      *gx = -std::cos(phi) * 25.0 * r, *gy = -std::sin(phi) * 25.0 * r;
+    } else if (r >= 0.2 && r < 0.4) {
+      *gx = -std::cos(phi) * (4.0 - 20.0 * r + 25.0 * r * r) / r;
+      *gy = -std::sin(phi) * (4.0 - 20.0 * r + 25.0 * r * r) / r;
+    }
+  } else if (g_grav == 2) {
+    *gx = 0, *gy = -1;
+  } else if (g_grav == 3) {
+    const double M = 1 * kMsunCgs;
+    *gx = -std::cos(phi) * kGN * M / (r * r), *gy = -std::sin(phi) * kGN * M / (r * r);
+  } else if (g_grav == 4) {
+    const double M_vir = 1.0e12, M_d = 6.5e10, M_h = M_vir - M_d, R_vir = 261, c_vir = 20, R_s = R_vir / c_vir;
+    const double R_d = 3.5, xx = r / R_s;
+    const double a_d = kGN * M_d * r * std::pow(r * r + R_d * R_d, -1.5);
+    const double a_h = kGN * M_h * (std::log(1 + xx) - xx / (1 + xx)) /
+                       ((std::log(1 + c_vir) - c_vir / (1 + c_vir)) * r * r);
+    const double a = a_d + a_h;
+    *gx = -std::cos(phi) * a, *gy = -std::sin(phi) * a;
+  }
+}
+
+// calc_g_1D, static_grav.h:16-58: case 1, the vertical field of the same halo + disk at
+// cylindrical radius 13.92 kpc
+static double StaticG1D(double x)
+{
+  if (g_grav != 1) return 0.0;
+  const double r_disk = 13.9211647546, r_halo = std::sqrt(x * x + r_disk * r_disk);
+  const double M_vir = 1.0e12, M_d = 6.5e10, M_h = M_vir - M_d, R_vir = 261, c_vir = 20.0, R_h = R_vir / c_vir;
+  const double R_d = 3.5, z_d = 3.5 / 5.0;
+  const double phi_0_h = kGN * M_h / (std::log(1.0 + c_vir) - c_vir / (1.0 + c_vir)), xx = r_halo / R_h;
+  const double a_halo  = -phi_0_h * (std::log(1 + xx) - xx / (1 + xx)) / (r_halo * r_halo);
+  const double zs = std::sqrt(x * x + z_d * z_d), b = R_d + zs;
+  const double a_disk_z = -kGN * M_d * x * b / (std::pow(r_disk * r_disk + b * b, 1.5) * zs);
+  return (x / r_halo) * a_halo + a_disk_z;
+}
+
+// The time-centred source of the full update (hydro_cuda.cu:301-315 and the 1D / 2D forms at
+// :59-67, :142-153): momentum += dt g (d + d_n) / 2, energy += dt g . (d + d_n)(v + v_n) / 4, with
+// d, v at the start of the step and d_n, v_n after the flux update and floor.
+static void GravSource(double *U, size_t n_cells, size_t id, double dt, const double s[4], const double g[3], int dims)
+{
+  const double d = s[0], d_inv = 1.0 / d, d_n = U[id], d_inv_n = 1.0 / d_n;
+  double v[3], vn[3], e = 0.0;
+  for (int a = 0; a < dims; a++) {
+    v[a]  = s[1 + a] * d_inv;
+    vn[a] = U[(1 + a) * n_cells + id] * d_inv_n;
+  }
+  for (int a = 0; a < dims; a++) U[(1 + a) * n_cells + id] += 0.5 * dt * g[a] * (d + d_n);
+  for (int a = 0; a < dims; a++) e = a == 0 ? 0.25 * dt * g[0] * (d + d_n) * (v[0] + vn[0])
+                                            : e + 0.25 * dt * g[a] * (d + d_n) * (v[a] + vn[a]);
+  U[4 * n_cells + id] += e;
+}
+
+void orc_set_static_gravity(int custom_grav, int x_off, int y_off, int z_off, double xbound, double ybound,
+                            double zbound)
+{
+  g_grav     = custom_grav;
+  g_grav_off[0] = x_off, g_grav_off[1] = y_off, g_grav_off[2] = z_off;
+  g_grav_bnd[0] = xbound, g_grav_bnd[1] = ybound, g_grav_bnd[2] = zbound;
+}
+
+// The fields at one position (for the tests): dims 1, 2 or 3
+void orc_static_gravity_field(int dims, double x, double y, double z, double *g)
+{
+  g[0] = g[1] = g[2] = 0.0;
+  if (dims == 1) g[0] = StaticG1D(x);
+  if (dims == 2) StaticG2D(x, y, &g[0], &g[1]);
+  if (dims == 3) StaticG3D(x, y, z, &g[0], &g[1], &g[2]);
+}
+
 void orc_update_3d(double *U, const double *Fx, const double *Fy, const double *Fz, int nx, int ny, int nz,
                    int n_ghost, double dx, double dy, double dz, double dt, int n_fields, int density_floor_on,
                    double density_floor)
@@ -1086,6 +1197,8 @@ void orc_update_3d(double *U, const double *Fx, const double *Fy, const double *
   size_t n_cells = size_t(nx) * ny * nz;
   double dtodx = dt / dx, dtody = dt / dy, dtodz = dt / dz;
   std::vector<uint8_t> repair(n_cells, 0);
+  std::vector<double> start;
+  if (g_grav) start.assign(U, U + 4 * n_cells);  // d and momenta at the start of the step
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = n_ghost; k < nz - n_ghost; k++)
     for (int j = n_ghost; j < ny - n_ghost; j++)
@@ -1115,6 +1228,18 @@ void orc_update_3d(double *U, const double *Fx, const double *Fy, const double *
     for (int j = n_ghost; j < ny - n_ghost; j++)
       for (int i = n_ghost; i < nx - n_ghost; i++)
         if (repair[Idx(i, j, k, nx, ny)]) AverageCellSingleField(0, i, j, k, nx, ny, n_cells, U);
+  if (g_grav) {  // STATIC_GRAV, hydro_cuda.cu:301-315
+    for (int k = n_ghost; k < nz - n_ghost; k++)
+      for (int j = n_ghost; j < ny - n_ghost; j++)
+        for (int i = n_ghost; i < nx - n_ghost; i++) {
+          const size_t id = Idx(i, j, k, nx, ny);
+          const double s4[4] = {start[id], start[n_cells + id], start[2 * n_cells + id], start[3 * n_cells + id]};
+          double g[3];
+          StaticG3D(GravPos(0, i, n_ghost, dx), GravPos(1, j, n_ghost, dy), GravPos(2, k, n_ghost, dz), &g[0], &g[1],
+                    &g[2]);
+          GravSource(U, n_cells, id, dt, s4, g, 3);
+        }
+  }
 }
 
 // Update_Conserved_Variables_1D_half, src/integrators/VL_1D_cuda.cu:158-...
@@ -1188,11 +1313,17 @@ void orc_sync_energies_1d(double *U, int nx, int n_ghost)
 void orc_update_1d(double *U, const double *F, int n_cells, int n_ghost, double dx, double dt, int n_fields)
 {
   double dtodx = dt / dx;
-  for (int id = n_ghost; id < n_cells - n_ghost; id++)
+  for (int id = n_ghost; id < n_cells - n_ghost; id++) {
+    const double s4[4] = {U[id], U[n_cells + id], U[2 * n_cells + id], U[3 * n_cells + id]};
     for (int f = 0; f < n_fields; f++) {
       size_t o = size_t(f) * n_cells;
       U[o + id] += dtodx * (F[o + id - 1] - F[o + id]);
     }
+    if (g_grav) {  // STATIC_GRAV, hydro_cuda.cu:59-67
+      const double g[3] = {StaticG1D(GravPos(0, id, n_ghost, dx)), 0.0, 0.0};
+      GravSource(U, size_t(n_cells), size_t(id), dt, s4, g, 1);
+    }
+  }
 }
 
 // Temperature_Floor_Kernel, src/hydro/hydro_cuda.cu:1117-1153 (DE branch :1146-1151)
@@ -1548,9 +1679,15 @@ void orc_update_2d(double *U, const double *Fx, const double *Fy, int nx, int ny
   for (int j = n_ghost; j < ny - n_ghost; j++)
     for (int i = n_ghost; i < nx - n_ghost; i++) {
       const size_t id = Idx(i, j, 0, nx, ny), imo = id - 1, jmo = id - nx;
+      const double s4[4] = {U[id], U[n_cells + id], U[2 * n_cells + id], U[3 * n_cells + id]};
       for (int f = 0; f < n_fields; f++) {
         const size_t o = size_t(f) * n_cells;
         U[o + id] += dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]);
+      }
+      if (g_grav) {  // STATIC_GRAV, hydro_cuda.cu:142-153
+        double g[3] = {0.0, 0.0, 0.0};
+        StaticG2D(GravPos(0, i, n_ghost, dx), GravPos(1, j, n_ghost, dy), &g[0], &g[1]);
+        GravSource(U, n_cells, id, dt, s4, g, 2);
       }
     }
 }

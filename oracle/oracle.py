@@ -52,6 +52,8 @@ def lib():
             "orc_plmp": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_roe": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_exact": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_set_static_gravity": (None, [i, i, i, i, d, d, d]),
+            "orc_static_gravity_field": (None, [i, d, d, d, _dp]),
             "orc_ppmc_vl": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_hllc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_update_half_3d": (None, [_dp] * 5 + [i, i, i, d, d, d, d, d, i, i, d]),
@@ -169,6 +171,23 @@ def dual_energy(on=True):
 
 def pressure_from_de(E, U_total, U_advected, gamma):
     return lib().orc_pressure_from_de(E, U_total, U_advected, gamma)
+
+
+@contextlib.contextmanager
+def static_gravity(custom_grav, offsets=(0, 0, 0), bounds=(0.0, 0.0, 0.0)):
+    """The STATIC_GRAV build option (Parameters::custom_grav and the grid's offsets and lower
+    bounds) for the calls inside the block."""
+    lib().orc_set_static_gravity(int(custom_grav), *[int(o) for o in offsets], *[float(b) for b in bounds])
+    try:
+        yield
+    finally:
+        lib().orc_set_static_gravity(0, 0, 0, 0, 0.0, 0.0, 0.0)
+
+
+def gravity_field(dims, x, y=0.0, z=0.0):
+    g = np.zeros(3)
+    lib().orc_static_gravity_field(int(dims), float(x), float(y), float(z), ptr(g))
+    return g
 
 
 def run(U, nx, ny, nz, n_ghost, n_fields, dx, dy, dz, gamma, recon, flags, t, tout, max_steps,
