@@ -31,7 +31,7 @@ EXPORTS = [
     "chl_hydro_destroy", "chl_hydro_vl_step", "chl_hydro_update_grid", "chl_hydro_run", "chl_hydro_time_steps",
     "chl_hydro_kernel_times", "chl_hydro_autotune", "chl_hydro_get_config", "chl_calc_dt",
     "chl_apply_temperature_floor", "chl_set_ghost_cells", "chl_set_boundaries", "chl_reconstruct",
-    "chl_hllc_fluxes", "chl_hll_fluxes", "chl_roe_fluxes", "chl_exact_fluxes", "chl_ic_kelvin_helmholtz", "chl_ensemble_create", "chl_ensemble_destroy",
+    "chl_hllc_fluxes", "chl_hll_fluxes", "chl_roe_fluxes", "chl_exact_fluxes", "chl_hydro_set_static_gravity", "chl_ic_kelvin_helmholtz", "chl_ensemble_create", "chl_ensemble_destroy",
     "chl_ensemble_get_config", "chl_ensemble_persistent_pairs", "chl_ensemble_bind", "chl_ensemble_start",
     "chl_ensemble_advance", "chl_ensemble_advance_looped", "chl_ensemble_clocks", "chl_ensemble_autotune",
     "chl_ensemble_layout", "chl_turbulence_modes", "chl_ic_turbulence", "chl_init_gas_energy", "chl_mhd_create",
@@ -125,6 +125,7 @@ def lib():
         "chl_hll_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_roe_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_exact_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
+        "chl_hydro_set_static_gravity": (_i, [_vp, _i, _i, _i, _i, _d, _d, _d]),
         "chl_ic_kelvin_helmholtz": (_i, [_dp, _i, _i, _d, ctypes.c_uint64]),
         "chl_ensemble_create": (_i, [ctypes.POINTER(EnsembleConfig), _i, _ip, ctypes.POINTER(_vp)]),
         "chl_ensemble_destroy": (_i, [_vp]),
@@ -252,6 +253,11 @@ class Hydro:
         c = HydroConfig()
         check(lib().chl_hydro_get_config(self.h, ctypes.byref(c)))
         return c
+
+    def set_static_gravity(self, custom_grav, offsets=(0, 0, 0), bounds=(0.0, 0.0, 0.0)):
+        """STATIC_GRAV: the analytic field custom_grav and the grid's offsets / lower bounds."""
+        check(lib().chl_hydro_set_static_gravity(self.h, int(custom_grav), *[int(o) for o in offsets],
+                                                 *[float(b) for b in bounds]))
 
     def vl_step(self, U, dx, dy, dz, dt, gamma, density_floor=0.0):
         check(lib().chl_hydro_vl_step(self.h, U.ptr, dx, dy, dz, dt, gamma, density_floor))

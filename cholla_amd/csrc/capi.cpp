@@ -41,6 +41,7 @@ struct chl_hydro {
   size_t n_cells = 0;
   bool one_d     = false;
   bool two_d     = false;  // nz == 1 < ny: the unfused 2D step, W holds its scratch
+  chl::GravSpec grav{};    // static gravity (chl_hydro_set_static_gravity); id 0 = none
   int num_cu     = 256;
   void *W        = nullptr;  // half-step primitives (+ the pdV sources with dual energy)
   size_t W_bytes = 0;
@@ -145,6 +146,7 @@ chl::StepArgs MakeStepArgs(chl_hydro *h, double *U, double dx, double dy, double
   a.xcd_order = c.tile_order == 0;
   a.de        = c.dual_energy;
   a.riemann   = SolverKind(c.riemann);
+  a.grav      = h->grav;
   return a;
 }
 
@@ -193,7 +195,8 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
     // dual energy the energy sync (any number of blocks)
     bool fuse_dt = update_grid && (n <= 1024 || c.dual_energy);
     CHL_HIP(chl::LaunchVl1d(U, reinterpret_cast<double *>(h->W), n, ng, c.reconstruction, SolverKind(c.riemann),
-                            c.dual_energy != 0, dx, dt_dev, dt_host, gamma, fuse_dt ? h->dti() : nullptr, h->stream));
+                            c.dual_energy != 0, dx, dt_dev, dt_host, gamma, fuse_dt ? h->dti() : nullptr, h->grav,
+                            h->stream));
     if (update_grid && !fuse_dt) {
       CHL_HIP(hipMemsetAsync(h->dti(), 0, 8, h->stream));
       CHL_HIP(chl::LaunchCalcDt(U, n, 1, 1, ng, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
@@ -204,7 +207,7 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
   if (h->two_d) {  // no temperature floor: the reference's guard admits no 2D cell (hydro_cuda.cu:1130-1131)
     CHL_HIP(chl::LaunchVl2d(U, reinterpret_cast<double *>(h->W), c.nx, c.ny, ng, c.reconstruction,
                             SolverKind(c.riemann), dx, dy, dt_dev, dt_host, gamma, update_grid ? h->dti() : nullptr,
-                            h->stream));
+                            h->grav, h->stream));
     if (fill) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
     return CHL_OK;
   }
@@ -533,6 +536,29 @@ int chl_hydro_kernel_times(chl_hydro *h, double *U, double dx, double dy, double
   if (ms1) *ms1 = float(acc[0] / std::max(1, n_steps));
   if (ms2) *ms2 = float(acc[1] / std::max(1, n_steps));
   if (ms3) *ms3 = float(acc[2] / std::max(1, n_steps));
+  return CHL_OK;
+}
+
+int chl_hydro_set_static_gravity(chl_hydro *h, int custom_grav, int x_off, int y_off, int z_off, double xbound,
+                                 double ybound, double zbound)
+{
+  if (!h) return Fail(CHL_ERR_INVALID, "null handle");
+  const chl_hydro_config &c = h->cfg;
+  const int max_id          = h->two_d ? 4 : 1;  // calc_g_2D has four fields, calc_g_1D / calc_g_3D one
+  if (custom_grav < 0 || custom_grav > max_id) return Fail(CHL_ERR_INVALID, "custom_grav out of range for this box");
+  if (custom_grav != 0 && !h->one_d && !h->two_d &&
+      (c.dual_energy || c.riemann != CHL_RIEMANN_HLLC || c.tile2 >= chl::kNumDeTiles))
+    return Fail(CHL_ERR_UNSUPPORTED, "3D static gravity is built for HLLC without DE, stage-2 tile shapes 0 and 1");
+  chl::GravSpec g{custom_grav, {x_off, y_off, z_off}, {xbound, ybound, zbound}};
+  const chl::GravSpec &o = h->grav;
+  const bool same = o.id == g.id && o.off[0] == g.off[0] && o.off[1] == g.off[1] && o.off[2] == g.off[2] &&
+                    o.bound[0] == g.bound[0] && o.bound[1] == g.bound[1] && o.bound[2] == g.bound[2];
+  if (same) return CHL_OK;
+  h->grav = g;
+  if (h->graph) {  // the captured step holds the old arguments
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
   return CHL_OK;
 }
 

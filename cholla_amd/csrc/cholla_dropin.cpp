@@ -105,13 +105,17 @@ void VL_Algorithm_3D_CUDA(Real *d_conserved, Real *d_grav_potential, int nx, int
                           Real dt, int n_fields, int custom_grav, Real density_floor, Real *host_grav_potential)
 {
   (void)d_grav_potential, (void)x_off, (void)y_off, (void)z_off, (void)xbound, (void)ybound, (void)zbound;
-  (void)custom_grav, (void)host_grav_potential;  // gravity is outside this path
+  (void)custom_grav, (void)host_grav_potential;  // self-gravity (GRAVITY) is outside this path
 #ifdef MHD
   (void)n_fields;  // 8: the hydro fields and the three face-centred magnetic fields
   chl_mhd *m = AcquireMhd(nx, ny, nz, n_ghost);
   Check(chl_mhd_vl_step(m, d_conserved, dx, dy, dz, dt, gama, density_floor), "VL_Algorithm_3D_CUDA");
 #else
   chl_hydro *h = Acquire(g_vl3d, nx, ny, nz, n_ghost, n_fields);
+  #ifdef STATIC_GRAV
+  Check(chl_hydro_set_static_gravity(h, custom_grav, x_off, y_off, z_off, xbound, ybound, zbound),
+        "VL_Algorithm_3D_CUDA");
+  #endif
   Check(chl_hydro_vl_step(h, d_conserved, dx, dy, dz, dt, gama, density_floor), "VL_Algorithm_3D_CUDA");
 #endif
 }
@@ -133,6 +137,9 @@ void VL_Algorithm_1D_CUDA(Real *d_conserved, int nx, int x_off, int n_ghost, Rea
 {
   (void)x_off, (void)xbound, (void)custom_grav;
   chl_hydro *h = Acquire(g_vl1d, nx, 1, 1, n_ghost, n_fields);
+#ifdef STATIC_GRAV
+  Check(chl_hydro_set_static_gravity(h, custom_grav, x_off, 0, 0, xbound, 0.0, 0.0), "VL_Algorithm_1D_CUDA");
+#endif
   Check(chl_hydro_vl_step(h, d_conserved, dx, dx, dx, dt, gama, 0.0), "VL_Algorithm_1D_CUDA");
 }
 

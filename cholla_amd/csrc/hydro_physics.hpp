@@ -43,23 +43,31 @@ struct PolicyExact {
   static constexpr bool kExact = true;
   static constexpr int kId     = 0;
   static constexpr int kSolver = kRiemannHllc;
+  static constexpr bool kGrav  = false;
 };
 struct PolicyFast {
   using T                      = double;
   static constexpr bool kExact = false;
   static constexpr int kId     = 1;
   static constexpr int kSolver = kRiemannHllc;
+  static constexpr bool kGrav  = false;
 };
 struct PolicyMixed {
   using T                      = float;
   static constexpr bool kExact = false;
   static constexpr int kId     = 2;
   static constexpr int kSolver = kRiemannHllc;
+  static constexpr bool kGrav  = false;
 };
 // An arithmetic form with another Riemann solver in place of HLLC; everything else as Base.
 template <class Base, int S>
 struct SolverPolicy : Base {
   static constexpr int kSolver = S;
+};
+// An arithmetic form whose full update adds the static-gravity source (STATIC_GRAV).
+template <class Base>
+struct GravPolicy : Base {
+  static constexpr bool kGrav = true;
 };
 template <class Base>
 using HllPolicy = SolverPolicy<Base, kRiemannHll>;

@@ -460,7 +460,7 @@ __device__ __forceinline__ void Recon1D(const double *W, int n, int c, double ga
 // on top of the full-step pdV source Vl1dPdvKernel added.
 template <class P, int RK, bool DE>
 __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, double dx, const double *dt_dev,
-                                 double dt_host, double gamma, unsigned long long *dti)
+                                 double dt_host, double gamma, unsigned long long *dti, GravSpec grav)
 {
   int id       = blockIdx.x * blockDim.x + threadIdx.x;
   double m     = 0.0;
@@ -484,6 +484,11 @@ __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, doub
     un[2] = U[2 * n + id] + dtodx * (fm.my - fp.my);
     un[3] = U[3 * n + id] + dtodx * (fm.mz - fp.mz);
     un[4] = U[4 * n + id] + dtodx * (fm.E - fp.E);
+    if (grav.id != 0) {  // STATIC_GRAV, hydro_cuda.cu:59-67
+      const double start[4] = {U[id], U[n + id], U[2 * n + id], U[3 * n + id]};
+      const double g[1]     = {GravField1D(grav.id, grav.Pos(0, id, ng, dx))};
+      GravSource<1>(un, start, g, dt);
+    }
     for (int q = 0; q < 5; q++) U[q * n + id] = un[q];
     if constexpr (DE) {
       U[5 * n + id] += dtodx * (gm - gp);
@@ -598,7 +603,7 @@ __global__ void Vl2dHalfKernel(const double *U, double *Uh, const double *Fx, co
 // (hydro_cuda.cu:474-510) of the updated cell into *dti.
 __global__ void Vl2dUpdateKernel(double *U, const double *Fx, const double *Fy, int nx, int ny, int ng, double dx,
                                  double dy, const double *dt_dev, double dt_host, double gamma,
-                                 unsigned long long *dti)
+                                 unsigned long long *dti, GravSpec grav)
 {
   const long nc = long(nx) * ny;
   const long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -609,12 +614,19 @@ __global__ void Vl2dUpdateKernel(double *U, const double *Fx, const double *Fy, 
       const double dt = dt_dev ? *dt_dev : dt_host, dtodx = dt / dx, dtody = dt / dy;
       const long imo = id - 1, jmo = id - nx;
       double un[5];
+      const double start[4] = {U[id], U[nc + id], U[2 * nc + id], U[3 * nc + id]};
 #pragma unroll
       for (int f = 0; f < 5; f++) {
         const long o = f * nc;
         un[f]        = U[o + id] + (dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]));
-        U[o + id]    = un[f];
       }
+      if (grav.id != 0) {  // STATIC_GRAV, hydro_cuda.cu:142-153
+        double g[2];
+        GravField2D(grav.id, grav.Pos(0, i, ng, dx), grav.Pos(1, j, ng, dy), g[0], g[1]);
+        GravSource<2>(un, start, g, dt);
+      }
+#pragma unroll
+      for (int f = 0; f < 5; f++) U[f * nc + id] = un[f];
       const double d = un[0], dinv = 1.0 / d;
       const double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
       double pr       = (un[4] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
@@ -858,7 +870,7 @@ hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const doubl
 
 template <class P, bool DE>
 static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, double dx, const double *dt_dev,
-                              double dt, double gamma, unsigned long long *dti, hipStream_t s)
+                              double dt, double gamma, unsigned long long *dti, const GravSpec &grav, hipStream_t s)
 {
   const unsigned nb = Blocks(n, 256);
   hipLaunchKernelGGL((Vl1dStage1Kernel<P, DE>), dim3(nb), dim3(256), 0, s, U, W, n, dx, dt_dev, dt, gamma, dti);
@@ -866,10 +878,10 @@ static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, do
     hipLaunchKernelGGL(Vl1dPdvKernel, dim3(nb), dim3(256), 0, s, U, n, ng, dx, dt_dev, dt, gamma);
     if (recon == kPPMC) {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMC, true>), dim3(nb), dim3(256), 0, s, U, W, n, ng, dx, dt_dev, dt,
-                         gamma, nullptr);
+                         gamma, nullptr, grav);
     } else {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMC, true>), dim3(nb), dim3(256), 0, s, U, W, n, ng, dx, dt_dev, dt,
-                         gamma, nullptr);
+                         gamma, nullptr, grav);
     }
     hipLaunchKernelGGL(Vl1dSelectKernel, dim3(nb), dim3(256), 0, s, U, n, ng);
     hipLaunchKernelGGL(Vl1dSyncKernel, dim3(nb), dim3(256), 0, s, U, n, ng, dx, gamma, dti);
@@ -880,41 +892,43 @@ static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, do
     const int tpb      = dti ? 1024 : 256;
     if (recon == kPPMC) {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMC, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
-                         gamma, dti);
+                         gamma, dti, grav);
     } else if (recon == kPLMC) {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMC, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
-                         gamma, dti);
+                         gamma, dti, grav);
     } else if (recon == kPLMP) {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMP, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
-                         gamma, dti);
+                         gamma, dti, grav);
     } else {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPCM, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
-                         gamma, dti);
+                         gamma, dti, grav);
     }
     return hipGetLastError();
   }
 }
 
 hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, bool de, double dx,
-                      const double *dt_dev, double dt, double gamma, unsigned long long *dti, hipStream_t s)
+                      const double *dt_dev, double dt, double gamma, unsigned long long *dti, const GravSpec &grav,
+                      hipStream_t s)
 {
   if (dti && n > 1024 && !de) return hipErrorInvalidValue;
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPCM) return hipErrorInvalidValue;
   if (de) {  // HLLC with PLMC or PPMC
     if (riemann != kRiemannHllc || (recon != kPPMC && recon != kPLMC)) return hipErrorInvalidValue;
-    return LaunchVl1dP<PolicyExact, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+    return LaunchVl1dP<PolicyExact, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
   }
-  if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
-  if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+  if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
+  if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
   if (riemann == kRiemannExact)
-    return LaunchVl1dP<ExactPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
-  if (riemann == kRiemannHllc) return LaunchVl1dP<PolicyExact, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, s);
+    return LaunchVl1dP<ExactPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
+  if (riemann == kRiemannHllc) return LaunchVl1dP<PolicyExact, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
   return hipErrorInvalidValue;
 }
 
 template <class P>
 static hipError_t LaunchVl2dP(double *U, double *scratch, int nx, int ny, int ng, int recon, double dx, double dy,
-                              const double *dt_dev, double dt, double gamma, unsigned long long *dti, hipStream_t s)
+                              const double *dt_dev, double dt, double gamma, unsigned long long *dti,
+                              const GravSpec &grav, hipStream_t s)
 {
   const long nc = long(nx) * ny, fs = 5 * nc;
   double *Uh = scratch, *QLx = scratch + fs, *QRx = scratch + 2 * fs, *QLy = scratch + 3 * fs;
@@ -932,24 +946,25 @@ static hipError_t LaunchVl2dP(double *U, double *scratch, int nx, int ny, int ng
     hipLaunchKernelGGL(RiemannGridKernel<P>, dim3(Blocks(nc, 64)), dim3(64), 0, s, QLy, QRy, Fy, nx, ny, 1, gamma, 1);
   }
   hipLaunchKernelGGL(Vl2dUpdateKernel, dim3(nb), dim3(256), 0, s, U, Fx, Fy, nx, ny, ng, dx, dy, dt_dev, dt, gamma,
-                     dti);
+                     dti, grav);
   return hipGetLastError();
 }
 
 // VL_Algorithm_2D_CUDA (VL_2D_cuda.cu:24-168) in the exact form; scratch holds 7 x 5 x nx*ny
 // doubles (half state, four interface arrays, two flux arrays).
 hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, double dx, double dy,
-                      const double *dt_dev, double dt, double gamma, unsigned long long *dti, hipStream_t s)
+                      const double *dt_dev, double dt, double gamma, unsigned long long *dti, const GravSpec &grav,
+                      hipStream_t s)
 {
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPCM) return hipErrorInvalidValue;
   if (riemann == kRiemannHll)
-    return LaunchVl2dP<HllPolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, s);
+    return LaunchVl2dP<HllPolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   if (riemann == kRiemannRoe)
-    return LaunchVl2dP<RoePolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, s);
+    return LaunchVl2dP<RoePolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   if (riemann == kRiemannExact)
-    return LaunchVl2dP<ExactPolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, s);
+    return LaunchVl2dP<ExactPolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   if (riemann == kRiemannHllc)
-    return LaunchVl2dP<PolicyExact>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, s);
+    return LaunchVl2dP<PolicyExact>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   return hipErrorInvalidValue;
 }
 

@@ -27,6 +27,7 @@
 #include <cstdint>
 
 #include "hydro_physics.hpp"
+#include "static_gravity.hpp"
 
 namespace chl
 {
@@ -72,6 +73,7 @@ struct StepArgs {
   int xcd_order;           // 1: give each XCD a contiguous run of tiles (see TileOf)
   int de;                  // dual energy (host-side dispatch only; the kernels are templated on it)
   int riemann;             // kRiemannHllc / kRiemannHll (host-side dispatch only, as de)
+  GravSpec grav;           // static gravity (kernels under a GravPolicy only); grav.id 0 = none
 };
 
 enum : int {
@@ -774,6 +776,7 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
     if (own) {
       const size_t id = Lin(i, j, k, nx, ny);
       double un[NC];
+      double ustart[4];  // density and momenta at the start of the step (static gravity)
       {
         int fa = ty * TX + tx;
         // U += ((dtodx dFx + dtody dFy) + dtodz dFz), hydro_cuda.cu:241-257; GasEnergy
@@ -782,6 +785,9 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
         for (int q = 0; q < NC; q++) {
           double axy = acc[q] + dtody * (double(fl[q][fa]) - double(fl[q][fa + TX]));
           double u0  = U[q * nc + id];
+          if constexpr (P::kGrav) {
+            if (q < 4) ustart[q] = u0;
+          }
           if (DE && q == kGasEnergy) u0 = u0 + a.pdv[id];
           un[q] = u0 + (axy + dtodz * (double(fz_lo[q]) - double(fz_hi[q])));
         }
@@ -800,6 +806,12 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
                    U[id + size_t(nx) * ny]) /
                   6;
         }
+      }
+      if constexpr (P::kGrav) {  // STATIC_GRAV, hydro_cuda.cu:301-315
+        double g[3];
+        GravField3D(a.grav.id, a.grav.Pos(0, i, ng, a.dx), a.grav.Pos(1, j, ng, a.dy), a.grav.Pos(2, k, ng, a.dz), g[0],
+                    g[1], g[2]);
+        GravSource<3>(un, ustart, g, dt);
       }
       if constexpr (DE) {
 #pragma unroll

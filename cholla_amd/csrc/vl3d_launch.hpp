@@ -97,6 +97,10 @@ hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 template <class P, int RK>
 hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
 {
+  if (a.grav.id != 0) {  // static gravity: HLLC, the first two tile shapes, without DE
+    if (a.de || tile > 1 || a.riemann != kRiemannHllc) return hipErrorInvalidValue;
+    return tile == 0 ? LaunchStage2T<GravPolicy<P>, RK, 32, 8>(a, s) : LaunchStage2T<GravPolicy<P>, RK, 16, 16>(a, s);
+  }
   // PCM, PLMP, HLL, Roe and the exact solver: the first two tile shapes, without DE
   auto narrow = [&]() {
     if (a.de || tile > 1) return hipErrorInvalidValue;

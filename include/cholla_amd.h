@@ -147,6 +147,15 @@ int chl_set_ghost_cells(double *c_head, int nx, int ny, int nz, int n_fields, in
  * all faces in one launch */
 int chl_set_boundaries(double *dev_conserved, int nx, int ny, int nz, int n_fields, int n_ghost, const int flags[6]);
 
+/* STATIC_GRAV build option (src/gravity/static_grav.h; the source of hydro_cuda.cu:59-67,
+ * 142-153, 301-315): the analytic field Parameters::custom_grav selects (0 none; 1D and 3D:
+ * 1 NFW halo + Miyamoto-Nagai disk; 2D: 1 Gresho, 2 Rayleigh-Taylor, 3 Keplerian, 4 Kuzmin
+ * disk + NFW halo) and the grid's offsets and lower bounds, as VL_Algorithm_*_CUDA receive
+ * them (x_off.., xbound..).  Applies to every later step of the handle.  3D: HLLC without
+ * dual energy, stage-2 tile shapes 0 and 1. */
+int chl_hydro_set_static_gravity(chl_hydro *h, int custom_grav, int x_off, int y_off, int z_off, double xbound,
+                                 double ybound, double zbound);
+
 /* ---- reference-shaped stage kernels (PolicyExact), for checking the device
  * physics against the reference's unit-test vectors */
 /* PLMC_cuda / PPMC_VL (plmc_cuda.cu:24, ppmc_cuda.cu:540) on device arrays */

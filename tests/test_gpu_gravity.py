@@ -1,0 +1,132 @@
+"""GPU parity of the static-gravity option (STATIC_GRAV; chl_hydro_set_static_gravity) against
+the oracle (tests/test_oracle_gravity.py pins it): 3D in the fused stage 2 (every form), 2D
+and 1D in their integrators.  The fields use log, pow, atan2, cos and sin, whose last bit
+differs between the host's libm and the device's math library, so the bar is a per-field
+relative L1 of 1e-12 in the FP64 forms (1e-5 mixed); everything else in the step is
+bit-identical.  Offsets and bounds are non-trivial so the cell positions are checked too."""
+import sys
+
+import numpy as np
+import pytest
+
+import cholla_amd as chl
+from oracle import oracle as orc
+from test_gpu_parity import TOL, scaled_l1
+
+pytestmark = pytest.mark.gpu
+G = 5.0 / 3.0
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _require_gpu():
+    if chl.device_count() < 1:
+        pytest.fail("no HIP device visible: the GPU tests need a real MI355X")
+
+
+def disk_box(shape, ng, seed=3, p0=1e-3, vamp=1e-3):
+    """A gas at pressure ~p0 moving at ~vamp (the MW fields are in kpc / kyr)."""
+    rng = np.random.default_rng(seed)
+    N = tuple(n + 2 * ng for n in shape)
+    d = 1.0 + 0.1 * rng.uniform(-1, 1, N)
+    v = vamp * rng.uniform(-1, 1, (3,) + N)
+    p = p0 * (1.0 + 0.1 * rng.uniform(-1, 1, N))
+    U = np.stack([d, d * v[0], d * v[1], d * v[2], p / (G - 1) + 0.5 * d * (v ** 2).sum(0)]).reshape(-1).copy()
+    return U, N
+
+
+def rel_l1(A, B, nf=5):
+    A, B = A.reshape(nf, -1), B.reshape(nf, -1)
+    return [np.abs(A[f] - B[f]).sum() / max(np.abs(B[f]).sum(), 1e-300) for f in range(nf)]
+
+
+def run_3d(U, N, ng, h, precision, grav, off, bnd):
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(N, N, N, ng, chl.RECON_PPMC, precision, tile2=1)
+    hy.set_static_gravity(grav, off, bnd)
+    s, t = hy.run(Ud, h, h, h, G, 0.0, 1e30, 5, use_graph=True)
+    hy.close()
+    return Ud.download(), s, t
+
+
+def oracle_3d(U, N, ng, h, grav, off, bnd):
+    Uo = U.copy()
+    with orc.static_gravity(grav, off, bnd):
+        s, t = orc.run(Uo, N, N, N, ng, 5, h, h, h, G, orc.PPMC, [1] * 6, 0.0, 1e30, 5)
+    return Uo, s, t
+
+
+@pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
+def test_3d_static_gravity(precision):
+    """A Mach ~0.3 flow (velocities ~1e-2 kpc/kyr, sound speed 0.03), which the ~1e-7 kpc/kyr^2
+    field changes measurably over five steps."""
+    n, ng = 16, 4
+    U, (N, _, _) = disk_box((n, n, n), ng, vamp=1e-2)
+    orc.set_boundaries(U, N, N, N, 5, ng, [1] * 6)
+    h, off, bnd = 0.6, (3, -2, 5), (-7.0, 2.0, -3.0)
+    say("3D static gravity precision", precision)
+    Uo, s_o, t_o = oracle_3d(U, N, ng, h, 1, off, bnd)
+    A, s, t = run_3d(U, N, ng, h, precision, 1, off, bnd)
+    assert s == s_o == 5 and abs(t - t_o) <= TOL[precision] * t_o
+    l1 = scaled_l1(A, Uo, N, N, N, ng)
+    say("  scaled L1", ["%.2e" % x for x in l1])
+    assert max(l1) <= TOL[precision], l1
+    Uz, _, _ = oracle_3d(U, N, ng, h, 0, off, bnd)
+    assert max(scaled_l1(Uz, Uo, N, N, N, ng)) > 1e-6  # the source is live
+
+
+@pytest.mark.parametrize("grav", [1, 2, 3, 4])
+def test_2d_static_gravity(grav):
+    n, ng = 24, 3
+    U, (N, _) = disk_box((n, n), ng, seed=grav, p0=1.0)  # a sound speed ~1: dt g stays small
+    U = U.reshape(5, N, N)
+    U[0] += 1.0  # keep the Keplerian and RT cases well away from the density floor
+    U = U.reshape(-1).copy()
+    orc.set_boundaries(U, N, N, 1, 5, ng, [1, 1, 1, 1, 0, 0])
+    h = {1: 0.05, 2: 0.05, 3: 1.0, 4: 0.6}[grav]
+    off, bnd = (1, -3, 0), (-0.6, -0.4, 0.0)
+    if grav == 3:  # a patch 1e8 cm from a solar mass: g ~ 0.5
+        bnd = (1.0e8, 1.0e8, 0.0)
+    Uo = U.copy()
+    with orc.static_gravity(grav, off, bnd):
+        s_o, t_o = orc.run(Uo, N, N, 1, ng, 5, h, h, h, G, orc.PLMC, [1, 1, 1, 1, 0, 0], 0.0, 1e30, 6)
+    say("2D static gravity", grav)
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(N, N, 1, ng, chl.RECON_PLMC, chl.PREC_FP64_EXACT, bc=[1] * 6)
+    hy.set_static_gravity(grav, off, bnd)
+    s, t = hy.run(Ud, h, h, h, G, 0.0, 1e30, 6, use_graph=False)
+    hy.close()
+    assert s == s_o == 6
+    l1 = rel_l1(Ud.download(), Uo)
+    assert max(l1) <= 1e-12, l1
+
+
+def test_1d_static_gravity():
+    n, ng = 64, 3
+    U, (N,) = disk_box((n,), ng, seed=11)
+    orc.set_boundaries(U, N, 1, 1, 5, ng, [1, 1, 0, 0, 0, 0])
+    h, off, bnd = 0.1, (4, 0, 0), (-3.5, 0.0, 0.0)
+    Uo = U.copy()
+    with orc.static_gravity(1, off, bnd):
+        s_o, t_o = orc.run(Uo, N, 1, 1, ng, 5, h, h, h, G, orc.PLMC, [1, 1, 0, 0, 0, 0], 0.0, 1e30, 8)
+    say("1D static gravity")
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(N, 1, 1, ng, chl.RECON_PLMC, chl.PREC_FP64_EXACT, bc=[1] * 6)
+    hy.set_static_gravity(1, off, bnd)
+    s, t = hy.run(Ud, h, h, h, G, 0.0, 1e30, 8, use_graph=False)
+    hy.close()
+    assert s == s_o == 8
+    assert max(rel_l1(Ud.download(), Uo)) <= 1e-12
+
+
+def test_static_gravity_limits():
+    hy = chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64)  # default stage-2 tile 3
+    with pytest.raises(chl.ChollaError, match="tile shapes 0 and 1"):
+        hy.set_static_gravity(1)
+    with pytest.raises(chl.ChollaError, match="out of range"):
+        hy.set_static_gravity(2)
+    hy.close()
