@@ -1665,7 +1665,68 @@ void orc_update_half_2d(const double *U, double *Uh, const double *Fx, const dou
       for (int f = 0; f < n_fields; f++) {
         const size_t o = size_t(f) * n_cells;
         Uh[o + id]     = U[o + id] + dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]);
+        if (g_de && f == kGE) {  // the half-step pdV source (VL_2D_cuda.cu:203-218, 248-252)
+          const double d = U[id], d_inv = 1.0 / d;
+          const double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+          const double P  = (U[4 * n_cells + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma_1d() - 1.0);
+          const size_t ipo = id + 1, jpo = id + nx;
+          const double vx_imo = U[n_cells + imo] / U[imo], vx_ipo = U[n_cells + ipo] / U[ipo];
+          const double vy_jmo = U[2 * n_cells + jmo] / U[jmo], vy_jpo = U[2 * n_cells + jpo] / U[jpo];
+          Uh[o + id] = U[o + id] + dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]) +
+                       0.5 * P * (dtodx * (vx_imo - vx_ipo) + dtody * (vy_jmo - vy_jpo));
+        }
       }
+    }
+}
+
+// Partial_Update_Advected_Internal_Energy_2D, hydro_cuda.cu:702-750
+void orc_partial_update_ge_2d(double *U, int nx, int ny, int n_ghost, double dx, double dy, double dt, double gamma)
+{
+  const size_t n_cells = size_t(nx) * ny;
+  const double dtodx = dt / dx, dtody = dt / dy;
+  for (int j = n_ghost; j < ny - n_ghost; j++)
+    for (int i = n_ghost; i < nx - n_ghost; i++) {
+      const size_t id = Idx(i, j, 0, nx, ny), imo = id - 1, ipo = id + 1, jmo = id - nx, jpo = id + nx;
+      const double d = U[id], d_inv = 1.0 / d;
+      const double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+      const double E = U[4 * n_cells + id], GE = U[kGE * n_cells + id];
+      const double E_kin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
+      const double P     = std::fmax(PressureFromDE(E, E - E_kin, GE, gamma), kTiny);
+      const double vx_imo = U[n_cells + imo] / U[imo], vx_ipo = U[n_cells + ipo] / U[ipo];
+      const double vy_jmo = U[2 * n_cells + jmo] / U[jmo], vy_jpo = U[2 * n_cells + jpo] / U[jpo];
+      U[kGE * n_cells + id] += 0.5 * P * (dtodx * (vx_imo - vx_ipo) + dtody * (vy_jmo - vy_jpo));
+    }
+}
+
+// Select_Internal_Energy_2D and Sync_Energies_2D, hydro_cuda.cu:887-935, 1042-1068
+void orc_select_sync_2d(double *U, int nx, int ny, int n_ghost)
+{
+  const size_t n_cells = size_t(nx) * ny;
+  std::vector<double> sel(n_cells, 0.0);
+  for (int j = n_ghost; j < ny - n_ghost; j++)
+    for (int i = n_ghost; i < nx - n_ghost; i++) {
+      const size_t id  = Idx(i, j, 0, nx, ny);
+      const size_t imo = Idx(std::max(i - 1, n_ghost), j, 0, nx, ny), ipo = Idx(std::min(i + 1, nx - n_ghost - 1), j, 0, nx, ny);
+      const size_t jmo = Idx(i, std::max(j - 1, n_ghost), 0, nx, ny), jpo = Idx(i, std::min(j + 1, ny - n_ghost - 1), 0, nx, ny);
+      const double d = U[id], d_inv = 1.0 / d;
+      const double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+      const double E = U[4 * n_cells + id], U_adv = U[kGE * n_cells + id];
+      const double U_tot = E - 0.5 * d * (vx * vx + vy * vy + vz * vz);
+      const double *En   = U + 4 * n_cells;
+      double Emax        = std::fmax(En[imo], E);
+      Emax               = std::fmax(Emax, En[ipo]);
+      Emax               = std::fmax(Emax, En[jmo]);
+      Emax               = std::fmax(Emax, En[jpo]);
+      const double u     = ((U_tot / E > kEta1) || (U_tot / Emax > kEta2)) ? U_tot : U_adv;
+      sel[id]            = std::fmax(u, kTiny);
+    }
+  for (int j = n_ghost; j < ny - n_ghost; j++)
+    for (int i = n_ghost; i < nx - n_ghost; i++) {
+      const size_t id = Idx(i, j, 0, nx, ny);
+      U[kGE * n_cells + id] = sel[id];
+      const double d = U[id], d_inv = 1.0 / d;
+      const double vx = U[n_cells + id] * d_inv, vy = U[2 * n_cells + id] * d_inv, vz = U[3 * n_cells + id] * d_inv;
+      U[4 * n_cells + id] = 0.5 * d * (vx * vx + vy * vy + vz * vz) + sel[id];
     }
 }
 
@@ -1703,6 +1764,7 @@ void orc_vl2d(double *U, int nx, int ny, int n_ghost, double dx, double dy, doub
              n_fields);
   RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, 1, gamma, 0);
   RiemannFluxes(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, 1, gamma, 1);
+  g_gamma_1d = gamma;  // the half update's DE term needs it
   orc_update_half_2d(U, s.Uh.data(), s.Fx.data(), s.Fy.data(), nx, ny, dx, dy, 0.5 * dt, n_fields);
   if (recon == ORC_PLMC) {
     orc_plmc(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, 1, gamma, 0);
@@ -1719,7 +1781,9 @@ void orc_vl2d(double *U, int nx, int ny, int n_ghost, double dx, double dy, doub
   }
   RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, 1, gamma, 0);
   RiemannFluxes(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, 1, gamma, 1);
+  if (g_de) orc_partial_update_ge_2d(U, nx, ny, n_ghost, dx, dy, dt, gamma);  // VL_2D_cuda.cu:140-144
   orc_update_2d(U, s.Fx.data(), s.Fy.data(), nx, ny, n_ghost, dx, dy, dt, n_fields);
+  if (g_de) orc_select_sync_2d(U, nx, ny, n_ghost);  // VL_2D_cuda.cu:148-152
 }
 
 // VL_Algorithm_1D_CUDA, src/integrators/VL_1D_cuda.cu:30-146 (HLLC)
@@ -1767,7 +1831,6 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
   int steps  = 0;
   bool one_d = ny == 1 && nz == 1;
   if (g_de && n_fields != 6) return -1;  // DE: GasEnergy as field 5
-  if (g_de && !one_d && nz == 1) return -1;  // DE is not restated for 2D
   while (t < tout && steps < max_steps) {
     double dt = kCfl / dti;
     if (t + dt > tout) dt = tout - t;
