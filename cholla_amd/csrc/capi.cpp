@@ -100,7 +100,6 @@ int CheckConfig(const chl_hydro_config &c)
     if (c.ny < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "2D boxes need ny > 2*n_ghost");
     if (c.precision != CHL_PREC_FP64_EXACT)
       return Fail(CHL_ERR_UNSUPPORTED, "2D boxes run the FP64 reference-order form only");
-    if (c.dual_energy) return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for 3D and 1D boxes");
   } else if (!one_d) {
     if (c.ny < 2 * ng + 1 || c.nz < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "3D boxes need ny, nz > 2*n_ghost");
     if (c.tile1 >= chl::kNumTiles || c.tile2 >= chl::kNumTiles) return Fail(CHL_ERR_INVALID, "tile index out of range");
@@ -206,8 +205,8 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
   }
   if (h->two_d) {  // no temperature floor: the reference's guard admits no 2D cell (hydro_cuda.cu:1130-1131)
     CHL_HIP(chl::LaunchVl2d(U, reinterpret_cast<double *>(h->W), c.nx, c.ny, ng, c.reconstruction,
-                            SolverKind(c.riemann), dx, dy, dt_dev, dt_host, gamma, update_grid ? h->dti() : nullptr,
-                            h->grav, h->stream));
+                            SolverKind(c.riemann), c.dual_energy != 0, dx, dy, dt_dev, dt_host, gamma,
+                            update_grid ? h->dti() : nullptr, h->grav, h->stream));
     if (fill) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
     return CHL_OK;
   }
@@ -376,7 +375,7 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   size_t elem      = c.precision == CHL_PREC_MIXED ? sizeof(float) : sizeof(double);
   const size_t w_fields = 5 + size_t(c.dual_energy);
   h->W_bytes            = w_fields * h->n_cells * elem + (c.dual_energy ? h->n_cells * sizeof(double) : 0);
-  if (h->two_d) h->W_bytes = 35 * h->n_cells * sizeof(double);  // half state, 4 interface and 2 flux arrays
+  if (h->two_d) h->W_bytes = 42 * h->n_cells * sizeof(double);  // half state, 4 interface, 2 flux arrays of 6 fields
   hipError_t e     = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&h->W, h->W_bytes);
   if (e == hipSuccess && c.dual_energy)

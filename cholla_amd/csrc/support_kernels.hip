@@ -560,7 +560,7 @@ __global__ void Vl1dSyncKernel(double *U, int n, int ng, double dx, double gamma
 // PCM_Reconstruction_2D with the Riemann solve of P (VL_2D_cuda.cu:49-71, and stage 2 of
 // the PCM build): the x face i+1/2 and y face j+1/2 of every cell with a neighbour there,
 // from the conserved states as they are.  Thread 0 clears the step's dt reduction.
-template <class P>
+template <class P, bool DE = false>
 __global__ void Vl2dPcmFluxKernel(const double *U, double *Fx, double *Fy, int nx, int ny, double gamma,
                                   unsigned long long *dti_reset)
 {
@@ -570,19 +570,24 @@ __global__ void Vl2dPcmFluxKernel(const double *U, double *Fx, double *Fy, int n
   if (id >= nc) return;
   const int j = int(id / nx), i = int(id - long(j) * nx);
   auto state = [&](long c, int o1, int o2, int o3) {
-    return RStateFromCons<PolicyExact>(U[c], U[o1 * nc + c], U[o2 * nc + c], U[o3 * nc + c], U[4 * nc + c], gamma);
+    return RStateFromCons<PolicyExact, DE>(U[c], U[o1 * nc + c], U[o2 * nc + c], U[o3 * nc + c], U[4 * nc + c], gamma,
+                                           DE ? U[5 * nc + c] : 0.0);
   };
-  auto store = [&](double *F, const Cons<double> &f, int o1, int o2, int o3) {
+  auto solve = [&](double *F, long other, int o1, int o2, int o3) {
+    double fge     = 0.0;
+    Cons<double> f = RiemannFlux<P, DE>(state(id, o1, o2, o3), state(other, o1, o2, o3), gamma, &fge);
     F[id] = f.d, F[o1 * nc + id] = f.mx, F[o2 * nc + id] = f.my, F[o3 * nc + id] = f.mz, F[4 * nc + id] = f.E;
+    if constexpr (DE) F[5 * nc + id] = fge;
   };
-  if (i < nx - 1) store(Fx, RiemannFlux<P>(state(id, 1, 2, 3), state(id + 1, 1, 2, 3), gamma), 1, 2, 3);
-  if (j < ny - 1) store(Fy, RiemannFlux<P>(state(id, 2, 3, 1), state(id + nx, 2, 3, 1), gamma), 2, 3, 1);
+  if (i < nx - 1) solve(Fx, id + 1, 1, 2, 3);
+  if (j < ny - 1) solve(Fy, id + nx, 2, 3, 1);
 }
 
 // Update_Conserved_Variables_2D_half, VL_2D_cuda.cu:183-256 (no DE): cells 1..n-2 along
 // both axes, the y term added after the x term.
+template <bool DE = false>
 __global__ void Vl2dHalfKernel(const double *U, double *Uh, const double *Fx, const double *Fy, int nx, int ny,
-                               double dx, double dy, const double *dt_dev, double dt_host)
+                               double dx, double dy, const double *dt_dev, double dt_host, double gamma)
 {
   const long nc = long(nx) * ny;
   const long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -592,15 +597,91 @@ __global__ void Vl2dHalfKernel(const double *U, double *Uh, const double *Fx, co
   const double dt = 0.5 * (dt_dev ? *dt_dev : dt_host), dtodx = dt / dx, dtody = dt / dy;
   const long imo = id - 1, jmo = id - nx;
 #pragma unroll
-  for (int f = 0; f < 5; f++) {
+  for (int f = 0; f < 5 + int(DE); f++) {
     const long o = f * nc;
     Uh[o + id]   = U[o + id] + dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]);
   }
+  if constexpr (DE) {  // the half-step pdV source (VL_2D_cuda.cu:203-218, 248-252)
+    const double d = U[id], d_inv = 1.0 / d;
+    const double vx = U[nc + id] * d_inv, vy = U[2 * nc + id] * d_inv, vz = U[3 * nc + id] * d_inv;
+    const double pr = (U[4 * nc + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+    const long ipo = id + 1, jpo = id + nx, o = 5 * nc;
+    const double vx_imo = U[nc + imo] / U[imo], vx_ipo = U[nc + ipo] / U[ipo];
+    const double vy_jmo = U[2 * nc + jmo] / U[jmo], vy_jpo = U[2 * nc + jpo] / U[jpo];
+    Uh[o + id] = U[o + id] + dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]) +
+                 0.5 * pr * (dtodx * (vx_imo - vx_ipo) + dtody * (vy_jmo - vy_jpo));
+  }
+}
+
+// Partial_Update_Advected_Internal_Energy_2D (hydro_cuda.cu:702-750): the full-step pdV
+// source from the start-of-step state (writes GasEnergy only).
+__global__ void Vl2dPdvKernel(double *U, int nx, int ny, int ng, double dx, double dy, const double *dt_dev,
+                              double dt_host, double gamma)
+{
+  const long nc = long(nx) * ny;
+  const long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= nc) return;
+  const int j = int(id / nx), i = int(id - long(j) * nx);
+  if (i < ng || i >= nx - ng || j < ng || j >= ny - ng) return;
+  const double dt = dt_dev ? *dt_dev : dt_host, dtodx = dt / dx, dtody = dt / dy;
+  const double d = U[id], d_inv = 1.0 / d;
+  const double vx = U[nc + id] * d_inv, vy = U[2 * nc + id] * d_inv, vz = U[3 * nc + id] * d_inv;
+  const double E = U[4 * nc + id], GE = U[5 * nc + id];
+  const double pr = Fmax(PressureFromDE(E, E - 0.5 * d * (vx * vx + vy * vy + vz * vz), GE, gamma), kTinyNumber);
+  const long imo = id - 1, ipo = id + 1, jmo = id - nx, jpo = id + nx;
+  const double vx_imo = U[nc + imo] / U[imo], vx_ipo = U[nc + ipo] / U[ipo];
+  const double vy_jmo = U[2 * nc + jmo] / U[jmo], vy_jpo = U[2 * nc + jpo] / U[jpo];
+  U[5 * nc + id] += 0.5 * pr * (dtodx * (vx_imo - vx_ipo) + dtody * (vy_jmo - vy_jpo));
+}
+
+// Select_Internal_Energy_2D (hydro_cuda.cu:887-935): GasEnergy only; neighbours clamped.
+__global__ void Vl2dSelectKernel(double *U, int nx, int ny, int ng)
+{
+  const long nc = long(nx) * ny;
+  const long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= nc) return;
+  const int j = int(id / nx), i = int(id - long(j) * nx);
+  if (i < ng || i >= nx - ng || j < ng || j >= ny - ng) return;
+  const long imo = max(i - 1, ng) + long(j) * nx, ipo = min(i + 1, nx - ng - 1) + long(j) * nx;
+  const long jmo = i + long(max(j - 1, ng)) * nx, jpo = i + long(min(j + 1, ny - ng - 1)) * nx;
+  const double d = U[id], d_inv = 1.0 / d;
+  const double vx = U[nc + id] * d_inv, vy = U[2 * nc + id] * d_inv, vz = U[3 * nc + id] * d_inv;
+  const double E = U[4 * nc + id], U_adv = U[5 * nc + id];
+  const double U_tot = E - 0.5 * d * (vx * vx + vy * vy + vz * vz);
+  double Emax        = Fmax(U[4 * nc + imo], E);
+  Emax               = Fmax(Emax, U[4 * nc + ipo]);
+  Emax               = Fmax(Emax, U[4 * nc + jmo]);
+  Emax               = Fmax(Emax, U[4 * nc + jpo]);
+  const double u     = ((U_tot / E > kDeEta1) || (U_tot / Emax > kDeEta2)) ? U_tot : U_adv;
+  U[5 * nc + id]     = Fmax(u, kTinyNumber);
+}
+
+// Sync_Energies_2D (hydro_cuda.cu:1042-1068), then Calc_dt_2D of the synchronised cell.
+__global__ void Vl2dSyncKernel(double *U, int nx, int ny, int ng, double dx, double dy, double gamma,
+                               unsigned long long *dti)
+{
+  const long nc = long(nx) * ny;
+  const long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  double m      = 0.0;
+  if (id < nc) {
+    const int j = int(id / nx), i = int(id - long(j) * nx);
+    if (i >= ng && i < nx - ng && j >= ng && j < ny - ng) {
+      const double d = U[id], d_inv = 1.0 / d;
+      const double vx = U[nc + id] * d_inv, vy = U[2 * nc + id] * d_inv, vz = U[3 * nc + id] * d_inv;
+      const double E  = 0.5 * d * (vx * vx + vy * vy + vz * vz) + U[5 * nc + id];
+      U[4 * nc + id]  = E;
+      double pr       = Fmax((E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0), 1.0e-20);
+      const double cs = __builtin_sqrt(d_inv * gamma * pr);
+      m               = Fmax((Fabs(vx) + cs) / dx, (Fabs(vy) + cs) / dy);
+    }
+  }
+  if (dti) BlockMaxToGlobal(m, dti);
 }
 
 // Update_Conserved_Variables_2D (hydro_cuda.cu:84-160, no gravity, no DE) on the real
 // cells, the x and y terms summed before they are added, then Calc_dt_2D
 // (hydro_cuda.cu:474-510) of the updated cell into *dti.
+template <bool DE = false>
 __global__ void Vl2dUpdateKernel(double *U, const double *Fx, const double *Fy, int nx, int ny, int ng, double dx,
                                  double dy, const double *dt_dev, double dt_host, double gamma,
                                  unsigned long long *dti, GravSpec grav)
@@ -627,6 +708,10 @@ __global__ void Vl2dUpdateKernel(double *U, const double *Fx, const double *Fy, 
       }
 #pragma unroll
       for (int f = 0; f < 5; f++) U[f * nc + id] = un[f];
+      if constexpr (DE) {  // GasEnergy: its fluxes on top of Vl2dPdvKernel's source; dt after the sync
+        const long o = 5 * nc;
+        U[o + id] += dtodx * (Fx[o + imo] - Fx[o + id]) + dtody * (Fy[o + jmo] - Fy[o + id]);
+      }
       const double d = un[0], dinv = 1.0 / d;
       const double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
       double pr       = (un[4] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
@@ -635,7 +720,7 @@ __global__ void Vl2dUpdateKernel(double *U, const double *Fx, const double *Fy, 
       m               = Fmax((Fabs(vx) + cs) / dx, (Fabs(vy) + cs) / dy);
     }
   }
-  if (dti) BlockMaxToGlobal(m, dti);
+  if (!DE && dti) BlockMaxToGlobal(m, dti);
 }
 
 // Calc_dt_2D, hydro_cuda.cu:474-510
@@ -680,7 +765,7 @@ __device__ __forceinline__ void DirSlots(int dir, int &o1, int &o2, int &o3)
 // PLMC_cuda / PPMC_VL / PLMP_cuda over a whole grid with the reference's thread guard and
 // interface layout (plmc_cuda.cu:24-300, ppmc_cuda.cu:540-695): Q_L at cell i,
 // Q_R at cell i-1 along dir, conserved variables in global momentum slots.
-template <int RK>
+template <int RK, bool DE = false>
 __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
 {
   constexpr int r  = ReconTraits<RK>::kRadius;
@@ -699,6 +784,7 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
     if (ThreadGuard(r + 1, nx, ny, nz, xid, yid, zid)) return;
   }
   Prim<double> s[NS];
+  double ge[NS];
 #pragma unroll
   for (int m = 0; m < NS; m++) {
     size_t id = Lin(xid + (m - r) * ix, yid + (m - r) * iy, zid + (m - r) * iz, nx, ny);
@@ -706,17 +792,25 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
       const double d = U[id], vx = U[o1 * nc + id] / d, vy = U[o2 * nc + id] / d, vz = U[o3 * nc + id] / d;
       s[m] = {d, vx, vy, vz, Fmax((U[4 * nc + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0),
                                   kTinyNumber)};
+    } else if constexpr (DE) {  // Load_Data's DE branch, reconstruction.h:179-191
+      const double d = U[id], vx = U[o1 * nc + id] / d, vy = U[o2 * nc + id] / d, vz = U[o3 * nc + id] / d;
+      const double E = U[4 * nc + id], g = U[5 * nc + id];
+      s[m]  = {d, vx, vy, vz, PressureFromDE(E, E - 0.5 * d * SquareMag(vx, vy, vz), g, gamma)};
+      ge[m] = g / d;
     } else {
       s[m] = LoadPrim(U[id], U[o1 * nc + id], U[o2 * nc + id], U[o3 * nc + id], U[4 * nc + id], gamma);
     }
   }
   Prim<double> L, R;
+  double gL = 0.0, gR = 0.0;
   if constexpr (RK == kPPMC) {
     ReconPpmc<PolicyExact>(s, gamma, L, R);
+    if constexpr (DE) ReconPpmcScalar<PolicyExact>(ge, gL, gR);
   } else if constexpr (RK == kPLMP) {
     ReconPlmp<PolicyExact>(s, L, R);
   } else {
     ReconPlmc<PolicyExact>(s, gamma, L, R);
+    if constexpr (DE) ReconPlmcScalar<PolicyExact>(ge, gL, gR);
   }
   const double inv_gm1 = 1.0 / (gamma - 1.0);
   auto write = [&](const Prim<double> &q, double *Q, size_t id) {  // Write_Data, reconstruction.h:918-958
@@ -732,30 +826,36 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
   };
   write(L, QL, Lin(xid, yid, zid, nx, ny));
   write(R, QR, Lin(xid - ix, yid - iy, zid - iz, nx, ny));
+  if constexpr (DE) {  // Write_Data, reconstruction.h:960-962
+    QL[5 * nc + Lin(xid, yid, zid, nx, ny)]                = L.d * gL;
+    QR[5 * nc + Lin(xid - ix, yid - iy, zid - iz, nx, ny)] = R.d * gR;
+  }
 }
 
 // Calculate_HLLC_Fluxes_CUDA over a whole grid (hllc_cuda.cu:19-307), or, under
 // HllPolicy / RoePolicy / ExactPolicy, Calculate_HLL_Fluxes_CUDA (hll_cuda.cu:19-268) /
 // Calculate_Roe_Fluxes_CUDA (roe_cuda.cu:19-400) / Calculate_Exact_Fluxes_CUDA (exact_cuda.cu:20-334)
-template <class P>
+template <class P, bool DE = false>
 __global__ void RiemannGridKernel(const double *QL, const double *QR, double *F, int nx, int ny, int nz, double gamma,
-                               int dir)
+                                  int dir)
 {
   const size_t nc = size_t(nx) * ny * nz;
   size_t t        = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nc) return;
   int o1, o2, o3;
   DirSlots(dir, o1, o2, o3);
-  RState<double> l = RStateFromCons<PolicyExact>(QL[t], QL[o1 * nc + t], QL[o2 * nc + t], QL[o3 * nc + t],
-                                                 QL[4 * nc + t], gamma);
-  RState<double> r = RStateFromCons<PolicyExact>(QR[t], QR[o1 * nc + t], QR[o2 * nc + t], QR[o3 * nc + t],
-                                                 QR[4 * nc + t], gamma);
-  Cons<double> f   = RiemannFlux<P>(l, r, gamma);
+  RState<double> l = RStateFromCons<PolicyExact, DE>(QL[t], QL[o1 * nc + t], QL[o2 * nc + t], QL[o3 * nc + t],
+                                                     QL[4 * nc + t], gamma, DE ? QL[5 * nc + t] : 0.0);
+  RState<double> r = RStateFromCons<PolicyExact, DE>(QR[t], QR[o1 * nc + t], QR[o2 * nc + t], QR[o3 * nc + t],
+                                                     QR[4 * nc + t], gamma, DE ? QR[5 * nc + t] : 0.0);
+  double fge       = 0.0;
+  Cons<double> f   = RiemannFlux<P, DE>(l, r, gamma, &fge);
   F[t]             = f.d;
   F[o1 * nc + t]   = f.mx;
   F[o2 * nc + t]   = f.my;
   F[o3 * nc + t]   = f.mz;
   F[4 * nc + t]    = f.E;
+  if constexpr (DE) F[5 * nc + t] = fge;
 }
 
 // ============================================================================ launchers
@@ -925,46 +1025,66 @@ hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int rieman
   return hipErrorInvalidValue;
 }
 
-template <class P>
+template <class P, bool DE>
 static hipError_t LaunchVl2dP(double *U, double *scratch, int nx, int ny, int ng, int recon, double dx, double dy,
                               const double *dt_dev, double dt, double gamma, unsigned long long *dti,
                               const GravSpec &grav, hipStream_t s)
 {
-  const long nc = long(nx) * ny, fs = 5 * nc;
+  const long nc = long(nx) * ny, fs = 6 * nc;  // 6 fields of room per array (GasEnergy last with DE)
   double *Uh = scratch, *QLx = scratch + fs, *QRx = scratch + 2 * fs, *QLy = scratch + 3 * fs;
   double *QRy = scratch + 4 * fs, *Fx = scratch + 5 * fs, *Fy = scratch + 6 * fs;
-  const unsigned nb = Blocks(nc, 256);
-  hipLaunchKernelGGL(Vl2dPcmFluxKernel<P>, dim3(nb), dim3(256), 0, s, U, Fx, Fy, nx, ny, gamma, dti);
-  hipLaunchKernelGGL(Vl2dHalfKernel, dim3(nb), dim3(256), 0, s, U, Uh, Fx, Fy, nx, ny, dx, dy, dt_dev, dt);
+  const unsigned nb = Blocks(nc, 256), nb64 = Blocks(nc, 64);
+  hipLaunchKernelGGL((Vl2dPcmFluxKernel<P, DE>), dim3(nb), dim3(256), 0, s, U, Fx, Fy, nx, ny, gamma, dti);
+  hipLaunchKernelGGL((Vl2dHalfKernel<DE>), dim3(nb), dim3(256), 0, s, U, Uh, Fx, Fy, nx, ny, dx, dy, dt_dev, dt,
+                     gamma);
   if (recon == kPCM) {
-    hipLaunchKernelGGL(Vl2dPcmFluxKernel<P>, dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, gamma, nullptr);
+    hipLaunchKernelGGL((Vl2dPcmFluxKernel<P, DE>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, gamma, nullptr);
   } else {
-    hipError_t e = LaunchReconGrid(recon, Uh, QLx, QRx, nx, ny, 1, gamma, 0, s);
-    if (e == hipSuccess) e = LaunchReconGrid(recon, Uh, QLy, QRy, nx, ny, 1, gamma, 1, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(RiemannGridKernel<P>, dim3(Blocks(nc, 64)), dim3(64), 0, s, QLx, QRx, Fx, nx, ny, 1, gamma, 0);
-    hipLaunchKernelGGL(RiemannGridKernel<P>, dim3(Blocks(nc, 64)), dim3(64), 0, s, QLy, QRy, Fy, nx, ny, 1, gamma, 1);
+    if constexpr (DE) {  // PLMC or PPMC (the options refuse PLMP with DE)
+      if (recon == kPPMC) {
+        hipLaunchKernelGGL((ReconGridKernel<kPPMC, true>), dim3(nb64), dim3(64), 0, s, Uh, QLx, QRx, nx, ny, 1, gamma, 0);
+        hipLaunchKernelGGL((ReconGridKernel<kPPMC, true>), dim3(nb64), dim3(64), 0, s, Uh, QLy, QRy, nx, ny, 1, gamma, 1);
+      } else {
+        hipLaunchKernelGGL((ReconGridKernel<kPLMC, true>), dim3(nb64), dim3(64), 0, s, Uh, QLx, QRx, nx, ny, 1, gamma, 0);
+        hipLaunchKernelGGL((ReconGridKernel<kPLMC, true>), dim3(nb64), dim3(64), 0, s, Uh, QLy, QRy, nx, ny, 1, gamma, 1);
+      }
+    } else {
+      hipError_t e = LaunchReconGrid(recon, Uh, QLx, QRx, nx, ny, 1, gamma, 0, s);
+      if (e == hipSuccess) e = LaunchReconGrid(recon, Uh, QLy, QRy, nx, ny, 1, gamma, 1, s);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((RiemannGridKernel<P, DE>), dim3(nb64), dim3(64), 0, s, QLx, QRx, Fx, nx, ny, 1, gamma, 0);
+    hipLaunchKernelGGL((RiemannGridKernel<P, DE>), dim3(nb64), dim3(64), 0, s, QLy, QRy, Fy, nx, ny, 1, gamma, 1);
   }
-  hipLaunchKernelGGL(Vl2dUpdateKernel, dim3(nb), dim3(256), 0, s, U, Fx, Fy, nx, ny, ng, dx, dy, dt_dev, dt, gamma,
-                     dti, grav);
+  if constexpr (DE) hipLaunchKernelGGL(Vl2dPdvKernel, dim3(nb), dim3(256), 0, s, U, nx, ny, ng, dx, dy, dt_dev, dt, gamma);
+  hipLaunchKernelGGL((Vl2dUpdateKernel<DE>), dim3(nb), dim3(256), 0, s, U, Fx, Fy, nx, ny, ng, dx, dy, dt_dev, dt,
+                     gamma, dti, grav);
+  if constexpr (DE) {
+    hipLaunchKernelGGL(Vl2dSelectKernel, dim3(nb), dim3(256), 0, s, U, nx, ny, ng);
+    hipLaunchKernelGGL(Vl2dSyncKernel, dim3(nb), dim3(256), 0, s, U, nx, ny, ng, dx, dy, gamma, dti);
+  }
   return hipGetLastError();
 }
 
-// VL_Algorithm_2D_CUDA (VL_2D_cuda.cu:24-168) in the exact form; scratch holds 7 x 5 x nx*ny
-// doubles (half state, four interface arrays, two flux arrays).
-hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, double dx, double dy,
-                      const double *dt_dev, double dt, double gamma, unsigned long long *dti, const GravSpec &grav,
-                      hipStream_t s)
+// VL_Algorithm_2D_CUDA (VL_2D_cuda.cu:24-168) in the exact form; scratch holds 7 x 6 x nx*ny
+// doubles (half state, four interface arrays, two flux arrays, 6 fields of room each).
+hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, bool de, double dx,
+                      double dy, const double *dt_dev, double dt, double gamma, unsigned long long *dti,
+                      const GravSpec &grav, hipStream_t s)
 {
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPCM) return hipErrorInvalidValue;
+  if (de) {  // HLLC with PCM, PLMC or PPMC
+    if (riemann != kRiemannHllc || recon == kPLMP) return hipErrorInvalidValue;
+    return LaunchVl2dP<PolicyExact, true>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+  }
   if (riemann == kRiemannHll)
-    return LaunchVl2dP<HllPolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+    return LaunchVl2dP<HllPolicy<PolicyExact>, false>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   if (riemann == kRiemannRoe)
-    return LaunchVl2dP<RoePolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+    return LaunchVl2dP<RoePolicy<PolicyExact>, false>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   if (riemann == kRiemannExact)
-    return LaunchVl2dP<ExactPolicy<PolicyExact>>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+    return LaunchVl2dP<ExactPolicy<PolicyExact>, false>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   if (riemann == kRiemannHllc)
-    return LaunchVl2dP<PolicyExact>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+    return LaunchVl2dP<PolicyExact, false>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   return hipErrorInvalidValue;
 }
 

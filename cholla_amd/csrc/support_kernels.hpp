@@ -32,9 +32,9 @@ hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const doubl
 hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, bool de, double dx,
                       const double *dt_dev, double dt, double gamma, unsigned long long *dti, const GravSpec &grav,
                       hipStream_t s);
-hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, double dx, double dy,
-                      const double *dt_dev, double dt, double gamma, unsigned long long *dti, const GravSpec &grav,
-                      hipStream_t s);
+hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, bool de, double dx,
+                      double dy, const double *dt_dev, double dt, double gamma, unsigned long long *dti,
+                      const GravSpec &grav, hipStream_t s);
 hipError_t LaunchReconGrid(int recon, const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma,
                            int dir, hipStream_t s);
 hipError_t LaunchRiemannGrid(int riemann, const double *QL, const double *QR, double *F, int nx, int ny, int nz,

@@ -65,7 +65,7 @@ typedef struct chl_hydro_config {
   int tile1, tile2;      /* tile shapes of the two fused stages, -1 = default */
   int kchunk1, kchunk2;  /* z cells per workgroup in the two stages, 0 = default */
   int tile_order;        /* 0: each XCD takes a contiguous run of tiles (default); 1: dispatch order */
-  int dual_energy;       /* DE build option, 0/1: GasEnergy as field 5 (n_fields 6); 3D and 1D, HLLC with PLMC or PPMC */
+  int dual_energy;       /* DE build option, 0/1: GasEnergy as field 5 (n_fields 6); HLLC with PLMC or PPMC */
   int positivity_fallback; /* 0/1, PPMC in 3D only; no counterpart in the reference: where a
                               cell's PPMC interface density or pressure comes out non-positive,
                               both of its interfaces in that sweep take the cell-centred values

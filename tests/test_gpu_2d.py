@@ -73,4 +73,40 @@ def test_2d_limits():
     with pytest.raises(chl.ChollaError, match="2D boxes"):
         chl.Hydro(40, 40, 1, 4, chl.RECON_PPMC, chl.PREC_MIXED)
     with pytest.raises(chl.ChollaError, match="dual energy"):
-        chl.Hydro(40, 40, 1, 4, chl.RECON_PPMC, chl.PREC_FP64_EXACT, dual_energy=True)
+        chl.Hydro(40, 40, 1, 3, chl.RECON_PLMP, chl.PREC_FP64_EXACT, dual_energy=True)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC])
+@pytest.mark.parametrize("case", ["warm", "cold_x", "cold_y"])
+def test_dual_energy_2d_bitwise(case, recon, use_graph):
+    """2D DE (VL_2D_cuda.cu:140-152, 203-252): a warm flow and cold compressions along each axis."""
+    from test_oracle_de_2d import warm_2d
+    from test_oracle_de_1d import state_1d
+    ng = NG[recon]
+    if case == "warm":
+        U, N = warm_2d(16, ng)
+        nx = ny = N
+    else:
+        n = 32
+        x = (np.arange(n) + 0.5) / n
+        U1, nx = state_1d(np.ones(n), 3.0 + 0.5 * np.sin(2 * np.pi * x), 0.2 * np.cos(2 * np.pi * x), 0 * x,
+                          1e-4 * np.ones(n), ng)
+        ny = 3 * ng
+        U = np.repeat(U1.reshape(6, 1, nx), ny, axis=1)
+        if case == "cold_y":
+            U = U.transpose(0, 2, 1)[[0, 2, 1, 3, 4, 5]]
+            nx, ny = ny, nx
+        U = U.reshape(-1).copy()
+        orc.set_boundaries(U, nx, ny, 1, 6, ng, [1, 1, 1, 1, 0, 0])
+    h = 1.0 / 32
+    Uo = U.copy()
+    s_o, t_o = orc.run(Uo, nx, ny, 1, ng, 6, h, h, h, G, recon, [1, 1, 1, 1, 0, 0], 0.0, 1e30, 10,
+                       dual_energy_on=True)
+    say("2D DE", case, "recon", recon, "graph", use_graph)
+    Ud = chl.DeviceArray.from_host(U)
+    hd = chl.Hydro(nx, ny, 1, ng, RECON[recon], chl.PREC_FP64_EXACT, bc=[1] * 6, dual_energy=True)
+    s, t = hd.run(Ud, h, h, h, G, 0.0, 1e30, 10, use_graph=use_graph)
+    hd.close()
+    assert s == s_o == 10 and t == t_o
+    np.testing.assert_array_equal(Ud.download(), Uo)
