@@ -468,15 +468,20 @@ void orc_plmc(const double *U, double *QL, double *QR, int nx, int ny, int nz, d
 
 // Interface_Values_PLM, src/reconstruction/plmp_cuda.cu:325-365: Van Leer limited slope of
 // one primitive variable; *left and *right are the values at the cell's left and right faces.
-static void PlmpInterface(double qm, double q0, double qp, double *left, double *right)
+// (the slope itself is Calculate_Slope of ppmp_cuda.cu:687-720, the same steps)
+static double LimitedSlope(double qm, double q0, double qp)
 {
   double dL = q0 - qm, dR = qp - q0, dC = 0.5 * (qp - qm);
   double dG = dL * dR > 0.0 ? 2.0 * dL * dR / (dL + dR) : 0.0;
   double a = std::fmin(std::fabs(dL), std::fabs(dR));
   double b = std::fmin(std::fabs(dC), std::fabs(dG));
-  double dm = (dC < 0 ? -1.0 : 1.0) * std::fmin(2.0 * a, b);  // sgn_CUDA, global_cuda.h:41-48
-  *left     = q0 - 0.5 * dm;
-  *right    = q0 + 0.5 * dm;
+  return (dC < 0 ? -1.0 : 1.0) * std::fmin(2.0 * a, b);  // sgn_CUDA, global_cuda.h:41-48
+}
+static void PlmpInterface(double qm, double q0, double qp, double *left, double *right)
+{
+  const double dm = LimitedSlope(qm, q0, qp);
+  *left           = q0 - 0.5 * dm;
+  *right          = q0 + 0.5 * dm;
 }
 
 // PLMP_cuda in the VL build (no half-step evolution, plmp_cuda.cu:231-284 compiled out),
@@ -517,6 +522,67 @@ void orc_plmp(const double *U, double *QL, double *QR, int nx, int ny, int nz, d
         };
         write(lf, QR, Idx(i - ix, j - iy, k - iz, nx, ny));  // right state of face i-1/2
         write(rf, QL, Idx(i, j, k, nx, ny));                 // left state of face i+1/2
+      }
+}
+
+// Interface_Values_PPM, src/reconstruction/ppmp_cuda.cu:722-755: parabolic interfaces from
+// the limited slopes of cells i-1, i, i+1, then the monotonicity constraints in order
+static void PpmpInterfaces(double qm, double q0, double qp, double sm, double s0, double sp, double *left,
+                           double *right)
+{
+  double L = 0.5 * (q0 + qm) - (1.0 / 6.0) * (s0 - sm);
+  double R = 0.5 * (qp + q0) - (1.0 / 6.0) * (sp - s0);
+  if ((R - q0) * (q0 - L) <= 0) L = R = q0;
+  if (6.0 * (R - L) * (q0 - 0.5 * (L + R)) > (R - L) * (R - L)) L = 3.0 * q0 - 2.0 * R;
+  if (6.0 * (R - L) * (q0 - 0.5 * (L + R)) < -(R - L) * (R - L)) R = 3.0 * q0 - 2.0 * L;
+  L = std::fmax(std::fmin(q0, qm), L);
+  L = std::fmin(std::fmax(q0, qm), L);
+  R = std::fmax(std::fmin(q0, qp), R);
+  R = std::fmin(std::fmax(q0, qp), R);
+  *left = L, *right = R;
+}
+
+// PPMP_cuda in the VL build (ppmp_cuda.cu:28-683; STEEPENING and FLATTENING are not defined
+// there and the CTU tracing at :474-640 is compiled out), hydro without DE or scalars:
+// primitives as PLMP forms them, Calculate_Slope (:687-720, the PLMP slope) on cells i-1, i,
+// i+1, parabolic interfaces, floors, conserved variables.
+void orc_ppmp(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
+{
+  size_t n_cells = size_t(nx) * ny * nz;
+  int o1, o2, o3;
+  DirOrder(dir, o1, o2, o3);
+  int ix = dir == 0, iy = dir == 1, iz = dir == 2;
+  // ppmp_cuda.cu:125-149: two cells off the low end and three off the high end along dir
+  int xs = 2 * ix, xe = ix ? nx - 3 : nx, ys = 2 * iy, ye = iy ? ny - 3 : ny, zs = 2 * iz, ze = iz ? nz - 3 : nz;
+  auto load = [&](int i, int j, int k, double q[5]) {
+    size_t id = Idx(i, j, k, nx, ny);
+    double d = U[id], vx = U[o1 * n_cells + id] / d, vy = U[o2 * n_cells + id] / d, vz = U[o3 * n_cells + id] / d;
+    double p = (U[4 * n_cells + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+    q[0] = d, q[1] = vx, q[2] = vy, q[3] = vz, q[4] = std::fmax(p, kTiny);
+  };
+  auto slope = LimitedSlope;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = zs; k < ze; k++)
+    for (int j = ys; j < ye; j++)
+      for (int i = xs; i < xe; i++) {
+        double c[5][5], lf[5], rf[5];
+        for (int m = 0; m < 5; m++) load(i + (m - 2) * ix, j + (m - 2) * iy, k + (m - 2) * iz, c[m]);
+        for (int v = 0; v < 5; v++) {
+          double sm = slope(c[0][v], c[1][v], c[2][v]), s0 = slope(c[1][v], c[2][v], c[3][v]);
+          double sp = slope(c[2][v], c[3][v], c[4][v]);
+          PpmpInterfaces(c[1][v], c[2][v], c[3][v], sm, s0, sp, &lf[v], &rf[v]);
+        }
+        lf[0] = std::fmax(lf[0], kTiny), rf[0] = std::fmax(rf[0], kTiny);
+        lf[4] = std::fmax(lf[4], kTiny), rf[4] = std::fmax(rf[4], kTiny);
+        auto write = [&](const double q[5], double *Q, size_t id) {
+          Q[id]                = q[0];
+          Q[o1 * n_cells + id] = q[0] * q[1];
+          Q[o2 * n_cells + id] = q[0] * q[2];
+          Q[o3 * n_cells + id] = q[0] * q[3];
+          Q[4 * n_cells + id]  = q[4] / (gamma - 1.0) + 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        };
+        write(lf, QR, Idx(i - ix, j - iy, k - iz, nx, ny));
+        write(rf, QL, Idx(i, j, k, nx, ny));
       }
 }
 
@@ -1596,7 +1662,7 @@ void orc_set_boundaries(double *c, int nx, int ny, int nz, int n_fields, int n_g
 // Integrators
 // ---------------------------------------------------------------------------
 
-enum { ORC_PCM = 0, ORC_PLMC = 1, ORC_PPMC = 2, ORC_PLMP = 3 };
+enum { ORC_PCM = 0, ORC_PLMC = 1, ORC_PPMC = 2, ORC_PLMP = 3, ORC_PPMP = 4 };
 
 struct Scratch {
   size_t n = 0;
@@ -1636,6 +1702,10 @@ void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double 
     orc_plmp(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, nz, gamma, 0);
     orc_plmp(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, nz, gamma, 1);
     orc_plmp(s.Uh.data(), s.QLz.data(), s.QRz.data(), nx, ny, nz, gamma, 2);
+  } else if (recon == ORC_PPMP) {  // VL_3D_cuda.cu:244-252
+    orc_ppmp(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, nz, gamma, 0);
+    orc_ppmp(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, nz, gamma, 1);
+    orc_ppmp(s.Uh.data(), s.QLz.data(), s.QRz.data(), nx, ny, nz, gamma, 2);
   } else {
     orc_pcm_3d(s.Uh.data(), s.QLx.data(), s.QRx.data(), s.QLy.data(), s.QRy.data(), s.QLz.data(), s.QRz.data(), nx,
                ny, nz, n_fields);
@@ -1775,6 +1845,9 @@ void orc_vl2d(double *U, int nx, int ny, int n_ghost, double dx, double dy, doub
   } else if (recon == ORC_PLMP) {
     orc_plmp(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, 1, gamma, 0);
     orc_plmp(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, 1, gamma, 1);
+  } else if (recon == ORC_PPMP) {
+    orc_ppmp(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, ny, 1, gamma, 0);
+    orc_ppmp(s.Uh.data(), s.QLy.data(), s.QRy.data(), nx, ny, 1, gamma, 1);
   } else {
     orc_pcm_3d(s.Uh.data(), s.QLx.data(), s.QRx.data(), s.QLy.data(), s.QRy.data(), s.QLz.data(), s.QRz.data(), nx,
                ny, 1, n_fields);
@@ -1802,6 +1875,8 @@ void orc_vl1d(double *U, int nx, int n_ghost, double dx, double dt, double gamma
     orc_ppmc_vl(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, 1, 1, gamma, 0);
   } else if (recon == ORC_PLMP) {  // VL_1D_cuda.cu:98-101
     orc_plmp(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, 1, 1, gamma, 0);
+  } else if (recon == ORC_PPMP) {  // VL_1D_cuda.cu:102-105
+    orc_ppmp(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, 1, 1, gamma, 0);
   } else {
     orc_pcm_1d(s.Uh.data(), s.QLx.data(), s.QRx.data(), nx, n_fields);
   }

@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcholla_oracle.so")
 
-PCM, PLMC, PPMC, PLMP = 0, 1, 2, 3
+PCM, PLMC, PPMC, PLMP, PPMP = 0, 1, 2, 3, 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -50,6 +50,7 @@ def lib():
             "orc_pcm_1d": (None, [_dp, _dp, _dp, i, i]),
             "orc_plmc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_plmp": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_ppmp": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_roe": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_exact": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_set_static_gravity": (None, [i, i, i, i, d, d, d]),
