@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcholla_amd.so")
 
 CHL_OK = 0
-RECON_PCM, RECON_PLMC, RECON_PPMC, RECON_PLMP = 0, 1, 2, 3
+RECON_PCM, RECON_PLMC, RECON_PPMC, RECON_PLMP, RECON_PPMP = 0, 1, 2, 3, 4
 RIEMANN_HLLC = 0
 RIEMANN_HLL = 1
 RIEMANN_ROE = 2

@@ -80,11 +80,12 @@ int CheckConfig(const chl_hydro_config &c)
     return Fail(CHL_ERR_UNSUPPORTED, "riemann must be HLLC, HLL, ROE or EXACT");
   if (c.riemann != CHL_RIEMANN_HLLC && c.dual_energy)
     return Fail(CHL_ERR_UNSUPPORTED, "the HLL, Roe and exact solvers are built without dual energy");
-  if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC && c.reconstruction != CHL_RECON_PLMP &&
-      c.reconstruction != CHL_RECON_PCM)
-    return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PCM, PLMC, PPMC or PLMP");
-  const bool simple_recon = c.reconstruction == CHL_RECON_PCM || c.reconstruction == CHL_RECON_PLMP;
-  if (simple_recon && c.dual_energy) return Fail(CHL_ERR_UNSUPPORTED, "PCM and PLMP are built without dual energy");
+  if (c.reconstruction < CHL_RECON_PCM || c.reconstruction > CHL_RECON_PPMP)
+    return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PCM, PLMC, PPMC, PLMP or PPMP");
+  const bool simple_recon = c.reconstruction == CHL_RECON_PCM || c.reconstruction == CHL_RECON_PLMP ||
+                            c.reconstruction == CHL_RECON_PPMP;
+  if (simple_recon && c.dual_energy)
+    return Fail(CHL_ERR_UNSUPPORTED, "PCM, PLMP and PPMP are built without dual energy");
   if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
   if (c.positivity_fallback != 0 && c.positivity_fallback != 1)
     return Fail(CHL_ERR_INVALID, "positivity_fallback must be 0 or 1");
@@ -108,7 +109,7 @@ int CheckConfig(const chl_hydro_config &c)
     if (c.riemann != CHL_RIEMANN_HLLC && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
       return Fail(CHL_ERR_UNSUPPORTED, "the HLL, Roe and exact solvers are built for tile shapes 0 and 1 only");
     if (simple_recon && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
-      return Fail(CHL_ERR_UNSUPPORTED, "PCM and PLMP are built for tile shapes 0 and 1 only");
+      return Fail(CHL_ERR_UNSUPPORTED, "PCM, PLMP and PPMP are built for tile shapes 0 and 1 only");
     if (c.tile_order != 0 && c.tile_order != 1) return Fail(CHL_ERR_INVALID, "tile_order must be 0 or 1");
   } else {
     if (c.precision != CHL_PREC_FP64_EXACT)
@@ -355,7 +356,7 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   chl_hydro_config c = *cfg;
   if (c.tile1 < 0) c.tile1 = 0;
   const bool narrow = c.dual_energy || c.riemann != CHL_RIEMANN_HLLC || c.reconstruction == CHL_RECON_PCM ||
-                     c.reconstruction == CHL_RECON_PLMP;  // kernels built for tile shapes 0 and 1 only
+                     c.reconstruction == CHL_RECON_PLMP || c.reconstruction == CHL_RECON_PPMP;  // kernels built for tile shapes 0 and 1 only
   if (c.tile2 < 0) c.tile2 = narrow ? 0 : 3;
   CHL_TRY(CheckConfig(c));
   auto *h    = new chl_hydro();
@@ -587,7 +588,7 @@ int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, doub
   const int R1     = ReconRadius(c.reconstruction) + 1;
   const int nz1    = c.nz - 2 * c.n_ghost + 2 * R1, nz2 = c.nz - 2 * c.n_ghost;
   const bool narrow = c.dual_energy || c.riemann != CHL_RIEMANN_HLLC || c.reconstruction == CHL_RECON_PCM ||
-                     c.reconstruction == CHL_RECON_PLMP;
+                     c.reconstruction == CHL_RECON_PLMP || c.reconstruction == CHL_RECON_PPMP;
   const int ntiles  = narrow ? chl::kNumDeTiles : chl::kNumTiles;
   std::vector<int> chunks = {8, 16, 32, 64, 128};
   float best1 = 1e30f, best2 = 1e30f;

@@ -16,7 +16,8 @@ constexpr double kCfl = 0.3;  // C_cfl, src/grid/grid3D.cpp:157
 
 inline int ReconRadius(int recon)
 {
-  return recon == CHL_RECON_PPMC ? 2 : ((recon == CHL_RECON_PLMC || recon == CHL_RECON_PLMP) ? 1 : 0);
+  if (recon == CHL_RECON_PPMC || recon == CHL_RECON_PPMP) return 2;
+  return (recon == CHL_RECON_PLMC || recon == CHL_RECON_PLMP) ? 1 : 0;
 }
 }  // namespace chl_capi
 

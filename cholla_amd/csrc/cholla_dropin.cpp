@@ -20,6 +20,8 @@ namespace
 constexpr int kRecon = CHL_RECON_PPMC;
 #elif defined(PLMP)
 constexpr int kRecon = CHL_RECON_PLMP;
+#elif defined(PPMP)
+constexpr int kRecon = CHL_RECON_PPMP;
 #elif defined(PCM)
 constexpr int kRecon = CHL_RECON_PCM;
 #else

@@ -472,6 +472,57 @@ __device__ __forceinline__ void ReconPlmp(const Prim<typename P::T> s[3], Prim<t
   L_iph.p = Fmax(L_iph.p, T(kTinyNumber));
 }
 
+// PPMP_cuda body in the VL build (ppmp_cuda.cu:300-352, 640-650; STEEPENING and FLATTENING
+// undefined): Calculate_Slope (the PLMP slope, :687-720) on cells i-1, i, i+1 of each
+// primitive, Interface_Values_PPM (:722-755), then the density and pressure floors.
+// s[0..4] = cells i-2 .. i+2.
+template <class P>
+__device__ __forceinline__ typename P::T LimitedSlope(typename P::T qm, typename P::T q0, typename P::T qp)
+{
+  using T    = typename P::T;
+  const T dL = q0 - qm, dR = qp - q0, dC = T(0.5) * (qp - qm);
+  const T dG = dL * dR > T(0.0) ? Div1<P>(T(2.0) * dL * dR, dL + dR) : T(0.0);
+  return SignOf(dC) * Fmin(T(2.0) * Fmin(Fabs(dL), Fabs(dR)), Fmin(Fabs(dC), Fabs(dG)));
+}
+
+template <class P>
+__device__ __forceinline__ void PpmpInterfaces(const typename P::T q[5], typename P::T &lo, typename P::T &hi)
+{
+  using T    = typename P::T;
+  const T sm = LimitedSlope<P>(q[0], q[1], q[2]), s0 = LimitedSlope<P>(q[1], q[2], q[3]);
+  const T sp = LimitedSlope<P>(q[2], q[3], q[4]);
+  const T qm = q[1], qi = q[2], qp = q[3];
+  T L = T(0.5) * (qi + qm) - (T(1.0) / T(6.0)) * (s0 - sm);
+  T R = T(0.5) * (qp + qi) - (T(1.0) / T(6.0)) * (sp - s0);
+  if ((R - qi) * (qi - L) <= T(0)) L = R = qi;
+  if (T(6.0) * (R - L) * (qi - T(0.5) * (L + R)) > (R - L) * (R - L)) L = T(3.0) * qi - T(2.0) * R;
+  if (T(6.0) * (R - L) * (qi - T(0.5) * (L + R)) < -(R - L) * (R - L)) R = T(3.0) * qi - T(2.0) * L;
+  L  = Fmin(Fmax(qi, qm), Fmax(Fmin(qi, qm), L));
+  R  = Fmin(Fmax(qi, qp), Fmax(Fmin(qi, qp), R));
+  lo = L, hi = R;
+}
+
+template <class P>
+__device__ __forceinline__ void ReconPpmp(const Prim<typename P::T> s[5], Prim<typename P::T> &L_iph,
+                                          Prim<typename P::T> &R_imh)
+{
+  using T = typename P::T;
+  T q[5];
+#define CHL_PPMP_VAR(field)                                   \
+  for (int m = 0; m < 5; m++) q[m] = s[m].field;              \
+  PpmpInterfaces<P>(q, R_imh.field, L_iph.field);
+  CHL_PPMP_VAR(d)
+  CHL_PPMP_VAR(vx)
+  CHL_PPMP_VAR(vy)
+  CHL_PPMP_VAR(vz)
+  CHL_PPMP_VAR(p)
+#undef CHL_PPMP_VAR
+  R_imh.d = Fmax(R_imh.d, T(kTinyNumber));
+  L_iph.d = Fmax(L_iph.d, T(kTinyNumber));
+  R_imh.p = Fmax(R_imh.p, T(kTinyNumber));
+  L_iph.p = Fmax(L_iph.p, T(kTinyNumber));
+}
+
 // PLMC_cuda body (VL build: no characteristic tracing), src/reconstruction/plmc_cuda.cu:73-291.
 // s[0..2] = cells i-1, i, i+1.
 template <class P>

@@ -430,7 +430,7 @@ __device__ __forceinline__ void Recon1D(const double *W, int n, int c, double ga
     const double d = W[q], mx = W[n + q], my = W[2 * n + q], mz = W[3 * n + q], E = W[4 * n + q];
     if constexpr (RK == kPCM) {
       s[m] = {d, mx, my, mz, E};
-    } else if constexpr (RK == kPLMP) {
+    } else if constexpr (RK == kPLMP || RK == kPPMP) {
       const double vx = mx / d, vy = my / d, vz = mz / d;
       s[m] = {d, vx, vy, vz, Fmax((E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0), kTinyNumber)};
     } else if constexpr (DE) {
@@ -445,6 +445,8 @@ __device__ __forceinline__ void Recon1D(const double *W, int n, int c, double ga
     L = R = s[0];
   } else if constexpr (RK == kPLMP) {
     ReconPlmp<PolicyExact>(s, L, R);
+  } else if constexpr (RK == kPPMP) {
+    ReconPpmp<PolicyExact>(s, L, R);
   } else if constexpr (RK == kPPMC) {
     ReconPpmc<PolicyExact>(s, gamma, L, R);
     if constexpr (DE) ReconPpmcScalar<PolicyExact>(ge, geL, geR);
@@ -777,9 +779,9 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
   int o1, o2, o3;
   DirSlots(dir, o1, o2, o3);
   int ix = dir == 0, iy = dir == 1, iz = dir == 2;
-  if constexpr (RK == kPLMP) {  // plmp_cuda.cu:80-103: one cell off each end along dir only
+  if constexpr (RK == kPLMP || RK == kPPMP) {  // plmp_cuda.cu:80-103, ppmp_cuda.cu:125-149: along dir only
     const int a = ix ? xid : (iy ? yid : zid), na = ix ? nx : (iy ? ny : nz);
-    if (xid >= nx || yid >= ny || zid >= nz || a < 1 || a >= na - 2) return;
+    if (xid >= nx || yid >= ny || zid >= nz || a < r || a >= na - r - 1) return;
   } else {
     if (ThreadGuard(r + 1, nx, ny, nz, xid, yid, zid)) return;
   }
@@ -788,7 +790,7 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
 #pragma unroll
   for (int m = 0; m < NS; m++) {
     size_t id = Lin(xid + (m - r) * ix, yid + (m - r) * iy, zid + (m - r) * iz, nx, ny);
-    if constexpr (RK == kPLMP) {  // plmp_cuda.cu:104-120
+    if constexpr (RK == kPLMP || RK == kPPMP) {  // plmp_cuda.cu:104-120
       const double d = U[id], vx = U[o1 * nc + id] / d, vy = U[o2 * nc + id] / d, vz = U[o3 * nc + id] / d;
       s[m] = {d, vx, vy, vz, Fmax((U[4 * nc + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0),
                                   kTinyNumber)};
@@ -808,6 +810,8 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
     if constexpr (DE) ReconPpmcScalar<PolicyExact>(ge, gL, gR);
   } else if constexpr (RK == kPLMP) {
     ReconPlmp<PolicyExact>(s, L, R);
+  } else if constexpr (RK == kPPMP) {
+    ReconPpmp<PolicyExact>(s, L, R);
   } else {
     ReconPlmc<PolicyExact>(s, gamma, L, R);
     if constexpr (DE) ReconPlmcScalar<PolicyExact>(ge, gL, gR);
@@ -818,7 +822,7 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
     Q[o1 * nc + id] = q.d * q.vx;
     Q[o2 * nc + id] = q.d * q.vy;
     Q[o3 * nc + id] = q.d * q.vz;
-    if constexpr (RK == kPLMP) {  // plmp_cuda.cu:224-225
+    if constexpr (RK == kPLMP || RK == kPPMP) {  // plmp_cuda.cu:224-225, ppmp_cuda.cu:645-650
       Q[4 * nc + id] = q.p / (gamma - 1.0) + 0.5 * q.d * (q.vx * q.vx + q.vy * q.vy + q.vz * q.vz);
     } else {
       Q[4 * nc + id] = EnergyFromPrim<PolicyExact>(q, gamma, inv_gm1);
@@ -999,9 +1003,14 @@ static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, do
     } else if (recon == kPLMP) {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMP, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
                          gamma, dti, grav);
-    } else {
+    } else if (recon == kPPMP) {
+      hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMP, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
+                         gamma, dti, grav);
+    } else if (recon == kPCM) {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPCM, false>), dim3(nb2), dim3(tpb), 0, s, U, W, n, ng, dx, dt_dev, dt,
                          gamma, dti, grav);
+    } else {
+      return hipErrorInvalidValue;
     }
     return hipGetLastError();
   }
@@ -1012,7 +1021,7 @@ hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int rieman
                       hipStream_t s)
 {
   if (dti && n > 1024 && !de) return hipErrorInvalidValue;
-  if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPCM) return hipErrorInvalidValue;
+  if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPPMP && recon != kPCM) return hipErrorInvalidValue;
   if (de) {  // HLLC with PLMC or PPMC
     if (riemann != kRiemannHllc || (recon != kPPMC && recon != kPLMC)) return hipErrorInvalidValue;
     return LaunchVl1dP<PolicyExact, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
@@ -1072,7 +1081,7 @@ hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int re
                       double dy, const double *dt_dev, double dt, double gamma, unsigned long long *dti,
                       const GravSpec &grav, hipStream_t s)
 {
-  if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPCM) return hipErrorInvalidValue;
+  if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPPMP && recon != kPCM) return hipErrorInvalidValue;
   if (de) {  // HLLC with PCM, PLMC or PPMC
     if (riemann != kRiemannHllc || recon == kPLMP) return hipErrorInvalidValue;
     return LaunchVl2dP<PolicyExact, true>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
@@ -1100,6 +1109,9 @@ hipError_t LaunchReconGrid(int recon, const double *U, double *QL, double *QR, i
                        dir);
   } else if (recon == kPLMP) {
     hipLaunchKernelGGL(ReconGridKernel<kPLMP>, dim3(Blocks(nc, 64)), dim3(64), 0, s, U, QL, QR, nx, ny, nz, gamma,
+                       dir);
+  } else if (recon == kPPMP) {
+    hipLaunchKernelGGL(ReconGridKernel<kPPMP>, dim3(Blocks(nc, 64)), dim3(64), 0, s, U, QL, QR, nx, ny, nz, gamma,
                        dir);
   } else {
     return hipErrorInvalidValue;

@@ -32,7 +32,7 @@
 namespace chl
 {
 
-enum ReconKind : int { kPCM = 0, kPLMC = 1, kPPMC = 2, kPLMP = 3 };
+enum ReconKind : int { kPCM = 0, kPLMC = 1, kPPMC = 2, kPLMP = 3, kPPMP = 4 };
 
 template <int R>
 struct ReconTraits;
@@ -47,6 +47,10 @@ struct ReconTraits<kPPMC> {
 template <>
 struct ReconTraits<kPLMP> {
   static constexpr int kRadius = 1;
+};
+template <>
+struct ReconTraits<kPPMP> {
+  static constexpr int kRadius = 2;
 };
 template <>
 struct ReconTraits<kPCM> {
@@ -478,8 +482,8 @@ __device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, type
     Rot<DIR>(s[m].vx, s[m].vy, s[m].vz, st[m].vx, st[m].vy, st[m].vz);
     if constexpr (RK == kPCM) {
       st[m].p = s[m].p_or_E;  // exact form: (d, momenta, E) carried through as they are
-    } else if constexpr (P::kExact && RK == kPLMP) {
-      // plmp_cuda.cu:113-120: the squared velocities summed left to right
+    } else if constexpr (P::kExact && (RK == kPLMP || RK == kPPMP)) {
+      // plmp_cuda.cu:113-120 (ppmp_cuda.cu:160-171): the squared velocities summed left to right
       const T v2 = (st[m].vx * st[m].vx + st[m].vy * st[m].vy) + st[m].vz * st[m].vz;
       st[m].p    = Fmax((s[m].p_or_E - T(0.5) * st[m].d * v2) * (gamma - T(1.0)), T(kTinyNumber));
     } else if constexpr (P::kExact) {
@@ -499,6 +503,8 @@ __device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, type
     L = R = st[0];
   } else if constexpr (RK == kPLMP) {
     ReconPlmp<P>(st, L, R);
+  } else if constexpr (RK == kPPMP) {
+    ReconPpmp<P>(st, L, R);
   } else if constexpr (RK == kPPMC) {
     bool fb = ReconPpmc<P>(st, gamma, L, R, pos_fallback);
     if constexpr (DE) {
@@ -523,7 +529,7 @@ __device__ __forceinline__ RState<typename P::T> IfaceState(const Prim<typename 
   using T = typename P::T;
   if constexpr (P::kExact && RK == kPCM) {
     return RStateFromCons<P>(q.d, q.vx, q.vy, q.vz, q.p, gamma);  // q holds (d, momenta, E)
-  } else if constexpr (P::kExact && RK == kPLMP) {
+  } else if constexpr (P::kExact && (RK == kPLMP || RK == kPPMP)) {  // ppmp_cuda.cu:645-650 likewise
     const T E = q.p / (gamma - T(1.0)) + T(0.5) * q.d * ((q.vx * q.vx + q.vy * q.vy) + q.vz * q.vz);
     return RStateFromCons<P>(q.d, q.d * q.vx, q.d * q.vy, q.d * q.vz, E, gamma);
   } else {

@@ -109,7 +109,7 @@ hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
       return tile == 0 ? LaunchStage2T<Q, RK, 32, 8>(a, s) : LaunchStage2T<Q, RK, 16, 16>(a, s);
     });
   };
-  if constexpr (RK == kPCM || RK == kPLMP) {
+  if constexpr (RK == kPCM || RK == kPLMP || RK == kPPMP) {
     return narrow();
   } else {
     if (a.riemann != kRiemannHllc) return narrow();
@@ -144,6 +144,7 @@ hipError_t LaunchStage2(const StepArgs &a, int recon, int tile, hipStream_t s)
   if (recon == kPPMC) return LaunchStage2R<P, kPPMC>(a, tile, s);
   if (recon == kPLMC) return LaunchStage2R<P, kPLMC>(a, tile, s);
   if (recon == kPLMP) return LaunchStage2R<P, kPLMP>(a, tile, s);
+  if (recon == kPPMP) return LaunchStage2R<P, kPPMP>(a, tile, s);
   if (recon == kPCM) return LaunchStage2R<P, kPCM>(a, tile, s);
   return hipErrorInvalidValue;
 }

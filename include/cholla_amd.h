@@ -36,6 +36,7 @@ extern "C" {
 #define CHL_RECON_PLMC 1 /* PLMC (builds/make.type.hydro:5) */
 #define CHL_RECON_PPMC 2 /* PPMC */
 #define CHL_RECON_PLMP 3 /* PLMP: limited in the primitive variables (src/reconstruction/plmp_cuda.cu) */
+#define CHL_RECON_PPMP 4 /* PPMP: parabolic in the primitive variables (src/reconstruction/ppmp_cuda.cu) */
 #define CHL_RIEMANN_HLLC 0 /* HLLC (builds/make.type.hydro) */
 #define CHL_RIEMANN_HLL 1  /* HLL, Davis wave speeds (-DHLL, src/riemann_solvers/hll_cuda.cu) */
 #define CHL_RIEMANN_ROE 2  /* Roe with the HLLE fallback (-DROE, src/riemann_solvers/roe_cuda.cu) */
@@ -55,8 +56,8 @@ typedef struct chl_hydro_config {
                             2D: nz = 1 (1D and 2D boxes run the FP64 reference-order form) */
   int n_ghost;           /* Header::n_ghost: PLMC 3, PPMC 4 (src/grid/grid3D.cpp:53-76) */
   int n_fields;          /* 5 (hydro), or 6 with dual_energy (GasEnergy last); no scalars */
-  int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL.  PCM and PLMP: no dual
-                            energy, tile shapes 0 and 1 (n_ghost: PCM 2, PLMP 3) */
+  int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL.  PCM, PLMP and PPMP: no
+                            dual energy, tile shapes 0 and 1 (n_ghost: PCM 2, PLMP 3, PPMP 4) */
   int riemann;           /* CHL_RIEMANN_HLLC, or _HLL / _ROE / _EXACT (no dual energy; tile shapes 0, 1) */
   int precision;         /* CHL_PREC_* */
   int density_floor;     /* DENSITY_FLOOR build option, 0/1 */

@@ -14,7 +14,8 @@ from test_oracle_2d import NG, kh_2d
 
 pytestmark = pytest.mark.gpu
 G = 5.0 / 3.0
-RECON = {orc.PLMC: chl.RECON_PLMC, orc.PPMC: chl.RECON_PPMC, orc.PLMP: chl.RECON_PLMP, orc.PCM: chl.RECON_PCM}
+RECON = {orc.PLMC: chl.RECON_PLMC, orc.PPMC: chl.RECON_PPMC, orc.PLMP: chl.RECON_PLMP, orc.PCM: chl.RECON_PCM,
+         orc.PPMP: chl.RECON_PPMP}
 SOLVER = {orc.HLLC: chl.RIEMANN_HLLC, orc.HLL: chl.RIEMANN_HLL, orc.ROE: chl.RIEMANN_ROE, orc.EXACT: chl.RIEMANN_EXACT}
 
 
@@ -42,7 +43,7 @@ def run_both(U, N, ng, recon, solver, h, hy, steps, flags, use_graph):
 
 @pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("aspect", [1.0, 0.5])
-@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC, orc.PLMP, orc.PCM])
+@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC, orc.PLMP, orc.PCM, orc.PPMP])
 def test_kh_2d_bitwise(recon, aspect, use_graph):
     ng, n = NG[recon], 32
     U, N = kh_2d(n, ng, seed=6)

@@ -1,4 +1,4 @@
-"""GPU parity of the PLMP and PCM reconstruction options (the reference's -DPLMP and -DPCM
+"""GPU parity of the PLMP, PPMP and PCM reconstruction options (the reference's -DPLMP, -DPPMP, -DPCM
 builds: PLMP_cuda, src/reconstruction/plmp_cuda.cu:20-365, and PCM_Reconstruction_3D/1D,
 src/reconstruction/pcm_cuda.cu, in the second VL stage; chl_hydro_config.reconstruction =
 CHL_RECON_PLMP / CHL_RECON_PCM) against the oracle (tests/test_oracle_plmp.py pins PLMP).
@@ -20,7 +20,7 @@ from oracle import oracle as orc
 from test_gpu_parity import TOL, _random_periodic_box, field_l1, scaled_l1
 
 pytestmark = pytest.mark.gpu
-NG = {chl.RECON_PLMP: 3, chl.RECON_PCM: 2}
+NG = {chl.RECON_PLMP: 3, chl.RECON_PCM: 2, chl.RECON_PPMP: 4}
 
 
 def say(*a):
@@ -35,7 +35,7 @@ def _require_gpu():
 
 
 @pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
-@pytest.mark.parametrize("recon", [chl.RECON_PLMP, chl.RECON_PCM])
+@pytest.mark.parametrize("recon", [chl.RECON_PLMP, chl.RECON_PCM, chl.RECON_PPMP])
 def test_single_vl_step(recon, precision):
     n, ng = 20, NG[recon]
     U, N, g = _random_periodic_box(n, ng, seed=13)
@@ -58,7 +58,7 @@ def test_single_vl_step(recon, precision):
 
 @pytest.mark.parametrize("tile", [0, 1])
 @pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
-@pytest.mark.parametrize("recon", [chl.RECON_PLMP, chl.RECON_PCM])
+@pytest.mark.parametrize("recon", [chl.RECON_PLMP, chl.RECON_PCM, chl.RECON_PPMP])
 def test_kelvin_helmholtz(recon, precision, tile):
     n, ng, g = 24, NG[recon], 5 / 3
     N = n + 2 * ng
@@ -97,9 +97,9 @@ def test_plmp_with_hll_exact():
     np.testing.assert_array_equal(Ud.download(), Uo)
 
 
-@pytest.mark.parametrize("recon", [chl.RECON_PLMP, chl.RECON_PCM, chl.RECON_PLMC, chl.RECON_PPMC])
+@pytest.mark.parametrize("recon", [chl.RECON_PLMP, chl.RECON_PCM, chl.RECON_PLMC, chl.RECON_PPMC, chl.RECON_PPMP])
 def test_sod_1d(recon):
-    ng = 4 if recon == chl.RECON_PPMC else 3
+    ng = 4 if recon in (chl.RECON_PPMC, chl.RECON_PPMP) else 3
     U, nx, dx = sod_ic(ng=ng)
     orc.set_boundaries(U, nx, 1, 1, 5, ng, [3, 3, 0, 0, 0, 0])
     Uo = U.copy()

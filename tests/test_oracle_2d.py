@@ -16,7 +16,7 @@ from oracle import oracle as orc
 from helpers import sod_ic
 
 G = 5.0 / 3.0
-NG = {orc.PLMC: 3, orc.PPMC: 4, orc.PLMP: 3, orc.PCM: 2}
+NG = {orc.PLMC: 3, orc.PPMC: 4, orc.PLMP: 3, orc.PCM: 2, orc.PPMP: 4}
 
 
 def kh_2d(n, ng, seed=4):

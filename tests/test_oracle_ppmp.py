@@ -96,3 +96,4 @@ def test_sod_ppmp_converges_and_beats_plmp():
     e100, e200 = sod_l1(100, orc.PPMP), sod_l1(200, orc.PPMP)
     assert e100 < 1e-2 and e200 < 0.7 * e100, (e100, e200)
     assert e100 < sod_l1(100, orc.PLMP)
+
