@@ -37,6 +37,7 @@ EXPORTS = [
     "chl_ensemble_layout", "chl_turbulence_modes", "chl_ic_turbulence", "chl_init_gas_energy", "chl_mhd_create",
     "chl_mhd_destroy",
     "chl_mhd_vl_step", "chl_mhd_run", "chl_mhd_time_steps", "chl_mhd_calc_dt", "chl_mhd_divergence",
+    "chl_mhd_set_ghost_cells", "chl_mhd_set_boundaries",
     "chl_hlld_fluxes", "chl_mhd_reconstruct_plmc", "chl_ct_electric_fields", "chl_update_magnetic_field",
     "chl_magnetic_field_from_vector_potential", "chl_ic_orszag_tang", "chl_pack_real_cells",
     "chl_pack_real_cells_f32", "chl_unpack_real_cells",
@@ -120,6 +121,8 @@ def lib():
         "chl_apply_temperature_floor": (_i, [_vp, _i, _i, _i, _i, _i, _d]),
         "chl_set_ghost_cells": (_i, [_vp, _i, _i, _i, _i, _i, _i, _ip, _i, _i, _i, _i, _i, _i, _i]),
         "chl_set_boundaries": (_i, [_vp, _i, _i, _i, _i, _i, _ip]),
+        "chl_mhd_set_ghost_cells": (_i, [_vp, _i, _i, _i, _i, _i, _i, _ip, _i, _i, _i, _i, _i, _i, _i]),
+        "chl_mhd_set_boundaries": (_i, [_vp, _i, _i, _i, _i, _ip]),
         "chl_reconstruct": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_hllc_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_hll_fluxes": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
@@ -320,6 +323,18 @@ def set_ghost_cells(U, nx, ny, nz, n_ghost, flags, isize, jsize, ksize, imin, jm
     f = (ctypes.c_int * 6)(*flags)
     check(lib().chl_set_ghost_cells(U.ptr, nx, ny, nz, n_fields, nx * ny * nz, n_ghost, f, isize, jsize, ksize, imin,
                                     jmin, kmin, direction))
+
+
+def mhd_set_boundaries(U, nx, ny, nz, n_ghost, flags):
+    """The MHD build's boundary fill of an 8-field state (no outflow diode)."""
+    f = (ctypes.c_int * 6)(*flags)
+    check(lib().chl_mhd_set_boundaries(U.ptr, nx, ny, nz, n_ghost, f))
+
+
+def mhd_set_ghost_cells(U, nx, ny, nz, n_ghost, flags, isize, jsize, ksize, imin, jmin, kmin, direction):
+    f = (ctypes.c_int * 6)(*flags)
+    check(lib().chl_mhd_set_ghost_cells(U.ptr, nx, ny, nz, 8, nx * ny * nz, n_ghost, f, isize, jsize, ksize, imin,
+                                        jmin, kmin, direction))
 
 
 def apply_temperature_floor(U, nx, ny, nz, n_ghost, U_floor):

@@ -42,6 +42,7 @@ struct chl_hydro {
   bool one_d     = false;
   bool two_d     = false;  // nz == 1 < ny: the unfused 2D step, W holds its scratch
   chl::GravSpec grav{};    // static gravity (chl_hydro_set_static_gravity); id 0 = none
+  bool tile2_auto = false; // the caller left tile2 to the library (-1)
   int num_cu     = 256;
   void *W        = nullptr;  // half-step primitives (+ the pdV sources with dual energy)
   size_t W_bytes = 0;
@@ -193,14 +194,13 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
     const int n = c.nx;
     // the dt reduction rides on the last kernel of the step: the one-block stage 2, or with
     // dual energy the energy sync (any number of blocks)
+    // (stage 1 zeroes the accumulator in-kernel, so a captured step holds no memset node)
     bool fuse_dt = update_grid && (n <= 1024 || c.dual_energy);
     CHL_HIP(chl::LaunchVl1d(U, reinterpret_cast<double *>(h->W), n, ng, c.reconstruction, SolverKind(c.riemann),
-                            c.dual_energy != 0, dx, dt_dev, dt_host, gamma, fuse_dt ? h->dti() : nullptr, h->grav,
-                            h->stream));
-    if (update_grid && !fuse_dt) {
-      CHL_HIP(hipMemsetAsync(h->dti(), 0, 8, h->stream));
+                            c.dual_energy != 0, dx, dt_dev, dt_host, gamma, fuse_dt ? h->dti() : nullptr,
+                            update_grid ? h->dti() : nullptr, h->grav, h->stream));
+    if (update_grid && !fuse_dt)
       CHL_HIP(chl::LaunchCalcDt(U, n, 1, 1, ng, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
-    }
     if (fill) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
     return CHL_OK;
   }
@@ -360,6 +360,7 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   if (c.tile2 < 0) c.tile2 = narrow ? 0 : 3;
   CHL_TRY(CheckConfig(c));
   auto *h    = new chl_hydro();
+  h->tile2_auto = cfg->tile2 < 0;
   h->one_d   = c.ny == 1 && c.nz == 1;
   h->two_d   = c.nz == 1 && c.ny > 1;
   h->n_cells = size_t(c.nx) * c.ny * c.nz;
@@ -546,9 +547,19 @@ int chl_hydro_set_static_gravity(chl_hydro *h, int custom_grav, int x_off, int y
   const chl_hydro_config &c = h->cfg;
   const int max_id          = h->two_d ? 4 : 1;  // calc_g_2D has four fields, calc_g_1D / calc_g_3D one
   if (custom_grav < 0 || custom_grav > max_id) return Fail(CHL_ERR_INVALID, "custom_grav out of range for this box");
-  if (custom_grav != 0 && !h->one_d && !h->two_d &&
-      (c.dual_energy || c.riemann != CHL_RIEMANN_HLLC || c.tile2 >= chl::kNumDeTiles))
+  const bool grav3d = custom_grav != 0 && !h->one_d && !h->two_d;
+  if (grav3d && (c.dual_energy || c.riemann != CHL_RIEMANN_HLLC))
     return Fail(CHL_ERR_UNSUPPORTED, "3D static gravity is built for HLLC without DE, stage-2 tile shapes 0 and 1");
+  if (grav3d && c.tile2 >= chl::kNumDeTiles) {
+    if (!h->tile2_auto)
+      return Fail(CHL_ERR_UNSUPPORTED, "3D static gravity is built for HLLC without DE, stage-2 tile shapes 0 and 1");
+    // the library's default stage-2 tile has no gravity kernel: take the 16x16 one
+    h->cfg.tile2 = 1;
+    if (h->graph) {
+      (void)hipGraphExecDestroy(h->graph);
+      h->graph = nullptr;
+    }
+  }
   chl::GravSpec g{custom_grav, {x_off, y_off, z_off}, {xbound, ybound, zbound}};
   const chl::GravSpec &o = h->grav;
   const bool same = o.id == g.id && o.off[0] == g.off[0] && o.off[1] == g.off[1] && o.off[2] == g.off[2] &&
@@ -588,7 +599,7 @@ int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, doub
   const int R1     = ReconRadius(c.reconstruction) + 1;
   const int nz1    = c.nz - 2 * c.n_ghost + 2 * R1, nz2 = c.nz - 2 * c.n_ghost;
   const bool narrow = c.dual_energy || c.riemann != CHL_RIEMANN_HLLC || c.reconstruction == CHL_RECON_PCM ||
-                     c.reconstruction == CHL_RECON_PLMP || c.reconstruction == CHL_RECON_PPMP;
+                     c.reconstruction == CHL_RECON_PLMP || c.reconstruction == CHL_RECON_PPMP || h->grav.id != 0;
   const int ntiles  = narrow ? chl::kNumDeTiles : chl::kNumTiles;
   std::vector<int> chunks = {8, 16, 32, 64, 128};
   float best1 = 1e30f, best2 = 1e30f;
@@ -672,29 +683,67 @@ int chl_apply_temperature_floor(double *U, int nx, int ny, int nz, int n_ghost, 
   return CHL_OK;
 }
 
-int chl_set_ghost_cells(double *c_head, int nx, int ny, int nz, int n_fields, int n_cells, int n_ghost,
-                        const int flags[6], int isize, int jsize, int ksize, int imin, int jmin, int kmin, int dir)
+}  // extern "C"
+
+namespace
+{
+int SetGhostCellsImpl(double *c_head, int nx, int ny, int nz, int n_fields, int n_cells, int n_ghost,
+                      const int flags[6], int isize, int jsize, int ksize, int imin, int jmin, int kmin, int dir,
+                      bool mhd)
 {
   if (!c_head || !flags) return Fail(CHL_ERR_INVALID, "null argument");
   if (long(nx) * ny * nz != long(n_cells)) return Fail(CHL_ERR_INVALID, "n_cells != nx*ny*nz");
+  if (dir < 0 || dir > 5) return Fail(CHL_ERR_INVALID, "dir must be 0..5");
+  if (isize < 0 || jsize < 0 || ksize < 0 || imin < 0 || jmin < 0 || kmin < 0 || imin + isize > nx ||
+      jmin + jsize > ny || kmin + ksize > nz)
+    return Fail(CHL_ERR_INVALID, "slab extents outside the grid");
   if (flags[dir] == 4 || flags[dir] == 5) return CHL_OK;
   CHL_HIP(chl::LaunchGhostSlab(c_head, nx, ny, nz, n_fields, n_ghost, flags, isize, jsize, ksize, imin, jmin, kmin,
-                               dir, nullptr));
+                               dir, nullptr, mhd));
   CHL_HIP(hipDeviceSynchronize());
   return CHL_OK;
 }
 
-int chl_set_boundaries(double *U, int nx, int ny, int nz, int n_fields, int n_ghost, const int flags_in[6])
+int SetBoundariesImpl(double *U, int nx, int ny, int nz, int n_fields, int n_ghost, const int flags_in[6], bool mhd)
 {
   if (!U || !flags_in) return Fail(CHL_ERR_INVALID, "null argument");
   int flags[6] = {0, 0, 0, 0, 0, 0};
   if (nx > 1) flags[0] = flags_in[0], flags[1] = flags_in[1];
   if (ny > 1) flags[2] = flags_in[2], flags[3] = flags_in[3];
   if (nz > 1) flags[4] = flags_in[4], flags[5] = flags_in[5];
-  CHL_HIP(chl::LaunchGhostFill(U, nx, ny, nz, n_fields, n_ghost, flags, nullptr));
+  CHL_HIP(chl::LaunchGhostFill(U, nx, ny, nz, n_fields, n_ghost, flags, nullptr, mhd));
   CHL_HIP(hipDeviceSynchronize());
   return CHL_OK;
 }
+}  // namespace
+
+extern "C" {
+
+int chl_set_ghost_cells(double *c_head, int nx, int ny, int nz, int n_fields, int n_cells, int n_ghost,
+                        const int flags[6], int isize, int jsize, int ksize, int imin, int jmin, int kmin, int dir)
+{
+  return SetGhostCellsImpl(c_head, nx, ny, nz, n_fields, n_cells, n_ghost, flags, isize, jsize, ksize, imin, jmin,
+                           kmin, dir, false);
+}
+
+int chl_mhd_set_ghost_cells(double *c_head, int nx, int ny, int nz, int n_fields, int n_cells, int n_ghost,
+                            const int flags[6], int isize, int jsize, int ksize, int imin, int jmin, int kmin, int dir)
+{
+  return SetGhostCellsImpl(c_head, nx, ny, nz, n_fields, n_cells, n_ghost, flags, isize, jsize, ksize, imin, jmin,
+                           kmin, dir, true);
+}
+
+int chl_set_boundaries(double *U, int nx, int ny, int nz, int n_fields, int n_ghost, const int flags_in[6])
+{
+  return SetBoundariesImpl(U, nx, ny, nz, n_fields, n_ghost, flags_in, false);
+}
+
+int chl_mhd_set_boundaries(double *U, int nx, int ny, int nz, int n_ghost, const int flags_in[6])
+{
+  return SetBoundariesImpl(U, nx, ny, nz, 8, n_ghost, flags_in, true);
+}
+
+}  // extern "C"
 
 namespace
 {
@@ -719,6 +768,8 @@ int SnapshotRegion(int nx, int ny, int nz, int ng, int nxr, int nyr, int nzr, in
   return CHL_OK;
 }
 }  // namespace
+
+extern "C" {
 
 int chl_pack_real_cells(const double *d_field, double *d_buf, int nx, int ny, int nz, int n_ghost, int nx_real,
                         int ny_real, int nz_real, int mhd_direction)

@@ -16,6 +16,11 @@
 namespace
 {
 
+#if defined(MHD) && !defined(PLMC)
+  // The MHD integrator here is built with PLMC only (builds/make.type.mhd); the reference also
+  // admits PCM and PPMC with MHD (src/utils/error_handling.cpp:98-101).
+  #error "cholla_dropin: the MHD build supports -DPLMC reconstruction only"
+#endif
 #if defined(PPMC)
 constexpr int kRecon = CHL_RECON_PPMC;
 #elif defined(PLMP)
@@ -173,7 +178,39 @@ void Apply_Temperature_Floor(Real *dev_conserved, int nx, int ny, int nz, int n_
 void SetGhostCells(Real *c_head, int nx, int ny, int nz, int n_fields, int n_cells, int n_ghost, int flags[],
                    int isize, int jsize, int ksize, int imin, int jmin, int kmin, int dir)
 {
+#ifdef MHD
+  // the MHD build's fill has no outflow diode (src/grid/cuda_boundaries.cu:121-147 is #ifndef MHD)
+  Check(chl_mhd_set_ghost_cells(c_head, nx, ny, nz, n_fields, n_cells, n_ghost, flags, isize, jsize, ksize, imin, jmin,
+                                kmin, dir),
+        "SetGhostCells");
+#else
   Check(chl_set_ghost_cells(c_head, nx, ny, nz, n_fields, n_cells, n_ghost, flags, isize, jsize, ksize, imin, jmin,
                             kmin, dir),
         "SetGhostCells");
+#endif
 }
+
+namespace mhd
+{
+// The body of mhd::checkMagneticDivergence (src/mhd/magnetic_divergence.cu:79-123) on the
+// grid's state: the maximum |div B| over the grid, printed; above 1e-9, or negative, the
+// process exits as chexit(-1) does.
+Real CheckMagneticDivergenceOfState(const Real *dev_conserved, int nx, int ny, int nz, Real dx, Real dy, Real dz)
+{
+  Real div = 0;
+  Check(chl_mhd_divergence(dev_conserved, nx, ny, nz, dx, dy, dz, &div), "checkMagneticDivergence");
+  const Real limit = 1.0E-9;
+  if (div > limit) {
+    std::printf(
+        "The magnetic divergence has exceeded the maximum allowed value. "
+        "Divergence = %7.4e, the maximum allowed divergence = %7.4e\n",
+        div, limit);
+    std::exit(-1);
+  } else if (div < 0.0) {
+    std::printf("The magnetic divergence is negative. Divergence = %7.4e\n", div);
+    std::exit(-1);
+  }
+  std::printf("Global maximum magnetic divergence = %7.4e\n", div);
+  return div;
+}
+}  // namespace mhd

@@ -8,8 +8,11 @@
 //   Execute_Hydro_Integrator  src/grid/grid3D.cpp:413-471
 //   Set_Boundary_Conditions / Set_Boundaries / Set_Boundary_Extents
 //                        src/grid/boundary_conditions.cpp:96-111, 165-413, 419-505
+//   MHD: mhd::checkMagneticDivergence after every step (src/main.cpp:375-378)
+//   STATIC_GRAV: Parameters::custom_grav from the optional third argument (default 0)
 //
 // I/O (little-endian, native):  in-file = header + state, out-file = header + state + run info
+//   usage:  dropin_driver in.bin out.bin [custom_grav]
 //   header: int32 nx, ny, nz, n_ghost, n_fields, max_steps, flags[6];
 //           float64 dx, dy, dz, gamma, t, tout, U_floor, density_floor
 //   state:  float64 x n_fields*nx*ny*nz (field-major, ghosts included)
@@ -34,20 +37,25 @@ struct Header {
   double dx, dy, dz, gamma, t, tout, U_floor, density_floor;
 };
 
+struct Conserved {
+  Real *device = nullptr;  // Grid3D::Conserved::device
+};
+
 struct Grid {
   Header H;
-  Real *dev = nullptr;
-  Real dt   = 0;
+  Conserved C;
+  Real dt         = 0;
+  int custom_grav = 0;
 
   void set_dt(Real dti) { dt = kCfl / dti; }
 
   void Execute_Hydro_Integrator()
   {
     if (H.nx > 1 && H.ny == 1 && H.nz == 1) {
-      VL_Algorithm_1D_CUDA(dev, H.nx, 0, H.n_ghost, H.dx, 0.0, dt, H.n_fields, 0);
+      VL_Algorithm_1D_CUDA(C.device, H.nx, 0, H.n_ghost, H.dx, 0.0, dt, H.n_fields, custom_grav);
     } else {
-      VL_Algorithm_3D_CUDA(dev, nullptr, H.nx, H.ny, H.nz, 0, 0, 0, H.n_ghost, H.dx, H.dy, H.dz, 0.0, 0.0, 0.0, dt,
-                           H.n_fields, 0, H.density_floor, nullptr);
+      VL_Algorithm_3D_CUDA(C.device, nullptr, H.nx, H.ny, H.nz, 0, 0, 0, H.n_ghost, H.dx, H.dy, H.dz, 0.0, 0.0, 0.0,
+                           dt, H.n_fields, custom_grav, H.density_floor, nullptr);
     }
   }
 
@@ -55,10 +63,10 @@ struct Grid {
   {
     Execute_Hydro_Integrator();
 #ifdef TEMPERATURE_FLOOR
-    Apply_Temperature_Floor(dev, H.nx, H.ny, H.nz, H.n_ghost, H.n_fields, H.U_floor);
+    Apply_Temperature_Floor(C.device, H.nx, H.ny, H.nz, H.n_ghost, H.n_fields, H.U_floor);
 #endif
     // grid3D.cpp:119 passes n_cells where n_fields is declared
-    return Calc_dt_GPU(dev, H.nx, H.ny, H.nz, H.n_ghost, H.nx * H.ny * H.nz, H.dx, H.dy, H.dz, gama);
+    return Calc_dt_GPU(C.device, H.nx, H.ny, H.nz, H.n_ghost, H.nx * H.ny * H.nz, H.dx, H.dy, H.dz, gama);
   }
 
   void Set_Boundary_Extents(int dir, int *imin, int *imax)
@@ -84,7 +92,7 @@ struct Grid {
     if (flags[dir] == 4) return;
     int imin[3], imax[3];
     Set_Boundary_Extents(dir, imin, imax);
-    SetGhostCells(dev, H.nx, H.ny, H.nz, H.n_fields, H.nx * H.ny * H.nz, H.n_ghost, flags, imax[0] - imin[0],
+    SetGhostCells(C.device, H.nx, H.ny, H.nz, H.n_fields, H.nx * H.ny * H.nz, H.n_ghost, flags, imax[0] - imin[0],
                   imax[1] - imin[1], imax[2] - imin[2], imin[0], imin[1], imin[2], dir);
   }
 
@@ -109,11 +117,12 @@ void Die(const char *msg)
 
 int main(int argc, char **argv)
 {
-  if (argc != 3) {
-    std::fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+  if (argc != 3 && argc != 4) {
+    std::fprintf(stderr, "usage: %s in.bin out.bin [custom_grav]\n", argv[0]);
     return 2;
   }
   Grid G;
+  if (argc == 4) G.custom_grav = std::atoi(argv[3]);
   FILE *f = std::fopen(argv[1], "rb");
   if (!f || std::fread(&G.H, sizeof(Header), 1, f) != 1) Die("cannot read header");
   size_t n = size_t(G.H.n_fields) * G.H.nx * G.H.ny * G.H.nz;
@@ -121,12 +130,12 @@ int main(int argc, char **argv)
   if (std::fread(host.data(), sizeof(Real), n, f) != n) Die("cannot read state");
   std::fclose(f);
   gama = G.H.gamma;
-  if (chl_malloc(reinterpret_cast<void **>(&G.dev), n * sizeof(Real)) != CHL_OK) Die("chl_malloc");
-  if (chl_memcpy_htod(G.dev, host.data(), n * sizeof(Real)) != CHL_OK) Die("upload");
+  if (chl_malloc(reinterpret_cast<void **>(&G.C.device), n * sizeof(Real)) != CHL_OK) Die("chl_malloc");
+  if (chl_memcpy_htod(G.C.device, host.data(), n * sizeof(Real)) != CHL_OK) Die("upload");
 
   // main.cpp:190 sets the ghost cells before the loop, :245 computes the first dti
   G.Set_Boundary_Conditions_Grid();
-  Real dti   = Calc_dt_GPU(G.dev, G.H.nx, G.H.ny, G.H.nz, G.H.n_ghost, G.H.nx * G.H.ny * G.H.nz, G.H.dx, G.H.dy,
+  Real dti   = Calc_dt_GPU(G.C.device, G.H.nx, G.H.ny, G.H.nz, G.H.n_ghost, G.H.nx * G.H.ny * G.H.nz, G.H.dx, G.H.dy,
                            G.H.dz, gama);
   Real t     = G.H.t;
   int32_t ns = 0;
@@ -137,11 +146,14 @@ int main(int argc, char **argv)
     t += G.dt;  // Update_Time
     ns++;
     G.Set_Boundary_Conditions_Grid();
+#ifdef MHD
+    mhd::checkMagneticDivergence(G);
+#endif
   }
-  if (chl_memcpy_dtoh(host.data(), G.dev, n * sizeof(Real)) != CHL_OK) Die("download");
+  if (chl_memcpy_dtoh(host.data(), G.C.device, n * sizeof(Real)) != CHL_OK) Die("download");
   Free_Memory_VL_3D();
   Free_Memory_VL_1D();
-  chl_free(G.dev);
+  chl_free(G.C.device);
   FILE *o = std::fopen(argv[2], "wb");
   if (!o) Die("cannot write output");
   std::fwrite(&G.H, sizeof(Header), 1, o);

@@ -92,7 +92,7 @@ int EnqueueDeviceStep(chl_mhd *h, double *U, double dx, double dy, double dz, do
   chl::mhd::MhdArgs a = MakeArgs(h, U, dx, dy, dz, gamma, 0.0, h->dt(), density_floor, U_floor, true);
   CHL_HIP(chl::LaunchMhdStep(c.precision, a, h->stream));
   CHL_HIP(chl::LaunchMhdCalcDt(U, c.nx, c.ny, c.nz, c.n_ghost, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
-  CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, 8, c.n_ghost, c.bc_flags, h->stream));
+  CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, 8, c.n_ghost, c.bc_flags, h->stream, true));
   return CHL_OK;
 }
 

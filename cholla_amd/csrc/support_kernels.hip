@@ -56,8 +56,9 @@ __device__ __forceinline__ int FindIndexDev(int ig, int n, int flag, int face, i
 // value written is what SetGhostCellsKernel (cuda_boundaries.cu:71-149) writes for
 // that slab: the mapped real cell (all three indices mapped by SetBoundaryMapping,
 // :161-238), momentum sign flips for reflective faces and the outflow diode of
-// transmissive faces.  Sources are always real cells, so one launch reproduces the
-// six sequential launches exactly.
+// transmissive faces (hydro states only: the MHD build compiles the diode out,
+// cuda_boundaries.cu:121-147, and fl.diode is 0 for it).  Sources are always real cells,
+// so one launch reproduces the six sequential launches exactly.
 __global__ void GhostFillKernel(double *c, int nx, int ny, int nz, int n_fields, int ng, GhostFlags fl, long n_x,
                                 long n_y, long n_z)
 {
@@ -120,7 +121,7 @@ __global__ void GhostFillKernel(double *c, int nx, int ny, int nz, int n_fields,
   if (flags[0] == 2 || flags[1] == 2) c[g + nc] *= a[0];
   if (flags[2] == 2 || flags[3] == 2) c[g + 2 * nc] *= a[1];
   if (flags[4] == 2 || flags[5] == 2) c[g + 3 * nc] *= a[2];
-  if (flags[dir] == 3) {
+  if (fl.diode && flags[dir] == 3) {
     size_t mom = g + size_t(dir / 2 + 1) * nc;
     if (dir % 2 == 0) {
       if (c[mom] > 0.0) {
@@ -173,7 +174,7 @@ __global__ void GhostSlabKernel(double *c, int nx, int ny, int nz, int n_fields,
   if (flags[0] == 2 || flags[1] == 2) c[g + nc] *= a[0];
   if (flags[2] == 2 || flags[3] == 2) c[g + 2 * nc] *= a[1];
   if (flags[4] == 2 || flags[5] == 2) c[g + 3 * nc] *= a[2];
-  if (flags[dir] == 3) {
+  if (fl.diode && flags[dir] == 3) {
     size_t mom = g + size_t(dir / 2 + 1) * nc;
     if (dir % 2 == 0) {
       if (c[mom] > 0.0) {
@@ -866,10 +867,11 @@ __global__ void RiemannGridKernel(const double *QL, const double *QR, double *F,
 static inline unsigned Blocks(long n, int tpb) { return unsigned((n + tpb - 1) / tpb); }
 
 hipError_t LaunchGhostFill(double *U, int nx, int ny, int nz, int n_fields, int ng, const int flags[6],
-                           hipStream_t s)
+                           hipStream_t s, bool mhd)
 {
   GhostFlags fl;
   for (int q = 0; q < 6; q++) fl.f[q] = flags[q];
+  fl.diode = mhd ? 0 : 1;
   if (nx <= 1) return hipSuccess;
   const int jl = ny > 1 ? ng : 0, ju = ny > 1 ? ny - ng : ny;
   const int kl = nz > 1 ? ng : 0, ku = nz > 1 ? nz - ng : nz;
@@ -884,10 +886,11 @@ hipError_t LaunchGhostFill(double *U, int nx, int ny, int nz, int n_fields, int 
 }
 
 hipError_t LaunchGhostSlab(double *U, int nx, int ny, int nz, int n_fields, int ng, const int flags[6], int isize,
-                           int jsize, int ksize, int imin, int jmin, int kmin, int dir, hipStream_t s)
+                           int jsize, int ksize, int imin, int jmin, int kmin, int dir, hipStream_t s, bool mhd)
 {
   GhostFlags fl;
   for (int q = 0; q < 6; q++) fl.f[q] = flags[q];
+  fl.diode = mhd ? 0 : 1;
   long tot = long(isize) * jsize * ksize;
   if (tot <= 0) return hipSuccess;
   hipLaunchKernelGGL(GhostSlabKernel, dim3(Blocks(tot, 256)), dim3(256), 0, s, U, nx, ny, nz, n_fields, ng, fl, isize,
@@ -972,12 +975,16 @@ hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const doubl
   return hipGetLastError();
 }
 
+// dti_reset: the inverse-dt accumulator, zeroed by stage 1's first thread (null: left alone);
+// dti: the same accumulator when this launch sequence also reduces into it (null: a later
+// launch does).  Nothing is cleared with a memset, so the sequence can be graph-captured.
 template <class P, bool DE>
 static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, double dx, const double *dt_dev,
-                              double dt, double gamma, unsigned long long *dti, const GravSpec &grav, hipStream_t s)
+                              double dt, double gamma, unsigned long long *dti, unsigned long long *dti_reset,
+                              const GravSpec &grav, hipStream_t s)
 {
   const unsigned nb = Blocks(n, 256);
-  hipLaunchKernelGGL((Vl1dStage1Kernel<P, DE>), dim3(nb), dim3(256), 0, s, U, W, n, dx, dt_dev, dt, gamma, dti);
+  hipLaunchKernelGGL((Vl1dStage1Kernel<P, DE>), dim3(nb), dim3(256), 0, s, U, W, n, dx, dt_dev, dt, gamma, dti_reset);
   if constexpr (DE) {
     hipLaunchKernelGGL(Vl1dPdvKernel, dim3(nb), dim3(256), 0, s, U, n, ng, dx, dt_dev, dt, gamma);
     if (recon == kPPMC) {
@@ -1017,20 +1024,21 @@ static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, do
 }
 
 hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, bool de, double dx,
-                      const double *dt_dev, double dt, double gamma, unsigned long long *dti, const GravSpec &grav,
+                      const double *dt_dev, double dt, double gamma, unsigned long long *dti,
+                      unsigned long long *dti_reset, const GravSpec &grav,
                       hipStream_t s)
 {
   if (dti && n > 1024 && !de) return hipErrorInvalidValue;
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPPMP && recon != kPCM) return hipErrorInvalidValue;
   if (de) {  // HLLC with PLMC or PPMC
     if (riemann != kRiemannHllc || (recon != kPPMC && recon != kPLMC)) return hipErrorInvalidValue;
-    return LaunchVl1dP<PolicyExact, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
+    return LaunchVl1dP<PolicyExact, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
   }
-  if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
-  if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
+  if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+  if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
   if (riemann == kRiemannExact)
-    return LaunchVl1dP<ExactPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
-  if (riemann == kRiemannHllc) return LaunchVl1dP<PolicyExact, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, grav, s);
+    return LaunchVl1dP<ExactPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+  if (riemann == kRiemannHllc) return LaunchVl1dP<PolicyExact, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
   return hipErrorInvalidValue;
 }
 

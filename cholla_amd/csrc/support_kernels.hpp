@@ -9,11 +9,16 @@ namespace chl
 
 struct GhostFlags {
   int f[6];
+  int diode;  // 1: transmissive faces apply the hydro outflow diode; 0 for MHD states
 };
 
-hipError_t LaunchGhostFill(double *U, int nx, int ny, int nz, int n_fields, int ng, const int flags[6], hipStream_t s);
+// mhd: the state is the MHD build's (8 fields), whose fill has no outflow diode
+// (src/grid/cuda_boundaries.cu:121-147 is compiled under #ifndef MHD).
+hipError_t LaunchGhostFill(double *U, int nx, int ny, int nz, int n_fields, int ng, const int flags[6], hipStream_t s,
+                           bool mhd = false);
 hipError_t LaunchGhostSlab(double *U, int nx, int ny, int nz, int n_fields, int ng, const int flags[6], int isize,
-                           int jsize, int ksize, int imin, int jmin, int kmin, int dir, hipStream_t s);
+                           int jsize, int ksize, int imin, int jmin, int kmin, int dir, hipStream_t s,
+                           bool mhd = false);
 hipError_t LaunchCalcDt(const double *U, int nx, int ny, int nz, int ng, double dx, double dy, double dz,
                         double gamma, unsigned long long *dti, int num_cu, hipStream_t s);
 hipError_t LaunchTemperatureFloor(double *U, int nx, int ny, int nz, int ng, double U_floor, int n_fields,
@@ -29,9 +34,11 @@ hipError_t LaunchDtAdvance(const unsigned long long *dti, double *dt_dev, double
                            int *n_step, hipStream_t s);
 hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const double *modes, double vscale, double rho,
                               double p, double gamma, hipStream_t s);
+// dti: reduce the next inverse dt into it (stage 2 with one block, or the DE energy sync);
+// dti_reset: zero it first, from stage 1 (both may be the same pointer).
 hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, bool de, double dx,
-                      const double *dt_dev, double dt, double gamma, unsigned long long *dti, const GravSpec &grav,
-                      hipStream_t s);
+                      const double *dt_dev, double dt, double gamma, unsigned long long *dti,
+                      unsigned long long *dti_reset, const GravSpec &grav, hipStream_t s);
 hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, bool de, double dx,
                       double dy, const double *dt_dev, double dt, double gamma, unsigned long long *dti,
                       const GravSpec &grav, hipStream_t s);

@@ -276,6 +276,12 @@ int chl_mhd_time_steps(chl_mhd *h, double *d_conserved, double dx, double dy, do
 /* Calc_dt_GPU with MHD (src/hydro/hydro_cuda.cu:514-597, mhdInverseCrossingTime) */
 int chl_mhd_calc_dt(const double *dev_conserved, int nx, int ny, int nz, int n_ghost, double dx, double dy,
                     double dz, double gamma, double *dti_out);
+/* SetGhostCells and Set_Boundary_Conditions as the MHD build compiles them: the same slab
+ * fills on 8-field states without the transmissive outflow diode, which
+ * src/grid/cuda_boundaries.cu:121-147 leaves out under MHD (face fields copy like the rest). */
+int chl_mhd_set_ghost_cells(double *c_head, int nx, int ny, int nz, int n_fields, int n_cells, int n_ghost,
+                            const int flags[6], int isize, int jsize, int ksize, int imin, int jmin, int kmin, int dir);
+int chl_mhd_set_boundaries(double *dev_conserved, int nx, int ny, int nz, int n_ghost, const int flags[6]);
 /* mhd::checkMagneticDivergence (src/mhd/magnetic_divergence.cu:79): max |div B| */
 int chl_mhd_divergence(const double *dev_conserved, int nx, int ny, int nz, double dx, double dy, double dz,
                        double *max_divergence);

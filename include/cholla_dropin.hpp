@@ -11,9 +11,7 @@
 // Build options read at compile time, as the reference's make.type files set them:
 //   PLMC or PPMC, DENSITY_FLOOR, TEMPERATURE_FLOOR, MHD (with PLMC: the HLLD + CT integrator on
 //   8-field states, builds/make.type.mhd), and CHL_PRECISION (0 = FP64 in the reference's
-//   operation order [default], 1 = FP64 fast, 2 = mixed FP32/FP64).  Under MHD,
-//   mhd::checkMagneticDivergence(G) maps to chl_mhd_divergence(G.C.device, G.H.nx, G.H.ny,
-//   G.H.nz, G.H.dx, G.H.dy, G.H.dz, &div) (src/mhd/magnetic_divergence.cu:79).
+//   operation order [default], 1 = FP64 fast, 2 = mixed FP32/FP64), STATIC_GRAV.
 #pragma once
 
 #ifndef TYPEDEF_DEFINED_REAL
@@ -39,6 +37,20 @@ Real Calc_dt_GPU(Real *dev_conserved, int nx, int ny, int nz, int n_ghost, int n
                  Real gamma);
 void Apply_Temperature_Floor(Real *dev_conserved, int nx, int ny, int nz, int n_ghost, int n_fields, Real U_floor);
 
-// src/grid/cuda_boundaries.h:15-16
+// src/grid/cuda_boundaries.h:15-16 (under MHD without the outflow diode, cuda_boundaries.cu:121-147)
 void SetGhostCells(Real *c_head, int nx, int ny, int nz, int n_fields, int n_cells, int n_ghost, int flags[],
                    int isize, int jsize, int ksize, int imin, int jmin, int kmin, int dir);
+
+namespace mhd
+{
+// src/mhd/magnetic_divergence.h: Real checkMagneticDivergence(Grid3D const &G), called every
+// step by src/main.cpp:375-378.  Written as a template over the grid type so this header needs
+// no Cholla header; Cholla's Grid3D (G.C.device, G.H.nx .. G.H.dz) instantiates it at the call
+// site unchanged.  Prints the maximum |div B| and exits above 1e-9 (magnetic_divergence.cu:79-123).
+Real CheckMagneticDivergenceOfState(const Real *dev_conserved, int nx, int ny, int nz, Real dx, Real dy, Real dz);
+template <class Grid>
+Real checkMagneticDivergence(Grid const &G)
+{
+  return CheckMagneticDivergenceOfState(G.C.device, G.H.nx, G.H.ny, G.H.nz, G.H.dx, G.H.dy, G.H.dz);
+}
+}  // namespace mhd

@@ -1593,9 +1593,10 @@ static long BoundaryMapping(int ig, int jg, int kg, double *a, const int *flags,
   return idx;
 }
 
-// SetGhostCellsKernel over one face slab, src/grid/cuda_boundaries.cu:71-159
-void orc_set_ghost_cells(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags, int isize,
-                         int jsize, int ksize, int imin, int jmin, int kmin, int dir)
+// SetGhostCellsKernel over one face slab, src/grid/cuda_boundaries.cu:71-159.  diode: the
+// transmissive outflow diode of :121-147, which the MHD build compiles out (#ifndef MHD).
+static void SetGhostSlab(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags, int isize,
+                         int jsize, int ksize, int imin, int jmin, int kmin, int dir, bool diode)
 {
   size_t n_cells = size_t(nx) * ny * nz;
   for (int kk = 0; kk < ksize; kk++)
@@ -1610,7 +1611,7 @@ void orc_set_ghost_cells(double *c, int nx, int ny, int nz, int n_fields, int n_
         if (flags[0] == 2 || flags[1] == 2) c[g + n_cells] *= a[0];
         if (flags[2] == 2 || flags[3] == 2) c[g + 2 * n_cells] *= a[1];
         if (flags[4] == 2 || flags[5] == 2) c[g + 3 * n_cells] *= a[2];
-        if (flags[dir] == 3) {  // hydro outflow diode, cuda_boundaries.cu:121-147
+        if (diode && flags[dir] == 3) {  // hydro outflow diode, cuda_boundaries.cu:121-147
           size_t mom = g + size_t(dir / 2 + 1) * n_cells;
           if (dir % 2 == 0) {
             if (c[mom] > 0.0) {
@@ -1627,9 +1628,23 @@ void orc_set_ghost_cells(double *c, int nx, int ny, int nz, int n_fields, int n_
       }
 }
 
+void orc_set_ghost_cells(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags, int isize,
+                         int jsize, int ksize, int imin, int jmin, int kmin, int dir)
+{
+  SetGhostSlab(c, nx, ny, nz, n_fields, n_ghost, flags, isize, jsize, ksize, imin, jmin, kmin, dir, true);
+}
+
+// the same slab as the MHD build fills it: no outflow diode
+void orc_set_ghost_cells_mhd(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags,
+                             int isize, int jsize, int ksize, int imin, int jmin, int kmin, int dir)
+{
+  SetGhostSlab(c, nx, ny, nz, n_fields, n_ghost, flags, isize, jsize, ksize, imin, jmin, kmin, dir, false);
+}
+
 // Set_Boundary_Conditions (single process) -> Set_Boundaries(dir) for dir 0..5 with
 // the slab extents of Set_Boundary_Extents, src/grid/boundary_conditions.cpp:96-111, 413-505
-void orc_set_boundaries(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags_in)
+static void SetBoundaries(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags_in,
+                          bool diode)
 {
   int flags[6] = {0, 0, 0, 0, 0, 0};
   if (nx > 1) flags[0] = flags_in[0], flags[1] = flags_in[1];
@@ -1653,9 +1668,20 @@ void orc_set_boundaries(double *c, int nx, int ny, int nz, int n_fields, int n_g
       imin[0] = 0, imax[0] = nx, imin[1] = 0, imax[1] = ny;
       imin[2] = dir == 4 ? 0 : nz - n_ghost, imax[2] = dir == 4 ? n_ghost : nz;
     }
-    orc_set_ghost_cells(c, nx, ny, nz, n_fields, n_ghost, flags, imax[0] - imin[0], imax[1] - imin[1],
-                        imax[2] - imin[2], imin[0], imin[1], imin[2], dir);
+    SetGhostSlab(c, nx, ny, nz, n_fields, n_ghost, flags, imax[0] - imin[0], imax[1] - imin[1], imax[2] - imin[2],
+                 imin[0], imin[1], imin[2], dir, diode);
   }
+}
+
+void orc_set_boundaries(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags_in)
+{
+  SetBoundaries(c, nx, ny, nz, n_fields, n_ghost, flags_in, true);
+}
+
+// the MHD build's boundary fill (8 fields, no outflow diode)
+void orc_set_boundaries_mhd(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags_in)
+{
+  SetBoundaries(c, nx, ny, nz, n_fields, n_ghost, flags_in, false);
 }
 
 // ---------------------------------------------------------------------------

@@ -48,7 +48,7 @@ void orc_update_3d(double *U, const double *Fx, const double *Fy, const double *
                    int n_ghost, double dx, double dy, double dz, double dt, int n_fields, int density_floor_on,
                    double density_floor);
 void orc_temperature_floor(double *U, int nx, int ny, int nz, int n_ghost, double U_floor);
-void orc_set_boundaries(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags_in);
+void orc_set_boundaries_mhd(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags_in);
 }
 
 namespace
@@ -872,7 +872,7 @@ int orc_mhd_run(double *U, int nx, int ny, int nz, int n_ghost, double dx, doubl
     if (dt_log) dt_log[steps] = dt;
     t += dt;
     steps++;
-    orc_set_boundaries(U, nx, ny, nz, 8, n_ghost, flags);
+    orc_set_boundaries_mhd(U, nx, ny, nz, 8, n_ghost, flags);
   }
   *t_io = t;
   return steps;

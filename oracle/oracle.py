@@ -72,6 +72,8 @@ def lib():
             "orc_calc_dti": (d, [_dp, i, i, i, i, d, d, d, d]),
             "orc_set_ghost_cells": (None, [_dp, i, i, i, i, i, _ip, i, i, i, i, i, i, i]),
             "orc_set_boundaries": (None, [_dp, i, i, i, i, i, _ip]),
+            "orc_set_boundaries_mhd": (None, [_dp, i, i, i, i, i, _ip]),
+            "orc_set_ghost_cells_mhd": (None, [_dp, i, i, i, i, i, _ip, i, i, i, i, i, i, i]),
             "orc_vl3d": (None, [_dp, i, i, i, i, d, d, d, d, d, i, i, i, d]),
             "orc_vl1d": (None, [_dp, i, i, d, d, d, i, i]),
             "orc_run": (i, [_dp, i, i, i, i, i, d, d, d, d, i, _ip, _dp, d, i, i, d, i, d, _dp]),
@@ -133,6 +135,12 @@ def calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma):
 def set_boundaries(U, nx, ny, nz, n_fields, n_ghost, flags):
     f, fp = iptr(flags)
     lib().orc_set_boundaries(ptr(U), nx, ny, nz, n_fields, n_ghost, fp)
+
+
+def mhd_set_boundaries(U, nx, ny, nz, n_ghost, flags):
+    """The MHD build's boundary fill (8 fields, no outflow diode)."""
+    f, fp = iptr(flags)
+    lib().orc_set_boundaries_mhd(ptr(U), nx, ny, nz, 8, n_ghost, fp)
 
 
 @contextlib.contextmanager

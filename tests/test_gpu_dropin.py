@@ -136,3 +136,59 @@ def test_dropin_dual_energy_ppmc(tmp_path):
                           dual_energy_on=True)
     assert ns == steps_o == 8 and t == to
     np.testing.assert_array_equal(A, Uo)
+
+
+def test_dropin_static_gravity_plmc(tmp_path):
+    """The -DSTATIC_GRAV build of the adapter with Parameters::custom_grav = 1 (NFW halo +
+    Miyamoto-Nagai disk) on a 3D box, driven like Cholla's loop: the adapter leaves the stage-2
+    tile to the library, which must take one with a gravity kernel.  The field calls log, pow and
+    sqrt, whose last bit differs between the host's and the device's math libraries, so the bar
+    is a per-field relative L1 of 1e-12 against the oracle's driver loop."""
+    from test_gpu_gravity import G, disk_box, rel_l1
+
+    n, ng = 16, 3
+    U, (N, _, _) = disk_box((n, n, n), ng, vamp=1e-2)
+    orc.set_boundaries(U, N, N, N, 5, ng, [1] * 6)
+    h = 0.6
+    exe = os.path.join(os.path.dirname(chl.LIB_PATH), "dropin_driver_plmc_grav")
+    fin, fout = os.path.join(tmp_path, "in.bin"), os.path.join(tmp_path, "out.bin")
+    with open(fin, "wb") as f:
+        f.write(struct.pack(HDR, N, N, N, ng, 5, 5, *([1] * 6), h, h, h, G, 0.0, 1e30, 0.0, 0.0))
+        f.write(np.ascontiguousarray(U, dtype="<f8").tobytes())
+    print("[gpu-case] driver dropin_driver_plmc_grav", N, flush=True)
+    r = subprocess.run([exe, fin, fout, "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = open(fout, "rb").read()
+    hsz = struct.calcsize(HDR)
+    A = np.frombuffer(raw[hsz:hsz + 8 * U.size], dtype="<f8").copy()
+    ns, = struct.unpack("<i", raw[hsz + 8 * U.size:hsz + 8 * U.size + 4])
+    Uo = U.copy()
+    with orc.static_gravity(1, (0, 0, 0), (0.0, 0.0, 0.0)):
+        steps_o, _ = orc.run(Uo, N, N, N, ng, 5, h, h, h, G, orc.PLMC, [1] * 6, 0.0, 1e30, 5)
+    assert ns == steps_o == 5
+    l1 = rel_l1(interior(A, 5, N, N, N, ng), interior(Uo, 5, N, N, N, ng))
+    assert max(l1) <= 1e-12, l1
+    # the field changed the state: the same run without gravity differs by far more
+    Un = U.copy()
+    orc.run(Un, N, N, N, ng, 5, h, h, h, G, orc.PLMC, [1] * 6, 0.0, 1e30, 5)
+    assert max(rel_l1(interior(Un, 5, N, N, N, ng), interior(Uo, 5, N, N, N, ng))) > 1e-9
+
+
+def test_dropin_mhd_checks_divergence(tmp_path):
+    """The MHD driver calls mhd::checkMagneticDivergence(G) after every step (main.cpp:375-378),
+    which prints the maximum |div B| and would exit above 1e-9; on an Orszag-Tang box it prints
+    one round-off value per step."""
+    n, ng, g = 16, 4, 5 / 3
+    N = n + 2 * ng
+    h = 1.0 / n
+    U = chl.ic_orszag_tang(n, ng, g, seed=9)
+    fin, fout = os.path.join(tmp_path, "in.bin"), os.path.join(tmp_path, "out.bin")
+    with open(fin, "wb") as f:
+        f.write(struct.pack(HDR, N, N, N, ng, 8, 3, *([1] * 6), h, h, h, g, 0.0, 1e30, 0.0, 0.0))
+        f.write(np.ascontiguousarray(U, dtype="<f8").tobytes())
+    exe = os.path.join(os.path.dirname(chl.LIB_PATH), "dropin_driver_mhd")
+    r = subprocess.run([exe, fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [l for l in r.stdout.splitlines() if l.startswith("Global maximum magnetic divergence")]
+    assert len(lines) == 3, r.stdout
+    assert all(float(l.split("=")[1]) < 1e-10 for l in lines), lines

@@ -124,9 +124,15 @@ def test_1d_static_gravity():
 
 
 def test_static_gravity_limits():
-    hy = chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64)  # default stage-2 tile 3
+    hy = chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, tile2=3)  # an explicit tile without gravity
     with pytest.raises(chl.ChollaError, match="tile shapes 0 and 1"):
         hy.set_static_gravity(1)
     with pytest.raises(chl.ChollaError, match="out of range"):
         hy.set_static_gravity(2)
+    hy.close()
+    # a stage-2 tile left to the library moves to one with a gravity kernel
+    hy = chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64)
+    assert hy.config().tile2 == 3
+    hy.set_static_gravity(1)
+    assert hy.config().tile2 == 1
     hy.close()

@@ -263,3 +263,57 @@ def test_mhd_fast_linear_wave():
     l1 = [np.abs(U1[f] - U0[f]).mean() for f in range(8)]
     assert max(l1) < 6.5e-8 and math.sqrt(sum(x * x for x in l1)) < 6.5e-8, l1
     h.close()
+
+
+@pytest.mark.parametrize("flags", [[3] * 6, [2, 3, 1, 1, 3, 2]])
+def test_mhd_ghost_fill_has_no_diode(flags):
+    """The MHD boundary fill (one launch, and slab by slab as SetGhostCells) is bit-identical to
+    the oracle's MHD fill, which has no outflow diode (cuda_boundaries.cu:121-147 is #ifndef MHD)."""
+    say("MHD ghost fill", flags)
+    ng, nx, ny, nz = 4, 13, 12, 14
+    rng = np.random.default_rng(5)
+    base = rng.uniform(0.5, 1.5, (8, nz, ny, nx))
+    base[4] += 5.0
+    base[1:4] -= 1.0
+    U = base.reshape(-1).copy()
+    Uo = U.copy()
+    orc.mhd_set_boundaries(Uo, nx, ny, nz, ng, flags)
+    Ud = chl.DeviceArray.from_host(U)
+    chl.mhd_set_boundaries(Ud, nx, ny, nz, ng, flags)
+    np.testing.assert_array_equal(Ud.download(), Uo)
+    Ud2 = chl.DeviceArray.from_host(U)
+    jl, ju, kl, ku = ng, ny - ng, ng, nz - ng
+    for dr in range(6):
+        if dr < 2:
+            i0, i1 = (0, ng) if dr == 0 else (nx - ng, nx)
+            j0, j1, k0, k1 = jl, ju, kl, ku
+        elif dr < 4:
+            i0, i1 = 0, nx
+            j0, j1 = (0, ng) if dr == 2 else (ny - ng, ny)
+            k0, k1 = kl, ku
+        else:
+            i0, i1, j0, j1 = 0, nx, 0, ny
+            k0, k1 = (0, ng) if dr == 4 else (nz - ng, nz)
+        chl.mhd_set_ghost_cells(Ud2, nx, ny, nz, ng, flags, i1 - i0, j1 - j0, k1 - k0, i0, j0, k0, dr)
+    np.testing.assert_array_equal(Ud2.download(), Uo)
+
+
+def test_mhd_outflow_run_matches_oracle():
+    """An Orszag-Tang box with transmissive x faces and reflective y faces: 6 device-timed steps
+    of the FP64 reference-order form, bit-identical to the MHD oracle's driver loop (whose
+    boundary fill has no diode)."""
+    n, ng, g = 16, 4, 5 / 3
+    N = n + 2 * ng
+    h_ = 1.0 / n
+    flags = [3, 3, 2, 2, 1, 1]
+    U = chl.ic_orszag_tang(n, ng, g, seed=13, amplitude=1e-3)
+    orc.mhd_set_boundaries(U, N, N, N, ng, flags)
+    Uo = U.copy()
+    steps_o, t_o = orc.mhd_run(Uo, N, N, N, ng, h_, h_, h_, g, flags, 0.0, 1e30, 6)
+    say("MHD outflow run 16^3, flags", flags)
+    Ud = chl.DeviceArray.from_host(U)
+    h = chl.Mhd(N, N, N, ng, chl.PREC_FP64_EXACT, bc=flags)
+    steps, t = h.run(Ud, h_, h_, h_, g, 0.0, 1e30, 6, use_graph=True)
+    assert steps == steps_o == 6 and t == t_o
+    np.testing.assert_array_equal(Ud.download(), Uo)
+    h.close()

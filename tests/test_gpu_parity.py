@@ -291,6 +291,26 @@ def test_linear_wave_system(recon):
     assert max(l1) < allowed, l1
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_sod_1d_large_unfused_dt(use_graph):
+    """A 1D tube above 1024 cells without dual energy: the next inverse dt is its own grid-wide
+    reduction after the step, whose accumulator stage 1 zeroes in-kernel (no memset in the
+    captured step).  2000 cells, PPMC + HLLC, to t = 0.05: bit-identical to the oracle."""
+    U, nx, dx = sod_ic(n_real=2000, ng=4)
+    ng = 4
+    orc.set_boundaries(U, nx, 1, 1, 5, ng, [3, 3, 0, 0, 0, 0])
+    Uo = U.copy()
+    steps_o, to = orc.run(Uo, nx, 1, 1, ng, 5, dx, dx, dx, 1.4, orc.PPMC, [3, 3, 0, 0, 0, 0], 0.0, 0.05, 100000)
+    say("Sod 1D, 2000 cells, graph", use_graph)
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(nx, 1, 1, ng, chl.RECON_PPMC, chl.PREC_FP64_EXACT, bc=[3, 3, 1, 1, 1, 1])
+    steps, t = hy.run(Ud, dx, dx, dx, 1.4, 0.0, 0.05, 100000, use_graph=use_graph)
+    assert steps == steps_o and t == to
+    A = Ud.download().reshape(5, nx)
+    np.testing.assert_array_equal(A[:, ng:nx - ng], Uo.reshape(5, nx)[:, ng:nx - ng])
+    hy.close()
+
+
 def test_sod_1d():
     U, nx, dx = sod_ic()
     ng = 3

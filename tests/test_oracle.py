@@ -249,53 +249,85 @@ def test_hllc_consistency_upwind_symmetry(seed):
 
 
 # ----------------------------------------------------------------------------- ghost cells
+def _ghost_fill_expected(base, flags, ng, diode):
+    """Brute-force ghost fill of a cubic box (SetGhostCellsKernel + FindIndex semantics, one slab
+    owner per ghost cell): every ghost cell maps to a real cell, reflective axes flip that axis's
+    momentum, and with the hydro diode a transmissive face zeroes an inflowing normal momentum and
+    removes its kinetic energy."""
+    N = base.shape[1]
+    out = base.copy()
+    for k in range(N):
+        for j in range(N):
+            for i in range(N):
+                idx, sgn = [], [1.0, 1.0, 1.0]
+                for ax, (g_, f_lo, f_hi) in enumerate(((i, flags[0], flags[1]), (j, flags[2], flags[3]),
+                                                       (k, flags[4], flags[5]))):
+                    if g_ < ng:
+                        f = f_lo
+                        r = {1: g_ + N - 2 * ng, 2: 2 * ng - g_ - 1, 3: ng}[f]
+                    elif g_ >= N - ng:
+                        f = f_hi
+                        r = {1: g_ - N + 2 * ng, 2: 2 * (N - ng) - g_ - 1, 3: N - ng - 1}[f]
+                    else:
+                        f, r = 0, g_
+                    if f == 2:
+                        sgn[ax] = -1.0
+                    idx.append(r)
+                if (i, j, k) == tuple(idx):
+                    continue
+                src = base[:, idx[2], idx[1], idx[0]].copy()
+                for ax in range(3):
+                    if flags[2 * ax] == 2 or flags[2 * ax + 1] == 2:
+                        src[1 + ax] *= sgn[ax]
+                # the face that owns this ghost cell: z slabs, then y, then x (boundary_conditions.cpp:419-505)
+                if k < ng or k >= N - ng:
+                    face = 4 if k < ng else 5
+                elif j < ng or j >= N - ng:
+                    face = 2 if j < ng else 3
+                else:
+                    face = 0 if i < ng else 1
+                if diode and flags[face] == 3:
+                    m = 1 + face // 2
+                    if (face % 2 == 0 and src[m] > 0) or (face % 2 == 1 and src[m] < 0):
+                        src[4] -= 0.5 * src[m] * src[m] / src[0]
+                        src[m] = 0.0
+                out[:, k, j, i] = src
+    return out
+
+
 def test_ghost_fill_periodic_reflective_transmissive_bruteforce():
     ng, n = 3, 4
     N = n + 2 * ng
     rng = np.random.default_rng(7)
     base = rng.uniform(0.5, 1.5, (5, N, N, N))
     base[4] += 5.0
+    base[1:4] -= 1.0  # momenta of both signs, so the diode acts on some transmissive ghosts
     for flags in ([1] * 6, [2] * 6, [3] * 6, [1, 1, 2, 2, 3, 3], [3, 2, 1, 1, 2, 3]):
         U = base.copy().reshape(-1)
         orc.set_boundaries(U, N, N, N, 5, ng, flags)
         V = U.reshape(5, N, N, N)
-        R = base  # real cells are untouched by the fill
-        for k in range(N):
-            for j in range(N):
-                for i in range(N):
-                    idx, sgn = [], [1.0, 1.0, 1.0]
-                    for ax, (g_, f_lo, f_hi) in enumerate(((i, flags[0], flags[1]), (j, flags[2], flags[3]),
-                                                           (k, flags[4], flags[5]))):
-                        if g_ < ng:
-                            f = f_lo
-                            r = {1: g_ + N - 2 * ng, 2: 2 * ng - g_ - 1, 3: ng}[f]
-                        elif g_ >= N - ng:
-                            f = f_hi
-                            r = {1: g_ - N + 2 * ng, 2: 2 * (N - ng) - g_ - 1, 3: N - ng - 1}[f]
-                        else:
-                            f, r = 0, g_
-                        if f == 2:
-                            sgn[ax] = -1.0
-                        idx.append(r)
-                    if (i, j, k) == tuple(idx):
-                        continue
-                    src = R[:, idx[2], idx[1], idx[0]].copy()
-                    for ax in range(3):
-                        if flags[2 * ax] == 2 or flags[2 * ax + 1] == 2:
-                            src[1 + ax] *= sgn[ax]
-                    # the face that owns this ghost cell: z slabs, then y, then x (boundary_conditions.cpp:419-505)
-                    if k < ng or k >= N - ng:
-                        face = 4 if k < ng else 5
-                    elif j < ng or j >= N - ng:
-                        face = 2 if j < ng else 3
-                    else:
-                        face = 0 if i < ng else 1
-                    if flags[face] == 3:
-                        m = 1 + face // 2
-                        if (face % 2 == 0 and src[m] > 0) or (face % 2 == 1 and src[m] < 0):
-                            src[4] -= 0.5 * src[m] * src[m] / src[0]
-                            src[m] = 0.0
-                    assert np.array_equal(V[:, k, j, i], src), (flags, i, j, k)
+        assert np.array_equal(V, _ghost_fill_expected(base, flags, ng, diode=True)), flags
+
+
+def test_ghost_fill_mhd_has_no_diode():
+    """The MHD build compiles the outflow diode out (cuda_boundaries.cu:121-147 is #ifndef MHD):
+    transmissive ghosts of an 8-field state are plain copies of the nearest real cell (reflective
+    faces still flip the momentum), face fields included.  The hydro fill of the same data must
+    differ, so a diode left in the MHD fill fails."""
+    ng, n = 4, 4
+    N = n + 2 * ng
+    rng = np.random.default_rng(8)
+    base = rng.uniform(0.5, 1.5, (8, N, N, N))
+    base[4] += 5.0
+    base[1:4] -= 1.0
+    for flags in ([3] * 6, [2, 3, 1, 1, 3, 2]):
+        U = base.copy().reshape(-1)
+        orc.mhd_set_boundaries(U, N, N, N, ng, flags)
+        exp = _ghost_fill_expected(base, flags, ng, diode=False)
+        assert np.array_equal(U.reshape(8, N, N, N), exp), flags
+        H = base.copy().reshape(-1)
+        orc.set_boundaries(H, N, N, N, 8, ng, flags)
+        assert not np.array_equal(H, U)
 
 
 # ----------------------------------------------------------------------------- integrated steps
