@@ -36,38 +36,6 @@ using chl_capi::kCfl;
 using chl_capi::ReconRadius;
 }  // namespace
 
-struct chl_hydro {
-  chl_hydro_config cfg{};
-  size_t n_cells = 0;
-  bool one_d     = false;
-  bool two_d     = false;  // nz == 1 < ny: the unfused 2D step, W holds its scratch
-  chl::GravSpec grav{};    // static gravity (chl_hydro_set_static_gravity); id 0 = none
-  bool tile2_auto = false; // the caller left tile2 to the library (-1)
-  int num_cu     = 256;
-  void *W        = nullptr;  // half-step primitives (+ the pdV sources with dual energy)
-  size_t W_bytes = 0;
-  double *pdv    = nullptr;  // inside W: dual energy's full-step pdV source per cell
-  // device scalars: [0] dti bits, [1] dt, [2] t, [3] n_step (int in a double slot)
-  unsigned char *scal = nullptr;
-  hipStream_t stream  = nullptr;
-  hipGraphExec_t graph = nullptr;
-  // key of the captured graph
-  double *g_U = nullptr;
-  double g_key[7]{};
-
-  unsigned long long *dti() { return reinterpret_cast<unsigned long long *>(scal); }
-  double *dt() { return reinterpret_cast<double *>(scal + 8); }
-  double *t() { return reinterpret_cast<double *>(scal + 16); }
-  int *nstep() { return reinterpret_cast<int *>(scal + 24); }
-
-  bool PeriodicFusable() const
-  {
-    for (int q = 0; q < 6; q++)
-      if (cfg.bc_flags[q] != CHL_BC_PERIODIC) return false;
-    int ng = cfg.n_ghost;
-    return cfg.nx - 2 * ng >= ng && cfg.ny - 2 * ng >= ng && cfg.nz - 2 * ng >= ng;
-  }
-};
 
 namespace
 {
@@ -134,6 +102,34 @@ int SolverKind(int riemann)
 
 int DefaultChunk(int n_real) { return std::max(8, std::min(n_real, 32)); }
 
+// Gravity records a step keeps for repaired cells (static gravity only).
+constexpr int kRepairGravCap = 4096;
+
+// The repair scratch of a 3D box: counts, gravity records, the side buffer of the six
+// boundary layers, one mark byte per 4096 cells and one per cell.  RepairBytes sizes it,
+// PlaceRepair points h->rep into it.
+size_t RepairLayout(const chl_hydro_config &c, size_t n_cells, size_t off[4])
+{
+  const size_t ng = size_t(c.n_ghost), Lx = c.nx - 2 * ng, Ly = c.ny - 2 * ng, Lz = c.nz - 2 * ng;
+  const size_t chunk = size_t(1) << chl::kRepairChunkShift;
+  const size_t n_chunks = (n_cells + chunk - 1) / chunk;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  off[0] = 256;                                                            // gravity records
+  off[1] = off[0] + up(size_t(kRepairGravCap) * 5 * sizeof(double));      // side buffer
+  off[2] = off[1] + up(2 * (Ly * Lz + Lx * Lz + Lx * Ly) * sizeof(double));  // chunk marks
+  off[3] = off[2] + up(n_chunks);                                          // cell marks
+  return off[3] + up(n_chunks * chunk);
+}
+
+hipError_t AllocRepair(chl_hydro *h)
+{
+  const size_t bytes = chl_capi::RepairBytes(h->cfg);
+  hipError_t e       = hipMalloc(&h->rep_buf, bytes);
+  if (e != hipSuccess) return e;
+  chl_capi::PlaceRepair(h, h->rep_buf);
+  return hipMemsetAsync(h->rep_buf, 0, bytes, h->stream);
+}
+
 // Arguments of the fused stage kernels for this box (kchunk is set per stage).
 chl::StepArgs MakeStepArgs(chl_hydro *h, double *U, double dx, double dy, double dz, double gamma, double dt_host,
                            const double *dt_dev, double density_floor, double U_floor, int flags)
@@ -148,6 +144,7 @@ chl::StepArgs MakeStepArgs(chl_hydro *h, double *U, double dx, double dy, double
   a.de        = c.dual_energy;
   a.riemann   = SolverKind(c.riemann);
   a.grav      = h->grav;
+  a.rep       = h->rep;
   return a;
 }
 
@@ -211,18 +208,37 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
     if (fill) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
     return CHL_OK;
   }
-  int flags = 0;
+  // A build without both floors repairs crashed cells from the 3x3x3 block of the update
+  // alone (Average_Cell_All_Fields, hydro_cuda.cu:375-381): there the temperature floor, the
+  // dt reduction and the ghost fill run as their own kernels after the repairs, not fused
+  // into stage 2 (the dual-energy kernels, which carry them otherwise, already follow).
+  const bool crash   = !(c.density_floor && c.temperature_floor);
+  const bool unfused = crash && !c.dual_energy;
+  int flags          = 0;
   if (c.density_floor) flags |= chl::kFlagDensityFloor;
-  if (update_grid && c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
-  if (update_grid) flags |= chl::kFlagReduceDt;
+  if (crash) flags |= chl::kFlagCrashRepair;
+  if (update_grid && c.temperature_floor && !unfused) flags |= chl::kFlagTemperatureFloor;
+  if (update_grid && !unfused) flags |= chl::kFlagReduceDt;
+  if (update_grid) flags |= chl::kFlagResetDti;
   if (c.positivity_fallback) flags |= chl::kFlagPositivityFallback;
   if (c.reconstruction == CHL_RECON_PCM) flags |= chl::kFlagConservedW;
-  const bool fused_ghosts = fill && h->PeriodicFusable();
+  const bool fused_ghosts = fill && h->PeriodicFusable() && !unfused;
   if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
   chl::StepArgs a = MakeStepArgs(h, U, dx, dy, dz, gamma, dt_host, dt_dev, density_floor, U_floor, flags);
+  // stage 2 overwrites ghost cells with new periodic images: a density repair next to one
+  // reads the pre-step value from the side buffer instead
+  if (!(fused_ghosts && c.density_floor && !c.dual_energy)) a.rep.side = nullptr;
+  if (a.rep.side) CHL_HIP(chl::LaunchRepairSide(a, h->stream));
   CHL_TRY(LaunchStage1(h, a));
   CHL_TRY(LaunchStage2(h, a));
+  CHL_HIP(c.precision == CHL_PREC_FP64_EXACT ? chl::LaunchRepairExact(a, h->stream)
+                                             : chl::LaunchRepairFast(c.precision, a, h->stream));
   CHL_TRY(LaunchDualEnergy(h, a));
+  if (unfused && update_grid) {
+    if (c.temperature_floor)
+      CHL_HIP(chl::LaunchTemperatureFloor(U, c.nx, c.ny, c.nz, ng, U_floor, c.n_fields, h->stream));
+    CHL_HIP(chl::LaunchCalcDt(U, c.nx, c.ny, c.nz, ng, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
+  }
   if (fill && !fused_ghosts) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
   return CHL_OK;
 }
@@ -292,6 +308,54 @@ int EnsureGraph(chl_hydro *h, double *U, double dx, double dy, double dz, double
 
 }  // namespace
 
+size_t chl_capi::RepairBytes(const chl_hydro_config &c)
+{
+  size_t off[4];
+  return RepairLayout(c, size_t(c.nx) * c.ny * c.nz, off);
+}
+
+void chl_capi::PlaceRepair(chl_hydro *h, void *base)
+{
+  size_t off[4];
+  (void)RepairLayout(h->cfg, h->n_cells, off);
+  unsigned char *b = static_cast<unsigned char *>(base);
+  h->rep.count     = reinterpret_cast<unsigned *>(b);
+  h->rep.grav      = reinterpret_cast<double *>(b + off[0]);
+  h->rep.grav_cap  = kRepairGravCap;
+  h->rep.side      = reinterpret_cast<double *>(b + off[1]);
+  h->rep.chunk     = b + off[2];
+  h->rep.mark      = b + off[3];
+}
+
+size_t chl_capi::HalfStepBytes(const chl_hydro_config &c, size_t *w_main)
+{
+  const size_t n    = size_t(c.nx) * c.ny * c.nz;
+  const size_t elem = c.precision == CHL_PREC_MIXED ? sizeof(float) : sizeof(double);
+  // W: 5 half-step fields, the pressure's low part in the mixed form, GasEnergy with DE
+  const size_t w_fields = 5 + size_t(c.precision == CHL_PREC_MIXED) + size_t(c.dual_energy);
+  const size_t main_    = (w_fields * n * elem + 255) / 256 * 256;  // pdV sources start aligned
+  if (w_main) *w_main = main_;
+  return main_ + (c.dual_energy ? n * sizeof(double) : 0);
+}
+
+int chl_capi::PrepareConfig(chl_hydro_config &c)
+{
+  if (c.tile1 < 0) c.tile1 = 0;
+  const bool narrow = c.dual_energy || c.riemann != CHL_RIEMANN_HLLC || c.reconstruction == CHL_RECON_PCM ||
+                      c.reconstruction == CHL_RECON_PLMP || c.reconstruction == CHL_RECON_PPMP;  // kernels built for tile shapes 0 and 1 only
+  if (c.tile2 < 0) c.tile2 = narrow ? 0 : 3;
+  CHL_TRY(CheckConfig(c));
+  if (c.kchunk1 <= 0) c.kchunk1 = DefaultChunk(c.nz - 2 * c.n_ghost + 2 * (ReconRadius(c.reconstruction) + 1));
+  if (c.kchunk2 <= 0) c.kchunk2 = DefaultChunk(c.nz - 2 * c.n_ghost);
+  return CHL_OK;
+}
+
+int chl_capi::EnqueueVlStep(chl_hydro *h, double *U, double dx, double dy, double dz, double dt, double gamma,
+                            double density_floor)
+{
+  return EnqueueStep(h, U, dx, dy, dz, gamma, dt, nullptr, density_floor, 0.0, false, false);
+}
+
 extern "C" {
 
 const char *chl_version(void) { return "cholla_amd 0.1 (gfx950)"; }
@@ -354,18 +418,12 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return Fail(CHL_ERR_NO_DEVICE, "no HIP device");
   chl_hydro_config c = *cfg;
-  if (c.tile1 < 0) c.tile1 = 0;
-  const bool narrow = c.dual_energy || c.riemann != CHL_RIEMANN_HLLC || c.reconstruction == CHL_RECON_PCM ||
-                     c.reconstruction == CHL_RECON_PLMP || c.reconstruction == CHL_RECON_PPMP;  // kernels built for tile shapes 0 and 1 only
-  if (c.tile2 < 0) c.tile2 = narrow ? 0 : 3;
-  CHL_TRY(CheckConfig(c));
+  CHL_TRY(chl_capi::PrepareConfig(c));
   auto *h    = new chl_hydro();
   h->tile2_auto = cfg->tile2 < 0;
   h->one_d   = c.ny == 1 && c.nz == 1;
   h->two_d   = c.nz == 1 && c.ny > 1;
   h->n_cells = size_t(c.nx) * c.ny * c.nz;
-  if (c.kchunk1 <= 0) c.kchunk1 = DefaultChunk(c.nz - 2 * c.n_ghost + 2 * (ReconRadius(c.reconstruction) + 1));
-  if (c.kchunk2 <= 0) c.kchunk2 = DefaultChunk(c.nz - 2 * c.n_ghost);
   h->cfg = c;
   int dev;
   hipDeviceProp_t p;
@@ -374,17 +432,17 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
     return Fail(CHL_ERR_HIP, "hipGetDeviceProperties failed");
   }
   h->num_cu        = p.multiProcessorCount;
-  size_t elem      = c.precision == CHL_PREC_MIXED ? sizeof(float) : sizeof(double);
-  const size_t w_fields = 5 + size_t(c.dual_energy);
-  h->W_bytes            = w_fields * h->n_cells * elem + (c.dual_energy ? h->n_cells * sizeof(double) : 0);
+  size_t w_main    = 0;
+  h->W_bytes       = chl_capi::HalfStepBytes(c, &w_main);
   if (h->two_d) h->W_bytes = 42 * h->n_cells * sizeof(double);  // half state, 4 interface, 2 flux arrays of 6 fields
   hipError_t e     = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&h->W, h->W_bytes);
   if (e == hipSuccess && c.dual_energy)
-    h->pdv = reinterpret_cast<double *>(static_cast<unsigned char *>(h->W) + w_fields * h->n_cells * elem);
+    h->pdv = reinterpret_cast<double *>(static_cast<unsigned char *>(h->W) + w_main);
   if (e == hipSuccess) e = hipMemsetAsync(h->W, 0, h->W_bytes, h->stream);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&h->scal), 64);
   if (e == hipSuccess) e = hipMemsetAsync(h->scal, 0, 64, h->stream);
+  if (e == hipSuccess && !h->one_d && !h->two_d) e = AllocRepair(h);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) {
     chl_hydro_destroy(h);
@@ -400,6 +458,7 @@ int chl_hydro_destroy(chl_hydro *h)
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->graph) (void)hipGraphExecDestroy(h->graph);
   if (h->W) (void)hipFree(h->W);
+  if (h->rep_buf) (void)hipFree(h->rep_buf);
   if (h->scal) (void)hipFree(h->scal);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -512,14 +571,18 @@ int chl_hydro_kernel_times(chl_hydro *h, double *U, double dx, double dy, double
   const bool fused_ghosts = h->PeriodicFusable();
   if (fused_ghosts) flags |= chl::kFlagPeriodicGhosts;
   chl::StepArgs a = MakeStepArgs(h, U, dx, dy, dz, gamma, 0.0, h->dt(), 0.0, 0.0, flags);
+  if (!(fused_ghosts && c.density_floor && !c.dual_energy)) a.rep.side = nullptr;
   for (int s = 0; s < n_steps; s++) {
     CHL_HIP(hipEventRecord(ev[0], h->stream));
     CHL_HIP(chl::LaunchDtAdvance(h->dti(), h->dt(), h->t(), tout, kCfl, h->nstep(), h->stream));
+    if (a.rep.side) CHL_HIP(chl::LaunchRepairSide(a, h->stream));
     CHL_HIP(hipEventRecord(ev[1], h->stream));
     CHL_TRY(LaunchStage1(h, a));
     CHL_HIP(hipEventRecord(ev[2], h->stream));
     CHL_TRY(LaunchStage2(h, a));
     CHL_HIP(hipEventRecord(ev[3], h->stream));
+    CHL_HIP(c.precision == CHL_PREC_FP64_EXACT ? chl::LaunchRepairExact(a, h->stream)
+                                               : chl::LaunchRepairFast(c.precision, a, h->stream));
     CHL_TRY(LaunchDualEnergy(h, a));
     CHL_HIP(hipEventRecord(ev[4], h->stream));
     if (!fused_ghosts)
@@ -596,6 +659,7 @@ int chl_hydro_autotune(chl_hydro *h, const double *U, double dx, double dy, doub
                                      (c.temperature_floor ? chl::kFlagTemperatureFloor : 0) |
                                      (c.positivity_fallback ? chl::kFlagPositivityFallback : 0) |
                                      (c.reconstruction == CHL_RECON_PCM ? chl::kFlagConservedW : 0));
+  a.rep = chl::RepairState{};  // timing runs on a scratch copy: nothing is marked or repaired
   const int R1     = ReconRadius(c.reconstruction) + 1;
   const int nz1    = c.nz - 2 * c.n_ghost + 2 * R1, nz2 = c.nz - 2 * c.n_ghost;
   const bool narrow = c.dual_energy || c.riemann != CHL_RIEMANN_HLLC || c.reconstruction == CHL_RECON_PCM ||

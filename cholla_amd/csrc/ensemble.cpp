@@ -118,6 +118,7 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
   const int ng = c.n_ghost, R1 = ReconRadius(c.reconstruction) + 1;
   const chl::TileShape s1 = chl::kTiles[c.tile1], s2 = chl::kTiles[c.tile2];
   const size_t elem       = c.precision == CHL_PREC_MIXED ? sizeof(float) : sizeof(double);
+  const size_t w_fields   = 5 + size_t(c.precision == CHL_PREC_MIXED) + size_t(c.dual_energy);
   e->hbox.assign(e->n, chl::BoxDesc{});
   std::vector<size_t> w_off(e->n);
   size_t w_total = 0;
@@ -139,8 +140,9 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
     d.dx = e->dxyz[3 * b], d.dy = e->dxyz[3 * b + 1], d.dz = e->dxyz[3 * b + 2];
     d.tout   = e->tout[b];
     w_off[b] = w_total;
-    // W: 5 (6 with dual energy) half-step fields, then dual energy's pdV sources in FP64
-    w_total += ((5 + c.dual_energy) * e->Cells(b) * elem + 255) / 256 * 256;
+    // W: 5 half-step fields (one more, the pressure's low part, in the mixed form; one
+    // more, GasEnergy, with dual energy), then dual energy's pdV sources in FP64
+    w_total += (w_fields * e->Cells(b) * elem + 255) / 256 * 256;
     if (c.dual_energy) w_total += (e->Cells(b) * sizeof(double) + 255) / 256 * 256;
   }
   if (w_total > e->W_bytes) {
@@ -153,7 +155,7 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
   for (int b = 0; b < e->n; b++) {
     e->hbox[b].W   = e->W + w_off[b];
     e->hbox[b].pdv = c.dual_energy ? reinterpret_cast<double *>(
-                                         e->W + w_off[b] + ((6 * e->Cells(b) * elem + 255) / 256 * 256))
+                                         e->W + w_off[b] + ((w_fields * e->Cells(b) * elem + 255) / 256 * 256))
                                    : nullptr;
   }
 

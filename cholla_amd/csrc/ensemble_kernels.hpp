@@ -49,6 +49,7 @@ struct BoxDesc {
   int items2;          // t2[0] * t2[1] * t2[2]
   double dx, dy, dz;
   double tout;         // output time of the box (dt is clamped to reach it)
+  RepairState rep;     // the box's repair marks (see RepairKernel)
 };
 
 // Time state of one box.
@@ -94,7 +95,7 @@ struct EnsembleArgs {
 __device__ __forceinline__ StepArgs BoxStepArgs(const EnsembleArgs &e, int b, int kchunk)
 {
   const BoxDesc &d = e.box[b];
-  StepArgs a;
+  StepArgs a{};
   a.U = d.U, a.W = d.W, a.pdv = d.pdv;
   a.nx = d.nx, a.ny = d.ny, a.nz = d.nz, a.ng = e.ng;
   a.dx = d.dx, a.dy = d.dy, a.dz = d.dz;
@@ -106,6 +107,7 @@ __device__ __forceinline__ StepArgs BoxStepArgs(const EnsembleArgs &e, int b, in
   a.U_floor       = e.U_floor;
   a.flags         = e.flags;
   a.kchunk        = kchunk;
+  a.rep           = d.rep;
   return a;
 }
 

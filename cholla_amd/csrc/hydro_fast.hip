@@ -15,4 +15,9 @@ hipError_t LaunchStage2Fast(int policy, const StepArgs &a, int recon, int tile, 
   if (policy == PolicyMixed::kId) return LaunchStage2<PolicyMixed>(a, recon, tile, s);
   return LaunchStage2<PolicyFast>(a, recon, tile, s);
 }
+hipError_t LaunchRepairFast(int policy, const StepArgs &a, hipStream_t s)
+{
+  if (policy == PolicyMixed::kId) return LaunchRepair<PolicyMixed>(a, s);
+  return LaunchRepair<PolicyFast>(a, s);
+}
 }  // namespace chl

@@ -13,6 +13,10 @@
 //                solve with hardware rcp and sqrt; the conserved state, its
 //                flux-difference accumulation, pressure recovery and dt stay FP64
 //                (handled by the callers, which convert at the boundary).
+//                Reconstruction runs on deviations from the cell-centred state and
+//                the pressure is carried as an FP32 pair (high part + low part), so
+//                the FP32 rounding in a face value scales with the local variation,
+//                not with the size of the state (see SplitPressure below).
 //
 // States are held "rotated": vx / mx is the component normal to the sweep
 // direction, (vy, vz) the two transverse ones in the reference's cyclic order
@@ -76,10 +80,33 @@ using RoePolicy = SolverPolicy<Base, kRiemannRoe>;
 template <class Base>
 using ExactPolicy = SolverPolicy<Base, kRiemannExact>;
 
+// The mixed form's pressure treatment ("split pressure").  A uniform pressure p0 adds
+// exactly p0 to the normal-momentum flux of every face solver here and cancels in a
+// cell's flux difference, so the FP32 solve carries only the pressure relative to a
+// reference: each face takes the high FP32 part of its left cell's FP64 pressure as p0,
+// the solver's momentum flux is returned less p0, and the update adds the FP64
+// difference of the two faces' references back.  The state's FP32 rounding then enters
+// the small momenta through local pressure differences only: an O(p) rounding in the
+// normal-momentum flux of a nearly static direction was the error that dominated.
+template <class P>
+constexpr bool SplitPressure = P::kId == PolicyMixed::kId;
+
 template <typename T>
 struct Prim {
   T d, vx, vy, vz, p;
 };
+// A reconstructed interface value as the stage-2 tiles exchange it.  Under the split
+// pressure p holds the face pressure less the cell's reference ph (the high FP32 part
+// of the cell's pressure) and ph that reference; otherwise it is a plain Prim.
+template <typename T, bool Split>
+struct IPrim : Prim<T> {
+};
+template <typename T>
+struct IPrim<T, true> : Prim<T> {
+  T ph;
+};
+template <class P>
+using IPrimOf = IPrim<typename P::T, SplitPressure<P>>;
 template <typename T>
 struct Cons {
   T d, mx, my, mz, E;
@@ -379,13 +406,17 @@ __device__ __forceinline__ void PpmSingleReduced(T im2, T im1, T i0, T ip1, T ip
 // reference floors each quantity at TINY_NUMBER on its own, which can pair a 1e-20
 // density with a finite pressure (an unbounded sound speed, and an HLLC flux that
 // injects energy on the scale of p/1e-20).  Returns whether the fallback fired.
+//
+// ReconPpmcCore is the projection, limiting and back-projection alone, with the
+// eigenvectors taken from q: the map is linear in the stencil values (and PPM shifts
+// with them), so the mixed form hands it deviations from the centre cell with the
+// centre's absolute state as q, and adds the centre back afterwards.
 template <class P>
-__device__ __forceinline__ bool ReconPpmc(const Prim<typename P::T> s[5], typename P::T gamma,
-                                          Prim<typename P::T> &L_iph, Prim<typename P::T> &R_imh,
-                                          bool fallback = false)
+__device__ __forceinline__ void ReconPpmcCore(const Prim<typename P::T> s[5], const Prim<typename P::T> &q,
+                                              typename P::T gamma, Prim<typename P::T> &L_iph,
+                                              Prim<typename P::T> &R_imh)
 {
   using T = typename P::T;
-  const Prim<T> &q = s[2];
   SoundSpeed<P> cs(q, gamma);
   Char5<T> a[5];
 #pragma unroll
@@ -404,8 +435,18 @@ __device__ __forceinline__ bool ReconPpmc(const Prim<typename P::T> s[5], typena
     PpmSingle<P>(a[0].a3, a[1].a3, a[2].a3, a[3].a3, a[4].a3, L.a3, R.a3);
     PpmSingle<P>(a[0].a4, a[1].a4, a[2].a4, a[3].a4, a[4].a4, L.a4, R.a4);
   }
-  L_iph   = CharToPrim<P>(q, L, cs);
-  R_imh   = CharToPrim<P>(q, R, cs);
+  L_iph = CharToPrim<P>(q, L, cs);
+  R_imh = CharToPrim<P>(q, R, cs);
+}
+
+template <class P>
+__device__ __forceinline__ bool ReconPpmc(const Prim<typename P::T> s[5], typename P::T gamma,
+                                          Prim<typename P::T> &L_iph, Prim<typename P::T> &R_imh,
+                                          bool fallback = false)
+{
+  using T = typename P::T;
+  const Prim<T> &q = s[2];
+  ReconPpmcCore<P>(s, q, gamma, L_iph, R_imh);
   bool fb = false;
   if (fallback) {
     fb = !(L_iph.d > T(0) && R_imh.d > T(0) && L_iph.p > T(0) && R_imh.p > T(0));
@@ -457,15 +498,22 @@ __device__ __forceinline__ void PlmpInterface(typename P::T qm, typename P::T q0
 // :231-284 is compiled out): each primitive limited on its own, then the density and
 // pressure floors.  s[0..2] = cells i-1, i, i+1.
 template <class P>
-__device__ __forceinline__ void ReconPlmp(const Prim<typename P::T> s[3], Prim<typename P::T> &L_iph,
-                                          Prim<typename P::T> &R_imh)
+__device__ __forceinline__ void ReconPlmpCore(const Prim<typename P::T> s[3], Prim<typename P::T> &L_iph,
+                                              Prim<typename P::T> &R_imh)
 {
-  using T = typename P::T;
   PlmpInterface<P>(s[0].d, s[1].d, s[2].d, R_imh.d, L_iph.d);
   PlmpInterface<P>(s[0].vx, s[1].vx, s[2].vx, R_imh.vx, L_iph.vx);
   PlmpInterface<P>(s[0].vy, s[1].vy, s[2].vy, R_imh.vy, L_iph.vy);
   PlmpInterface<P>(s[0].vz, s[1].vz, s[2].vz, R_imh.vz, L_iph.vz);
   PlmpInterface<P>(s[0].p, s[1].p, s[2].p, R_imh.p, L_iph.p);
+}
+
+template <class P>
+__device__ __forceinline__ void ReconPlmp(const Prim<typename P::T> s[3], Prim<typename P::T> &L_iph,
+                                          Prim<typename P::T> &R_imh)
+{
+  using T = typename P::T;
+  ReconPlmpCore<P>(s, L_iph, R_imh);
   R_imh.d = Fmax(R_imh.d, T(kTinyNumber));
   L_iph.d = Fmax(L_iph.d, T(kTinyNumber));
   R_imh.p = Fmax(R_imh.p, T(kTinyNumber));
@@ -503,8 +551,8 @@ __device__ __forceinline__ void PpmpInterfaces(const typename P::T q[5], typenam
 }
 
 template <class P>
-__device__ __forceinline__ void ReconPpmp(const Prim<typename P::T> s[5], Prim<typename P::T> &L_iph,
-                                          Prim<typename P::T> &R_imh)
+__device__ __forceinline__ void ReconPpmpCore(const Prim<typename P::T> s[5], Prim<typename P::T> &L_iph,
+                                              Prim<typename P::T> &R_imh)
 {
   using T = typename P::T;
   T q[5];
@@ -517,6 +565,14 @@ __device__ __forceinline__ void ReconPpmp(const Prim<typename P::T> s[5], Prim<t
   CHL_PPMP_VAR(vz)
   CHL_PPMP_VAR(p)
 #undef CHL_PPMP_VAR
+}
+
+template <class P>
+__device__ __forceinline__ void ReconPpmp(const Prim<typename P::T> s[5], Prim<typename P::T> &L_iph,
+                                          Prim<typename P::T> &R_imh)
+{
+  using T = typename P::T;
+  ReconPpmpCore<P>(s, L_iph, R_imh);
   R_imh.d = Fmax(R_imh.d, T(kTinyNumber));
   L_iph.d = Fmax(L_iph.d, T(kTinyNumber));
   R_imh.p = Fmax(R_imh.p, T(kTinyNumber));
@@ -524,14 +580,17 @@ __device__ __forceinline__ void ReconPpmp(const Prim<typename P::T> s[5], Prim<t
 }
 
 // PLMC_cuda body (VL build: no characteristic tracing), src/reconstruction/plmc_cuda.cu:73-291.
-// s[0..2] = cells i-1, i, i+1.
+// s[0..2] = cells i-1, i, i+1.  ReconPlmcCore leaves out the final floors and takes the
+// eigenvectors from qe (the centre cell's absolute state; s may hold deviations from it,
+// as for ReconPpmcCore: slopes, limiters and interface limits all shift with the values).
 template <class P>
-__device__ __forceinline__ void ReconPlmc(const Prim<typename P::T> s[3], typename P::T gamma,
-                                          Prim<typename P::T> &L_iph, Prim<typename P::T> &R_imh)
+__device__ __forceinline__ void ReconPlmcCore(const Prim<typename P::T> s[3], const Prim<typename P::T> &qe,
+                                              typename P::T gamma, Prim<typename P::T> &L_iph,
+                                              Prim<typename P::T> &R_imh)
 {
   using T = typename P::T;
   const Prim<T> &im = s[0], &q = s[1], &ip = s[2];
-  SoundSpeed<P> cs(q, gamma);
+  SoundSpeed<P> cs(qe, gamma);
   // Compute_Slope (coef 1 and 0.5), reconstruction.h:224-251
   Prim<T> dL{T(1.0) * (q.d - im.d), T(1.0) * (q.vx - im.vx), T(1.0) * (q.vy - im.vy), T(1.0) * (q.vz - im.vz),
              T(1.0) * (q.p - im.p)};
@@ -541,15 +600,15 @@ __device__ __forceinline__ void ReconPlmc(const Prim<typename P::T> s[3], typena
              T(0.5) * (ip.p - im.p)};
   Prim<T> dG{VlSlope(dL.d, dR.d, T(0)), VlSlope(dL.vx, dR.vx, T(0)), VlSlope(dL.vy, dR.vy, T(0)),
              VlSlope(dL.vz, dR.vz, T(0)), VlSlope(dL.p, dR.p, T(0))};
-  Char5<T> aL = PrimToChar<P>(q, dL, cs), aR = PrimToChar<P>(q, dR, cs), aC = PrimToChar<P>(q, dC, cs),
-           aG = PrimToChar<P>(q, dG, cs);
+  Char5<T> aL = PrimToChar<P>(qe, dL, cs), aR = PrimToChar<P>(qe, dR, cs), aC = PrimToChar<P>(qe, dC, cs),
+           aG = PrimToChar<P>(qe, dG, cs);
   Char5<T> m;
   m.a0       = MonotonizeOne(aL.a0, aR.a0, aC.a0, aG.a0);
   m.a1       = MonotonizeOne(aL.a1, aR.a1, aC.a1, aG.a1);
   m.a2       = MonotonizeOne(aL.a2, aR.a2, aC.a2, aG.a2);
   m.a3       = MonotonizeOne(aL.a3, aR.a3, aC.a3, aG.a3);
   m.a4       = MonotonizeOne(aL.a4, aR.a4, aC.a4, aG.a4);
-  Prim<T> dm = CharToPrim<P>(q, m, cs);
+  Prim<T> dm = CharToPrim<P>(qe, m, cs);
   // Calc_Interface_Linear, reconstruction.h:645-673
   const T hp = T(1.0) * T(0.5), hm = T(-1.0) * T(0.5);
   L_iph = {q.d + hp * dm.d, q.vx + hp * dm.vx, q.vy + hp * dm.vy, q.vz + hp * dm.vz, q.p + hp * dm.p};
@@ -566,6 +625,14 @@ __device__ __forceinline__ void ReconPlmc(const Prim<typename P::T> s[3], typena
   lim(L_iph.vy, R_imh.vy, im.vy, q.vy, ip.vy);
   lim(L_iph.vz, R_imh.vz, im.vz, q.vz, ip.vz);
   lim(L_iph.p, R_imh.p, im.p, q.p, ip.p);
+}
+
+template <class P>
+__device__ __forceinline__ void ReconPlmc(const Prim<typename P::T> s[3], typename P::T gamma,
+                                          Prim<typename P::T> &L_iph, Prim<typename P::T> &R_imh)
+{
+  using T = typename P::T;
+  ReconPlmcCore<P>(s, s[1], gamma, L_iph, R_imh);
   R_imh.d = Fmax(R_imh.d, T(kTinyNumber));
   L_iph.d = Fmax(L_iph.d, T(kTinyNumber));
   R_imh.p = Fmax(R_imh.p, T(kTinyNumber));
@@ -613,7 +680,21 @@ struct RState {
   T vx, vy, vz, p;     // primitive
   T inv_d;             // 1/d (non-exact policies only)
   T dge;               // conserved GasEnergy (dual energy only)
+  T pd, p0;            // split pressure only: p less the face reference p0, and p0
 };
+
+// The pressure a solver puts in the normal-momentum flux and in pressure differences:
+// the pressure itself, or under the split pressure its part above the face reference
+// (the solvers then return that momentum flux less p0).
+template <class P>
+__device__ __forceinline__ typename P::T MomP(const RState<typename P::T> &s)
+{
+  if constexpr (SplitPressure<P>) {
+    return s.pd;
+  } else {
+    return s.p;
+  }
+}
 
 // Side state from a conserved interface value, as Calculate_HLLC_Fluxes_CUDA
 // derives it (hllc_cuda.cu:104-131): velocities by division, pressure from the
@@ -719,8 +800,10 @@ __device__ __forceinline__ Cons<typename P::T> Hllc(const RState<typename P::T> 
   Sl = ((Sl < T(0)) ? T(-1) : T(1)) * Fmax(Fabs(Sl), T(0));
   Sr = ((Sr < T(0)) ? T(-1) : T(1)) * Fmax(Fabs(Sr), T(0));
 
-  Cons<T> fl{l.mx, l.mx * l.vx + l.p, l.my * l.vx, l.mz * l.vx, (l.E + l.p) * l.vx};
-  Cons<T> fr{r.mx, r.mx * r.vx + r.p, r.my * r.vx, r.mz * r.vx, (r.E + r.p) * r.vx};
+  // momentum-flux pressures (the pressures themselves outside the split-pressure form)
+  const T pml = MomP<P>(l), pmr = MomP<P>(r);
+  Cons<T> fl{l.mx, l.mx * l.vx + pml, l.my * l.vx, l.mz * l.vx, (l.E + l.p) * l.vx};
+  Cons<T> fr{r.mx, r.mx * r.vx + pmr, r.my * r.vx, r.mz * r.vx, (r.E + r.p) * r.vx};
   T f_ge_l = T(0), f_ge_r = T(0);
   if constexpr (DE) f_ge_l = l.dge * l.vx, f_ge_r = r.dge * r.vx;
   if (Sl > T(0.0)) {
@@ -732,16 +815,17 @@ __device__ __forceinline__ Cons<typename P::T> Hllc(const RState<typename P::T> 
     return fr;
   }
 
-  T Sm = Div1<P>(r.d * r.vx * (Sr - r.vx) - l.d * l.vx * (Sl - l.vx) + l.p - r.p,
+  T Sm = Div1<P>(r.d * r.vx * (Sr - r.vx) - l.d * l.vx * (Sl - l.vx) + pml - pmr,
                  r.d * (Sr - r.vx) - l.d * (Sl - l.vx));
-  T ps      = l.d * (l.vx - Sl) * (l.vx - Sm) + l.p;
+  T psm     = l.d * (l.vx - Sl) * (l.vx - Sm) + pml;  // star pressure (less p0 when split)
+  T ps      = SplitPressure<P> ? psm + l.p0 : psm;
   T inv_lsm = Rcp<P>(Sl - Sm), inv_rsm = Rcp<P>(Sr - Sm);
   T dls     = Div<P>(l.d * (Sl - l.vx), Sl - Sm, inv_lsm);
-  T mxls    = Div<P>(l.mx * (Sl - l.vx) + ps - l.p, Sl - Sm, inv_lsm);
+  T mxls    = Div<P>(l.mx * (Sl - l.vx) + psm - pml, Sl - Sm, inv_lsm);
   T myls = dls * l.vy, mzls = dls * l.vz;
   T Els  = Div<P>(l.E * (Sl - l.vx) - l.p * l.vx + ps * Sm, Sl - Sm, inv_lsm);
   T drs  = Div<P>(r.d * (Sr - r.vx), Sr - Sm, inv_rsm);
-  T mxrs = Div<P>(r.mx * (Sr - r.vx) + ps - r.p, Sr - Sm, inv_rsm);
+  T mxrs = Div<P>(r.mx * (Sr - r.vx) + psm - pmr, Sr - Sm, inv_rsm);
   T myrs = drs * r.vy, mzrs = drs * r.vz;
   T Ers  = Div<P>(r.E * (Sr - r.vx) - r.p * r.vx + ps * Sm, Sr - Sm, inv_rsm);
   T aSm  = Fabs(Sm);
@@ -776,8 +860,8 @@ __device__ __forceinline__ Cons<typename P::T> Hll(const RState<typename P::T> &
   }
   const T Sl = Fmin(r.vx - cfr, l.vx - cfl);
   const T Sr = Fmax(l.vx + cfl, r.vx + cfr);
-  Cons<T> fl{l.mx, l.mx * l.vx + l.p, l.my * l.vx, l.mz * l.vx, (l.E + l.p) * l.vx};
-  Cons<T> fr{r.mx, r.mx * r.vx + r.p, r.my * r.vx, r.mz * r.vx, (r.E + r.p) * r.vx};
+  Cons<T> fl{l.mx, l.mx * l.vx + MomP<P>(l), l.my * l.vx, l.mz * l.vx, (l.E + l.p) * l.vx};
+  Cons<T> fr{r.mx, r.mx * r.vx + MomP<P>(r), r.my * r.vx, r.mz * r.vx, (r.E + r.p) * r.vx};
   T f_ge_l = T(0), f_ge_r = T(0);
   if constexpr (DE) f_ge_l = l.dge * l.vx, f_ge_r = r.dge * r.vx;
   if (Sl > T(0.0)) {
@@ -824,11 +908,19 @@ __device__ __forceinline__ Cons<typename P::T> Roe(const RState<typename P::T> &
   const T asq = g1 * Fmax((H - T(0.5) * vsq), T(kTinyNumber));
   const T a   = Sqrt<P>(asq);
   const T lm = vx - a, l0 = vx, lp = vx + a;
-  Cons<T> fl{l.mx, l.mx * l.vx + l.p, l.mx * l.vy, l.mx * l.vz, (l.E + l.p) * l.vx};
-  Cons<T> fr{r.mx, r.mx * r.vx + r.p, r.mx * r.vy, r.mx * r.vz, (r.E + r.p) * r.vx};
+  const T pml = MomP<P>(l), pmr = MomP<P>(r);
+  Cons<T> fl{l.mx, l.mx * l.vx + pml, l.mx * l.vy, l.mx * l.vz, (l.E + l.p) * l.vx};
+  Cons<T> fr{r.mx, r.mx * r.vx + pmr, r.mx * r.vy, r.mx * r.vz, (r.E + r.p) * r.vx};
   if (lm >= T(0.0)) return fl;
   if (lp <= T(0.0)) return fr;
-  const T dd = r.d - l.d, dmx = r.mx - l.mx, dmy = r.my - l.my, dmz = r.mz - l.mz, dE = r.E - l.E;
+  const T dd = r.d - l.d, dmx = r.mx - l.mx, dmy = r.my - l.my, dmz = r.mz - l.mz;
+  T dE = r.E - l.E;
+  if constexpr (SplitPressure<P>) {
+    // the energy jump from the pressure jump relative to the face reference, so the
+    // O(p) rounding of the two energies does not enter the wave strengths
+    dE = Div1<P>(pmr - pml, g1) + (T(0.5) * r.d * SquareMag(r.vx, r.vy, r.vz) -
+                                   T(0.5) * l.d * SquareMag(l.vx, l.vy, l.vz));
+  }
   const T Na = Div1<P>(T(0.5), asq);
   const T inv_asq = Rcp<P>(asq);
   T s0 = T(0.0), s1 = T(0.0), s2 = T(0.0), s3 = T(0.0), s4 = T(0.0);
@@ -862,7 +954,7 @@ __device__ __forceinline__ Cons<typename P::T> Roe(const RState<typename P::T> &
     const T tmp = Div1<P>(T(0.5) * (bp + bm), bp - bm);
     auto hlle_avg = [&](T hl, T hr) { return T(0.5) * (hl + hr) + (hl - hr) * tmp; };
     f.d  = hlle_avg(l.mx - bm * l.d, r.mx - bp * r.d);
-    f.mx = hlle_avg(l.mx * (l.vx - bm) + l.p, r.mx * (r.vx - bp) + r.p);
+    f.mx = hlle_avg(l.mx * (l.vx - bm) + pml, r.mx * (r.vx - bp) + pmr);
     f.my = hlle_avg(l.my * (l.vx - bm), r.my * (r.vx - bp));
     f.mz = hlle_avg(l.mz * (l.vx - bm), r.mz * (r.vx - bp));
     f.E  = hlle_avg(l.E * (l.vx - bm) + l.p * l.vx, r.E * (r.vx - bp) + r.p * r.vx);
@@ -980,6 +1072,12 @@ __device__ __forceinline__ Cons<typename P::T> ExactFlux(const RState<typename P
   using T            = typename P::T;
   const double gamma = double(gamma_t);
   ExactSideState L{double(l.d), double(l.vx), double(l.p), 0.0}, R{double(r.d), double(r.vx), double(r.p), 0.0};
+  double p0 = 0.0;  // the momentum flux is returned less p0 (split pressure only)
+  if constexpr (SplitPressure<P>) {
+    p0  = double(l.p0);
+    L.p = Fmax(p0 + double(l.pd), kTinyNumber);
+    R.p = Fmax(p0 + double(r.pd), kTinyNumber);
+  }
   L.c = sqrt(gamma * L.p / L.d);
   R.c = sqrt(gamma * R.p / R.d);
   double pm = 0.0, vm = 0.0, ds, vs, ps;
@@ -988,7 +1086,8 @@ __device__ __forceinline__ Cons<typename P::T> ExactFlux(const RState<typename P
   const bool left = vs >= 0;
   const double vy = left ? double(l.vy) : double(r.vy), vz = left ? double(l.vz) : double(r.vz);
   const double Es = (ps / (gamma - 1.0)) + 0.5 * ds * (vs * vs + vy * vy + vz * vz);
-  return Cons<T>{T(ds * vs), T(ds * vs * vs + ps), T(ds * vs * vy), T(ds * vs * vz), T((Es + ps) * vs)};
+  const double fm = SplitPressure<P> ? ds * vs * vs + (ps - p0) : ds * vs * vs + ps;
+  return Cons<T>{T(ds * vs), T(fm), T(ds * vs * vy), T(ds * vs * vz), T((Es + ps) * vs)};
 }
 
 // The solver of the form P (HLLC unless P is a SolverPolicy).

@@ -57,6 +57,26 @@ struct ReconTraits<kPCM> {
   static constexpr int kRadius = 0;
 };
 
+// Cells whose update ends in one of the reference's repairs: the average of the six
+// neighbours' densities for a non-positive density under the density floor
+// (Average_Cell_Single_Field, hydro_cuda.cu:293-297, 1155-1177), and the average of the
+// healthy cells of the 3x3x3 block when the density or energy comes out negative or NaN in
+// a build without both floors (Average_Cell_All_Fields, hydro_cuda.cu:375-381, 1179-1268).
+// The reference runs them inside the update kernel on neighbours other threads are
+// updating at the same time; here stage 2 only marks such a cell and RepairKernel, the
+// next launch, repairs the marks in cell order from the finished update.
+enum : unsigned char { kRepairDensity = 1, kRepairAll = 2 };
+constexpr int kRepairChunkShift = 12;  // marks are found through one byte per 4096 cells
+
+struct RepairState {
+  unsigned char *mark;   // per cell: 0, kRepairDensity or kRepairAll
+  unsigned char *chunk;  // per 4096 cells: 1 if any cell in it is marked
+  unsigned *count;       // [0] marked cells this step, [1] gravity records, [2] records that did not fit
+  double *side;          // fused periodic ghosts: pre-step density of the six boundary layers
+  double *grav;          // static gravity: (cell id, d, mx, my, mz) of marked cells at the step's start
+  int grav_cap;          // records grav holds
+};
+
 // Parameters shared by both stages.
 struct StepArgs {
   double *U;         // conserved state, n_fields x n_cells (in/out)
@@ -78,6 +98,7 @@ struct StepArgs {
   int de;                  // dual energy (host-side dispatch only; the kernels are templated on it)
   int riemann;             // kRiemannHllc / kRiemannHll (host-side dispatch only, as de)
   GravSpec grav;           // static gravity (kernels under a GravPolicy only); grav.id 0 = none
+  RepairState rep;         // marked cells (kFlagDensityFloor / kFlagCrashRepair), see RepairKernel
 };
 
 enum : int {
@@ -87,6 +108,8 @@ enum : int {
   kFlagReduceDt           = 8,
   kFlagPositivityFallback = 16,  // PPMC positivity fallback (see ReconPpmc)
   kFlagConservedW         = 32,  // exact form, PCM: stage 1 stores momenta, not velocities
+  kFlagCrashRepair        = 64,  // a build without both floors: mark negative / NaN cells (kRepairAll)
+  kFlagResetDti           = 128, // stage 1 zeroes the inverse-dt accumulator (a later kernel fills it)
 };
 
 __device__ __forceinline__ size_t Lin(int i, int j, int k, int nx, int ny)
@@ -149,18 +172,37 @@ __device__ __forceinline__ void Unrot(T n, T t1, T t2, T &gx, T &gy, T &gz)
 // plus the velocities (divisions done once per cell) and the direction-free
 // pressure used by the non-exact policies; with dual energy also the conserved
 // GasEnergy.
+// The low FP32 part of the FP64 pressure, carried by the mixed form (the only form
+// with T = float) for its split-pressure faces: p + plo is the FP64 pressure to
+// about 2^-48 relative.
+template <typename T>
+struct PressureLo {
+};
+template <>
+struct PressureLo<float> {
+  float plo;
+};
+
 template <typename T, bool DE = false>
-struct PcmCell {
+struct PcmCell : PressureLo<T> {
   T d, mx, my, mz, E, vx, vy, vz, p;
 };
 template <typename T>
-struct PcmCell<T, true> {
+struct PcmCell<T, true> : PressureLo<T> {
   T d, mx, my, mz, E, vx, vy, vz, p, ge;
 };
 
 // Number of conserved fields a stage carries: 5, or 6 with GasEnergy.
 template <bool DE>
 constexpr int kNumCons = DE ? 6 : 5;
+
+// W (the half-step primitives) of a form: density, three velocities, pressure (energy
+// under PolicyExact), the pressure's low part under the split pressure, then the
+// specific GasEnergy with dual energy.
+template <class P>
+constexpr int kWGasEnergy = SplitPressure<P> ? 6 : 5;
+template <class P, bool DE>
+constexpr int kWFields = kWGasEnergy<P> + (DE ? 1 : 0);
 
 template <class P, int DIR, bool DE = false>
 __device__ __forceinline__ RState<typename P::T> PcmSide(const PcmCell<typename P::T, DE> &c, typename P::T gamma)
@@ -210,6 +252,7 @@ __device__ __forceinline__ PcmCell<typename P::T, DE> MakePcmCell(double d, doub
     }
     c.d = T(d), c.mx = T(mx), c.my = T(my), c.mz = T(mz), c.E = T(E);
     c.vx = T(vx), c.vy = T(vy), c.vz = T(vz), c.p = T(p);
+    if constexpr (SplitPressure<P>) c.plo = float(p - double(c.p));
   }
   if constexpr (DE) c.ge = T(ge);
   return c;
@@ -234,6 +277,23 @@ __device__ __forceinline__ void FaceFlux(const RState<typename P::T> &l, const R
   Cons<T> f = RiemannFlux<P, DE>(l, r, gamma, &fge);
   FluxGlobal<P, DIR>(f, out);
   if constexpr (DE) out[kGasEnergy] = fge;
+}
+
+// The flux through the face between stage-1 cells a (left) and b (right).  Under the
+// split pressure the face reference is a.p (the high part of a's pressure): the left
+// state's momentum pressure is a.plo and the right state's (b.p - a.p) + b.plo, and
+// the normal-momentum flux comes back less a.p (the update adds the references back).
+template <class P, int DIR, bool DE>
+__device__ __forceinline__ void PcmFaceFlux(const PcmCell<typename P::T, DE> &a, const PcmCell<typename P::T, DE> &b,
+                                            typename P::T gamma, typename P::T *out)
+{
+  RState<typename P::T> l = PcmSide<P, DIR, DE>(a, gamma), r = PcmSide<P, DIR, DE>(b, gamma);
+  if constexpr (SplitPressure<P>) {
+    l.p0 = r.p0 = a.p;
+    l.pd        = a.plo;
+    r.pd        = (b.p - a.p) + b.plo;
+  }
+  FaceFlux<P, DIR, DE>(l, r, gamma, out);
 }
 
 // Dual energy: the DE pressure of a cell and its centred velocity-difference source
@@ -317,7 +377,7 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
   PcmCell<T, DE> c_mid = load_cell(ic, jc, k0);      // cell k
   double u_mid[NC];
   T fz_lo[NC];
-  FaceFlux<P, 2, DE>(PcmSide<P, 2, DE>(c_lo, gamma), PcmSide<P, 2, DE>(c_mid, gamma), gamma, fz_lo);
+  PcmFaceFlux<P, 2, DE>(c_lo, c_mid, gamma, fz_lo);
 
   for (int k = k0; k < k1; k++) {
     {
@@ -347,21 +407,19 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
     for (int it = tid; it < NFX; it += NT) {
       int fj = it / (TX + 1), fi = it - fj * (TX + 1);
       T g[NC];
-      FaceFlux<P, 0, DE>(PcmSide<P, 0, DE>(plane[fj + 1][fi], gamma), PcmSide<P, 0, DE>(plane[fj + 1][fi + 1], gamma),
-                         gamma, g);
+      PcmFaceFlux<P, 0, DE>(plane[fj + 1][fi], plane[fj + 1][fi + 1], gamma, g);
 #pragma unroll
       for (int q = 0; q < NC; q++) fx[q][fj][fi] = g[q];
     }
     for (int it = tid; it < NFY; it += NT) {
       int fj = it / TX, fi = it - fj * TX;
       T g[NC];
-      FaceFlux<P, 1, DE>(PcmSide<P, 1, DE>(plane[fj][fi + 1], gamma), PcmSide<P, 1, DE>(plane[fj + 1][fi + 1], gamma),
-                         gamma, g);
+      PcmFaceFlux<P, 1, DE>(plane[fj][fi + 1], plane[fj + 1][fi + 1], gamma, g);
 #pragma unroll
       for (int q = 0; q < NC; q++) fy[q][fj][fi] = g[q];
     }
     T fz_hi[NC];
-    FaceFlux<P, 2, DE>(PcmSide<P, 2, DE>(c_mid, gamma), PcmSide<P, 2, DE>(c_hi, gamma), gamma, fz_hi);
+    PcmFaceFlux<P, 2, DE>(c_mid, c_hi, gamma, fz_hi);
     __syncthreads();
     if (own) {
       double uh[NC];
@@ -371,6 +429,12 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
         uh[q] = u_mid[q] + dtodx * (double(fx[q][ty][tx]) - double(fx[q][ty][tx + 1])) +
                 dtody * (double(fy[q][ty][tx]) - double(fy[q][ty + 1][tx])) +
                 dtodz * (double(fz_lo[q]) - double(fz_hi[q]));
+      }
+      if constexpr (SplitPressure<P>) {
+        // the face references (left cells' pressures) the normal-momentum fluxes left out
+        uh[1] += dtodx * (double(plane[ty + 1][tx].p) - double(plane[ty + 1][tx + 1].p));
+        uh[2] += dtody * (double(plane[ty][tx + 1].p) - double(plane[ty + 1][tx + 1].p));
+        uh[3] += dtodz * (double(c_lo.p) - double(c_mid.p));
       }
       if constexpr (DE) {
         // velocity differences of the neighbours at the start of the step, each a
@@ -401,6 +465,11 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
         for (int q = 1; q < NC; q++) uh[q] *= (a.density_floor / d0);
       }
       size_t id = Lin(i, j, k, nx, ny);
+      // the pressure's low part (split pressure): p - float(p), in W's field 5
+      auto store_p = [&](double p) {
+        W[4 * nc + id] = T(p);
+        if constexpr (SplitPressure<P>) W[5 * nc + id] = T(p - double(T(p)));
+      };
       if constexpr (DE) {
         // Load_Data under DE (reconstruction.h:179-191): the exact form keeps E and
         // GasEnergy and recovers p and ge per sweep; the others store p and ge
@@ -418,8 +487,8 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
           W[nc + id]     = T(vx);
           W[2 * nc + id] = T(vy);
           W[3 * nc + id] = T(vz);
-          W[4 * nc + id] = T(PressureFromDE(uh[4], uh[4] - E_kin, uh[5], a.gamma));
-          W[5 * nc + id] = T(uh[5] / uh[0]);
+          store_p(PressureFromDE(uh[4], uh[4] - E_kin, uh[5], a.gamma));
+          W[kWGasEnergy<P> * nc + id] = T(uh[5] / uh[0]);
         }
       } else {
         Prim<double> w = LoadPrim(uh[0], uh[1], uh[2], uh[3], uh[4], a.gamma);
@@ -429,7 +498,11 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
         W[nc + id]     = cons_w ? T(uh[1]) : T(w.vx);
         W[2 * nc + id] = cons_w ? T(uh[2]) : T(w.vy);
         W[3 * nc + id] = cons_w ? T(uh[3]) : T(w.vz);
-        W[4 * nc + id] = P::kExact ? T(uh[4]) : T(w.p);  // see WCell
+        if constexpr (P::kExact) {
+          W[4 * nc + id] = T(uh[4]);  // see WCell
+        } else {
+          store_p(w.p);
+        }
       }
     }
 #pragma unroll
@@ -444,7 +517,7 @@ __global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
 {
   __shared__ Stage1Smem<P, TX, TY, DE> sm;
   // the inverse-dt accumulator is reset here; stage 2 (next launch) fills it
-  if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && threadIdx.x == 0) *a.dti = 0ull;
+  if ((a.flags & (kFlagReduceDt | kFlagResetDti)) && blockIdx.x == 0 && threadIdx.x == 0) *a.dti = 0ull;
   int bx, by, bz;
   if (!TileOf(a, bx, by, bz)) return;
   Stage1Tile<P, TX, TY, DE>(a, R1, bx, by, bz, sm);
@@ -458,19 +531,78 @@ __global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
 // direction and is recomputed per sweep.  With dual energy the sixth slot holds the
 // specific GasEnergy, or under PolicyExact the conserved GasEnergy (Load_Data
 // divides it by the density, reconstruction.h:191).
+// The mixed form also stores the pressure's low part (PressureLo) as field 5, which
+// moves the GasEnergy slot to field 6.
 template <typename T, bool DE = false>
-struct WCell {
+struct WCell : PressureLo<T> {
   T d, vx, vy, vz, p_or_E;
 };
 template <typename T>
-struct WCell<T, true> {
+struct WCell<T, true> : PressureLo<T> {
   T d, vx, vy, vz, p_or_E, g;
 };
 
+// The split-pressure form of ReconDir: the stencil as deviations from its centre cell c
+// (the pressure's from the FP32 pairs, so they resolve differences far below the
+// pressure's own FP32 rounding), the reconstruction's linear core on those with c's
+// eigenvectors, then c added back.  The interface pressures stay relative to c's high
+// part, which each interface carries as ph; the pressure floor is applied where the
+// face states are formed (FaceStates).
 template <class P, int RK, int DIR, bool DE = false>
-__device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, typename P::T gamma, bool pos_fallback,
-                                         Prim<typename P::T> &L, Prim<typename P::T> &R, typename P::T *geL = nullptr,
-                                         typename P::T *geR = nullptr)
+__device__ __forceinline__ void ReconDirSplit(const WCell<float, DE> *s, float gamma, bool pos_fallback,
+                                              IPrim<float, true> &L, IPrim<float, true> &R, float *geL, float *geR)
+{
+  using T          = float;
+  constexpr int r  = ReconTraits<RK>::kRadius;
+  constexpr int NS = 2 * r + 1;
+  const WCell<T, DE> &c = s[r];
+  Prim<T> qc;  // the centre's absolute state (rotated)
+  qc.d = c.d;
+  Rot<DIR>(c.vx, c.vy, c.vz, qc.vx, qc.vy, qc.vz);
+  qc.p = c.p_or_E;
+  Prim<T> st[NS];
+  T ge[NS];
+#pragma unroll
+  for (int m = 0; m < NS; m++) {
+    st[m].d = s[m].d - c.d;
+    Rot<DIR>(s[m].vx - c.vx, s[m].vy - c.vy, s[m].vz - c.vz, st[m].vx, st[m].vy, st[m].vz);
+    st[m].p = (s[m].p_or_E - c.p_or_E) + (s[m].plo - c.plo);
+    if constexpr (DE) ge[m] = s[m].g;
+  }
+  Prim<T> dl{}, dr{};
+  if constexpr (RK == kPLMP) {
+    ReconPlmpCore<P>(st, dl, dr);
+  } else if constexpr (RK == kPPMP) {
+    ReconPpmpCore<P>(st, dl, dr);
+  } else if constexpr (RK == kPPMC) {
+    ReconPpmcCore<P>(st, qc, gamma, dl, dr);
+  } else if constexpr (RK == kPLMC) {
+    ReconPlmcCore<P>(st, qc, gamma, dl, dr);
+  }
+  if constexpr (DE) {
+    if constexpr (RK == kPPMC) {
+      ReconPpmcScalar<P>(ge, *geL, *geR);
+    } else if constexpr (RK == kPLMC) {
+      ReconPlmcScalar<P>(ge, *geL, *geR);
+    }
+  }
+  // PPMC's positivity fallback (see ReconPpmc): the cell-centred values on both sides
+  if (RK == kPPMC && pos_fallback &&
+      !(qc.d + dl.d > T(0) && qc.d + dr.d > T(0) && qc.p + dl.p > T(0) && qc.p + dr.p > T(0))) {
+    dl = dr = Prim<T>{};
+    if constexpr (DE) *geL = *geR = ge[r];
+  }
+  L.d = Fmax(qc.d + dl.d, T(kTinyNumber)), R.d = Fmax(qc.d + dr.d, T(kTinyNumber));
+  L.vx = qc.vx + dl.vx, L.vy = qc.vy + dl.vy, L.vz = qc.vz + dl.vz;
+  R.vx = qc.vx + dr.vx, R.vy = qc.vy + dr.vy, R.vz = qc.vz + dr.vz;
+  L.p = dl.p + c.plo, R.p = dr.p + c.plo;
+  L.ph = R.ph = c.p_or_E;
+}
+
+template <class P, int RK, int DIR, bool DE = false>
+__device__ __forceinline__ void ReconDirPlain(const WCell<typename P::T, DE> *s, typename P::T gamma,
+                                              bool pos_fallback, Prim<typename P::T> &L, Prim<typename P::T> &R,
+                                              typename P::T *geL, typename P::T *geR)
 {
   using T          = typename P::T;
   constexpr int NS = 2 * ReconTraits<RK>::kRadius + 1;
@@ -517,6 +649,20 @@ __device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, type
   }
 }
 
+// The interface values of the cell at the centre of the stencil s[0..2r] (its left
+// value R at i-1/2 and right value L at i+1/2), rotated into the sweep frame of DIR.
+template <class P, int RK, int DIR, bool DE = false>
+__device__ __forceinline__ void ReconDir(const WCell<typename P::T, DE> *s, typename P::T gamma, bool pos_fallback,
+                                         IPrimOf<P> &L, IPrimOf<P> &R, typename P::T *geL = nullptr,
+                                         typename P::T *geR = nullptr)
+{
+  if constexpr (SplitPressure<P>) {
+    ReconDirSplit<P, RK, DIR, DE>(s, gamma, pos_fallback, L, R, geL, geR);
+  } else {
+    ReconDirPlain<P, RK, DIR, DE>(s, gamma, pos_fallback, L, R, geL, geR);
+  }
+}
+
 // The Riemann input of one interface value: the conserved variables the reference's
 // reconstruction writes to dev_bounds, as the Riemann solver reads them back.  PCM and
 // PLMP form them in their own order under the exact form (pcm_cuda.cu copies the state;
@@ -537,6 +683,47 @@ __device__ __forceinline__ RState<typename P::T> IfaceState(const Prim<typename 
   }
 }
 
+// The two Riemann inputs of one face from its left and right interface values.  Under
+// the split pressure the face reference p0 is the left cell's high pressure part; both
+// sides' momentum pressures are expressed relative to it (held at >= -p0, so p0 + pd is
+// never negative) and the absolute pressure, floored at TINY_NUMBER, feeds the rest.
+template <class P, int RK, bool DE = false>
+__device__ __forceinline__ void FaceStates(const IPrimOf<P> &L, const IPrimOf<P> &R, typename P::T gamma,
+                                           typename P::T inv_gm1, typename P::T geL, typename P::T geR,
+                                           RState<typename P::T> &l, RState<typename P::T> &r)
+{
+  using T = typename P::T;
+  if constexpr (SplitPressure<P>) {
+    const T p0 = L.ph;
+    Prim<T> a  = L, b = R;
+    a.p        = Fmax(L.ph + L.p, T(kTinyNumber));
+    b.p        = Fmax(R.ph + R.p, T(kTinyNumber));
+    l          = RStateFromPrim<P, DE>(a, gamma, inv_gm1, geL);
+    r          = RStateFromPrim<P, DE>(b, gamma, inv_gm1, geR);
+    l.p0 = r.p0 = p0;
+    l.pd        = Fmax(L.p, -p0);
+    r.pd        = Fmax(R.p + (R.ph - p0), -p0);
+    if constexpr (DE) {  // the GasEnergy pressure replaced the interface pressure
+      if (l.p != a.p) l.pd = l.p - p0;
+      if (r.p != b.p) r.pd = r.p - p0;
+    }
+  } else {
+    l = IfaceState<P, RK, DE>(L, gamma, inv_gm1, geL);
+    r = IfaceState<P, RK, DE>(R, gamma, inv_gm1, geR);
+  }
+}
+
+// FaceFlux of the face between two reconstructed interface values.
+template <class P, int RK, int DIR, bool DE>
+__device__ __forceinline__ void IfaceFlux(const IPrimOf<P> &L, const IPrimOf<P> &R, typename P::T gamma,
+                                          typename P::T inv_gm1, typename P::T geL, typename P::T geR,
+                                          typename P::T *out)
+{
+  RState<typename P::T> l, r;
+  FaceStates<P, RK, DE>(L, R, gamma, inv_gm1, geL, geR, l, r);
+  FaceFlux<P, DIR, DE>(l, r, gamma, out);
+}
+
 // LDS footprint of stage 2 (bytes) for a given tile; used by the host to pick tiles.
 template <class P, int RK, int TX, int TY>
 struct Stage2Shape {
@@ -554,7 +741,7 @@ struct Stage2Shape {
 // FP64 divide/sqrt chains and LDS round trips without latency cover.
 template <class P>
 struct Stage2Occupancy {
-  static constexpr int kMinWaves = P::kId == PolicyMixed::kId ? 1 : 2;
+  static constexpr int kMinWaves = P::kId == PolicyMixed::kId ? 3 : 2;
 };
 
 // Interface values of the passive GasEnergy (dual energy only).
@@ -576,12 +763,50 @@ struct Stage2Smem {
   using T = typename P::T;
   using S = Stage2Shape<P, RK, TX, TY>;
   WCell<T, DE> plane[S::PY][S::PX];
-  Prim<T> qL[S::NQ];
-  Prim<T> qR[S::NQ];
+  IPrimOf<P> qL[S::NQ];
+  IPrimOf<P> qR[S::NQ];
   GeIfaces<T, S::NQ, DE> qg;
   T fl[kNumCons<DE>][S::NF];
   double red[TX * TY / 64];
 };
+
+// ---------------------------------------------------------------------------- repair marks
+// The crash test of hydro_cuda.cu:375-376: negative or NaN density or energy.
+__device__ __forceinline__ bool CrashedCell(double d, double E) { return d < 0.0 || d != d || E < 0.0 || E != E; }
+
+__device__ __forceinline__ void MarkRepair(const RepairState &r, size_t id, int kind)
+{
+  if (!r.mark) return;  // a launch without repair scratch leaves the cell as the update left it
+  r.mark[id]                          = (unsigned char)kind;
+  r.chunk[id >> kRepairChunkShift]    = 1;
+  atomicAdd(r.count, 1u);
+}
+
+// Static gravity adds its source after the repair (hydro_cuda.cu:301-315) from the cell's
+// state at the start of the step, which the marked cell leaves here.
+__device__ __forceinline__ void RecordRepairGrav(const RepairState &r, size_t id, const double s[4])
+{
+  if (!r.mark) return;
+  unsigned slot = atomicAdd(r.count + 1, 1u);
+  if (int(slot) >= r.grav_cap) {
+    atomicAdd(r.count + 2, 1u);
+    return;
+  }
+  double *g = r.grav + 5 * size_t(slot);
+  g[0] = double(id), g[1] = s[0], g[2] = s[1], g[3] = s[2], g[4] = s[3];
+}
+
+// Side buffer of the fused periodic fill: face f (x-lo, x-hi, y-lo, y-hi, z-lo, z-hi) holds
+// the densities of the ghost layer next to the real cells on that face as they were at the
+// start of the step, before stage 2 overwrote the ghosts with the new periodic images
+// (RepairSideKernel copies them).  Lx, Ly, Lz: real cells per axis.
+__device__ __forceinline__ size_t RepairSideIndex(int face, int a, int b, int Lx, int Ly, int Lz)
+{
+  const size_t fx = size_t(Ly) * Lz, fy = size_t(Lx) * Lz, fz = size_t(Lx) * Ly;
+  const size_t base[6] = {0, fx, 2 * fx, 2 * fx + fy, 2 * fx + 2 * fy, 2 * fx + 2 * fy + fz};
+  const size_t row     = face < 2 ? size_t(Ly) : size_t(Lx);
+  return base[face] + size_t(b) * row + a;  // (a, b) = (j, k) on x faces, (i, k) on y, (i, j) on z
+}
 
 // Every periodic image of real cell (i, j, k) inside the ghost layers gets the
 // cell's values (the host enables this only when every axis holds >= ng real cells,
@@ -661,7 +886,8 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
     size_t id = Lin(ii, jj, kk, nx, ny);
     WCell<T, DE> c;
     c.d = W[id], c.vx = W[nc + id], c.vy = W[2 * nc + id], c.vz = W[3 * nc + id], c.p_or_E = W[4 * nc + id];
-    if constexpr (DE) c.g = W[5 * nc + id];
+    if constexpr (SplitPressure<P>) c.plo = W[5 * nc + id];
+    if constexpr (DE) c.g = W[kWGasEnergy<P> * nc + id];
     return c;
   };
 
@@ -670,18 +896,20 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
   WCell<T, DE> win[NS];
 #pragma unroll
   for (int m = 0; m < NS; m++) win[m] = loadW(ic, jc, k0 - 1 - r + m);
-  Prim<T> qLz, qRz;
+  IPrimOf<P> qLz, qRz;
   T gLz = T(0), gRz = T(0);
   ReconDir<P, RK, 2, DE>(win, gamma, pf, qLz, qRz, &gLz, &gRz);  // cell k0-1
-  Prim<T> qL_lo = qLz;                                       // left state of face k0-1/2
-  T gL_lo       = gLz;
+  IPrimOf<P> qL_lo = qLz;                                    // left state of face k0-1/2
+  T gL_lo          = gLz;
 #pragma unroll
   for (int m = 0; m < NS - 1; m++) win[m] = win[m + 1];
   win[NS - 1] = loadW(ic, jc, k0 + r);
   ReconDir<P, RK, 2, DE>(win, gamma, pf, qLz, qRz, &gLz, &gRz);  // cell k0
   T fz_lo[NC];                                               // flux through face k-1/2 of the current cell
-  FaceFlux<P, 2, DE>(IfaceState<P, RK, DE>(qL_lo, gamma, inv_gm1, gL_lo), IfaceState<P, RK, DE>(qRz, gamma, inv_gm1, gRz),
-                     gamma, fz_lo);
+  IfaceFlux<P, RK, 2, DE>(qL_lo, qRz, gamma, inv_gm1, gL_lo, gRz, fz_lo);
+  // split pressure: the reference pressures of faces k-1/2 and k+1/2 (their left cells')
+  T zref_lo = T(0), zref_hi = T(0);
+  if constexpr (SplitPressure<P>) zref_lo = qL_lo.ph;
   qL_lo = qLz;  // left state of face k0+1/2
   gL_lo = gLz;
   double dti_max = 0.0;
@@ -712,8 +940,8 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
     win[NS - 1] = loadW(ic, jc, k + 1 + r);
     ReconDir<P, RK, 2, DE>(win, gamma, pf, qLz, qRz, &gLz, &gRz);
     T fz_hi[NC];
-    FaceFlux<P, 2, DE>(IfaceState<P, RK, DE>(qL_lo, gamma, inv_gm1, gL_lo),
-                       IfaceState<P, RK, DE>(qRz, gamma, inv_gm1, gRz), gamma, fz_hi);
+    IfaceFlux<P, RK, 2, DE>(qL_lo, qRz, gamma, inv_gm1, gL_lo, gRz, fz_hi);
+    if constexpr (SplitPressure<P>) zref_hi = qL_lo.ph;
     qL_lo = qLz;
     gL_lo = gLz;
     __syncthreads();
@@ -735,11 +963,9 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
       int ql = fj * (TX + 2) + fi;
       T g[NC];
       if constexpr (DE) {
-        FaceFlux<P, 0, DE>(IfaceState<P, RK, DE>(qL[ql], gamma, inv_gm1, qgL[ql]),
-                           IfaceState<P, RK, DE>(qR[ql + 1], gamma, inv_gm1, qgR[ql + 1]), gamma, g);
+        IfaceFlux<P, RK, 0, DE>(qL[ql], qR[ql + 1], gamma, inv_gm1, qgL[ql], qgR[ql + 1], g);
       } else {
-        FaceFlux<P, 0, DE>(IfaceState<P, RK>(qL[ql], gamma, inv_gm1), IfaceState<P, RK>(qR[ql + 1], gamma, inv_gm1),
-                           gamma, g);
+        IfaceFlux<P, RK, 0, DE>(qL[ql], qR[ql + 1], gamma, inv_gm1, T(0), T(0), g);
       }
 #pragma unroll
       for (int q = 0; q < NC; q++) fl[q][it] = g[q];
@@ -750,6 +976,13 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
       int fa = ty * (TX + 1) + tx;
 #pragma unroll
       for (int q = 0; q < NC; q++) acc[q] = dtodx * (double(fl[q][fa]) - double(fl[q][fa + 1]));
+      if constexpr (SplitPressure<P>) {
+        // the x and y faces' references (cells i-1, i and j-1, j), read here: the plane
+        // is rewritten after the last barrier of this iteration
+        const double pc = double(plane[ty + H][tx + H].p_or_E);
+        acc[1] += dtodx * (double(plane[ty + H][tx + H - 1].p_or_E) - pc);
+        acc[2] += dtody * (double(plane[ty + H - 1][tx + H].p_or_E) - pc);
+      }
     }
     // y reconstruction of cells -1..TY of every column
     for (int it = tid; it < S::NQY; it += NT) {
@@ -769,11 +1002,9 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
       const int lq = fj * TX + fi, rq = (fj + 1) * TX + fi;
       T g[NC];
       if constexpr (DE) {
-        FaceFlux<P, 1, DE>(IfaceState<P, RK, DE>(qL[lq], gamma, inv_gm1, qgL[lq]),
-                           IfaceState<P, RK, DE>(qR[rq], gamma, inv_gm1, qgR[rq]), gamma, g);
+        IfaceFlux<P, RK, 1, DE>(qL[lq], qR[rq], gamma, inv_gm1, qgL[lq], qgR[rq], g);
       } else {
-        FaceFlux<P, 1, DE>(IfaceState<P, RK>(qL[lq], gamma, inv_gm1), IfaceState<P, RK>(qR[rq], gamma, inv_gm1), gamma,
-                           g);
+        IfaceFlux<P, RK, 1, DE>(qL[lq], qR[rq], gamma, inv_gm1, T(0), T(0), g);
       }
 #pragma unroll
       for (int q = 0; q < NC; q++) fl[q][it] = g[q];
@@ -787,6 +1018,7 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
         int fa = ty * TX + tx;
         // U += ((dtodx dFx + dtody dFy) + dtodz dFz), hydro_cuda.cu:241-257; GasEnergy
         // first takes the pdV source (the separate Partial_Update kernel, VL_3D_cuda.cu:314-324)
+        if constexpr (SplitPressure<P>) acc[3] += dtodz * (double(zref_lo) - double(zref_hi));  // z faces' references
 #pragma unroll
         for (int q = 0; q < NC; q++) {
           double axy = acc[q] + dtody * (double(fl[q][fa]) - double(fl[q][fa + TX]));
@@ -798,6 +1030,7 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
           un[q] = u0 + (axy + dtodz * (double(fz_lo[q]) - double(fz_hi[q])));
         }
       }
+      int mark = 0;
       if ((a.flags & kFlagDensityFloor) && un[0] < a.density_floor) {  // hydro_cuda.cu:279-299
         if (un[0] > 0) {
           double d0 = un[0];
@@ -805,21 +1038,26 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
 #pragma unroll
           for (int q = 1; q < NC; q++) un[q] *= (a.density_floor / d0);
         } else {
-          // Average_Cell_Single_Field (hydro_cuda.cu:1155-1177) over the six
-          // neighbours as they are in memory now; the reference reads them
-          // while other threads update them as well.
-          un[0] = (U[id - 1] + U[id + 1] + U[id - nx] + U[id + nx] + U[id - size_t(nx) * ny] +
-                   U[id + size_t(nx) * ny]) /
-                  6;
+          mark = kRepairDensity;  // Average_Cell_Single_Field, done by RepairKernel
         }
       }
-      if constexpr (P::kGrav) {  // STATIC_GRAV, hydro_cuda.cu:301-315
-        double g[3];
-        GravField3D(a.grav.id, a.grav.Pos(0, i, ng, a.dx), a.grav.Pos(1, j, ng, a.dy), a.grav.Pos(2, k, ng, a.dz), g[0],
-                    g[1], g[2]);
-        GravSource<3>(un, ustart, g, dt);
+      if constexpr (P::kGrav) {  // STATIC_GRAV, hydro_cuda.cu:301-315 (after the repair)
+        if (mark == 0) {
+          double g[3];
+          GravField3D(a.grav.id, a.grav.Pos(0, i, ng, a.dx), a.grav.Pos(1, j, ng, a.dy), a.grav.Pos(2, k, ng, a.dz),
+                      g[0], g[1], g[2]);
+          GravSource<3>(un, ustart, g, dt);
+        } else {
+          RecordRepairGrav(a.rep, id, ustart);
+        }
       }
-      if constexpr (DE) {
+      if (mark == 0 && (a.flags & kFlagCrashRepair) && CrashedCell(un[0], un[4])) mark = kRepairAll;
+      if (mark) {
+        // the update as it stands; the repair, floors, dt and images follow in RepairKernel
+#pragma unroll
+        for (int q = 0; q < NC; q++) U[q * nc + id] = un[q];
+        MarkRepair(a.rep, id, mark);
+      } else if constexpr (DE) {
 #pragma unroll
         for (int q = 0; q < NC; q++) U[q * nc + id] = un[q];
       } else {
@@ -841,6 +1079,7 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
     }
 #pragma unroll
     for (int q = 0; q < NC; q++) fz_lo[q] = fz_hi[q];
+    zref_lo = zref_hi;
   }
 
   if (!DE && (a.flags & kFlagReduceDt)) {
@@ -865,6 +1104,217 @@ Stage2Kernel(StepArgs a)
   int bx, by, bz;
   if (!TileOf(a, bx, by, bz)) return;
   Stage2Tile<P, RK, TX, TY, DE>(a, bx, by, bz, sm);
+}
+
+// ============================================================================ repairs
+constexpr int kRepairThreads = 1024;  // one workgroup; 4 cells per lane cover a mark chunk
+
+// Exclusive prefix sum of v over the workgroup (wave64 scan, then the wave totals).
+__device__ __forceinline__ int RepairBlockScan(int v, int *s_wave, int &total)
+{
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) s_wave[w] = x;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int q = 0; q < kRepairThreads / 64; q++) {
+      int t    = s_wave[q];
+      s_wave[q] = acc;
+      acc += t;
+    }
+    s_wave[kRepairThreads / 64] = acc;
+  }
+  __syncthreads();
+  total = s_wave[kRepairThreads / 64];
+  return s_wave[w] + x - v;
+}
+
+// Density of the neighbour (i+di, j+dj, k+dk) of real cell (i, j, k) for the density
+// average: the state in memory, except that a ghost neighbour of a box whose periodic images
+// stage 2 wrote takes the pre-step density its image held (the side buffer).
+__device__ __forceinline__ double RepairNeighbourDensity(const StepArgs &a, int i, int j, int k, int di, int dj, int dk)
+{
+  const int ng = a.ng, Lx = a.nx - 2 * ng, Ly = a.ny - 2 * ng, Lz = a.nz - 2 * ng;
+  const int ii = i + di, jj = j + dj, kk = k + dk;
+  if (a.rep.side) {
+    const int ri = i - ng, rj = j - ng, rk = k - ng;
+    if (ii < ng) return a.rep.side[RepairSideIndex(0, rj, rk, Lx, Ly, Lz)];
+    if (ii >= a.nx - ng) return a.rep.side[RepairSideIndex(1, rj, rk, Lx, Ly, Lz)];
+    if (jj < ng) return a.rep.side[RepairSideIndex(2, ri, rk, Lx, Ly, Lz)];
+    if (jj >= a.ny - ng) return a.rep.side[RepairSideIndex(3, ri, rk, Lx, Ly, Lz)];
+    if (kk < ng) return a.rep.side[RepairSideIndex(4, ri, rj, Lx, Ly, Lz)];
+    if (kk >= a.nz - ng) return a.rep.side[RepairSideIndex(5, ri, rj, Lx, Ly, Lz)];
+  }
+  return a.U[Lin(ii, jj, kk, a.nx, a.ny)];
+}
+
+// Average_Cell_All_Fields, hydro_cuda.cu:1179-1268: the mean density, momentum-weighted
+// velocity and pressure of the cells of the 3x3x3 block with positive density and
+// pressure (the cell itself included), in the reference's summation order; with dual
+// energy GasEnergy takes the mean internal energy.  (Its printf diagnostics are left out.)
+template <bool DE>
+__device__ void RepairAllFields(const StepArgs &a, int i, int j, int k)
+{
+  const size_t nc = size_t(a.nx) * a.ny * a.nz, id = Lin(i, j, k, a.nx, a.ny);
+  const double gamma = a.gamma;
+  double d_av = 0.0, vx_av = 0.0, vy_av = 0.0, vz_av = 0.0, P_av = 0.0;
+  int N = 0;
+  for (int kk = k - 1; kk <= k + 1; kk++)
+    for (int jj = j - 1; jj <= j + 1; jj++)
+      for (int ii = i - 1; ii <= i + 1; ii++) {
+        const size_t n = Lin(ii, jj, kk, a.nx, a.ny);
+        const double d = a.U[n], mx = a.U[nc + n], my = a.U[2 * nc + n], mz = a.U[3 * nc + n];
+        const double P = (a.U[4 * nc + n] - (0.5 / d) * (mx * mx + my * my + mz * mz)) * (gamma - 1.0);
+        if (d > 0.0 && P > 0.0) {
+          d_av += d;
+          vx_av += mx;
+          vy_av += my;
+          vz_av += mz;
+          P_av += P / (gamma - 1.0);
+          N++;
+        }
+      }
+  P_av  = P_av / N;
+  vx_av = vx_av / d_av;
+  vy_av = vy_av / d_av;
+  vz_av = vz_av / d_av;
+  d_av  = d_av / N;
+  a.U[id]          = d_av;
+  a.U[nc + id]     = d_av * vx_av;
+  a.U[2 * nc + id] = d_av * vy_av;
+  a.U[3 * nc + id] = d_av * vz_av;
+  a.U[4 * nc + id] = P_av / (gamma - 1.0) + 0.5 * d_av * (vx_av * vx_av + vy_av * vy_av + vz_av * vz_av);
+  if constexpr (DE) a.U[kGasEnergy * nc + id] = P_av / (gamma - 1.0);
+}
+
+// The repairs stage 2 marked, from the finished update, in cell order (the order the
+// oracle's serial loops use), by one workgroup:
+//   pass 0  density marks: the six-neighbour density average (Average_Cell_Single_Field,
+//           hydro_cuda.cu:1155-1177), static gravity's source, then the crash test, which
+//           turns the mark into an all-field mark when it fails;
+//   pass 1  all-field marks: Average_Cell_All_Fields;
+//   pass 2  what stage 2 fuses into its epilogue for unmarked cells (temperature floor,
+//           inverse-dt reduction, periodic images; dual energy leaves them to its own
+//           kernels), and the marks are cleared.
+// A step without marks returns after reading the count.
+template <class P, bool DE>
+__global__ void __launch_bounds__(kRepairThreads) RepairKernel(StepArgs a)
+{
+  constexpr int NT = kRepairThreads, NC = kNumCons<DE>, CH = 1 << kRepairChunkShift;
+  static_assert(CH == 4 * NT, "one mark chunk is four cells per lane");
+  __shared__ unsigned s_count;
+  __shared__ int s_wave[NT / 64 + 1];
+  __shared__ unsigned s_chunks[NT];
+  __shared__ unsigned s_cells[CH];
+  const int tid = threadIdx.x;
+  if (tid == 0) s_count = a.rep.count[0];
+  __syncthreads();
+  if (s_count == 0) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, ng = a.ng;
+  const size_t nc = size_t(nx) * ny * nz, n_chunks = (nc + CH - 1) / CH;
+  const double dt = StepDt(a);
+  for (int pass = 0; pass < 3; pass++) {
+    for (size_t c0 = 0; c0 < n_chunks; c0 += NT) {
+      const size_t c = c0 + tid;
+      const int has  = (c < n_chunks && a.rep.chunk[c]) ? 1 : 0;
+      int n_has;
+      const int at = RepairBlockScan(has, s_wave, n_has);
+      if (has) s_chunks[at] = unsigned(c - c0);
+      __syncthreads();
+      for (int h = 0; h < n_has; h++) {
+        const size_t base = (c0 + s_chunks[h]) * CH;
+        int m = 0;
+        unsigned char mk[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const size_t id = base + 4 * size_t(tid) + q;
+          mk[q]           = id < nc ? a.rep.mark[id] : 0;
+          const bool take = pass == 0 ? mk[q] == kRepairDensity : (pass == 1 ? mk[q] == kRepairAll : mk[q] != 0);
+          m += take ? 1 : 0;
+        }
+        int n_cells;
+        int o = RepairBlockScan(m, s_wave, n_cells);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const bool take = pass == 0 ? mk[q] == kRepairDensity : (pass == 1 ? mk[q] == kRepairAll : mk[q] != 0);
+          if (take) s_cells[o++] = unsigned(4 * tid + q);
+        }
+        __syncthreads();
+        if (pass < 2) {
+          if (tid == 0) {
+            for (int e = 0; e < n_cells; e++) {
+              const size_t id = base + s_cells[e];
+              const int i = int(id % nx), j = int((id / nx) % ny), k = int(id / (size_t(nx) * ny));
+              if (pass == 1) {
+                RepairAllFields<DE>(a, i, j, k);
+                continue;
+              }
+              // hydro_cuda.cu:1165-1176, the same summation order
+              const double v_l = RepairNeighbourDensity(a, i, j, k, -1, 0, 0);
+              const double v_r = RepairNeighbourDensity(a, i, j, k, 1, 0, 0);
+              const double v_d = RepairNeighbourDensity(a, i, j, k, 0, -1, 0);
+              const double v_u = RepairNeighbourDensity(a, i, j, k, 0, 1, 0);
+              const double v_b = RepairNeighbourDensity(a, i, j, k, 0, 0, -1);
+              const double v_t = RepairNeighbourDensity(a, i, j, k, 0, 0, 1);
+              a.U[id]          = (v_l + v_r + v_d + v_u + v_b + v_t) / 6;
+              if (a.grav.id != 0) {  // hydro_cuda.cu:301-315, from the recorded start of the step
+                const unsigned nrec = min(a.rep.count[1], unsigned(a.rep.grav_cap));
+                for (unsigned r = 0; r < nrec; r++) {
+                  const double *g = a.rep.grav + 5 * size_t(r);
+                  if (size_t(g[0]) != id) continue;
+                  double un[NC], gg[3];
+#pragma unroll
+                  for (int q = 0; q < NC; q++) un[q] = a.U[q * nc + id];
+                  GravField3D(a.grav.id, a.grav.Pos(0, i, ng, a.dx), a.grav.Pos(1, j, ng, a.dy),
+                              a.grav.Pos(2, k, ng, a.dz), gg[0], gg[1], gg[2]);
+                  GravSource<3>(un, g + 1, gg, dt);
+#pragma unroll
+                  for (int q = 0; q < NC; q++) a.U[q * nc + id] = un[q];
+                  break;
+                }
+              }
+              if ((a.flags & kFlagCrashRepair) && CrashedCell(a.U[id], a.U[4 * nc + id])) a.rep.mark[id] = kRepairAll;
+            }
+          }
+        } else {
+          for (int e = tid; e < n_cells; e += NT) {
+            const size_t id = base + s_cells[e];
+            a.rep.mark[id]  = 0;
+            if constexpr (!DE) {
+              const int i = int(id % nx), j = int((id / nx) % ny), k = int(id / (size_t(nx) * ny));
+              double un[5];
+#pragma unroll
+              for (int q = 0; q < 5; q++) un[q] = a.U[q * nc + id];
+              double dinv = 1.0 / un[0];
+              double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
+              if (a.flags & kFlagTemperatureFloor) {  // as the stage-2 epilogue
+                double Ekin = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz);
+                double uint = P::kExact ? (un[4] - Ekin) / un[0] : (un[4] - Ekin) * dinv;
+                if (uint < a.U_floor) un[4] = Ekin + un[0] * a.U_floor;
+                a.U[4 * nc + id] = un[4];
+              }
+              if (a.flags & kFlagReduceDt) {
+                double m_ = InverseCrossingTimeP<P>(un[4], un[0], dinv, vx, vy, vz, a.dx, a.dy, a.dz, 1.0 / a.dx,
+                                                    1.0 / a.dy, 1.0 / a.dz, a.gamma);
+                atomicMax(a.dti, (unsigned long long)__double_as_longlong(m_));
+              }
+              if (a.flags & kFlagPeriodicGhosts) WritePeriodicImages<5>(a.U, nc, nx, ny, nz, ng, i, j, k, un);
+            }
+          }
+        }
+        __syncthreads();
+      }
+      __syncthreads();
+    }
+  }
+  for (size_t c = tid; c < n_chunks; c += NT) a.rep.chunk[c] = 0;
+  if (tid == 0) a.rep.count[0] = 0, a.rep.count[1] = 0;
 }
 
 }  // namespace chl

@@ -149,6 +149,21 @@ hipError_t LaunchStage2(const StepArgs &a, int recon, int tile, hipStream_t s)
   return hipErrorInvalidValue;
 }
 
+// The repairs after stage 2 (RepairKernel): one workgroup.
+template <class P>
+hipError_t LaunchRepair(const StepArgs &a, hipStream_t s)
+{
+  if (a.de) {
+    hipLaunchKernelGGL((RepairKernel<P, true>), dim3(1), dim3(kRepairThreads), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((RepairKernel<P, false>), dim3(1), dim3(kRepairThreads), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+// The side buffer of a density repair next to fused periodic images (hydro_exact.hip).
+hipError_t LaunchRepairSide(const StepArgs &a, hipStream_t s);
+
 // Explicitly instantiated per policy in hydro_exact.hip / hydro_fast.hip.
 hipError_t LaunchStage1Policy(int policy, const StepArgs &a, int R1, int tile, hipStream_t s);
 hipError_t LaunchStage2Policy(int policy, const StepArgs &a, int recon, int tile, hipStream_t s);
@@ -156,5 +171,7 @@ hipError_t LaunchStage1Exact(const StepArgs &a, int R1, int tile, hipStream_t s)
 hipError_t LaunchStage2Exact(const StepArgs &a, int recon, int tile, hipStream_t s);
 hipError_t LaunchStage1Fast(int policy, const StepArgs &a, int R1, int tile, hipStream_t s);
 hipError_t LaunchStage2Fast(int policy, const StepArgs &a, int recon, int tile, hipStream_t s);
+hipError_t LaunchRepairExact(const StepArgs &a, hipStream_t s);
+hipError_t LaunchRepairFast(int policy, const StepArgs &a, hipStream_t s);
 
 }  // namespace chl

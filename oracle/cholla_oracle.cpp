@@ -1144,6 +1144,47 @@ static void AverageCellSingleField(int f, int i, int j, int k, int nx, int ny, s
   U[f * n_cells + Idx(i, j, k, nx, ny)] = (vl + vr + vd + vu + vb + vt) / 6;
 }
 
+// Average_Cell_All_Fields, src/hydro/hydro_cuda.cu:1179-1268 (no scalars, printf left out):
+// the mean density, momentum-weighted velocity and pressure of the cells of the 3x3x3 block
+// with positive density and pressure, summed k, j, i with i fastest; with dual energy
+// GasEnergy takes the mean internal energy.
+static void AverageCellAllFields(int i, int j, int k, int nx, int ny, size_t n_cells, double gamma, double *U)
+{
+  double d_av = 0.0, vx_av = 0.0, vy_av = 0.0, vz_av = 0.0, P_av = 0.0;
+  int N = 0;
+  for (int kk = k - 1; kk <= k + 1; kk++)
+    for (int jj = j - 1; jj <= j + 1; jj++)
+      for (int ii = i - 1; ii <= i + 1; ii++) {
+        const size_t n = Idx(ii, jj, kk, nx, ny);
+        const double d = U[n], mx = U[n_cells + n], my = U[2 * n_cells + n], mz = U[3 * n_cells + n];
+        const double P = (U[4 * n_cells + n] - (0.5 / d) * (mx * mx + my * my + mz * mz)) * (gamma - 1.0);
+        if (d > 0.0 && P > 0.0) {
+          d_av += d;
+          vx_av += mx;
+          vy_av += my;
+          vz_av += mz;
+          P_av += P / (gamma - 1.0);
+          N++;
+        }
+      }
+  P_av  = P_av / N;
+  vx_av = vx_av / d_av;
+  vy_av = vy_av / d_av;
+  vz_av = vz_av / d_av;
+  d_av  = d_av / N;
+  const size_t id   = Idx(i, j, k, nx, ny);
+  U[id]             = d_av;
+  U[n_cells + id]   = d_av * vx_av;
+  U[2 * n_cells + id] = d_av * vy_av;
+  U[3 * n_cells + id] = d_av * vz_av;
+  U[4 * n_cells + id] = P_av / (gamma - 1.0) + 0.5 * d_av * (vx_av * vx_av + vy_av * vy_av + vz_av * vz_av);
+  if (g_de) U[kGE * n_cells + id] = P_av / (gamma - 1.0);
+}
+
+// A build without both DENSITY_FLOOR and TEMPERATURE_FLOOR checks each updated cell for a
+// negative or NaN density or energy and repairs it (hydro_cuda.cu:372-382).
+static int g_crash_repair = 0;
+
 // Update_Conserved_Variables_3D, src/hydro/hydro_cuda.cu:172-402 (no gravity, no scalars)
 // STATIC_GRAV build: the field id (Parameters::custom_grav, 0 = none) and the grid's place in
 // the domain (x_off.., xbound..), set by orc_set_static_gravity
@@ -1258,7 +1299,7 @@ void orc_static_gravity_field(int dims, double x, double y, double z, double *g)
 
 void orc_update_3d(double *U, const double *Fx, const double *Fy, const double *Fz, int nx, int ny, int nz,
                    int n_ghost, double dx, double dy, double dz, double dt, int n_fields, int density_floor_on,
-                   double density_floor)
+                   double density_floor, double gamma)
 {
   size_t n_cells = size_t(nx) * ny * nz;
   double dtodx = dt / dx, dtody = dt / dy, dtodz = dt / dz;
@@ -1304,6 +1345,17 @@ void orc_update_3d(double *U, const double *Fx, const double *Fy, const double *
           StaticG3D(GravPos(0, i, n_ghost, dx), GravPos(1, j, n_ghost, dy), GravPos(2, k, n_ghost, dz), &g[0], &g[1],
                     &g[2]);
           GravSource(U, n_cells, id, dt, s4, g, 3);
+        }
+  }
+  // The crash repair (hydro_cuda.cu:372-382), after the floor and the gravity source; like
+  // the density average it runs after the sweep, serially in cell order.
+  if (g_crash_repair) {
+    for (int k = n_ghost; k < nz - n_ghost; k++)
+      for (int j = n_ghost; j < ny - n_ghost; j++)
+        for (int i = n_ghost; i < nx - n_ghost; i++) {
+          const size_t id = Idx(i, j, k, nx, ny);
+          const double d = U[id], E = U[4 * n_cells + id];
+          if (d < 0.0 || d != d || E < 0.0 || E != E) AverageCellAllFields(i, j, k, nx, ny, n_cells, gamma, U);
         }
   }
 }
@@ -1702,6 +1754,51 @@ struct Scratch {
 };
 static Scratch g_scratch;
 
+// ---------------------------------------------------------------------------- error-budget model
+// A model of the mixed form's rounding, for the accuracy analysis only: after each Riemann
+// solve of orc_vl3d every face flux F gets a deterministic pseudo-random relative error
+// u * scale * 2^-23 * |R|, u uniform in [-1/2, 1/2], where R is F itself (mode 1: an FP32
+// evaluation of the flux, rounding on the flux's own magnitude) or F - F_phys(left cell)
+// (mode 2: a flux evaluated in FP32 as a deviation from an FP64 reference flux of the cell
+// left of the face).  Off (scale 0) unless orc_set_flux_rounding_model turns it on.
+static double g_model_scale = 0.0;
+static int g_model_mode     = 0;
+static uint64_t g_model_call = 1;
+
+static inline double ModelU(uint64_t key)  // splitmix64 -> [-1/2, 1/2)
+{
+  uint64_t z = key + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return double(z >> 11) * 0x1p-53 - 0.5;
+}
+
+// The physical flux of cell id's state along dir, in global field order.
+static void PhysicalFlux(const double *U, size_t id, size_t n, int dir, double gamma, double f[5])
+{
+  const int o1 = 1 + dir, o2 = 1 + (dir + 1) % 3, o3 = 1 + (dir + 2) % 3;
+  const double d = U[id], mn = U[o1 * n + id], m2 = U[o2 * n + id], m3 = U[o3 * n + id], E = U[4 * n + id];
+  const double u = mn / d, v = m2 / d, w = m3 / d;
+  const double p = (gamma - 1.0) * (E - 0.5 * d * (u * u + v * v + w * w));
+  f[0] = mn, f[o1] = mn * u + p, f[o2] = mn * v, f[o3] = mn * w, f[4] = (E + p) * u;
+}
+
+static void ModelFluxRounding(double *F, const double *U, int nx, int ny, int nz, int dir, double gamma)
+{
+  if (g_model_scale == 0.0) return;
+  const size_t n      = size_t(nx) * ny * nz;
+  const uint64_t base = (g_model_call++) << 36;
+  for (size_t id = 0; id < n; id++) {
+    double f0[5] = {0, 0, 0, 0, 0};
+    if (g_model_mode == 2) PhysicalFlux(U, id, n, dir, gamma, f0);
+    for (int q = 0; q < 5; q++) {
+      const double r = std::fabs(F[q * n + id] - f0[q]);
+      F[q * n + id] += g_model_scale * 0x1p-23 * r * ModelU(base + id * 8 + q);
+    }
+  }
+}
+
 // VL_Algorithm_3D_CUDA, src/integrators/VL_3D_cuda.cu:38-360 (hydro; HLLC)
 void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double dy, double dz, double dt,
               double gamma, int n_fields, int recon, int density_floor_on, double density_floor)
@@ -1714,6 +1811,9 @@ void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double 
   RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, nz, gamma, 0);
   RiemannFluxes(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, nz, gamma, 1);
   RiemannFluxes(s.QLz.data(), s.QRz.data(), s.Fz.data(), nx, ny, nz, gamma, 2);
+  ModelFluxRounding(s.Fx.data(), U, nx, ny, nz, 0, gamma);
+  ModelFluxRounding(s.Fy.data(), U, nx, ny, nz, 1, gamma);
+  ModelFluxRounding(s.Fz.data(), U, nx, ny, nz, 2, gamma);
   orc_update_half_3d(U, s.Uh.data(), s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, dx, dy, dz, 0.5 * dt,
                      gamma, n_fields, density_floor_on, density_floor);
   if (recon == ORC_PLMC) {
@@ -1739,9 +1839,12 @@ void orc_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double 
   RiemannFluxes(s.QLx.data(), s.QRx.data(), s.Fx.data(), nx, ny, nz, gamma, 0);
   RiemannFluxes(s.QLy.data(), s.QRy.data(), s.Fy.data(), nx, ny, nz, gamma, 1);
   RiemannFluxes(s.QLz.data(), s.QRz.data(), s.Fz.data(), nx, ny, nz, gamma, 2);
+  ModelFluxRounding(s.Fx.data(), s.Uh.data(), nx, ny, nz, 0, gamma);
+  ModelFluxRounding(s.Fy.data(), s.Uh.data(), nx, ny, nz, 1, gamma);
+  ModelFluxRounding(s.Fz.data(), s.Uh.data(), nx, ny, nz, 2, gamma);
   if (g_de) orc_partial_update_ge_3d(U, nx, ny, nz, n_ghost, dx, dy, dz, dt, gamma);  // VL_3D_cuda.cu:314-324
   orc_update_3d(U, s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, n_ghost, dx, dy, dz, dt, n_fields,
-                density_floor_on, density_floor);
+                density_floor_on, density_floor, gamma);
   if (g_de) {  // VL_3D_cuda.cu:349-357
     orc_select_internal_energy_3d(U, nx, ny, nz, n_ghost);
     orc_sync_energies_3d(U, nx, ny, nz, n_ghost);
@@ -1940,7 +2043,10 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
     } else if (nz == 1) {  // 2D: no temperature floor (its guard, hydro_cuda.cu:1130-1131, admits no cell)
       orc_vl2d(U, nx, ny, n_ghost, dx, dy, dt, gamma, n_fields, recon);
     } else {
+      const int crash0 = g_crash_repair;
+      g_crash_repair   = !(density_floor_on && temperature_floor_on);
       orc_vl3d(U, nx, ny, nz, n_ghost, dx, dy, dz, dt, gamma, n_fields, recon, density_floor_on, density_floor);
+      g_crash_repair = crash0;
       if (temperature_floor_on) orc_temperature_floor(U, nx, ny, nz, n_ghost, U_floor);
     }
     dti = orc_calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, gamma);
@@ -1956,6 +2062,16 @@ int orc_run(double *U, int nx, int ny, int nz, int n_ghost, int n_fields, double
 // The DE build option (dual energy) as a run-time switch for every stage above.
 void orc_set_dual_energy(int on) { g_de = on ? 1 : 0; }
 void orc_set_positivity_fallback(int on) { g_pos_fallback = on ? 1 : 0; }
+// The error-budget model of the mixed form's rounding (see ModelFluxRounding); scale 0 = off.
+void orc_set_flux_rounding_model(double scale, int mode)
+{
+  g_model_scale = scale;
+  g_model_mode  = mode;
+  g_model_call  = 1;
+}
+// orc_vl3d called on its own: whether the build lacks one of the floors (crash repair)
+void orc_set_crash_repair(int on) { g_crash_repair = on ? 1 : 0; }
+int orc_get_crash_repair() { return g_crash_repair; }
 void orc_set_riemann(int solver) { g_riemann = (solver >= 1 && solver <= 3) ? solver : 0; }
 int orc_get_dual_energy() { return g_de; }
 

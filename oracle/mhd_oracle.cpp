@@ -46,7 +46,9 @@ void orc_update_half_3d(const double *U, double *Uh, const double *Fx, const dou
 int orc_get_dual_energy();
 void orc_update_3d(double *U, const double *Fx, const double *Fy, const double *Fz, int nx, int ny, int nz,
                    int n_ghost, double dx, double dy, double dz, double dt, int n_fields, int density_floor_on,
-                   double density_floor);
+                   double density_floor, double gamma);
+void orc_set_crash_repair(int on);
+int orc_get_crash_repair();
 void orc_temperature_floor(double *U, int nx, int ny, int nz, int n_ghost, double U_floor);
 void orc_set_boundaries_mhd(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags_in);
 }
@@ -849,7 +851,7 @@ void orc_mhd_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, dou
   orc_hlld(s.QLz.data(), s.QRz.data(), Uh + kBz * n, s.Fz.data(), long(n), g, 2);
   orc_ct_electric_fields(s.Fx.data(), s.Fy.data(), s.Fz.data(), Uh, s.E.data(), nx, ny, nz, 8);
   orc_update_3d(U, s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, n_ghost, dx, dy, dz, dt, 5, density_floor_on,
-                density_floor);
+                density_floor, g);
   orc_update_magnetic(U, U, s.E.data(), nx, ny, nz, dt, dx, dy, dz);
 }
 
@@ -866,7 +868,10 @@ int orc_mhd_run(double *U, int nx, int ny, int nz, int n_ghost, double dx, doubl
   while (t < tout && steps < max_steps) {
     double dt = kCfl / dti;
     if (t + dt > tout) dt = tout - t;
+    const int crash0 = orc_get_crash_repair();
+    orc_set_crash_repair(!(density_floor_on && temperature_floor_on));
     orc_mhd_vl3d(U, nx, ny, nz, n_ghost, dx, dy, dz, dt, g, density_floor_on, density_floor);
+    orc_set_crash_repair(crash0);
     if (temperature_floor_on) orc_temperature_floor(U, nx, ny, nz, n_ghost, U_floor);
     dti = orc_mhd_calc_dti(U, nx, ny, nz, n_ghost, dx, dy, dz, g);
     if (dt_log) dt_log[steps] = dt;

@@ -67,7 +67,9 @@ def lib():
             "orc_set_riemann": (None, [i]),
             "orc_hll": (None, [_dp, _dp, _dp, i, i, i, d, i]),
             "orc_pressure_from_de": (d, [d, d, d, d]),
-            "orc_update_3d": (None, [_dp] * 4 + [i, i, i, i, d, d, d, d, i, i, d]),
+            "orc_update_3d": (None, [_dp] * 4 + [i, i, i, i, d, d, d, d, i, i, d, d]),
+            "orc_set_crash_repair": (None, [i]),
+            "orc_set_flux_rounding_model": (None, [d, i]),
             "orc_temperature_floor": (None, [_dp, i, i, i, i, d]),
             "orc_calc_dti": (d, [_dp, i, i, i, i, d, d, d, d]),
             "orc_set_ghost_cells": (None, [_dp, i, i, i, i, i, _ip, i, i, i, i, i, i, i]),
@@ -165,6 +167,17 @@ def riemann(solver=HLLC):
         yield
     finally:
         lib().orc_set_riemann(HLLC)
+
+
+@contextlib.contextmanager
+def crash_repair(on=True):
+    """A build without both floors for orc_vl3d calls inside the block: cells that come out of
+    the update with a negative or NaN density or energy get Average_Cell_All_Fields."""
+    lib().orc_set_crash_repair(int(on))
+    try:
+        yield
+    finally:
+        lib().orc_set_crash_repair(0)
 
 
 @contextlib.contextmanager
