@@ -40,7 +40,7 @@ EXPORTS = [
     "chl_mhd_set_ghost_cells", "chl_mhd_set_boundaries",
     "chl_hlld_fluxes", "chl_mhd_reconstruct_plmc", "chl_ct_electric_fields", "chl_update_magnetic_field",
     "chl_magnetic_field_from_vector_potential", "chl_ic_orszag_tang", "chl_pack_real_cells",
-    "chl_pack_real_cells_f32", "chl_unpack_real_cells",
+    "chl_pack_real_cells_f32", "chl_unpack_real_cells", "chl_hydro_vl_step_noalloc", "chl_noalloc_scratch_bytes",
 ]
 LAUNCH_SPLIT, LAUNCH_PERSISTENT = 0, 1
 
@@ -111,6 +111,8 @@ def lib():
         "chl_hydro_create": (_i, [ctypes.POINTER(HydroConfig), ctypes.POINTER(_vp)]),
         "chl_hydro_destroy": (_i, [_vp]),
         "chl_hydro_vl_step": (_i, [_vp, _vp, _d, _d, _d, _d, _d, _d]),
+        "chl_hydro_vl_step_noalloc": (_i, [ctypes.POINTER(HydroConfig), _vp, _d, _d, _d, _d, _d, _d, _vp]),
+        "chl_noalloc_scratch_bytes": (ctypes.c_size_t, []),
         "chl_hydro_update_grid": (_i, [_vp, _vp, _d, _d, _d, _d, _d, _d, _d, _i, _dp]),
         "chl_hydro_run": (_i, [_vp, _vp, _d, _d, _d, _d, _d, _d, _d, _d, _i, _i, _dp, _ip]),
         "chl_hydro_time_steps": (_i, [_vp, _vp, _d, _d, _d, _d, _i, _i, ctypes.POINTER(ctypes.c_float), _dp]),

@@ -41,6 +41,7 @@ UNITS = [
     ("mhd_exact.hip", ["-ffp-contract=off"]),
     ("mhd_fast.hip", ["-ffp-contract=on"]),
     ("capi.cpp", ["-ffp-contract=off"]),
+    ("noalloc.hip", ["-ffp-contract=off"]),
     ("ensemble.cpp", ["-ffp-contract=off"]),
     ("mhd.cpp", ["-ffp-contract=off"]),
 ]

@@ -101,6 +101,19 @@ int chl_hydro_destroy(chl_hydro *h);
 int chl_hydro_vl_step(chl_hydro *h, double *d_conserved, double dx, double dy, double dz, double dt, double gamma,
                       double density_floor);
 
+/* chl_hydro_vl_step without a handle and without allocating device memory.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  cfg describes a 3D box as for
+ * chl_hydro_create; the step runs the same kernels as chl_hydro_vl_step, with its scratch
+ * (half-step state, repair marks) in a fixed device array of the library of
+ * chl_noalloc_scratch_bytes() bytes on each GPU, held until the process ends; a box that
+ * needs more is refused.  The launches go on `stream` (a hipStream_t; NULL = the default
+ * stream) and the call returns without synchronizing.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU race. */
+int chl_hydro_vl_step_noalloc(const chl_hydro_config *cfg, double *d_conserved, double dx, double dy, double dz,
+                              double dt, double gamma, double density_floor, void *stream);
+size_t chl_noalloc_scratch_bytes(void);
+
 /* Grid3D::Update_Hydro_Grid (src/grid/grid3D.cpp:474-585) fused: VL step, temperature
  * floor (U_floor as grid3D.cpp:497), and the next inverse timestep (Calc_dt_GPU)
  * returned in *dti_out.  With fill_ghosts != 0 the ghost cells are also refreshed,
