@@ -108,14 +108,15 @@ constexpr int kRepairGravCap = 4096;
 // The repair scratch of a 3D box: counts, gravity records, the side buffer of the six
 // boundary layers, one mark byte per 4096 cells and one per cell.  RepairBytes sizes it,
 // PlaceRepair points h->rep into it.
-size_t RepairLayout(const chl_hydro_config &c, size_t n_cells, size_t off[4])
+size_t RepairLayout(int nx, int ny, int nz, int ng_, int grav_cap, size_t off[4])
 {
-  const size_t ng = size_t(c.n_ghost), Lx = c.nx - 2 * ng, Ly = c.ny - 2 * ng, Lz = c.nz - 2 * ng;
+  const size_t ng = size_t(ng_), Lx = nx - 2 * ng, Ly = ny - 2 * ng, Lz = nz - 2 * ng;
+  const size_t n_cells = size_t(nx) * ny * nz;
   const size_t chunk = size_t(1) << chl::kRepairChunkShift;
   const size_t n_chunks = (n_cells + chunk - 1) / chunk;
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   off[0] = 256;                                                            // gravity records
-  off[1] = off[0] + up(size_t(kRepairGravCap) * 5 * sizeof(double));      // side buffer
+  off[1] = off[0] + up(size_t(grav_cap) * 5 * sizeof(double));            // side buffer
   off[2] = off[1] + up(2 * (Ly * Lz + Lx * Lz + Lx * Ly) * sizeof(double));  // chunk marks
   off[3] = off[2] + up(n_chunks);                                          // cell marks
   return off[3] + up(n_chunks * chunk);
@@ -310,21 +311,34 @@ int EnsureGraph(chl_hydro *h, double *U, double dx, double dy, double dz, double
 
 size_t chl_capi::RepairBytes(const chl_hydro_config &c)
 {
+  return RepairBytesFor(c.nx, c.ny, c.nz, c.n_ghost, kRepairGravCap);
+}
+
+size_t chl_capi::RepairBytesFor(int nx, int ny, int nz, int ng, int grav_cap)
+{
   size_t off[4];
-  return RepairLayout(c, size_t(c.nx) * c.ny * c.nz, off);
+  return RepairLayout(nx, ny, nz, ng, grav_cap, off);
+}
+
+chl::RepairState chl_capi::RepairAt(void *base, int nx, int ny, int nz, int ng, int grav_cap)
+{
+  size_t off[4];
+  (void)RepairLayout(nx, ny, nz, ng, grav_cap, off);
+  unsigned char *b = static_cast<unsigned char *>(base);
+  chl::RepairState r{};
+  r.count    = reinterpret_cast<unsigned *>(b);
+  r.grav     = grav_cap > 0 ? reinterpret_cast<double *>(b + off[0]) : nullptr;
+  r.grav_cap = grav_cap;
+  r.side     = reinterpret_cast<double *>(b + off[1]);
+  r.chunk    = b + off[2];
+  r.mark     = b + off[3];
+  return r;
 }
 
 void chl_capi::PlaceRepair(chl_hydro *h, void *base)
 {
-  size_t off[4];
-  (void)RepairLayout(h->cfg, h->n_cells, off);
-  unsigned char *b = static_cast<unsigned char *>(base);
-  h->rep.count     = reinterpret_cast<unsigned *>(b);
-  h->rep.grav      = reinterpret_cast<double *>(b + off[0]);
-  h->rep.grav_cap  = kRepairGravCap;
-  h->rep.side      = reinterpret_cast<double *>(b + off[1]);
-  h->rep.chunk     = b + off[2];
-  h->rep.mark      = b + off[3];
+  const chl_hydro_config &c = h->cfg;
+  h->rep                    = RepairAt(base, c.nx, c.ny, c.nz, c.n_ghost, kRepairGravCap);
 }
 
 size_t chl_capi::HalfStepBytes(const chl_hydro_config &c, size_t *w_main)

@@ -68,6 +68,10 @@ size_t HalfStepBytes(const chl_hydro_config &c, size_t *w_main);
 // Bytes of a 3D box's repair scratch, and h->rep pointed into one of that size at base.
 size_t RepairBytes(const chl_hydro_config &c);
 void PlaceRepair(chl_hydro *h, void *base);
+// The same for a box of nx x ny x nz cells (ghosts included) with room for grav_cap static
+// gravity records (0: none); RepairAt lays the state out at base.
+size_t RepairBytesFor(int nx, int ny, int nz, int ng, int grav_cap);
+chl::RepairState RepairAt(void *base, int nx, int ny, int nz, int ng, int grav_cap);
 // Enqueue one VL step of a 3D box on h->stream (chl_hydro_vl_step's work, no synchronize).
 int EnqueueVlStep(chl_hydro *h, double *U, double dx, double dy, double dz, double dt, double gamma,
                   double density_floor);

@@ -67,6 +67,7 @@ struct chl_ensemble {
   int qoff[chl::kQueueXcds + 1] = {};  // persistent queue of XCD x: [qoff[x], qoff[x + 1])
   bool bound = false, started = false;
   double gamma = 5.0 / 3.0, density_floor = 0.0, U_floor = 0.0;
+  long max_face = 0;  // the largest side buffer of a box (cells), for RepairSide launches
 
   // head buffer: the persistent queues' heads (XCD x at x * kHeadStride), then the status word
   static constexpr size_t kHeadBytes = 4096;
@@ -105,6 +106,9 @@ int CheckEnsembleConfig(const chl_ensemble_config &c)
   if (c.dual_energy && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
     return Fail(CHL_ERR_UNSUPPORTED, "dual energy is built for tile shapes 0 and 1 only");
   if (c.kchunk1 < 1 || c.kchunk2 < 1) return Fail(CHL_ERR_INVALID, "kchunk must be positive");
+  if (!c.density_floor || !c.temperature_floor)
+    return Fail(CHL_ERR_UNSUPPORTED, "ensembles run the default hydro build (density and temperature floors): a build "
+                                     "without both needs the crash repair's unfused floor, dt and fill");
   return CHL_OK;
 }
 
@@ -145,6 +149,16 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
     w_total += (w_fields * e->Cells(b) * elem + 255) / 256 * 256;
     if (c.dual_energy) w_total += (e->Cells(b) * sizeof(double) + 255) / 256 * 256;
   }
+  // then each box's repair state (marks, counts, side buffer; no gravity records)
+  std::vector<size_t> r_off(e->n);
+  e->max_face = 0;
+  for (int b = 0; b < e->n; b++) {
+    const chl::BoxDesc &d = e->hbox[b];
+    r_off[b]              = w_total;
+    w_total += chl_capi::RepairBytesFor(d.nx, d.ny, d.nz, ng, 0);
+    const long Lx = d.nx - 2 * ng, Ly = d.ny - 2 * ng, Lz = d.nz - 2 * ng;
+    e->max_face   = std::max(e->max_face, 2 * (Ly * Lz + Lx * Lz + Lx * Ly));
+  }
   if (w_total > e->W_bytes) {
     if (e->W) (void)hipFree(e->W);
     e->W       = nullptr;
@@ -152,7 +166,12 @@ int BuildPlan(chl_ensemble *e, const std::vector<double *> *U_override = nullptr
     CHL_HIP(hipMalloc(reinterpret_cast<void **>(&e->W), w_total));
     e->W_bytes = w_total;
   }
+  // the repair state starts at zero (the region may have held another layout's data)
+  CHL_HIP(hipMemsetAsync(e->W + r_off[0], 0, w_total - r_off[0], e->stream));
   for (int b = 0; b < e->n; b++) {
+    const chl::BoxDesc &d = e->hbox[b];
+    e->hbox[b].rep = chl_capi::RepairAt(e->W + r_off[b], d.nx, d.ny, d.nz, ng, 0);
+    if (!c.density_floor) e->hbox[b].rep.side = nullptr;
     e->hbox[b].W   = e->W + w_off[b];
     e->hbox[b].pdv = c.dual_energy ? reinterpret_cast<double *>(
                                          e->W + w_off[b] + ((w_fields * e->Cells(b) * elem + 255) / 256 * 256))
@@ -268,9 +287,11 @@ int EnqueueStep(chl_ensemble *e)
 {
   const chl_ensemble_config &c = e->cfg;
   CHL_HIP(chl::LaunchEnsembleClock(e->clk, e->box, e->n, kCfl, e->head, e->stream));
+  if (c.density_floor) CHL_HIP(chl::LaunchEnsembleRepairSide(Args(e, 1), e->max_face, e->stream));
   if (c.launch == CHL_LAUNCH_SPLIT) {
     CHL_TRY(EnqueueStage(e, 1));
     CHL_TRY(EnqueueStage(e, 2));
+    CHL_HIP(chl::LaunchEnsembleRepair(c.precision, Args(e, 1), e->stream));
     if (c.dual_energy) {
       CHL_HIP(chl::LaunchEnsembleDe(0, Args(e, 1), c.tile2, e->stream));
       CHL_HIP(chl::LaunchEnsembleDe(1, Args(e, 1), c.tile2, e->stream));
@@ -281,6 +302,9 @@ int EnqueueStep(chl_ensemble *e)
     const int grid = e->per_cu * e->num_cu;
     CHL_HIP(chl::LaunchEnsemblePersistent(c.precision, Args(e, 2), c.reconstruction, PairIndex(c.tile1, c.tile2),
                                           grid, e->stream));
+    // the repairs follow the whole launch (with dual energy, after the selection and sync
+    // the launch carries: a repaired cell's selection saw its unrepaired density)
+    CHL_HIP(chl::LaunchEnsembleRepair(c.precision, Args(e, 1), e->stream));
   }
   return CHL_OK;
 }
@@ -516,12 +540,16 @@ int chl_ensemble_advance_looped(chl_ensemble *e, int n_steps, float *ms_out)
         a.density_floor = e->density_floor, a.U_floor = e->U_floor;
         a.flags = Args(e, 0).flags;
         a.de     = c.dual_energy;
+        a.rep    = d.rep;
         a.kchunk = d.kchunk1;
+        if (a.rep.side) CHL_HIP(chl::LaunchRepairSide(a, e->stream));
         CHL_HIP(pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage1Exact(a, R1, c.tile1, e->stream)
                                            : chl::LaunchStage1Fast(pol, a, R1, c.tile1, e->stream));
         a.kchunk = d.kchunk2;
         CHL_HIP(pol == CHL_PREC_FP64_EXACT ? chl::LaunchStage2Exact(a, c.reconstruction, c.tile2, e->stream)
                                            : chl::LaunchStage2Fast(pol, a, c.reconstruction, c.tile2, e->stream));
+        CHL_HIP(pol == CHL_PREC_FP64_EXACT ? chl::LaunchRepairExact(a, e->stream)
+                                           : chl::LaunchRepairFast(pol, a, e->stream));
         if (c.dual_energy) {
           CHL_HIP(chl::LaunchDeSelect(d.U, d.nx, d.ny, d.nz, c.n_ghost, e->stream));
           CHL_HIP(chl::LaunchDeSync(d.U, d.nx, d.ny, d.nz, c.n_ghost, a.flags, a.U_floor, a.dx, a.dy, a.dz, a.gamma,

@@ -30,6 +30,14 @@ hipError_t LaunchEnsemblePersistent(int policy, const EnsembleArgs &e, int recon
   return EnsPersistent<PolicyFast>(e, recon, pair, grid, nullptr, s);
 }
 
+hipError_t LaunchEnsembleRepairExact(const EnsembleArgs &e, hipStream_t s);
+hipError_t LaunchEnsembleRepair(int policy, const EnsembleArgs &e, hipStream_t s)
+{
+  if (policy == PolicyExact::kId) return LaunchEnsembleRepairExact(e, s);
+  if (policy == PolicyMixed::kId) return EnsRepair<PolicyMixed>(e, s);
+  return EnsRepair<PolicyFast>(e, s);
+}
+
 hipError_t EnsemblePersistentSlots(int policy, int recon, int pair, int de, int *per_cu)
 {
   EnsembleArgs none{};

@@ -151,6 +151,19 @@ EnsembleStage2Kernel(EnsembleArgs e)
   Stage2Tile<P, RK, TX, TY, DE>(a, bx, by, bz, sm);
 }
 
+// The repairs stage 2 marked (RepairBox), one workgroup per box, after the step's stage 2;
+// a box without marks returns after reading its count.
+template <class P, bool DE>
+__global__ void __launch_bounds__(kRepairThreads) EnsembleRepairKernel(EnsembleArgs e)
+{
+  StepArgs a = BoxStepArgs(e, blockIdx.x, 1);
+  RepairBox<P, DE>(a);
+}
+
+// The side buffers of the boxes (RepairSideCell), before stage 2 rewrites their ghosts:
+// blockIdx.y is the box.
+__global__ void EnsembleRepairSideKernel(EnsembleArgs e);
+
 // Dual-energy phase PH (0: selection, 1: sync) over the real cells of stage-2 tile
 // (bx, by, bz) of box d; returns this lane's inverse-dt maximum (sync only).
 template <int TX, int TY, int PH>
@@ -328,5 +341,9 @@ hipError_t EnsemblePersistentSlots(int policy, int recon, int pair, int de, int 
 hipError_t LaunchEnsembleDe(int phase, const EnsembleArgs &e, int tile, hipStream_t s);
 hipError_t LaunchEnsembleClock(BoxClock *clk, const BoxDesc *box, int n_boxes, double cfl, unsigned int *head,
                                hipStream_t s);
+// The repairs of every box after stage 2 (EnsembleRepairKernel), and the boxes' side buffers
+// before it (max_face_cells: the largest side buffer of a box, in cells).
+hipError_t LaunchEnsembleRepair(int policy, const EnsembleArgs &e, hipStream_t s);
+hipError_t LaunchEnsembleRepairSide(const EnsembleArgs &e, long max_face_cells, hipStream_t s);
 
 }  // namespace chl

@@ -161,6 +161,18 @@ hipError_t EnsPersistentR(const EnsembleArgs &e, int pair, int grid, int *per_cu
   return hipErrorInvalidValue;
 }
 
+template <class P>
+hipError_t EnsRepair(const EnsembleArgs &e, hipStream_t s)
+{
+  if (e.n_boxes == 0) return hipSuccess;
+  if (e.de) {
+    hipLaunchKernelGGL((EnsembleRepairKernel<P, true>), dim3(e.n_boxes), dim3(kRepairThreads), 0, s, e);
+  } else {
+    hipLaunchKernelGGL((EnsembleRepairKernel<P, false>), dim3(e.n_boxes), dim3(kRepairThreads), 0, s, e);
+  }
+  return hipGetLastError();
+}
+
 // per_cu != null: occupancy query instead of a launch.
 template <class P>
 hipError_t EnsPersistent(const EnsembleArgs &e, int recon, int pair, int grid, int *per_cu, hipStream_t s)

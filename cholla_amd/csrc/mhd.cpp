@@ -14,6 +14,7 @@
 #include "capi_internal.hpp"
 #include "mhd_kernels.hpp"
 #include "support_kernels.hpp"
+#include "vl3d_launch.hpp"
 
 namespace
 {
@@ -31,6 +32,8 @@ struct chl_mhd {
   double *E          = nullptr;
   double *Ec         = nullptr;  // cell-centred electric-field terms (3 fields)
   unsigned char *scal = nullptr;  // [0] dti bits, [1] dt, [2] t, [3] n_step; [4] divergence bits
+  void *rep_buf      = nullptr;  // the repair marks and counts (chl::RepairState)
+  chl::RepairState rep{};
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   double *g_U        = nullptr;
@@ -75,12 +78,37 @@ chl::mhd::MhdArgs MakeArgs(chl_mhd *h, double *U, double dx, double dy, double d
   a.nx = c.nx, a.ny = c.ny, a.nz = c.nz, a.ng = c.n_ghost;
   a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dt = dt_host, a.dt_dev = dt_dev, a.dti = h->dti();
   a.density_floor = density_floor, a.U_floor = U_floor;
-  int flags = 0;
+  // without both floors the crash repair reads the 3x3x3 block as the update left it, so the
+  // temperature floor runs after the repairs (RepairStep) instead of in the full update
+  const bool crash = !(c.density_floor && c.temperature_floor);
+  int flags        = 0;
   if (c.density_floor) flags |= chl::kFlagDensityFloor;
-  if (update_grid && c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
+  if (crash) flags |= chl::kFlagCrashRepair;
+  if (update_grid && c.temperature_floor && !crash) flags |= chl::kFlagTemperatureFloor;
   if (update_grid) flags |= chl::kFlagReduceDt;
   a.flags = flags;
+  a.rep   = h->rep;
   return a;
+}
+
+// The repairs the full update marked (RepairKernel over the five hydro fields, as
+// Average_Cell_Single_Field / Average_Cell_All_Fields act on them), then, in a build without
+// both floors, the temperature floor.
+int RepairStep(chl_mhd *h, const chl::mhd::MhdArgs &m, bool update_grid)
+{
+  const chl_mhd_config &c = h->cfg;
+  chl::StepArgs a{};
+  a.U = m.U, a.nx = m.nx, a.ny = m.ny, a.nz = m.nz, a.ng = m.ng;
+  a.dx = m.dx, a.dy = m.dy, a.dz = m.dz, a.gamma = m.gamma, a.dt = m.dt, a.dt_dev = m.dt_dev, a.dti = m.dti;
+  a.density_floor = m.density_floor, a.U_floor = m.U_floor;
+  a.flags = m.flags & (chl::kFlagDensityFloor | chl::kFlagCrashRepair | chl::kFlagTemperatureFloor);
+  a.rep   = m.rep;
+  a.rep.side = nullptr;  // the MHD ghost fill follows the step: ghosts hold pre-step values
+  CHL_HIP(c.precision == CHL_PREC_FP64_EXACT ? chl::LaunchRepairExact(a, h->stream)
+                                             : chl::LaunchRepairFast(c.precision, a, h->stream));
+  if (update_grid && c.temperature_floor && (m.flags & chl::kFlagCrashRepair))
+    CHL_HIP(chl::LaunchTemperatureFloor(m.U, c.nx, c.ny, c.nz, c.n_ghost, m.U_floor, 8, h->stream));
+  return CHL_OK;
 }
 
 // dt from dti on the device, the step, the next dti, the ghost refresh
@@ -91,6 +119,7 @@ int EnqueueDeviceStep(chl_mhd *h, double *U, double dx, double dy, double dz, do
   CHL_HIP(chl::LaunchDtAdvance(h->dti(), h->dt(), h->t(), tout, kCfl, h->nstep(), h->stream));
   chl::mhd::MhdArgs a = MakeArgs(h, U, dx, dy, dz, gamma, 0.0, h->dt(), density_floor, U_floor, true);
   CHL_HIP(chl::LaunchMhdStep(c.precision, a, h->stream));
+  CHL_TRY(RepairStep(h, a, true));
   CHL_HIP(chl::LaunchMhdCalcDt(U, c.nx, c.ny, c.nz, c.n_ghost, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
   CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, 8, c.n_ghost, c.bc_flags, h->stream, true));
   return CHL_OK;
@@ -157,6 +186,10 @@ int chl_mhd_create(const chl_mhd_config *cfg, chl_mhd **out)
   if (e == hipSuccess) e = hipMalloc(&h->E, 3 * n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->Ec, 3 * n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&h->scal), 64);
+  const size_t rep_bytes = chl_capi::RepairBytesFor(cfg->nx, cfg->ny, cfg->nz, cfg->n_ghost, 0);
+  if (e == hipSuccess) e = hipMalloc(&h->rep_buf, rep_bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(h->rep_buf, 0, rep_bytes, h->stream);
+  if (e == hipSuccess) h->rep = chl_capi::RepairAt(h->rep_buf, cfg->nx, cfg->ny, cfg->nz, cfg->n_ghost, 0);
   // cleared once: cells and edges outside the computed regions keep these values
   if (e == hipSuccess) e = hipMemsetAsync(h->Uh, 0, 8 * n * sizeof(double), h->stream);
   if (e == hipSuccess) e = hipMemsetAsync(h->F, 0, 21 * n * h->F_elem, h->stream);
@@ -181,6 +214,7 @@ int chl_mhd_destroy(chl_mhd *h)
   if (h->E) (void)hipFree(h->E);
   if (h->Ec) (void)hipFree(h->Ec);
   if (h->scal) (void)hipFree(h->scal);
+  if (h->rep_buf) (void)hipFree(h->rep_buf);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CHL_OK;
@@ -192,6 +226,7 @@ int chl_mhd_vl_step(chl_mhd *h, double *U, double dx, double dy, double dz, doub
   if (!h || !U) return Fail(CHL_ERR_INVALID, "null argument");
   chl::mhd::MhdArgs a = MakeArgs(h, U, dx, dy, dz, gamma, dt, nullptr, density_floor, 0.0, false);
   CHL_HIP(chl::LaunchMhdStep(h->cfg.precision, a, h->stream));
+  CHL_TRY(RepairStep(h, a, false));
   CHL_HIP(hipStreamSynchronize(h->stream));
   return CHL_OK;
 }

@@ -722,6 +722,7 @@ struct MhdArgs {
   unsigned long long *dti;
   double density_floor, U_floor;
   int flags;  // kFlag* bits of vl3d_kernels.hpp
+  RepairState rep;  // cells the full update leaves to RepairKernel (vl3d_kernels.hpp)
 };
 
 __device__ __forceinline__ double MhdDt(const MhdArgs &a) { return a.dt_dev ? *a.dt_dev : a.dt; }
@@ -928,6 +929,7 @@ __global__ void __launch_bounds__(256) MhdFullKernel(MhdArgs a)
                            dtody * (double(Fy[o + id - sy]) - double(Fy[o + id])) +
                            dtodz * (double(Fz[o + id - sz]) - double(Fz[o + id])));
     }
+    int mark = 0;
     if ((a.flags & kFlagDensityFloor) && un[0] < a.density_floor) {
       if (un[0] > 0) {
         const double d0 = un[0];
@@ -935,10 +937,13 @@ __global__ void __launch_bounds__(256) MhdFullKernel(MhdArgs a)
 #pragma unroll
         for (int f = 1; f < 5; f++) un[f] *= (a.density_floor / d0);
       } else {
-        un[0] = (U[id - 1] + U[id + 1] + U[id - sy] + U[id + sy] + U[id - sz] + U[id + sz]) / 6;
+        mark = kRepairDensity;  // Average_Cell_Single_Field, done by RepairKernel in cell order
       }
     }
-    if (a.flags & kFlagTemperatureFloor) {
+    if (mark == 0 && (a.flags & kFlagCrashRepair) && CrashedCell(un[0], un[4])) mark = kRepairAll;
+    if (mark) {
+      MarkRepair(a.rep, id, mark);
+    } else if (a.flags & kFlagTemperatureFloor) {
       const double dinv = 1.0 / un[0];
       const double vx = un[1] * dinv, vy = un[2] * dinv, vz = un[3] * dinv;
       const double Ekin = 0.5 * un[0] * (vx * vx + vy * vy + vz * vz);

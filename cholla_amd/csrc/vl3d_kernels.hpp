@@ -1204,7 +1204,7 @@ __device__ void RepairAllFields(const StepArgs &a, int i, int j, int k)
 //           kernels), and the marks are cleared.
 // A step without marks returns after reading the count.
 template <class P, bool DE>
-__global__ void __launch_bounds__(kRepairThreads) RepairKernel(StepArgs a)
+__device__ void RepairBox(const StepArgs &a)
 {
   constexpr int NT = kRepairThreads, NC = kNumCons<DE>, CH = 1 << kRepairChunkShift;
   static_assert(CH == 4 * NT, "one mark chunk is four cells per lane");
@@ -1213,6 +1213,7 @@ __global__ void __launch_bounds__(kRepairThreads) RepairKernel(StepArgs a)
   __shared__ unsigned s_chunks[NT];
   __shared__ unsigned s_cells[CH];
   const int tid = threadIdx.x;
+  if (!a.rep.count) return;  // no repair scratch
   if (tid == 0) s_count = a.rep.count[0];
   __syncthreads();
   if (s_count == 0) return;
@@ -1315,6 +1316,39 @@ __global__ void __launch_bounds__(kRepairThreads) RepairKernel(StepArgs a)
   }
   for (size_t c = tid; c < n_chunks; c += NT) a.rep.chunk[c] = 0;
   if (tid == 0) a.rep.count[0] = 0, a.rep.count[1] = 0;
+}
+
+template <class P, bool DE>
+__global__ void __launch_bounds__(kRepairThreads) RepairKernel(StepArgs a)
+{
+  RepairBox<P, DE>(a);
+}
+
+// One cell t of the side buffer (RepairSideIndex): the depth-1 ghost layer's density on the
+// six faces, copied before stage 2 rewrites the ghosts.
+__device__ __forceinline__ void RepairSideCell(const double *U, double *side, int nx, int ny, int nz, int ng, long t)
+{
+  const int Lx = nx - 2 * ng, Ly = ny - 2 * ng, Lz = nz - 2 * ng;
+  const long fx = long(Ly) * Lz, fy = long(Lx) * Lz, fz = long(Lx) * Ly;
+  int face, a, b;
+  if (t < 2 * fx) {
+    face = int(t / fx), t -= face * fx, a = int(t % Ly), b = int(t / Ly);
+  } else if ((t -= 2 * fx) < 2 * fy) {
+    face = 2 + int(t / fy), t -= (face - 2) * fy, a = int(t % Lx), b = int(t / Lx);
+  } else if ((t -= 2 * fy) < 2 * fz) {
+    face = 4 + int(t / fz), t -= (face - 4) * fz, a = int(t % Lx), b = int(t / Lx);
+  } else {
+    return;
+  }
+  int i, j, k;
+  if (face < 2) {
+    i = face == 0 ? ng - 1 : nx - ng, j = ng + a, k = ng + b;
+  } else if (face < 4) {
+    i = ng + a, j = face == 2 ? ng - 1 : ny - ng, k = ng + b;
+  } else {
+    i = ng + a, j = ng + b, k = face == 4 ? ng - 1 : nz - ng;
+  }
+  side[RepairSideIndex(face, a, b, Lx, Ly, Lz)] = U[Lin(i, j, k, nx, ny)];
 }
 
 }  // namespace chl

@@ -162,3 +162,67 @@ def test_temperature_floor_in_the_fused_epilogue():
     eint = (V[4] - 0.5 * (V[1] ** 2 + V[2] ** 2 + V[3] ** 2) / V[0]) / V[0]
     assert eint.min() >= 3.0 * (1 - 1e-12)
     hy.close()
+
+
+@pytest.mark.parametrize("launch,looped", [(chl.LAUNCH_SPLIT, False), (chl.LAUNCH_PERSISTENT, False),
+                                           (chl.LAUNCH_SPLIT, True)])
+def test_ensemble_repairs_each_box(launch, looped):
+    """Two rough boxes of different sizes in one ensemble (split, persistent and per-box loop
+    launches): one step equals the oracle's on each box, repairs included."""
+    ng = 4
+    boxes = [rough_box(16, ng, seed=0), rough_box(12, ng, seed=2)]
+    states = [U for U, _ in boxes]
+    dims = [(N, N, N) for _, N in boxes]
+    dxyz = [(1.0 / (N - 2 * ng),) * 3 for _, N in boxes]
+    say("ensemble repairs, launch", launch, "looped", looped)
+    ens = chl.Ensemble(dims, ng, chl.RECON_PPMC, chl.PREC_FP64_EXACT, launch, tile1=0, tile2=0)
+    dev = [chl.DeviceArray.from_host(U) for U in states]
+    ens.bind(dev, np.array(dxyz).reshape(-1))
+    ens.start(G)
+    if looped:
+        ens.advance_looped(1)
+    else:
+        ens.advance(1)
+    for b, U in enumerate(states):
+        N = dims[b][0]
+        Uo = U.copy()
+        orc.run(Uo, N, N, N, ng, 5, *dxyz[b], G, orc.PPMC, [1] * 6, 0.0, 1e30, 1)
+        np.testing.assert_array_equal(dev[b].download(), Uo)
+    ens.close()
+
+
+def rough_mhd_box(n, ng, seed):
+    """The rough hydro box with a uniform field (no divergence) and its magnetic energy."""
+    U5, N = rough_box(n, ng, seed=seed, mach=4.0, d_lo=1e-3)
+    V = U5.reshape(5, -1)
+    B = np.array([0.1, 0.05, 0.02])
+    return np.concatenate([V[:4].reshape(-1), V[4] + 0.5 * (B ** 2).sum(),
+                           np.full(N ** 3, B[0]), np.full(N ** 3, B[1]), np.full(N ** 3, B[2])]).copy(), N
+
+
+@pytest.mark.parametrize("floors", [(True, True), (False, False)])
+def test_mhd_repairs(floors):
+    """An MHD step with six times the CFL dt on the rough box leaves five non-positive
+    densities and negative energies; the density repair (both floors) and the crash repair
+    (neither floor) match the oracle bit for bit."""
+    dfl, tfl = floors
+    n, ng = 16, 4
+    U, N = rough_mhd_box(n, ng, seed=0)
+    h = 1.0 / n
+    dt = 6.0 * 0.3 / orc.mhd_calc_dti(U, N, N, N, ng, h, h, h, G)
+    say("MHD repairs, floors", floors)
+    Uo = U.copy()
+    with orc.crash_repair(not (dfl and tfl)):
+        orc.lib().orc_mhd_vl3d(orc.ptr(Uo), N, N, N, ng, h, h, h, dt, G, int(dfl), 0.0)
+    D = chl.DeviceArray.from_host(U)
+    hm = chl.Mhd(N, N, N, ng, chl.PREC_FP64_EXACT, density_floor=dfl, temperature_floor=tfl)
+    hm.vl_step(D, h, h, h, dt, G, density_floor=0.0)
+    A = D.download()
+    # ghost faces of the result are the boundary fill's business on both sides (the oracle's
+    # unfused field update also writes ghost faces, from this step's wild ghost-zone fluxes)
+    orc.set_boundaries(A, N, N, N, 8, ng, [1] * 6)
+    orc.set_boundaries(Uo, N, N, N, 8, ng, [1] * 6)
+    a = Uo.reshape(8, N, N, N)[:, ng:-ng, ng:-ng, ng:-ng]
+    assert (a[0] > 0).all() and np.isfinite(a).all()
+    np.testing.assert_array_equal(A, Uo)
+    hm.close()
