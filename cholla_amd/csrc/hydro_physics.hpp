@@ -395,6 +395,51 @@ __device__ __forceinline__ void PpmSingleReduced(T im2, T im1, T i0, T ip1, T ip
   L_iph = L;
 }
 
+// PpmSingleReduced on deviations from the centre value (i0 = 0), as the mixed form's
+// reconstruction hands it the stencil; the same expressions with i0 substituted.
+template <typename T>
+__device__ __forceinline__ void PpmSingleCentred(T im2, T im1, T ip1, T ip2, T &L_iph, T &R_imh)
+{
+  const T C2    = T(1.25);
+  const T sixth = T(1.0 / 6.0);
+  T Rv  = T(7.0 / 12.0) * im1 - T(1.0 / 12.0) * (im2 + ip1);
+  T Lv  = T(7.0 / 12.0) * ip1 - T(1.0 / 12.0) * (im1 + ip2);
+  T s2m = im2 - T(2.0) * im1;
+  T s2c = im1 + ip1;
+  T s2p = ip2 - T(2.0) * ip1;
+  auto lim = [&](T asum, T iface, T s2a, T s2b, bool ext) -> T {
+    T s2f   = T(3.0) * asum - T(6.0) * iface;
+    T mn    = __builtin_fmin(__builtin_fmin(s2f, s2a), s2b);
+    T mx    = __builtin_fmax(__builtin_fmax(s2f, s2a), s2b);
+    T mag   = Fmin(C2 * Fmin(Fabs(s2a), Fabs(s2b)), Fabs(s2f));
+    T limit = mn >= T(0) ? mag : (mx < T(0) ? -mag : T(0));
+    return ext ? T(0.5) * asum - limit * sixth : iface;
+  };
+  // (iface - c_left) * (c_right - iface) < 0 with the centre at 0
+  T R = lim(im1, Rv, s2c, s2m, (Rv - im1) * Rv > T(0));
+  T L = lim(ip1, Lv, s2p, s2c, Lv * (ip1 - Lv) < T(0));
+  T s2face = T(6.0) * (R + L);
+  T mn4    = __builtin_fmin(__builtin_fmin(__builtin_fmin(s2m, s2c), s2p), s2face);
+  T mx4    = __builtin_fmax(__builtin_fmax(__builtin_fmax(s2m, s2c), s2p), s2face);
+  T mag2   = Fmin(C2 * Fmin(Fmin(Fabs(s2m), Fabs(s2c)), Fabs(s2p)), Fabs(s2face));
+  T s2lim  = mn4 >= T(0) ? mag2 : (mx4 < T(0) ? -mag2 : T(0));
+  T cmax   = Fmax(Fmax(Fabs(im2), Fabs(im1)), Fmax(Fabs(ip1), Fabs(ip2)));
+  T rho    = Fabs(s2face) > T(1.0e-12) * cmax ? s2lim * __builtin_amdgcn_rcpf(s2face) : T(0);
+  // sfl = -R, sfr = L
+  if ((R * L) >= T(0) || (ip1 * im1) >= T(0)) {
+    if (rho <= T(1.0) - T(1.0e-12)) {
+      R = rho * R;
+      L = rho * L;
+    }
+  } else {
+    const T R0 = R, L0 = L, aR = Fabs(R0), aL = Fabs(L0);
+    if (aR >= T(2.0) * aL) R = -T(2.0) * L0;
+    if (aL >= T(2.0) * aR) L = -T(2.0) * R0;
+  }
+  R_imh = R;
+  L_iph = L;
+}
+
 // ---------------------------------------------------------------------------- PPMC (VL)
 // PPMC_VL body, src/reconstruction/ppmc_cuda.cu:603-684: project the 5-point stencil
 // on cell i's characteristics, limit each characteristic field with PPM, project
@@ -420,9 +465,18 @@ __device__ __forceinline__ void ReconPpmcCore(const Prim<typename P::T> s[5], co
   SoundSpeed<P> cs(q, gamma);
   Char5<T> a[5];
 #pragma unroll
-  for (int m = 0; m < 5; m++) a[m] = PrimToChar<P>(q, s[m], cs);
+  for (int m = 0; m < 5; m++) {
+    if (SplitPressure<P> && m == 2) continue;  // deviations: the centre projects to zero
+    a[m] = PrimToChar<P>(q, s[m], cs);
+  }
   Char5<T> L, R;
-  if constexpr (P::kId == PolicyMixed::kId) {
+  if constexpr (SplitPressure<P>) {
+    PpmSingleCentred(a[0].a0, a[1].a0, a[3].a0, a[4].a0, L.a0, R.a0);
+    PpmSingleCentred(a[0].a1, a[1].a1, a[3].a1, a[4].a1, L.a1, R.a1);
+    PpmSingleCentred(a[0].a2, a[1].a2, a[3].a2, a[4].a2, L.a2, R.a2);
+    PpmSingleCentred(a[0].a3, a[1].a3, a[3].a3, a[4].a3, L.a3, R.a3);
+    PpmSingleCentred(a[0].a4, a[1].a4, a[3].a4, a[4].a4, L.a4, R.a4);
+  } else if constexpr (P::kId == PolicyMixed::kId) {
     PpmSingleReduced(a[0].a0, a[1].a0, a[2].a0, a[3].a0, a[4].a0, L.a0, R.a0);
     PpmSingleReduced(a[0].a1, a[1].a1, a[2].a1, a[3].a1, a[4].a1, L.a1, R.a1);
     PpmSingleReduced(a[0].a2, a[1].a2, a[2].a2, a[3].a2, a[4].a2, L.a2, R.a2);
