@@ -757,18 +757,52 @@ struct GeIfaces<T, N, true> {
   __device__ T *R() { return gR; }
 };
 
-// LDS of one stage-2 tile.
+// Whether a stage-2 tile runs its x and y work as one list (both directions'
+// reconstructions in one pass, both directions' fluxes in the next): the lists then
+// fill whole waves (a 16x16 tile has 288 + 288 reconstructions, 9 full wave64 passes
+// instead of 2 x 5 with a half-empty wave each) and the plane takes three barriers
+// instead of five.  It holds both directions' interface values and fluxes in LDS at
+// once, so it is taken only where the tiles that keep the kernel at its minimum
+// occupancy still fit in a CU's 160 KiB.
+template <class P, int RK, int TX, int TY, bool DE = false>
+struct Stage2Merge {
+  using T                    = typename P::T;
+  using S                    = Stage2Shape<P, RK, TX, TY>;
+  static constexpr size_t kBytes = sizeof(WCell<T, DE>) * S::PY * S::PX +
+                                   size_t(S::NQX + S::NQY) * (2 * sizeof(IPrimOf<P>) + (DE ? 2 * sizeof(T) : 0)) +
+                                   size_t(kNumCons<DE>) * (S::NFX + S::NFY) * sizeof(T) + 8 * (TX * TY / 64);
+  static constexpr int kWaves  = TX * TY / 64;
+  static constexpr int kGroups = (4 * Stage2Occupancy<P>::kMinWaves + kWaves - 1) / kWaves;
+  static constexpr bool value  = kBytes * kGroups <= 160 * 1024;
+};
+
+// LDS of one stage-2 tile.  Merged lists keep the x items first, then the y items.
 template <class P, int RK, int TX, int TY, bool DE = false>
 struct Stage2Smem {
   using T = typename P::T;
   using S = Stage2Shape<P, RK, TX, TY>;
+  static constexpr bool kMerge = Stage2Merge<P, RK, TX, TY, DE>::value;
+  static constexpr int NQS     = kMerge ? S::NQX + S::NQY : S::NQ;
+  static constexpr int NFS     = kMerge ? S::NFX + S::NFY : S::NF;
   WCell<T, DE> plane[S::PY][S::PX];
-  IPrimOf<P> qL[S::NQ];
-  IPrimOf<P> qR[S::NQ];
-  GeIfaces<T, S::NQ, DE> qg;
-  T fl[kNumCons<DE>][S::NF];
+  IPrimOf<P> qL[NQS];
+  IPrimOf<P> qR[NQS];
+  GeIfaces<T, NQS, DE> qg;
+  T fl[kNumCons<DE>][NFS];
   double red[TX * TY / 64];
 };
+
+// The y sweep's view of a cell: the velocity triple rotated into (n, t1, t2) = (vy, vz,
+// vx) where the x sweep's is (vx, vy, vz), so one reconstruction and one Riemann code
+// path serve both directions of a merged list (runtime selects, no divergence).
+template <typename T, bool DE>
+__device__ __forceinline__ void RotateForY(WCell<T, DE> &c, bool y)
+{
+  const T vx = c.vx, vy = c.vy, vz = c.vz;
+  c.vx = y ? vy : vx;
+  c.vy = y ? vz : vy;
+  c.vz = y ? vx : vz;
+}
 
 // ---------------------------------------------------------------------------- repair marks
 // The crash test of hydro_cuda.cu:375-376: negative or NaN density or energy.
@@ -847,6 +881,7 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
 {
   using T          = typename P::T;
   using S          = Stage2Shape<P, RK, TX, TY>;
+  using SM         = Stage2Smem<P, RK, TX, TY, DE>;
   constexpr int NT = TX * TY;
   constexpr int r  = ReconTraits<RK>::kRadius;
   constexpr int NS = 2 * r + 1;
@@ -945,69 +980,135 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
     qL_lo = qLz;
     gL_lo = gLz;
     __syncthreads();
-    // x reconstruction of cells -1..TX of every row
-    for (int it = tid; it < S::NQX; it += NT) {
-      int rj = it / (TX + 2), ri = it - rj * (TX + 2);  // plane column of cell = ri - 1 + H
-      WCell<T, DE> s[NS];
-#pragma unroll
-      for (int m = 0; m < NS; m++) s[m] = plane[rj + H][ri - 1 + H - r + m];
-      if constexpr (DE) {
-        ReconDir<P, RK, 0, DE>(s, gamma, pf, qL[it], qR[it], &qgL[it], &qgR[it]);
-      } else {
-        ReconDir<P, RK, 0>(s, gamma, pf, qL[it], qR[it]);
-      }
-    }
-    __syncthreads();
-    for (int it = tid; it < S::NFX; it += NT) {
-      int fj = it / (TX + 1), fi = it - fj * (TX + 1);  // between recon cells fi and fi+1 of row fj
-      int ql = fj * (TX + 2) + fi;
-      T g[NC];
-      if constexpr (DE) {
-        IfaceFlux<P, RK, 0, DE>(qL[ql], qR[ql + 1], gamma, inv_gm1, qgL[ql], qgR[ql + 1], g);
-      } else {
-        IfaceFlux<P, RK, 0, DE>(qL[ql], qR[ql + 1], gamma, inv_gm1, T(0), T(0), g);
-      }
-#pragma unroll
-      for (int q = 0; q < NC; q++) fl[q][it] = g[q];
-    }
-    __syncthreads();
-    double acc[NC];
-    {
-      int fa = ty * (TX + 1) + tx;
-#pragma unroll
-      for (int q = 0; q < NC; q++) acc[q] = dtodx * (double(fl[q][fa]) - double(fl[q][fa + 1]));
+    double acc[NC];  // dtodx dFx (then + dtody dFy at the update) of the lane's cell
+    if constexpr (SM::kMerge) {
+      // split pressure: the x and y faces' references (cells i-1, i and j-1, j), read
+      // before the plane is rewritten at the top of the next iteration
+      double ref_x = 0.0, ref_y = 0.0;
       if constexpr (SplitPressure<P>) {
-        // the x and y faces' references (cells i-1, i and j-1, j), read here: the plane
-        // is rewritten after the last barrier of this iteration
         const double pc = double(plane[ty + H][tx + H].p_or_E);
-        acc[1] += dtodx * (double(plane[ty + H][tx + H - 1].p_or_E) - pc);
-        acc[2] += dtody * (double(plane[ty + H - 1][tx + H].p_or_E) - pc);
+        ref_x           = dtodx * (double(plane[ty + H][tx + H - 1].p_or_E) - pc);
+        ref_y           = dtody * (double(plane[ty + H - 1][tx + H].p_or_E) - pc);
       }
-    }
-    // y reconstruction of cells -1..TY of every column
-    for (int it = tid; it < S::NQY; it += NT) {
-      int rj = it / TX, ri = it - rj * TX;
-      WCell<T, DE> s[NS];
+      // x reconstruction of cells -1..TX of every row, then y reconstruction of cells
+      // -1..TY of every column, as one list; y items in the y sweep's frame
+      for (int it = tid; it < S::NQX + S::NQY; it += NT) {
+        const bool y = it >= S::NQX;
+        int row, col, stride;
+        if (!y) {
+          int rj = it / (TX + 2), ri = it - rj * (TX + 2);
+          row = rj + H, col = ri - 1 + H - r, stride = 1;
+        } else {
+          int iy = it - S::NQX, rj = iy / TX, ri = iy - rj * TX;
+          row = rj - 1 + H - r, col = ri + H, stride = S::PX;
+        }
+        const WCell<T, DE> *base = &plane[row][col];
+        WCell<T, DE> st[NS];
 #pragma unroll
-      for (int m = 0; m < NS; m++) s[m] = plane[rj - 1 + H - r + m][ri + H];
-      if constexpr (DE) {
-        ReconDir<P, RK, 1, DE>(s, gamma, pf, qL[it], qR[it], &qgL[it], &qgR[it]);
-      } else {
-        ReconDir<P, RK, 1>(s, gamma, pf, qL[it], qR[it]);
+        for (int m = 0; m < NS; m++) {
+          st[m] = base[m * stride];
+          RotateForY(st[m], y);
+        }
+        if constexpr (DE) {
+          ReconDir<P, RK, 0, DE>(st, gamma, pf, qL[it], qR[it], &qgL[it], &qgR[it]);
+        } else {
+          ReconDir<P, RK, 0>(st, gamma, pf, qL[it], qR[it]);
+        }
       }
-    }
-    __syncthreads();
-    for (int it = tid; it < S::NFY; it += NT) {
-      int fj = it / TX, fi = it - fj * TX;  // between recon rows fj and fj+1 of column fi
-      const int lq = fj * TX + fi, rq = (fj + 1) * TX + fi;
-      T g[NC];
-      if constexpr (DE) {
-        IfaceFlux<P, RK, 1, DE>(qL[lq], qR[rq], gamma, inv_gm1, qgL[lq], qgR[rq], g);
-      } else {
-        IfaceFlux<P, RK, 1, DE>(qL[lq], qR[rq], gamma, inv_gm1, T(0), T(0), g);
-      }
+      __syncthreads();
+      // x faces, then y faces; the fluxes stay in each sweep's frame
+      for (int it = tid; it < S::NFX + S::NFY; it += NT) {
+        int lq, rq;
+        if (it < S::NFX) {
+          int fj = it / (TX + 1), fi = it - fj * (TX + 1);
+          lq = fj * (TX + 2) + fi, rq = lq + 1;
+        } else {
+          int iy = it - S::NFX, fj = iy / TX, fi = iy - fj * TX;
+          lq = S::NQX + fj * TX + fi, rq = lq + TX;
+        }
+        T g[NC];
+        if constexpr (DE) {
+          IfaceFlux<P, RK, 0, DE>(qL[lq], qR[rq], gamma, inv_gm1, qgL[lq], qgR[rq], g);
+        } else {
+          IfaceFlux<P, RK, 0, DE>(qL[lq], qR[rq], gamma, inv_gm1, T(0), T(0), g);
+        }
 #pragma unroll
-      for (int q = 0; q < NC; q++) fl[q][it] = g[q];
+        for (int q = 0; q < NC; q++) fl[q][it] = g[q];
+      }
+      __syncthreads();
+      {
+        int fa = ty * (TX + 1) + tx;
+#pragma unroll
+        for (int q = 0; q < NC; q++) acc[q] = dtodx * (double(fl[q][fa]) - double(fl[q][fa + 1]));
+        if constexpr (SplitPressure<P>) {
+          acc[1] += ref_x;
+          acc[2] += ref_y;
+        }
+      }
+    } else {
+      // x reconstruction of cells -1..TX of every row
+      for (int it = tid; it < S::NQX; it += NT) {
+        int rj = it / (TX + 2), ri = it - rj * (TX + 2);  // plane column of cell = ri - 1 + H
+        WCell<T, DE> st[NS];
+#pragma unroll
+        for (int m = 0; m < NS; m++) st[m] = plane[rj + H][ri - 1 + H - r + m];
+        if constexpr (DE) {
+          ReconDir<P, RK, 0, DE>(st, gamma, pf, qL[it], qR[it], &qgL[it], &qgR[it]);
+        } else {
+          ReconDir<P, RK, 0>(st, gamma, pf, qL[it], qR[it]);
+        }
+      }
+      __syncthreads();
+      for (int it = tid; it < S::NFX; it += NT) {
+        int fj = it / (TX + 1), fi = it - fj * (TX + 1);  // between recon cells fi and fi+1 of row fj
+        int ql = fj * (TX + 2) + fi;
+        T g[NC];
+        if constexpr (DE) {
+          IfaceFlux<P, RK, 0, DE>(qL[ql], qR[ql + 1], gamma, inv_gm1, qgL[ql], qgR[ql + 1], g);
+        } else {
+          IfaceFlux<P, RK, 0, DE>(qL[ql], qR[ql + 1], gamma, inv_gm1, T(0), T(0), g);
+        }
+#pragma unroll
+        for (int q = 0; q < NC; q++) fl[q][it] = g[q];
+      }
+      __syncthreads();
+      {
+        int fa = ty * (TX + 1) + tx;
+#pragma unroll
+        for (int q = 0; q < NC; q++) acc[q] = dtodx * (double(fl[q][fa]) - double(fl[q][fa + 1]));
+        if constexpr (SplitPressure<P>) {
+          // the x and y faces' references (cells i-1, i and j-1, j), read here: the plane
+          // is rewritten after the last barrier of this iteration
+          const double pc = double(plane[ty + H][tx + H].p_or_E);
+          acc[1] += dtodx * (double(plane[ty + H][tx + H - 1].p_or_E) - pc);
+          acc[2] += dtody * (double(plane[ty + H - 1][tx + H].p_or_E) - pc);
+        }
+      }
+      // y reconstruction of cells -1..TY of every column
+      for (int it = tid; it < S::NQY; it += NT) {
+        int rj = it / TX, ri = it - rj * TX;
+        WCell<T, DE> st[NS];
+#pragma unroll
+        for (int m = 0; m < NS; m++) st[m] = plane[rj - 1 + H - r + m][ri + H];
+        if constexpr (DE) {
+          ReconDir<P, RK, 1, DE>(st, gamma, pf, qL[it], qR[it], &qgL[it], &qgR[it]);
+        } else {
+          ReconDir<P, RK, 1>(st, gamma, pf, qL[it], qR[it]);
+        }
+      }
+      __syncthreads();
+      for (int it = tid; it < S::NFY; it += NT) {
+        int fj = it / TX, fi = it - fj * TX;  // between recon rows fj and fj+1 of column fi
+        const int lq = fj * TX + fi, rq = (fj + 1) * TX + fi;
+        T g[NC];
+        if constexpr (DE) {
+          IfaceFlux<P, RK, 1, DE>(qL[lq], qR[rq], gamma, inv_gm1, qgL[lq], qgR[rq], g);
+        } else {
+          IfaceFlux<P, RK, 1, DE>(qL[lq], qR[rq], gamma, inv_gm1, T(0), T(0), g);
+        }
+#pragma unroll
+        for (int q = 0; q < NC; q++) fl[q][it] = g[q];
+      }
     }
     __syncthreads();
     if (own) {
@@ -1015,13 +1116,16 @@ __device__ __forceinline__ void Stage2Tile(const StepArgs &a, int bx, int by, in
       double un[NC];
       double ustart[4];  // density and momenta at the start of the step (static gravity)
       {
-        int fa = ty * TX + tx;
+        // y fluxes: global order in the separate phases; merged, after the x faces and in
+        // the y sweep's frame (d, my, mz, mx, E[, GasEnergy])
+        const int fa = (SM::kMerge ? S::NFX : 0) + ty * TX + tx;
         // U += ((dtodx dFx + dtody dFy) + dtodz dFz), hydro_cuda.cu:241-257; GasEnergy
         // first takes the pdV source (the separate Partial_Update kernel, VL_3D_cuda.cu:314-324)
         if constexpr (SplitPressure<P>) acc[3] += dtodz * (double(zref_lo) - double(zref_hi));  // z faces' references
 #pragma unroll
         for (int q = 0; q < NC; q++) {
-          double axy = acc[q] + dtody * (double(fl[q][fa]) - double(fl[q][fa + TX]));
+          const int qy = SM::kMerge ? (q == 1 ? 3 : (q == 2 ? 1 : (q == 3 ? 2 : q))) : q;
+          double axy = acc[q] + dtody * (double(fl[qy][fa]) - double(fl[qy][fa + TX]));
           double u0  = U[q * nc + id];
           if constexpr (P::kGrav) {
             if (q < 4) ustart[q] = u0;
