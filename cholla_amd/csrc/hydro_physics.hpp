@@ -160,20 +160,35 @@ __device__ __forceinline__ typename P::T Div1(typename P::T a, typename P::T b)
   }
 }
 
+// fmax / fmin / fabs in the operands' own precision: the unsuffixed builtins are the
+// double ones, which would widen FP32 operands (a conversion per operand and an FP64
+// min/max).
 template <typename T>
 __device__ __forceinline__ T Fmax(T a, T b)
 {
-  return __builtin_fmax(a, b);
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_fmaxf(a, b);
+  } else {
+    return __builtin_fmax(a, b);
+  }
 }
 template <typename T>
 __device__ __forceinline__ T Fmin(T a, T b)
 {
-  return __builtin_fmin(a, b);
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_fminf(a, b);
+  } else {
+    return __builtin_fmin(a, b);
+  }
 }
 template <typename T>
 __device__ __forceinline__ T Fabs(T a)
 {
-  return __builtin_fabs(a);
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_fabsf(a);
+  } else {
+    return __builtin_fabs(a);
+  }
 }
 template <typename T>
 __device__ __forceinline__ T SignOf(T a)  // SIGN macro, src/global/global.h:144
@@ -365,8 +380,8 @@ __device__ __forceinline__ void PpmSingleReduced(T im2, T im1, T i0, T ip1, T ip
   T s2p = i0 + ip2 - T(2.0) * ip1;
   auto lim = [&](T asum, T c_left, T c_right, T iface, T s2a, T s2b) -> T {
     T s2f   = T(3.0) * asum - T(6.0) * iface;
-    T mn    = __builtin_fmin(__builtin_fmin(s2f, s2a), s2b);
-    T mx    = __builtin_fmax(__builtin_fmax(s2f, s2a), s2b);
+    T mn    = Fmin(Fmin(s2f, s2a), s2b);
+    T mx    = Fmax(Fmax(s2f, s2a), s2b);
     T mag   = Fmin(C2 * Fmin(Fabs(s2a), Fabs(s2b)), Fabs(s2f));
     T limit = mn >= T(0) ? mag : (mx < T(0) ? -mag : T(0));
     bool ext = (iface - c_left) * (c_right - iface) < T(0);
@@ -375,8 +390,8 @@ __device__ __forceinline__ void PpmSingleReduced(T im2, T im1, T i0, T ip1, T ip
   T R = lim(a_m, im1, i0, Rv, s2c, s2m);
   T L = lim(a_p, i0, ip1, Lv, s2p, s2c);
   T s2face = T(6.0) * (R + L) - T(12.0) * i0;
-  T mn4    = __builtin_fmin(__builtin_fmin(__builtin_fmin(s2m, s2c), s2p), s2face);
-  T mx4    = __builtin_fmax(__builtin_fmax(__builtin_fmax(s2m, s2c), s2p), s2face);
+  T mn4    = Fmin(Fmin(Fmin(s2m, s2c), s2p), s2face);
+  T mx4    = Fmax(Fmax(Fmax(s2m, s2c), s2p), s2face);
   T mag2   = Fmin(C2 * Fmin(Fmin(Fabs(s2m), Fabs(s2c)), Fabs(s2p)), Fabs(s2face));
   T s2lim  = mn4 >= T(0) ? mag2 : (mx4 < T(0) ? -mag2 : T(0));
   T cmax   = Fmax(Fmax(Fmax(Fabs(im2), Fabs(im1)), Fmax(Fabs(i0), Fabs(ip1))), Fabs(ip2));
@@ -409,8 +424,8 @@ __device__ __forceinline__ void PpmSingleCentred(T im2, T im1, T ip1, T ip2, T &
   T s2p = ip2 - T(2.0) * ip1;
   auto lim = [&](T asum, T iface, T s2a, T s2b, bool ext) -> T {
     T s2f   = T(3.0) * asum - T(6.0) * iface;
-    T mn    = __builtin_fmin(__builtin_fmin(s2f, s2a), s2b);
-    T mx    = __builtin_fmax(__builtin_fmax(s2f, s2a), s2b);
+    T mn    = Fmin(Fmin(s2f, s2a), s2b);
+    T mx    = Fmax(Fmax(s2f, s2a), s2b);
     T mag   = Fmin(C2 * Fmin(Fabs(s2a), Fabs(s2b)), Fabs(s2f));
     T limit = mn >= T(0) ? mag : (mx < T(0) ? -mag : T(0));
     return ext ? T(0.5) * asum - limit * sixth : iface;
@@ -419,8 +434,8 @@ __device__ __forceinline__ void PpmSingleCentred(T im2, T im1, T ip1, T ip2, T &
   T R = lim(im1, Rv, s2c, s2m, (Rv - im1) * Rv > T(0));
   T L = lim(ip1, Lv, s2p, s2c, Lv * (ip1 - Lv) < T(0));
   T s2face = T(6.0) * (R + L);
-  T mn4    = __builtin_fmin(__builtin_fmin(__builtin_fmin(s2m, s2c), s2p), s2face);
-  T mx4    = __builtin_fmax(__builtin_fmax(__builtin_fmax(s2m, s2c), s2p), s2face);
+  T mn4    = Fmin(Fmin(Fmin(s2m, s2c), s2p), s2face);
+  T mx4    = Fmax(Fmax(Fmax(s2m, s2c), s2p), s2face);
   T mag2   = Fmin(C2 * Fmin(Fmin(Fabs(s2m), Fabs(s2c)), Fabs(s2p)), Fabs(s2face));
   T s2lim  = mn4 >= T(0) ? mag2 : (mx4 < T(0) ? -mag2 : T(0));
   T cmax   = Fmax(Fmax(Fabs(im2), Fabs(im1)), Fmax(Fabs(ip1), Fabs(ip2)));
