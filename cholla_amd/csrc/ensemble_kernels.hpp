@@ -236,6 +236,10 @@ union SmemPair {
 // enough: outside thread-group-split mode the hardware treats it as a no-op.
 __device__ __forceinline__ void SignalTileDone(int *count)
 {
+  // The wait intrinsic is declared as touching no memory, so nothing stops the compiler
+  // from moving a load or store of the tile past it; a signal fence (a compiler barrier,
+  // no instruction) keeps every memory access of the tile ahead of the wait.
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) atomicAdd(count, 1);

@@ -30,7 +30,6 @@ struct chl_mhd {
   void *F            = nullptr;  // 3 x 7 fields of the flux element type
   size_t F_elem      = 8;
   double *E          = nullptr;
-  double *Ec         = nullptr;  // cell-centred electric-field terms (3 fields)
   unsigned char *scal = nullptr;  // [0] dti bits, [1] dt, [2] t, [3] n_step; [4] divergence bits
   void *rep_buf      = nullptr;  // the repair marks and counts (chl::RepairState)
   chl::RepairState rep{};
@@ -73,7 +72,7 @@ chl::mhd::MhdArgs MakeArgs(chl_mhd *h, double *U, double dx, double dy, double d
 {
   const chl_mhd_config &c = h->cfg;
   chl::mhd::MhdArgs a{};
-  a.U = U, a.Uh = h->Uh, a.E = h->E, a.Ec = h->Ec;
+  a.U = U, a.Uh = h->Uh, a.E = h->E;
   for (int d = 0; d < 3; d++) a.F[d] = static_cast<unsigned char *>(h->F) + size_t(d) * 7 * h->n_cells * h->F_elem;
   a.nx = c.nx, a.ny = c.ny, a.nz = c.nz, a.ng = c.n_ghost;
   a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dt = dt_host, a.dt_dev = dt_dev, a.dti = h->dti();
@@ -184,7 +183,6 @@ int chl_mhd_create(const chl_mhd_config *cfg, chl_mhd **out)
   if (e == hipSuccess) e = hipMalloc(&h->Uh, 8 * n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->F, 21 * n * h->F_elem);
   if (e == hipSuccess) e = hipMalloc(&h->E, 3 * n * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&h->Ec, 3 * n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&h->scal), 64);
   const size_t rep_bytes = chl_capi::RepairBytesFor(cfg->nx, cfg->ny, cfg->nz, cfg->n_ghost, 0);
   if (e == hipSuccess) e = hipMalloc(&h->rep_buf, rep_bytes);
@@ -212,7 +210,6 @@ int chl_mhd_destroy(chl_mhd *h)
   if (h->Uh) (void)hipFree(h->Uh);
   if (h->F) (void)hipFree(h->F);
   if (h->E) (void)hipFree(h->E);
-  if (h->Ec) (void)hipFree(h->Ec);
   if (h->scal) (void)hipFree(h->scal);
   if (h->rep_buf) (void)hipFree(h->rep_buf);
   if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -23,16 +23,20 @@ __device__ __forceinline__ void BlockMaxAtomic(double m, unsigned long long *out
   }
 }
 
-// Calc_dt_3D with MHD (src/hydro/hydro_cuda.cu:514-560)
-__global__ void __launch_bounds__(256) MhdCalcDtKernel(const double *U, int nx, int ny, int nz, int ng, double dx,
-                                                       double dy, double dz, double g, unsigned long long *dti)
+// Calc_dt_3D with MHD (src/hydro/hydro_cuda.cu:514-560) over the real cells, marching them
+// (MakeMhdMarch; the maximum is independent of the order the cells are visited in)
+__global__ void __launch_bounds__(256) MhdCalcDtKernel(const double *U, int nx, int ny, int nz, double dx,
+                                                       double dy, double dz, double g, mhd::MhdRegion mr,
+                                                       unsigned long long *dti)
 {
   const size_t n = size_t(nx) * ny * nz;
-  double m       = 0.0;
-  for (size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x; id < n; id += size_t(blockDim.x) * gridDim.x) {
-    const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
-    if (x > ng - 1 && x < nx - ng && y > ng - 1 && y < ny - ng && z > ng - 1 && z < nz - ng)
-      m = Fmax(m, mhd::MhdInverseCrossingTime(U, id, x, y, z, n, nx, ny, dx, dy, dz, g));
+  int x, y, z0, z1;
+  double m = 0.0;
+  if (mhd::MarchCell(mr, x, y, z0, z1)) {  // every lane reaches the workgroup reduction
+    for (int z = z0; z < z1; z++) {
+      const size_t id = size_t(x) + size_t(y) * nx + size_t(z) * size_t(nx) * ny;
+      m               = Fmax(m, mhd::MhdInverseCrossingTime(U, id, x, y, z, n, nx, ny, dx, dy, dz, g));
+    }
   }
   BlockMaxAtomic(m, dti);
 }
@@ -136,9 +140,14 @@ hipError_t LaunchMhdStepExact(const mhd::MhdArgs &a, hipStream_t s) { return Mhd
 hipError_t LaunchMhdCalcDt(const double *U, int nx, int ny, int nz, int ng, double dx, double dy, double dz,
                            double gamma, unsigned long long *dti, int num_cu, hipStream_t s)
 {
-  const size_t n = size_t(nx) * ny * nz;
-  unsigned nb    = unsigned(std::min<size_t>(MhdBlocks(n), size_t(num_cu) * 8));
-  hipLaunchKernelGGL(MhdCalcDtKernel, dim3(nb), dim3(256), 0, s, U, nx, ny, nz, ng, dx, dy, dz, gamma, dti);
+  const int Lx = nx - 2 * ng, Ly = ny - 2 * ng, Lz = nz - 2 * ng;
+  if (Lx <= 0 || Ly <= 0 || Lz <= 0) return hipSuccess;
+  // about 8 workgroups per CU, each marching a chunk of whole z planes
+  const int nxy    = (Lx * Ly + 255) / 256;
+  const int zsplit = std::max(1, std::min(Lz, num_cu * 8 / nxy));
+  const mhd::MhdRegion mr = mhd::MakeMhdMarch(ng, nx - ng - 1, ny - ng - 1, nz - ng - 1, (Lz + zsplit - 1) / zsplit);
+  hipLaunchKernelGGL(MhdCalcDtKernel, dim3(mhd::MhdRegionBlocks(mr)), dim3(256), 0, s, U, nx, ny, nz, dx, dy, dz, gamma,
+                     mr, dti);
   return hipGetLastError();
 }
 
@@ -171,7 +180,7 @@ hipError_t LaunchCtGrid(const double *Fx, const double *Fy, const double *Fz, co
                         int ny, int nz, hipStream_t s)
 {
   hipLaunchKernelGGL((mhd::MhdEmfKernel<double>), dim3(MhdBlocks(size_t(nx) * ny * nz)), dim3(256), 0, s, Fx, Fy,
-                     Fz, U, E, nx, ny, nz, 1, nx - 1, ny - 1, nz - 1, nullptr, nullptr);
+                     Fz, U, E, nx, ny, nz, 1, nx - 1, ny - 1, nz - 1);
   return hipGetLastError();
 }
 

@@ -5,12 +5,13 @@
 // full update, full face-field update: 14 launches that stream the 7-field interface
 // arrays Q_L/Q_R (6 of them) and the flux arrays through HBM.  Here:
 //
-//   MhdFlux1Kernel     PCM states + HLLD on the three faces i+1/2 of every cell
-//                      (no interface arrays: the PCM states are the cells themselves)
-//   MhdEmfKernel       CT edge fields (S&G 2009 eq. 22-24) over a region
+//   MhdFlux1AllKernel  PCM states + HLLD on the three faces i+1/2 of every cell, one
+//                      launch (no interface arrays: the PCM states are the cells themselves)
+//   MhdCtTileKernel    CT edge fields (S&G 2009 eq. 22-24) over a region, x-y tiles
+//                      marching in z with the flux values and cell terms in LDS
 //   MhdHalfKernel      dt/2 update of the hydro fields and of the face fields
-//   MhdFlux2Kernel     PLMC of the two cells of each face (in registers) + HLLD
-//   MhdEmfKernel       CT again, on the half-step state
+//   MhdFlux2TileKernel PLMC of each cell once per direction + HLLD (tiles, LDS hand-off)
+//   MhdCtTileKernel    CT again, on the half-step state
 //   MhdFullKernel      dt update of the real cells' hydro fields (density and
 //                      temperature floors fused) and of their face fields
 //   MhdCalcDtKernel    next inverse dt (mhdInverseCrossingTime)
@@ -579,95 +580,20 @@ __device__ __forceinline__ double CellEmf(const double *U, int ctDir, long ic, i
   return (mm3 * b2 - mm2 * b3) / d;
 }
 
-// The three cell-centred terms of every cell with each index in [lo, hi]: each is shared by
-// the 8 edge slopes around the cell, so the edge kernel reads them instead of recomputing
-// them (a division and 7 state loads) per slope.
-template <int kUnused = 0>  // a template, so every translation unit may instantiate it
-__global__ void __launch_bounds__(256) MhdCellEmfKernel(const double *U, double *Ec, int nx, int ny, int nz, int lo,
-                                                        int hix, int hiy, int hiz, const double *dt_dev)
-{
-  const size_t n  = size_t(nx) * ny * nz;
-  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (id >= n) return;
-  if (dt_dev && *dt_dev == 0.0) return;
-  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
-  if (x < lo || y < lo || z < lo || x > hix || y > hiy || z > hiz) return;
-  const long nU = 8 * long(n);
-#pragma unroll
-  for (int c = 0; c < 3; c++) Ec[c * n + id] = CellEmf(U, c, long(id), nx, ny, long(n), nU);
-}
-
-// The three edge fields of edge id from the cell-centred terms Ec (MhdCellEmfKernel): the
-// arithmetic of the per-slope form below, term for term.  Every index is an offset of at
-// most one cell per axis below id, inside the grid for the kernel's regions, so the loads
-// need no bounds guards.
-template <typename TF>
-__device__ __forceinline__ void EmfEdgesFromCellTerms(const TF *Fx, const TF *Fy, const TF *Fz, const double *Ec,
-                                                      double *E, long id, long n, int nx, int ny)
-{
-  const long sx = 1, sy = nx, sz = long(nx) * ny;
-  auto sh  = [&](int ax) -> long { return ax == 0 ? sx : (ax == 1 ? sy : (ax == 2 ? sz : 0)); };
-  auto flx = [&](const TF *a, long o, int f) { return double(a[id - o + f * n]); };
-  // _ctSlope with its cell-centred term read back (CtSlopeM's operands)
-  auto slope = [&](const TF *fl, double sgn, int ctd, int a, int b, int c, int d) {
-    const double ef = sgn * flx(fl, sh(c) + sh(d), kBx + int(sgn == 1));
-    return ef - Ec[ctd * n + id - sh(a) - sh(b)];
-  };
-  auto up = [](double s, auto pos_f, auto neg_f) -> double {
-    // both slopes are loaded up front: their loads overlap, and a branch per lane would
-    // serialise them behind the sign test
-    const double pos = pos_f(), neg = neg_f();
-    return s > 0.0 ? pos : (s < 0.0 ? neg : 0.5 * (pos + neg));
-  };
-#define CHL_SLOPE(fl, sg, cd, a, b, c, d) [&] { return slope(fl, sg, cd, a, b, c, d); }
-  {  // x electric field
-    const double syp = up(flx(Fz, sz, 0), CHL_SLOPE(Fy, -1, 0, 2, -1, 1, 2), CHL_SLOPE(Fy, -1, 0, -1, -1, 1, -1));
-    const double syn = up(flx(Fz, sy + sz, 0), CHL_SLOPE(Fy, -1, 0, 1, 2, 1, 2), CHL_SLOPE(Fy, -1, 0, 1, -1, 1, -1));
-    const double szp = up(flx(Fy, sy, 0), CHL_SLOPE(Fz, 1, 0, 1, -1, 1, 2), CHL_SLOPE(Fz, 1, 0, -1, -1, 2, -1));
-    const double szn = up(flx(Fy, sy + sz, 0), CHL_SLOPE(Fz, 1, 0, 1, 2, 1, 2), CHL_SLOPE(Fz, 1, 0, 2, -1, -1, 2));
-    const double fyp = +flx(Fz, sz, 6), fyn = +flx(Fz, sy + sz, 6);
-    const double fzp = -flx(Fy, sy, 5), fzn = -flx(Fy, sy + sz, 5);
-    E[id] = 0.25 * (+fyp + fyn + fzp + fzn + syp + syn + szp + szn);
-  }
-  {  // y electric field
-    const double sxp = up(flx(Fz, sz, 0), CHL_SLOPE(Fx, 1, 1, 2, -1, 0, 2), CHL_SLOPE(Fx, 1, 1, -1, -1, 0, -1));
-    const double sxn = up(flx(Fz, sx + sz, 0), CHL_SLOPE(Fx, 1, 1, 0, 2, 0, 2), CHL_SLOPE(Fx, 1, 1, 0, -1, 0, -1));
-    const double szp = up(flx(Fx, sx, 0), CHL_SLOPE(Fz, -1, 1, 0, -1, 0, 2), CHL_SLOPE(Fz, -1, 1, -1, -1, 2, -1));
-    const double szn = up(flx(Fx, sx + sz, 0), CHL_SLOPE(Fz, -1, 1, 0, 2, 0, 2), CHL_SLOPE(Fz, -1, 1, 2, -1, 2, -1));
-    const double fxp = -flx(Fz, sz, 5), fxn = -flx(Fz, sx + sz, 5);
-    const double fzp = +flx(Fx, sx, 6), fzn = +flx(Fx, sx + sz, 6);
-    E[n + id] = 0.25 * (+fxp + fxn + fzp + fzn + sxp + sxn + szp + szn);
-  }
-  {  // z electric field
-    const double syp = up(flx(Fx, sx, 0), CHL_SLOPE(Fy, 1, 2, 0, -1, 0, 1), CHL_SLOPE(Fy, 1, 2, -1, -1, 1, -1));
-    const double syn = up(flx(Fx, sx + sy, 0), CHL_SLOPE(Fy, 1, 2, 0, 1, 0, 1), CHL_SLOPE(Fy, 1, 2, 1, -1, 1, -1));
-    const double sxp = up(flx(Fy, sy, 0), CHL_SLOPE(Fx, -1, 2, 1, -1, 0, 1), CHL_SLOPE(Fx, -1, 2, -1, -1, 0, -1));
-    const double sxn = up(flx(Fy, sx + sy, 0), CHL_SLOPE(Fx, -1, 2, 0, 1, 0, 1), CHL_SLOPE(Fx, -1, 2, 0, -1, 0, -1));
-    const double fxp = +flx(Fy, sy, 6), fxn = +flx(Fy, sx + sy, 6);
-    const double fyp = -flx(Fx, sx, 5), fyn = -flx(Fx, sx + sy, 5);
-    E[2 * n + id] = 0.25 * (+fxp + fxn + fyp + fyn + sxp + sxn + syp + syn);
-  }
-#undef CHL_SLOPE
-}
-
-// mhd::Calculate_CT_Electric_Fields for edges with every index in [lo, hi] (ct_electric_fields.cu:20-279).
-// Ec: the cell-centred terms from MhdCellEmfKernel over the cells [lo - 1, hi] (every cell an
-// edge slope of the region reads), or null to compute them per slope.
+// mhd::Calculate_CT_Electric_Fields for edges with every index in [lo, hi] (ct_electric_fields.cu:20-279),
+// one lane per edge with bounds-guarded loads, in the shape of the reference's kernel: the
+// reference-vector entry point (chl_ct_electric_fields) runs it over whole grids.  The step
+// uses MhdCtTileKernel, which has the same arithmetic per edge.
 template <typename TF>
 __global__ void __launch_bounds__(256) MhdEmfKernel(const TF *Fx, const TF *Fy, const TF *Fz, const double *U,
                                                     double *E, int nx, int ny, int nz, int lo, int hix, int hiy,
-                                                    int hiz, const double *dt_dev, const double *Ec)
+                                                    int hiz)
 {
   const size_t n  = size_t(nx) * ny * nz;
   const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (id >= n) return;
-  if (dt_dev && *dt_dev == 0.0) return;
   const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
   if (x < lo || y < lo || z < lo || x > hix || y > hiy || z > hiz) return;
-  if (Ec) {
-    EmfEdgesFromCellTerms(Fx, Fy, Fz, Ec, E, long(id), long(n), nx, ny);
-    return;
-  }
   const long nF = 7 * long(n), nU = 8 * long(n);
   auto F = [&](const TF *a, int i, int j, int k, int f) {
     return AtF(a, long(i) + long(j) * nx + long(k) * long(nx) * ny + long(f) * long(n), nF);
@@ -708,13 +634,117 @@ __global__ void __launch_bounds__(256) MhdEmfKernel(const TF *Fx, const TF *Fy, 
   }
 }
 
+// Two z planes of the CT tile: for the cells [i0 - 1, i0 + CTX - 1] x [j0 - 1, j0 + CTY - 1]
+// of one plane, fields 0, 5 and 6 of each direction's fluxes (the upwind mass flux and the
+// two transverse field fluxes) and the three cell-centred terms of the slopes (CellEmf).
+template <typename TF, int CTX, int CTY>
+struct CtPlane {
+  TF f[3][3][CTY + 1][CTX + 1];
+  double ec[3][CTY + 1][CTX + 1];
+};
+
+// mhd::Calculate_CT_Electric_Fields (ct_electric_fields.cu:20-279) for the edges with every
+// index in [lo, hi], from a CTX x CTY tile of edges marching in z over a chunk of kchunk
+// planes: each plane's flux values and cell-centred terms are loaded or computed once into
+// LDS, where the 8 edges around each cell read them.  The arithmetic of each edge is that of
+// MhdEmfKernel (slopes, upwind selection and sums in the same order).  Every cell read lies
+// in [lo - 1, hi] per axis, which the flux stages and the state cover.
+template <typename TF, int CTX, int CTY>
+__global__ void __launch_bounds__(CTX *CTY) MhdCtTileKernel(const TF *Fx, const TF *Fy, const TF *Fz, const double *U,
+                                                            double *E, int nx, int ny, int nz, int lo, int hix,
+                                                            int hiy, int hiz, int kchunk, const double *dt_dev)
+{
+  __shared__ CtPlane<TF, CTX, CTY> pl[2];
+  if (dt_dev && *dt_dev == 0.0) return;
+  constexpr int NT = CTX * CTY;
+  // tiles in XCD-contiguous order (XcdItem): x fastest, then y, then the z chunk
+  const int gx = (hix - lo + CTX) / CTX, gy = (hiy - lo + CTY) / CTY, gz = (hiz - lo + kchunk) / kchunk;
+  const int q = XcdItem(int(blockIdx.x), gx * gy * gz);
+  if (q >= gx * gy * gz) return;
+  const int bx = q % gx, by = (q / gx) % gy, bz = q / (gx * gy);
+  const int tx = int(threadIdx.x) % CTX, ty = int(threadIdx.x) / CTX;
+  const int i0 = lo + bx * CTX, j0 = lo + by * CTY;
+  const int k0 = lo + bz * kchunk, k1 = min(k0 + kchunk - 1, hiz);
+  const size_t n = size_t(nx) * ny * nz, sy = size_t(nx), sz = size_t(nx) * ny;
+  const long nU  = 8 * long(n);
+  const TF *Fl[3] = {Fx, Fy, Fz};
+  auto stage = [&](int kk) {
+    CtPlane<TF, CTX, CTY> &P = pl[kk & 1];
+    for (int c = int(threadIdx.x); c < (CTX + 1) * (CTY + 1); c += NT) {
+      const int cy = c / (CTX + 1), cx = c - cy * (CTX + 1);
+      const int gi = i0 - 1 + cx, gj = j0 - 1 + cy;
+      if (gi > hix || gj > hiy) continue;  // read by no edge of the region
+      const size_t cid = size_t(gi) + size_t(gj) * sy + size_t(kk) * sz;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        P.f[d][0][cy][cx] = Fl[d][cid];
+        P.f[d][1][cy][cx] = Fl[d][size_t(kBx) * n + cid];
+        P.f[d][2][cy][cx] = Fl[d][size_t(kBx + 1) * n + cid];
+      }
+#pragma unroll
+      for (int q = 0; q < 3; q++) P.ec[q][cy][cx] = CellEmf(U, q, long(cid), nx, ny, long(n), nU);
+    }
+  };
+  const int i = i0 + tx, j = j0 + ty;
+  const bool mine = i <= hix && j <= hiy;
+  if (k0 <= k1) stage(k0 - 1);
+  for (int k = k0; k <= k1; k++) {
+    stage(k);
+    __syncthreads();
+    if (mine) {
+      // (ox, oy, oz): the cell (i - ox, j - oy, k - oz) of the edge's 2 x 2 x 2 block
+      auto flx = [&](int dir, int ox, int oy, int oz, int fi) -> double {
+        return double(pl[(k - oz) & 1].f[dir][fi][ty + 1 - oy][tx + 1 - ox]);
+      };
+      // _ctSlope with the cell-centred term read back; a..d as in MhdEmfKernel's S(): the
+      // axes (or -1) whose one-cell shifts locate the cell term (a, b) and the flux (c, d)
+      auto slope = [&](int dir, double sgn, int ctd, int a, int b, int c, int d) -> double {
+        const double ef = sgn * flx(dir, (c == 0) + (d == 0), (c == 1) + (d == 1), (c == 2) + (d == 2), 1 + int(sgn == 1));
+        const int ox = (a == 0) + (b == 0), oy = (a == 1) + (b == 1), oz = (a == 2) + (b == 2);
+        return ef - pl[(k - oz) & 1].ec[ctd][ty + 1 - oy][tx + 1 - ox];
+      };
+      auto up = [](double sgn_flux, double pos, double neg) -> double {
+        return sgn_flux > 0.0 ? pos : (sgn_flux < 0.0 ? neg : 0.5 * (pos + neg));
+      };
+      const size_t id = size_t(i) + size_t(j) * sy + size_t(k) * sz;
+      {  // x electric field
+        const double syp = up(flx(2, 0, 0, 1, 0), slope(1, -1, 0, 2, -1, 1, 2), slope(1, -1, 0, -1, -1, 1, -1));
+        const double syn = up(flx(2, 0, 1, 1, 0), slope(1, -1, 0, 1, 2, 1, 2), slope(1, -1, 0, 1, -1, 1, -1));
+        const double szp = up(flx(1, 0, 1, 0, 0), slope(2, 1, 0, 1, -1, 1, 2), slope(2, 1, 0, -1, -1, 2, -1));
+        const double szn = up(flx(1, 0, 1, 1, 0), slope(2, 1, 0, 1, 2, 1, 2), slope(2, 1, 0, 2, -1, -1, 2));
+        const double fyp = +flx(2, 0, 0, 1, 2), fyn = +flx(2, 0, 1, 1, 2);
+        const double fzp = -flx(1, 0, 1, 0, 1), fzn = -flx(1, 0, 1, 1, 1);
+        E[id] = 0.25 * (+fyp + fyn + fzp + fzn + syp + syn + szp + szn);
+      }
+      {  // y electric field
+        const double sxp = up(flx(2, 0, 0, 1, 0), slope(0, 1, 1, 2, -1, 0, 2), slope(0, 1, 1, -1, -1, 0, -1));
+        const double sxn = up(flx(2, 1, 0, 1, 0), slope(0, 1, 1, 0, 2, 0, 2), slope(0, 1, 1, 0, -1, 0, -1));
+        const double szp = up(flx(0, 1, 0, 0, 0), slope(2, -1, 1, 0, -1, 0, 2), slope(2, -1, 1, -1, -1, 2, -1));
+        const double szn = up(flx(0, 1, 0, 1, 0), slope(2, -1, 1, 0, 2, 0, 2), slope(2, -1, 1, 2, -1, 2, -1));
+        const double fxp = -flx(2, 0, 0, 1, 1), fxn = -flx(2, 1, 0, 1, 1);
+        const double fzp = +flx(0, 1, 0, 0, 2), fzn = +flx(0, 1, 0, 1, 2);
+        E[n + id] = 0.25 * (+fxp + fxn + fzp + fzn + sxp + sxn + szp + szn);
+      }
+      {  // z electric field
+        const double syp = up(flx(0, 1, 0, 0, 0), slope(1, 1, 2, 0, -1, 0, 1), slope(1, 1, 2, -1, -1, 1, -1));
+        const double syn = up(flx(0, 1, 1, 0, 0), slope(1, 1, 2, 0, 1, 0, 1), slope(1, 1, 2, 1, -1, 1, -1));
+        const double sxp = up(flx(1, 0, 1, 0, 0), slope(0, -1, 2, 1, -1, 0, 1), slope(0, -1, 2, -1, -1, 0, -1));
+        const double sxn = up(flx(1, 1, 1, 0, 0), slope(0, -1, 2, 0, 1, 0, 1), slope(0, -1, 2, 0, -1, 0, -1));
+        const double fxp = +flx(1, 0, 1, 0, 2), fxn = +flx(1, 1, 1, 0, 2);
+        const double fyp = -flx(0, 1, 0, 0, 1), fyn = -flx(0, 1, 1, 0, 1);
+        E[2 * n + id] = 0.25 * (+fxp + fxn + fyp + fyn + sxp + sxn + syp + syn);
+      }
+    }
+    __syncthreads();  // the next plane overwrites the older one
+  }
+}
+
 // ---------------------------------------------------------------------------- step arguments
 struct MhdArgs {
   double *U;          // state (8 fields, in/out)
   double *Uh;         // half-step state (8 fields)
   void *F[3];         // flux arrays (7 fields each, P::T)
   double *E;          // edge electric fields (3 fields)
-  double *Ec;         // cell-centred electric-field terms of the edge slopes (3 fields)
   int nx, ny, nz, ng;
   double dx, dy, dz, gamma;
   const double *dt_dev;  // device dt (device-timed step), or null
@@ -727,52 +757,136 @@ struct MhdArgs {
 
 __device__ __forceinline__ double MhdDt(const MhdArgs &a) { return a.dt_dev ? *a.dt_dev : a.dt; }
 
-// Stage-1 fluxes along DIR: PCM states (the cells, with their cell-centred fields) + HLLD on
-// the face i+1/2 of every cell whose face lies inside the grid.  One launch per direction.
-template <class P, int DIR>
-__global__ void __launch_bounds__(256) MhdFlux1Kernel(MhdArgs a)
+// Region launches: a 1D grid of 256-lane workgroups over the box [lo, lo + W) x [lo, lo + H)
+// x [lo, lo + D).  A row block is 256 consecutive cells of the region's x-y plane (row-major,
+// rows of W cells: no lane idles at the row ends of a region whose width is not a multiple of
+// 64).  MakeMhdSlab gives each lane one cell and orders the workgroups z-fastest within slabs of
+// kSlab planes, inside each XCD's contiguous run of items (XcdItem): a cell's z neighbours are
+// loaded by workgroups running at about the same time on the same L2, and its y neighbours
+// by the next row block's.  MakeMhdMarch gives each lane a column of kchunk planes to visit in
+// turn (the reductions, whose workgroups each fold into one atomic).
+struct MhdRegion {
+  int lo, W, H, D;
+  int kz;     // planes per slab (MakeMhdSlab) or per column (MakeMhdMarch)
+  int nxy;    // row blocks per plane
+  int items;  // workgroups
+};
+constexpr int kSlab = 8;
+
+inline MhdRegion MakeMhdRegion(int lo, int hix, int hiy, int hiz, int kz)
 {
-  using T         = typename P::T;
-  const int nx    = a.nx, ny = a.ny, nz = a.nz;
-  const size_t n  = size_t(nx) * ny * nz;
-  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (DIR == 0 && (a.flags & kFlagReduceDt) && id == 0) *a.dti = 0ull;  // MhdCalcDtKernel fills it later
-  if (id >= n) return;
-  if (MhdDt(a) == 0.0) return;
-  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
-  const int c = DIR == 0 ? x : (DIR == 1 ? y : z), nn = DIR == 0 ? nx : (DIR == 1 ? ny : nz);
-  if (c > nn - 2) return;
-  const double *U   = a.U;
-  const T g         = T(a.gamma);
-  const size_t step = DIR == 0 ? 1 : (DIR == 1 ? size_t(nx) : size_t(nx) * ny);
-  auto pcm = [&](int xx, int yy, int zz, size_t cid) {
-    double mx, my, mz, cx, cy, cz, bx, by, bz;
-    RotM(DIR, U[n + cid], U[2 * n + cid], U[3 * n + cid], mx, my, mz);
-    CellB(U, cid, xx, yy, zz, n, nx, ny, cx, cy, cz);
-    RotM(DIR, cx, cy, cz, bx, by, bz);
-    return ConsM<T>{T(U[cid]), T(mx), T(my), T(mz), T(U[4 * n + cid]), T(by), T(bz)};
+  MhdRegion m;
+  m.lo = lo, m.W = hix - lo + 1, m.H = hiy - lo + 1, m.D = hiz - lo + 1, m.kz = kz;
+  m.nxy   = (m.W * m.H + 255) / 256;
+  m.items = m.nxy * ((m.D + kz - 1) / kz) * kz;
+  return m;
+}
+inline MhdRegion MakeMhdSlab(int lo, int hix, int hiy, int hiz) { return MakeMhdRegion(lo, hix, hiy, hiz, kSlab); }
+inline MhdRegion MakeMhdMarch(int lo, int hix, int hiy, int hiz, int kchunk)
+{
+  MhdRegion m = MakeMhdRegion(lo, hix, hiy, hiz, kchunk);
+  m.items     = m.nxy * ((m.D + kchunk - 1) / kchunk);
+  return m;
+}
+inline unsigned MhdRegionBlocks(const MhdRegion &m) { return unsigned((m.items + 7) / 8 * 8); }
+
+// (x, y) of this lane in row block bxy; false past the plane's last cell.
+__device__ __forceinline__ bool RowCell(const MhdRegion &m, int bxy, int &x, int &y)
+{
+  const int idx = bxy * 256 + int(threadIdx.x);
+  if (idx >= m.W * m.H) return false;
+  const int r = idx / m.W;
+  y = m.lo + r;
+  x = m.lo + (idx - r * m.W);
+  return true;
+}
+
+// The cell of this lane in a slab launch; false if it has none.
+__device__ __forceinline__ bool SlabCell(const MhdRegion &m, int &x, int &y, int &z)
+{
+  const int q = XcdItem(int(blockIdx.x), m.items);
+  if (q >= m.items) return false;
+  const int zc = q % m.kz, r = q / m.kz;
+  const int bxy = r % m.nxy, slab = r / m.nxy;
+  z = m.lo + slab * m.kz + zc;
+  if (z >= m.lo + m.D) return false;
+  return RowCell(m, bxy, x, y);
+}
+
+// This lane's column (x, y) and z range [z0, z1) in a march launch; false if it has none.
+__device__ __forceinline__ bool MarchCell(const MhdRegion &m, int &x, int &y, int &z0, int &z1)
+{
+  const int q = XcdItem(int(blockIdx.x), m.items);
+  if (q >= m.items) return false;
+  const int bxy = q % m.nxy, bz = q / m.nxy;
+  z0 = m.lo + bz * m.kz;
+  z1 = min(z0 + m.kz, m.lo + m.D);
+  return RowCell(m, bxy, x, y);
+}
+
+// The three stage-1 faces of cell (x, y, z) = id.
+template <class P>
+__device__ __forceinline__ void Flux1Cell(const MhdArgs &a, int x, int y, int z, size_t id, size_t n, size_t sy,
+                                          size_t sz)
+{
+  using T      = typename P::T;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const double *U = a.U;
+  const T g       = T(a.gamma);
+  struct Cell {
+    double d, m[3], E, b[3];
   };
-  const ConsM<T> qL = pcm(x, y, z, id);
-  const ConsM<T> qR = pcm(x + (DIR == 0), y + (DIR == 1), z + (DIR == 2), id + step);
-  ConsM<T> f;
-  if (HlldM<P>(qL, qR, T(U[(kBx + DIR) * n + id]), g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[DIR]), id, n, DIR, f);
+  auto load = [&](int xx, int yy, int zz, size_t cid) {
+    Cell c;
+    c.d = U[cid], c.m[0] = U[n + cid], c.m[1] = U[2 * n + cid], c.m[2] = U[3 * n + cid], c.E = U[4 * n + cid];
+    CellB(U, cid, xx, yy, zz, n, nx, ny, c.b[0], c.b[1], c.b[2]);
+    return c;
+  };
+  auto pcm = [&](int dir, const Cell &c) {
+    double mx, my, mz, bx, by, bz;
+    RotM(dir, c.m[0], c.m[1], c.m[2], mx, my, mz);
+    RotM(dir, c.b[0], c.b[1], c.b[2], bx, by, bz);
+    return ConsM<T>{T(c.d), T(mx), T(my), T(mz), T(c.E), T(by), T(bz)};
+  };
+  const Cell own = load(x, y, z, id);
+  auto face = [&](int dir, int xn, int yn, int zn, size_t nid) {
+    const ConsM<T> qL = pcm(dir, own);
+    const ConsM<T> qR = pcm(dir, load(xn, yn, zn, nid));
+    ConsM<T> f;
+    if (HlldM<P>(qL, qR, T(U[(kBx + dir) * n + id]), g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[dir]), id, n, dir, f);
+  };
+  if (x <= nx - 2) face(0, x + 1, y, z, id + 1);
+  if (y <= ny - 2) face(1, x, y + 1, z, id + sy);
+  if (z <= nz - 2) face(2, x, y, z + 1, id + sz);
+}
+
+// Stage-1 fluxes of all three directions in one launch over the whole grid (MakeMhdSlab): each lane loads its cell once, forms the PCM state of every face it is the
+// left cell of (i+1/2, j+1/2, k+1/2) by rotation, and solves the three faces with its upper
+// neighbours: the PCM states (the cells with their cell-centred fields) and HLLD on the face
+// i+1/2 of every cell whose face lies inside the grid.
+template <class P>
+__global__ void __launch_bounds__(256) MhdFlux1AllKernel(MhdArgs a, MhdRegion m)
+{
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && threadIdx.x == 0) *a.dti = 0ull;  // MhdCalcDtKernel fills it
+  int x, y, z;
+  if (!SlabCell(m, x, y, z)) return;  // the slabs cover the whole grid
+  if (MhdDt(a) == 0.0) return;
+  const size_t n  = size_t(nx) * ny * nz;
+  const size_t sy = size_t(nx), sz = size_t(nx) * ny;
+  Flux1Cell<P>(a, x, y, z, size_t(x) + size_t(y) * sy + size_t(z) * sz, n, sy, sz);
 }
 
 // Half update: hydro fields of cells [1, n-2] (Update_Conserved_Variables_3D_half,
 // VL_3D_cuda.cu:379-495, with the density floor) and their face fields
 // (Update_Magnetic_Field_3D, magnetic_update.cu:21-81, from U into Uh).
 template <class P>
-__global__ void __launch_bounds__(256) MhdHalfKernel(MhdArgs a)
+__device__ __forceinline__ void HalfCell(const MhdArgs &a, double dt, int x, int y, int z)
 {
-  using T         = typename P::T;
-  const int nx    = a.nx, ny = a.ny, nz = a.nz;
-  const size_t n  = size_t(nx) * ny * nz;
-  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (id >= n) return;
-  const double dt = 0.5 * MhdDt(a);
-  if (dt == 0.0) return;
-  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
-  if (x < 1 || y < 1 || z < 1 || x > nx - 2 || y > ny - 2 || z > nz - 2) return;
+  using T        = typename P::T;
+  const int nx   = a.nx, ny = a.ny, nz = a.nz;
+  const size_t n = size_t(nx) * ny * nz;
+  const size_t id = size_t(x) + size_t(y) * nx + size_t(z) * size_t(nx) * ny;
   const double dtodx = dt / a.dx, dtody = dt / a.dy, dtodz = dt / a.dz;
   const size_t sx = 1, sy = size_t(nx), sz = size_t(nx) * ny;
   const T *Fx = reinterpret_cast<const T *>(a.F[0]), *Fy = reinterpret_cast<const T *>(a.F[1]),
@@ -811,6 +925,16 @@ __global__ void __launch_bounds__(256) MhdHalfKernel(MhdArgs a)
     const double ey2 = E[n + id + sz], ey3 = E[n + id + sx + sz];
     Uh[7 * n + id] = U[7 * n + id] + dtody * (ex3 - ex2) + dtodx * (ey2 - ey3);
   }
+}
+
+template <class P>
+__global__ void __launch_bounds__(256) MhdHalfKernel(MhdArgs a, MhdRegion m)
+{
+  int x, y, z;
+  if (!SlabCell(m, x, y, z)) return;  // the slabs cover [1, n-2]^3
+  const double dt = 0.5 * MhdDt(a);
+  if (dt == 0.0) return;
+  HalfCell<P>(a, dt, x, y, z);
 }
 
 // Stage-2 fluxes along DIR on faces with every index in [lo, hi]: PLMC of the two cells of
@@ -855,8 +979,11 @@ __global__ void __launch_bounds__(256) MhdFlux2TileKernel(MhdArgs a, int lo, int
   const size_t n = size_t(nx) * ny * a.nz;
   const int his = DIR == 2 ? hiz : hiy, hit = DIR == 2 ? hiy : hiz;  // slow and third axes' bounds
   const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
-  const int x0 = lo + int(blockIdx.x) * TW, s0 = lo + int(blockIdx.y) * TH, t = lo + int(blockIdx.z);
-  if (t > hit) return;  // uniform per workgroup
+  // tiles in XCD-contiguous order (XcdItem): x fastest, then the slow axis, then the third
+  const int gx = (hix - lo + TW) / TW, gs = (his - lo + TH) / TH, gt = hit - lo + 1;
+  const int q = XcdItem(int(blockIdx.x), gx * gs * gt);
+  if (q >= gx * gs * gt) return;  // uniform per workgroup
+  const int x0 = lo + (q % gx) * TW, s0 = lo + ((q / gx) % gs) * TH, t = lo + q / (gx * gs);
   const double *Uh = a.Uh;
   auto cell_ok = [&](int cx, int cs) {  // a cell some face of the region reads
     return cx <= hix + (DIR == 0) && cs <= his + (DIR != 0);
@@ -903,17 +1030,12 @@ __global__ void __launch_bounds__(256) MhdFlux2TileKernel(MhdArgs a, int lo, int
 // floor and the temperature floor (Temperature_Floor_Kernel, hydro_cuda.cu:1117-1153);
 // face fields whose own index is in [ng-1, n-ng-1] and other indices in [ng, n-ng-1].
 template <class P>
-__global__ void __launch_bounds__(256) MhdFullKernel(MhdArgs a)
+__device__ __forceinline__ void FullCell(const MhdArgs &a, double dt, int x, int y, int z)
 {
-  using T         = typename P::T;
-  const int nx    = a.nx, ny = a.ny, nz = a.nz, ng = a.ng;
-  const size_t n  = size_t(nx) * ny * nz;
-  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (id >= n) return;
-  const double dt = MhdDt(a);
-  if (dt == 0.0) return;
-  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
-  if (x < ng - 1 || y < ng - 1 || z < ng - 1 || x > nx - ng - 1 || y > ny - ng - 1 || z > nz - ng - 1) return;
+  using T        = typename P::T;
+  const int nx   = a.nx, ny = a.ny, nz = a.nz, ng = a.ng;
+  const size_t n = size_t(nx) * ny * nz;
+  const size_t id = size_t(x) + size_t(y) * nx + size_t(z) * size_t(nx) * ny;
   const bool rx = x >= ng, ry = y >= ng, rz = z >= ng;
   const double dtodx = dt / a.dx, dtody = dt / a.dy, dtodz = dt / a.dz;
   const size_t sx = 1, sy = size_t(nx), sz = size_t(nx) * ny;
@@ -968,6 +1090,16 @@ __global__ void __launch_bounds__(256) MhdFullKernel(MhdArgs a)
     const double ey2 = E[n + id + sz], ey3 = E[n + id + sx + sz];
     U[7 * n + id] = U[7 * n + id] + dtody * (ex3 - ex2) + dtodx * (ey2 - ey3);
   }
+}
+
+template <class P>
+__global__ void __launch_bounds__(256) MhdFullKernel(MhdArgs a, MhdRegion m)
+{
+  int x, y, z;
+  if (!SlabCell(m, x, y, z)) return;  // the slabs cover [ng-1, n-ng-1]^3
+  const double dt = MhdDt(a);
+  if (dt == 0.0) return;
+  FullCell<P>(a, dt, x, y, z);
 }
 
 // ---------------------------------------------------------------------------- dt, divergence

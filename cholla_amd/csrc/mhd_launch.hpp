@@ -15,35 +15,36 @@ inline unsigned MhdBlocks(size_t n) { return unsigned((n + 255) / 256); }
 template <class P>
 hipError_t MhdStepT(const mhd::MhdArgs &a, hipStream_t s)
 {
-  using T          = typename P::T;
-  const size_t n   = size_t(a.nx) * a.ny * a.nz;
-  const unsigned g = MhdBlocks(n);
+  using T     = typename P::T;
   const T *Fx = reinterpret_cast<const T *>(a.F[0]), *Fy = reinterpret_cast<const T *>(a.F[1]),
           *Fz = reinterpret_cast<const T *>(a.F[2]);
-  const int ng = a.ng;
-  hipLaunchKernelGGL((mhd::MhdFlux1Kernel<P, 0>), dim3(g), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((mhd::MhdFlux1Kernel<P, 1>), dim3(g), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((mhd::MhdFlux1Kernel<P, 2>), dim3(g), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(mhd::MhdCellEmfKernel<0>, dim3(g), dim3(256), 0, s, a.U, a.Ec, a.nx, a.ny, a.nz, 1, a.nx - 1,
-                     a.ny - 1, a.nz - 1, a.dt_dev);
-  hipLaunchKernelGGL((mhd::MhdEmfKernel<T>), dim3(g), dim3(256), 0, s, Fx, Fy, Fz, a.U, a.E, a.nx, a.ny, a.nz, 2,
-                     a.nx - 1, a.ny - 1, a.nz - 1, a.dt_dev, a.Ec);
-  hipLaunchKernelGGL((mhd::MhdHalfKernel<P>), dim3(g), dim3(256), 0, s, a);
+  const int nx = a.nx, ny = a.ny, nz = a.nz, ng = a.ng;
+  constexpr int CTX = 32, CTY = 8, CTZ = 32;  // edge tile and z chunk of the CT launches
+  auto ct = [&](const double *Us, int lo, int hx, int hy, int hz) {
+    const unsigned tiles = unsigned(((hx - lo + CTX) / CTX) * ((hy - lo + CTY) / CTY) * ((hz - lo + CTZ) / CTZ));
+    hipLaunchKernelGGL((mhd::MhdCtTileKernel<T, CTX, CTY>), dim3((tiles + 7) / 8 * 8), dim3(CTX * CTY), 0, s, Fx, Fy,
+                       Fz, Us, a.E, nx, ny, nz, lo, hx, hy, hz, CTZ, a.dt_dev);
+  };
+  // stage-1 faces i+1/2 of every cell with i <= n-2 (other indices over the whole grid)
+  const mhd::MhdRegion m1 = mhd::MakeMhdSlab(0, nx - 1, ny - 1, nz - 1);
+  hipLaunchKernelGGL((mhd::MhdFlux1AllKernel<P>), dim3(mhd::MhdRegionBlocks(m1)), dim3(256), 0, s, a, m1);
+  ct(a.U, 2, nx - 1, ny - 1, nz - 1);  // edges [2, n-1]
+  const mhd::MhdRegion mh = mhd::MakeMhdSlab(1, nx - 2, ny - 2, nz - 2);
+  hipLaunchKernelGGL((mhd::MhdHalfKernel<P>), dim3(mhd::MhdRegionBlocks(mh)), dim3(256), 0, s, a, mh);
   {  // stage-2 faces with every index in [ng - 1, n - ng]
-    const int lo = ng - 1, hx = a.nx - ng, hy = a.ny - ng, hz = a.nz - ng;
+    const int lo = ng - 1, hx = nx - ng, hy = ny - ng, hz = nz - ng;
     const unsigned ex = unsigned(hx - lo + 1), ey = unsigned(hy - lo + 1), ez = unsigned(hz - lo + 1);
-    hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 0>), dim3((ex + 63) / 64, (ey + 3) / 4, ez), dim3(256), 0, s, a,
-                       lo, hx, hy, hz);
-    hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 1>), dim3((ex + 63) / 64, (ey + 3) / 4, ez), dim3(256), 0, s, a,
-                       lo, hx, hy, hz);
-    hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 2>), dim3((ex + 63) / 64, (ez + 3) / 4, ey), dim3(256), 0, s, a,
-                       lo, hx, hy, hz);
+    auto blocks = [](unsigned t) { return dim3((t + 7) / 8 * 8); };
+    hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 0>), blocks((ex + 63) / 64 * ((ey + 3) / 4) * ez), dim3(256), 0,
+                       s, a, lo, hx, hy, hz);
+    hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 1>), blocks((ex + 63) / 64 * ((ey + 3) / 4) * ez), dim3(256), 0,
+                       s, a, lo, hx, hy, hz);
+    hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 2>), blocks((ex + 63) / 64 * ((ez + 3) / 4) * ey), dim3(256), 0,
+                       s, a, lo, hx, hy, hz);
   }
-  hipLaunchKernelGGL(mhd::MhdCellEmfKernel<0>, dim3(g), dim3(256), 0, s, a.Uh, a.Ec, a.nx, a.ny, a.nz, ng - 1,
-                     a.nx - ng, a.ny - ng, a.nz - ng, a.dt_dev);
-  hipLaunchKernelGGL((mhd::MhdEmfKernel<T>), dim3(g), dim3(256), 0, s, Fx, Fy, Fz, a.Uh, a.E, a.nx, a.ny, a.nz, ng,
-                     a.nx - ng, a.ny - ng, a.nz - ng, a.dt_dev, a.Ec);
-  hipLaunchKernelGGL((mhd::MhdFullKernel<P>), dim3(g), dim3(256), 0, s, a);
+  ct(a.Uh, ng, nx - ng, ny - ng, nz - ng);  // edges [ng, n-ng]
+  const mhd::MhdRegion mf = mhd::MakeMhdSlab(ng - 1, nx - ng - 1, ny - ng - 1, nz - ng - 1);
+  hipLaunchKernelGGL((mhd::MhdFullKernel<P>), dim3(mhd::MhdRegionBlocks(mf)), dim3(256), 0, s, a, mf);
   return hipGetLastError();
 }
 
