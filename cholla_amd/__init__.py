@@ -38,7 +38,7 @@ EXPORTS = [
     "chl_mhd_destroy",
     "chl_mhd_vl_step", "chl_mhd_run", "chl_mhd_time_steps", "chl_mhd_calc_dt", "chl_mhd_divergence",
     "chl_mhd_set_ghost_cells", "chl_mhd_set_boundaries",
-    "chl_hlld_fluxes", "chl_mhd_reconstruct_plmc", "chl_ct_electric_fields", "chl_update_magnetic_field",
+    "chl_hlld_fluxes", "chl_mhd_reconstruct_plmc", "chl_mhd_reconstruct_ppmc", "chl_ct_electric_fields", "chl_update_magnetic_field",
     "chl_magnetic_field_from_vector_potential", "chl_ic_orszag_tang", "chl_pack_real_cells",
     "chl_pack_real_cells_f32", "chl_unpack_real_cells", "chl_hydro_vl_step_noalloc", "chl_noalloc_scratch_bytes",
 ]
@@ -77,7 +77,7 @@ class MhdConfig(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("n_ghost", ctypes.c_int),
         ("precision", ctypes.c_int), ("density_floor", ctypes.c_int), ("temperature_floor", ctypes.c_int),
-        ("bc_flags", ctypes.c_int * 6),
+        ("bc_flags", ctypes.c_int * 6), ("reconstruction", ctypes.c_int),
     ]
 
 
@@ -155,6 +155,7 @@ def lib():
         "chl_mhd_divergence": (_i, [_vp, _i, _i, _i, _d, _d, _d, _dp]),
         "chl_hlld_fluxes": (_i, [_vp, _vp, _vp, _vp, ctypes.c_long, _d, _i]),
         "chl_mhd_reconstruct_plmc": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
+        "chl_mhd_reconstruct_ppmc": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i]),
         "chl_ct_electric_fields": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i]),
         "chl_update_magnetic_field": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d]),
         "chl_magnetic_field_from_vector_potential": (_i, [_dp, _i, _i, _i, _d, _d, _d, _dp]),
@@ -495,9 +496,10 @@ class Mhd:
     """MHD integrator handle for one box (chl_mhd_* entry points); states hold 8 fields."""
 
     def __init__(self, nx, ny, nz, n_ghost=4, precision=PREC_FP64_EXACT, bc=(1, 1, 1, 1, 1, 1), density_floor=True,
-                 temperature_floor=True):
+                 temperature_floor=True, reconstruction=None):
         cfg = MhdConfig()
         cfg.nx, cfg.ny, cfg.nz, cfg.n_ghost, cfg.precision = nx, ny, nz, n_ghost, precision
+        cfg.reconstruction = RECON_PLMC if reconstruction is None else reconstruction
         cfg.density_floor, cfg.temperature_floor = int(density_floor), int(temperature_floor)
         for q in range(6):
             cfg.bc_flags[q] = bc[q]
@@ -554,6 +556,10 @@ def hlld_fluxes(QL, QR, magX, F, n_cells, gamma, direction):
 
 def mhd_reconstruct_plmc(U, QL, QR, nx, ny, nz, gamma, direction):
     check(lib().chl_mhd_reconstruct_plmc(U.ptr, QL.ptr, QR.ptr, nx, ny, nz, gamma, direction))
+
+
+def mhd_reconstruct_ppmc(U, QL, QR, nx, ny, nz, gamma, direction):
+    check(lib().chl_mhd_reconstruct_ppmc(U.ptr, QL.ptr, QR.ptr, nx, ny, nz, gamma, direction))
 
 
 def ct_electric_fields(Fx, Fy, Fz, U, E, nx, ny, nz):

@@ -52,6 +52,8 @@ DRIVERS = {
     "dropin_driver_plmc": ["-DPLMC", "-DDENSITY_FLOOR", "-DTEMPERATURE_FLOOR"],
     "dropin_driver_ppmc": ["-DPPMC", "-DDENSITY_FLOOR", "-DTEMPERATURE_FLOOR"],
     "dropin_driver_mhd": ["-DMHD", "-DPLMC", "-DHLLD", "-DDENSITY_FLOOR", "-DTEMPERATURE_FLOOR"],
+    "dropin_driver_mhd_ppmc": ["-DMHD", "-DPPMC", "-DHLLD", "-DDENSITY_FLOOR", "-DTEMPERATURE_FLOOR"],
+    "dropin_driver_mhd_pcm": ["-DMHD", "-DPCM", "-DHLLD", "-DDENSITY_FLOOR", "-DTEMPERATURE_FLOOR"],
     "dropin_driver_ppmc_de": ["-DPPMC", "-DDE", "-DDENSITY_FLOOR", "-DTEMPERATURE_FLOOR"],
     "dropin_driver_plmc_grav": ["-DPLMC", "-DSTATIC_GRAV", "-DDENSITY_FLOOR", "-DTEMPERATURE_FLOOR"],
 }

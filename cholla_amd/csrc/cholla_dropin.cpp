@@ -16,10 +16,9 @@
 namespace
 {
 
-#if defined(MHD) && !defined(PLMC)
-  // The MHD integrator here is built with PLMC only (builds/make.type.mhd); the reference also
-  // admits PCM and PPMC with MHD (src/utils/error_handling.cpp:98-101).
-  #error "cholla_dropin: the MHD build supports -DPLMC reconstruction only"
+#if defined(MHD) && (defined(PLMP) || defined(PPMP))
+  // src/utils/error_handling.cpp:98-101: MHD admits PCM, PLMC and PPMC only
+  #error "cholla_dropin: MHD supports PCM, PLMC and PPMC reconstruction"
 #endif
 #if defined(PPMC)
 constexpr int kRecon = CHL_RECON_PPMC;
@@ -65,7 +64,8 @@ struct Slot {
 Slot g_vl3d, g_vl1d;
 
 #ifdef MHD
-// The MHD build's VL integrator (-DMHD -DPLMC -DHLLD -DVL, builds/make.type.mhd).
+// The MHD build's VL integrator (-DMHD -DHLLD -DVL with -DPLMC as in builds/make.type.mhd,
+// or -DPCM / -DPPMC).
 struct MhdSlot {
   chl_mhd *h = nullptr;
   int nx = 0, ny = 0, nz = 0, ng = 0;
@@ -81,6 +81,7 @@ chl_mhd *AcquireMhd(int nx, int ny, int nz, int ng)
   c.nx = nx, c.ny = ny, c.nz = nz, c.n_ghost = ng, c.precision = CHL_PRECISION;
   c.density_floor = kDensityFloor, c.temperature_floor = 0;
   for (int q = 0; q < 6; q++) c.bc_flags[q] = CHL_BC_PERIODIC;
+  c.reconstruction = kRecon;
   Check(chl_mhd_create(&c, &s.h), "chl_mhd_create");
   s.nx = nx, s.ny = ny, s.nz = nz, s.ng = ng;
   return s.h;

@@ -55,7 +55,11 @@ namespace
 
 int CheckMhdConfig(const chl_mhd_config &c)
 {
-  if (c.n_ghost < 4) return Fail(CHL_ERR_INVALID, "MHD PLMC needs n_ghost >= 4 (PLMC 3 + 1, src/grid/grid3D.cpp:53-76)");
+  if (c.reconstruction != CHL_RECON_PCM && c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC)
+    return Fail(CHL_ERR_UNSUPPORTED, "MHD reconstruction must be PCM, PLMC or PPMC (src/utils/error_handling.cpp:98-101)");
+  if (c.n_ghost < chl::mhd::MhdReconGhosts(c.reconstruction))
+    return Fail(CHL_ERR_INVALID, "MHD needs n_ghost >= 3 (PCM), 4 (PLMC) or 5 (PPMC): the reconstruction's ghosts + 1 "
+                                 "for the face fields (src/grid/grid3D.cpp:53-76)");
   if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
   const int ng = c.n_ghost;
   if (c.nx < 3 * ng || c.ny < 3 * ng || c.nz < 3 * ng)
@@ -86,6 +90,7 @@ chl::mhd::MhdArgs MakeArgs(chl_mhd *h, double *U, double dx, double dy, double d
   if (update_grid && c.temperature_floor && !crash) flags |= chl::kFlagTemperatureFloor;
   if (update_grid) flags |= chl::kFlagReduceDt;
   a.flags = flags;
+  a.recon = c.reconstruction;
   a.rep   = h->rep;
   return a;
 }
@@ -330,6 +335,14 @@ int chl_mhd_reconstruct_plmc(const double *U, double *QL, double *QR, int nx, in
 {
   if (!U || !QL || !QR || dir < 0 || dir > 2) return Fail(CHL_ERR_INVALID, "bad argument");
   CHL_HIP(chl::LaunchMhdPlmcGrid(U, QL, QR, nx, ny, nz, gamma, dir, nullptr));
+  CHL_HIP(hipDeviceSynchronize());
+  return CHL_OK;
+}
+
+int chl_mhd_reconstruct_ppmc(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
+{
+  if (!U || !QL || !QR || dir < 0 || dir > 2) return Fail(CHL_ERR_INVALID, "bad argument");
+  CHL_HIP(chl::LaunchMhdPpmcGrid(U, QL, QR, nx, ny, nz, gamma, dir, nullptr));
   CHL_HIP(hipDeviceSynchronize());
   return CHL_OK;
 }

@@ -76,6 +76,25 @@ __global__ void __launch_bounds__(256) HlldGridKernel(const double *QL, const do
   if (mhd::HlldM<PolicyExact>(load(QL), load(QR), magX[id], g, f)) mhd::StoreFluxM(F, size_t(id), size_t(n), dir, f);
 }
 
+// reconstruction::Write_Data with MHD (reconstruction.h:918-970) into interface arrays: the
+// conserved state at index at, momenta in their global slots, the face field of at for the
+// energy.
+__device__ __forceinline__ void WriteGridState(const double *U, const mhd::PrimM<double> &s, double *Q, size_t at,
+                                               size_t n, int dir, double g)
+{
+  mhd::ConsM<double> c = mhd::WriteDataM<PolicyExact>(s, U[(mhd::kBx + dir) * n + at], g);
+  double gx, gy, gz;
+  if (dir == 0) {
+    gx = c.mx, gy = c.my, gz = c.mz;
+  } else if (dir == 1) {
+    gy = c.mx, gz = c.my, gx = c.mz;
+  } else {
+    gz = c.mx, gx = c.my, gy = c.mz;
+  }
+  Q[at] = c.d, Q[n + at] = gx, Q[2 * n + at] = gy, Q[3 * n + at] = gz, Q[4 * n + at] = c.E;
+  Q[5 * n + at] = c.by, Q[6 * n + at] = c.bz;
+}
+
 // PLMC_cuda with MHD over a grid (plmc_cuda.cu:24-300): Q_L at i, Q_R at i-1
 __global__ void __launch_bounds__(256) MhdPlmcGridKernel(const double *U, double *QL, double *QR, int nx, int ny,
                                                          int nz, double g, int dir)
@@ -96,21 +115,32 @@ __global__ void __launch_bounds__(256) MhdPlmcGridKernel(const double *U, double
   mhd::PrimM<double> Lp, Rm;
   mhd::PlmcCellM<PolicyExact>(U, x, y, z, nx, ny, n, dir, g, Lp, Rm);
   const size_t step = dir == 0 ? 1 : (dir == 1 ? size_t(nx) : size_t(nx) * ny);
-  auto write        = [&](const mhd::PrimM<double> &s, double *Q, size_t at) {
-    mhd::ConsM<double> c = mhd::WriteDataM<PolicyExact>(s, U[(mhd::kBx + dir) * n + at], g);
-    double gx, gy, gz;
-    if (dir == 0) {
-      gx = c.mx, gy = c.my, gz = c.mz;
-    } else if (dir == 1) {
-      gy = c.mx, gz = c.my, gx = c.mz;
-    } else {
-      gz = c.mx, gx = c.my, gy = c.mz;
-    }
-    Q[at] = c.d, Q[n + at] = gx, Q[2 * n + at] = gy, Q[3 * n + at] = gz, Q[4 * n + at] = c.E;
-    Q[5 * n + at] = c.by, Q[6 * n + at] = c.bz;
-  };
-  write(Lp, QL, id);
-  write(Rm, QR, id - step);
+  WriteGridState(U, Lp, QL, id, n, dir, g);
+  WriteGridState(U, Rm, QR, id - step, n, dir, g);
+}
+
+// PPMC_VL with MHD over a grid (ppmc_cuda.cu:540-695): Q_L at i, Q_R at i-1
+__global__ void __launch_bounds__(256) MhdPpmcGridKernel(const double *U, double *QL, double *QR, int nx, int ny,
+                                                         int nz, double g, int dir)
+{
+  const size_t n  = size_t(nx) * ny * nz;
+  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
+  // reconstruction::Thread_Guard<3> (reconstruction.h:95-119)
+  bool out = x < 2 || x >= nx - 3;
+  if (ny > 1) out = y < 2 || y >= ny - 3 || out;
+  if (nz > 1) {
+    out = z < 2 || z >= nz - 3 || out;
+  } else {
+    out = z >= nz || out;
+  }
+  if (out) return;
+  mhd::PrimM<double> Lp, Rm;
+  mhd::PpmcCellM<PolicyExact>(U, x, y, z, nx, ny, n, dir, g, Lp, Rm);
+  const size_t step = dir == 0 ? 1 : (dir == 1 ? size_t(nx) : size_t(nx) * ny);
+  WriteGridState(U, Lp, QL, id, n, dir, g);
+  WriteGridState(U, Rm, QR, id - step, n, dir, g);
 }
 
 // mhd::Update_Magnetic_Field_3D over a grid (magnetic_update.cu:21-81)
@@ -172,6 +202,14 @@ hipError_t LaunchMhdPlmcGrid(const double *U, double *QL, double *QR, int nx, in
                              hipStream_t s)
 {
   hipLaunchKernelGGL(MhdPlmcGridKernel, dim3(MhdBlocks(size_t(nx) * ny * nz)), dim3(256), 0, s, U, QL, QR, nx, ny,
+                     nz, gamma, dir);
+  return hipGetLastError();
+}
+
+hipError_t LaunchMhdPpmcGrid(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir,
+                             hipStream_t s)
+{
+  hipLaunchKernelGGL(MhdPpmcGridKernel, dim3(MhdBlocks(size_t(nx) * ny * nz)), dim3(256), 0, s, U, QL, QR, nx, ny,
                      nz, gamma, dir);
   return hipGetLastError();
 }

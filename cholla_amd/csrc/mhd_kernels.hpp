@@ -325,6 +325,59 @@ __device__ __forceinline__ void PlmcCellM(const double *U, int x, int y, int z, 
   Lp.p = Fmax(Lp.p, T(kTinyNumber));
 }
 
+// PPMC_VL with MHD for one cell (ppmc_cuda.cu:540-695): the 5-cell stencil projected on the
+// cell's seven MHD characteristics, PPM_Single_Variable on each (the reduced FP32 form under
+// the mixed policy), projected back, density and pressure floored: the interface states at
+// +1/2 (Lp) and -1/2 (Rm) of cell (x,y,z) along dir.
+template <class P>
+__device__ __forceinline__ void PpmcCellM(const double *U, int x, int y, int z, int nx, int ny, size_t n, int dir,
+                                          double g, PrimM<typename P::T> &Lp, PrimM<typename P::T> &Rm)
+{
+  using T      = typename P::T;
+  const int di = dir == 0, dj = dir == 1, dk = dir == 2;
+  const PrimM<T> c0  = LoadDataM<P>(U, x, y, z, nx, ny, n, dir, g);
+  const PrimM<T> cm1 = LoadDataM<P>(U, x - di, y - dj, z - dk, nx, ny, n, dir, g);
+  const PrimM<T> cp1 = LoadDataM<P>(U, x + di, y + dj, z + dk, nx, ny, n, dir, g);
+  const PrimM<T> cm2 = LoadDataM<P>(U, x - 2 * di, y - 2 * dj, z - 2 * dk, nx, ny, n, dir, g);
+  const PrimM<T> cp2 = LoadDataM<P>(U, x + 2 * di, y + 2 * dj, z + 2 * dk, nx, ny, n, dir, g);
+  const T gg        = T(g);
+  const T inv_d     = Div1<P>(T(1.0), c0.d);
+  const T cs        = Sqrt<P>(Div<P>(gg * c0.p, c0.d, inv_d));  // Calc_Sound_Speed
+  const T cs2       = cs * cs;
+  const T inv_cs2   = Div1<P>(T(1.0), cs2);
+  const EigenM<T> e = ComputeEigenM<P>(c0, cs, cs2, gg);
+  const Char7<T> a0  = PrimToCharM<P>(c0, c0, e, cs2, inv_d, inv_cs2);
+  const Char7<T> am1 = PrimToCharM<P>(c0, cm1, e, cs2, inv_d, inv_cs2);
+  const Char7<T> am2 = PrimToCharM<P>(c0, cm2, e, cs2, inv_d, inv_cs2);
+  const Char7<T> ap1 = PrimToCharM<P>(c0, cp1, e, cs2, inv_d, inv_cs2);
+  const Char7<T> ap2 = PrimToCharM<P>(c0, cp2, e, cs2, inv_d, inv_cs2);
+  Char7<T> L, R;
+  auto ppm = [](T im2, T im1, T i0, T ip1, T ip2, T &l, T &r) {
+    if constexpr (P::kId == PolicyMixed::kId) {
+      PpmSingleReduced(im2, im1, i0, ip1, ip2, l, r);
+    } else {
+      PpmSingle<P>(im2, im1, i0, ip1, ip2, l, r);
+    }
+  };
+  ppm(am2.a0, am1.a0, a0.a0, ap1.a0, ap2.a0, L.a0, R.a0);
+  ppm(am2.a1, am1.a1, a0.a1, ap1.a1, ap2.a1, L.a1, R.a1);
+  ppm(am2.a2, am1.a2, a0.a2, ap1.a2, ap2.a2, L.a2, R.a2);
+  ppm(am2.a3, am1.a3, a0.a3, ap1.a3, ap2.a3, L.a3, R.a3);
+  ppm(am2.a4, am1.a4, a0.a4, ap1.a4, ap2.a4, L.a4, R.a4);
+  ppm(am2.a5, am1.a5, a0.a5, ap1.a5, ap2.a5, L.a5, R.a5);
+  ppm(am2.a6, am1.a6, a0.a6, ap1.a6, ap2.a6, L.a6, R.a6);
+  Lp   = CharToPrimM<P>(c0, L, e, cs2);
+  Rm   = CharToPrimM<P>(c0, R, e, cs2);
+  Rm.d = Fmax(Rm.d, T(kTinyNumber));
+  Lp.d = Fmax(Lp.d, T(kTinyNumber));
+  Rm.p = Fmax(Rm.p, T(kTinyNumber));
+  Lp.p = Fmax(Lp.p, T(kTinyNumber));
+}
+
+// Stage-2 reconstruction of the MHD build (CHL_RECON_*): ghost cells it needs, and the
+// stencil half-width of the cell reconstruction (PCM forms its states from the cells).
+__host__ __device__ constexpr int MhdReconGhosts(int recon) { return recon == 0 ? 3 : (recon == 2 ? 5 : 4); }
+
 // reconstruction::Write_Data with MHD (reconstruction.h:918-970), kept in registers:
 // conserved interface state with the face field bx for the energy
 template <class P>
@@ -752,6 +805,7 @@ struct MhdArgs {
   unsigned long long *dti;
   double density_floor, U_floor;
   int flags;  // kFlag* bits of vl3d_kernels.hpp
+  int recon;  // stage-2 reconstruction: CHL_RECON_PCM, _PLMC or _PPMC (0, 1, 2)
   RepairState rep;  // cells the full update leaves to RepairKernel (vl3d_kernels.hpp)
 };
 
@@ -877,6 +931,22 @@ __global__ void __launch_bounds__(256) MhdFlux1AllKernel(MhdArgs a, MhdRegion m)
   Flux1Cell<P>(a, x, y, z, size_t(x) + size_t(y) * sy + size_t(z) * sz, n, sy, sz);
 }
 
+// Stage-2 fluxes of the PCM build (pcm_cuda.cu:256-420 on the half-step state, then HLLD): the
+// faces i+1/2, j+1/2, k+1/2 of every cell with each index in [lo, hi] (MakeMhdSlab), the
+// interface states being the cells themselves as in stage 1.
+template <class P>
+__global__ void __launch_bounds__(256) MhdFlux2PcmKernel(MhdArgs a, MhdRegion m)
+{
+  int x, y, z;
+  if (!SlabCell(m, x, y, z)) return;
+  if (MhdDt(a) == 0.0) return;
+  const size_t n  = size_t(a.nx) * a.ny * a.nz;
+  const size_t sy = size_t(a.nx), sz = size_t(a.nx) * a.ny;
+  MhdArgs h = a;
+  h.U       = a.Uh;  // the cells of the half-step state
+  Flux1Cell<P>(h, x, y, z, size_t(x) + size_t(y) * sy + size_t(z) * sz, n, sy, sz);
+}
+
 // Half update: hydro fields of cells [1, n-2] (Update_Conserved_Variables_3D_half,
 // VL_3D_cuda.cu:379-495, with the density floor) and their face fields
 // (Update_Magnetic_Field_3D, magnetic_update.cu:21-81, from U into Uh).
@@ -961,13 +1031,14 @@ __global__ void __launch_bounds__(256) MhdFlux2Kernel(MhdArgs a, int lo, int hix
   if (HlldM<P>(qL, qR, bx, g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[DIR]), id, n, DIR, f);
 }
 
-// Stage-2 fluxes along DIR as above, with each cell's PLMC computed once: a workgroup
+// Stage-2 fluxes along DIR as above, with each cell's reconstruction (RK: 1 PLMC, 2 PPMC)
+// computed once: a workgroup
 // takes 64 (x) by 4 (the slow axis: y, or z for DIR 2) faces of one plane of the third
 // axis, reconstructs the cells of those faces plus the next layer along DIR, keeps each
 // cell's face-(i+1/2) state in registers and passes its face-(i-1/2) state to the
 // neighbour through LDS.  The per-face kernel reconstructs every cell twice (once for each
 // of its faces); the arithmetic of each reconstruction and flux is the same.
-template <class P, int DIR>
+template <class P, int DIR, int RK = 1>
 __global__ void __launch_bounds__(256) MhdFlux2TileKernel(MhdArgs a, int lo, int hix, int hiy, int hiz)
 {
   using T          = typename P::T;
@@ -990,7 +1061,11 @@ __global__ void __launch_bounds__(256) MhdFlux2TileKernel(MhdArgs a, int lo, int
   };
   auto reconstruct = [&](int cx, int cs, PrimM<T> &Lp, PrimM<T> &Rm) {
     const int y = DIR == 2 ? t : cs, z = DIR == 2 ? cs : t;
-    PlmcCellM<P>(Uh, cx, y, z, nx, ny, n, DIR, a.gamma, Lp, Rm);
+    if constexpr (RK == 2) {
+      PpmcCellM<P>(Uh, cx, y, z, nx, ny, n, DIR, a.gamma, Lp, Rm);
+    } else {
+      PlmcCellM<P>(Uh, cx, y, z, nx, ny, n, DIR, a.gamma, Lp, Rm);
+    }
   };
   PrimM<T> Lp, Rm;
   const int fx = x0 + tx, fs = s0 + ty;
@@ -1130,6 +1205,8 @@ hipError_t LaunchMhdDivergence(const double *U, int nx, int ny, int nz, double d
 hipError_t LaunchHlldGrid(const double *QL, const double *QR, const double *magX, double *F, long n_cells,
                           double gamma, int dir, hipStream_t s);
 hipError_t LaunchMhdPlmcGrid(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir,
+                             hipStream_t s);
+hipError_t LaunchMhdPpmcGrid(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir,
                              hipStream_t s);
 hipError_t LaunchCtGrid(const double *Fx, const double *Fy, const double *Fz, const double *U, double *E, int nx,
                         int ny, int nz, hipStream_t s);

@@ -264,11 +264,14 @@ int chl_ensemble_autotune(chl_ensemble *e, int steps_per_candidate, chl_ensemble
  * magnetic_x/y/z (magnetic_?[i] on the face i+1/2 of cell i; src/grid/grid_enum.h:60-66). */
 typedef struct chl_mhd_config {
   int nx, ny, nz;        /* totals incl. ghosts */
-  int n_ghost;           /* PLMC 3 + 1 for the face fields = 4 (src/grid/grid3D.cpp:53-76) */
-  int precision;         /* CHL_PREC_*; the mixed form runs PLMC and HLLD in FP32 */
+  int n_ghost;           /* >= the reconstruction's ghosts + 1 for the face fields: PCM 3, PLMC 4,
+                            PPMC 5 (src/grid/grid3D.cpp:53-76) */
+  int precision;         /* CHL_PREC_*; the mixed form runs the reconstruction and HLLD in FP32 */
   int density_floor;     /* 0/1 */
   int temperature_floor; /* 0/1 */
   int bc_flags[6];
+  int reconstruction;    /* stage 2: CHL_RECON_PCM, CHL_RECON_PLMC or CHL_RECON_PPMC
+                            (src/utils/error_handling.cpp:98-101) */
 } chl_mhd_config;
 
 typedef struct chl_mhd chl_mhd; /* owns the MHD integrator scratch of one box */
@@ -304,6 +307,9 @@ int chl_hlld_fluxes(const double *dev_bounds_L, const double *dev_bounds_R, cons
                     double *dev_flux, long n_cells, double gamma, int dir);
 /* PLMC_cuda with MHD (src/reconstruction/plmc_cuda.cu:24) */
 int chl_mhd_reconstruct_plmc(const double *dev_conserved, double *dev_bounds_L, double *dev_bounds_R, int nx, int ny,
+                             int nz, double gamma, int dir);
+/* PPMC_VL with MHD (src/reconstruction/ppmc_cuda.cu:540) */
+int chl_mhd_reconstruct_ppmc(const double *dev_conserved, double *dev_bounds_L, double *dev_bounds_R, int nx, int ny,
                              int nz, double gamma, int dir);
 /* mhd::Calculate_CT_Electric_Fields (src/mhd/ct_electric_fields.cu:20) */
 int chl_ct_electric_fields(const double *fluxX, const double *fluxY, const double *fluxZ,

@@ -51,6 +51,7 @@ void orc_set_crash_repair(int on);
 int orc_get_crash_repair();
 void orc_temperature_floor(double *U, int nx, int ny, int nz, int n_ghost, double U_floor);
 void orc_set_boundaries_mhd(double *c, int nx, int ny, int nz, int n_fields, int n_ghost, const int *flags_in);
+void orc_ppm_single_variable(double im2, double im1, double i0, double ip1, double ip2, double *L_iph, double *R_imh);
 }
 
 namespace
@@ -660,6 +661,51 @@ void orc_mhd_plmc(const double *U, double *QL, double *QR, int nx, int ny, int n
       }
 }
 
+// PPMC_VL with MHD, src/reconstruction/ppmc_cuda.cu:540-695: the 5-cell stencil projected on
+// cell i's seven MHD characteristics, PPM_Single_Variable on each, projected back, density and
+// pressure floored, written as conserved states (Q_L at i, Q_R at i-1).
+void orc_mhd_ppmc(const double *U, double *QL, double *QR, int nx, int ny, int nz, double g, int dir)
+{
+  const size_t n = size_t(nx) * ny * nz;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        // reconstruction::Thread_Guard<3>, reconstruction.h:95-119
+        bool guard = i < 2 || i >= nx - 3;
+        if (ny > 1) guard = guard || j < 2 || j >= ny - 3;
+        if (nz > 1) guard = guard || k < 2 || k >= nz - 3;
+        if (guard) continue;
+        const int di = dir == 0, dj = dir == 1, dk = dir == 2;
+        const PrimM c0  = LoadData(U, i, j, k, nx, ny, n, dir, g);
+        const PrimM cm1 = LoadData(U, i - di, j - dj, k - dk, nx, ny, n, dir, g);
+        const PrimM cp1 = LoadData(U, i + di, j + dj, k + dk, nx, ny, n, dir, g);
+        const PrimM cm2 = LoadData(U, i - 2 * di, j - 2 * dj, k - 2 * dk, nx, ny, n, dir, g);
+        const PrimM cp2 = LoadData(U, i + 2 * di, j + 2 * dj, k + 2 * dk, nx, ny, n, dir, g);
+        const double cs  = std::sqrt(g * c0.p / c0.d);  // Calc_Sound_Speed, hydro_utilities.h:209-212
+        const double cs2 = cs * cs;
+        const Eigen e    = ComputeEigen(c0, cs, cs2, g);
+        const CharM a0 = PrimToChar(c0, c0, e, cs2), am1 = PrimToChar(c0, cm1, e, cs2);
+        const CharM am2 = PrimToChar(c0, cm2, e, cs2), ap1 = PrimToChar(c0, cp1, e, cs2);
+        const CharM ap2 = PrimToChar(c0, cp2, e, cs2);
+        CharM L, R;
+        orc_ppm_single_variable(am2.a0, am1.a0, a0.a0, ap1.a0, ap2.a0, &L.a0, &R.a0);
+        orc_ppm_single_variable(am2.a1, am1.a1, a0.a1, ap1.a1, ap2.a1, &L.a1, &R.a1);
+        orc_ppm_single_variable(am2.a2, am1.a2, a0.a2, ap1.a2, ap2.a2, &L.a2, &R.a2);
+        orc_ppm_single_variable(am2.a3, am1.a3, a0.a3, ap1.a3, ap2.a3, &L.a3, &R.a3);
+        orc_ppm_single_variable(am2.a4, am1.a4, a0.a4, ap1.a4, ap2.a4, &L.a4, &R.a4);
+        orc_ppm_single_variable(am2.a5, am1.a5, a0.a5, ap1.a5, ap2.a5, &L.a5, &R.a5);
+        orc_ppm_single_variable(am2.a6, am1.a6, a0.a6, ap1.a6, ap2.a6, &L.a6, &R.a6);
+        PrimM Lp = CharToPrim(c0, L, e, cs2), Rm = CharToPrim(c0, R, e, cs2);
+        Rm.d = std::fmax(Rm.d, kTiny);
+        Lp.d = std::fmax(Lp.d, kTiny);
+        Rm.p = std::fmax(Rm.p, kTiny);
+        Lp.p = std::fmax(Lp.p, kTiny);
+        WriteData(Lp, QL, U, Idx(i, j, k, nx, ny), n, dir, g);
+        WriteData(Rm, QR, U, Idx(i - di, j - dj, k - dk, nx, ny), n, dir, g);
+      }
+}
+
 // mhd::Calculate_HLLD_Fluxes_CUDA over a grid, hlld_cuda.cu:31-164.  magX: the face
 // field normal to the interfaces (the state's magnetic_x/y/z array for dir 0/1/2).
 void orc_hlld(const double *QL, const double *QR, const double *magX, double *F, long n_cells, double g, int dir)
@@ -827,7 +873,13 @@ double orc_mhd_divergence(const double *U, int nx, int ny, int nz, double dx, do
   return m;
 }
 
-// VL_Algorithm_3D_CUDA with MHD (PLMC + HLLD + CT), src/integrators/VL_3D_cuda.cu:38-360
+// Reconstruction of the MHD step's second stage (the build's PCM / PLMC / PPMC option):
+// 0 PCM, 1 PLMC (the default), 2 PPMC.
+static int g_mhd_recon = 1;
+void orc_mhd_set_reconstruction(int recon) { g_mhd_recon = recon; }
+int orc_mhd_get_reconstruction() { return g_mhd_recon; }
+
+// VL_Algorithm_3D_CUDA with MHD (PCM, PLMC or PPMC + HLLD + CT), src/integrators/VL_3D_cuda.cu:38-360
 void orc_mhd_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, double dy, double dz, double dt, double g,
                   int density_floor_on, double density_floor)
 {
@@ -843,9 +895,15 @@ void orc_mhd_vl3d(double *U, int nx, int ny, int nz, int n_ghost, double dx, dou
   orc_update_half_3d(U, Uh, s.Fx.data(), s.Fy.data(), s.Fz.data(), nx, ny, nz, dx, dy, dz, 0.5 * dt, g, 5,
                      density_floor_on, density_floor);
   orc_update_magnetic(U, Uh, s.E.data(), nx, ny, nz, 0.5 * dt, dx, dy, dz);
-  orc_mhd_plmc(Uh, s.QLx.data(), s.QRx.data(), nx, ny, nz, g, 0);
-  orc_mhd_plmc(Uh, s.QLy.data(), s.QRy.data(), nx, ny, nz, g, 1);
-  orc_mhd_plmc(Uh, s.QLz.data(), s.QRz.data(), nx, ny, nz, g, 2);
+  if (g_mhd_recon == 0) {  // VL_3D_cuda.cu:226-229 (PCM), 239-247 (PLMC), 257-265 (PPMC)
+    orc_mhd_pcm_3d(Uh, s.QLx.data(), s.QRx.data(), s.QLy.data(), s.QRy.data(), s.QLz.data(), s.QRz.data(), nx, ny,
+                   nz);
+  } else {
+    auto recon = g_mhd_recon == 2 ? orc_mhd_ppmc : orc_mhd_plmc;
+    recon(Uh, s.QLx.data(), s.QRx.data(), nx, ny, nz, g, 0);
+    recon(Uh, s.QLy.data(), s.QRy.data(), nx, ny, nz, g, 1);
+    recon(Uh, s.QLz.data(), s.QRz.data(), nx, ny, nz, g, 2);
+  }
   orc_hlld(s.QLx.data(), s.QRx.data(), Uh + kBx * n, s.Fx.data(), long(n), g, 0);
   orc_hlld(s.QLy.data(), s.QRy.data(), Uh + kBy * n, s.Fy.data(), long(n), g, 1);
   orc_hlld(s.QLz.data(), s.QRz.data(), Uh + kBz * n, s.Fz.data(), long(n), g, 2);

@@ -95,6 +95,9 @@ def lib():
             "orc_mhd_inverse_crossing_time": (d, [d] * 13),
             "orc_mhd_pcm_3d": (None, [_dp] * 7 + [i, i, i]),
             "orc_mhd_plmc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_mhd_ppmc": (None, [_dp, _dp, _dp, i, i, i, d, i]),
+            "orc_mhd_set_reconstruction": (None, [i]),
+            "orc_mhd_get_reconstruction": (i, []),
             "orc_hlld": (None, [_dp, _dp, _dp, _dp, l, d, i]),
             "orc_ct_slope": (d, [_dp, _dp, d, i, i, i, i, i, i, i, i, i, i, l, l, l]),
             "orc_ct_electric_fields": (None, [_dp] * 5 + [i, i, i, i]),
@@ -178,6 +181,21 @@ def crash_repair(on=True):
         yield
     finally:
         lib().orc_set_crash_repair(0)
+
+
+MHD_PCM, MHD_PLMC, MHD_PPMC = 0, 1, 2
+
+
+@contextlib.contextmanager
+def mhd_reconstruction(recon):
+    """The MHD build's stage-2 reconstruction (MHD_PCM, MHD_PLMC or MHD_PPMC) for the MHD
+    step calls inside the block."""
+    prev = lib().orc_mhd_get_reconstruction()
+    lib().orc_mhd_set_reconstruction(int(recon))
+    try:
+        yield
+    finally:
+        lib().orc_mhd_set_reconstruction(prev)
 
 
 @contextlib.contextmanager

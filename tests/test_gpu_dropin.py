@@ -105,16 +105,21 @@ def test_dropin_mixed_boundaries_plmc(tmp_path):
     np.testing.assert_array_equal(A, Uo)
 
 
-def test_dropin_mhd_orszag_tang(tmp_path):
-    """The MHD build of the adapter (-DMHD -DPLMC) driven like Cholla's loop: bit-identical to
-    the MHD oracle's driver loop on an Orszag-Tang box."""
-    n, ng, g = 16, 4, 5 / 3
+@pytest.mark.parametrize("driver,recon,ng", [("dropin_driver_mhd", orc.MHD_PLMC, 4),
+                                              ("dropin_driver_mhd_ppmc", orc.MHD_PPMC, 5),
+                                              ("dropin_driver_mhd_pcm", orc.MHD_PCM, 3)])
+def test_dropin_mhd_orszag_tang(tmp_path, driver, recon, ng):
+    """The MHD builds of the adapter (-DMHD with -DPLMC, -DPPMC or -DPCM, each with the
+    reference's ghost count) driven like Cholla's loop: bit-identical to the MHD oracle's
+    driver loop on an Orszag-Tang box."""
+    n, g = 16, 5 / 3
     N = n + 2 * ng
     h = 1.0 / n
     U = chl.ic_orszag_tang(n, ng, g, seed=9)
-    A, ns, t = run_driver("dropin_driver_mhd", U, (N, N, N), ng, (h, h, h), g, 0.0, 1e30, 6, [1] * 6, tmp_path, nf=8)
+    A, ns, t = run_driver(driver, U, (N, N, N), ng, (h, h, h), g, 0.0, 1e30, 6, [1] * 6, tmp_path, nf=8)
     Uo = U.copy()
-    n_o, t_o = orc.mhd_run(Uo, N, N, N, ng, h, h, h, g, [1] * 6, 0.0, 1e30, 6)
+    with orc.mhd_reconstruction(recon):
+        n_o, t_o = orc.mhd_run(Uo, N, N, N, ng, h, h, h, g, [1] * 6, 0.0, 1e30, 6)
     assert ns == n_o == 6 and t == t_o
     np.testing.assert_array_equal(A, Uo)
 

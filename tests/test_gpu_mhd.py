@@ -92,6 +92,23 @@ def test_plmc_mhd_kernel_reference_vectors():
                 assert near(v, fid.get(str(i), 0.0)), (d, i, v)
 
 
+def test_ppmc_mhd_kernel_reference_vectors():
+    """PPMC_VL with MHD (the reference-shaped FP64 kernel) against tALLPpmcVLReconstructor's
+    MHD fiducials, all seven interface fields, both sides, three directions."""
+    g = GOLD["ppmc_mhd"]
+    nx = g["nx"]
+    n = nx ** 3
+    grid = orc.uniform(g["seed"], *g["dist"], 8 * n).copy()
+    U = chl.DeviceArray.from_host(grid)
+    for d in range(3):
+        say("PPMC-MHD reference vectors dir", d)
+        QL, QR = chl.DeviceArray(7 * n, np.zeros(7 * n)), chl.DeviceArray(7 * n, np.zeros(7 * n))
+        chl.mhd_reconstruct_ppmc(U, QL, QR, nx, nx, nx, g["gamma"], d)
+        for arr, fid in ((QL.download(), g["left"][d]), (QR.download(), g["right"][d])):
+            for i, v in enumerate(arr):
+                assert near(v, fid.get(str(i), 0.0)), (d, i, v)
+
+
 @pytest.mark.parametrize("case", ["positive", "negative", "zero"])
 def test_ct_kernel_reference_vectors(case):
     g = GOLD["ct_electric_fields"]
@@ -222,6 +239,56 @@ def test_orszag_tang_run(precision):
     assert div < 1e-11 * bmax / h_
     assert div == orc.mhd_divergence(A, N, N, N, h_, h_, h_)
     h.close()
+
+
+@pytest.mark.parametrize("recon,ng", [(chl.RECON_PPMC, 5), (chl.RECON_PCM, 3)])
+@pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
+def test_orszag_tang_reconstructions(recon, ng, precision):
+    """The MHD build's other reconstructions (ppmc_cuda.cu:540-695, pcm_cuda.cu:256-420 on the
+    half-step state), each with the reference's ghost count: Orszag-Tang 24^3 with the z
+    perturbation, 12 device-timed steps in a hipGraph against the MHD oracle's loop."""
+    n, g = 24, 5 / 3
+    N = n + 2 * ng
+    h_ = 1.0 / n
+    U = chl.ic_orszag_tang(n, ng, g, seed=11, amplitude=1e-3)
+    Uo = U.copy()
+    with orc.mhd_reconstruction(recon):
+        steps_o, t_o = orc.mhd_run(Uo, N, N, N, ng, h_, h_, h_, g, [1] * 6, 0.0, 1e30, 12)
+    say("Orszag-Tang 24^3 x12 recon", recon, "precision", precision)
+    Ud = chl.DeviceArray.from_host(U)
+    h = chl.Mhd(N, N, N, ng, precision, reconstruction=recon)
+    steps, t = h.run(Ud, h_, h_, h_, g, 0.0, 1e30, 12, use_graph=True)
+    A = Ud.download()
+    assert steps == steps_o == 12
+    l1 = mhd_l1(A, Uo, (N, N, N), ng)
+    div = chl.mhd_divergence(Ud, N, N, N, h_, h_, h_)
+    bmax = np.abs(A.reshape(8, N, N, N)[5:]).max()
+    say("  L1", ["%.2e" % v for v in l1], "t %.6g vs %.6g" % (t, t_o), "max|div B| %.2e" % div)
+    if precision == chl.PREC_FP64_EXACT:
+        assert t == t_o
+        np.testing.assert_array_equal(A, Uo)
+    else:
+        assert abs(t - t_o) <= max(TOL[precision], 1e-14) * t_o
+        if precision == chl.PREC_MIXED:
+            check_mixed(l1, OT_MIXED_MISS)
+        else:
+            assert max(l1) <= TOL[precision], l1
+    assert div < 1e-11 * bmax / h_
+    h.close()
+
+
+def test_mhd_reconstruction_ghost_rule():
+    """chl_mhd_create refuses fewer ghost cells than the reconstruction needs plus the face
+    layer (grid3D.cpp:53-76: PCM 3, PLMC 4, PPMC 5) and the reconstructions MHD does not admit
+    (error_handling.cpp:98-101)."""
+    for recon, need in ((chl.RECON_PCM, 3), (chl.RECON_PLMC, 4), (chl.RECON_PPMC, 5)):
+        N = 3 * need + 6
+        with pytest.raises(chl.ChollaError):
+            chl.Mhd(N, N, N, need - 1, chl.PREC_FP64, reconstruction=recon)
+        chl.Mhd(N, N, N, need, chl.PREC_FP64, reconstruction=recon).close()
+    for recon in (chl.RECON_PLMP, chl.RECON_PPMP):
+        with pytest.raises(chl.ChollaError):
+            chl.Mhd(24, 24, 24, 5, chl.PREC_FP64, reconstruction=recon)
 
 
 def test_graph_replay_matches_eager_mhd():

@@ -92,6 +92,60 @@ def test_plmc_mhd_vectors():
                 assert near(v, fid.get(str(i), 0.0)), (d, i, v)
 
 
+def test_ppmc_mhd_vectors():
+    """PPMC_VL with MHD against tALLPpmcVLReconstructor's MHD fiducials (one reconstructed
+    cell of a 6^3 random grid, all seven interface fields, left and right, three directions)."""
+    g = GOLD["ppmc_mhd"]
+    nx = g["nx"]
+    n = nx ** 3
+    grid = orc.uniform(g["seed"], *g["dist"], 8 * n).copy()
+    for d in range(3):
+        QL, QR = np.zeros(7 * n), np.zeros(7 * n)
+        L.orc_mhd_ppmc(P(grid), P(QL), P(QR), nx, nx, nx, g["gamma"], d)
+        for arr, fid in ((QL, g["left"][d]), (QR, g["right"][d])):
+            for i, v in enumerate(arr):
+                assert near(v, fid.get(str(i), 0.0)), (d, i, v)
+
+
+def _mhd_state(nx, d, vx, vy, vz, p, b, gamma=5.0 / 3.0):
+    """8-field MHD state on nx^3 from per-cell primitive arrays (face fields = the cell's)."""
+    n = nx ** 3
+    U = np.zeros((8, n))
+    bx, by, bz = b
+    U[0] = d
+    U[1], U[2], U[3] = d * vx, d * vy, d * vz
+    U[5], U[6], U[7] = bx, by, bz
+    U[4] = p / (gamma - 1) + 0.5 * d * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz)
+    return U.ravel().copy()
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_ppmc_mhd_closed_forms(d):
+    """PPMC-MHD pinned by what the mathematics fixes: a uniform state reproduces itself at both
+    interfaces, and a density profile linear along the sweep (velocity, pressure and field
+    uniform: only the entropy wave carries it) gives the linear interpolant d_i +- slope/2,
+    which PPM's limiter leaves alone on smooth monotone data."""
+    nx, g = 8, 5.0 / 3.0
+    n = nx ** 3
+    idx = np.indices((nx, nx, nx))  # (z, y, x)
+    coord = idx[2 - d].ravel().astype(float)
+    for dens, slope in ((np.full(n, 1.3), 0.0), (1.0 + 0.05 * coord, 0.05)):
+        U = _mhd_state(nx, dens, 0.2, -0.1, 0.3, 0.9, (0.4, 0.7, -0.5), g)
+        QL, QR = np.zeros(7 * n), np.zeros(7 * n)
+        L.orc_mhd_ppmc(P(U), P(QL), P(QR), nx, nx, nx, g, d)
+        QL, QR = QL.reshape(7, n), QR.reshape(7, n)
+        step = (1, nx, nx * nx)[d]
+        for c in range(n):
+            z, y, x = c // (nx * nx), (c // nx) % nx, c % nx
+            if min(x, y, z) < 2 or max(x, y, z) >= nx - 3:
+                continue
+            assert abs(QL[0, c] - (dens[c] + 0.5 * slope)) < 1e-13, (d, c, QL[0, c])
+            assert abs(QR[0, c - step] - (dens[c] - 0.5 * slope)) < 1e-13, (d, c, QR[0, c - step])
+            if slope == 0.0:  # every conserved slot of a uniform state comes back
+                for f in range(1, 5):
+                    assert abs(QL[f, c] - U.reshape(8, n)[f, c]) < 1e-12, (d, c, f)
+
+
 def hlld_flux(left, right, gamma, d, solver=None):
     """The reference test's Compute_Fluxes (hlld_cuda_tests.cu:58-140): rotate the sweep-frame
     momenta into the global slots of direction d, pass Bx as the face field, map back."""
