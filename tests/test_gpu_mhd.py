@@ -316,24 +316,56 @@ def test_mhd_calc_dt_matches_oracle():
     assert got == want
 
 
-def test_mhd_fast_linear_wave():
-    """mhd_system_tests.cpp fast magnetosonic wave along x (PLMC): the reference's exact step
-    count and L1 bound, on the GPU in the FP64 reference-order form."""
-    g, ng, N = 5 / 3, 4, 32
-    eig = [x / (2 * math.sqrt(5)) for x in (2, 4, -2, 0, 9, 0, 4, 0)]
-    U, (nx, ny, nz), (dx, dy, dz) = mhd_linear_wave_ic(2 * N, N, N, ng, 1.0, 0.5, 0.5, g, 1.0, 0.0, 1 / g,
+_P5 = 1.0 / (2.0 * math.sqrt(5.0))
+# mhd_system_tests.cpp:125-390 along x (pitch = yaw = 0, domain 0.5, N = 32): wave speed, the
+# reference's step count, right eigenvector (rho, mx, my, mz, E, Bx, By, Bz), background vx, and
+# runL1ErrorTest's (L1 bound, maximum-error bound) for PCM, PLMC and PPMC.
+MHD_WAVES = {
+    "fast_right": (2.0, 214, [_P5 * x for x in (2, 4, -2, 0, 9, 0, 4, 0)], 0.0,
+                   {0: (4.2e-7, 5.4e-7), 1: (6.5e-8, 6.5e-8), 2: (6.11e-8, 5.5e-8)}),
+    "fast_left": (2.0, 214, [_P5 * x for x in (2, -4, 2, 0, 9, 0, 4, 0)], 0.0,
+                  {0: (4.2e-7, 5.4e-7), 1: (6.5e-8, 6.5e-8), 2: (6.11e-8, 5.5e-8)}),
+    "slow_right": (0.5, 854, [_P5 * x for x in (4, 2, 4, 0, 3, 0, -2, 0)], 0.0,
+                   {0: (4.0e-7, 4.0e-7), 1: (2.0e-8, 2.75e-8), 2: (1.45e-9, 1.3e-9)}),
+    "slow_left": (0.5, 854, [_P5 * x for x in (4, -2, -4, 0, 3, 0, -2, 0)], 0.0,
+                  {0: (4.0e-7, 4.0e-7), 1: (2.0e-8, 2.75e-8), 2: (1.45e-9, 1.3e-9)}),
+    "alfven_right": (1.0, 427, [0, 0, 0, -1, 0, 0, 0, 1], 0.0,
+                     {0: (4.0e-7, 4.0e-7), 1: (3.0e-8, 3.0e-8), 2: (1.95e-9, 2.16e-9)}),
+    "alfven_left": (1.0, 427, [0, 0, 0, 1, 0, 0, 0, 1], 0.0,
+                    {0: (4.0e-7, 4.0e-7), 1: (3.0e-8, 3.0e-8), 2: (1.95e-9, 2.16e-9)}),
+    "contact": (1.0, 641, [1, 1, 0, 0, 0.5, 0, 0, 0], 1.0,
+                {0: (5.4e-7, 5.4e-7), 1: (3.0e-8, 3.0e-8), 2: (1.41e-9, 1.5e-9)}),
+}
+
+
+@pytest.mark.parametrize("recon", [chl.RECON_PCM, chl.RECON_PLMC, chl.RECON_PPMC])
+@pytest.mark.parametrize("wave", sorted(MHD_WAVES))
+def test_mhd_linear_waves(wave, recon):
+    """The reference's MHD linear-wave system tests along x, every wave family and every
+    reconstruction MHD admits, on the GPU in the FP64 reference-order form: the reference's
+    exact step count to t = 2 * domain / speed, then runL1ErrorTest (system_tester.cpp:207-330):
+    per-field mean |final - initial| and the norm of those below the first bound, the largest
+    pointwise difference below the second."""
+    speed, n_steps, eig, vx, bounds = MHD_WAVES[wave]
+    l1_bound, max_bound = bounds[recon]
+    g, N = 5 / 3, 32
+    ng = {chl.RECON_PCM: 3, chl.RECON_PLMC: 4, chl.RECON_PPMC: 5}[recon]
+    U, (nx, ny, nz), (dx, dy, dz) = mhd_linear_wave_ic(2 * N, N, N, ng, 1.0, 0.5, 0.5, g, 1.0, vx, 1 / g,
                                                        (1.0, 1.5, 0.0), 1e-6, eig)
     orc.set_boundaries(U, nx, ny, nz, 8, ng, [1] * 6)
     U0 = interior(U, 8, nz, ny, nx, ng).copy()
-    say("MHD fast wave 64x32x32")
+    say("MHD linear wave", wave, "recon", recon)
     Ud = chl.DeviceArray.from_host(U)
-    h = chl.Mhd(nx, ny, nz, ng, chl.PREC_FP64_EXACT)
-    steps, t = h.run(Ud, dx, dy, dz, g, 0.0, 0.5, 100000)
-    assert steps == 214
+    h = chl.Mhd(nx, ny, nz, ng, chl.PREC_FP64_EXACT, reconstruction=recon)
+    steps, t = h.run(Ud, dx, dy, dz, g, 0.0, 2 * 0.5 / speed, 100000)
+    h.close()
     U1 = interior(Ud.download(), 8, nz, ny, nx, ng)
     l1 = [np.abs(U1[f] - U0[f]).mean() for f in range(8)]
-    assert max(l1) < 6.5e-8 and math.sqrt(sum(x * x for x in l1)) < 6.5e-8, l1
-    h.close()
+    err = max(np.abs(U1[f] - U0[f]).max() for f in range(8))
+    say("  steps", steps, "L1", ["%.2e" % v for v in l1], "max %.2e" % err)
+    assert steps == n_steps
+    assert max(l1) < l1_bound and math.sqrt(sum(x * x for x in l1)) < l1_bound, l1
+    assert err < max_bound, err
 
 
 @pytest.mark.parametrize("flags", [[3] * 6, [2, 3, 1, 1, 3, 2]])
