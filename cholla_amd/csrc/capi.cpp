@@ -47,8 +47,6 @@ int CheckConfig(const chl_hydro_config &c)
     return Fail(CHL_ERR_UNSUPPORTED, "n_fields must be 5, or 6 with dual_energy (no passive scalars)");
   if (c.riemann < CHL_RIEMANN_HLLC || c.riemann > CHL_RIEMANN_EXACT)
     return Fail(CHL_ERR_UNSUPPORTED, "riemann must be HLLC, HLL, ROE or EXACT");
-  if (c.riemann != CHL_RIEMANN_HLLC && c.dual_energy)
-    return Fail(CHL_ERR_UNSUPPORTED, "the HLL, Roe and exact solvers are built without dual energy");
   if (c.reconstruction < CHL_RECON_PCM || c.reconstruction > CHL_RECON_PPMP)
     return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PCM, PLMC, PPMC, PLMP or PPMP");
   const bool simple_recon = c.reconstruction == CHL_RECON_PCM || c.reconstruction == CHL_RECON_PLMP ||

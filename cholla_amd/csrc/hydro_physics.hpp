@@ -960,9 +960,13 @@ __device__ __forceinline__ Cons<typename P::T> Hll(const RState<typename P::T> &
 // those tests adds the fourth wave's coefficient to the density only (roe_cuda.cu:330-334);
 // it is kept so, as the switch decides which flux comes out.  Expressions keep the
 // reference's association so PolicyExact reproduces its bits.
-template <class P>
+//
+// With dual energy the GasEnergy flux goes to *fge (roe_cuda.cu:168-225, 344-397): the
+// upwind side's mass flux times its specific GasEnergy, the HLLE average of
+// dge (v - b), or the Roe mass flux times the upwind specific GasEnergy.
+template <class P, bool DE = false>
 __device__ __forceinline__ Cons<typename P::T> Roe(const RState<typename P::T> &l, const RState<typename P::T> &r,
-                                                   typename P::T gamma)
+                                                   typename P::T gamma, typename P::T *fge = nullptr)
 {
   using T    = typename P::T;
   const T g1 = gamma - T(1.0);
@@ -980,8 +984,16 @@ __device__ __forceinline__ Cons<typename P::T> Roe(const RState<typename P::T> &
   const T pml = MomP<P>(l), pmr = MomP<P>(r);
   Cons<T> fl{l.mx, l.mx * l.vx + pml, l.mx * l.vy, l.mx * l.vz, (l.E + l.p) * l.vx};
   Cons<T> fr{r.mx, r.mx * r.vx + pmr, r.mx * r.vy, r.mx * r.vz, (r.E + r.p) * r.vx};
-  if (lm >= T(0.0)) return fl;
-  if (lp <= T(0.0)) return fr;
+  T gel = T(0), ger = T(0);
+  if constexpr (DE) gel = Div1<P>(l.dge, l.d), ger = Div1<P>(r.dge, r.d);
+  if (lm >= T(0.0)) {
+    if constexpr (DE) *fge = l.mx * gel;
+    return fl;
+  }
+  if (lp <= T(0.0)) {
+    if constexpr (DE) *fge = r.mx * ger;
+    return fr;
+  }
   const T dd = r.d - l.d, dmx = r.mx - l.mx, dmy = r.my - l.my, dmz = r.mz - l.mz;
   T dE = r.E - l.E;
   if constexpr (SplitPressure<P>) {
@@ -1027,12 +1039,14 @@ __device__ __forceinline__ Cons<typename P::T> Roe(const RState<typename P::T> &
     f.my = hlle_avg(l.my * (l.vx - bm), r.my * (r.vx - bp));
     f.mz = hlle_avg(l.mz * (l.vx - bm), r.mz * (r.vx - bp));
     f.E  = hlle_avg(l.E * (l.vx - bm) + l.p * l.vx, r.E * (r.vx - bp) + r.p * r.vx);
+    if constexpr (DE) *fge = hlle_avg(l.dge * (l.vx - bm), r.dge * (r.vx - bp));
   } else {
     f.d  = T(0.5) * (fl.d + fr.d - s0);
     f.mx = T(0.5) * (fl.mx + fr.mx - s1);
     f.my = T(0.5) * (fl.my + fr.my - s2);
     f.mz = T(0.5) * (fl.mz + fr.mz - s3);
     f.E  = T(0.5) * (fl.E + fr.E - s4);
+    if constexpr (DE) *fge = f.d >= T(0.0) ? f.d * gel : f.d * ger;
   }
   return f;
 }
@@ -1133,10 +1147,12 @@ __device__ __forceinline__ void ExactSample(double pm, double vm, const ExactSid
 
 // Calculate_Exact_Fluxes_CUDA body, exact_cuda.cu:82-135: the Godunov flux of the sampled
 // state, transverse velocities upwinded by the sampled velocity.  The reference's vacuum
-// warning (a printf, :95-98) is left out.
-template <class P>
+// warning (a printf, :95-98) is left out.  With dual energy the GasEnergy flux, the sampled
+// mass flux times the upwind side's specific GasEnergy, goes to *fge (exact_cuda.cu:122-133).
+template <class P, bool DE = false>
 __device__ __forceinline__ Cons<typename P::T> ExactFlux(const RState<typename P::T> &l,
-                                                         const RState<typename P::T> &r, typename P::T gamma_t)
+                                                         const RState<typename P::T> &r, typename P::T gamma_t,
+                                                         typename P::T *fge = nullptr)
 {
   using T            = typename P::T;
   const double gamma = double(gamma_t);
@@ -1156,6 +1172,10 @@ __device__ __forceinline__ Cons<typename P::T> ExactFlux(const RState<typename P
   const double vy = left ? double(l.vy) : double(r.vy), vz = left ? double(l.vz) : double(r.vz);
   const double Es = (ps / (gamma - 1.0)) + 0.5 * ds * (vs * vs + vy * vy + vz * vz);
   const double fm = SplitPressure<P> ? ds * vs * vs + (ps - p0) : ds * vs * vs + ps;
+  if constexpr (DE) {
+    const double ge = left ? double(l.dge) / double(l.d) : double(r.dge) / double(r.d);
+    *fge            = T(ds * vs * ge);
+  }
   return Cons<T>{T(ds * vs), T(fm), T(ds * vs * vy), T(ds * vs * vz), T((Es + ps) * vs)};
 }
 
@@ -1168,11 +1188,9 @@ __device__ __forceinline__ Cons<typename P::T> RiemannFlux(const RState<typename
   if constexpr (P::kSolver == kRiemannHll) {
     return Hll<P, DE>(l, r, gamma, fge);
   } else if constexpr (P::kSolver == kRiemannRoe) {
-    static_assert(!DE, "the Roe solver is built without dual energy");
-    return Roe<P>(l, r, gamma);
+    return Roe<P, DE>(l, r, gamma, fge);
   } else if constexpr (P::kSolver == kRiemannExact) {
-    static_assert(!DE, "the exact solver is built without dual energy");
-    return ExactFlux<P>(l, r, gamma);
+    return ExactFlux<P, DE>(l, r, gamma, fge);
   } else {
     return Hllc<P, DE>(l, r, gamma, fge);
   }

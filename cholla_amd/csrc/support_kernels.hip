@@ -1030,8 +1030,12 @@ hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int rieman
 {
   if (dti && n > 1024 && !de) return hipErrorInvalidValue;
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPPMP && recon != kPCM) return hipErrorInvalidValue;
-  if (de) {  // HLLC with PLMC or PPMC
-    if (riemann != kRiemannHllc || (recon != kPPMC && recon != kPLMC)) return hipErrorInvalidValue;
+  if (de) {  // every solver with PLMC or PPMC
+    if (recon != kPPMC && recon != kPLMC) return hipErrorInvalidValue;
+    if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+    if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<PolicyExact>, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+    if (riemann == kRiemannExact) return LaunchVl1dP<ExactPolicy<PolicyExact>, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+    if (riemann != kRiemannHllc) return hipErrorInvalidValue;
     return LaunchVl1dP<PolicyExact, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
   }
   if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
@@ -1090,8 +1094,15 @@ hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int re
                       const GravSpec &grav, hipStream_t s)
 {
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPPMP && recon != kPCM) return hipErrorInvalidValue;
-  if (de) {  // HLLC with PCM, PLMC or PPMC
-    if (riemann != kRiemannHllc || recon == kPLMP) return hipErrorInvalidValue;
+  if (de) {  // every solver with PCM, PLMC or PPMC
+    if (recon == kPLMP || recon == kPPMP) return hipErrorInvalidValue;
+    if (riemann == kRiemannHll)
+      return LaunchVl2dP<HllPolicy<PolicyExact>, true>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+    if (riemann == kRiemannRoe)
+      return LaunchVl2dP<RoePolicy<PolicyExact>, true>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+    if (riemann == kRiemannExact)
+      return LaunchVl2dP<ExactPolicy<PolicyExact>, true>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+    if (riemann != kRiemannHllc) return hipErrorInvalidValue;
     return LaunchVl2dP<PolicyExact, true>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   }
   if (riemann == kRiemannHll)

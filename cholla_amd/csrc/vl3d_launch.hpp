@@ -63,10 +63,11 @@ hipError_t WithSolver(int riemann, Fn &&fn)
 template <class P>
 hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 {
-  if (a.riemann != kRiemannHllc) {  // HLL, Roe, exact: the first two tile shapes, without DE
-    if (a.de || tile > 1) return hipErrorInvalidValue;
+  if (a.riemann != kRiemannHllc) {  // HLL, Roe, exact: the first two tile shapes
+    if (tile > 1) return hipErrorInvalidValue;
     return WithSolver<P>(a.riemann, [&](auto q) {
       using Q = decltype(q);
+      if (a.de) return tile == 0 ? LaunchStage1T<Q, 32, 8, true>(a, R1, s) : LaunchStage1T<Q, 16, 16, true>(a, R1, s);
       return tile == 0 ? LaunchStage1T<Q, 32, 8>(a, R1, s) : LaunchStage1T<Q, 16, 16>(a, R1, s);
     });
   }
@@ -101,11 +102,17 @@ hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
     if (a.de || tile > 1 || a.riemann != kRiemannHllc) return hipErrorInvalidValue;
     return tile == 0 ? LaunchStage2T<GravPolicy<P>, RK, 32, 8>(a, s) : LaunchStage2T<GravPolicy<P>, RK, 16, 16>(a, s);
   }
-  // PCM, PLMP, HLL, Roe and the exact solver: the first two tile shapes, without DE
+  // PCM, PLMP, PPMP, HLL, Roe and the exact solver: the first two tile shapes; dual energy
+  // with the characteristic reconstructions only
   auto narrow = [&]() {
-    if (a.de || tile > 1) return hipErrorInvalidValue;
+    if (tile > 1) return hipErrorInvalidValue;
     return WithSolver<P>(a.riemann, [&](auto q) {
       using Q = decltype(q);
+      if constexpr (RK == kPLMC || RK == kPPMC) {
+        if (a.de) return tile == 0 ? LaunchStage2T<Q, RK, 32, 8, true>(a, s) : LaunchStage2T<Q, RK, 16, 16, true>(a, s);
+      } else {
+        if (a.de) return hipErrorInvalidValue;
+      }
       return tile == 0 ? LaunchStage2T<Q, RK, 32, 8>(a, s) : LaunchStage2T<Q, RK, 16, 16>(a, s);
     });
   };

@@ -77,6 +77,26 @@ def test_2d_limits():
         chl.Hydro(40, 40, 1, 3, chl.RECON_PLMP, chl.PREC_FP64_EXACT, dual_energy=True)
 
 
+@pytest.mark.parametrize("solver", [orc.HLL, orc.ROE, orc.EXACT])
+def test_dual_energy_2d_solvers_bitwise(solver):
+    """2D DE with the HLL, Roe and exact solvers: bit-identical to the oracle's loop."""
+    from test_oracle_de_2d import warm_2d
+    ng = NG[orc.PPMC]
+    U, N = warm_2d(16, ng)
+    h = 1.0 / 32
+    Uo = U.copy()
+    s_o, t_o = orc.run(Uo, N, N, 1, ng, 6, h, h, h, G, orc.PPMC, [1, 1, 1, 1, 0, 0], 0.0, 1e30, 10,
+                       dual_energy_on=True, solver=solver)
+    say("2D DE solver", solver)
+    Ud = chl.DeviceArray.from_host(U)
+    hd = chl.Hydro(N, N, 1, ng, chl.RECON_PPMC, chl.PREC_FP64_EXACT, bc=[1] * 6, dual_energy=True,
+                   riemann=SOLVER[solver])
+    s, t = hd.run(Ud, h, h, h, G, 0.0, 1e30, 10)
+    hd.close()
+    assert s == s_o == 10 and t == t_o
+    np.testing.assert_array_equal(Ud.download(), Uo)
+
+
 @pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC])
 @pytest.mark.parametrize("case", ["warm", "cold_x", "cold_y"])

@@ -5,10 +5,11 @@ Bars:
   * PolicyExact: bit-identical to the oracle's driver loop (all six fields, ghosts
     included) on a box that exercises both sides of the DE_ETA_1 switch (a hypersonic
     slab in a slow medium), with and without a temperature floor;
-  * PolicyFast: per-field relative L1 <= 1e-12 against the oracle; PolicyMixed <= 5e-5:
+  * PolicyFast: strict per-field relative L1 <= 1e-12 against the oracle; PolicyMixed <= 5e-5:
     the slab's internal energy is 3e-4 of its total energy, so FP32 rounding of the
     energy fluxes reaches the internal energy amplified by ~3e3 (measured: 6e-6 density,
-    1.2e-5 energy and momentum, 1.3e-5 GasEnergy after 10 PPMC steps);
+    1.2e-5 energy and momentum, 1.3e-5 GasEnergy after 10 PPMC steps); m_z, the seeded
+    noise against |m_x| ~ 100, has its own bound (DE_MISS);
   * tile shapes, z-chunks and graph replay give identical bits; a uniform supersonic
     flow is a fixed point in every form; the drop-in adapter built with -DDE is
     bit-identical to the oracle.
@@ -57,13 +58,17 @@ def slab_box(n, ng, seed=3):
     return periodic_state(n, n, n, ng, d, vx, vy, vz, p)
 
 
-def oracle_run(U, dims, ng, recon, steps, U_floor=0.0):
+SOLVERS = {chl.RIEMANN_HLLC: orc.HLLC, chl.RIEMANN_HLL: orc.HLL, chl.RIEMANN_ROE: orc.ROE,
+           chl.RIEMANN_EXACT: orc.EXACT}
+
+
+def oracle_run(U, dims, ng, recon, steps, U_floor=0.0, riemann=chl.RIEMANN_HLLC):
     Uo = U.copy()
     nx, ny, nz = dims
     h = 1.0 / (nx - 2 * ng)
     dts = np.zeros(steps)
     n, t = orc.run(Uo, nx, ny, nz, ng, 6, h, h, h, G, ORC_RECON[recon], [1] * 6, 0.0, 1e30, steps, U_floor=U_floor,
-                   dt_log=dts, dual_energy_on=True)
+                   dt_log=dts, dual_energy_on=True, solver=SOLVERS[riemann])
     return Uo, n, t
 
 
@@ -95,6 +100,33 @@ def test_de_exact_matches_oracle(recon, U_floor):
     np.testing.assert_array_equal(A, Uo)
 
 
+DE_NAMES = ["density", "momentum_x", "momentum_y", "momentum_z", "Energy", "GasEnergy"]
+# Strict per-field L1 against the oracle.  On the slab box m_z is the seeded 0.02 noise against
+# |m_x| ~ 100 in the slab: FP32 rounding of the momentum fluxes lands on it (as on the KH box),
+# so in the mixed form it carries its own bound, ~3x the largest measured value (1.1e-4, HLL).
+DE_MISS = {chl.PREC_FP64: {}, chl.PREC_MIXED: {"momentum_z": 3e-4}}
+
+
+def check_de(l1, precision):
+    miss = DE_MISS[precision]
+    bad = [(DE_NAMES[f], v) for f, v in enumerate(l1) if not v <= miss.get(DE_NAMES[f], TOL[precision])]
+    assert not bad, bad
+
+
+def warm_box(n, ng, seed=3):
+    """Smooth subsonic flow with seeded noise: the thermal fraction stays far above DE_ETA_1
+    everywhere (the total-energy pressure is selected), so every solver stays in its
+    well-conditioned regime; the GasEnergy flux and the pdV source still run."""
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*(((np.arange(n) + 0.5) / n),) * 3, indexing="ij")
+    d = 1 + 0.2 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) + 0.02 * rng.uniform(-1, 1, (n, n, n))
+    vx = 0.5 * np.sin(2 * np.pi * y) + 0.05 * rng.uniform(-1, 1, (n, n, n))
+    vy = 0.3 * np.cos(2 * np.pi * z) + 0.05 * rng.uniform(-1, 1, (n, n, n))
+    vz = 0.2 * np.sin(2 * np.pi * x) + 0.05 * rng.uniform(-1, 1, (n, n, n))
+    p = 1 + 0.1 * rng.uniform(-1, 1, (n, n, n))
+    return periodic_state(n, n, n, ng, d, vx, vy, vz, p)
+
+
 @pytest.mark.parametrize("precision", [chl.PREC_FP64, chl.PREC_MIXED])
 @pytest.mark.parametrize("recon", [chl.RECON_PLMC, chl.RECON_PPMC])
 def test_de_fast_and_mixed_l1(recon, precision):
@@ -105,10 +137,48 @@ def test_de_fast_and_mixed_l1(recon, precision):
     A, ns, t = gpu_run(U, dims, ng, recon, precision, 10)
     assert ns == no == 10
     a, b = interior(A, 6, N, N, N, ng), interior(Uo, 6, N, N, N, ng)
-    mag = np.sqrt(b[1] ** 2 + b[2] ** 2 + b[3] ** 2).sum()
-    l1 = [np.abs(a[f] - b[f]).sum() / (mag if f in (1, 2, 3) else np.abs(b[f]).sum()) for f in range(6)]
-    say("DE recon", recon, "precision", precision, "L1", ["%.2e" % v for v in l1])
-    assert max(l1) <= TOL[precision], l1
+    l1 = [np.abs(a[f] - b[f]).sum() / np.abs(b[f]).sum() for f in range(6)]
+    say("DE recon", recon, "precision", precision, "strict L1", ["%.2e" % v for v in l1])
+    check_de(l1, precision)
+    assert abs(t - to) <= 1e-6 * to
+
+
+@pytest.mark.parametrize("recon", [chl.RECON_PLMC, chl.RECON_PPMC])
+@pytest.mark.parametrize("riemann", [chl.RIEMANN_HLL, chl.RIEMANN_ROE, chl.RIEMANN_EXACT])
+def test_de_solvers_exact_matches_oracle(riemann, recon):
+    """DE with the HLL, Roe and exact solvers (their GasEnergy fluxes: hll_cuda.cu, roe_cuda.cu
+    :168-225 and :344-397, exact_cuda.cu:122-140): bit-identical to the oracle's DE loop with
+    the same solver on the hypersonic-slab box, 10 steps."""
+    n, ng = 16, NG[recon]
+    U, dims = slab_box(n, ng, seed=7)
+    say("DE exact solver", riemann, "recon", recon)
+    Uo, no, to = oracle_run(U, dims, ng, recon, 10, riemann=riemann)
+    A, ns, t = gpu_run(U, dims, ng, recon, chl.PREC_FP64_EXACT, 10, riemann=riemann)
+    assert ns == no == 10 and t == to
+    np.testing.assert_array_equal(A, Uo)
+
+
+@pytest.mark.parametrize("precision", [chl.PREC_FP64, chl.PREC_MIXED])
+@pytest.mark.parametrize("riemann", [chl.RIEMANN_HLL, chl.RIEMANN_ROE, chl.RIEMANN_EXACT])
+@pytest.mark.parametrize("box", ["slab", "warm"])
+def test_de_solvers_fast_and_mixed_l1(box, riemann, precision):
+    """The same combinations in the fast FP64 and mixed forms against the oracle, strict
+    per-field L1.  Roe runs the warm box only: on the Mach-77 slab its fast and mixed forms go
+    non-finite within 10 steps (7 with DE, 10 without), where the reference-order form
+    survives -- the Roe linearisation at that Mach number sits at the edge of its HLLE
+    fallback, and the forms' different rounding tips it."""
+    if box == "slab" and riemann == chl.RIEMANN_ROE:
+        pytest.skip("Roe's fast and mixed forms are not robust at Mach 77 (see docstring)")
+    n, ng, recon = 16, NG[chl.RECON_PPMC], chl.RECON_PPMC
+    U, dims = slab_box(n, ng, seed=9) if box == "slab" else warm_box(n, ng, seed=9)
+    N = dims[0]
+    Uo, no, to = oracle_run(U, dims, ng, recon, 10, riemann=riemann)
+    A, ns, t = gpu_run(U, dims, ng, recon, precision, 10, riemann=riemann)
+    assert ns == no == 10
+    a, b = interior(A, 6, N, N, N, ng), interior(Uo, 6, N, N, N, ng)
+    l1 = [np.abs(a[f] - b[f]).sum() / np.abs(b[f]).sum() for f in range(6)]
+    say("DE", box, "solver", riemann, "precision", precision, "strict L1", ["%.2e" % v for v in l1])
+    check_de(l1, precision)
     assert abs(t - to) <= 1e-6 * to
 
 
@@ -171,8 +241,29 @@ def test_dual_energy_1d_exact_matches_oracle(case, recon, ng, use_graph):
     np.testing.assert_array_equal(Ud.download(), Uo)
 
 
+@pytest.mark.parametrize("riemann", [chl.RIEMANN_HLL, chl.RIEMANN_ROE, chl.RIEMANN_EXACT])
+def test_dual_energy_1d_solvers(riemann):
+    """1D DE with the other solvers (VL_1D_cuda.cu with -DDE and -DHLL / -DROE / -DEXACT):
+    bit-identical to the oracle's loop on the warm flow, PPMC."""
+    from test_oracle_de_1d import state_1d
+    n, ng, recon = 64, 4, chl.RECON_PPMC
+    x = (np.arange(n) + 0.5) / n
+    d, vx, p = 1.0 + 0.2 * np.sin(2 * np.pi * x), 0.5 * np.sin(4 * np.pi * x), 1.0 + 0.1 * np.cos(6 * np.pi * x)
+    U, nx = state_1d(d, vx, 0.3 * np.cos(2 * np.pi * x), 0.1 * np.ones(n), p, ng)
+    Uo = U.copy()
+    s_o, t_o = orc.run(Uo, nx, 1, 1, ng, 6, 1.0 / n, 1.0 / n, 1.0 / n, G, recon, [1, 1, 0, 0, 0, 0], 0.0, 1e30, 12,
+                       dual_energy_on=True, solver=SOLVERS[riemann])
+    say("DE 1D solver", riemann)
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(nx, 1, 1, ng, recon, chl.PREC_FP64_EXACT, bc=[1] * 6, dual_energy=True, riemann=riemann)
+    s, t = hy.run(Ud, 1.0 / n, 1.0 / n, 1.0 / n, G, 0.0, 1e30, 12)
+    hy.close()
+    assert s == s_o == 12 and t == t_o
+    np.testing.assert_array_equal(Ud.download(), Uo)
+
+
 def test_dual_energy_1d_limits():
     with pytest.raises(chl.ChollaError, match="dual energy"):
         chl.Hydro(100, 1, 1, 3, chl.RECON_PLMP, chl.PREC_FP64_EXACT, dual_energy=True)
     with pytest.raises(chl.ChollaError, match="dual energy"):
-        chl.Hydro(100, 1, 1, 4, chl.RECON_PPMC, chl.PREC_FP64_EXACT, dual_energy=True, riemann=chl.RIEMANN_ROE)
+        chl.Hydro(100, 1, 1, 4, chl.RECON_PPMP, chl.PREC_FP64_EXACT, dual_energy=True, riemann=chl.RIEMANN_ROE)
