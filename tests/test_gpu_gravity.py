@@ -83,6 +83,36 @@ def test_3d_static_gravity(precision):
     assert max(strict_l1(Uz, Uo, 5, N, N, N, ng)) > 1e-6  # the source is live
 
 
+@pytest.mark.parametrize("precision", [chl.PREC_FP64_EXACT, chl.PREC_FP64, chl.PREC_MIXED])
+def test_3d_static_gravity_dual_energy(precision):
+    """STATIC_GRAV with DE in 3D: the source on the DE build's update (GasEnergy untouched by
+    it, hydro_cuda.cu:301-315) against the oracle's DE loop with the same field."""
+    n, ng = 16, 4
+    U5, (N, _, _) = disk_box((n, n, n), ng, vamp=1e-2)
+    U5 = U5.reshape(5, -1)
+    d, ek = U5[0], 0.5 * (U5[1] ** 2 + U5[2] ** 2 + U5[3] ** 2) / U5[0]
+    U = np.concatenate([U5, (U5[4] - ek)[None]]).reshape(-1).copy()  # GasEnergy = internal energy
+    orc.set_boundaries(U, N, N, N, 6, ng, [1] * 6)
+    h, off, bnd = 0.6, (3, -2, 5), (-7.0, 2.0, -3.0)
+    say("3D static gravity + DE precision", precision)
+    Uo = U.copy()
+    with orc.static_gravity(1, off, bnd):
+        s_o, t_o = orc.run(Uo, N, N, N, ng, 6, h, h, h, G, orc.PPMC, [1] * 6, 0.0, 1e30, 5, dual_energy_on=True)
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(N, N, N, ng, chl.RECON_PPMC, precision, tile2=1, dual_energy=True)
+    hy.set_static_gravity(1, off, bnd)
+    s, t = hy.run(Ud, h, h, h, G, 0.0, 1e30, 5, use_graph=True)
+    hy.close()
+    A = Ud.download()
+    assert s == s_o == 5 and abs(t - t_o) <= TOL[precision] * t_o
+    l1 = strict_l1(A, Uo, 6, N, N, N, ng)
+    say("  strict L1", ["%.2e" % x for x in l1])
+    if precision == chl.PREC_MIXED:
+        check_mixed(l1, names=["density", "momentum_x", "momentum_y", "momentum_z", "Energy", "GasEnergy"])
+    else:
+        assert max(l1) <= TOL[precision], l1
+
+
 @pytest.mark.parametrize("grav", [1, 2, 3, 4])
 def test_2d_static_gravity(grav):
     n, ng = 24, 3
