@@ -382,9 +382,9 @@ class Ensemble:
 
     def __init__(self, dims, n_ghost, reconstruction=RECON_PPMC, precision=PREC_MIXED, launch=LAUNCH_SPLIT,
                  density_floor=True, temperature_floor=True, tile1=-1, tile2=-1, kchunk1=0, kchunk2=0,
-                 dual_energy=False, positivity_fallback=False):
+                 dual_energy=False, positivity_fallback=False, riemann=RIEMANN_HLLC):
         cfg = EnsembleConfig()
-        cfg.n_ghost, cfg.reconstruction, cfg.riemann = n_ghost, reconstruction, RIEMANN_HLLC
+        cfg.n_ghost, cfg.reconstruction, cfg.riemann = n_ghost, reconstruction, riemann
         cfg.n_fields, cfg.dual_energy = (6 if dual_energy else 5), int(dual_energy)
         cfg.positivity_fallback = int(positivity_fallback)
         cfg.precision, cfg.density_floor, cfg.temperature_floor = precision, int(density_floor), int(temperature_floor)

@@ -88,16 +88,30 @@ int CheckEnsembleConfig(const chl_ensemble_config &c)
   if (c.dual_energy != 0 && c.dual_energy != 1) return Fail(CHL_ERR_INVALID, "dual_energy must be 0 or 1");
   if (c.n_fields != 5 + c.dual_energy)
     return Fail(CHL_ERR_UNSUPPORTED, "n_fields must be 5, or 6 with dual_energy (no passive scalars)");
-  if (c.riemann != CHL_RIEMANN_HLLC) return Fail(CHL_ERR_UNSUPPORTED, "only the HLLC solver is built");
-  if (c.reconstruction != CHL_RECON_PLMC && c.reconstruction != CHL_RECON_PPMC)
-    return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PLMC or PPMC");
+  if (c.riemann < CHL_RIEMANN_HLLC || c.riemann > CHL_RIEMANN_EXACT)
+    return Fail(CHL_ERR_UNSUPPORTED, "riemann must be HLLC, HLL, ROE or EXACT");
+  if (c.reconstruction < CHL_RECON_PCM || c.reconstruction > CHL_RECON_PPMP)
+    return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PCM, PLMC, PPMC, PLMP or PPMP");
+  // the HLLC + PLMC/PPMC kernels are compiled for every tile shape and both launches; the
+  // other solvers and reconstructions for the split launch on tile shapes 0 and 1, dual energy
+  // with the characteristic reconstructions only (as for single boxes)
+  const bool simple_recon = c.reconstruction == CHL_RECON_PCM || c.reconstruction == CHL_RECON_PLMP ||
+                            c.reconstruction == CHL_RECON_PPMP;
+  if (c.riemann != CHL_RIEMANN_HLLC || simple_recon) {
+    if (c.launch != CHL_LAUNCH_SPLIT)
+      return Fail(CHL_ERR_UNSUPPORTED, "the HLL, Roe and exact solvers and PCM, PLMP and PPMP run the split launch");
+    if (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles)
+      return Fail(CHL_ERR_UNSUPPORTED,
+                  "the HLL, Roe and exact solvers and PCM, PLMP and PPMP are built for tile shapes 0 and 1 only");
+  }
+  if (simple_recon && c.dual_energy) return Fail(CHL_ERR_UNSUPPORTED, "PCM, PLMP and PPMP are built without dual energy");
   if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
   if (c.positivity_fallback != 0 && c.positivity_fallback != 1)
     return Fail(CHL_ERR_INVALID, "positivity_fallback must be 0 or 1");
   if (c.positivity_fallback && c.reconstruction != CHL_RECON_PPMC)
     return Fail(CHL_ERR_UNSUPPORTED, "the positivity fallback is built for PPMC only");
   if (c.n_ghost < ReconRadius(c.reconstruction) + 2)
-    return Fail(CHL_ERR_INVALID, "n_ghost too small for the reconstruction (PLMC needs 3, PPMC 4)");
+    return Fail(CHL_ERR_INVALID, "n_ghost too small for the reconstruction (PCM needs 2, PLMC and PLMP 3, PPMC and PPMP 4)");
   if (c.launch != CHL_LAUNCH_SPLIT && c.launch != CHL_LAUNCH_PERSISTENT) return Fail(CHL_ERR_INVALID, "bad launch");
   if (c.tile1 < 0 || c.tile1 >= chl::kNumTiles || c.tile2 < 0 || c.tile2 >= chl::kNumTiles)
     return Fail(CHL_ERR_INVALID, "tile index out of range");
@@ -266,9 +280,11 @@ chl::EnsembleArgs Args(chl_ensemble *e, int list)
   if (c.density_floor) flags |= chl::kFlagDensityFloor;
   if (c.temperature_floor) flags |= chl::kFlagTemperatureFloor;
   if (c.positivity_fallback) flags |= chl::kFlagPositivityFallback;
+  if (c.reconstruction == CHL_RECON_PCM) flags |= chl::kFlagConservedW;  // stage 1 stores momenta for PCM
   a.flags = flags, a.head = e->head, a.status = e->status(), a.spin_limit = kSpinLimit;
   for (int x = 0; x <= chl::kQueueXcds; x++) a.qoff[x] = e->qoff[x];
-  a.de = c.dual_energy;
+  a.de      = c.dual_energy;
+  a.riemann = c.riemann;  // CHL_RIEMANN_* and kRiemann* share their values
   if (list == 0) a.items = e->items, a.n_items = e->n1;
   if (list == 1) a.items = e->items + e->n1, a.n_items = e->n2;
   if (list == 2) a.items = e->items + e->n1 + e->n2, a.n_items = e->nq;
@@ -394,8 +410,11 @@ int chl_ensemble_create(const chl_ensemble_config *cfg, int n_boxes, const int *
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return Fail(CHL_ERR_NO_DEVICE, "no HIP device");
   chl_ensemble_config c = *cfg;
-  if (c.tile1 < 0) c.tile1 = c.dual_energy ? kDefaultDeTile : kDefaultTile1;
-  if (c.tile2 < 0) c.tile2 = c.dual_energy ? kDefaultDeTile : kDefaultTile2;
+  // the other solvers and reconstructions have kernels for tile shapes 0 and 1 only
+  const bool narrow = c.riemann != CHL_RIEMANN_HLLC || c.reconstruction == CHL_RECON_PCM ||
+                      c.reconstruction == CHL_RECON_PLMP || c.reconstruction == CHL_RECON_PPMP;
+  if (c.tile1 < 0) c.tile1 = c.dual_energy || narrow ? kDefaultDeTile : kDefaultTile1;
+  if (c.tile2 < 0) c.tile2 = c.dual_energy || narrow ? kDefaultDeTile : kDefaultTile2;
   if (c.kchunk1 <= 0) c.kchunk1 = kDefaultKchunk1;
   if (c.kchunk2 <= 0) c.kchunk2 = kDefaultKchunk2;
   CHL_TRY(CheckEnsembleConfig(c));
@@ -540,6 +559,7 @@ int chl_ensemble_advance_looped(chl_ensemble *e, int n_steps, float *ms_out)
         a.density_floor = e->density_floor, a.U_floor = e->U_floor;
         a.flags = Args(e, 0).flags;
         a.de     = c.dual_energy;
+        a.riemann = c.riemann;
         a.rep    = d.rep;
         a.kchunk = d.kchunk1;
         if (a.rep.side) CHL_HIP(chl::LaunchRepairSide(a, e->stream));
@@ -625,7 +645,11 @@ int chl_ensemble_autotune(chl_ensemble *e, int steps, chl_ensemble_config *chose
   chl_ensemble_config sc = orig;
   sc.launch              = CHL_LAUNCH_SPLIT;
   float b1 = std::numeric_limits<float>::infinity(), b2 = b1;
-  const int ntiles = orig.dual_energy ? chl::kNumDeTiles : chl::kNumTiles;
+  // the other solvers and reconstructions: tile shapes 0 and 1, split launch only
+  const bool narrow = orig.riemann != CHL_RIEMANN_HLLC || orig.reconstruction == CHL_RECON_PCM ||
+                      orig.reconstruction == CHL_RECON_PLMP || orig.reconstruction == CHL_RECON_PPMP;
+  const int ntiles = orig.dual_energy || narrow ? chl::kNumDeTiles : chl::kNumTiles;
+  if (narrow) sc.tile1 = std::min(sc.tile1, chl::kNumDeTiles - 1), sc.tile2 = std::min(sc.tile2, chl::kNumDeTiles - 1);
   for (int t = 0; t < ntiles && rc == CHL_OK; t++)
     for (int kc : chunks) {
       chl_ensemble_config c = sc;
@@ -651,7 +675,7 @@ int chl_ensemble_autotune(chl_ensemble *e, int steps, chl_ensemble_config *chose
   }
   // persistent launch: whole steps
   const int kc_pairs[5][2] = {{8, 16}, {8, 32}, {16, 16}, {16, 32}, {32, 32}};
-  for (int p = 0; p < chl::kNumPairs && rc == CHL_OK; p++)
+  for (int p = 0; p < chl::kNumPairs && rc == CHL_OK && !narrow; p++)
     for (const auto &kc : kc_pairs) {
       if (orig.dual_energy && !DePairIdx(p)) continue;
       chl_ensemble_config c = orig;

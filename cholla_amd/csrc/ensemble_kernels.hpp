@@ -90,6 +90,7 @@ struct EnsembleArgs {
   int qoff[kQueueXcds + 1];  // persistent queue of XCD x: items[qoff[x] .. qoff[x + 1])
   long long spin_limit;
   int de;              // dual energy (host-side dispatch only; the kernels are templated on it)
+  int riemann;         // kRiemann* (host-side dispatch only: the solver is part of the policy)
 };
 
 __device__ __forceinline__ StepArgs BoxStepArgs(const EnsembleArgs &e, int b, int kchunk)

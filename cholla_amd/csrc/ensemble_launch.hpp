@@ -49,6 +49,14 @@ hipError_t EnsDe(const EnsembleArgs &e, int tile, hipStream_t s)
 template <class P>
 hipError_t EnsStage1(const EnsembleArgs &e, int tile, hipStream_t s)
 {
+  if (e.riemann != kRiemannHllc) {  // HLL, Roe, exact: the first two tile shapes
+    if (tile > 1) return hipErrorInvalidValue;
+    return WithSolver<P>(e.riemann, [&](auto q) {
+      using Q = decltype(q);
+      if (e.de) return tile == 0 ? EnsStage1T<Q, 32, 8, true>(e, s) : EnsStage1T<Q, 16, 16, true>(e, s);
+      return tile == 0 ? EnsStage1T<Q, 32, 8>(e, s) : EnsStage1T<Q, 16, 16>(e, s);
+    });
+  }
   if (e.de) {
     switch (tile) {
       case 0:
@@ -76,7 +84,23 @@ hipError_t EnsStage1(const EnsembleArgs &e, int tile, hipStream_t s)
 template <class P, int RK>
 hipError_t EnsStage2R(const EnsembleArgs &e, int tile, hipStream_t s)
 {
-  if (e.de) {
+  // PCM, PLMP, PPMP and the HLL, Roe and exact solvers: the first two tile shapes; dual energy
+  // with the characteristic reconstructions only
+  if (RK == kPCM || RK == kPLMP || RK == kPPMP || e.riemann != kRiemannHllc) {
+    if (tile > 1) return hipErrorInvalidValue;
+    return WithSolver<P>(e.riemann, [&](auto q) {
+      using Q = decltype(q);
+      if constexpr (RK == kPLMC || RK == kPPMC) {
+        if (e.de) return tile == 0 ? EnsStage2T<Q, RK, 32, 8, true>(e, s) : EnsStage2T<Q, RK, 16, 16, true>(e, s);
+      } else {
+        if (e.de) return hipErrorInvalidValue;
+      }
+      return tile == 0 ? EnsStage2T<Q, RK, 32, 8>(e, s) : EnsStage2T<Q, RK, 16, 16>(e, s);
+    });
+  }
+  if constexpr (RK == kPCM || RK == kPLMP || RK == kPPMP) {
+    return hipErrorInvalidValue;  // handled above
+  } else if (e.de) {
     switch (tile) {
       case 0:
         return EnsStage2T<P, RK, 32, 8, true>(e, s);
@@ -84,20 +108,21 @@ hipError_t EnsStage2R(const EnsembleArgs &e, int tile, hipStream_t s)
         return EnsStage2T<P, RK, 16, 16, true>(e, s);
     }
     return hipErrorInvalidValue;
+  } else {
+    switch (tile) {
+      case 0:
+        return EnsStage2T<P, RK, 32, 8>(e, s);
+      case 1:
+        return EnsStage2T<P, RK, 16, 16>(e, s);
+      case 2:
+        return EnsStage2T<P, RK, 64, 4>(e, s);
+      case 3:
+        return EnsStage2T<P, RK, 16, 8>(e, s);
+      case 4:
+        return EnsStage2T<P, RK, 32, 16>(e, s);
+    }
+    return hipErrorInvalidValue;
   }
-  switch (tile) {
-    case 0:
-      return EnsStage2T<P, RK, 32, 8>(e, s);
-    case 1:
-      return EnsStage2T<P, RK, 16, 16>(e, s);
-    case 2:
-      return EnsStage2T<P, RK, 64, 4>(e, s);
-    case 3:
-      return EnsStage2T<P, RK, 16, 8>(e, s);
-    case 4:
-      return EnsStage2T<P, RK, 32, 16>(e, s);
-  }
-  return hipErrorInvalidValue;
 }
 
 template <class P>
@@ -105,6 +130,9 @@ hipError_t EnsStage2(const EnsembleArgs &e, int recon, int tile, hipStream_t s)
 {
   if (recon == kPPMC) return EnsStage2R<P, kPPMC>(e, tile, s);
   if (recon == kPLMC) return EnsStage2R<P, kPLMC>(e, tile, s);
+  if (recon == kPLMP) return EnsStage2R<P, kPLMP>(e, tile, s);
+  if (recon == kPPMP) return EnsStage2R<P, kPPMP>(e, tile, s);
+  if (recon == kPCM) return EnsStage2R<P, kPCM>(e, tile, s);
   return hipErrorInvalidValue;
 }
 

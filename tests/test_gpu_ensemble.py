@@ -90,6 +90,52 @@ def test_exact_ensemble_matches_oracle_per_box(launch, recon, ng):
         np.testing.assert_array_equal(out[b], Uo)
 
 
+ENS_OPTIONS = [  # (reconstruction, riemann, n_ghost, oracle reconstruction, oracle solver)
+    (chl.RECON_PPMC, chl.RIEMANN_HLL, 4, orc.PPMC, orc.HLL),
+    (chl.RECON_PPMC, chl.RIEMANN_ROE, 4, orc.PPMC, orc.ROE),
+    (chl.RECON_PLMC, chl.RIEMANN_EXACT, 3, orc.PLMC, orc.EXACT),
+    (chl.RECON_PCM, chl.RIEMANN_HLLC, 2, orc.PCM, orc.HLLC),
+    (chl.RECON_PLMP, chl.RIEMANN_HLLC, 3, orc.PLMP, orc.HLLC),
+    (chl.RECON_PPMP, chl.RIEMANN_HLL, 4, orc.PPMP, orc.HLL),
+]
+
+
+@pytest.mark.parametrize("looped", [False, True])
+@pytest.mark.parametrize("opt", range(len(ENS_OPTIONS)))
+def test_exact_ensemble_options_match_oracle_per_box(opt, looped):
+    """The make.type options inside the ensemble (split launch and the per-box loop): the HLL,
+    Roe and exact solvers and the PCM, PLMP and PPMP reconstructions, each box bit-identical to
+    the oracle's loop with the same options."""
+    recon, riemann, ng, orc_recon, orc_solver = ENS_OPTIONS[opt]
+    steps = 5
+    states, dims, dxyz = make_boxes(ng, seed0=300 + opt)
+    say("exact ensemble options", ENS_OPTIONS[opt][:3], "looped", looped)
+    out, t, ns = run_ensemble(states, dims, dxyz, ng, steps, chl.PREC_FP64_EXACT, chl.LAUNCH_SPLIT, recon,
+                              looped=looped, riemann=riemann)
+    for b, U in enumerate(states):
+        Uo = U.copy()
+        nx, ny, nz = dims[b]
+        n_o, t_o = orc.run(Uo, nx, ny, nz, ng, 5, *dxyz[b], 5 / 3, orc_recon, [1] * 6, 0.0, 1e30, steps,
+                           solver=orc_solver)
+        assert ns[b] == n_o == steps and t[b] == t_o, (b, t[b], t_o)
+        np.testing.assert_array_equal(out[b], Uo)
+
+
+def test_ensemble_option_limits():
+    dims = [(20, 20, 20)]
+    with pytest.raises(chl.ChollaError, match="split launch"):
+        chl.Ensemble(dims, 4, chl.RECON_PPMC, chl.PREC_FP64, chl.LAUNCH_PERSISTENT, riemann=chl.RIEMANN_ROE,
+                     tile1=0, tile2=0)
+    with pytest.raises(chl.ChollaError, match="tile shapes 0 and 1"):
+        chl.Ensemble(dims, 4, chl.RECON_PPMP, chl.PREC_FP64, chl.LAUNCH_SPLIT, tile1=2, tile2=1)
+    with pytest.raises(chl.ChollaError, match="dual energy"):
+        chl.Ensemble(dims, 3, chl.RECON_PLMP, chl.PREC_FP64, chl.LAUNCH_SPLIT, dual_energy=True)
+    # a tile left to the library lands on a shape the option has kernels for
+    ens = chl.Ensemble(dims, 4, chl.RECON_PPMC, chl.PREC_FP64, chl.LAUNCH_SPLIT, riemann=chl.RIEMANN_HLL)
+    assert ens.config().tile1 <= 1 and ens.config().tile2 <= 1
+    ens.close()
+
+
 @pytest.mark.parametrize("precision", [chl.PREC_FP64, chl.PREC_MIXED])
 def test_launch_forms_identical(precision):
     """Split, persistent (every compiled tile pair) and the per-box loop agree bit for bit."""
