@@ -107,7 +107,7 @@ enum : int {
   kFlagPeriodicGhosts     = 4,  // write periodic ghost images from the stage-2 epilogue
   kFlagReduceDt           = 8,
   kFlagPositivityFallback = 16,  // PPMC positivity fallback (see ReconPpmc)
-  kFlagConservedW         = 32,  // exact form, PCM: stage 1 stores momenta, not velocities
+  kFlagConservedW         = 32,  // PCM: stage 1 stores momenta (exact form) or the total-energy pressure (DE)
   kFlagCrashRepair        = 64,  // a build without both floors: mark negative / NaN cells (kRepairAll)
   kFlagResetDti           = 128, // stage 1 zeroes the inverse-dt accumulator (a later kernel fills it)
 };
@@ -474,10 +474,12 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
         // Load_Data under DE (reconstruction.h:179-191): the exact form keeps E and
         // GasEnergy and recovers p and ge per sweep; the others store p and ge
         if constexpr (P::kExact) {
+          // PCM hands the conserved half state itself to the Riemann solver (pcm_cuda.cu)
+          const bool cons_w = (a.flags & kFlagConservedW) != 0;
           W[id]          = T(uh[0]);
-          W[nc + id]     = T(uh[1] / uh[0]);
-          W[2 * nc + id] = T(uh[2] / uh[0]);
-          W[3 * nc + id] = T(uh[3] / uh[0]);
+          W[nc + id]     = cons_w ? T(uh[1]) : T(uh[1] / uh[0]);
+          W[2 * nc + id] = cons_w ? T(uh[2]) : T(uh[2] / uh[0]);
+          W[3 * nc + id] = cons_w ? T(uh[3]) : T(uh[3] / uh[0]);
           W[4 * nc + id] = T(uh[4]);
           W[5 * nc + id] = T(uh[5]);
         } else {
@@ -487,7 +489,14 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
           W[nc + id]     = T(vx);
           W[2 * nc + id] = T(vy);
           W[3 * nc + id] = T(vz);
-          store_p(PressureFromDE(uh[4], uh[4] - E_kin, uh[5], a.gamma));
+          if (a.flags & kFlagConservedW) {
+            // PCM hands the conserved state to the solver (pcm_cuda.cu): the total-energy
+            // pressure re-forms the cell's own E, and the face state's DE switch then picks the
+            // pressure Get_Pressure_From_DE would
+            store_p((uh[4] - E_kin) * (a.gamma - 1.0));
+          } else {
+            store_p(PressureFromDE(uh[4], uh[4] - E_kin, uh[5], a.gamma));
+          }
           W[kWGasEnergy<P> * nc + id] = T(uh[5] / uh[0]);
         }
       } else {
@@ -584,6 +593,8 @@ __device__ __forceinline__ void ReconDirSplit(const WCell<float, DE> *s, float g
       ReconPpmcScalar<P>(ge, *geL, *geR);
     } else if constexpr (RK == kPLMC) {
       ReconPlmcScalar<P>(ge, *geL, *geR);
+    } else if constexpr (RK == kPCM) {
+      *geL = *geR = ge[0];
     }
   }
   // PPMC's positivity fallback (see ReconPpmc): the cell-centred values on both sides
@@ -614,6 +625,7 @@ __device__ __forceinline__ void ReconDirPlain(const WCell<typename P::T, DE> *s,
     Rot<DIR>(s[m].vx, s[m].vy, s[m].vz, st[m].vx, st[m].vy, st[m].vz);
     if constexpr (RK == kPCM) {
       st[m].p = s[m].p_or_E;  // exact form: (d, momenta, E) carried through as they are
+      if constexpr (DE) ge[m] = s[m].g;  // exact form: the conserved GasEnergy, else the specific one
     } else if constexpr (P::kExact && (RK == kPLMP || RK == kPPMP)) {
       // plmp_cuda.cu:113-120 (ppmp_cuda.cu:160-171): the squared velocities summed left to right
       const T v2 = (st[m].vx * st[m].vx + st[m].vy * st[m].vy) + st[m].vz * st[m].vz;
@@ -633,6 +645,7 @@ __device__ __forceinline__ void ReconDirPlain(const WCell<typename P::T, DE> *s,
   }
   if constexpr (RK == kPCM) {
     L = R = st[0];
+    if constexpr (DE) *geL = *geR = ge[0];
   } else if constexpr (RK == kPLMP) {
     ReconPlmp<P>(st, L, R);
   } else if constexpr (RK == kPPMP) {
@@ -674,7 +687,8 @@ __device__ __forceinline__ RState<typename P::T> IfaceState(const Prim<typename 
 {
   using T = typename P::T;
   if constexpr (P::kExact && RK == kPCM) {
-    return RStateFromCons<P>(q.d, q.vx, q.vy, q.vz, q.p, gamma);  // q holds (d, momenta, E)
+    // q holds (d, momenta, E) and ge the conserved GasEnergy
+    return RStateFromCons<P, DE>(q.d, q.vx, q.vy, q.vz, q.p, gamma, DE ? ge : T(0));
   } else if constexpr (P::kExact && (RK == kPLMP || RK == kPPMP)) {  // ppmp_cuda.cu:645-650 likewise
     const T E = q.p / (gamma - T(1.0)) + T(0.5) * q.d * ((q.vx * q.vx + q.vy * q.vy) + q.vz * q.vz);
     return RStateFromCons<P>(q.d, q.d * q.vx, q.d * q.vy, q.d * q.vz, E, gamma);

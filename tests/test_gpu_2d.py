@@ -98,7 +98,7 @@ def test_dual_energy_2d_solvers_bitwise(solver):
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC])
+@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC, orc.PCM])
 @pytest.mark.parametrize("case", ["warm", "cold_x", "cold_y"])
 def test_dual_energy_2d_bitwise(case, recon, use_graph):
     """2D DE (VL_2D_cuda.cu:140-152, 203-252): a warm flow and cold compressions along each axis."""

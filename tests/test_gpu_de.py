@@ -27,8 +27,8 @@ from test_oracle_de import periodic_state
 pytestmark = pytest.mark.gpu
 G = 5.0 / 3.0
 TOL = {chl.PREC_FP64_EXACT: 0.0, chl.PREC_FP64: 1e-12, chl.PREC_MIXED: 5e-5}
-NG = {chl.RECON_PLMC: 3, chl.RECON_PPMC: 4}
-ORC_RECON = {chl.RECON_PLMC: orc.PLMC, chl.RECON_PPMC: orc.PPMC}
+NG = {chl.RECON_PCM: 2, chl.RECON_PLMC: 3, chl.RECON_PPMC: 4}
+ORC_RECON = {chl.RECON_PCM: orc.PCM, chl.RECON_PLMC: orc.PLMC, chl.RECON_PPMC: orc.PPMC}
 
 
 def say(*a):
@@ -84,7 +84,7 @@ def gpu_run(U, dims, ng, recon, precision, steps, U_floor=0.0, use_graph=True, *
 
 
 @pytest.mark.parametrize("U_floor", [0.0, 1.5])
-@pytest.mark.parametrize("recon", [chl.RECON_PLMC, chl.RECON_PPMC])
+@pytest.mark.parametrize("recon", [chl.RECON_PCM, chl.RECON_PLMC, chl.RECON_PPMC])
 def test_de_exact_matches_oracle(recon, U_floor):
     n, ng = 16, NG[recon]
     U, dims = slab_box(n, ng)
@@ -128,7 +128,7 @@ def warm_box(n, ng, seed=3):
 
 
 @pytest.mark.parametrize("precision", [chl.PREC_FP64, chl.PREC_MIXED])
-@pytest.mark.parametrize("recon", [chl.RECON_PLMC, chl.RECON_PPMC])
+@pytest.mark.parametrize("recon", [chl.RECON_PCM, chl.RECON_PLMC, chl.RECON_PPMC])
 def test_de_fast_and_mixed_l1(recon, precision):
     n, ng = 16, NG[recon]
     U, dims = slab_box(n, ng, seed=5)
@@ -267,3 +267,5 @@ def test_dual_energy_1d_limits():
         chl.Hydro(100, 1, 1, 3, chl.RECON_PLMP, chl.PREC_FP64_EXACT, dual_energy=True)
     with pytest.raises(chl.ChollaError, match="dual energy"):
         chl.Hydro(100, 1, 1, 4, chl.RECON_PPMP, chl.PREC_FP64_EXACT, dual_energy=True, riemann=chl.RIEMANN_ROE)
+    with pytest.raises(chl.ChollaError, match="dual energy"):
+        chl.Hydro(100, 1, 1, 2, chl.RECON_PCM, chl.PREC_FP64_EXACT, dual_energy=True)

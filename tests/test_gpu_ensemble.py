@@ -130,6 +130,31 @@ def test_ensemble_option_limits():
         chl.Ensemble(dims, 4, chl.RECON_PPMP, chl.PREC_FP64, chl.LAUNCH_SPLIT, tile1=2, tile2=1)
     with pytest.raises(chl.ChollaError, match="dual energy"):
         chl.Ensemble(dims, 3, chl.RECON_PLMP, chl.PREC_FP64, chl.LAUNCH_SPLIT, dual_energy=True)
+
+
+def test_pcm_de_ensemble_matches_oracle_per_box():
+    """PCM with dual energy inside the ensemble (split launch), each box bit-identical to the
+    oracle's DE loop."""
+    from test_oracle_de import periodic_state
+    ng, steps = 2, 5
+    states, dims, dxyz = [], [], []
+    for b, r in enumerate(REAL[:3]):
+        rng = np.random.default_rng(500 + b)
+        shape = (r[2], r[1], r[0])
+        d = 1 + 0.1 * rng.uniform(-1, 1, shape)
+        v = 0.5 * rng.uniform(-1, 1, (3,) + shape)
+        p = 1 + 0.1 * rng.uniform(-1, 1, shape)
+        U, dm = periodic_state(r[0], r[1], r[2], ng, d, v[0], v[1], v[2], p)
+        states.append(U), dims.append(dm), dxyz.append((1.0 / r[0], 1.0 / r[1], 1.0 / r[2]))
+    out, t, ns = run_ensemble(states, dims, dxyz, ng, steps, chl.PREC_FP64_EXACT, chl.LAUNCH_SPLIT, chl.RECON_PCM,
+                              dual_energy=True)
+    for b, U in enumerate(states):
+        Uo = U.copy()
+        nx, ny, nz = dims[b]
+        n_o, t_o = orc.run(Uo, nx, ny, nz, ng, 6, *dxyz[b], 5 / 3, orc.PCM, [1] * 6, 0.0, 1e30, steps,
+                           dual_energy_on=True)
+        assert ns[b] == n_o == steps and t[b] == t_o
+        np.testing.assert_array_equal(out[b], Uo)
     # a tile left to the library lands on a shape the option has kernels for
     ens = chl.Ensemble(dims, 4, chl.RECON_PPMC, chl.PREC_FP64, chl.LAUNCH_SPLIT, riemann=chl.RIEMANN_HLL)
     assert ens.config().tile1 <= 1 and ens.config().tile2 <= 1
