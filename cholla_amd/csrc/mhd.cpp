@@ -29,7 +29,6 @@ struct chl_mhd {
   double *Uh         = nullptr;
   void *F            = nullptr;  // 3 x 7 fields of the flux element type
   size_t F_elem      = 8;
-  double *E          = nullptr;
   unsigned char *scal = nullptr;  // [0] dti bits, [1] dt, [2] t, [3] n_step; [4] divergence bits
   void *rep_buf      = nullptr;  // the repair marks and counts (chl::RepairState)
   chl::RepairState rep{};
@@ -76,7 +75,7 @@ chl::mhd::MhdArgs MakeArgs(chl_mhd *h, double *U, double dx, double dy, double d
 {
   const chl_mhd_config &c = h->cfg;
   chl::mhd::MhdArgs a{};
-  a.U = U, a.Uh = h->Uh, a.E = h->E;
+  a.U = U, a.Uh = h->Uh;
   for (int d = 0; d < 3; d++) a.F[d] = static_cast<unsigned char *>(h->F) + size_t(d) * 7 * h->n_cells * h->F_elem;
   a.nx = c.nx, a.ny = c.ny, a.nz = c.nz, a.ng = c.n_ghost;
   a.dx = dx, a.dy = dy, a.dz = dz, a.gamma = gamma, a.dt = dt_host, a.dt_dev = dt_dev, a.dti = h->dti();
@@ -115,27 +114,29 @@ int RepairStep(chl_mhd *h, const chl::mhd::MhdArgs &m, bool update_grid)
   return CHL_OK;
 }
 
-// dt from dti on the device, the step, the next dti, the ghost refresh
+// The step with its own dt: the stage-1 flux launch reduces dti from the state it reads
+// (the state the reference's Calc_dt reads at the end of the previous step: the repairs and
+// floors have run, and the ghost refresh changes no real cell), the device clock turns it into
+// dt and clears it, then the rest of the step, the repairs and the ghost refresh.
 int EnqueueDeviceStep(chl_mhd *h, double *U, double dx, double dy, double dz, double gamma, double density_floor,
                       double U_floor, double tout)
 {
   const chl_mhd_config &c = h->cfg;
-  CHL_HIP(chl::LaunchDtAdvance(h->dti(), h->dt(), h->t(), tout, kCfl, h->nstep(), h->stream));
   chl::mhd::MhdArgs a = MakeArgs(h, U, dx, dy, dz, gamma, 0.0, h->dt(), density_floor, U_floor, true);
+  a.flags |= chl::kFlagFoldDt;
+  a.clk_dt = h->dt(), a.clk_t = h->t(), a.clk_nstep = h->nstep(), a.clk_tout = tout, a.clk_cfl = kCfl;
   CHL_HIP(chl::LaunchMhdStep(c.precision, a, h->stream));
   CHL_TRY(RepairStep(h, a, true));
-  CHL_HIP(chl::LaunchMhdCalcDt(U, c.nx, c.ny, c.nz, c.n_ghost, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
   CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, 8, c.n_ghost, c.bc_flags, h->stream, true));
   return CHL_OK;
 }
 
-int InitialDti(chl_mhd *h, const double *U, double dx, double dy, double dz, double gamma, double t0)
+// Device clock at t0 with dti = 0, which the first step's flux launch fills.
+int InitialDti(chl_mhd *h, double t0)
 {
-  const chl_mhd_config &c = h->cfg;
-  unsigned char buf[32]   = {};
+  unsigned char buf[32] = {};
   std::memcpy(buf + 16, &t0, 8);
   CHL_HIP(hipMemcpyAsync(h->scal, buf, 32, hipMemcpyHostToDevice, h->stream));
-  CHL_HIP(chl::LaunchMhdCalcDt(U, c.nx, c.ny, c.nz, c.n_ghost, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
   return CHL_OK;
 }
 
@@ -187,16 +188,14 @@ int chl_mhd_create(const chl_mhd_config *cfg, chl_mhd **out)
   hipError_t e   = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&h->Uh, 8 * n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->F, 21 * n * h->F_elem);
-  if (e == hipSuccess) e = hipMalloc(&h->E, 3 * n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&h->scal), 64);
   const size_t rep_bytes = chl_capi::RepairBytesFor(cfg->nx, cfg->ny, cfg->nz, cfg->n_ghost, 0);
   if (e == hipSuccess) e = hipMalloc(&h->rep_buf, rep_bytes);
   if (e == hipSuccess) e = hipMemsetAsync(h->rep_buf, 0, rep_bytes, h->stream);
   if (e == hipSuccess) h->rep = chl_capi::RepairAt(h->rep_buf, cfg->nx, cfg->ny, cfg->nz, cfg->n_ghost, 0);
-  // cleared once: cells and edges outside the computed regions keep these values
+  // cleared once: cells and faces outside the computed regions keep these values
   if (e == hipSuccess) e = hipMemsetAsync(h->Uh, 0, 8 * n * sizeof(double), h->stream);
   if (e == hipSuccess) e = hipMemsetAsync(h->F, 0, 21 * n * h->F_elem, h->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(h->E, 0, 3 * n * sizeof(double), h->stream);
   if (e == hipSuccess) e = hipMemsetAsync(h->scal, 0, 64, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) {
@@ -214,7 +213,6 @@ int chl_mhd_destroy(chl_mhd *h)
   h->DropGraph();
   if (h->Uh) (void)hipFree(h->Uh);
   if (h->F) (void)hipFree(h->F);
-  if (h->E) (void)hipFree(h->E);
   if (h->scal) (void)hipFree(h->scal);
   if (h->rep_buf) (void)hipFree(h->rep_buf);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -237,7 +235,7 @@ int chl_mhd_run(chl_mhd *h, double *U, double dx, double dy, double dz, double g
                 double U_floor, double t0, double tout, int max_steps, int use_graph, double *t_out, int *steps_out)
 {
   if (!h || !U) return Fail(CHL_ERR_INVALID, "null argument");
-  CHL_TRY(InitialDti(h, U, dx, dy, dz, gamma, t0));
+  CHL_TRY(InitialDti(h, t0));
   if (use_graph) CHL_TRY(EnsureGraph(h, U, dx, dy, dz, gamma, density_floor, U_floor, tout));
   double t = t0;
   int done = 0;
@@ -266,7 +264,7 @@ int chl_mhd_time_steps(chl_mhd *h, double *U, double dx, double dy, double dz, d
 {
   if (!h || !U || n_steps < 0) return Fail(CHL_ERR_INVALID, "bad argument");
   const double tout = std::numeric_limits<double>::infinity();
-  CHL_TRY(InitialDti(h, U, dx, dy, dz, gamma, 0.0));
+  CHL_TRY(InitialDti(h, 0.0));
   if (use_graph) CHL_TRY(EnsureGraph(h, U, dx, dy, dz, gamma, 0.0, 0.0, tout));
   hipEvent_t e0, e1;
   CHL_HIP(hipEventCreate(&e0));

@@ -8,20 +8,7 @@ namespace chl
 namespace
 {
 
-// Block maximum of non-negative doubles folded into *out with one atomic per block.
-__device__ __forceinline__ void BlockMaxAtomic(double m, unsigned long long *out)
-{
-  __shared__ double red[1024 / 64];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = Fmax(m, __shfl_xor(m, off, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = red[0];
-    for (int w = 1; w < int(blockDim.x) / 64; w++) b = Fmax(b, red[w]);
-    atomicMax(out, (unsigned long long)__double_as_longlong(b));
-  }
-}
+using mhd::BlockMaxAtomic;
 
 // Calc_dt_3D with MHD (src/hydro/hydro_cuda.cu:514-560) over the real cells, marching them
 // (MakeMhdMarch; the maximum is independent of the order the cells are visited in)

@@ -6,15 +6,16 @@
 // arrays Q_L/Q_R (6 of them) and the flux arrays through HBM.  Here:
 //
 //   MhdFlux1AllKernel  PCM states + HLLD on the three faces i+1/2 of every cell, one
-//                      launch (no interface arrays: the PCM states are the cells themselves)
-//   MhdCtTileKernel    CT edge fields (S&G 2009 eq. 22-24) over a region, x-y tiles
-//                      marching in z with the flux values and cell terms in LDS
-//   MhdHalfKernel      dt/2 update of the hydro fields and of the face fields
-//   MhdFlux2TileKernel PLMC of each cell once per direction + HLLD (tiles, LDS hand-off)
-//   MhdCtTileKernel    CT again, on the half-step state
-//   MhdFullKernel      dt update of the real cells' hydro fields (density and
-//                      temperature floors fused) and of their face fields
-//   MhdCalcDtKernel    next inverse dt (mhdInverseCrossingTime)
+//                      launch (no interface arrays: the PCM states are the cells
+//                      themselves); in the device-timed step it also reduces the step's
+//                      inverse dt (mhdInverseCrossingTime) from the state it reads
+//   MhdCtUpdateKernel  CT edge fields (S&G 2009 eq. 22-24) and the dt/2 update of the
+//                      hydro and face fields in one launch: x-y tiles marching in z, the
+//                      flux values, cell terms and edge fields in LDS (never in HBM)
+//   MhdFlux2TileKernel PLMC or PPMC of each cell once per direction + HLLD (tiles)
+//   MhdCtUpdateKernel  CT on the half-step fluxes and state, fused with the dt update of
+//                      the real cells' hydro fields (density and temperature floors fused)
+//                      and of their face fields
 //
 // Only the cells and faces the real cells depend on are computed, every read in
 // bounds; the reference's edge cells that read past the grid do not feed real cells.
@@ -636,7 +637,7 @@ __device__ __forceinline__ double CellEmf(const double *U, int ctDir, long ic, i
 // mhd::Calculate_CT_Electric_Fields for edges with every index in [lo, hi] (ct_electric_fields.cu:20-279),
 // one lane per edge with bounds-guarded loads, in the shape of the reference's kernel: the
 // reference-vector entry point (chl_ct_electric_fields) runs it over whole grids.  The step
-// uses MhdCtTileKernel, which has the same arithmetic per edge.
+// computes its edges in MhdCtUpdateKernel (CtEdge), with the same arithmetic per edge.
 template <typename TF>
 __global__ void __launch_bounds__(256) MhdEmfKernel(const TF *Fx, const TF *Fy, const TF *Fz, const double *U,
                                                     double *E, int nx, int ny, int nz, int lo, int hix, int hiy,
@@ -687,108 +688,93 @@ __global__ void __launch_bounds__(256) MhdEmfKernel(const TF *Fx, const TF *Fy, 
   }
 }
 
-// Two z planes of the CT tile: for the cells [i0 - 1, i0 + CTX - 1] x [j0 - 1, j0 + CTY - 1]
-// of one plane, fields 0, 5 and 6 of each direction's fluxes (the upwind mass flux and the
-// two transverse field fluxes) and the three cell-centred terms of the slopes (CellEmf).
-template <typename TF, int CTX, int CTY>
+// One z plane of cells staged for the CT edges: for SX x SY cells of the plane, fields 0, 5
+// and 6 of each direction's fluxes (the upwind mass flux and the two transverse field fluxes)
+// and the three cell-centred terms of the slopes (CellEmf).
+template <typename TF, int SX, int SY>
 struct CtPlane {
-  TF f[3][3][CTY + 1][CTX + 1];
-  double ec[3][CTY + 1][CTX + 1];
+  TF f[3][3][SY][SX];
+  double ec[3][SY][SX];
 };
 
-// mhd::Calculate_CT_Electric_Fields (ct_electric_fields.cu:20-279) for the edges with every
-// index in [lo, hi], from a CTX x CTY tile of edges marching in z over a chunk of kchunk
-// planes: each plane's flux values and cell-centred terms are loaded or computed once into
-// LDS, where the 8 edges around each cell read them.  The arithmetic of each edge is that of
-// MhdEmfKernel (slopes, upwind selection and sums in the same order).  Every cell read lies
-// in [lo - 1, hi] per axis, which the flux stages and the state cover.
-template <typename TF, int CTX, int CTY>
-__global__ void __launch_bounds__(CTX *CTY) MhdCtTileKernel(const TF *Fx, const TF *Fy, const TF *Fz, const double *U,
-                                                            double *E, int nx, int ny, int nz, int lo, int hix,
-                                                            int hiy, int hiz, int kchunk, const double *dt_dev)
+// Stage the cell cid at [cy][cx] of P: its flux fields 0, 5 and 6 and its cell terms, from loads of the cell's density,
+// momenta and face fields and of the face fields below it, which are CellEmf's operands in
+// its order (CellEmf guards reads outside the array; the cells staged here have every one of
+// them inside).  own (or null): the cell's 8 fields, for an update of that same state.
+template <typename TF, int SX, int SY>
+__device__ __forceinline__ void CtStageCell(CtPlane<TF, SX, SY> &P, int cx, int cy, const TF *const Fl[3],
+                                            const double *U, size_t cid, size_t n, size_t sy, size_t sz, double *own)
 {
-  __shared__ CtPlane<TF, CTX, CTY> pl[2];
-  if (dt_dev && *dt_dev == 0.0) return;
-  constexpr int NT = CTX * CTY;
-  // tiles in XCD-contiguous order (XcdItem): x fastest, then y, then the z chunk
-  const int gx = (hix - lo + CTX) / CTX, gy = (hiy - lo + CTY) / CTY, gz = (hiz - lo + kchunk) / kchunk;
-  const int q = XcdItem(int(blockIdx.x), gx * gy * gz);
-  if (q >= gx * gy * gz) return;
-  const int bx = q % gx, by = (q / gx) % gy, bz = q / (gx * gy);
-  const int tx = int(threadIdx.x) % CTX, ty = int(threadIdx.x) / CTX;
-  const int i0 = lo + bx * CTX, j0 = lo + by * CTY;
-  const int k0 = lo + bz * kchunk, k1 = min(k0 + kchunk - 1, hiz);
-  const size_t n = size_t(nx) * ny * nz, sy = size_t(nx), sz = size_t(nx) * ny;
-  const long nU  = 8 * long(n);
-  const TF *Fl[3] = {Fx, Fy, Fz};
-  auto stage = [&](int kk) {
-    CtPlane<TF, CTX, CTY> &P = pl[kk & 1];
-    for (int c = int(threadIdx.x); c < (CTX + 1) * (CTY + 1); c += NT) {
-      const int cy = c / (CTX + 1), cx = c - cy * (CTX + 1);
-      const int gi = i0 - 1 + cx, gj = j0 - 1 + cy;
-      if (gi > hix || gj > hiy) continue;  // read by no edge of the region
-      const size_t cid = size_t(gi) + size_t(gj) * sy + size_t(kk) * sz;
 #pragma unroll
-      for (int d = 0; d < 3; d++) {
-        P.f[d][0][cy][cx] = Fl[d][cid];
-        P.f[d][1][cy][cx] = Fl[d][size_t(kBx) * n + cid];
-        P.f[d][2][cy][cx] = Fl[d][size_t(kBx + 1) * n + cid];
-      }
+  for (int d = 0; d < 3; d++) {
+    P.f[d][0][cy][cx] = Fl[d][cid];
+    P.f[d][1][cy][cx] = Fl[d][size_t(kBx) * n + cid];
+    P.f[d][2][cy][cx] = Fl[d][size_t(kBx + 1) * n + cid];
+  }
+  const double d     = U[cid];
+  const double m[3]  = {U[n + cid], U[2 * n + cid], U[3 * n + cid]};
+  const double b[3]  = {U[5 * n + cid], U[6 * n + cid], U[7 * n + cid]};
+  const double bm[3] = {U[5 * n + cid - 1], U[6 * n + cid - sy], U[7 * n + cid - sz]};
 #pragma unroll
-      for (int q = 0; q < 3; q++) P.ec[q][cy][cx] = CellEmf(U, q, long(cid), nx, ny, long(n), nU);
-    }
+  for (int q = 0; q < 3; q++) {  // CellEmf(U, q, cid): (m3 b2 - m2 b3) / d
+    const int m1 = (q + 1) % 3, m2 = (q + 2) % 3;
+    const double b2 = 0.5 * (b[m1] + bm[m1]);
+    const double b3 = 0.5 * (b[m2] + bm[m2]);
+    P.ec[q][cy][cx] = (m[m2] * b2 - m[m1] * b3) / d;
+  }
+  if (own) {
+    own[0] = d, own[1] = m[0], own[2] = m[1], own[3] = m[2], own[4] = U[4 * n + cid];
+    own[5] = b[0], own[6] = b[1], own[7] = b[2];
+  }
+}
+
+// The three electric fields of the edge whose own cell is staged at [ey + 1][ex + 1] of
+// plane k (pl[k & 1]; plane k - 1 in pl[(k - 1) & 1]).  The arithmetic of MhdEmfKernel
+// (ct_electric_fields.cu:20-279): slopes, upwind selection and sums in the same order.
+template <typename TF, int SX, int SY>
+__device__ __forceinline__ void CtEdge(const CtPlane<TF, SX, SY> *pl, int k, int ex, int ey, double &Ex, double &Ey,
+                                       double &Ez)
+{
+  // (ox, oy, oz): the cell (i - ox, j - oy, k - oz) of the edge's 2 x 2 x 2 block
+  auto flx = [&](int dir, int ox, int oy, int oz, int fi) -> double {
+    return double(pl[(k - oz) & 1].f[dir][fi][ey + 1 - oy][ex + 1 - ox]);
   };
-  const int i = i0 + tx, j = j0 + ty;
-  const bool mine = i <= hix && j <= hiy;
-  if (k0 <= k1) stage(k0 - 1);
-  for (int k = k0; k <= k1; k++) {
-    stage(k);
-    __syncthreads();
-    if (mine) {
-      // (ox, oy, oz): the cell (i - ox, j - oy, k - oz) of the edge's 2 x 2 x 2 block
-      auto flx = [&](int dir, int ox, int oy, int oz, int fi) -> double {
-        return double(pl[(k - oz) & 1].f[dir][fi][ty + 1 - oy][tx + 1 - ox]);
-      };
-      // _ctSlope with the cell-centred term read back; a..d as in MhdEmfKernel's S(): the
-      // axes (or -1) whose one-cell shifts locate the cell term (a, b) and the flux (c, d)
-      auto slope = [&](int dir, double sgn, int ctd, int a, int b, int c, int d) -> double {
-        const double ef = sgn * flx(dir, (c == 0) + (d == 0), (c == 1) + (d == 1), (c == 2) + (d == 2), 1 + int(sgn == 1));
-        const int ox = (a == 0) + (b == 0), oy = (a == 1) + (b == 1), oz = (a == 2) + (b == 2);
-        return ef - pl[(k - oz) & 1].ec[ctd][ty + 1 - oy][tx + 1 - ox];
-      };
-      auto up = [](double sgn_flux, double pos, double neg) -> double {
-        return sgn_flux > 0.0 ? pos : (sgn_flux < 0.0 ? neg : 0.5 * (pos + neg));
-      };
-      const size_t id = size_t(i) + size_t(j) * sy + size_t(k) * sz;
-      {  // x electric field
-        const double syp = up(flx(2, 0, 0, 1, 0), slope(1, -1, 0, 2, -1, 1, 2), slope(1, -1, 0, -1, -1, 1, -1));
-        const double syn = up(flx(2, 0, 1, 1, 0), slope(1, -1, 0, 1, 2, 1, 2), slope(1, -1, 0, 1, -1, 1, -1));
-        const double szp = up(flx(1, 0, 1, 0, 0), slope(2, 1, 0, 1, -1, 1, 2), slope(2, 1, 0, -1, -1, 2, -1));
-        const double szn = up(flx(1, 0, 1, 1, 0), slope(2, 1, 0, 1, 2, 1, 2), slope(2, 1, 0, 2, -1, -1, 2));
-        const double fyp = +flx(2, 0, 0, 1, 2), fyn = +flx(2, 0, 1, 1, 2);
-        const double fzp = -flx(1, 0, 1, 0, 1), fzn = -flx(1, 0, 1, 1, 1);
-        E[id] = 0.25 * (+fyp + fyn + fzp + fzn + syp + syn + szp + szn);
-      }
-      {  // y electric field
-        const double sxp = up(flx(2, 0, 0, 1, 0), slope(0, 1, 1, 2, -1, 0, 2), slope(0, 1, 1, -1, -1, 0, -1));
-        const double sxn = up(flx(2, 1, 0, 1, 0), slope(0, 1, 1, 0, 2, 0, 2), slope(0, 1, 1, 0, -1, 0, -1));
-        const double szp = up(flx(0, 1, 0, 0, 0), slope(2, -1, 1, 0, -1, 0, 2), slope(2, -1, 1, -1, -1, 2, -1));
-        const double szn = up(flx(0, 1, 0, 1, 0), slope(2, -1, 1, 0, 2, 0, 2), slope(2, -1, 1, 2, -1, 2, -1));
-        const double fxp = -flx(2, 0, 0, 1, 1), fxn = -flx(2, 1, 0, 1, 1);
-        const double fzp = +flx(0, 1, 0, 0, 2), fzn = +flx(0, 1, 0, 1, 2);
-        E[n + id] = 0.25 * (+fxp + fxn + fzp + fzn + sxp + sxn + szp + szn);
-      }
-      {  // z electric field
-        const double syp = up(flx(0, 1, 0, 0, 0), slope(1, 1, 2, 0, -1, 0, 1), slope(1, 1, 2, -1, -1, 1, -1));
-        const double syn = up(flx(0, 1, 1, 0, 0), slope(1, 1, 2, 0, 1, 0, 1), slope(1, 1, 2, 1, -1, 1, -1));
-        const double sxp = up(flx(1, 0, 1, 0, 0), slope(0, -1, 2, 1, -1, 0, 1), slope(0, -1, 2, -1, -1, 0, -1));
-        const double sxn = up(flx(1, 1, 1, 0, 0), slope(0, -1, 2, 0, 1, 0, 1), slope(0, -1, 2, 0, -1, 0, -1));
-        const double fxp = +flx(1, 0, 1, 0, 2), fxn = +flx(1, 1, 1, 0, 2);
-        const double fyp = -flx(0, 1, 0, 0, 1), fyn = -flx(0, 1, 1, 0, 1);
-        E[2 * n + id] = 0.25 * (+fxp + fxn + fyp + fyn + sxp + sxn + syp + syn);
-      }
-    }
-    __syncthreads();  // the next plane overwrites the older one
+  // _ctSlope with the cell-centred term read back; a..d as in MhdEmfKernel's S(): the axes
+  // (or -1) whose one-cell shifts locate the cell term (a, b) and the flux (c, d)
+  auto slope = [&](int dir, double sgn, int ctd, int a, int b, int c, int d) -> double {
+    const double ef = sgn * flx(dir, (c == 0) + (d == 0), (c == 1) + (d == 1), (c == 2) + (d == 2), 1 + int(sgn == 1));
+    const int ox = (a == 0) + (b == 0), oy = (a == 1) + (b == 1), oz = (a == 2) + (b == 2);
+    return ef - pl[(k - oz) & 1].ec[ctd][ey + 1 - oy][ex + 1 - ox];
+  };
+  auto up = [](double sgn_flux, double pos, double neg) -> double {
+    return sgn_flux > 0.0 ? pos : (sgn_flux < 0.0 ? neg : 0.5 * (pos + neg));
+  };
+  {  // x electric field
+    const double syp = up(flx(2, 0, 0, 1, 0), slope(1, -1, 0, 2, -1, 1, 2), slope(1, -1, 0, -1, -1, 1, -1));
+    const double syn = up(flx(2, 0, 1, 1, 0), slope(1, -1, 0, 1, 2, 1, 2), slope(1, -1, 0, 1, -1, 1, -1));
+    const double szp = up(flx(1, 0, 1, 0, 0), slope(2, 1, 0, 1, -1, 1, 2), slope(2, 1, 0, -1, -1, 2, -1));
+    const double szn = up(flx(1, 0, 1, 1, 0), slope(2, 1, 0, 1, 2, 1, 2), slope(2, 1, 0, 2, -1, -1, 2));
+    const double fyp = +flx(2, 0, 0, 1, 2), fyn = +flx(2, 0, 1, 1, 2);
+    const double fzp = -flx(1, 0, 1, 0, 1), fzn = -flx(1, 0, 1, 1, 1);
+    Ex = 0.25 * (+fyp + fyn + fzp + fzn + syp + syn + szp + szn);
+  }
+  {  // y electric field
+    const double sxp = up(flx(2, 0, 0, 1, 0), slope(0, 1, 1, 2, -1, 0, 2), slope(0, 1, 1, -1, -1, 0, -1));
+    const double sxn = up(flx(2, 1, 0, 1, 0), slope(0, 1, 1, 0, 2, 0, 2), slope(0, 1, 1, 0, -1, 0, -1));
+    const double szp = up(flx(0, 1, 0, 0, 0), slope(2, -1, 1, 0, -1, 0, 2), slope(2, -1, 1, -1, -1, 2, -1));
+    const double szn = up(flx(0, 1, 0, 1, 0), slope(2, -1, 1, 0, 2, 0, 2), slope(2, -1, 1, 2, -1, 2, -1));
+    const double fxp = -flx(2, 0, 0, 1, 1), fxn = -flx(2, 1, 0, 1, 1);
+    const double fzp = +flx(0, 1, 0, 0, 2), fzn = +flx(0, 1, 0, 1, 2);
+    Ey = 0.25 * (+fxp + fxn + fzp + fzn + sxp + sxn + szp + szn);
+  }
+  {  // z electric field
+    const double syp = up(flx(0, 1, 0, 0, 0), slope(1, 1, 2, 0, -1, 0, 1), slope(1, 1, 2, -1, -1, 1, -1));
+    const double syn = up(flx(0, 1, 1, 0, 0), slope(1, 1, 2, 0, 1, 0, 1), slope(1, 1, 2, 1, -1, 1, -1));
+    const double sxp = up(flx(1, 0, 1, 0, 0), slope(0, -1, 2, 1, -1, 0, 1), slope(0, -1, 2, -1, -1, 0, -1));
+    const double sxn = up(flx(1, 1, 1, 0, 0), slope(0, -1, 2, 0, 1, 0, 1), slope(0, -1, 2, 0, -1, 0, -1));
+    const double fxp = +flx(1, 0, 1, 0, 2), fxn = +flx(1, 1, 1, 0, 2);
+    const double fyp = -flx(0, 1, 0, 0, 1), fyn = -flx(0, 1, 1, 0, 1);
+    Ez = 0.25 * (+fxp + fxn + fyp + fyn + sxp + sxn + syp + syn);
   }
 }
 
@@ -797,7 +783,6 @@ struct MhdArgs {
   double *U;          // state (8 fields, in/out)
   double *Uh;         // half-step state (8 fields)
   void *F[3];         // flux arrays (7 fields each, P::T)
-  double *E;          // edge electric fields (3 fields)
   int nx, ny, nz, ng;
   double dx, dy, dz, gamma;
   const double *dt_dev;  // device dt (device-timed step), or null
@@ -806,6 +791,11 @@ struct MhdArgs {
   double density_floor, U_floor;
   int flags;  // kFlag* bits of vl3d_kernels.hpp
   int recon;  // stage-2 reconstruction: CHL_RECON_PCM, _PLMC or _PPMC (0, 1, 2)
+  // kFlagFoldDt: the device clock DtAdvanceKernel advances between the stage-1 fluxes (which
+  // reduce dti) and the rest of the step; dt_dev points at clk_dt
+  double *clk_dt, *clk_t;
+  int *clk_nstep;
+  double clk_tout, clk_cfl;
   RepairState rep;  // cells the full update leaves to RepairKernel (vl3d_kernels.hpp)
 };
 
@@ -878,6 +868,38 @@ __device__ __forceinline__ bool MarchCell(const MhdRegion &m, int &x, int &y, in
   return RowCell(m, bxy, x, y);
 }
 
+// ---------------------------------------------------------------------------- inverse dt
+// mhdInverseCrossingTime (hydro_cuda.cu:420-435) with the cell-centred field
+__device__ __forceinline__ double MhdInverseCrossingTime(const double *U, size_t id, int x, int y, int z, size_t n,
+                                                         int nx, int ny, double dx, double dy, double dz, double g)
+{
+  const double d = U[id], dinv = 1.0 / d;
+  const double vx = U[n + id] * dinv, vy = U[2 * n + id] * dinv, vz = U[3 * n + id] * dinv;
+  double bx, by, bz;
+  CellB(U, id, x, y, z, n, nx, ny, bx, by, bz);
+  const double gasP = PressurePrimM(U[4 * n + id], d, vx, vy, vz, g, bx, by, bz);
+  const double cf   = Magnetosonic<PolicyExact>(d, gasP, bx, by, bz, g, 1.0);
+  double m          = Fmax((Fabs(vx) + cf) / dx, (Fabs(vy) + cf) / dy);
+  m                 = Fmax(m, (Fabs(vz) + cf) / dz);
+  return Fmax(m, 0.0);
+}
+
+// Block maximum of non-negative doubles folded into *out with one atomic per block; every
+// lane of the block calls it.
+__device__ __forceinline__ void BlockMaxAtomic(double m, unsigned long long *out)
+{
+  __shared__ double red[1024 / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = Fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = red[0];
+    for (int w = 1; w < int(blockDim.x) / 64; w++) b = Fmax(b, red[w]);
+    atomicMax(out, (unsigned long long)__double_as_longlong(b));
+  }
+}
+
 // The three stage-1 faces of cell (x, y, z) = id.
 template <class P>
 __device__ __forceinline__ void Flux1Cell(const MhdArgs &a, int x, int y, int z, size_t id, size_t n, size_t sy,
@@ -918,16 +940,33 @@ __device__ __forceinline__ void Flux1Cell(const MhdArgs &a, int x, int y, int z,
 // left cell of (i+1/2, j+1/2, k+1/2) by rotation, and solves the three faces with its upper
 // neighbours: the PCM states (the cells with their cell-centred fields) and HLLD on the face
 // i+1/2 of every cell whose face lies inside the grid.
+//
+// With kFlagFoldDt the launch also reduces the inverse crossing time of the real cells of the
+// state it reads into *a.dti (which holds 0 or that same maximum), for the step's own dt: the
+// device-timed step then needs no separate pass over the state.  The dt is not known yet, so
+// the fluxes are computed even in a step that the output time will make empty.
 template <class P>
 __global__ void __launch_bounds__(256) MhdFlux1AllKernel(MhdArgs a, MhdRegion m)
 {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && threadIdx.x == 0) *a.dti = 0ull;  // MhdCalcDtKernel fills it
-  int x, y, z;
-  if (!SlabCell(m, x, y, z)) return;  // the slabs cover the whole grid
-  if (MhdDt(a) == 0.0) return;
   const size_t n  = size_t(nx) * ny * nz;
   const size_t sy = size_t(nx), sz = size_t(nx) * ny;
+  int x, y, z;
+  if (a.flags & kFlagFoldDt) {
+    double dti = 0.0;
+    if (SlabCell(m, x, y, z)) {  // the slabs cover the whole grid
+      const size_t id = size_t(x) + size_t(y) * sy + size_t(z) * sz;
+      Flux1Cell<P>(a, x, y, z, id, n, sy, sz);
+      const int ng = a.ng;
+      if (x >= ng && y >= ng && z >= ng && x < nx - ng && y < ny - ng && z < nz - ng)
+        dti = MhdInverseCrossingTime(a.U, id, x, y, z, n, nx, ny, a.dx, a.dy, a.dz, a.gamma);
+    }
+    BlockMaxAtomic(dti, a.dti);  // every lane gets here
+    return;
+  }
+  if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && threadIdx.x == 0) *a.dti = 0ull;  // MhdCalcDtKernel fills it
+  if (!SlabCell(m, x, y, z)) return;
+  if (MhdDt(a) == 0.0) return;
   Flux1Cell<P>(a, x, y, z, size_t(x) + size_t(y) * sy + size_t(z) * sz, n, sy, sz);
 }
 
@@ -947,28 +986,39 @@ __global__ void __launch_bounds__(256) MhdFlux2PcmKernel(MhdArgs a, MhdRegion m)
   Flux1Cell<P>(h, x, y, z, size_t(x) + size_t(y) * sy + size_t(z) * sz, n, sy, sz);
 }
 
+// The edge fields the face-field update of one cell reads (Update_Magnetic_Field_3D,
+// magnetic_update.cu:21-81), named by field and by the offset of the edge's index from the
+// cell's: ey_x = Ey[i+1, j, k], ex_yz = Ex[i, j+1, k+1], ...  Edges of plane k: ey_x, ez_x,
+// ez_xy, ez_y, ex_y; of plane k+1: ey_xz, ex_yz, ex_z, ey_z.
+struct EdgeSet {
+  double ey_x, ez_x, ez_xy, ez_y, ex_y;
+  double ey_xz, ex_yz, ex_z, ey_z;
+};
+
+// The state and flux values one cell's update reads: its 8 fields and the 5 hydro fluxes of
+// its lower (fm) and upper (fp) faces per direction.
+template <typename TF>
+struct CellIn {
+  double u[8];
+  TF fm[3][5], fp[3][5];
+};
+
 // Half update: hydro fields of cells [1, n-2] (Update_Conserved_Variables_3D_half,
 // VL_3D_cuda.cu:379-495, with the density floor) and their face fields
 // (Update_Magnetic_Field_3D, magnetic_update.cu:21-81, from U into Uh).
 template <class P>
-__device__ __forceinline__ void HalfCell(const MhdArgs &a, double dt, int x, int y, int z)
+__device__ __forceinline__ void HalfCell(const MhdArgs &a, double dt, int x, int y, int z,
+                                         const CellIn<typename P::T> &c, const EdgeSet &e)
 {
-  using T        = typename P::T;
-  const int nx   = a.nx, ny = a.ny, nz = a.nz;
-  const size_t n = size_t(nx) * ny * nz;
-  const size_t id = size_t(x) + size_t(y) * nx + size_t(z) * size_t(nx) * ny;
+  const size_t n  = size_t(a.nx) * a.ny * a.nz;
+  const size_t id = size_t(x) + size_t(y) * a.nx + size_t(z) * size_t(a.nx) * a.ny;
   const double dtodx = dt / a.dx, dtody = dt / a.dy, dtodz = dt / a.dz;
-  const size_t sx = 1, sy = size_t(nx), sz = size_t(nx) * ny;
-  const T *Fx = reinterpret_cast<const T *>(a.F[0]), *Fy = reinterpret_cast<const T *>(a.F[1]),
-          *Fz = reinterpret_cast<const T *>(a.F[2]);
-  const double *U = a.U;
-  double *Uh      = a.Uh;
+  double *Uh = a.Uh;
   double uh[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
-    const size_t o = size_t(f) * n;
-    uh[f] = U[o + id] + dtodx * (double(Fx[o + id - sx]) - double(Fx[o + id])) +
-            dtody * (double(Fy[o + id - sy]) - double(Fy[o + id])) + dtodz * (double(Fz[o + id - sz]) - double(Fz[o + id]));
+    uh[f] = c.u[f] + dtodx * (double(c.fm[0][f]) - double(c.fp[0][f])) +
+            dtody * (double(c.fm[1][f]) - double(c.fp[1][f])) + dtodz * (double(c.fm[2][f]) - double(c.fp[2][f]));
   }
   if ((a.flags & kFlagDensityFloor) && uh[0] < a.density_floor) {
     const double d0 = uh[0];
@@ -979,32 +1029,9 @@ __device__ __forceinline__ void HalfCell(const MhdArgs &a, double dt, int x, int
 #pragma unroll
   for (int f = 0; f < 5; f++) Uh[size_t(f) * n + id] = uh[f];
   // face fields whose edge fields were computed (edges [2, n-1]): own index >= 1, others >= 2
-  const double *E = a.E;
-  if (y >= 2 && z >= 2) {
-    const double ey1 = E[n + id + sx], ey3 = E[n + id + sx + sz];
-    const double ez1 = E[2 * n + id + sx], ez3 = E[2 * n + id + sx + sy];
-    Uh[5 * n + id] = U[5 * n + id] + dtodz * (ey3 - ey1) + dtody * (ez1 - ez3);
-  }
-  if (x >= 2 && z >= 2) {
-    const double ez2 = E[2 * n + id + sy], ez3 = E[2 * n + id + sx + sy];
-    const double ex1 = E[id + sy], ex3 = E[id + sy + sz];
-    Uh[6 * n + id] = U[6 * n + id] + dtodx * (ez3 - ez2) + dtodz * (ex1 - ex3);
-  }
-  if (x >= 2 && y >= 2) {
-    const double ex2 = E[id + sz], ex3 = E[id + sy + sz];
-    const double ey2 = E[n + id + sz], ey3 = E[n + id + sx + sz];
-    Uh[7 * n + id] = U[7 * n + id] + dtody * (ex3 - ex2) + dtodx * (ey2 - ey3);
-  }
-}
-
-template <class P>
-__global__ void __launch_bounds__(256) MhdHalfKernel(MhdArgs a, MhdRegion m)
-{
-  int x, y, z;
-  if (!SlabCell(m, x, y, z)) return;  // the slabs cover [1, n-2]^3
-  const double dt = 0.5 * MhdDt(a);
-  if (dt == 0.0) return;
-  HalfCell<P>(a, dt, x, y, z);
+  if (y >= 2 && z >= 2) Uh[5 * n + id] = c.u[5] + dtodz * (e.ey_xz - e.ey_x) + dtody * (e.ez_x - e.ez_xy);
+  if (x >= 2 && z >= 2) Uh[6 * n + id] = c.u[6] + dtodx * (e.ez_xy - e.ez_y) + dtodz * (e.ex_y - e.ex_yz);
+  if (x >= 2 && y >= 2) Uh[7 * n + id] = c.u[7] + dtody * (e.ex_yz - e.ex_z) + dtodx * (e.ey_z - e.ey_xz);
 }
 
 // Stage-2 fluxes along DIR on faces with every index in [lo, hi]: PLMC of the two cells of
@@ -1105,26 +1132,22 @@ __global__ void __launch_bounds__(256) MhdFlux2TileKernel(MhdArgs a, int lo, int
 // floor and the temperature floor (Temperature_Floor_Kernel, hydro_cuda.cu:1117-1153);
 // face fields whose own index is in [ng-1, n-ng-1] and other indices in [ng, n-ng-1].
 template <class P>
-__device__ __forceinline__ void FullCell(const MhdArgs &a, double dt, int x, int y, int z)
+__device__ __forceinline__ void FullCell(const MhdArgs &a, double dt, int x, int y, int z,
+                                         const CellIn<typename P::T> &c, const EdgeSet &e)
 {
-  using T        = typename P::T;
   const int nx   = a.nx, ny = a.ny, nz = a.nz, ng = a.ng;
   const size_t n = size_t(nx) * ny * nz;
   const size_t id = size_t(x) + size_t(y) * nx + size_t(z) * size_t(nx) * ny;
   const bool rx = x >= ng, ry = y >= ng, rz = z >= ng;
   const double dtodx = dt / a.dx, dtody = dt / a.dy, dtodz = dt / a.dz;
-  const size_t sx = 1, sy = size_t(nx), sz = size_t(nx) * ny;
   double *U       = a.U;
   if (rx && ry && rz) {
-    const T *Fx = reinterpret_cast<const T *>(a.F[0]), *Fy = reinterpret_cast<const T *>(a.F[1]),
-            *Fz = reinterpret_cast<const T *>(a.F[2]);
     double un[5];
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-      const size_t o = size_t(f) * n;
-      un[f] = U[o + id] + (dtodx * (double(Fx[o + id - sx]) - double(Fx[o + id])) +
-                           dtody * (double(Fy[o + id - sy]) - double(Fy[o + id])) +
-                           dtodz * (double(Fz[o + id - sz]) - double(Fz[o + id])));
+      un[f] = c.u[f] + (dtodx * (double(c.fm[0][f]) - double(c.fp[0][f])) +
+                        dtody * (double(c.fm[1][f]) - double(c.fp[1][f])) +
+                        dtodz * (double(c.fm[2][f]) - double(c.fp[2][f])));
     }
     int mark = 0;
     if ((a.flags & kFlagDensityFloor) && un[0] < a.density_floor) {
@@ -1149,48 +1172,183 @@ __device__ __forceinline__ void FullCell(const MhdArgs &a, double dt, int x, int
 #pragma unroll
     for (int f = 0; f < 5; f++) U[size_t(f) * n + id] = un[f];
   }
-  const double *E = a.E;
-  if (ry && rz) {
-    const double ey1 = E[n + id + sx], ey3 = E[n + id + sx + sz];
-    const double ez1 = E[2 * n + id + sx], ez3 = E[2 * n + id + sx + sy];
-    U[5 * n + id] = U[5 * n + id] + dtodz * (ey3 - ey1) + dtody * (ez1 - ez3);
-  }
-  if (rx && rz) {
-    const double ez2 = E[2 * n + id + sy], ez3 = E[2 * n + id + sx + sy];
-    const double ex1 = E[id + sy], ex3 = E[id + sy + sz];
-    U[6 * n + id] = U[6 * n + id] + dtodx * (ez3 - ez2) + dtodz * (ex1 - ex3);
-  }
-  if (rx && ry) {
-    const double ex2 = E[id + sz], ex3 = E[id + sy + sz];
-    const double ey2 = E[n + id + sz], ey3 = E[n + id + sx + sz];
-    U[7 * n + id] = U[7 * n + id] + dtody * (ex3 - ex2) + dtodx * (ey2 - ey3);
-  }
+  if (ry && rz) U[5 * n + id] = c.u[5] + dtodz * (e.ey_xz - e.ey_x) + dtody * (e.ez_x - e.ez_xy);
+  if (rx && rz) U[6 * n + id] = c.u[6] + dtodx * (e.ez_xy - e.ez_y) + dtodz * (e.ex_y - e.ex_yz);
+  if (rx && ry) U[7 * n + id] = c.u[7] + dtody * (e.ex_yz - e.ex_z) + dtodx * (e.ey_z - e.ey_xz);
 }
 
-template <class P>
-__global__ void __launch_bounds__(256) MhdFullKernel(MhdArgs a, MhdRegion m)
+// CT edge fields and the update of one stage in one launch.  FULL false: the half update
+// (HalfCell, U -> Uh over the cells [clo, ch*]) from the edges of the stage-1 fluxes and U;
+// FULL true: the full update (FullCell, U in place) from the edges of the stage-2 fluxes and
+// the half-step state.  A CTX x CTY tile of cells marches in z over a chunk of kchunk planes;
+// at plane k it stages the cells of plane k + 1 (CtStageCell), computes the (CTX+1) x (CTY+1)
+// edges of plane k + 1 into LDS (CtEdge, the arithmetic of MhdEmfKernel), and updates its
+// cells of plane k from the edges of plane k (kept in registers since the previous plane) and
+// of plane k + 1.  The edge fields never reach HBM and each cell's state and fluxes are read
+// once per stage.  Edges outside [elo, eh*] are not computed and read as 0: no update that
+// runs uses them.  FULL: the tile's lowest column, row and plane also update the face fields
+// FullCell updates on the cells just below [clo, ch*] (Bx of (clo-1, j, k), By of (i, clo-1,
+// k), Bz of (i, j, clo-1)), whose edges are the tile's lowest ones; then the cells [clo, ch*]
+// are the real cells, and the tiles divide them evenly.
+template <class P, int CTX, int CTY, bool FULL>
+__global__ void __launch_bounds__(CTX *CTY) MhdCtUpdateKernel(MhdArgs a, int clo, int chx, int chy, int chz,
+                                                              int elo, int ehx, int ehy, int ehz, int kchunk)
 {
-  int x, y, z;
-  if (!SlabCell(m, x, y, z)) return;  // the slabs cover [ng-1, n-ng-1]^3
-  const double dt = MhdDt(a);
+  using TF         = typename P::T;
+  constexpr int NT = CTX * CTY, EX = CTX + 1, EY = CTY + 1;
+  __shared__ CtPlane<TF, CTX + 2, CTY + 2> pl[2];
+  __shared__ double eds[3][EY][EX];
+  const double dt = FULL ? MhdDt(a) : 0.5 * MhdDt(a);
   if (dt == 0.0) return;
-  FullCell<P>(a, dt, x, y, z);
-}
-
-// ---------------------------------------------------------------------------- dt, divergence
-// mhdInverseCrossingTime (hydro_cuda.cu:420-435) with the cell-centred field
-__device__ __forceinline__ double MhdInverseCrossingTime(const double *U, size_t id, int x, int y, int z, size_t n,
-                                                         int nx, int ny, double dx, double dy, double dz, double g)
-{
-  const double d = U[id], dinv = 1.0 / d;
-  const double vx = U[n + id] * dinv, vy = U[2 * n + id] * dinv, vz = U[3 * n + id] * dinv;
-  double bx, by, bz;
-  CellB(U, id, x, y, z, n, nx, ny, bx, by, bz);
-  const double gasP = PressurePrimM(U[4 * n + id], d, vx, vy, vz, g, bx, by, bz);
-  const double cf   = Magnetosonic<PolicyExact>(d, gasP, bx, by, bz, g, 1.0);
-  double m          = Fmax((Fabs(vx) + cf) / dx, (Fabs(vy) + cf) / dy);
-  m                 = Fmax(m, (Fabs(vz) + cf) / dz);
-  return Fmax(m, 0.0);
+  // tiles in XCD-contiguous order (XcdItem): x fastest, then y, then the z chunk
+  const int gx = (chx - clo + CTX) / CTX, gy = (chy - clo + CTY) / CTY, gz = (chz - clo + kchunk) / kchunk;
+  const int q = XcdItem(int(blockIdx.x), gx * gy * gz);
+  if (q >= gx * gy * gz) return;
+  const int bx = q % gx, by = (q / gx) % gy, bz = q / (gx * gy);
+  const int tx = int(threadIdx.x) % CTX, ty = int(threadIdx.x) / CTX;
+  const int i0 = clo + bx * CTX, j0 = clo + by * CTY;
+  const int k0 = clo + bz * kchunk, k1 = min(k0 + kchunk - 1, chz);
+  const int nx = a.nx, ny = a.ny;
+  const size_t n        = size_t(nx) * ny * a.nz;
+  const TF *const Fl[3] = {reinterpret_cast<const TF *>(a.F[0]), reinterpret_cast<const TF *>(a.F[1]),
+                           reinterpret_cast<const TF *>(a.F[2])};
+  const double *Us = FULL ? a.Uh : a.U;  // the state of the edges' cell terms
+  const size_t sy = size_t(nx), sz = size_t(nx) * ny;
+  // staged cell (cx, cy) = (i0 - 1 + cx, j0 - 1 + cy); edge (ex, ey) = (i0 + ex, j0 + ey),
+  // whose own cell is staged at [ey + 1][ex + 1].  Staged: the cells the computed edges read
+  // and the lower x / y neighbours of the updated cells (whose mass fluxes the update reads
+  // back), every index in [slo, eh*].  Each thread stages its own cell (tx + 1, ty + 1), and
+  // the first NH threads one cell of the tile's rim each.
+  constexpr int SX = CTX + 2, SY = CTY + 2, NH = 2 * SX + 2 * (SY - 2);
+  static_assert(NH <= NT, "one rim cell per thread");
+  const int slo = min(elo, clo) - 1;
+  auto in_box = [&](int gi, int gj, int kk) {
+    return gi >= slo && gj >= slo && kk >= slo && gi <= ehx && gj <= ehy && kk <= ehz;
+  };
+  auto stage = [&](int kk, double *own) {
+    CtPlane<TF, SX, SY> &Pl = pl[kk & 1];
+    const size_t base       = size_t(kk) * sz;
+    {
+      const int gi = i0 + tx, gj = j0 + ty;
+      if (in_box(gi, gj, kk)) CtStageCell(Pl, tx + 1, ty + 1, Fl, Us, size_t(gi) + size_t(gj) * sy + base, n, sy, sz, own);
+    }
+    const int t = int(threadIdx.x);
+    if (t < NH) {
+      int cx, cy;
+      if (t < SX) {
+        cx = t, cy = 0;
+      } else if (t < 2 * SX) {
+        cx = t - SX, cy = SY - 1;
+      } else {
+        const int r = t - 2 * SX;
+        cx = (r & 1) ? SX - 1 : 0, cy = 1 + (r >> 1);
+      }
+      const int gi = i0 - 1 + cx, gj = j0 - 1 + cy;
+      if (in_box(gi, gj, kk)) CtStageCell(Pl, cx, cy, Fl, Us, size_t(gi) + size_t(gj) * sy + base, n, sy, sz, nullptr);
+    }
+  };
+  auto edges = [&](int ek) {
+    for (int c = int(threadIdx.x); c < EX * EY; c += NT) {
+      const int ey = c / EX, ex = c - ey * EX;
+      const int gi = i0 + ex, gj = j0 + ey;
+      double vx = 0.0, vy = 0.0, vz = 0.0;
+      if (gi >= elo && gj >= elo && ek >= elo && gi <= ehx && gj <= ehy && ek <= ehz) CtEdge(pl, ek, ex, ey, vx, vy, vz);
+      eds[0][ey][ex] = vx, eds[1][ey][ex] = vy, eds[2][ey][ex] = vz;
+    }
+  };
+  const int i = i0 + tx, j = j0 + ty;
+  const bool mine = i <= chx && j <= chy;
+  const bool left_x = FULL && tx == 0 && i0 == clo && mine, left_y = FULL && ty == 0 && j0 == clo && mine;
+  EdgeSet e{};
+  double lx_ey = 0.0, lx_ez = 0.0, lx_ez1 = 0.0;  // Ey(i0, j, k), Ez(i0, j, k), Ez(i0, j+1, k)
+  double ly_ez = 0.0, ly_ez1 = 0.0, ly_ex = 0.0;  // Ez(i, j0, k), Ez(i+1, j0, k), Ex(i, j0, k)
+  auto read_k = [&]() {  // this thread's edges of the plane in LDS, as plane k
+    e.ey_x = eds[1][ty][tx + 1], e.ez_x = eds[2][ty][tx + 1], e.ez_xy = eds[2][ty + 1][tx + 1];
+    e.ez_y = eds[2][ty + 1][tx], e.ex_y = eds[0][ty + 1][tx];
+    if constexpr (FULL) {
+      lx_ey = eds[1][ty][0], lx_ez = eds[2][ty][0], lx_ez1 = eds[2][ty + 1][0];
+      ly_ez = eds[2][0][tx], ly_ez1 = eds[2][0][tx + 1], ly_ex = eds[0][0][tx];
+    }
+  };
+  auto read_k1 = [&](EdgeSet &s) {  // ... as plane k + 1
+    s.ey_xz = eds[1][ty][tx + 1], s.ex_yz = eds[0][ty + 1][tx], s.ex_z = eds[0][ty][tx], s.ey_z = eds[1][ty][tx];
+  };
+  CellIn<TF> lc{};  // the face field below the tile (FULL: Bz of (i, j, clo - 1), then Bx / By)
+  const bool left_z = FULL && k0 == clo && mine;
+  if (left_z) lc.u[7] = a.U[7 * n + size_t(i) + size_t(j) * sy + size_t(clo - 1) * sz];
+  // the half update's cell state comes from its own staging (own_k: plane k, own_n: k + 1);
+  // fz_lo: the hydro fluxes of the face below the cell, carried from the plane below
+  double own_k[8] = {}, own_n[8] = {};
+  TF fz_lo[5] = {};
+  if (mine) {
+    const size_t id0 = size_t(i) + size_t(j) * sy + size_t(k0) * sz;
+#pragma unroll
+    for (int f = 0; f < 5; f++) fz_lo[f] = Fl[2][size_t(f) * n + id0 - sz];
+  }
+  stage(k0 - 1, nullptr);
+  stage(k0, FULL ? nullptr : own_k);
+  __syncthreads();
+  edges(k0);
+  __syncthreads();
+  read_k();
+  if (left_z) {  // Bz of (i, j, clo - 1): the edges of plane clo
+    EdgeSet zl{};
+    read_k1(zl);
+    FullCell<P>(a, dt, i, j, clo - 1, lc, zl);
+  }
+  for (int k = k0; k <= k1; k++) {
+    stage(k + 1, FULL ? nullptr : own_n);
+    __syncthreads();
+    edges(k + 1);
+    __syncthreads();
+    read_k1(e);
+    if (mine) {
+      // the cell's update inputs: the state (staged, or loaded for the full update, whose
+      // edges use the half-step state), the mass fluxes from the staged plane k, the other
+      // fluxes loaded, and the z-face below from the plane below
+      const size_t id          = size_t(i) + size_t(j) * sy + size_t(k) * sz;
+      const CtPlane<TF, SX, SY> &Pk = pl[k & 1];
+      CellIn<TF> c;
+#pragma unroll
+      for (int f = 0; f < 8; f++) c.u[f] = FULL ? a.U[size_t(f) * n + id] : own_k[f];
+      c.fp[0][0] = Pk.f[0][0][ty + 1][tx + 1], c.fm[0][0] = Pk.f[0][0][ty + 1][tx];
+      c.fp[1][0] = Pk.f[1][0][ty + 1][tx + 1], c.fm[1][0] = Pk.f[1][0][ty][tx + 1];
+      c.fp[2][0] = Pk.f[2][0][ty + 1][tx + 1], c.fm[2][0] = fz_lo[0];
+#pragma unroll
+      for (int f = 1; f < 5; f++) {
+        const size_t o = size_t(f) * n + id;
+        c.fp[0][f] = Fl[0][o], c.fm[0][f] = Fl[0][o - 1];
+        c.fp[1][f] = Fl[1][o], c.fm[1][f] = Fl[1][o - sy];
+        c.fp[2][f] = Fl[2][o], c.fm[2][f] = fz_lo[f];
+      }
+#pragma unroll
+      for (int f = 0; f < 5; f++) fz_lo[f] = c.fp[2][f];
+      if constexpr (FULL) {
+        const double bl_x = left_x ? a.U[5 * n + id - 1] : 0.0;
+        const double bl_y = left_y ? a.U[6 * n + id - sy] : 0.0;
+        FullCell<P>(a, dt, i, j, k, c, e);
+        if (left_x) {  // Bx of (clo - 1, j, k)
+          EdgeSet l{};
+          l.ey_x = lx_ey, l.ez_x = lx_ez, l.ez_xy = lx_ez1, l.ey_xz = eds[1][ty][0];
+          lc.u[5] = bl_x;
+          FullCell<P>(a, dt, clo - 1, j, k, lc, l);
+        }
+        if (left_y) {  // By of (i, clo - 1, k)
+          EdgeSet l{};
+          l.ez_y = ly_ez, l.ez_xy = ly_ez1, l.ex_y = ly_ex, l.ex_yz = eds[0][0][tx];
+          lc.u[6] = bl_y;
+          FullCell<P>(a, dt, i, clo - 1, k, lc, l);
+        }
+      } else {
+        HalfCell<P>(a, dt, i, j, k, c, e);
+      }
+    }
+    if constexpr (!FULL) {
+#pragma unroll
+      for (int f = 0; f < 8; f++) own_k[f] = own_n[f];
+    }
+    read_k();  // plane k + 1 is the next plane k; LDS is rewritten only after the next barrier
+  }
 }
 
 }  // namespace mhd

@@ -301,10 +301,14 @@ __global__ void GasEnergyInitKernel(double *U, long nc)
 
 // Device-side time control for graph-captured stepping: dt = C_cfl / dti clamped
 // to the output time as in main.cpp:254-261, and t advanced by dt.
-__global__ void DtAdvanceKernel(const unsigned long long *dti_bits, double *dt_dev, double *t_dev, double tout,
-                                double cfl, int *n_step)
+// take: the accumulator is read and zeroed by one atomic exchange.  (A plain read followed by
+// a store of 0 is not safe here: the compiler makes the read a scalar load, and the vector
+// store may reach the cache before that load does.)
+__global__ void DtAdvanceKernel(unsigned long long *dti_bits, double *dt_dev, double *t_dev, double tout,
+                                double cfl, int *n_step, int take)
 {
-  double dti = __longlong_as_double((long long)*dti_bits);
+  const unsigned long long bits = take ? atomicExch(dti_bits, 0ull) : *dti_bits;
+  double dti = __longlong_as_double((long long)bits);
   double t   = *t_dev;
   double dt  = cfl / dti;
   if (t + dt > tout) dt = tout - t;
@@ -951,10 +955,10 @@ hipError_t LaunchDeSync(double *U, int nx, int ny, int nz, int ng, int flags, do
   return hipGetLastError();
 }
 
-hipError_t LaunchDtAdvance(const unsigned long long *dti, double *dt_dev, double *t_dev, double tout, double cfl,
-                           int *n_step, hipStream_t s)
+hipError_t LaunchDtAdvance(unsigned long long *dti, double *dt_dev, double *t_dev, double tout, double cfl,
+                           int *n_step, hipStream_t s, bool take)
 {
-  hipLaunchKernelGGL(DtAdvanceKernel, dim3(1), dim3(1), 0, s, dti, dt_dev, t_dev, tout, cfl, n_step);
+  hipLaunchKernelGGL(DtAdvanceKernel, dim3(1), dim3(1), 0, s, dti, dt_dev, t_dev, tout, cfl, n_step, int(take));
   return hipGetLastError();
 }
 

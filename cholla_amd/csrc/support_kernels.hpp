@@ -30,8 +30,10 @@ hipError_t LaunchDeSelect(double *U, int nx, int ny, int nz, int ng, hipStream_t
 hipError_t LaunchGasEnergyInit(double *U, long n_cells, hipStream_t s);
 hipError_t LaunchDeSync(double *U, int nx, int ny, int nz, int ng, int flags, double U_floor, double dx, double dy,
                         double dz, double gamma, unsigned long long *dti, hipStream_t s);
-hipError_t LaunchDtAdvance(const unsigned long long *dti, double *dt_dev, double *t_dev, double tout, double cfl,
-                           int *n_step, hipStream_t s);
+// take: read the accumulator and zero it in one atomic exchange (for a reduction that follows
+// in the same stream)
+hipError_t LaunchDtAdvance(unsigned long long *dti, double *dt_dev, double *t_dev, double tout, double cfl,
+                           int *n_step, hipStream_t s, bool take = false);
 hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const double *modes, double vscale, double rho,
                               double p, double gamma, hipStream_t s);
 // dti: reduce the next inverse dt into it (stage 2 with one block, or the DE energy sync);

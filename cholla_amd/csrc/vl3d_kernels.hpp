@@ -110,6 +110,7 @@ enum : int {
   kFlagConservedW         = 32,  // PCM: stage 1 stores momenta (exact form) or the total-energy pressure (DE)
   kFlagCrashRepair        = 64,  // a build without both floors: mark negative / NaN cells (kRepairAll)
   kFlagResetDti           = 128, // stage 1 zeroes the inverse-dt accumulator (a later kernel fills it)
+  kFlagFoldDt             = 256, // MHD: the stage-1 flux launch reduces the step's own inverse dt
 };
 
 __device__ __forceinline__ size_t Lin(int i, int j, int k, int nx, int ny)
