@@ -68,8 +68,6 @@ int CheckConfig(const chl_hydro_config &c)
   bool two_d = c.nz == 1 && c.ny > 1;
   if (two_d) {
     if (c.ny < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "2D boxes need ny > 2*n_ghost");
-    if (c.precision != CHL_PREC_FP64_EXACT)
-      return Fail(CHL_ERR_UNSUPPORTED, "2D boxes run the FP64 reference-order form only");
   } else if (!one_d) {
     if (c.ny < 2 * ng + 1 || c.nz < 2 * ng + 1) return Fail(CHL_ERR_INVALID, "3D boxes need ny, nz > 2*n_ghost");
     if (c.tile1 >= chl::kNumTiles || c.tile2 >= chl::kNumTiles) return Fail(CHL_ERR_INVALID, "tile index out of range");
@@ -80,9 +78,6 @@ int CheckConfig(const chl_hydro_config &c)
     if (simple_recon && (c.tile1 >= chl::kNumDeTiles || c.tile2 >= chl::kNumDeTiles))
       return Fail(CHL_ERR_UNSUPPORTED, "PCM, PLMP and PPMP are built for tile shapes 0 and 1 only");
     if (c.tile_order != 0 && c.tile_order != 1) return Fail(CHL_ERR_INVALID, "tile_order must be 0 or 1");
-  } else {
-    if (c.precision != CHL_PREC_FP64_EXACT)
-      return Fail(CHL_ERR_UNSUPPORTED, "1D boxes run the FP64 reference-order form only");
   }
   // 2^31 guard on per-axis products used in 32-bit grid arithmetic of launch shapes
   if (double(c.nx) * c.ny * c.nz * 5 > 9.0e15) return Fail(CHL_ERR_INVALID, "box too large");
@@ -195,7 +190,7 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
     // (stage 1 zeroes the accumulator in-kernel, so a captured step holds no memset node)
     bool fuse_dt = update_grid && (n <= 1024 || c.dual_energy);
     CHL_HIP(chl::LaunchVl1d(U, reinterpret_cast<double *>(h->W), n, ng, c.reconstruction, SolverKind(c.riemann),
-                            c.dual_energy != 0, dx, dt_dev, dt_host, gamma, fuse_dt ? h->dti() : nullptr,
+                            c.dual_energy != 0, c.precision, dx, dt_dev, dt_host, gamma, fuse_dt ? h->dti() : nullptr,
                             update_grid ? h->dti() : nullptr, h->grav, h->stream));
     if (update_grid && !fuse_dt)
       CHL_HIP(chl::LaunchCalcDt(U, n, 1, 1, ng, dx, dy, dz, gamma, h->dti(), h->num_cu, h->stream));
@@ -204,7 +199,7 @@ int EnqueueStep(chl_hydro *h, double *U, double dx, double dy, double dz, double
   }
   if (h->two_d) {  // no temperature floor: the reference's guard admits no 2D cell (hydro_cuda.cu:1130-1131)
     CHL_HIP(chl::LaunchVl2d(U, reinterpret_cast<double *>(h->W), c.nx, c.ny, ng, c.reconstruction,
-                            SolverKind(c.riemann), c.dual_energy != 0, dx, dy, dt_dev, dt_host, gamma,
+                            SolverKind(c.riemann), c.dual_energy != 0, c.precision, dx, dy, dt_dev, dt_host, gamma,
                             update_grid ? h->dti() : nullptr, h->grav, h->stream));
     if (fill) CHL_HIP(chl::LaunchGhostFill(U, c.nx, c.ny, c.nz, c.n_fields, ng, c.bc_flags, h->stream));
     return CHL_OK;
@@ -449,6 +444,8 @@ int chl_hydro_create(const chl_hydro_config *cfg, chl_hydro **out)
   size_t w_main    = 0;
   h->W_bytes       = chl_capi::HalfStepBytes(c, &w_main);
   if (h->two_d) h->W_bytes = 42 * h->n_cells * sizeof(double);  // half state, 4 interface, 2 flux arrays of 6 fields
+  if (h->one_d)  // the 1D integrator's half state is FP64 in every form
+    h->W_bytes = std::max(h->W_bytes, (5 + size_t(c.dual_energy)) * h->n_cells * sizeof(double));
   hipError_t e     = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&h->W, h->W_bytes);
   if (e == hipSuccess && c.dual_energy)

@@ -376,7 +376,67 @@ __global__ void TurbulenceIcKernel(double *U, int n, int ng, int n_modes, const 
   U[4 * nc + id] = (fmax(p, kTinyNumber) / (gamma - 1.)) + 0.5 * rho * (vx * vx + ((vy * vy) + (vz * vz)));
 }
 
-// ============================================================================ VL 1D
+// ============================================================================ VL 1D and 2D: arithmetic forms
+// The 1D and 2D integrators run in the three forms of the 3D one.  P is the form (with its
+// Riemann solver): PolicyExact is the reference's order in FP64 (the oracle's bits);
+// PolicyFast FP64 with the reciprocal-multiply and FMA arithmetic; PolicyMixed the
+// reconstruction and Riemann fluxes in FP32, with the conserved state and its updates, the
+// primitive (pressure) recovery and dt in FP64.  The mixed solvers take the split pressure
+// of the 3D kernels: each face's reference p0 is the high FP32 part of its left cell's FP64
+// pressure, both sides' momentum pressures are relative to it, and the normal-momentum flux
+// comes back less p0, added back here in FP64 (the sum of two FP32 values, exact).
+
+// The Riemann input of a cell's conserved values (the PCM states).  FP64 forms: the form's
+// own recovery.  Mixed: the FP64 recovery, then the state in FP32; plo the low part of the
+// FP64 pressure (p64 - p).
+template <class P, bool DE>
+__device__ __forceinline__ RState<typename P::T> SideFromCons(double d, double mx, double my, double mz, double E,
+                                                              double gamma, double dge, double &plo)
+{
+  using T = typename P::T;
+  if constexpr (sizeof(T) == sizeof(double)) {
+    plo = 0.0;
+    return RStateFromCons<P, DE>(d, mx, my, mz, E, gamma, dge);
+  } else {
+    const RState<double> s = RStateFromCons<PolicyFast, DE>(d, mx, my, mz, E, gamma, dge);
+    RState<T> r{};
+    r.d = T(s.d), r.mx = T(s.mx), r.my = T(s.my), r.mz = T(s.mz), r.E = T(s.E);
+    r.vx = T(s.vx), r.vy = T(s.vy), r.vz = T(s.vz), r.p = T(s.p), r.inv_d = T(s.inv_d), r.dge = T(s.dge);
+    plo = s.p - double(r.p);
+    return r;
+  }
+}
+
+// The flux of one face in FP64 from its two Riemann inputs (the split pressure's p0 added
+// back to the normal-momentum flux); fge the GasEnergy flux under DE.
+template <class P, bool DE>
+__device__ __forceinline__ Cons<double> FluxFp64(const RState<typename P::T> &l, const RState<typename P::T> &r,
+                                                 typename P::T gamma, double *fge)
+{
+  using T = typename P::T;
+  T g     = T(0);
+  const Cons<T> f = RiemannFlux<P, DE>(l, r, gamma, &g);
+  Cons<double> o{double(f.d), double(f.mx), double(f.my), double(f.mz), double(f.E)};
+  if constexpr (SplitPressure<P>) o.mx += double(l.p0);
+  *fge = double(g);
+  return o;
+}
+
+// The flux between two cells' PCM states a (left) and b (right), with the low parts of
+// their FP64 pressures (the 3D kernels' PcmFaceFlux split).
+template <class P, bool DE>
+__device__ __forceinline__ Cons<double> PcmFlux(RState<typename P::T> a, double alo, RState<typename P::T> b,
+                                                double blo, typename P::T gamma, double *fge)
+{
+  using T = typename P::T;
+  if constexpr (SplitPressure<P>) {
+    a.p0 = b.p0 = a.p;
+    b.pd        = (b.p - a.p) + T(blo);
+    a.pd        = T(alo);
+  }
+  return FluxFp64<P, DE>(a, b, gamma, fge);
+}
+
 // Stage 1 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:64-91): PCM states, the Riemann
 // solve of P (HLLC, or another solver under a SolverPolicy) on faces id-1/2 and id+1/2
 // of every cell 1..n-2 and the dt/2 update; W holds the conserved half state.  With
@@ -387,19 +447,21 @@ template <class P, bool DE>
 __global__ void Vl1dStage1Kernel(const double *U, double *W, int n, double dx, const double *dt_dev, double dt_host,
                                  double gamma, unsigned long long *dti_reset)
 {
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  using T = typename P::T;
+  int id  = blockIdx.x * blockDim.x + threadIdx.x;
   if (dti_reset && id == 0) *dti_reset = 0ull;  // a later launch of this step reduces into it
   if (id < 1 || id >= n - 1) return;
   const double dt    = 0.5 * (dt_dev ? *dt_dev : dt_host);
   const double dtodx = dt / dx;
-  auto side = [&](int c) {
-    return RStateFromCons<PolicyExact, DE>(U[c], U[n + c], U[2 * n + c], U[3 * n + c], U[4 * n + c], gamma,
-                                           DE ? U[5 * n + c] : 0.0);
+  double lo[3];
+  auto side = [&](int c, double &plo) {
+    return SideFromCons<P, DE>(U[c], U[n + c], U[2 * n + c], U[3 * n + c], U[4 * n + c], gamma,
+                               DE ? U[5 * n + c] : 0.0, plo);
   };
-  RState<double> a = side(id - 1), b = side(id), cc = side(id + 1);
+  RState<T> a = side(id - 1, lo[0]), b = side(id, lo[1]), cc = side(id + 1, lo[2]);
   double gm = 0.0, gp = 0.0;
-  Cons<double> fm = RiemannFlux<P, DE>(a, b, gamma, &gm);
-  Cons<double> fp = RiemannFlux<P, DE>(b, cc, gamma, &gp);
+  Cons<double> fm = PcmFlux<P, DE>(a, lo[0], b, lo[1], T(gamma), &gm);
+  Cons<double> fp = PcmFlux<P, DE>(b, lo[1], cc, lo[2], T(gamma), &gp);
   W[id]         = U[id] + dtodx * (fm.d - fp.d);
   W[n + id]     = U[n + id] + dtodx * (fm.mx - fp.mx);
   W[2 * n + id] = U[2 * n + id] + dtodx * (fm.my - fp.my);
@@ -414,51 +476,101 @@ __global__ void Vl1dStage1Kernel(const double *U, double *W, int n, double dx, c
   }
 }
 
-// The interface values of cell c from the conserved half state W: primitives as
-// Load_Data forms them (PLMC, PPMC) or as PLMP_cuda does (plmp_cuda.cu:104-120), then
-// the reconstruction; PCM passes the conserved values through (pcm_cuda.cu:13-97).
-// With dual energy (PLMC and PPMC only) the pressure is Load_Data's DE pressure and the
-// specific GasEnergy is reconstructed beside (reconstruction.h:179-191, plmc_cuda.cu,
-// ppmc_cuda.cu:668-672).
-template <int RK, bool DE = false>
-__device__ __forceinline__ void Recon1D(const double *W, int n, int c, double gamma, Prim<double> &L, Prim<double> &R,
-                                        double &geL, double &geR)
+// The interface values of cell c from the conserved half state W (fields nf apart) along a
+// line of stride st, whose normal and transverse momenta are the fields o1, o2, o3:
+// primitives as Load_Data forms them (PLMC, PPMC) or as PLMP_cuda does
+// (plmp_cuda.cu:104-120), in FP64, then the reconstruction in the form's precision; PCM
+// passes the conserved values through in the exact form (pcm_cuda.cu:13-97) and the
+// primitives in the others.  With dual energy (PLMC and PPMC only) the pressure is
+// Load_Data's DE pressure and the specific GasEnergy is reconstructed beside
+// (reconstruction.h:179-191, plmc_cuda.cu, ppmc_cuda.cu:668-672).  pc: the cell's own
+// pressure in the form's precision (the mixed form's face reference).
+template <class P, int RK, bool DE = false>
+__device__ __forceinline__ void ReconLine(const double *W, long nf, long c, long st, int o1, int o2, int o3,
+                                          double gamma, Prim<typename P::T> &L, Prim<typename P::T> &R,
+                                          typename P::T &geL, typename P::T &geR, typename P::T &pc)
 {
+  using T          = typename P::T;
   constexpr int r  = ReconTraits<RK>::kRadius;
   constexpr int NS = 2 * r + 1;
   static_assert(!DE || RK == kPLMC || RK == kPPMC, "dual energy is built for PLMC and PPMC");
-  Prim<double> s[NS];
-  double ge[NS];
+  Prim<T> s[NS];
+  Prim<double> w[NS];  // the FP64 primitives (the forms other than the exact PCM pass-through)
+  T ge[NS];
 #pragma unroll
   for (int m = 0; m < NS; m++) {
-    int q = c - r + m;
-    const double d = W[q], mx = W[n + q], my = W[2 * n + q], mz = W[3 * n + q], E = W[4 * n + q];
-    if constexpr (RK == kPCM) {
+    const long q   = c + long(m - r) * st;
+    const double d = W[q], mx = W[o1 * nf + q], my = W[o2 * nf + q], mz = W[o3 * nf + q], E = W[4 * nf + q];
+    if constexpr (RK == kPCM && P::kExact) {
       s[m] = {d, mx, my, mz, E};
+      continue;
     } else if constexpr (RK == kPLMP || RK == kPPMP) {
       const double vx = mx / d, vy = my / d, vz = mz / d;
-      s[m] = {d, vx, vy, vz, Fmax((E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0), kTinyNumber)};
+      w[m] = {d, vx, vy, vz, Fmax((E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0), kTinyNumber)};
     } else if constexpr (DE) {
-      const double vx = mx / d, vy = my / d, vz = mz / d, g = W[5 * n + q];
-      s[m]  = {d, vx, vy, vz, PressureFromDE(E, E - 0.5 * d * SquareMag(vx, vy, vz), g, gamma)};
-      ge[m] = g / d;
+      const double vx = mx / d, vy = my / d, vz = mz / d, g = W[5 * nf + q];
+      w[m]  = {d, vx, vy, vz, PressureFromDE(E, E - 0.5 * d * SquareMag(vx, vy, vz), g, gamma)};
+      ge[m] = T(g / d);
     } else {
-      s[m] = LoadPrim(d, mx, my, mz, E, gamma);
+      w[m] = LoadPrim(d, mx, my, mz, E, gamma);
     }
+    s[m] = {T(w[m].d), T(w[m].vx), T(w[m].vy), T(w[m].vz), T(w[m].p)};
   }
+  pc = s[r].p;
+  const T g = T(gamma);
   if constexpr (RK == kPCM) {
     L = R = s[0];
   } else if constexpr (RK == kPLMP) {
-    ReconPlmp<PolicyExact>(s, L, R);
+    ReconPlmp<P>(s, L, R);
   } else if constexpr (RK == kPPMP) {
-    ReconPpmp<PolicyExact>(s, L, R);
+    ReconPpmp<P>(s, L, R);
+  } else if constexpr (RK == kPPMC && SplitPressure<P>) {
+    // the mixed form's PPMC works on deviations from the centre cell (ReconPpmcCore), here
+    // taken in FP64, with the centre's state for the eigenvectors; the centre is added back
+    Prim<T> dv[NS];
+#pragma unroll
+    for (int m = 0; m < NS; m++)
+      dv[m] = {T(w[m].d - w[r].d), T(w[m].vx - w[r].vx), T(w[m].vy - w[r].vy), T(w[m].vz - w[r].vz),
+               T(w[m].p - w[r].p)};
+    ReconPpmcCore<P>(dv, s[r], g, L, R);
+    const Prim<T> &q = s[r];
+    L = {L.d + q.d, L.vx + q.vx, L.vy + q.vy, L.vz + q.vz, L.p + q.p};
+    R = {R.d + q.d, R.vx + q.vx, R.vy + q.vy, R.vz + q.vz, R.p + q.p};
+    L.d = Fmax(L.d, T(kTinyNumber)), R.d = Fmax(R.d, T(kTinyNumber));
+    L.p = Fmax(L.p, T(kTinyNumber)), R.p = Fmax(R.p, T(kTinyNumber));
+    if constexpr (DE) ReconPpmcScalar<P>(ge, geL, geR);
   } else if constexpr (RK == kPPMC) {
-    ReconPpmc<PolicyExact>(s, gamma, L, R);
-    if constexpr (DE) ReconPpmcScalar<PolicyExact>(ge, geL, geR);
+    ReconPpmc<P>(s, g, L, R);
+    if constexpr (DE) ReconPpmcScalar<P>(ge, geL, geR);
   } else {
-    ReconPlmc<PolicyExact>(s, gamma, L, R);
-    if constexpr (DE) ReconPlmcScalar<PolicyExact>(ge, geL, geR);
+    ReconPlmc<P>(s, g, L, R);
+    if constexpr (DE) ReconPlmcScalar<P>(ge, geL, geR);
   }
+}
+
+// The flux of the face between a cell whose interface value there is L and the next cell,
+// whose value there is R; p0 the left cell's own pressure (the mixed form's reference).
+template <class P, int RK, bool DE>
+__device__ __forceinline__ Cons<double> ReconFlux(const Prim<typename P::T> &L, const Prim<typename P::T> &R,
+                                                  typename P::T geL, typename P::T geR, typename P::T p0,
+                                                  typename P::T gamma, typename P::T inv_gm1, double *fge)
+{
+  using T = typename P::T;
+  RState<T> l, r;
+  if constexpr (SplitPressure<P>) {
+    Prim<T> a = L, b = R;
+    a.p       = Fmax(L.p, T(kTinyNumber));
+    b.p       = Fmax(R.p, T(kTinyNumber));
+    l         = RStateFromPrim<P, DE>(a, gamma, inv_gm1, geL);
+    r         = RStateFromPrim<P, DE>(b, gamma, inv_gm1, geR);
+    l.p0 = r.p0 = p0;
+    l.pd        = Fmax(l.p - p0, -p0);
+    r.pd        = Fmax(r.p - p0, -p0);
+  } else {
+    l = IfaceState<P, RK, DE>(L, gamma, inv_gm1, geL);
+    r = IfaceState<P, RK, DE>(R, gamma, inv_gm1, geR);
+  }
+  return FluxFp64<P, DE>(l, r, gamma, fge);
 }
 
 // Stage 2 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:95-137) for the real cells, with the
@@ -469,21 +581,20 @@ template <class P, int RK, bool DE>
 __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, double dx, const double *dt_dev,
                                  double dt_host, double gamma, unsigned long long *dti, GravSpec grav)
 {
-  int id       = blockIdx.x * blockDim.x + threadIdx.x;
-  double m     = 0.0;
+  using T         = typename P::T;
+  int id          = blockIdx.x * blockDim.x + threadIdx.x;
+  double m        = 0.0;
   const double dt = dt_dev ? *dt_dev : dt_host;
   if (id >= ng && id < n - ng) {
-    const double inv_gm1 = 1.0 / (gamma - 1.0);
-    Prim<double> Lm, Rm, L0, R0, Lp, Rp;
-    double gLm = 0, gRm = 0, gL0 = 0, gR0 = 0, gLp = 0, gRp = 0;
-    Recon1D<RK, DE>(W, n, id - 1, gamma, Lm, Rm, gLm, gRm);
-    Recon1D<RK, DE>(W, n, id, gamma, L0, R0, gL0, gR0);
-    Recon1D<RK, DE>(W, n, id + 1, gamma, Lp, Rp, gLp, gRp);
+    const T g = T(gamma), inv_gm1 = T(1.0 / (gamma - 1.0));
+    Prim<T> Lm, Rm, L0, R0, Lp, Rp;
+    T gLm = 0, gRm = 0, gL0 = 0, gR0 = 0, gLp = 0, gRp = 0, pm, p0, pp;
+    ReconLine<P, RK, DE>(W, n, id - 1, 1, 1, 2, 3, gamma, Lm, Rm, gLm, gRm, pm);
+    ReconLine<P, RK, DE>(W, n, id, 1, 1, 2, 3, gamma, L0, R0, gL0, gR0, p0);
+    ReconLine<P, RK, DE>(W, n, id + 1, 1, 1, 2, 3, gamma, Lp, Rp, gLp, gRp, pp);
     double gm = 0.0, gp = 0.0;
-    Cons<double> fm = RiemannFlux<P, DE>(IfaceState<PolicyExact, RK, DE>(Lm, gamma, inv_gm1, gLm),
-                                         IfaceState<PolicyExact, RK, DE>(R0, gamma, inv_gm1, gR0), gamma, &gm);
-    Cons<double> fp = RiemannFlux<P, DE>(IfaceState<PolicyExact, RK, DE>(L0, gamma, inv_gm1, gL0),
-                                         IfaceState<PolicyExact, RK, DE>(Rp, gamma, inv_gm1, gRp), gamma, &gp);
+    Cons<double> fm = ReconFlux<P, RK, DE>(Lm, R0, gLm, gR0, pm, g, inv_gm1, &gm);
+    Cons<double> fp = ReconFlux<P, RK, DE>(L0, Rp, gL0, gRp, p0, g, inv_gm1, &gp);
     const double dtodx = dt / dx;
     double un[5];
     un[0] = U[id] + dtodx * (fm.d - fp.d);
@@ -493,8 +604,8 @@ __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, doub
     un[4] = U[4 * n + id] + dtodx * (fm.E - fp.E);
     if (grav.id != 0) {  // STATIC_GRAV, hydro_cuda.cu:59-67
       const double start[4] = {U[id], U[n + id], U[2 * n + id], U[3 * n + id]};
-      const double g[1]     = {GravField1D(grav.id, grav.Pos(0, id, ng, dx))};
-      GravSource<1>(un, start, g, dt);
+      const double gf[1]    = {GravField1D(grav.id, grav.Pos(0, id, ng, dx))};
+      GravSource<1>(un, start, gf, dt);
     }
     for (int q = 0; q < 5; q++) U[q * n + id] = un[q];
     if constexpr (DE) {
@@ -571,23 +682,52 @@ template <class P, bool DE = false>
 __global__ void Vl2dPcmFluxKernel(const double *U, double *Fx, double *Fy, int nx, int ny, double gamma,
                                   unsigned long long *dti_reset)
 {
+  using T       = typename P::T;
   const long nc = long(nx) * ny;
   const long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (dti_reset && id == 0) *dti_reset = 0ull;
   if (id >= nc) return;
   const int j = int(id / nx), i = int(id - long(j) * nx);
-  auto state = [&](long c, int o1, int o2, int o3) {
-    return RStateFromCons<PolicyExact, DE>(U[c], U[o1 * nc + c], U[o2 * nc + c], U[o3 * nc + c], U[4 * nc + c], gamma,
-                                           DE ? U[5 * nc + c] : 0.0);
+  auto state = [&](long c, int o1, int o2, int o3, double &plo) {
+    return SideFromCons<P, DE>(U[c], U[o1 * nc + c], U[o2 * nc + c], U[o3 * nc + c], U[4 * nc + c], gamma,
+                               DE ? U[5 * nc + c] : 0.0, plo);
   };
   auto solve = [&](double *F, long other, int o1, int o2, int o3) {
-    double fge     = 0.0;
-    Cons<double> f = RiemannFlux<P, DE>(state(id, o1, o2, o3), state(other, o1, o2, o3), gamma, &fge);
+    double fge = 0.0, alo, blo;
+    const RState<T> a = state(id, o1, o2, o3, alo), b = state(other, o1, o2, o3, blo);
+    const Cons<double> f = PcmFlux<P, DE>(a, alo, b, blo, T(gamma), &fge);
     F[id] = f.d, F[o1 * nc + id] = f.mx, F[o2 * nc + id] = f.my, F[o3 * nc + id] = f.mz, F[4 * nc + id] = f.E;
     if constexpr (DE) F[5 * nc + id] = fge;
   };
   if (i < nx - 1) solve(Fx, id + 1, 1, 2, 3);
   if (j < ny - 1) solve(Fy, id + nx, 2, 3, 1);
+}
+
+// Stage 2 of the 2D integrator in the fast and mixed forms: each face's two interface values
+// reconstructed from the half state along its line (ReconLine, as the 1D integrator does) and
+// solved in one launch, without the exact form's interface arrays; the x faces i+1/2 with
+// i in [ng-1, nx-ng-1] of the real rows and the y faces likewise, which the update reads.
+template <class P, int RK, bool DE>
+__global__ void Vl2dFlux2Kernel(const double *Uh, double *Fx, double *Fy, int nx, int ny, int ng, double gamma)
+{
+  using T       = typename P::T;
+  const long nc = long(nx) * ny;
+  const long id = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= nc) return;
+  const int j = int(id / nx), i = int(id - long(j) * nx);
+  const T g = T(gamma), inv_gm1 = T(1.0 / (gamma - 1.0));
+  auto face = [&](double *F, long st, int o1, int o2, int o3) {
+    Prim<T> La, Ra, Lb, Rb;
+    T gLa = 0, gRa = 0, gLb = 0, gRb = 0, pa, pb;
+    ReconLine<P, RK, DE>(Uh, nc, id, st, o1, o2, o3, gamma, La, Ra, gLa, gRa, pa);
+    ReconLine<P, RK, DE>(Uh, nc, id + st, st, o1, o2, o3, gamma, Lb, Rb, gLb, gRb, pb);
+    double fge          = 0.0;
+    const Cons<double> f = ReconFlux<P, RK, DE>(La, Rb, gLa, gRb, pa, g, inv_gm1, &fge);
+    F[id] = f.d, F[o1 * nc + id] = f.mx, F[o2 * nc + id] = f.my, F[o3 * nc + id] = f.mz, F[4 * nc + id] = f.E;
+    if constexpr (DE) F[5 * nc + id] = fge;
+  };
+  if (j >= ng && j < ny - ng && i >= ng - 1 && i < nx - ng) face(Fx, 1, 1, 2, 3);
+  if (i >= ng && i < nx - ng && j >= ng - 1 && j < ny - ng) face(Fy, nx, 2, 3, 1);
 }
 
 // Update_Conserved_Variables_2D_half, VL_2D_cuda.cu:183-256 (no DE): cells 1..n-2 along
@@ -1027,26 +1167,37 @@ static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, do
   }
 }
 
-hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, bool de, double dx,
+template <class B, bool DE>
+static hipError_t LaunchVl1dS(int riemann, double *U, double *W, int n, int ng, int recon, double dx,
+                              const double *dt_dev, double dt, double gamma, unsigned long long *dti,
+                              unsigned long long *dti_reset, const GravSpec &grav, hipStream_t s)
+{
+  if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<B>, DE>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+  if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<B>, DE>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+  if (riemann == kRiemannExact)
+    return LaunchVl1dP<ExactPolicy<B>, DE>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+  if (riemann == kRiemannHllc) return LaunchVl1dP<B, DE>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, bool de, int form, double dx,
                       const double *dt_dev, double dt, double gamma, unsigned long long *dti,
-                      unsigned long long *dti_reset, const GravSpec &grav,
-                      hipStream_t s)
+                      unsigned long long *dti_reset, const GravSpec &grav, hipStream_t s)
 {
   if (dti && n > 1024 && !de) return hipErrorInvalidValue;
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPPMP && recon != kPCM) return hipErrorInvalidValue;
-  if (de) {  // every solver with PLMC or PPMC
-    if (recon != kPPMC && recon != kPLMC) return hipErrorInvalidValue;
-    if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
-    if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<PolicyExact>, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
-    if (riemann == kRiemannExact) return LaunchVl1dP<ExactPolicy<PolicyExact>, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
-    if (riemann != kRiemannHllc) return hipErrorInvalidValue;
-    return LaunchVl1dP<PolicyExact, true>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+  if (de && recon != kPPMC && recon != kPLMC) return hipErrorInvalidValue;  // every solver with PLMC or PPMC
+  switch (form) {
+    case PolicyExact::kId:
+      return de ? LaunchVl1dS<PolicyExact, true>(riemann, U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s)
+                : LaunchVl1dS<PolicyExact, false>(riemann, U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+    case PolicyFast::kId:
+      return de ? LaunchVl1dS<PolicyFast, true>(riemann, U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s)
+                : LaunchVl1dS<PolicyFast, false>(riemann, U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
+    case PolicyMixed::kId:
+      return de ? LaunchVl1dS<PolicyMixed, true>(riemann, U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s)
+                : LaunchVl1dS<PolicyMixed, false>(riemann, U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
   }
-  if (riemann == kRiemannHll) return LaunchVl1dP<HllPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
-  if (riemann == kRiemannRoe) return LaunchVl1dP<RoePolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
-  if (riemann == kRiemannExact)
-    return LaunchVl1dP<ExactPolicy<PolicyExact>, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
-  if (riemann == kRiemannHllc) return LaunchVl1dP<PolicyExact, false>(U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s);
   return hipErrorInvalidValue;
 }
 
@@ -1064,6 +1215,22 @@ static hipError_t LaunchVl2dP(double *U, double *scratch, int nx, int ny, int ng
                      gamma);
   if (recon == kPCM) {
     hipLaunchKernelGGL((Vl2dPcmFluxKernel<P, DE>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, gamma, nullptr);
+  } else if constexpr (!P::kExact) {  // the fast and mixed forms: one launch, no interface arrays
+    if (recon == kPPMC) {
+      hipLaunchKernelGGL((Vl2dFlux2Kernel<P, kPPMC, DE>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, ng, gamma);
+    } else if (recon == kPLMC) {
+      hipLaunchKernelGGL((Vl2dFlux2Kernel<P, kPLMC, DE>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, ng, gamma);
+    } else if constexpr (!DE) {
+      if (recon == kPLMP) {
+        hipLaunchKernelGGL((Vl2dFlux2Kernel<P, kPLMP, false>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, ng, gamma);
+      } else if (recon == kPPMP) {
+        hipLaunchKernelGGL((Vl2dFlux2Kernel<P, kPPMP, false>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, ng, gamma);
+      } else {
+        return hipErrorInvalidValue;
+      }
+    } else {
+      return hipErrorInvalidValue;
+    }
   } else {
     if constexpr (DE) {  // PLMC or PPMC (the options refuse PLMP with DE)
       if (recon == kPPMC) {
@@ -1091,32 +1258,41 @@ static hipError_t LaunchVl2dP(double *U, double *scratch, int nx, int ny, int ng
   return hipGetLastError();
 }
 
-// VL_Algorithm_2D_CUDA (VL_2D_cuda.cu:24-168) in the exact form; scratch holds 7 x 6 x nx*ny
+template <class B, bool DE>
+static hipError_t LaunchVl2dS(int riemann, double *U, double *scratch, int nx, int ny, int ng, int recon, double dx,
+                              double dy, const double *dt_dev, double dt, double gamma, unsigned long long *dti,
+                              const GravSpec &grav, hipStream_t s)
+{
+  if (riemann == kRiemannHll)
+    return LaunchVl2dP<HllPolicy<B>, DE>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+  if (riemann == kRiemannRoe)
+    return LaunchVl2dP<RoePolicy<B>, DE>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+  if (riemann == kRiemannExact)
+    return LaunchVl2dP<ExactPolicy<B>, DE>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+  if (riemann == kRiemannHllc)
+    return LaunchVl2dP<B, DE>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+  return hipErrorInvalidValue;
+}
+
+// VL_Algorithm_2D_CUDA (VL_2D_cuda.cu:24-168) in the form `form'; scratch holds 7 x 6 x nx*ny
 // doubles (half state, four interface arrays, two flux arrays, 6 fields of room each).
-hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, bool de, double dx,
-                      double dy, const double *dt_dev, double dt, double gamma, unsigned long long *dti,
+hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, bool de, int form,
+                      double dx, double dy, const double *dt_dev, double dt, double gamma, unsigned long long *dti,
                       const GravSpec &grav, hipStream_t s)
 {
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPPMP && recon != kPCM) return hipErrorInvalidValue;
-  if (de) {  // every solver with PCM, PLMC or PPMC
-    if (recon == kPLMP || recon == kPPMP) return hipErrorInvalidValue;
-    if (riemann == kRiemannHll)
-      return LaunchVl2dP<HllPolicy<PolicyExact>, true>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
-    if (riemann == kRiemannRoe)
-      return LaunchVl2dP<RoePolicy<PolicyExact>, true>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
-    if (riemann == kRiemannExact)
-      return LaunchVl2dP<ExactPolicy<PolicyExact>, true>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
-    if (riemann != kRiemannHllc) return hipErrorInvalidValue;
-    return LaunchVl2dP<PolicyExact, true>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+  if (de && (recon == kPLMP || recon == kPPMP)) return hipErrorInvalidValue;  // every solver with PCM, PLMC or PPMC
+  switch (form) {
+    case PolicyExact::kId:
+      return de ? LaunchVl2dS<PolicyExact, true>(riemann, U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s)
+                : LaunchVl2dS<PolicyExact, false>(riemann, U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+    case PolicyFast::kId:
+      return de ? LaunchVl2dS<PolicyFast, true>(riemann, U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s)
+                : LaunchVl2dS<PolicyFast, false>(riemann, U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
+    case PolicyMixed::kId:
+      return de ? LaunchVl2dS<PolicyMixed, true>(riemann, U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s)
+                : LaunchVl2dS<PolicyMixed, false>(riemann, U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   }
-  if (riemann == kRiemannHll)
-    return LaunchVl2dP<HllPolicy<PolicyExact>, false>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
-  if (riemann == kRiemannRoe)
-    return LaunchVl2dP<RoePolicy<PolicyExact>, false>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
-  if (riemann == kRiemannExact)
-    return LaunchVl2dP<ExactPolicy<PolicyExact>, false>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
-  if (riemann == kRiemannHllc)
-    return LaunchVl2dP<PolicyExact, false>(U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s);
   return hipErrorInvalidValue;
 }
 

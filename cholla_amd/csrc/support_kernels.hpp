@@ -38,11 +38,12 @@ hipError_t LaunchTurbulenceIc(double *U, int n, int ng, int n_modes, const doubl
                               double p, double gamma, hipStream_t s);
 // dti: reduce the next inverse dt into it (stage 2 with one block, or the DE energy sync);
 // dti_reset: zero it first, from stage 1 (both may be the same pointer).
-hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, bool de, double dx,
+// form: the arithmetic form's id (PolicyExact::kId, PolicyFast::kId, PolicyMixed::kId)
+hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int riemann, bool de, int form, double dx,
                       const double *dt_dev, double dt, double gamma, unsigned long long *dti,
                       unsigned long long *dti_reset, const GravSpec &grav, hipStream_t s);
-hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, bool de, double dx,
-                      double dy, const double *dt_dev, double dt, double gamma, unsigned long long *dti,
+hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int recon, int riemann, bool de, int form,
+                      double dx, double dy, const double *dt_dev, double dt, double gamma, unsigned long long *dti,
                       const GravSpec &grav, hipStream_t s);
 hipError_t LaunchReconGrid(int recon, const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma,
                            int dir, hipStream_t s);

@@ -53,7 +53,7 @@ extern "C" {
 
 typedef struct chl_hydro_config {
   int nx, ny, nz;        /* total cells per axis incl. ghosts (Header::nx, ny, nz); 1D: ny = nz = 1,
-                            2D: nz = 1 (1D and 2D boxes run the FP64 reference-order form) */
+                            2D: nz = 1 (every precision; the 1D half state is FP64 in every form) */
   int n_ghost;           /* Header::n_ghost: PLMC 3, PPMC 4 (src/grid/grid3D.cpp:53-76) */
   int n_fields;          /* 5 (hydro), or 6 with dual_energy (GasEnergy last); no scalars */
   int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL.  PCM, PLMP and PPMP: no

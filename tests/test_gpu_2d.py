@@ -71,8 +71,8 @@ def test_kh_2d_transmissive_reflective_bitwise():
 
 
 def test_2d_limits():
-    with pytest.raises(chl.ChollaError, match="2D boxes"):
-        chl.Hydro(40, 40, 1, 4, chl.RECON_PPMC, chl.PREC_MIXED)
+    with pytest.raises(chl.ChollaError, match="2D boxes need ny"):
+        chl.Hydro(40, 8, 1, 4, chl.RECON_PPMC, chl.PREC_MIXED)
     with pytest.raises(chl.ChollaError, match="dual energy"):
         chl.Hydro(40, 40, 1, 3, chl.RECON_PLMP, chl.PREC_FP64_EXACT, dual_energy=True)
 

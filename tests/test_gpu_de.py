@@ -208,8 +208,8 @@ def test_de_uniform_supersonic_fixed_point(precision):
 def test_de_config_checks():
     with pytest.raises(chl.ChollaError):
         chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, dual_energy=True, tile1=2)
-    with pytest.raises(chl.ChollaError, match="1D boxes"):  # 1D runs the exact form only, DE or not
-        chl.Hydro(100, 1, 1, 3, chl.RECON_PLMC, chl.PREC_FP64, dual_energy=True)
+    with pytest.raises(chl.ChollaError, match="PLMC and PPMC"):  # the 1D DE step: PLMC or PPMC, in every form
+        chl.Hydro(100, 1, 1, 3, chl.RECON_PCM, chl.PREC_FP64, dual_energy=True)
 
 
 # ----------------------------------------------------------------------------- 1D
