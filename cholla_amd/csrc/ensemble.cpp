@@ -104,8 +104,6 @@ int CheckEnsembleConfig(const chl_ensemble_config &c)
       return Fail(CHL_ERR_UNSUPPORTED,
                   "the HLL, Roe and exact solvers and PCM, PLMP and PPMP are built for tile shapes 0 and 1 only");
   }
-  if ((c.reconstruction == CHL_RECON_PLMP || c.reconstruction == CHL_RECON_PPMP) && c.dual_energy)
-    return Fail(CHL_ERR_UNSUPPORTED, "PLMP and PPMP are built without dual energy");
   if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
   if (c.positivity_fallback != 0 && c.positivity_fallback != 1)
     return Fail(CHL_ERR_INVALID, "positivity_fallback must be 0 or 1");

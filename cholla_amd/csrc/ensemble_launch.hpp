@@ -84,17 +84,13 @@ hipError_t EnsStage1(const EnsembleArgs &e, int tile, hipStream_t s)
 template <class P, int RK>
 hipError_t EnsStage2R(const EnsembleArgs &e, int tile, hipStream_t s)
 {
-  // PCM, PLMP, PPMP and the HLL, Roe and exact solvers: the first two tile shapes; dual energy
-  // with PCM and the characteristic reconstructions
+  // PCM, PLMP, PPMP and the HLL, Roe and exact solvers: the first two tile shapes, with or
+  // without dual energy
   if (RK == kPCM || RK == kPLMP || RK == kPPMP || e.riemann != kRiemannHllc) {
     if (tile > 1) return hipErrorInvalidValue;
     return WithSolver<P>(e.riemann, [&](auto q) {
       using Q = decltype(q);
-      if constexpr (RK == kPLMC || RK == kPPMC || RK == kPCM) {
-        if (e.de) return tile == 0 ? EnsStage2T<Q, RK, 32, 8, true>(e, s) : EnsStage2T<Q, RK, 16, 16, true>(e, s);
-      } else {
-        if (e.de) return hipErrorInvalidValue;
-      }
+      if (e.de) return tile == 0 ? EnsStage2T<Q, RK, 32, 8, true>(e, s) : EnsStage2T<Q, RK, 16, 16, true>(e, s);
       return tile == 0 ? EnsStage2T<Q, RK, 32, 8>(e, s) : EnsStage2T<Q, RK, 16, 16>(e, s);
     });
   }

@@ -731,6 +731,21 @@ __device__ __forceinline__ void ReconPlmcScalar(const typename P::T g[3], typena
   L_iph    = Fmin(Fmax(g[1], g[2]), L_iph);
 }
 
+// PLMP and PPMP reconstruct the specific GasEnergy as they do each primitive
+// (plmp_cuda.cu:204-206 with Interface_Values_PLM; ppmp_cuda.cu:353-361 with Calculate_Slope
+// and Interface_Values_PPM).
+template <class P>
+__device__ __forceinline__ void ReconPlmpScalar(const typename P::T g[3], typename P::T &L_iph, typename P::T &R_imh)
+{
+  PlmpInterface<P>(g[0], g[1], g[2], R_imh, L_iph);
+}
+
+template <class P>
+__device__ __forceinline__ void ReconPpmpScalar(const typename P::T g[5], typename P::T &L_iph, typename P::T &R_imh)
+{
+  PpmpInterfaces<P>(g, R_imh, L_iph);
+}
+
 template <class P>
 __device__ __forceinline__ void ReconPpmcScalar(const typename P::T g[5], typename P::T &L_iph, typename P::T &R_imh)
 {

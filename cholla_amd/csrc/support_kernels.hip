@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "de_cells.hpp"
 #include "ensemble_kernels.hpp"
@@ -493,7 +494,7 @@ __device__ __forceinline__ void ReconLine(const double *W, long nf, long c, long
   using T          = typename P::T;
   constexpr int r  = ReconTraits<RK>::kRadius;
   constexpr int NS = 2 * r + 1;
-  static_assert(!DE || RK == kPLMC || RK == kPPMC, "dual energy is built for PLMC and PPMC");
+  static_assert(!DE || RK != kPCM, "the PCM states carry the conserved GasEnergy (Vl2dPcmFluxKernel)");
   Prim<T> s[NS];
   Prim<double> w[NS];  // the FP64 primitives (the forms other than the exact PCM pass-through)
   T ge[NS];
@@ -506,7 +507,13 @@ __device__ __forceinline__ void ReconLine(const double *W, long nf, long c, long
       continue;
     } else if constexpr (RK == kPLMP || RK == kPPMP) {
       const double vx = mx / d, vy = my / d, vz = mz / d;
-      w[m] = {d, vx, vy, vz, Fmax((E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0), kTinyNumber)};
+      if constexpr (DE) {  // plmp_cuda.cu:116-121
+        const double g = W[5 * nf + q], E_kin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
+        w[m]  = {d, vx, vy, vz, Fmax(PressureFromDE(E, E - E_kin, g, gamma), kTinyNumber)};
+        ge[m] = T(g / d);
+      } else {
+        w[m] = {d, vx, vy, vz, Fmax((E - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0), kTinyNumber)};
+      }
     } else if constexpr (DE) {
       const double vx = mx / d, vy = my / d, vz = mz / d, g = W[5 * nf + q];
       w[m]  = {d, vx, vy, vz, PressureFromDE(E, E - 0.5 * d * SquareMag(vx, vy, vz), g, gamma)};
@@ -522,8 +529,10 @@ __device__ __forceinline__ void ReconLine(const double *W, long nf, long c, long
     L = R = s[0];
   } else if constexpr (RK == kPLMP) {
     ReconPlmp<P>(s, L, R);
+    if constexpr (DE) ReconPlmpScalar<P>(ge, geL, geR);
   } else if constexpr (RK == kPPMP) {
     ReconPpmp<P>(s, L, R);
+    if constexpr (DE) ReconPpmpScalar<P>(ge, geL, geR);
   } else if constexpr (RK == kPPMC && SplitPressure<P>) {
     // the mixed form's PPMC works on deviations from the centre cell (ReconPpmcCore), here
     // taken in FP64, with the centre's state for the eigenvectors; the centre is added back
@@ -935,10 +944,16 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
 #pragma unroll
   for (int m = 0; m < NS; m++) {
     size_t id = Lin(xid + (m - r) * ix, yid + (m - r) * iy, zid + (m - r) * iz, nx, ny);
-    if constexpr (RK == kPLMP || RK == kPPMP) {  // plmp_cuda.cu:104-120
+    if constexpr (RK == kPLMP || RK == kPPMP) {  // plmp_cuda.cu:104-132
       const double d = U[id], vx = U[o1 * nc + id] / d, vy = U[o2 * nc + id] / d, vz = U[o3 * nc + id] / d;
-      s[m] = {d, vx, vy, vz, Fmax((U[4 * nc + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0),
-                                  kTinyNumber)};
+      if constexpr (DE) {  // plmp_cuda.cu:116-121: Get_Pressure_From_DE
+        const double E = U[4 * nc + id], E_kin = 0.5 * d * (vx * vx + vy * vy + vz * vz), g = U[5 * nc + id];
+        s[m]  = {d, vx, vy, vz, Fmax(PressureFromDE(E, E - E_kin, g, gamma), kTinyNumber)};
+        ge[m] = g / d;
+      } else {
+        s[m] = {d, vx, vy, vz, Fmax((U[4 * nc + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0),
+                                    kTinyNumber)};
+      }
     } else if constexpr (DE) {  // Load_Data's DE branch, reconstruction.h:179-191
       const double d = U[id], vx = U[o1 * nc + id] / d, vy = U[o2 * nc + id] / d, vz = U[o3 * nc + id] / d;
       const double E = U[4 * nc + id], g = U[5 * nc + id];
@@ -955,8 +970,10 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
     if constexpr (DE) ReconPpmcScalar<PolicyExact>(ge, gL, gR);
   } else if constexpr (RK == kPLMP) {
     ReconPlmp<PolicyExact>(s, L, R);
+    if constexpr (DE) ReconPlmpScalar<PolicyExact>(ge, gL, gR);
   } else if constexpr (RK == kPPMP) {
     ReconPpmp<PolicyExact>(s, L, R);
+    if constexpr (DE) ReconPpmpScalar<PolicyExact>(ge, gL, gR);
   } else {
     ReconPlmc<PolicyExact>(s, gamma, L, R);
     if constexpr (DE) ReconPlmcScalar<PolicyExact>(ge, gL, gR);
@@ -975,7 +992,7 @@ __global__ void ReconGridKernel(const double *U, double *QL, double *QR, int nx,
   };
   write(L, QL, Lin(xid, yid, zid, nx, ny));
   write(R, QR, Lin(xid - ix, yid - iy, zid - iz, nx, ny));
-  if constexpr (DE) {  // Write_Data, reconstruction.h:960-962
+  if constexpr (DE) {  // Write_Data, reconstruction.h:960-962 (plmp_cuda.cu:228-229 likewise)
     QL[5 * nc + Lin(xid, yid, zid, nx, ny)]                = L.d * gL;
     QR[5 * nc + Lin(xid - ix, yid - iy, zid - iz, nx, ny)] = R.d * gR;
   }
@@ -1134,9 +1151,17 @@ static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, do
     if (recon == kPPMC) {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMC, true>), dim3(nb), dim3(256), 0, s, U, W, n, ng, dx, dt_dev, dt,
                          gamma, nullptr, grav);
-    } else {
+    } else if (recon == kPLMC) {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMC, true>), dim3(nb), dim3(256), 0, s, U, W, n, ng, dx, dt_dev, dt,
                          gamma, nullptr, grav);
+    } else if (recon == kPLMP) {
+      hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPLMP, true>), dim3(nb), dim3(256), 0, s, U, W, n, ng, dx, dt_dev, dt,
+                         gamma, nullptr, grav);
+    } else if (recon == kPPMP) {
+      hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMP, true>), dim3(nb), dim3(256), 0, s, U, W, n, ng, dx, dt_dev, dt,
+                         gamma, nullptr, grav);
+    } else {
+      return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(Vl1dSelectKernel, dim3(nb), dim3(256), 0, s, U, n, ng);
     hipLaunchKernelGGL(Vl1dSyncKernel, dim3(nb), dim3(256), 0, s, U, n, ng, dx, gamma, dti);
@@ -1186,7 +1211,7 @@ hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int rieman
 {
   if (dti && n > 1024 && !de) return hipErrorInvalidValue;
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPPMP && recon != kPCM) return hipErrorInvalidValue;
-  if (de && recon != kPPMC && recon != kPLMC) return hipErrorInvalidValue;  // every solver with PLMC or PPMC
+  if (de && recon == kPCM) return hipErrorInvalidValue;  // every solver with PLMC, PPMC, PLMP or PPMP
   switch (form) {
     case PolicyExact::kId:
       return de ? LaunchVl1dS<PolicyExact, true>(riemann, U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s)
@@ -1220,25 +1245,28 @@ static hipError_t LaunchVl2dP(double *U, double *scratch, int nx, int ny, int ng
       hipLaunchKernelGGL((Vl2dFlux2Kernel<P, kPPMC, DE>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, ng, gamma);
     } else if (recon == kPLMC) {
       hipLaunchKernelGGL((Vl2dFlux2Kernel<P, kPLMC, DE>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, ng, gamma);
-    } else if constexpr (!DE) {
-      if (recon == kPLMP) {
-        hipLaunchKernelGGL((Vl2dFlux2Kernel<P, kPLMP, false>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, ng, gamma);
-      } else if (recon == kPPMP) {
-        hipLaunchKernelGGL((Vl2dFlux2Kernel<P, kPPMP, false>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, ng, gamma);
-      } else {
-        return hipErrorInvalidValue;
-      }
+    } else if (recon == kPLMP) {
+      hipLaunchKernelGGL((Vl2dFlux2Kernel<P, kPLMP, DE>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, ng, gamma);
+    } else if (recon == kPPMP) {
+      hipLaunchKernelGGL((Vl2dFlux2Kernel<P, kPPMP, DE>), dim3(nb), dim3(256), 0, s, Uh, Fx, Fy, nx, ny, ng, gamma);
     } else {
       return hipErrorInvalidValue;
     }
   } else {
-    if constexpr (DE) {  // PLMC or PPMC (the options refuse PLMP with DE)
+    if constexpr (DE) {
+      auto both = [&](auto rk) {
+        constexpr int RK = decltype(rk)::value;
+        hipLaunchKernelGGL((ReconGridKernel<RK, true>), dim3(nb64), dim3(64), 0, s, Uh, QLx, QRx, nx, ny, 1, gamma, 0);
+        hipLaunchKernelGGL((ReconGridKernel<RK, true>), dim3(nb64), dim3(64), 0, s, Uh, QLy, QRy, nx, ny, 1, gamma, 1);
+      };
       if (recon == kPPMC) {
-        hipLaunchKernelGGL((ReconGridKernel<kPPMC, true>), dim3(nb64), dim3(64), 0, s, Uh, QLx, QRx, nx, ny, 1, gamma, 0);
-        hipLaunchKernelGGL((ReconGridKernel<kPPMC, true>), dim3(nb64), dim3(64), 0, s, Uh, QLy, QRy, nx, ny, 1, gamma, 1);
+        both(std::integral_constant<int, kPPMC>{});
+      } else if (recon == kPLMC) {
+        both(std::integral_constant<int, kPLMC>{});
+      } else if (recon == kPLMP) {
+        both(std::integral_constant<int, kPLMP>{});
       } else {
-        hipLaunchKernelGGL((ReconGridKernel<kPLMC, true>), dim3(nb64), dim3(64), 0, s, Uh, QLx, QRx, nx, ny, 1, gamma, 0);
-        hipLaunchKernelGGL((ReconGridKernel<kPLMC, true>), dim3(nb64), dim3(64), 0, s, Uh, QLy, QRy, nx, ny, 1, gamma, 1);
+        both(std::integral_constant<int, kPPMP>{});
       }
     } else {
       hipError_t e = LaunchReconGrid(recon, Uh, QLx, QRx, nx, ny, 1, gamma, 0, s);
@@ -1281,7 +1309,6 @@ hipError_t LaunchVl2d(double *U, double *scratch, int nx, int ny, int ng, int re
                       const GravSpec &grav, hipStream_t s)
 {
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPPMP && recon != kPCM) return hipErrorInvalidValue;
-  if (de && (recon == kPLMP || recon == kPPMP)) return hipErrorInvalidValue;  // every solver with PCM, PLMC or PPMC
   switch (form) {
     case PolicyExact::kId:
       return de ? LaunchVl2dS<PolicyExact, true>(riemann, U, scratch, nx, ny, ng, recon, dx, dy, dt_dev, dt, gamma, dti, grav, s)

@@ -594,6 +594,10 @@ __device__ __forceinline__ void ReconDirSplit(const WCell<float, DE> *s, float g
       ReconPpmcScalar<P>(ge, *geL, *geR);
     } else if constexpr (RK == kPLMC) {
       ReconPlmcScalar<P>(ge, *geL, *geR);
+    } else if constexpr (RK == kPLMP) {
+      ReconPlmpScalar<P>(ge, *geL, *geR);
+    } else if constexpr (RK == kPPMP) {
+      ReconPpmpScalar<P>(ge, *geL, *geR);
     } else if constexpr (RK == kPCM) {
       *geL = *geR = ge[0];
     }
@@ -628,9 +632,16 @@ __device__ __forceinline__ void ReconDirPlain(const WCell<typename P::T, DE> *s,
       st[m].p = s[m].p_or_E;  // exact form: (d, momenta, E) carried through as they are
       if constexpr (DE) ge[m] = s[m].g;  // exact form: the conserved GasEnergy, else the specific one
     } else if constexpr (P::kExact && (RK == kPLMP || RK == kPPMP)) {
-      // plmp_cuda.cu:113-120 (ppmp_cuda.cu:160-171): the squared velocities summed left to right
+      // plmp_cuda.cu:113-120 (ppmp_cuda.cu:160-171): the squared velocities summed left to right;
+      // under DE the pressure from Get_Pressure_From_DE with that kinetic energy
       const T v2 = (st[m].vx * st[m].vx + st[m].vy * st[m].vy) + st[m].vz * st[m].vz;
-      st[m].p    = Fmax((s[m].p_or_E - T(0.5) * st[m].d * v2) * (gamma - T(1.0)), T(kTinyNumber));
+      if constexpr (DE) {
+        const T E_kin = T(0.5) * st[m].d * v2;
+        st[m].p       = Fmax(PressureFromDE(s[m].p_or_E, s[m].p_or_E - E_kin, s[m].g, gamma), T(kTinyNumber));
+        ge[m]         = s[m].g / st[m].d;
+      } else {
+        st[m].p = Fmax((s[m].p_or_E - T(0.5) * st[m].d * v2) * (gamma - T(1.0)), T(kTinyNumber));
+      }
     } else if constexpr (P::kExact) {
       if constexpr (DE) {
         T E_kin  = T(0.5) * st[m].d * SquareMag(st[m].vx, st[m].vy, st[m].vz);
@@ -649,8 +660,10 @@ __device__ __forceinline__ void ReconDirPlain(const WCell<typename P::T, DE> *s,
     if constexpr (DE) *geL = *geR = ge[0];
   } else if constexpr (RK == kPLMP) {
     ReconPlmp<P>(st, L, R);
+    if constexpr (DE) ReconPlmpScalar<P>(ge, *geL, *geR);
   } else if constexpr (RK == kPPMP) {
     ReconPpmp<P>(st, L, R);
+    if constexpr (DE) ReconPpmpScalar<P>(ge, *geL, *geR);
   } else if constexpr (RK == kPPMC) {
     bool fb = ReconPpmc<P>(st, gamma, L, R, pos_fallback);
     if constexpr (DE) {
@@ -692,7 +705,7 @@ __device__ __forceinline__ RState<typename P::T> IfaceState(const Prim<typename 
     return RStateFromCons<P, DE>(q.d, q.vx, q.vy, q.vz, q.p, gamma, DE ? ge : T(0));
   } else if constexpr (P::kExact && (RK == kPLMP || RK == kPPMP)) {  // ppmp_cuda.cu:645-650 likewise
     const T E = q.p / (gamma - T(1.0)) + T(0.5) * q.d * ((q.vx * q.vx + q.vy * q.vy) + q.vz * q.vz);
-    return RStateFromCons<P>(q.d, q.d * q.vx, q.d * q.vy, q.d * q.vz, E, gamma);
+    return RStateFromCons<P, DE>(q.d, q.d * q.vx, q.d * q.vy, q.d * q.vz, E, gamma, DE ? q.d * ge : T(0));
   } else {
     return RStateFromPrim<P, DE>(q, gamma, inv_gm1, ge);
   }

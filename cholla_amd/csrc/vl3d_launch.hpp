@@ -109,17 +109,13 @@ hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
     }
     return tile == 0 ? LaunchStage2T<GravPolicy<P>, RK, 32, 8>(a, s) : LaunchStage2T<GravPolicy<P>, RK, 16, 16>(a, s);
   }
-  // PCM, PLMP, PPMP, HLL, Roe and the exact solver: the first two tile shapes; dual energy
-  // with PCM and the characteristic reconstructions
+  // PCM, PLMP, PPMP, HLL, Roe and the exact solver: the first two tile shapes, with or
+  // without dual energy
   auto narrow = [&]() {
     if (tile > 1) return hipErrorInvalidValue;
     return WithSolver<P>(a.riemann, [&](auto q) {
       using Q = decltype(q);
-      if constexpr (RK == kPLMC || RK == kPPMC || RK == kPCM) {
-        if (a.de) return tile == 0 ? LaunchStage2T<Q, RK, 32, 8, true>(a, s) : LaunchStage2T<Q, RK, 16, 16, true>(a, s);
-      } else {
-        if (a.de) return hipErrorInvalidValue;
-      }
+      if (a.de) return tile == 0 ? LaunchStage2T<Q, RK, 32, 8, true>(a, s) : LaunchStage2T<Q, RK, 16, 16, true>(a, s);
       return tile == 0 ? LaunchStage2T<Q, RK, 32, 8>(a, s) : LaunchStage2T<Q, RK, 16, 16>(a, s);
     });
   };

@@ -484,44 +484,67 @@ static void PlmpInterface(double qm, double q0, double qp, double *left, double 
   *right          = q0 + 0.5 * dm;
 }
 
+// The primitive stencil value PLMP_cuda and PPMP_cuda load (plmp_cuda.cu:108-132,
+// ppmp_cuda.cu:155-174): velocities by division, the pressure from the left-to-right sum of
+// squared velocities, or under DE Get_Pressure_From_DE with that kinetic energy
+// (plmp_cuda.cu:116-121), floored; q[5] the specific GasEnergy dge / d under DE.
+static void LoadPlmpPrim(const double *U, size_t id, size_t n_cells, int o1, int o2, int o3, double gamma, double q[6])
+{
+  double d = U[id], vx = U[o1 * n_cells + id] / d, vy = U[o2 * n_cells + id] / d, vz = U[o3 * n_cells + id] / d;
+  double p;
+  if (g_de) {
+    double E = U[4 * n_cells + id], E_kin = 0.5 * d * (vx * vx + vy * vy + vz * vz);
+    double dge = U[kGE * n_cells + id];
+    p          = PressureFromDE(E, E - E_kin, dge, gamma);
+    q[5]       = dge / d;
+  } else {
+    p = (U[4 * n_cells + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
+  }
+  q[0] = d, q[1] = vx, q[2] = vy, q[3] = vz, q[4] = std::fmax(p, kTiny);
+}
+
+// The interface value PLMP_cuda and PPMP_cuda write (plmp_cuda.cu:217-230, ppmp_cuda.cu:640-683):
+// conserved variables from the floored primitives, the GasEnergy as d * ge under DE.
+static void WritePlmpIface(const double q[6], double *Q, size_t id, size_t n_cells, int o1, int o2, int o3,
+                           double gamma)
+{
+  Q[id]                = q[0];
+  Q[o1 * n_cells + id] = q[0] * q[1];
+  Q[o2 * n_cells + id] = q[0] * q[2];
+  Q[o3 * n_cells + id] = q[0] * q[3];
+  Q[4 * n_cells + id]  = q[4] / (gamma - 1.0) + 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  if (g_de) Q[kGE * n_cells + id] = q[0] * q[5];
+}
+
 // PLMP_cuda in the VL build (no half-step evolution, plmp_cuda.cu:231-284 compiled out),
-// src/reconstruction/plmp_cuda.cu:20-320, hydro without DE or scalars: primitives by
-// division, pressure from the left-to-right sum of squared velocities, limited in the
-// primitive variables, then back to conserved variables.
+// src/reconstruction/plmp_cuda.cu:20-320, hydro without scalars: primitives as LoadPlmpPrim
+// forms them, each limited on its own (the specific GasEnergy too under DE,
+// plmp_cuda.cu:204-206), the density and pressure floors, then conserved variables.
 void orc_plmp(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
 {
   size_t n_cells = size_t(nx) * ny * nz;
   int o1, o2, o3;
   DirOrder(dir, o1, o2, o3);
   int ix = dir == 0, iy = dir == 1, iz = dir == 2;
+  const int nv = g_de ? 6 : 5;
   // plmp_cuda.cu:80-103: one cell off each end along dir, the full range across it
   int xs = ix, xe = ix ? nx - 2 : nx, ys = iy, ye = iy ? ny - 2 : ny, zs = iz, ze = iz ? nz - 2 : nz;
-  auto load = [&](int i, int j, int k, double q[5]) {
-    size_t id = Idx(i, j, k, nx, ny);
-    double d = U[id], vx = U[o1 * n_cells + id] / d, vy = U[o2 * n_cells + id] / d, vz = U[o3 * n_cells + id] / d;
-    double p = (U[4 * n_cells + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
-    q[0] = d, q[1] = vx, q[2] = vy, q[3] = vz, q[4] = std::fmax(p, kTiny);
+  auto load = [&](int i, int j, int k, double q[6]) {
+    LoadPlmpPrim(U, Idx(i, j, k, nx, ny), n_cells, o1, o2, o3, gamma, q);
   };
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = zs; k < ze; k++)
     for (int j = ys; j < ye; j++)
       for (int i = xs; i < xe; i++) {
-        double qm[5], q0[5], qp[5], lf[5], rf[5];
+        double qm[6], q0[6], qp[6], lf[6], rf[6];
         load(i - ix, j - iy, k - iz, qm);
         load(i, j, k, q0);
         load(i + ix, j + iy, k + iz, qp);
-        for (int v = 0; v < 5; v++) PlmpInterface(qm[v], q0[v], qp[v], &lf[v], &rf[v]);
+        for (int v = 0; v < nv; v++) PlmpInterface(qm[v], q0[v], qp[v], &lf[v], &rf[v]);
         lf[0] = std::fmax(lf[0], kTiny), rf[0] = std::fmax(rf[0], kTiny);
         lf[4] = std::fmax(lf[4], kTiny), rf[4] = std::fmax(rf[4], kTiny);
-        auto write = [&](const double q[5], double *Q, size_t id) {
-          Q[id]                = q[0];
-          Q[o1 * n_cells + id] = q[0] * q[1];
-          Q[o2 * n_cells + id] = q[0] * q[2];
-          Q[o3 * n_cells + id] = q[0] * q[3];
-          Q[4 * n_cells + id]  = q[4] / (gamma - 1.0) + 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-        };
-        write(lf, QR, Idx(i - ix, j - iy, k - iz, nx, ny));  // right state of face i-1/2
-        write(rf, QL, Idx(i, j, k, nx, ny));                 // left state of face i+1/2
+        WritePlmpIface(lf, QR, Idx(i - ix, j - iy, k - iz, nx, ny), n_cells, o1, o2, o3, gamma);  // face i-1/2
+        WritePlmpIface(rf, QL, Idx(i, j, k, nx, ny), n_cells, o1, o2, o3, gamma);                 // face i+1/2
       }
 }
 
@@ -543,46 +566,37 @@ static void PpmpInterfaces(double qm, double q0, double qp, double sm, double s0
 }
 
 // PPMP_cuda in the VL build (ppmp_cuda.cu:28-683; STEEPENING and FLATTENING are not defined
-// there and the CTU tracing at :474-640 is compiled out), hydro without DE or scalars:
-// primitives as PLMP forms them, Calculate_Slope (:687-720, the PLMP slope) on cells i-1, i,
-// i+1, parabolic interfaces, floors, conserved variables.
+// there and the CTU tracing at :474-640 is compiled out), hydro without scalars: primitives
+// as LoadPlmpPrim forms them, Calculate_Slope (:687-720, the PLMP slope) on cells i-1, i,
+// i+1, parabolic interfaces (the specific GasEnergy too under DE, :353-361), floors,
+// conserved variables.
 void orc_ppmp(const double *U, double *QL, double *QR, int nx, int ny, int nz, double gamma, int dir)
 {
   size_t n_cells = size_t(nx) * ny * nz;
   int o1, o2, o3;
   DirOrder(dir, o1, o2, o3);
   int ix = dir == 0, iy = dir == 1, iz = dir == 2;
+  const int nv = g_de ? 6 : 5;
   // ppmp_cuda.cu:125-149: two cells off the low end and three off the high end along dir
   int xs = 2 * ix, xe = ix ? nx - 3 : nx, ys = 2 * iy, ye = iy ? ny - 3 : ny, zs = 2 * iz, ze = iz ? nz - 3 : nz;
-  auto load = [&](int i, int j, int k, double q[5]) {
-    size_t id = Idx(i, j, k, nx, ny);
-    double d = U[id], vx = U[o1 * n_cells + id] / d, vy = U[o2 * n_cells + id] / d, vz = U[o3 * n_cells + id] / d;
-    double p = (U[4 * n_cells + id] - 0.5 * d * (vx * vx + vy * vy + vz * vz)) * (gamma - 1.0);
-    q[0] = d, q[1] = vx, q[2] = vy, q[3] = vz, q[4] = std::fmax(p, kTiny);
-  };
   auto slope = LimitedSlope;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = zs; k < ze; k++)
     for (int j = ys; j < ye; j++)
       for (int i = xs; i < xe; i++) {
-        double c[5][5], lf[5], rf[5];
-        for (int m = 0; m < 5; m++) load(i + (m - 2) * ix, j + (m - 2) * iy, k + (m - 2) * iz, c[m]);
-        for (int v = 0; v < 5; v++) {
+        double c[5][6], lf[6], rf[6];
+        for (int m = 0; m < 5; m++)
+          LoadPlmpPrim(U, Idx(i + (m - 2) * ix, j + (m - 2) * iy, k + (m - 2) * iz, nx, ny), n_cells, o1, o2, o3,
+                       gamma, c[m]);
+        for (int v = 0; v < nv; v++) {
           double sm = slope(c[0][v], c[1][v], c[2][v]), s0 = slope(c[1][v], c[2][v], c[3][v]);
           double sp = slope(c[2][v], c[3][v], c[4][v]);
           PpmpInterfaces(c[1][v], c[2][v], c[3][v], sm, s0, sp, &lf[v], &rf[v]);
         }
         lf[0] = std::fmax(lf[0], kTiny), rf[0] = std::fmax(rf[0], kTiny);
         lf[4] = std::fmax(lf[4], kTiny), rf[4] = std::fmax(rf[4], kTiny);
-        auto write = [&](const double q[5], double *Q, size_t id) {
-          Q[id]                = q[0];
-          Q[o1 * n_cells + id] = q[0] * q[1];
-          Q[o2 * n_cells + id] = q[0] * q[2];
-          Q[o3 * n_cells + id] = q[0] * q[3];
-          Q[4 * n_cells + id]  = q[4] / (gamma - 1.0) + 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-        };
-        write(lf, QR, Idx(i - ix, j - iy, k - iz, nx, ny));
-        write(rf, QL, Idx(i, j, k, nx, ny));
+        WritePlmpIface(lf, QR, Idx(i - ix, j - iy, k - iz, nx, ny), n_cells, o1, o2, o3, gamma);
+        WritePlmpIface(rf, QL, Idx(i, j, k, nx, ny), n_cells, o1, o2, o3, gamma);
       }
 }
 

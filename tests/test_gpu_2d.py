@@ -73,8 +73,6 @@ def test_kh_2d_transmissive_reflective_bitwise():
 def test_2d_limits():
     with pytest.raises(chl.ChollaError, match="2D boxes need ny"):
         chl.Hydro(40, 8, 1, 4, chl.RECON_PPMC, chl.PREC_MIXED)
-    with pytest.raises(chl.ChollaError, match="dual energy"):
-        chl.Hydro(40, 40, 1, 3, chl.RECON_PLMP, chl.PREC_FP64_EXACT, dual_energy=True)
 
 
 @pytest.mark.parametrize("solver", [orc.HLL, orc.ROE, orc.EXACT])
@@ -98,7 +96,7 @@ def test_dual_energy_2d_solvers_bitwise(solver):
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC, orc.PCM])
+@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC, orc.PCM, orc.PLMP, orc.PPMP])
 @pytest.mark.parametrize("case", ["warm", "cold_x", "cold_y"])
 def test_dual_energy_2d_bitwise(case, recon, use_graph):
     """2D DE (VL_2D_cuda.cu:140-152, 203-252): a warm flow and cold compressions along each axis."""

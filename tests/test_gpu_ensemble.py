@@ -128,15 +128,15 @@ def test_ensemble_option_limits():
                      tile1=0, tile2=0)
     with pytest.raises(chl.ChollaError, match="tile shapes 0 and 1"):
         chl.Ensemble(dims, 4, chl.RECON_PPMP, chl.PREC_FP64, chl.LAUNCH_SPLIT, tile1=2, tile2=1)
-    with pytest.raises(chl.ChollaError, match="dual energy"):
-        chl.Ensemble(dims, 3, chl.RECON_PLMP, chl.PREC_FP64, chl.LAUNCH_SPLIT, dual_energy=True)
 
 
-def test_pcm_de_ensemble_matches_oracle_per_box():
-    """PCM with dual energy inside the ensemble (split launch), each box bit-identical to the
-    oracle's DE loop."""
+@pytest.mark.parametrize("recon,orc_recon,ng", [(chl.RECON_PCM, orc.PCM, 2), (chl.RECON_PLMP, orc.PLMP, 3),
+                                               (chl.RECON_PPMP, orc.PPMP, 4)])
+def test_pcm_plmp_ppmp_de_ensemble_matches_oracle_per_box(recon, orc_recon, ng):
+    """PCM, PLMP and PPMP with dual energy inside the ensemble (split launch), each box
+    bit-identical to the oracle's DE loop."""
     from test_oracle_de import periodic_state
-    ng, steps = 2, 5
+    steps = 5
     states, dims, dxyz = [], [], []
     for b, r in enumerate(REAL[:3]):
         rng = np.random.default_rng(500 + b)
@@ -146,12 +146,13 @@ def test_pcm_de_ensemble_matches_oracle_per_box():
         p = 1 + 0.1 * rng.uniform(-1, 1, shape)
         U, dm = periodic_state(r[0], r[1], r[2], ng, d, v[0], v[1], v[2], p)
         states.append(U), dims.append(dm), dxyz.append((1.0 / r[0], 1.0 / r[1], 1.0 / r[2]))
-    out, t, ns = run_ensemble(states, dims, dxyz, ng, steps, chl.PREC_FP64_EXACT, chl.LAUNCH_SPLIT, chl.RECON_PCM,
+    say("DE ensemble recon", recon)
+    out, t, ns = run_ensemble(states, dims, dxyz, ng, steps, chl.PREC_FP64_EXACT, chl.LAUNCH_SPLIT, recon,
                               dual_energy=True)
     for b, U in enumerate(states):
         Uo = U.copy()
         nx, ny, nz = dims[b]
-        n_o, t_o = orc.run(Uo, nx, ny, nz, ng, 6, *dxyz[b], 5 / 3, orc.PCM, [1] * 6, 0.0, 1e30, steps,
+        n_o, t_o = orc.run(Uo, nx, ny, nz, ng, 6, *dxyz[b], 5 / 3, orc_recon, [1] * 6, 0.0, 1e30, steps,
                            dual_energy_on=True)
         assert ns[b] == n_o == steps and t[b] == t_o
         np.testing.assert_array_equal(out[b], Uo)

@@ -113,12 +113,12 @@ def test_sod_forms_solvers(solver, precision):
 
 
 @pytest.mark.parametrize("precision", [chl.PREC_FP64, chl.PREC_MIXED])
-@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC])
+@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC, orc.PLMP, orc.PPMP])
 def test_sod_forms_dual_energy(recon, precision):
     say("Sod 1D DE form", precision, "recon", recon)
     l1, _, _, _ = sod_run(recon, orc.HLLC, precision, de=True)
     say("  L1", ["%.2e" % v for v in l1])
-    check(l1, precision, SOD_MIXED_MISS if precision == chl.PREC_MIXED else None)
+    check(l1, precision, SOD_MIXED_MISS if precision == chl.PREC_MIXED else SOD_FP64_MISS.get(recon))
 
 
 def kh_run(recon, solver, precision, de=False, steps=10, n=32):
@@ -167,7 +167,7 @@ def test_kh_2d_forms_solvers(solver, precision):
 
 
 @pytest.mark.parametrize("precision", [chl.PREC_FP64, chl.PREC_MIXED])
-@pytest.mark.parametrize("recon", [orc.PCM, orc.PLMC, orc.PPMC])
+@pytest.mark.parametrize("recon", [orc.PCM, orc.PLMC, orc.PPMC, orc.PLMP, orc.PPMP])
 def test_kh_2d_forms_dual_energy(recon, precision):
     say("2D KH DE form", precision, "recon", recon)
     l1 = kh_run(recon, orc.HLLC, precision, de=True)

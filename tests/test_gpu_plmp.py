@@ -117,8 +117,6 @@ def test_sod_1d(recon):
 
 
 def test_option_limits():
-    with pytest.raises(chl.ChollaError, match="dual energy"):
-        chl.Hydro(24, 24, 24, 3, chl.RECON_PLMP, chl.PREC_FP64, dual_energy=True)
     with pytest.raises(chl.ChollaError, match="tile shapes 0 and 1"):
         chl.Hydro(24, 24, 24, 3, chl.RECON_PLMP, chl.PREC_FP64, tile2=3)
     with pytest.raises(chl.ChollaError, match="n_ghost"):

@@ -148,11 +148,11 @@ def test_sod_1d(name):
 @pytest.mark.parametrize("name", list(SOLVERS))
 def test_option_limits(name):
     solver = SOLVERS[name][0]
-    # dual energy is built for every solver with the characteristic reconstructions (tile
-    # shapes 0 and 1), not with PCM, PLMP or PPMP
+    # dual energy is built for every solver and reconstruction on tile shapes 0 and 1
     chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, riemann=solver, dual_energy=True, tile1=1,
               tile2=1).close()
-    with pytest.raises(chl.ChollaError, match="dual energy"):
-        chl.Hydro(24, 24, 24, 4, chl.RECON_PLMP, chl.PREC_FP64, riemann=solver, dual_energy=True)
+    chl.Hydro(24, 24, 24, 4, chl.RECON_PLMP, chl.PREC_FP64, riemann=solver, dual_energy=True).close()
+    with pytest.raises(chl.ChollaError, match="tile shapes 0 and 1"):
+        chl.Hydro(24, 24, 24, 4, chl.RECON_PLMP, chl.PREC_FP64, riemann=solver, dual_energy=True, tile2=3)
     with pytest.raises(chl.ChollaError, match="tile shapes 0 and 1"):
         chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, riemann=solver, tile2=3)
