@@ -669,8 +669,14 @@ __device__ __forceinline__ void ReconPlmcCore(const Prim<typename P::T> s[3], co
              T(0.5) * (ip.p - im.p)};
   Prim<T> dG{VlSlope(dL.d, dR.d, T(0)), VlSlope(dL.vx, dR.vx, T(0)), VlSlope(dL.vy, dR.vy, T(0)),
              VlSlope(dL.vz, dR.vz, T(0)), VlSlope(dL.p, dR.p, T(0))};
-  Char5<T> aL = PrimToChar<P>(qe, dL, cs), aR = PrimToChar<P>(qe, dR, cs), aC = PrimToChar<P>(qe, dC, cs),
-           aG = PrimToChar<P>(qe, dG, cs);
+  Char5<T> aL = PrimToChar<P>(qe, dL, cs), aR = PrimToChar<P>(qe, dR, cs), aG = PrimToChar<P>(qe, dG, cs);
+  Char5<T> aC;
+  if constexpr (P::kExact) {  // the reference projects the centred slope itself
+    aC = PrimToChar<P>(qe, dC, cs);
+  } else {  // dC = (dL + dR) / 2 and the projection is linear: one projection fewer
+    aC = Char5<T>{T(0.5) * (aL.a0 + aR.a0), T(0.5) * (aL.a1 + aR.a1), T(0.5) * (aL.a2 + aR.a2),
+                  T(0.5) * (aL.a3 + aR.a3), T(0.5) * (aL.a4 + aR.a4)};
+  }
   Char5<T> m;
   m.a0       = MonotonizeOne(aL.a0, aR.a0, aC.a0, aG.a0);
   m.a1       = MonotonizeOne(aL.a1, aR.a1, aC.a1, aG.a1);

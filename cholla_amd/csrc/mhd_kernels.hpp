@@ -295,7 +295,14 @@ __device__ __forceinline__ void PlmcCellM(const double *U, int x, int y, int z, 
                     VlM(dL.p, dR.p), T(0),              VlM(dL.by, dR.by), VlM(dL.bz, dR.bz)};
   const Char7<T> aL = PrimToCharM<P>(c0, dL, e, cs2, inv_d, inv_cs2);
   const Char7<T> aR = PrimToCharM<P>(c0, dR, e, cs2, inv_d, inv_cs2);
-  const Char7<T> aC = PrimToCharM<P>(c0, dC, e, cs2, inv_d, inv_cs2);
+  Char7<T> aC;
+  if constexpr (P::kExact) {  // the reference projects the centred slope itself
+    aC = PrimToCharM<P>(c0, dC, e, cs2, inv_d, inv_cs2);
+  } else {  // dC = (dL + dR) / 2 and the projection is linear: one projection fewer
+    aC = Char7<T>{T(0.5) * (aL.a0 + aR.a0), T(0.5) * (aL.a1 + aR.a1), T(0.5) * (aL.a2 + aR.a2),
+                  T(0.5) * (aL.a3 + aR.a3), T(0.5) * (aL.a4 + aR.a4), T(0.5) * (aL.a5 + aR.a5),
+                  T(0.5) * (aL.a6 + aR.a6)};
+  }
   const Char7<T> aG = PrimToCharM<P>(c0, dG, e, cs2, inv_d, inv_cs2);
   Char7<T> am;
   am.a0 = MonoM(aL.a0, aR.a0, aC.a0, aG.a0);
@@ -1032,30 +1039,6 @@ __device__ __forceinline__ void HalfCell(const MhdArgs &a, double dt, int x, int
   if (y >= 2 && z >= 2) Uh[5 * n + id] = c.u[5] + dtodz * (e.ey_xz - e.ey_x) + dtody * (e.ez_x - e.ez_xy);
   if (x >= 2 && z >= 2) Uh[6 * n + id] = c.u[6] + dtodx * (e.ez_xy - e.ez_y) + dtodz * (e.ex_y - e.ex_yz);
   if (x >= 2 && y >= 2) Uh[7 * n + id] = c.u[7] + dtody * (e.ex_yz - e.ex_z) + dtodx * (e.ey_z - e.ey_xz);
-}
-
-// Stage-2 fluxes along DIR on faces with every index in [lo, hi]: PLMC of the two cells of
-// the face (PLMC_cuda writes them to Q_L at i and Q_R at i, plmc_cuda.cu:292-298) + HLLD.
-template <class P, int DIR>
-__global__ void __launch_bounds__(256) MhdFlux2Kernel(MhdArgs a, int lo, int hix, int hiy, int hiz)
-{
-  using T         = typename P::T;
-  const int nx    = a.nx, ny = a.ny, nz = a.nz;
-  const size_t n  = size_t(nx) * ny * nz;
-  const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (id >= n) return;
-  if (MhdDt(a) == 0.0) return;
-  const int x = int(id % nx), y = int((id / nx) % ny), z = int(id / (size_t(nx) * ny));
-  if (x < lo || y < lo || z < lo || x > hix || y > hiy || z > hiz) return;
-  const double *Uh = a.Uh;
-  const T g        = T(a.gamma);
-  PrimM<T> Lp, Rm, tmp;
-  PlmcCellM<P>(Uh, x, y, z, nx, ny, n, DIR, a.gamma, Lp, tmp);
-  PlmcCellM<P>(Uh, x + (DIR == 0), y + (DIR == 1), z + (DIR == 2), nx, ny, n, DIR, a.gamma, tmp, Rm);
-  const T bx        = T(Uh[(kBx + DIR) * n + id]);
-  const ConsM<T> qL = WriteDataM<P>(Lp, bx, g), qR = WriteDataM<P>(Rm, bx, g);
-  ConsM<T> f;
-  if (HlldM<P>(qL, qR, bx, g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[DIR]), id, n, DIR, f);
 }
 
 // Stage-2 fluxes along DIR as above, with each cell's reconstruction (RK: 1 PLMC, 2 PPMC)
