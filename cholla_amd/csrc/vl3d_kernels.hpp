@@ -380,6 +380,33 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
   T fz_lo[NC];
   PcmFaceFlux<P, 2, DE>(c_lo, c_mid, gamma, fz_lo);
 
+  // The plane's halo, x sides (TY rows x 2) and y sides (TX cols x 2), no corners: one cell
+  // per lane of the first 2 (TX + TY).  Its conserved values for the next plane are loaded
+  // while the current plane's faces are solved, and converted once the next plane starts
+  // (mixed and exact forms: 3.94 -> 3.70 and 10.71 -> 10.33 ms on KH 512^3).  The fast FP64
+  // form loads them after the first barrier instead: the 10 registers the prefetch holds
+  // push it into spills.
+  static_assert(2 * TY + 2 * TX <= NT, "one halo cell per lane");
+  constexpr bool kPrefetch = P::kId != PolicyFast::kId;
+  const bool has_h = tid < 2 * TY + 2 * TX;
+  int hpi = 0, hpj = 0;
+  if (tid < 2 * TY) {
+    hpj = 1 + (tid >> 1);
+    hpi = (tid & 1) ? TX + 1 : 0;
+  } else {
+    const int g = tid - 2 * TY;
+    hpi         = 1 + (g >> 1);
+    hpj         = (g & 1) ? TY + 1 : 0;
+  }
+  const int hgi = min(i0 - 1 + hpi, nx - 1), hgj = min(j0 - 1 + hpj, ny - 1);
+  double h_raw[NC];
+  auto load_halo = [&](int kk) {
+    const size_t id = Lin(hgi, hgj, kk, nx, ny);
+#pragma unroll
+    for (int f = 0; f < NC; f++) h_raw[f] = U[size_t(f) * nc + id];  // GasEnergy (DE) is field 5
+  };
+  if (kPrefetch && has_h) load_halo(k0);
+
   for (int k = k0; k < k1; k++) {
     {
       size_t id = Lin(ic, jc, k, nx, ny);
@@ -389,19 +416,11 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
     PcmCell<T, DE> c_hi = load_cell(ic, jc, k + 1);
     __syncthreads();  // previous plane fully consumed
     plane[ty + 1][tx + 1] = c_mid;
-    // halo: x sides (PY-2 rows x 2) and y sides (PX-2 cols x 2), no corners
-    for (int h = tid; h < 2 * TY + 2 * TX; h += NT) {
-      int pi, pj;
-      if (h < 2 * TY) {
-        pj = 1 + (h >> 1);
-        pi = (h & 1) ? TX + 1 : 0;
-      } else {
-        int g = h - 2 * TY;
-        pi    = 1 + (g >> 1);
-        pj    = (g & 1) ? TY + 1 : 0;
-      }
-      int gi = min(i0 - 1 + pi, nx - 1), gj = min(j0 - 1 + pj, ny - 1);
-      plane[pj][pi] = load_cell(gi, gj, k);
+    if (has_h) {
+      if constexpr (!kPrefetch) load_halo(k);
+      plane[hpj][hpi] = MakePcmCell<P, DE>(h_raw[0], h_raw[1], h_raw[2], h_raw[3], h_raw[4],
+                                           DE ? h_raw[NC - 1] : 0.0, a.gamma);
+      if (kPrefetch && k + 1 < k1) load_halo(k + 1);
     }
     __syncthreads();
     // x faces: face fi sits between plane cells (fi, fj+1) and (fi+1, fj+1)
@@ -522,8 +541,14 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
   }
 }
 
+// Waves per SIMD the stage-1 kernel is compiled for: the mixed form fits 4 (127 VGPRs); the
+// exact form without DE 3 (36 bytes of spills cost less than the third wave would give back);
+// the others take the compiler's own allocation.
+template <class P, bool DE>
+constexpr int kStage1MinWaves = P::kId == PolicyMixed::kId ? 4 : (P::kExact && !DE ? 3 : 1);
+
 template <class P, int TX, int TY, bool DE = false>
-__global__ void __launch_bounds__(TX *TY) Stage1Kernel(StepArgs a, int R1)
+__global__ void __launch_bounds__(TX *TY, (kStage1MinWaves<P, DE>)) Stage1Kernel(StepArgs a, int R1)
 {
   __shared__ Stage1Smem<P, TX, TY, DE> sm;
   // the inverse-dt accumulator is reset here; stage 2 (next launch) fills it
