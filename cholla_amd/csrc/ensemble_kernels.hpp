@@ -124,7 +124,7 @@ __device__ __forceinline__ void TileCoords(int l, const int t[3], int &bx, int &
 __global__ void EnsembleClockKernel(BoxClock *clk, const BoxDesc *box, int n_boxes, double cfl, unsigned int *head);
 
 template <class P, int TX, int TY, bool DE = false>
-__global__ void __launch_bounds__(TX *TY) EnsembleStage1Kernel(EnsembleArgs e)
+__global__ void __launch_bounds__(TX *TY, (kStage1MinWaves<P, DE>)) EnsembleStage1Kernel(EnsembleArgs e)
 {
   __shared__ Stage1Smem<P, TX, TY, DE> sm;
   const int q = XcdItem(blockIdx.x, e.n_items);

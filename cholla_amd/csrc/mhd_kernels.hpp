@@ -12,7 +12,10 @@
 //   MhdCtUpdateKernel  CT edge fields (S&G 2009 eq. 22-24) and the dt/2 update of the
 //                      hydro and face fields in one launch: x-y tiles marching in z, the
 //                      flux values, cell terms and edge fields in LDS (never in HBM)
-//   MhdFlux2TileKernel PLMC or PPMC of each cell once per direction + HLLD (tiles)
+//   MhdFlux2TileKernel x faces: PLMC or PPMC of each cell once + HLLD, 64 x 4 face tiles
+//                      with the stencil's primitives staged in LDS
+//   MhdFlux2MarchKernel y and z faces: the same per face, each lane marching along the
+//                      direction with the stencil's primitives in a register window
 //   MhdCtUpdateKernel  CT on the half-step fluxes and state, fused with the dt update of
 //                      the real cells' hydro fields (density and temperature floors fused)
 //                      and of their face fields
@@ -269,17 +272,15 @@ __device__ __forceinline__ void LimitM(T &l, T &r, T vm, T v0, T vp)  // Plm_Lim
   l = Fmin(Fmax(v0, vp), l);
 }
 
-// PLMC_cuda with MHD for one cell (plmc_cuda.cu:24-300, VL build): the interface
-// states at +1/2 (Lp) and -1/2 (Rm) of cell (x,y,z) along dir.
+// PLMC_cuda with MHD for one cell (plmc_cuda.cu:24-300, VL build) from the primitives of the
+// cell (c0) and its neighbours along dir (cm, cp): the interface states at +1/2 (Lp) and
+// -1/2 (Rm).
 template <class P>
-__device__ __forceinline__ void PlmcCellM(const double *U, int x, int y, int z, int nx, int ny, size_t n, int dir,
-                                          double g, PrimM<typename P::T> &Lp, PrimM<typename P::T> &Rm)
+__device__ __forceinline__ void PlmcPrimsM(const PrimM<typename P::T> &cm, const PrimM<typename P::T> &c0,
+                                           const PrimM<typename P::T> &cp, double g, PrimM<typename P::T> &Lp,
+                                           PrimM<typename P::T> &Rm)
 {
-  using T        = typename P::T;
-  const int di   = dir == 0, dj = dir == 1, dk = dir == 2;
-  const PrimM<T> c0 = LoadDataM<P>(U, x, y, z, nx, ny, n, dir, g);
-  const PrimM<T> cm = LoadDataM<P>(U, x - di, y - dj, z - dk, nx, ny, n, dir, g);
-  const PrimM<T> cp = LoadDataM<P>(U, x + di, y + dj, z + dk, nx, ny, n, dir, g);
+  using T           = typename P::T;
   const T gg        = T(g);
   const T inv_d     = Div1<P>(T(1.0), c0.d);
   const T cs        = Sqrt<P>(Div<P>(gg * c0.p, c0.d, inv_d));  // Calc_Sound_Speed
@@ -333,21 +334,28 @@ __device__ __forceinline__ void PlmcCellM(const double *U, int x, int y, int z, 
   Lp.p = Fmax(Lp.p, T(kTinyNumber));
 }
 
-// PPMC_VL with MHD for one cell (ppmc_cuda.cu:540-695): the 5-cell stencil projected on the
-// cell's seven MHD characteristics, PPM_Single_Variable on each (the reduced FP32 form under
-// the mixed policy), projected back, density and pressure floored: the interface states at
-// +1/2 (Lp) and -1/2 (Rm) of cell (x,y,z) along dir.
+// PLMC with MHD for cell (x,y,z) along dir, its stencil loaded (Load_Data) here
 template <class P>
-__device__ __forceinline__ void PpmcCellM(const double *U, int x, int y, int z, int nx, int ny, size_t n, int dir,
+__device__ __forceinline__ void PlmcCellM(const double *U, int x, int y, int z, int nx, int ny, size_t n, int dir,
                                           double g, PrimM<typename P::T> &Lp, PrimM<typename P::T> &Rm)
 {
-  using T      = typename P::T;
   const int di = dir == 0, dj = dir == 1, dk = dir == 2;
-  const PrimM<T> c0  = LoadDataM<P>(U, x, y, z, nx, ny, n, dir, g);
-  const PrimM<T> cm1 = LoadDataM<P>(U, x - di, y - dj, z - dk, nx, ny, n, dir, g);
-  const PrimM<T> cp1 = LoadDataM<P>(U, x + di, y + dj, z + dk, nx, ny, n, dir, g);
-  const PrimM<T> cm2 = LoadDataM<P>(U, x - 2 * di, y - 2 * dj, z - 2 * dk, nx, ny, n, dir, g);
-  const PrimM<T> cp2 = LoadDataM<P>(U, x + 2 * di, y + 2 * dj, z + 2 * dk, nx, ny, n, dir, g);
+  PlmcPrimsM<P>(LoadDataM<P>(U, x - di, y - dj, z - dk, nx, ny, n, dir, g), LoadDataM<P>(U, x, y, z, nx, ny, n, dir, g),
+                LoadDataM<P>(U, x + di, y + dj, z + dk, nx, ny, n, dir, g), g, Lp, Rm);
+}
+
+// PPMC_VL with MHD for one cell (ppmc_cuda.cu:540-695) from the primitives of the cell (c0)
+// and its neighbours along dir (cm2, cm1, cp1, cp2): the 5-cell stencil projected on the
+// cell's seven MHD characteristics, PPM_Single_Variable on each (the reduced FP32 form under
+// the mixed policy), projected back, density and pressure floored: the interface states at
+// +1/2 (Lp) and -1/2 (Rm).
+template <class P>
+__device__ __forceinline__ void PpmcPrimsM(const PrimM<typename P::T> &cm2, const PrimM<typename P::T> &cm1,
+                                           const PrimM<typename P::T> &c0, const PrimM<typename P::T> &cp1,
+                                           const PrimM<typename P::T> &cp2, double g, PrimM<typename P::T> &Lp,
+                                           PrimM<typename P::T> &Rm)
+{
+  using T           = typename P::T;
   const T gg        = T(g);
   const T inv_d     = Div1<P>(T(1.0), c0.d);
   const T cs        = Sqrt<P>(Div<P>(gg * c0.p, c0.d, inv_d));  // Calc_Sound_Speed
@@ -380,6 +388,18 @@ __device__ __forceinline__ void PpmcCellM(const double *U, int x, int y, int z, 
   Lp.d = Fmax(Lp.d, T(kTinyNumber));
   Rm.p = Fmax(Rm.p, T(kTinyNumber));
   Lp.p = Fmax(Lp.p, T(kTinyNumber));
+}
+
+// PPMC with MHD for cell (x,y,z) along dir, its stencil loaded (Load_Data) here
+template <class P>
+__device__ __forceinline__ void PpmcCellM(const double *U, int x, int y, int z, int nx, int ny, size_t n, int dir,
+                                          double g, PrimM<typename P::T> &Lp, PrimM<typename P::T> &Rm)
+{
+  const int di = dir == 0, dj = dir == 1, dk = dir == 2;
+  PpmcPrimsM<P>(LoadDataM<P>(U, x - 2 * di, y - 2 * dj, z - 2 * dk, nx, ny, n, dir, g),
+                LoadDataM<P>(U, x - di, y - dj, z - dk, nx, ny, n, dir, g), LoadDataM<P>(U, x, y, z, nx, ny, n, dir, g),
+                LoadDataM<P>(U, x + di, y + dj, z + dk, nx, ny, n, dir, g),
+                LoadDataM<P>(U, x + 2 * di, y + 2 * dj, z + 2 * dk, nx, ny, n, dir, g), g, Lp, Rm);
 }
 
 // Stage-2 reconstruction of the MHD build (CHL_RECON_*): ghost cells it needs, and the
@@ -1041,73 +1061,136 @@ __device__ __forceinline__ void HalfCell(const MhdArgs &a, double dt, int x, int
   if (x >= 2 && y >= 2) Uh[7 * n + id] = c.u[7] + dtody * (e.ex_yz - e.ex_z) + dtodx * (e.ey_z - e.ey_xz);
 }
 
-// Stage-2 fluxes along DIR as above, with each cell's reconstruction (RK: 1 PLMC, 2 PPMC)
-// computed once: a workgroup
-// takes 64 (x) by 4 (the slow axis: y, or z for DIR 2) faces of one plane of the third
-// axis, reconstructs the cells of those faces plus the next layer along DIR, keeps each
-// cell's face-(i+1/2) state in registers and passes its face-(i-1/2) state to the
-// neighbour through LDS.  The per-face kernel reconstructs every cell twice (once for each
-// of its faces); the arithmetic of each reconstruction and flux is the same.
-template <class P, int DIR, int RK = 1>
-__global__ void __launch_bounds__(256) MhdFlux2TileKernel(MhdArgs a, int lo, int hix, int hiy, int hiz)
+// Stage-2 fluxes along x with each cell's primitives (Load_Data) recovered once and each
+// cell's reconstruction (RK: 1 PLMC, 2 PPMC) computed once: a workgroup takes 64 (x) by 4
+// (y) faces of one z plane, stages the primitives of the 64 + 1 + 2 R cells of each row in
+// LDS (one array per field), reconstructs the cells of its faces plus the next cell along x
+// from that stencil, keeps each cell's face-(i+1/2) state in registers and passes its
+// face-(i-1/2) state to the neighbour through LDS.  The arithmetic of each reconstruction
+// and flux is the per-cell one.  The fast FP64 form is held to 3 waves per SIMD (168 VGPRs,
+// no spills; the exact form would spill).
+template <class P, int RK = 1>
+__global__ void __launch_bounds__(256, (P::kId == PolicyFast::kId ? 3 : 1)) MhdFlux2TileKernel(MhdArgs a, int lo, int hix, int hiy, int hiz)
 {
   using T          = typename P::T;
-  constexpr int TW = 64, TH = 4;                       // tile: x by the slow axis
-  constexpr int CW = TW + (DIR == 0), CH = TH + (DIR != 0);  // cells: one more layer along DIR
-  __shared__ PrimM<T> rm[CH][CW];                      // each cell's face-(i-1/2) state
+  constexpr int TW = 64, TH = 4;           // faces: x by y
+  constexpr int R  = RK == 2 ? 2 : 1;      // stencil half-width
+  constexpr int SW = TW + 1 + 2 * R;       // staged cells per row: x0 - R .. x0 + TW + R
+  __shared__ T pr[8][TH][SW];              // the staged primitives, field-major
+  __shared__ PrimM<T> rm[TH][TW + 1];      // each cell's face-(i-1/2) state
   if (MhdDt(a) == 0.0) return;
   const int nx = a.nx, ny = a.ny;
   const size_t n = size_t(nx) * ny * a.nz;
-  const int his = DIR == 2 ? hiz : hiy, hit = DIR == 2 ? hiy : hiz;  // slow and third axes' bounds
   const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
-  // tiles in XCD-contiguous order (XcdItem): x fastest, then the slow axis, then the third
-  const int gx = (hix - lo + TW) / TW, gs = (his - lo + TH) / TH, gt = hit - lo + 1;
+  // tiles in XCD-contiguous order (XcdItem): x fastest, then y, then z
+  const int gx = (hix - lo + TW) / TW, gs = (hiy - lo + TH) / TH, gt = hiz - lo + 1;
   const int q = XcdItem(int(blockIdx.x), gx * gs * gt);
   if (q >= gx * gs * gt) return;  // uniform per workgroup
-  const int x0 = lo + (q % gx) * TW, s0 = lo + ((q / gx) % gs) * TH, t = lo + q / (gx * gs);
+  const int x0 = lo + (q % gx) * TW, s0 = lo + ((q / gx) % gs) * TH, z = lo + q / (gx * gs);
   const double *Uh = a.Uh;
-  auto cell_ok = [&](int cx, int cs) {  // a cell some face of the region reads
-    return cx <= hix + (DIR == 0) && cs <= his + (DIR != 0);
+  // stage the cells some reconstructed cell reads: x <= hix + 1 + R, y <= hiy
+  for (int i = int(threadIdx.x); i < TH * SW; i += 256) {
+    const int r = i / SW, c = i % SW, x = x0 - R + c, y = s0 + r;
+    if (x <= hix + 1 + R && y <= hiy) {
+      const PrimM<T> w = LoadDataM<P>(Uh, x, y, z, nx, ny, n, 0, a.gamma);
+      pr[0][r][c] = w.d, pr[1][r][c] = w.vx, pr[2][r][c] = w.vy, pr[3][r][c] = w.vz;
+      pr[4][r][c] = w.p, pr[5][r][c] = w.bx, pr[6][r][c] = w.by, pr[7][r][c] = w.bz;
+    }
+  }
+  __syncthreads();
+  auto cell = [&](int r, int c) {
+    return PrimM<T>{pr[0][r][c], pr[1][r][c], pr[2][r][c], pr[3][r][c],
+                    pr[4][r][c], pr[5][r][c], pr[6][r][c], pr[7][r][c]};
   };
-  auto reconstruct = [&](int cx, int cs, PrimM<T> &Lp, PrimM<T> &Rm) {
-    const int y = DIR == 2 ? t : cs, z = DIR == 2 ? cs : t;
+  auto reconstruct = [&](int r, int c, PrimM<T> &Lp, PrimM<T> &Rm) {  // the cell at column c + R
     if constexpr (RK == 2) {
-      PpmcCellM<P>(Uh, cx, y, z, nx, ny, n, DIR, a.gamma, Lp, Rm);
+      PpmcPrimsM<P>(cell(r, c), cell(r, c + 1), cell(r, c + 2), cell(r, c + 3), cell(r, c + 4), a.gamma, Lp, Rm);
     } else {
-      PlmcCellM<P>(Uh, cx, y, z, nx, ny, n, DIR, a.gamma, Lp, Rm);
+      PlmcPrimsM<P>(cell(r, c), cell(r, c + 1), cell(r, c + 2), a.gamma, Lp, Rm);
     }
   };
   PrimM<T> Lp, Rm;
   const int fx = x0 + tx, fs = s0 + ty;
-  const bool mine = cell_ok(fx, fs);
+  const bool mine = fx <= hix + 1 && fs <= hiy;  // a cell some face of the region reads
   if (mine) {
-    reconstruct(fx, fs, Lp, Rm);
+    reconstruct(ty, tx, Lp, Rm);
     rm[ty][tx] = Rm;
   }
-  // the extra layer along DIR
-  if constexpr (DIR == 0) {
-    if (threadIdx.x < TH && cell_ok(x0 + TW, s0 + int(threadIdx.x))) {
-      PrimM<T> l2, r2;
-      reconstruct(x0 + TW, s0 + int(threadIdx.x), l2, r2);
-      rm[threadIdx.x][TW] = r2;
-    }
-  } else {
-    if (threadIdx.x < TW && cell_ok(x0 + int(threadIdx.x), s0 + TH)) {
-      PrimM<T> l2, r2;
-      reconstruct(x0 + int(threadIdx.x), s0 + TH, l2, r2);
-      rm[TH][threadIdx.x] = r2;
-    }
+  // the next cell along x
+  if (threadIdx.x < TH && x0 + TW <= hix + 1 && s0 + int(threadIdx.x) <= hiy) {
+    PrimM<T> l2, r2;
+    reconstruct(int(threadIdx.x), TW, l2, r2);
+    rm[threadIdx.x][TW] = r2;
   }
   __syncthreads();
-  if (!mine || fx > hix || fs > his) return;
-  const int y = DIR == 2 ? t : fs, z = DIR == 2 ? fs : t;
-  const size_t id   = size_t(fx) + size_t(y) * nx + size_t(z) * size_t(nx) * ny;
-  const PrimM<T> Rn = DIR == 0 ? rm[ty][tx + 1] : rm[ty + 1][tx];
+  if (!mine || fx > hix) return;
+  const size_t id   = size_t(fx) + size_t(fs) * nx + size_t(z) * size_t(nx) * ny;
+  const PrimM<T> Rn = rm[ty][tx + 1];
   const T g         = T(a.gamma);
-  const T bx        = T(Uh[(kBx + DIR) * n + id]);
+  const T bx        = T(Uh[kBx * n + id]);
   const ConsM<T> qL = WriteDataM<P>(Lp, bx, g), qR = WriteDataM<P>(Rn, bx, g);
   ConsM<T> f;
-  if (HlldM<P>(qL, qR, bx, g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[DIR]), id, n, DIR, f);
+  if (HlldM<P>(qL, qR, bx, g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[0]), id, n, 0, f);
+}
+
+// Stage-2 fluxes along DIR = y or z by a march along DIR: each lane owns a column (x and the
+// third axis) of a 64 x 4 block and walks a chunk of kMarch faces.  Each cell's primitives
+// (Load_Data) are recovered once and slide through the reconstruction stencil's window, each
+// cell is reconstructed once, and its face-(i+1/2) state carries to the next face: a chunk of
+// K faces loads K + 2 R cells and reconstructs K + 1 (a 64 x 4 face tile across DIR would
+// reconstruct 5 rows for 4, and each reconstruction recovers its 2 R + 1 cells' primitives).
+// Each face's flux is the tile kernel's: HLLD of the two cells' interface states, stored at
+// the lower cell.
+constexpr int kMarch = 16;
+template <class P, int DIR, int RK = 1>
+__global__ void __launch_bounds__(256) MhdFlux2MarchKernel(MhdArgs a, int lo, int hix, int hiy, int hiz)
+{
+  static_assert(DIR == 1 || DIR == 2, "the x faces use MhdFlux2TileKernel");
+  using T = typename P::T;
+  if (MhdDt(a) == 0.0) return;
+  const int nx = a.nx, ny = a.ny;
+  const size_t n = size_t(nx) * ny * a.nz;
+  const int his = DIR == 2 ? hiz : hiy, hit = DIR == 2 ? hiy : hiz;  // march axis and third axis bounds
+  const int tx = threadIdx.x % 64, tt = threadIdx.x / 64;
+  // blocks in XCD-contiguous order: x fastest, then the third axis, then the chunk along DIR
+  const int gx = (hix - lo + 64) / 64, gt = (hit - lo + 4) / 4, gs = (his - lo + kMarch) / kMarch;
+  const int q = XcdItem(int(blockIdx.x), gx * gt * gs);
+  if (q >= gx * gt * gs) return;
+  const int x = lo + (q % gx) * 64 + tx, t = lo + ((q / gx) % gt) * 4 + tt;
+  const int s0 = lo + (q / (gx * gt)) * kMarch, s1 = min(s0 + kMarch - 1, his);
+  if (x > hix || t > hit) return;  // no barrier below
+  const double *Uh = a.Uh;
+  const T g        = T(a.gamma);
+  auto load = [&](int s) {
+    return DIR == 2 ? LoadDataM<P>(Uh, x, t, s, nx, ny, n, DIR, a.gamma)
+                    : LoadDataM<P>(Uh, x, s, t, nx, ny, n, DIR, a.gamma);
+  };
+  // the window holds cells s - R .. s + R of the cell s being reconstructed
+  constexpr int R = RK == 2 ? 2 : 1;
+  PrimM<T> w[2 * R + 1];
+  for (int i = 0; i < 2 * R + 1; i++) w[i] = load(s0 - R + i);
+  auto reconstruct = [&](PrimM<T> &Lp, PrimM<T> &Rm) {
+    if constexpr (RK == 2) {
+      PpmcPrimsM<P>(w[0], w[1], w[2], w[3], w[4], a.gamma, Lp, Rm);
+    } else {
+      PlmcPrimsM<P>(w[0], w[1], w[2], a.gamma, Lp, Rm);
+    }
+  };
+  PrimM<T> Lprev, Rskip;
+  reconstruct(Lprev, Rskip);
+  for (int s = s0; s <= s1; s++) {
+    for (int i = 0; i < 2 * R; i++) w[i] = w[i + 1];
+    w[2 * R] = load(s + 1 + R);
+    PrimM<T> Lnext, Rnext;
+    reconstruct(Lnext, Rnext);
+    const int y = DIR == 2 ? t : s, z = DIR == 2 ? s : t;
+    const size_t id   = size_t(x) + size_t(y) * nx + size_t(z) * size_t(nx) * ny;
+    const T bx        = T(Uh[(kBx + DIR) * n + id]);
+    const ConsM<T> qL = WriteDataM<P>(Lprev, bx, g), qR = WriteDataM<P>(Rnext, bx, g);
+    ConsM<T> f;
+    if (HlldM<P>(qL, qR, bx, g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[DIR]), id, n, DIR, f);
+    Lprev = Lnext;
+  }
 }
 
 // Full update (Update_Conserved_Variables_3D, hydro_cuda.cu:172-402, and

@@ -12,18 +12,20 @@ namespace chl
 inline unsigned MhdBlocks(size_t n) { return unsigned((n + 255) / 256); }
 
 // The three stage-2 flux launches of a reconstruction RK (1 PLMC, 2 PPMC) over the faces
-// with every index in [lo, hi]: 64 x 4 face tiles in XCD-contiguous order.
+// with every index in [lo, hi], in XCD-contiguous order: x faces in 64 x 4 face tiles, y and
+// z faces in 64 x 4 columns that march kMarch faces along the direction.
 template <class P, int RK>
 void MhdFlux2Tiles(const mhd::MhdArgs &a, int lo, int hx, int hy, int hz, hipStream_t s)
 {
   const unsigned ex = unsigned(hx - lo + 1), ey = unsigned(hy - lo + 1), ez = unsigned(hz - lo + 1);
   auto blocks = [](unsigned t) { return dim3((t + 7) / 8 * 8); };
-  hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 0, RK>), blocks((ex + 63) / 64 * ((ey + 3) / 4) * ez), dim3(256), 0,
+  constexpr unsigned K = unsigned(mhd::kMarch);
+  hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, RK>), blocks((ex + 63) / 64 * ((ey + 3) / 4) * ez), dim3(256), 0,
                      s, a, lo, hx, hy, hz);
-  hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 1, RK>), blocks((ex + 63) / 64 * ((ey + 3) / 4) * ez), dim3(256), 0,
-                     s, a, lo, hx, hy, hz);
-  hipLaunchKernelGGL((mhd::MhdFlux2TileKernel<P, 2, RK>), blocks((ex + 63) / 64 * ((ez + 3) / 4) * ey), dim3(256), 0,
-                     s, a, lo, hx, hy, hz);
+  hipLaunchKernelGGL((mhd::MhdFlux2MarchKernel<P, 1, RK>), blocks((ex + 63) / 64 * ((ez + 3) / 4) * ((ey + K - 1) / K)),
+                     dim3(256), 0, s, a, lo, hx, hy, hz);
+  hipLaunchKernelGGL((mhd::MhdFlux2MarchKernel<P, 2, RK>), blocks((ex + 63) / 64 * ((ey + 3) / 4) * ((ez + K - 1) / K)),
+                     dim3(256), 0, s, a, lo, hx, hy, hz);
 }
 
 // CT edge fields fused with one stage's update (MhdCtUpdateKernel) over the cells [clo, ch*]

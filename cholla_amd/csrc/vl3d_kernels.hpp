@@ -325,6 +325,16 @@ struct Stage1Smem {
   T fy[kNumCons<DE>][TY + 1][TX];
 };
 
+// The stage-1 halo prefetch (Stage1Tile) and the waves per SIMD the stage-1 kernels are
+// compiled for.  With HLLC and no DE: the mixed form fits 4 waves (127 VGPRs) with the
+// prefetch; the exact form 3 (36 bytes of spills cost less than the third wave gives back).
+// The fast FP64 form, DE and the other solvers keep the compiler's allocation and no
+// prefetch: its registers push them into spills or a wave lower.
+template <class P, bool DE>
+constexpr bool kStage1Prefetch = P::kId != PolicyFast::kId && !DE && P::kSolver == kRiemannHllc;
+template <class P, bool DE>
+constexpr int kStage1MinWaves = !kStage1Prefetch<P, DE> ? 1 : (P::kId == PolicyMixed::kId ? 4 : 3);
+
 // Stage 1 of one tile (bx, by) x z-chunk bz of the half-step region [lo, hi) per
 // axis, lo = ng - R1, hi = n - ng + R1.  Called by every lane of a TX*TY workgroup.
 // With dual energy (DE) it also advances GasEnergy with its pdV source, stores the
@@ -381,13 +391,12 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
   PcmFaceFlux<P, 2, DE>(c_lo, c_mid, gamma, fz_lo);
 
   // The plane's halo, x sides (TY rows x 2) and y sides (TX cols x 2), no corners: one cell
-  // per lane of the first 2 (TX + TY).  Its conserved values for the next plane are loaded
-  // while the current plane's faces are solved, and converted once the next plane starts
-  // (mixed and exact forms: 3.94 -> 3.70 and 10.71 -> 10.33 ms on KH 512^3).  The fast FP64
-  // form loads them after the first barrier instead: the 10 registers the prefetch holds
-  // push it into spills.
+  // per lane of the first 2 (TX + TY).  Under kStage1Prefetch its conserved values for the
+  // next plane are loaded while the current plane's faces are solved, and converted once the
+  // next plane starts (KH 512^3: mixed 3.94 -> 3.56 ms, exact 10.71 -> 10.28 ms); otherwise
+  // they are loaded after the first barrier.
   static_assert(2 * TY + 2 * TX <= NT, "one halo cell per lane");
-  constexpr bool kPrefetch = P::kId != PolicyFast::kId;
+  constexpr bool kPrefetch = kStage1Prefetch<P, DE>;
   const bool has_h = tid < 2 * TY + 2 * TX;
   int hpi = 0, hpj = 0;
   if (tid < 2 * TY) {
@@ -541,11 +550,6 @@ __device__ __forceinline__ void Stage1Tile(const StepArgs &a, int R1, int bx, in
   }
 }
 
-// Waves per SIMD the stage-1 kernel is compiled for: the mixed form fits 4 (127 VGPRs); the
-// exact form without DE 3 (36 bytes of spills cost less than the third wave would give back);
-// the others take the compiler's own allocation.
-template <class P, bool DE>
-constexpr int kStage1MinWaves = P::kId == PolicyMixed::kId ? 4 : (P::kExact && !DE ? 3 : 1);
 
 template <class P, int TX, int TY, bool DE = false>
 __global__ void __launch_bounds__(TX *TY, (kStage1MinWaves<P, DE>)) Stage1Kernel(StepArgs a, int R1)
