@@ -9,6 +9,7 @@
 //                      launch (no interface arrays: the PCM states are the cells
 //                      themselves); in the device-timed step it also reduces the step's
 //                      inverse dt (mhdInverseCrossingTime) from the state it reads
+//   MhdFlux1MarchKernel the same by x-y tiles marching in z, the plane staged in LDS
 //   MhdCtUpdateKernel  CT edge fields (S&G 2009 eq. 22-24) and the dt/2 update of the
 //                      hydro and face fields in one launch: x-y tiles marching in z, the
 //                      flux values, cell terms and edge fields in LDS (never in HBM)
@@ -897,18 +898,27 @@ __device__ __forceinline__ bool MarchCell(const MhdRegion &m, int &x, int &y, in
 
 // ---------------------------------------------------------------------------- inverse dt
 // mhdInverseCrossingTime (hydro_cuda.cu:420-435) with the cell-centred field
-__device__ __forceinline__ double MhdInverseCrossingTime(const double *U, size_t id, int x, int y, int z, size_t n,
-                                                         int nx, int ny, double dx, double dy, double dz, double g)
+// of a cell's density, momentum, energy and cell-centred field
+__device__ __forceinline__ double MhdInverseCrossingTimeOf(double d, double mx, double my, double mz, double E,
+                                                           double bx, double by, double bz, double dx, double dy,
+                                                           double dz, double g)
 {
-  const double d = U[id], dinv = 1.0 / d;
-  const double vx = U[n + id] * dinv, vy = U[2 * n + id] * dinv, vz = U[3 * n + id] * dinv;
-  double bx, by, bz;
-  CellB(U, id, x, y, z, n, nx, ny, bx, by, bz);
-  const double gasP = PressurePrimM(U[4 * n + id], d, vx, vy, vz, g, bx, by, bz);
+  const double dinv = 1.0 / d;
+  const double vx = mx * dinv, vy = my * dinv, vz = mz * dinv;
+  const double gasP = PressurePrimM(E, d, vx, vy, vz, g, bx, by, bz);
   const double cf   = Magnetosonic<PolicyExact>(d, gasP, bx, by, bz, g, 1.0);
   double m          = Fmax((Fabs(vx) + cf) / dx, (Fabs(vy) + cf) / dy);
   m                 = Fmax(m, (Fabs(vz) + cf) / dz);
   return Fmax(m, 0.0);
+}
+
+__device__ __forceinline__ double MhdInverseCrossingTime(const double *U, size_t id, int x, int y, int z, size_t n,
+                                                         int nx, int ny, double dx, double dy, double dz, double g)
+{
+  double bx, by, bz;
+  CellB(U, id, x, y, z, n, nx, ny, bx, by, bz);
+  return MhdInverseCrossingTimeOf(U[id], U[n + id], U[2 * n + id], U[3 * n + id], U[4 * n + id], bx, by, bz, dx, dy,
+                                  dz, g);
 }
 
 // Block maximum of non-negative doubles folded into *out with one atomic per block; every
@@ -995,6 +1005,99 @@ __global__ void __launch_bounds__(256) MhdFlux1AllKernel(MhdArgs a, MhdRegion m)
   if (!SlabCell(m, x, y, z)) return;
   if (MhdDt(a) == 0.0) return;
   Flux1Cell<P>(a, x, y, z, size_t(x) + size_t(y) * sy + size_t(z) * sz, n, sy, sz);
+}
+
+// The same stage-1 fluxes (and, with kFlagFoldDt, the same inverse-dt reduction) by TX x TY
+// workgroups that march a chunk of kz planes in z.  Each lane loads its cell of the next
+// plane once: it is the z neighbour of the current plane's cell and then the next plane's own
+// cell.  The x and y neighbours come from the current plane staged in LDS (the tile plus one
+// column and one row of halo cells), as the PCM states in the arithmetic type.  A cell's
+// values, rotations and HLLD are those of Flux1Cell.  Compiled for 4 waves per SIMD (the
+// mixed form fits 128 VGPRs without spills; the FP64 forms keep MhdFlux1AllKernel).
+template <class P, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY, 4) MhdFlux1MarchKernel(MhdArgs a, int kz)
+{
+  using T      = typename P::T;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const size_t n  = size_t(nx) * ny * nz;
+  const size_t sy = size_t(nx), sz = size_t(nx) * ny;
+  const bool fold = (a.flags & kFlagFoldDt) != 0;
+  if (!fold) {
+    if ((a.flags & kFlagReduceDt) && blockIdx.x == 0 && threadIdx.x == 0) *a.dti = 0ull;  // MhdCalcDtKernel fills it
+    if (MhdDt(a) == 0.0) return;
+  }
+  const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY, gz = (nz + kz - 1) / kz;
+  const int q = XcdItem(int(blockIdx.x), gx * gy * gz);
+  if (q >= gx * gy * gz) return;  // uniform per workgroup
+  const int tid = int(threadIdx.x), tx = tid % TX, ty = tid / TX;
+  const int x0 = (q % gx) * TX, y0 = ((q / gx) % gy) * TY;
+  const int x = x0 + tx, y = y0 + ty;
+  const int z0 = (q / (gx * gy)) * kz, z1 = min(z0 + kz, nz);
+  const bool in = x < nx && y < ny;
+  const double *U = a.U;
+  const T g       = T(a.gamma);
+  const int ng = a.ng;
+  const bool real_xy = x >= ng && y >= ng && x < nx - ng && y < ny - ng;
+  double dti = 0.0;
+  // a cell in the arithmetic type: d, m x/y/z, E, B x/y/z (global orientation); its inverse
+  // crossing time, from the FP64 values, is folded into dti as it is loaded
+  struct Cell {
+    T v[8];
+  };
+  auto load = [&](int xx, int yy, int zz, bool with_dt) {
+    const size_t cid = size_t(xx) + size_t(yy) * sy + size_t(zz) * sz;
+    const double d = U[cid], mx = U[n + cid], my = U[2 * n + cid], mz = U[3 * n + cid], E = U[4 * n + cid];
+    double bx, by, bz;
+    CellB(U, cid, xx, yy, zz, n, nx, ny, bx, by, bz);
+    if (with_dt) dti = Fmax(dti, MhdInverseCrossingTimeOf(d, mx, my, mz, E, bx, by, bz, a.dx, a.dy, a.dz, a.gamma));
+    return Cell{{T(d), T(mx), T(my), T(mz), T(E), T(bx), T(by), T(bz)}};
+  };
+  auto real = [&](int zz) { return fold && real_xy && zz >= ng && zz < nz - ng; };
+  __shared__ T pl[8][TY + 1][TX + 1];  // the current plane and its halo
+  auto stage = [&](const Cell &c, int r, int col) {
+#pragma unroll
+    for (int f = 0; f < 8; f++) pl[f][r][col] = c.v[f];
+  };
+  auto pcm = [&](int dir, const Cell &c) {
+    T mx, my, mz, bx, by, bz;
+    RotM(dir, c.v[1], c.v[2], c.v[3], mx, my, mz);
+    RotM(dir, c.v[5], c.v[6], c.v[7], bx, by, bz);
+    return ConsM<T>{c.v[0], mx, my, mz, c.v[4], by, bz};
+  };
+  auto staged = [&](int r, int col) {
+    Cell c;
+#pragma unroll
+    for (int f = 0; f < 8; f++) c.v[f] = pl[f][r][col];
+    return c;
+  };
+  auto solve = [&](int dir, size_t id, const ConsM<T> &qL, const ConsM<T> &qR) {
+    ConsM<T> f;
+    if (HlldM<P>(qL, qR, T(U[(kBx + dir) * n + id]), g, f)) StoreFluxM(reinterpret_cast<T *>(a.F[dir]), id, n, dir, f);
+  };
+  // halo lanes: the column x0 + TX (TY cells), then the row y0 + TY (TX cells)
+  static_assert(TX + TY <= TX * TY, "one halo cell per lane");
+  const bool is_h = tid < TX + TY;
+  const int hx = tid < TY ? x0 + TX : x0 + (tid - TY), hy = tid < TY ? y0 + tid : y0 + TY;
+  const int hr = tid < TY ? tid : TY, hc = tid < TY ? TX : tid - TY;
+  const bool has_h = is_h && hx < nx && hy < ny;
+  Cell own;
+  if (in) own = load(x, y, z0, real(z0));
+  for (int z = z0; z < z1; z++) {
+    if (in) stage(own, ty, tx);
+    if (has_h) stage(load(hx, hy, z, false), hr, hc);
+    Cell up;
+    if (in && z + 1 < nz) up = load(x, y, z + 1, z + 1 < z1 && real(z + 1));
+    __syncthreads();
+    if (in) {
+      const size_t id = size_t(x) + size_t(y) * sy + size_t(z) * sz;
+      if (x <= nx - 2) solve(0, id, pcm(0, own), pcm(0, staged(ty, tx + 1)));
+      if (y <= ny - 2) solve(1, id, pcm(1, own), pcm(1, staged(ty + 1, tx)));
+      if (z <= nz - 2) solve(2, id, pcm(2, own), pcm(2, up));
+    }
+    __syncthreads();
+    own = up;
+  }
+  if (fold) BlockMaxAtomic(dti, a.dti);  // every lane of the workgroup gets here
 }
 
 // Stage-2 fluxes of the PCM build (pcm_cuda.cu:256-420 on the half-step state, then HLLD): the
