@@ -51,8 +51,6 @@ int CheckConfig(const chl_hydro_config &c)
     return Fail(CHL_ERR_UNSUPPORTED, "reconstruction must be PCM, PLMC, PPMC, PLMP or PPMP");
   const bool simple_recon = c.reconstruction == CHL_RECON_PCM || c.reconstruction == CHL_RECON_PLMP ||
                             c.reconstruction == CHL_RECON_PPMP;
-  if (c.reconstruction == CHL_RECON_PCM && c.dual_energy && c.ny == 1 && c.nz == 1)
-    return Fail(CHL_ERR_UNSUPPORTED, "the 1D integrator's dual energy is built for PLMC, PPMC, PLMP and PPMP");
   if (c.precision < CHL_PREC_FP64_EXACT || c.precision > CHL_PREC_MIXED) return Fail(CHL_ERR_INVALID, "bad precision");
   if (c.positivity_fallback != 0 && c.positivity_fallback != 1)
     return Fail(CHL_ERR_INVALID, "positivity_fallback must be 0 or 1");

@@ -585,7 +585,9 @@ __device__ __forceinline__ Cons<double> ReconFlux(const Prim<typename P::T> &L, 
 // Stage 2 of VL_Algorithm_1D_CUDA (VL_1D_cuda.cu:95-137) for the real cells, with the
 // next inverse dt (Calc_dt_1D) reduced in the epilogue (without dual energy; with it the
 // energy sync that follows reduces it).  With DE the GasEnergy takes its flux difference
-// on top of the full-step pdV source Vl1dPdvKernel added.
+// on top of the full-step pdV source Vl1dPdvKernel added; PCM with DE passes the half state
+// through with its conserved GasEnergy (PCM_Reconstruction_1D, pcm_cuda.cu:13-97), so its
+// faces are stage 1's PCM faces on W.
 template <class P, int RK, bool DE>
 __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, double dx, const double *dt_dev,
                                  double dt_host, double gamma, unsigned long long *dti, GravSpec grav)
@@ -596,14 +598,26 @@ __global__ void Vl1dStage2Kernel(double *U, const double *W, int n, int ng, doub
   const double dt = dt_dev ? *dt_dev : dt_host;
   if (id >= ng && id < n - ng) {
     const T g = T(gamma), inv_gm1 = T(1.0 / (gamma - 1.0));
-    Prim<T> Lm, Rm, L0, R0, Lp, Rp;
-    T gLm = 0, gRm = 0, gL0 = 0, gR0 = 0, gLp = 0, gRp = 0, pm, p0, pp;
-    ReconLine<P, RK, DE>(W, n, id - 1, 1, 1, 2, 3, gamma, Lm, Rm, gLm, gRm, pm);
-    ReconLine<P, RK, DE>(W, n, id, 1, 1, 2, 3, gamma, L0, R0, gL0, gR0, p0);
-    ReconLine<P, RK, DE>(W, n, id + 1, 1, 1, 2, 3, gamma, Lp, Rp, gLp, gRp, pp);
     double gm = 0.0, gp = 0.0;
-    Cons<double> fm = ReconFlux<P, RK, DE>(Lm, R0, gLm, gR0, pm, g, inv_gm1, &gm);
-    Cons<double> fp = ReconFlux<P, RK, DE>(L0, Rp, gL0, gRp, p0, g, inv_gm1, &gp);
+    Cons<double> fm, fp;
+    if constexpr (RK == kPCM && DE) {
+      double lo[3];
+      auto side = [&](int c, double &plo) {
+        return SideFromCons<P, DE>(W[c], W[n + c], W[2 * n + c], W[3 * n + c], W[4 * n + c], gamma, W[5 * n + c],
+                                   plo);
+      };
+      RState<T> a = side(id - 1, lo[0]), b = side(id, lo[1]), cc = side(id + 1, lo[2]);
+      fm = PcmFlux<P, DE>(a, lo[0], b, lo[1], g, &gm);
+      fp = PcmFlux<P, DE>(b, lo[1], cc, lo[2], g, &gp);
+    } else {
+      Prim<T> Lm, Rm, L0, R0, Lp, Rp;
+      T gLm = 0, gRm = 0, gL0 = 0, gR0 = 0, gLp = 0, gRp = 0, pm, p0, pp;
+      ReconLine<P, RK, DE>(W, n, id - 1, 1, 1, 2, 3, gamma, Lm, Rm, gLm, gRm, pm);
+      ReconLine<P, RK, DE>(W, n, id, 1, 1, 2, 3, gamma, L0, R0, gL0, gR0, p0);
+      ReconLine<P, RK, DE>(W, n, id + 1, 1, 1, 2, 3, gamma, Lp, Rp, gLp, gRp, pp);
+      fm = ReconFlux<P, RK, DE>(Lm, R0, gLm, gR0, pm, g, inv_gm1, &gm);
+      fp = ReconFlux<P, RK, DE>(L0, Rp, gL0, gRp, p0, g, inv_gm1, &gp);
+    }
     const double dtodx = dt / dx;
     double un[5];
     un[0] = U[id] + dtodx * (fm.d - fp.d);
@@ -1160,6 +1174,9 @@ static hipError_t LaunchVl1dP(double *U, double *W, int n, int ng, int recon, do
     } else if (recon == kPPMP) {
       hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPPMP, true>), dim3(nb), dim3(256), 0, s, U, W, n, ng, dx, dt_dev, dt,
                          gamma, nullptr, grav);
+    } else if (recon == kPCM) {
+      hipLaunchKernelGGL((Vl1dStage2Kernel<P, kPCM, true>), dim3(nb), dim3(256), 0, s, U, W, n, ng, dx, dt_dev, dt,
+                         gamma, nullptr, grav);
     } else {
       return hipErrorInvalidValue;
     }
@@ -1211,7 +1228,6 @@ hipError_t LaunchVl1d(double *U, double *W, int n, int ng, int recon, int rieman
 {
   if (dti && n > 1024 && !de) return hipErrorInvalidValue;
   if (recon != kPPMC && recon != kPLMC && recon != kPLMP && recon != kPPMP && recon != kPCM) return hipErrorInvalidValue;
-  if (de && recon == kPCM) return hipErrorInvalidValue;  // every solver with PLMC, PPMC, PLMP or PPMP
   switch (form) {
     case PolicyExact::kId:
       return de ? LaunchVl1dS<PolicyExact, true>(riemann, U, W, n, ng, recon, dx, dt_dev, dt, gamma, dti, dti_reset, grav, s)

@@ -56,9 +56,9 @@ typedef struct chl_hydro_config {
                             2D: nz = 1 (every precision; the 1D half state is FP64 in every form) */
   int n_ghost;           /* Header::n_ghost: PLMC 3, PPMC 4 (src/grid/grid3D.cpp:53-76) */
   int n_fields;          /* 5 (hydro), or 6 with dual_energy (GasEnergy last); no scalars */
-  int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL.  PCM, PLMP and PPMP: no
-                            dual energy, tile shapes 0 and 1 (n_ghost: PCM 2, PLMP 3, PPMP 4) */
-  int riemann;           /* CHL_RIEMANN_HLLC, or _HLL / _ROE / _EXACT (no dual energy; tile shapes 0, 1) */
+  int reconstruction;    /* CHL_RECON_* ; stage 1 is always PCM as in VL.  PCM, PLMP and PPMP: 3D
+                            tile shapes 0 and 1 (n_ghost: PCM 2, PLMP 3, PPMP 4) */
+  int riemann;           /* CHL_RIEMANN_HLLC, or _HLL / _ROE / _EXACT (3D tile shapes 0 and 1) */
   int precision;         /* CHL_PREC_* */
   int density_floor;     /* DENSITY_FLOOR build option, 0/1 */
   int temperature_floor; /* TEMPERATURE_FLOOR build option, 0/1 */
@@ -66,7 +66,8 @@ typedef struct chl_hydro_config {
   int tile1, tile2;      /* tile shapes of the two fused stages, -1 = default */
   int kchunk1, kchunk2;  /* z cells per workgroup in the two stages, 0 = default */
   int tile_order;        /* 0: each XCD takes a contiguous run of tiles (default); 1: dispatch order */
-  int dual_energy;       /* DE build option, 0/1: GasEnergy as field 5 (n_fields 6); HLLC with PLMC or PPMC */
+  int dual_energy;       /* DE build option, 0/1: GasEnergy as field 5 (n_fields 6); every solver and
+                            reconstruction in 1D, 2D and 3D (3D: tile shapes 0 and 1) */
   int positivity_fallback; /* 0/1, PPMC in 3D only; no counterpart in the reference: where a
                               cell's PPMC interface density or pressure comes out non-positive,
                               both of its interfaces in that sweep take the cell-centred values

@@ -210,19 +210,18 @@ def test_de_uniform_supersonic_fixed_point(precision):
 def test_de_config_checks():
     with pytest.raises(chl.ChollaError):
         chl.Hydro(24, 24, 24, 4, chl.RECON_PPMC, chl.PREC_FP64, dual_energy=True, tile1=2)
-    with pytest.raises(chl.ChollaError, match="1D integrator.s dual energy"):  # no PCM with DE in 1D, in any form
-        chl.Hydro(100, 1, 1, 3, chl.RECON_PCM, chl.PREC_FP64, dual_energy=True)
 
 
 # ----------------------------------------------------------------------------- 1D
 @pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("recon,ng", [(chl.RECON_PLMC, 3), (chl.RECON_PPMC, 4), (chl.RECON_PLMP, 3),
-                                      (chl.RECON_PPMP, 4)])
+                                      (chl.RECON_PPMP, 4), (chl.RECON_PCM, 2)])
 @pytest.mark.parametrize("case", ["warm", "cold", "wide"])
 def test_dual_energy_1d_exact_matches_oracle(case, recon, ng, use_graph):
     """The 1D DE step (VL_1D_cuda.cu:126-142) bit for bit: a warm flow (total-energy pressure
     selected), a cold supersonic compression (advected internal energy selected) and a
-    1500-cell box (the dt reduction over several blocks of the energy sync)."""
+    1500-cell box (the dt reduction over several blocks of the energy sync).  PCM passes the
+    half state through with its conserved GasEnergy (PCM_Reconstruction_1D)."""
     from test_oracle_de_1d import state_1d
     n = 1500 if case == "wide" else 64
     x = (np.arange(n) + 0.5) / n
@@ -266,5 +265,5 @@ def test_dual_energy_1d_solvers(riemann):
 
 
 def test_dual_energy_1d_limits():
-    with pytest.raises(chl.ChollaError, match="dual energy"):
-        chl.Hydro(100, 1, 1, 2, chl.RECON_PCM, chl.PREC_FP64_EXACT, dual_energy=True)
+    with pytest.raises(chl.ChollaError, match="n_ghost too small"):
+        chl.Hydro(100, 1, 1, 1, chl.RECON_PCM, chl.PREC_FP64_EXACT, dual_energy=True)

@@ -10,7 +10,7 @@ FP64 reference order on the same inputs:
   measured values).
 
 Cases: the Sod tube of the 1D configuration (PLMC + HLLC to t = 0.2) with every reconstruction
-and every solver, dual energy with PLMC and PPMC; the 2D Kelvin-Helmholtz box of the 2D parity
+and every solver, dual energy with every reconstruction; the 2D Kelvin-Helmholtz box of the 2D parity
 tests (32^2, 10 steps) with every reconstruction and solver, dual energy with PCM, PLMC and PPMC.
 The exact form of both integrators is bit-identical to the oracle (test_gpu_parity.py,
 test_gpu_2d.py, test_gpu_de.py)."""
@@ -113,7 +113,7 @@ def test_sod_forms_solvers(solver, precision):
 
 
 @pytest.mark.parametrize("precision", [chl.PREC_FP64, chl.PREC_MIXED])
-@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC, orc.PLMP, orc.PPMP])
+@pytest.mark.parametrize("recon", [orc.PLMC, orc.PPMC, orc.PLMP, orc.PPMP, orc.PCM])
 def test_sod_forms_dual_energy(recon, precision):
     say("Sod 1D DE form", precision, "recon", recon)
     l1, _, _, _ = sod_run(recon, orc.HLLC, precision, de=True)
