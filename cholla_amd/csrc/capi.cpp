@@ -618,11 +618,8 @@ int chl_hydro_set_static_gravity(chl_hydro *h, int custom_grav, int x_off, int y
   const int max_id          = h->two_d ? 4 : 1;  // calc_g_2D has four fields, calc_g_1D / calc_g_3D one
   if (custom_grav < 0 || custom_grav > max_id) return Fail(CHL_ERR_INVALID, "custom_grav out of range for this box");
   const bool grav3d = custom_grav != 0 && !h->one_d && !h->two_d;
-  if (grav3d && c.riemann != CHL_RIEMANN_HLLC)
-    return Fail(CHL_ERR_UNSUPPORTED, "3D static gravity is built for HLLC, stage-2 tile shapes 0 and 1");
   if (grav3d && c.tile2 >= chl::kNumDeTiles) {
-    if (!h->tile2_auto)
-      return Fail(CHL_ERR_UNSUPPORTED, "3D static gravity is built for HLLC, stage-2 tile shapes 0 and 1");
+    if (!h->tile2_auto) return Fail(CHL_ERR_UNSUPPORTED, "3D static gravity is built for stage-2 tile shapes 0 and 1");
     // the library's default stage-2 tile has no gravity kernel: take the 16x16 one
     h->cfg.tile2 = 1;
     if (h->graph) {

@@ -98,16 +98,13 @@ hipError_t LaunchStage1(const StepArgs &a, int R1, int tile, hipStream_t s)
 template <class P, int RK>
 hipError_t LaunchStage2R(const StepArgs &a, int tile, hipStream_t s)
 {
-  if (a.grav.id != 0) {  // static gravity: HLLC, the first two tile shapes; DE with PLMC and PPMC
-    if (tile > 1 || a.riemann != kRiemannHllc) return hipErrorInvalidValue;
-    if (a.de) {
-      if constexpr (RK == kPLMC || RK == kPPMC) {
-        return tile == 0 ? LaunchStage2T<GravPolicy<P>, RK, 32, 8, true>(a, s)
-                         : LaunchStage2T<GravPolicy<P>, RK, 16, 16, true>(a, s);
-      }
-      return hipErrorInvalidValue;
-    }
-    return tile == 0 ? LaunchStage2T<GravPolicy<P>, RK, 32, 8>(a, s) : LaunchStage2T<GravPolicy<P>, RK, 16, 16>(a, s);
+  if (a.grav.id != 0) {  // static gravity: the first two tile shapes, every solver, with or without DE
+    if (tile > 1) return hipErrorInvalidValue;
+    return WithSolver<P>(a.riemann, [&](auto q) {
+      using Q = GravPolicy<decltype(q)>;
+      if (a.de) return tile == 0 ? LaunchStage2T<Q, RK, 32, 8, true>(a, s) : LaunchStage2T<Q, RK, 16, 16, true>(a, s);
+      return tile == 0 ? LaunchStage2T<Q, RK, 32, 8>(a, s) : LaunchStage2T<Q, RK, 16, 16>(a, s);
+    });
   }
   // PCM, PLMP, PPMP, HLL, Roe and the exact solver: the first two tile shapes, with or
   // without dual energy

@@ -166,8 +166,8 @@ int chl_set_boundaries(double *dev_conserved, int nx, int ny, int nz, int n_fiel
  * 142-153, 301-315): the analytic field Parameters::custom_grav selects (0 none; 1D and 3D:
  * 1 NFW halo + Miyamoto-Nagai disk; 2D: 1 Gresho, 2 Rayleigh-Taylor, 3 Keplerian, 4 Kuzmin
  * disk + NFW halo) and the grid's offsets and lower bounds, as VL_Algorithm_*_CUDA receive
- * them (x_off.., xbound..).  Applies to every later step of the handle.  3D: HLLC without
- * dual energy, stage-2 tile shapes 0 and 1. */
+ * them (x_off.., xbound..).  Applies to every later step of the handle.  3D: every solver and
+ * reconstruction, with or without dual energy, stage-2 tile shapes 0 and 1. */
 int chl_hydro_set_static_gravity(chl_hydro *h, int custom_grav, int x_off, int y_off, int z_off, double xbound,
                                  double ybound, double zbound);
 

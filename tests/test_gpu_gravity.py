@@ -113,6 +113,69 @@ def test_3d_static_gravity_dual_energy(precision):
         assert max(l1) <= TOL[precision], l1
 
 
+SOLVERS = {chl.RIEMANN_HLL: orc.HLL, chl.RIEMANN_ROE: orc.ROE, chl.RIEMANN_EXACT: orc.EXACT}
+RECONS = {chl.RECON_PCM: (orc.PCM, 2), chl.RECON_PLMP: (orc.PLMP, 3), chl.RECON_PPMP: (orc.PPMP, 4),
+          chl.RECON_PLMC: (orc.PLMC, 3), chl.RECON_PPMC: (orc.PPMC, 4)}
+
+
+@pytest.mark.parametrize("de", [False, True])
+@pytest.mark.parametrize("riemann", [chl.RIEMANN_HLL, chl.RIEMANN_ROE, chl.RIEMANN_EXACT])
+def test_3d_static_gravity_solvers(riemann, de):
+    """STATIC_GRAV with the HLL, Roe and exact solvers (the source is the full update's, the
+    solver only the faces'), with and without DE, exact form, PPMC: the oracle's loop with the
+    same solver and field, at the static-gravity bar."""
+    n, ng = 12, 4
+    U, (N, _, _) = disk_box((n, n, n), ng, vamp=1e-2, seed=5)
+    nf = 5
+    if de:
+        U5 = U.reshape(5, -1)
+        U = np.concatenate([U5, (U5[4] - 0.5 * (U5[1] ** 2 + U5[2] ** 2 + U5[3] ** 2) / U5[0])[None]])
+        U = U.reshape(-1).copy()
+        nf = 6
+    orc.set_boundaries(U, N, N, N, nf, ng, [1] * 6)
+    h, off, bnd = 0.6, (3, -2, 5), (-7.0, 2.0, -3.0)
+    say("3D static gravity solver", riemann, "DE", de)
+    Uo = U.copy()
+    with orc.static_gravity(1, off, bnd):
+        s_o, t_o = orc.run(Uo, N, N, N, ng, nf, h, h, h, G, orc.PPMC, [1] * 6, 0.0, 1e30, 5, dual_energy_on=de,
+                           solver=SOLVERS[riemann])
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(N, N, N, ng, chl.RECON_PPMC, chl.PREC_FP64_EXACT, riemann=riemann, dual_energy=de)
+    hy.set_static_gravity(1, off, bnd)
+    s, t = hy.run(Ud, h, h, h, G, 0.0, 1e30, 5, use_graph=True)
+    hy.close()
+    assert s == s_o == 5 and abs(t - t_o) <= TOL[chl.PREC_FP64_EXACT] * t_o
+    l1 = strict_l1(Ud.download(), Uo, nf, N, N, N, ng)
+    say("  strict L1", ["%.2e" % x for x in l1])
+    assert max(l1) <= TOL[chl.PREC_FP64], l1
+
+
+@pytest.mark.parametrize("recon", [chl.RECON_PCM, chl.RECON_PLMP, chl.RECON_PPMP, chl.RECON_PLMC])
+def test_3d_static_gravity_reconstructions_dual_energy(recon):
+    """STATIC_GRAV with DE under the other reconstructions, HLLC, exact form, against the
+    oracle's DE loop with the same field."""
+    n = 12
+    orc_recon, ng = RECONS[recon]
+    U5, (N, _, _) = disk_box((n, n, n), ng, vamp=1e-2, seed=6)
+    U5 = U5.reshape(5, -1)
+    U = np.concatenate([U5, (U5[4] - 0.5 * (U5[1] ** 2 + U5[2] ** 2 + U5[3] ** 2) / U5[0])[None]]).reshape(-1).copy()
+    orc.set_boundaries(U, N, N, N, 6, ng, [1] * 6)
+    h, off, bnd = 0.6, (3, -2, 5), (-7.0, 2.0, -3.0)
+    say("3D static gravity + DE recon", recon)
+    Uo = U.copy()
+    with orc.static_gravity(1, off, bnd):
+        s_o, t_o = orc.run(Uo, N, N, N, ng, 6, h, h, h, G, orc_recon, [1] * 6, 0.0, 1e30, 5, dual_energy_on=True)
+    Ud = chl.DeviceArray.from_host(U)
+    hy = chl.Hydro(N, N, N, ng, recon, chl.PREC_FP64_EXACT, dual_energy=True)
+    hy.set_static_gravity(1, off, bnd)
+    s, t = hy.run(Ud, h, h, h, G, 0.0, 1e30, 5, use_graph=True)
+    hy.close()
+    assert s == s_o == 5 and abs(t - t_o) <= TOL[chl.PREC_FP64_EXACT] * t_o
+    l1 = strict_l1(Ud.download(), Uo, 6, N, N, N, ng)
+    say("  strict L1", ["%.2e" % x for x in l1])
+    assert max(l1) <= TOL[chl.PREC_FP64], l1
+
+
 @pytest.mark.parametrize("grav", [1, 2, 3, 4])
 def test_2d_static_gravity(grav):
     n, ng = 24, 3
